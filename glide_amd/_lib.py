@@ -1,0 +1,158 @@
+"""ctypes binding of ``libglide_hip.so`` (declared in ``include/glide_hip.h``).
+
+The shared library is loaded on first use.  There is no fallback: when the
+library is missing, or a call is made with tensors that are not on a ROCm
+device, a :class:`GlideError` is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libglide_hip.so"
+
+USE_FORCING = 1
+USE_MASK = 2
+FREEZE_CALVING = 4
+FREEZE_PHI = 8
+
+
+class GlideError(RuntimeError):
+    """Raised when a native entry point reports a failure."""
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "n", "eps_reg", "flotation_reg_driving", "m", "u_reg", "water_drag", "flotation_reg_sliding",
+        "calving_rate", "flotation_reg_calving", "thklim")]
+
+
+LEVEL_POINTERS = (
+    "u", "v", "H", "H_prev", "phi", "mask",
+    "bed", "B", "beta", "smb",
+    "f_u", "f_v", "f_H", "r_u", "r_v", "r_H", "F_u", "F_v", "F_H", "gamma",
+    "w_u", "w_v", "w_H", "y_u", "y_v", "y_H", "z_u", "z_v", "z_H", "chi", "cphi",
+    "lambda_u", "lambda_v", "lambda_H",
+    "af_u", "af_v", "af_H", "ar_u", "ar_v", "ar_H", "vjp_u", "vjp_v", "vjp_H",
+    "wl_u", "wl_v", "wl_H", "zl_u", "zl_v", "zl_H",
+    "g_beta", "g_bed", "g_H_prev", "g_smb",
+    "work",
+)
+
+
+class Level(ctypes.Structure):
+    _fields_ = ([("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("dx", ctypes.c_double), ("phys", Physics)]
+                + [(n, ctypes.c_void_p) for n in LEVEL_POINTERS])
+
+
+class VankaConfigC(ctypes.Structure):
+    _fields_ = [("omega", ctypes.c_double), ("relax_phi", ctypes.c_double), ("newton_steps", ctypes.c_int32),
+                ("relaxation", ctypes.c_double), ("ssa_damping", ctypes.c_double), ("mc_damping", ctypes.c_double)]
+
+
+class FasConfigC(ctypes.Structure):
+    _fields_ = [("freeze_coarse_calving", ctypes.c_int32), ("freeze_coarse_phi", ctypes.c_int32),
+                ("coarsest_steps", ctypes.c_int32), ("pre_steps", ctypes.c_int32), ("post_steps", ctypes.c_int32),
+                ("finest_steps", ctypes.c_int32), ("maximum_vcycles", ctypes.c_int32),
+                ("relative_tolerance", ctypes.c_double), ("absolute_tolerance", ctypes.c_double),
+                ("compat_vprolong", ctypes.c_int32)]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [("vcycles", ctypes.c_int32), ("initial_norm", ctypes.c_double), ("final_norm", ctypes.c_double),
+                ("norms", ctypes.c_double * 64), ("norms3", ctypes.c_double * 3)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_S = ctypes.c_size_t
+_LP = ctypes.POINTER(Level)
+
+# name -> (restype, argtypes); every symbol the header declares.
+SIGNATURES = {
+    "glide_version": (ctypes.c_char_p, []),
+    "glide_last_error": (_I, [ctypes.c_char_p, _S]),
+    "glide_compute_grounded": (_I, [_LP, _D, _P]),
+    "glide_ssa_residual": (_I, [_LP, _D, _I, _P, _P, _P, _P]),
+    "glide_norms3": (_I, [_P, _S, _P, _S, _P, _S, ctypes.POINTER(_D), _P]),
+    "glide_vanka_sweeps": (_I, [_LP, _D, _I, ctypes.POINTER(VankaConfigC), _I, _P]),
+    "glide_vanka_dump": (_I, [_LP, _D, _P, _P, _P]),
+    "glide_restrict_vfacet": (_I, [_P, _P, _I, _I, _P]),
+    "glide_restrict_hfacet": (_I, [_P, _P, _I, _I, _P]),
+    "glide_restrict_cell": (_I, [_P, _P, _I, _I, _I, _P]),
+    "glide_prolong_vfacet_bilinear": (_I, [_P, _P, _I, _I, _P]),
+    "glide_prolong_hfacet_bilinear": (_I, [_P, _P, _I, _I, _P]),
+    "glide_prolong_cell": (_I, [_P, _P, _I, _I, _I, _P]),
+    "glide_fascd_solve": (_I, [_LP, _I, _I, _D, ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC),
+                               ctypes.POINTER(SolveReport), _P]),
+    "glide_fascd_vcycle": (_I, [_LP, _I, _I, _I, _D, ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC), _P]),
+    "glide_ssa_vjp": (_I, [_LP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "glide_vanka_adjoint_sweeps": (_I, [_LP, _D, _I, ctypes.POINTER(VankaConfigC), _I, _P]),
+    "glide_fas_adjoint_solve": (_I, [_LP, _I, _I, _D, ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC),
+                                     ctypes.POINTER(SolveReport), _P]),
+    "glide_fas_adjoint_vcycle": (_I, [_LP, _I, _I, _I, _D, ctypes.POINTER(FasConfigC),
+                                      ctypes.POINTER(VankaConfigC), _P]),
+    "glide_gradients": (_I, [_LP, _D, _I, _P]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load(path: Path | str | None = None):
+    """Load (once) and return the native library with typed signatures."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = Path(path) if path is not None else LIB_PATH
+        if not p.exists():
+            raise GlideError(f"native library not built: {p} (run glide_amd.build.build())")
+        lib = ctypes.CDLL(str(p))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def last_error() -> str:
+    lib = load()
+    buf = ctypes.create_string_buffer(1024)
+    lib.glide_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise GlideError(f"{what} failed ({status}): {last_error()}")
+
+
+def call(name: str, *args) -> None:
+    """Call a native entry point and raise on a non-zero status."""
+    fn = getattr(load(), name)
+    check(fn(*args), name)
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    """Device address of a contiguous ROCm tensor (None stays NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise GlideError("glide_amd kernels need tensors on a ROCm device (got a CPU tensor)")
+    if not t.is_contiguous():
+        raise GlideError("glide_amd kernels need contiguous tensors")
+    if t.dtype != torch.float64:
+        raise GlideError(f"glide_amd kernels compute in float64 (got {t.dtype})")
+    return t.data_ptr()
+
+
+def stream_handle(device: torch.device | None = None) -> int:
+    """hipStream_t of torch's current stream on `device`."""
+    return torch.cuda.current_stream(device).cuda_stream

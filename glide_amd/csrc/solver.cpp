@@ -1,0 +1,643 @@
+// C ABI of the MI355X SSA path and the native multigrid drivers.
+//
+// The FAS constrained-descent V-cycle (glide/multigrid.py:536-645) and the FAS
+// adjoint V-cycle (glide/multigrid.py:905-973) run here in C++ and enqueue
+// their kernels on the caller's stream; the only host synchronisation is the
+// per-V-cycle residual norm the stopping test needs (none when both
+// tolerances are non-positive: a fixed number of V-cycles, norms read once at
+// the end).  All field buffers belong to the caller; the library owns one
+// small per-device reduction scratch (allocated on first use).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/glide_hip.h"
+#include "ssa_kernels.hpp"
+
+using namespace glide;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+    return GLIDE_ERR_HIP;
+}
+
+#define HIP_TRY(expr, where)                       \
+    do {                                           \
+        hipError_t _e = (expr);                    \
+        if (_e != hipSuccess) return hip_fail(_e, where); \
+    } while (0)
+
+#define GLIDE_TRY(expr)              \
+    do {                             \
+        int _r = (expr);             \
+        if (_r != GLIDE_OK) return _r; \
+    } while (0)
+
+struct DeviceScratch {
+    double* partials = nullptr;  // 3 * kSumsqBlocks
+    double* norms = nullptr;     // 3 * 66 slots
+    double* host = nullptr;      // pinned mirror of `norms`
+};
+
+std::mutex g_scratch_mutex;
+std::vector<DeviceScratch> g_scratch;
+
+int scratch_for_current_device(DeviceScratch** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+    DeviceScratch& s = g_scratch[dev];
+    if (!s.partials) {
+        HIP_TRY(hipMalloc(&s.partials, sizeof(double) * 3 * kSumsqBlocks), "hipMalloc(reduction scratch)");
+        HIP_TRY(hipMalloc(&s.norms, sizeof(double) * 3 * 66), "hipMalloc(norm slots)");
+        HIP_TRY(hipHostMalloc(&s.host, sizeof(double) * 3 * 66, hipHostMallocDefault), "hipHostMalloc(norms)");
+    }
+    *out = &s;
+    return GLIDE_OK;
+}
+
+inline size_t nU(const glide_level& L) { return (size_t)L.ny * (L.nx + 1); }
+inline size_t nV(const glide_level& L) { return (size_t)(L.ny + 1) * L.nx; }
+inline size_t nC(const glide_level& L) { return (size_t)L.ny * L.nx; }
+
+typedef double R;
+inline R* P(void* p) { return static_cast<R*>(p); }
+inline const R* CP(const void* p) { return static_cast<const R*>(p); }
+
+Phys<R> make_phys(const glide_physics& g) {
+    Phys<R> p;
+    p.n = g.n; p.eps_reg = g.eps_reg; p.frd = g.flotation_reg_driving;
+    p.m = g.m; p.u_reg = g.u_reg; p.water_drag = g.water_drag; p.frs = g.flotation_reg_sliding;
+    p.calving_rate = g.calving_rate; p.frc = g.flotation_reg_calving; p.thklim = g.thklim;
+    p.glen_p = (1.0 - g.n) / (2.0 * g.n);
+    p.drag_q = (g.m - 1.0) / 2.0;
+    p.linear_drag = (g.m == 1.0 && g.u_reg > 0.0) ? 1 : 0;
+    return p;
+}
+
+SsaArgs<R> make_args(const glide_level& L, double dt, bool adjoint_forcing) {
+    SsaArgs<R> a;
+    a.ny = L.ny; a.nx = L.nx; a.dxi = 1.0 / L.dx; a.dt = dt;
+    a.P = make_phys(L.phys);
+    a.u = CP(L.u); a.v = CP(L.v); a.H = CP(L.H); a.phi = CP(L.phi); a.mask = CP(L.mask);
+    a.bed = CP(L.bed); a.B = CP(L.B); a.beta = CP(L.beta); a.gamma = CP(L.gamma);
+    a.f_u = CP(adjoint_forcing ? L.af_u : L.f_u);
+    a.f_v = CP(adjoint_forcing ? L.af_v : L.f_v);
+    a.f_H = CP(adjoint_forcing ? L.af_H : L.f_H);
+    return a;
+}
+
+int check_level(const glide_level* L) {
+    if (!L) return fail(GLIDE_ERR_INVALID_ARG, "level is NULL");
+    if (L->ny < 2 || L->nx < 2) return fail(GLIDE_ERR_SHAPE, "level must have ny >= 2 and nx >= 2");
+    if (!(L->dx > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dx must be positive");
+    return GLIDE_OK;
+}
+
+int require(const void* p, const char* name) {
+    if (!p) {
+        g_last_error = std::string("required buffer is NULL: ") + name;
+        return GLIDE_ERR_INVALID_ARG;
+    }
+    return GLIDE_OK;
+}
+
+#define REQ(ptr) GLIDE_TRY(require((ptr), #ptr))
+
+int check_ssa_inputs(const glide_level& L) {
+    REQ(L.u); REQ(L.v); REQ(L.H); REQ(L.phi); REQ(L.mask); REQ(L.bed); REQ(L.B); REQ(L.beta);
+    return GLIDE_OK;
+}
+
+// ---------------------------------------------------------------- primitives
+int grounded(const glide_level& L, double relax, hipStream_t st) {
+    HIP_TRY(launch_grounded<R>(P(L.phi), CP(L.H), CP(L.bed), L.phys.flotation_reg_driving, relax, (long)nC(L), st),
+            "grounded");
+    return GLIDE_OK;
+}
+
+int residual(const glide_level& L, double dt, int flags, R* ou, R* ov, R* oH, hipStream_t st) {
+    SsaArgs<R> a = make_args(L, dt, false);
+    const double cr = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
+    HIP_TRY(launch_residual<R>(a, ou, ov, oH, (flags & GLIDE_USE_FORCING) ? 1 : 0, (flags & GLIDE_USE_MASK) ? 1 : 0,
+                               cr, st),
+            "residual");
+    return GLIDE_OK;
+}
+
+int sumsq3_async(const R* a, size_t na, const R* b, size_t nb, const R* c, size_t nc, double* dev_out3,
+                 hipStream_t st) {
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    HIP_TRY(launch_sumsq3<R>(a, (long)na, b, (long)nb, c, (long)nc, s->partials, dev_out3, st), "sumsq3");
+    return GLIDE_OK;
+}
+
+int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
+                 hipStream_t st) {
+    const bool freeze_phi = (flags & GLIDE_FREEZE_PHI) != 0;
+    VankaArgs<R> vk;
+    vk.newton_steps = vc.newton_steps;
+    vk.ssa_damping = vc.ssa_damping;
+    vk.mc_damping = vc.mc_damping;
+    vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
+    vk.iters = nullptr;
+    for (int it = 0; it < n_iter; ++it) {
+        if (!freeze_phi) GLIDE_TRY(grounded(L, vc.relax_phi, st));
+        SsaArgs<R> a = make_args(L, dt, false);
+        HIP_TRY(launch_vanka_pass<R>(a, P(L.u), P(L.v), P(L.H), P(L.work), P(L.mask), vk, vc.omega, st), "vanka");
+    }
+    return GLIDE_OK;
+}
+
+int vjp(const glide_level& L, double dt, int flags, const R* lu, const R* lv, const R* lH, R* ou, R* ov, R* oH,
+        hipStream_t st) {
+    SsaArgs<R> a = make_args(L, dt, true);
+    const bool forcing = (flags & GLIDE_USE_FORCING) != 0;
+    const double cr = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
+    HIP_TRY(launch_vjp<R>(a, lu, lv, lH, forcing ? CP(L.af_u) : nullptr, forcing ? CP(L.af_v) : nullptr,
+                          forcing ? CP(L.af_H) : nullptr, ou, ov, oH, cr, st),
+            "vjp");
+    return GLIDE_OK;
+}
+
+int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
+                         hipStream_t st) {
+    VankaArgs<R> vk;
+    vk.newton_steps = 1;
+    vk.ssa_damping = vc.ssa_damping;
+    vk.mc_damping = vc.mc_damping;
+    vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
+    vk.iters = nullptr;
+    for (int it = 0; it < n_iter; ++it) {
+        GLIDE_TRY(vjp(L, dt, flags | GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u),
+                      P(L.ar_v), P(L.ar_H), st));
+        SsaArgs<R> a = make_args(L, dt, true);
+        HIP_TRY(launch_vanka_adjoint_pass<R>(a, CP(L.ar_u), CP(L.ar_v), CP(L.ar_H), P(L.lambda_u), P(L.lambda_v),
+                                             P(L.lambda_H), P(L.work), vk, vc.omega, st),
+                "vanka adjoint");
+    }
+    return GLIDE_OK;
+}
+
+int restrict_u(const void* f, void* c, const glide_level& C, hipStream_t st) {
+    HIP_TRY(launch_restrict_vfacet<R>(CP(f), P(c), C.ny, C.nx, st), "restrict u");
+    return GLIDE_OK;
+}
+int restrict_v(const void* f, void* c, const glide_level& C, hipStream_t st) {
+    HIP_TRY(launch_restrict_hfacet<R>(CP(f), P(c), C.ny, C.nx, st), "restrict v");
+    return GLIDE_OK;
+}
+int restrict_c(const void* f, void* c, const glide_level& C, int use_max, hipStream_t st) {
+    HIP_TRY(launch_restrict_cell<R>(CP(f), P(c), C.ny, C.nx, use_max, st), "restrict cell");
+    return GLIDE_OK;
+}
+int axpby(void* out, double a, const void* x, double b, const void* y, size_t n, hipStream_t st) {
+    HIP_TRY(launch_axpby<R>(P(out), a, CP(x), b, CP(y), 0.0, nullptr, (long)n, st), "axpby");
+    return GLIDE_OK;
+}
+int copy(void* dst, const void* src, size_t n, hipStream_t st) {
+    HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(R), hipMemcpyDeviceToDevice, st), "copy");
+    return GLIDE_OK;
+}
+int fill(void* dst, double val, size_t n, hipStream_t st) {
+    HIP_TRY(launch_fill<R>(P(dst), val, (long)n, st), "fill");
+    return GLIDE_OK;
+}
+
+// Restrict the forward state of level l to level l+1 (glide/multigrid.py:81-87).
+int restrict_state(const glide_level& F, const glide_level& C, hipStream_t st) {
+    GLIDE_TRY(restrict_u(F.u, C.u, C, st));
+    GLIDE_TRY(restrict_v(F.v, C.v, C, st));
+    GLIDE_TRY(restrict_c(F.H, C.H, C, 0, st));
+    GLIDE_TRY(restrict_c(F.H_prev, C.H_prev, C, 0, st));
+    GLIDE_TRY(restrict_c(F.phi, C.phi, C, 0, st));
+    GLIDE_TRY(restrict_c(F.mask, C.mask, C, 1, st));
+    return GLIDE_OK;
+}
+
+int prolong_v(const void* c, void* f, const glide_level& F, int compat, hipStream_t st) {
+    if (compat)
+        HIP_TRY(launch_prolong_vfacet_bilinear<R>(CP(c), P(f), F.ny, F.nx, st), "prolong v (compat)");
+    else
+        HIP_TRY(launch_prolong_hfacet_bilinear<R>(CP(c), P(f), F.ny, F.nx, st), "prolong v");
+    return GLIDE_OK;
+}
+
+struct FasCtx {
+    const glide_level* L;
+    int n_levels;
+    double dt;
+    glide_fas_config cfg;
+    const glide_vanka_config* vc;
+    hipStream_t st;
+};
+
+// FASCDSolver.vcycle (glide/multigrid.py:536-645).
+int fas_vcycle(FasCtx& X, int l, bool finest) {
+    const glide_level& L = X.L[l];
+    const bool coarse = !finest;
+    int sflags = 0;
+    if (coarse && X.cfg.freeze_coarse_calving) sflags |= GLIDE_FREEZE_CALVING;
+    if (coarse && X.cfg.freeze_coarse_phi) sflags |= GLIDE_FREEZE_PHI;
+    const double thk = L.phys.thklim;
+    hipStream_t st = X.st;
+    if (finest) {
+        GLIDE_TRY(copy(L.w_u, L.u, nU(L), st));
+        GLIDE_TRY(copy(L.w_v, L.v, nV(L), st));
+        GLIDE_TRY(copy(L.w_H, L.H, nC(L), st));
+        // chi = thklim - H
+        HIP_TRY(launch_axpby<R>(P(L.chi), -1.0, CP(L.H), 0.0, CP(L.H), 0.0, nullptr, (long)nC(L), st), "chi");
+        HIP_TRY(launch_fill<R>(P(L.gamma), thk, (long)nC(L), st), "fill");
+        GLIDE_TRY(axpby(L.chi, 1.0, L.chi, 1.0, L.gamma, nC(L), st));
+    }
+    if (l == X.n_levels - 1) {
+        GLIDE_TRY(axpby(L.gamma, 1.0, L.w_H, 1.0, L.chi, nC(L), st));
+        GLIDE_TRY(vanka_sweeps(L, X.dt, X.cfg.coarsest_steps, X.vc[l], sflags, st));
+        GLIDE_TRY(fill(L.gamma, thk, nC(L), st));
+        return GLIDE_OK;
+    }
+    const glide_level& N = X.L[l + 1];
+    GLIDE_TRY(restrict_c(L.chi, N.chi, N, 1, st));
+    // cphi = P_inj(-chi_c) + chi
+    HIP_TRY(launch_prolong_cell<R>(CP(N.chi), P(L.cphi), L.ny, L.nx, 0, st), "prolong chi");
+    GLIDE_TRY(axpby(L.cphi, -1.0, L.cphi, 1.0, L.chi, nC(L), st));
+    GLIDE_TRY(axpby(L.gamma, 1.0, L.w_H, 1.0, L.cphi, nC(L), st));
+    GLIDE_TRY(vanka_sweeps(L, X.dt, X.cfg.pre_steps, X.vc[l], sflags, st));
+    GLIDE_TRY(fill(L.gamma, thk, nC(L), st));
+    GLIDE_TRY(axpby(L.y_u, 1.0, L.u, -1.0, L.w_u, nU(L), st));
+    GLIDE_TRY(axpby(L.y_v, 1.0, L.v, -1.0, L.w_v, nV(L), st));
+    GLIDE_TRY(axpby(L.y_H, 1.0, L.H, -1.0, L.w_H, nC(L), st));
+    GLIDE_TRY(restrict_state(L, N, st));
+    GLIDE_TRY(copy(N.w_u, N.u, nU(N), st));
+    GLIDE_TRY(copy(N.w_v, N.v, nV(N), st));
+    GLIDE_TRY(copy(N.w_H, N.H, nC(N), st));
+    GLIDE_TRY(residual(L, X.dt, GLIDE_USE_FORCING | (sflags & GLIDE_FREEZE_CALVING), P(L.r_u), P(L.r_v), P(L.r_H), st));
+    GLIDE_TRY(restrict_u(L.r_u, N.r_u, N, st));
+    GLIDE_TRY(restrict_v(L.r_v, N.r_v, N, st));
+    GLIDE_TRY(restrict_c(L.r_H, N.r_H, N, 0, st));
+    if (!X.cfg.freeze_coarse_phi) GLIDE_TRY(grounded(N, 0.0, st));
+    GLIDE_TRY(residual(N, X.dt, X.cfg.freeze_coarse_calving ? GLIDE_FREEZE_CALVING : 0, P(N.F_u), P(N.F_v), P(N.F_H),
+                       st));
+    GLIDE_TRY(axpby(N.f_u, 1.0, N.F_u, -1.0, N.r_u, nU(N), st));
+    GLIDE_TRY(axpby(N.f_v, 1.0, N.F_v, -1.0, N.r_v, nV(N), st));
+    GLIDE_TRY(axpby(N.f_H, 1.0, N.F_H, -1.0, N.r_H, nC(N), st));
+    GLIDE_TRY(fas_vcycle(X, l + 1, false));
+    GLIDE_TRY(axpby(N.z_u, 1.0, N.u, -1.0, N.w_u, nU(N), st));
+    GLIDE_TRY(axpby(N.z_v, 1.0, N.v, -1.0, N.w_v, nV(N), st));
+    GLIDE_TRY(axpby(N.z_H, 1.0, N.H, -1.0, N.w_H, nC(N), st));
+    HIP_TRY(launch_prolong_vfacet_bilinear<R>(CP(N.z_u), P(L.z_u), L.ny, L.nx, st), "prolong z_u");
+    GLIDE_TRY(prolong_v(N.z_v, L.z_v, L, X.cfg.compat_vprolong, st));
+    HIP_TRY(launch_prolong_cell<R>(CP(N.z_H), P(L.z_H), L.ny, L.nx, 0, st), "prolong z_H");
+    // z += y; x = w + z
+    GLIDE_TRY(axpby(L.z_u, 1.0, L.z_u, 1.0, L.y_u, nU(L), st));
+    GLIDE_TRY(axpby(L.z_v, 1.0, L.z_v, 1.0, L.y_v, nV(L), st));
+    GLIDE_TRY(axpby(L.z_H, 1.0, L.z_H, 1.0, L.y_H, nC(L), st));
+    GLIDE_TRY(axpby(L.u, 1.0, L.w_u, 1.0, L.z_u, nU(L), st));
+    GLIDE_TRY(axpby(L.v, 1.0, L.w_v, 1.0, L.z_v, nV(L), st));
+    GLIDE_TRY(axpby(L.H, 1.0, L.w_H, 1.0, L.z_H, nC(L), st));
+    GLIDE_TRY(axpby(L.gamma, 1.0, L.w_H, 1.0, L.chi, nC(L), st));
+    GLIDE_TRY(vanka_sweeps(L, X.dt, X.cfg.post_steps, X.vc[l], sflags, st));
+    GLIDE_TRY(fill(L.gamma, thk, nC(L), st));
+    if (finest) GLIDE_TRY(vanka_sweeps(L, X.dt, X.cfg.finest_steps, X.vc[l], 0, st));
+    return GLIDE_OK;
+}
+
+// FASAdjointSolver.vcycle (glide/multigrid.py:905-973).
+int adj_vcycle(FasCtx& X, int l, bool finest) {
+    const glide_level& L = X.L[l];
+    hipStream_t st = X.st;
+    if (l == X.n_levels - 1) return vanka_adjoint_sweeps(L, X.dt, X.cfg.coarsest_steps, X.vc[l], 0, st);
+    const glide_level& N = X.L[l + 1];
+    GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.pre_steps, X.vc[l], 0, st));
+    GLIDE_TRY(restrict_state(L, N, st));
+    GLIDE_TRY(restrict_u(L.lambda_u, N.lambda_u, N, st));
+    GLIDE_TRY(restrict_v(L.lambda_v, N.lambda_v, N, st));
+    GLIDE_TRY(restrict_c(L.lambda_H, N.lambda_H, N, 0, st));
+    GLIDE_TRY(copy(N.wl_u, N.lambda_u, nU(N), st));
+    GLIDE_TRY(copy(N.wl_v, N.lambda_v, nV(N), st));
+    GLIDE_TRY(copy(N.wl_H, N.lambda_H, nC(N), st));
+    GLIDE_TRY(vjp(L, X.dt, GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u), P(L.ar_v),
+                  P(L.ar_H), st));
+    GLIDE_TRY(restrict_u(L.ar_u, N.ar_u, N, st));
+    GLIDE_TRY(restrict_v(L.ar_v, N.ar_v, N, st));
+    GLIDE_TRY(restrict_c(L.ar_H, N.ar_H, N, 0, st));
+    GLIDE_TRY(vjp(N, X.dt, 0, CP(N.lambda_u), CP(N.lambda_v), CP(N.lambda_H), P(N.vjp_u), P(N.vjp_v), P(N.vjp_H), st));
+    GLIDE_TRY(axpby(N.af_u, 1.0, N.vjp_u, -1.0, N.ar_u, nU(N), st));
+    GLIDE_TRY(axpby(N.af_v, 1.0, N.vjp_v, -1.0, N.ar_v, nV(N), st));
+    GLIDE_TRY(axpby(N.af_H, 1.0, N.vjp_H, -1.0, N.ar_H, nC(N), st));
+    GLIDE_TRY(adj_vcycle(X, l + 1, false));
+    GLIDE_TRY(axpby(N.zl_u, 1.0, N.lambda_u, -1.0, N.wl_u, nU(N), st));
+    GLIDE_TRY(axpby(N.zl_v, 1.0, N.lambda_v, -1.0, N.wl_v, nV(N), st));
+    GLIDE_TRY(axpby(N.zl_H, 1.0, N.lambda_H, -1.0, N.wl_H, nC(N), st));
+    HIP_TRY(launch_prolong_vfacet_bilinear<R>(CP(N.zl_u), P(L.zl_u), L.ny, L.nx, st), "prolong zl_u");
+    GLIDE_TRY(prolong_v(N.zl_v, L.zl_v, L, X.cfg.compat_vprolong, st));
+    HIP_TRY(launch_prolong_cell<R>(CP(N.zl_H), P(L.zl_H), L.ny, L.nx, 1, st), "prolong zl_H");
+    GLIDE_TRY(axpby(L.lambda_u, 1.0, L.lambda_u, 1.0, L.zl_u, nU(L), st));
+    GLIDE_TRY(axpby(L.lambda_v, 1.0, L.lambda_v, 1.0, L.zl_v, nV(L), st));
+    GLIDE_TRY(axpby(L.lambda_H, 1.0, L.lambda_H, 1.0, L.zl_H, nC(L), st));
+    GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.post_steps, X.vc[l], 0, st));
+    if (finest) GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.finest_steps, X.vc[l], 0, st));
+    return GLIDE_OK;
+}
+
+int check_hierarchy(const glide_level* levels, int n_levels, int start, const glide_fas_config* cfg,
+                    const glide_vanka_config* vc) {
+    if (!levels || !cfg || !vc) return fail(GLIDE_ERR_INVALID_ARG, "levels, cfg and vc must be non-NULL");
+    if (n_levels < 1 || start < 0 || start >= n_levels) return fail(GLIDE_ERR_INVALID_ARG, "bad level range");
+    if (cfg->maximum_vcycles > 64) return fail(GLIDE_ERR_INVALID_ARG, "maximum_vcycles must be <= 64");
+    for (int l = start; l < n_levels; ++l) {
+        GLIDE_TRY(check_level(&levels[l]));
+        GLIDE_TRY(check_ssa_inputs(levels[l]));
+        if (!levels[l].work) return fail(GLIDE_ERR_INVALID_ARG, "level workspace is NULL");
+        if (l > start) {
+            if (levels[l].ny * 2 != levels[l - 1].ny || levels[l].nx * 2 != levels[l - 1].nx)
+                return fail(GLIDE_ERR_SHAPE, "each coarse level must halve the finer level's ny and nx exactly");
+        }
+        if (cfg->compat_vprolong && n_levels - start > 1 && levels[l].ny != levels[l].nx)
+            return fail(GLIDE_ERR_SHAPE, "compat_vprolong requires square levels");
+        if (vc[l].newton_steps < 0) return fail(GLIDE_ERR_INVALID_ARG, "newton_steps must be >= 0");
+    }
+    return GLIDE_OK;
+}
+
+// Shared solve loop of the forward and adjoint FAS drivers
+// (glide/multigrid.py:502-533, 873-903).
+template <typename Residual, typename Cycle>
+int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, Cycle&& cycle) {
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    hipStream_t st = X.st;
+    const bool fixed = !(X.cfg.relative_tolerance > 0.0) && !(X.cfg.absolute_tolerance > 0.0);
+    // initial residual (slot 0)
+    GLIDE_TRY(resid(true, s->norms));
+    double r0 = 0.0, absn = 0.0, rel = 1.0;
+    double n3[3] = {0, 0, 0};
+    auto read_slots = [&](int nslots) -> int {
+        HIP_TRY(hipMemcpyAsync(s->host, s->norms, sizeof(double) * 3 * nslots, hipMemcpyDeviceToHost, st), "norms D2H");
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return GLIDE_OK;
+    };
+    int it = 0;
+    if (fixed) {
+        for (; it < X.cfg.maximum_vcycles; ++it) {
+            GLIDE_TRY(cycle());
+            GLIDE_TRY(resid(false, s->norms + 3 * (it + 1)));
+        }
+        GLIDE_TRY(read_slots(it + 1));
+        r0 = std::sqrt(s->host[0] + s->host[1] + s->host[2]);
+        absn = r0;
+        for (int k = 0; k < it; ++k) {
+            const double* q = s->host + 3 * (k + 1);
+            absn = std::sqrt(q[0] + q[1] + q[2]);
+            if (rep) rep->norms[k] = absn;
+        }
+        const double* q = s->host + 3 * it;
+        n3[0] = q[0]; n3[1] = q[1]; n3[2] = q[2];
+    } else {
+        GLIDE_TRY(read_slots(1));
+        r0 = std::sqrt(s->host[0] + s->host[1] + s->host[2]);
+        n3[0] = s->host[0]; n3[1] = s->host[1]; n3[2] = s->host[2];
+        absn = r0;
+        while (rel > X.cfg.relative_tolerance && absn > X.cfg.absolute_tolerance && it < X.cfg.maximum_vcycles) {
+            GLIDE_TRY(cycle());
+            GLIDE_TRY(resid(false, s->norms));
+            GLIDE_TRY(read_slots(1));
+            n3[0] = s->host[0]; n3[1] = s->host[1]; n3[2] = s->host[2];
+            absn = std::sqrt(n3[0] + n3[1] + n3[2]);
+            rel = absn / r0;
+            if (rep) rep->norms[it] = absn;
+            ++it;
+        }
+    }
+    if (rep) {
+        rep->vcycles = it;
+        rep->initial_norm = r0;
+        rep->final_norm = absn;
+        rep->norms3[0] = std::sqrt(n3[0]); rep->norms3[1] = std::sqrt(n3[1]); rep->norms3[2] = std::sqrt(n3[2]);
+    }
+    return GLIDE_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* glide_version(void) { return "glide-mi355x 0.1.0 (gfx950, FP64)"; }
+
+int glide_last_error(char* buf, size_t len) {
+    const std::string& e = g_last_error;
+    if (buf && len > 0) {
+        size_t n = e.size() < len - 1 ? e.size() : len - 1;
+        std::memcpy(buf, e.data(), n);
+        buf[n] = '\0';
+    }
+    return (int)e.size();
+}
+
+int glide_compute_grounded(const glide_level* L, double relaxation, void* stream) {
+    GLIDE_TRY(check_level(L));
+    REQ(L->phi); REQ(L->H); REQ(L->bed);
+    return grounded(*L, relaxation, (hipStream_t)stream);
+}
+
+int glide_ssa_residual(const glide_level* L, double dt, int flags, void* out_u, void* out_v, void* out_H,
+                       void* stream) {
+    GLIDE_TRY(check_level(L));
+    GLIDE_TRY(check_ssa_inputs(*L));
+    REQ(out_u); REQ(out_v); REQ(out_H);
+    if ((flags & GLIDE_USE_FORCING)) { REQ(L->f_u); REQ(L->f_v); REQ(L->f_H); }
+    if ((flags & GLIDE_USE_MASK)) { REQ(L->gamma); }
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    return residual(*L, dt, flags, P(out_u), P(out_v), P(out_H), (hipStream_t)stream);
+}
+
+int glide_norms3(const void* a, size_t na, const void* b, size_t nb, const void* c, size_t nc, double* out3,
+                 void* stream) {
+    if (!out3) return fail(GLIDE_ERR_INVALID_ARG, "out3 is NULL");
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    hipStream_t st = (hipStream_t)stream;
+    GLIDE_TRY(sumsq3_async(CP(a), a ? na : 0, CP(b), b ? nb : 0, CP(c), c ? nc : 0, s->norms + 3 * 65, st));
+    HIP_TRY(hipMemcpyAsync(s->host + 3 * 65, s->norms + 3 * 65, 3 * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+    for (int k = 0; k < 3; ++k) out3[k] = std::sqrt(s->host[3 * 65 + k]);
+    return GLIDE_OK;
+}
+
+int glide_vanka_sweeps(const glide_level* L, double dt, int n_iter, const glide_vanka_config* vc, int flags,
+                       void* stream) {
+    GLIDE_TRY(check_level(L));
+    GLIDE_TRY(check_ssa_inputs(*L));
+    if (!vc) return fail(GLIDE_ERR_INVALID_ARG, "vc is NULL");
+    REQ(L->f_u); REQ(L->f_v); REQ(L->f_H); REQ(L->gamma); REQ(L->work);
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    if (n_iter < 0) return fail(GLIDE_ERR_INVALID_ARG, "n_iter must be >= 0");
+    return vanka_sweeps(*L, dt, n_iter, *vc, flags, (hipStream_t)stream);
+}
+
+int glide_vanka_dump(const glide_level* L, double dt, void* J, void* r, void* stream) {
+    GLIDE_TRY(check_level(L));
+    GLIDE_TRY(check_ssa_inputs(*L));
+    REQ(J); REQ(r); REQ(L->f_u); REQ(L->f_v); REQ(L->f_H); REQ(L->gamma);
+    SsaArgs<R> a = make_args(*L, dt, false);
+    HIP_TRY(launch_vanka_dump<R>(a, P(J), P(r), L->phys.calving_rate, (hipStream_t)stream), "vanka dump");
+    return GLIDE_OK;
+}
+
+int glide_restrict_vfacet(const void* fine, void* coarse, int ny_coarse, int nx_coarse, void* stream) {
+    if (!fine || !coarse || ny_coarse < 1 || nx_coarse < 1) return fail(GLIDE_ERR_INVALID_ARG, "bad restrict args");
+    HIP_TRY(launch_restrict_vfacet<R>(CP(fine), P(coarse), ny_coarse, nx_coarse, (hipStream_t)stream), "restrict");
+    return GLIDE_OK;
+}
+int glide_restrict_hfacet(const void* fine, void* coarse, int ny_coarse, int nx_coarse, void* stream) {
+    if (!fine || !coarse || ny_coarse < 1 || nx_coarse < 1) return fail(GLIDE_ERR_INVALID_ARG, "bad restrict args");
+    HIP_TRY(launch_restrict_hfacet<R>(CP(fine), P(coarse), ny_coarse, nx_coarse, (hipStream_t)stream), "restrict");
+    return GLIDE_OK;
+}
+int glide_restrict_cell(const void* fine, void* coarse, int ny_coarse, int nx_coarse, int use_max, void* stream) {
+    if (!fine || !coarse || ny_coarse < 1 || nx_coarse < 1) return fail(GLIDE_ERR_INVALID_ARG, "bad restrict args");
+    HIP_TRY(launch_restrict_cell<R>(CP(fine), P(coarse), ny_coarse, nx_coarse, use_max, (hipStream_t)stream),
+            "restrict");
+    return GLIDE_OK;
+}
+int glide_prolong_vfacet_bilinear(const void* coarse, void* fine, int ny_fine, int nx_fine, void* stream) {
+    if (!fine || !coarse || ny_fine < 2 || nx_fine < 2 || (ny_fine & 1) || (nx_fine & 1))
+        return fail(GLIDE_ERR_INVALID_ARG, "bad prolong args");
+    HIP_TRY(launch_prolong_vfacet_bilinear<R>(CP(coarse), P(fine), ny_fine, nx_fine, (hipStream_t)stream), "prolong");
+    return GLIDE_OK;
+}
+int glide_prolong_hfacet_bilinear(const void* coarse, void* fine, int ny_fine, int nx_fine, void* stream) {
+    if (!fine || !coarse || ny_fine < 2 || nx_fine < 2 || (ny_fine & 1) || (nx_fine & 1))
+        return fail(GLIDE_ERR_INVALID_ARG, "bad prolong args");
+    HIP_TRY(launch_prolong_hfacet_bilinear<R>(CP(coarse), P(fine), ny_fine, nx_fine, (hipStream_t)stream), "prolong");
+    return GLIDE_OK;
+}
+int glide_prolong_cell(const void* coarse, void* fine, int ny_fine, int nx_fine, int bilinear, void* stream) {
+    if (!fine || !coarse || ny_fine < 2 || nx_fine < 2 || (ny_fine & 1) || (nx_fine & 1))
+        return fail(GLIDE_ERR_INVALID_ARG, "bad prolong args");
+    HIP_TRY(launch_prolong_cell<R>(CP(coarse), P(fine), ny_fine, nx_fine, bilinear, (hipStream_t)stream), "prolong");
+    return GLIDE_OK;
+}
+
+int glide_fascd_solve(const glide_level* levels, int n_levels, int start_level, double dt,
+                      const glide_fas_config* cfg, const glide_vanka_config* vc, glide_solve_report* report,
+                      void* stream) {
+    GLIDE_TRY(check_hierarchy(levels, n_levels, start_level, cfg, vc));
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    FasCtx X{levels, n_levels, dt, *cfg, vc, (hipStream_t)stream};
+    const glide_level& L = levels[start_level];
+    hipStream_t st = X.st;
+    // set_rhs: f_H = H_prev/dt + smb (glide/operators.py:275-276)
+    HIP_TRY(launch_axpby<R>(P(L.f_H), 1.0 / dt, CP(L.H_prev), 1.0, CP(L.smb), 0.0, nullptr, (long)nC(L), st), "rhs");
+    auto resid = [&](bool initial, double* slot) -> int {
+        if (initial) GLIDE_TRY(grounded(L, 0.0, st));
+        GLIDE_TRY(residual(L, dt, GLIDE_USE_FORCING | GLIDE_USE_MASK, P(L.r_u), P(L.r_v), P(L.r_H), st));
+        return sumsq3_async(CP(L.r_u), nU(L), CP(L.r_v), nV(L), CP(L.r_H), nC(L), slot, st);
+    };
+    auto cycle = [&]() -> int { return fas_vcycle(X, start_level, true); };
+    return solve_loop(X, start_level, report, resid, cycle);
+}
+
+int glide_fascd_vcycle(const glide_level* levels, int n_levels, int level, int finest, double dt,
+                       const glide_fas_config* cfg, const glide_vanka_config* vc, void* stream) {
+    GLIDE_TRY(check_hierarchy(levels, n_levels, level, cfg, vc));
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    FasCtx X{levels, n_levels, dt, *cfg, vc, (hipStream_t)stream};
+    return fas_vcycle(X, level, finest != 0);
+}
+
+int glide_fas_adjoint_vcycle(const glide_level* levels, int n_levels, int level, int finest, double dt,
+                             const glide_fas_config* cfg, const glide_vanka_config* vc, void* stream) {
+    GLIDE_TRY(check_hierarchy(levels, n_levels, level, cfg, vc));
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    FasCtx X{levels, n_levels, dt, *cfg, vc, (hipStream_t)stream};
+    return adj_vcycle(X, level, finest != 0);
+}
+
+int glide_ssa_vjp(const glide_level* L, double dt, int flags, const void* lambda_u, const void* lambda_v,
+                  const void* lambda_H, void* out_u, void* out_v, void* out_H, void* stream) {
+    GLIDE_TRY(check_level(L));
+    GLIDE_TRY(check_ssa_inputs(*L));
+    REQ(lambda_u); REQ(lambda_v); REQ(lambda_H); REQ(out_u); REQ(out_v); REQ(out_H);
+    if (flags & GLIDE_USE_FORCING) { REQ(L->af_u); REQ(L->af_v); REQ(L->af_H); }
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    return vjp(*L, dt, flags, CP(lambda_u), CP(lambda_v), CP(lambda_H), P(out_u), P(out_v), P(out_H),
+               (hipStream_t)stream);
+}
+
+int glide_vanka_adjoint_sweeps(const glide_level* L, double dt, int n_iter, const glide_vanka_config* vc, int flags,
+                               void* stream) {
+    GLIDE_TRY(check_level(L));
+    GLIDE_TRY(check_ssa_inputs(*L));
+    if (!vc) return fail(GLIDE_ERR_INVALID_ARG, "vc is NULL");
+    REQ(L->lambda_u); REQ(L->lambda_v); REQ(L->lambda_H); REQ(L->af_u); REQ(L->af_v); REQ(L->af_H);
+    REQ(L->ar_u); REQ(L->ar_v); REQ(L->ar_H); REQ(L->work);
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    return vanka_adjoint_sweeps(*L, dt, n_iter, *vc, flags, (hipStream_t)stream);
+}
+
+int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_level, double dt,
+                            const glide_fas_config* cfg, const glide_vanka_config* vc, glide_solve_report* report,
+                            void* stream) {
+    GLIDE_TRY(check_hierarchy(levels, n_levels, start_level, cfg, vc));
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    FasCtx X{levels, n_levels, dt, *cfg, vc, (hipStream_t)stream};
+    const glide_level& L = levels[start_level];
+    hipStream_t st = X.st;
+    auto resid = [&](bool, double* slot) -> int {
+        GLIDE_TRY(vjp(L, dt, GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u), P(L.ar_v),
+                      P(L.ar_H), st));
+        return sumsq3_async(CP(L.ar_u), nU(L), CP(L.ar_v), nV(L), CP(L.ar_H), nC(L), slot, st);
+    };
+    auto cycle = [&]() -> int { return adj_vcycle(X, start_level, true); };
+    return solve_loop(X, start_level, report, resid, cycle);
+}
+
+int glide_gradients(const glide_level* L, double dt, int which, void* stream) {
+    GLIDE_TRY(check_level(L));
+    hipStream_t st = (hipStream_t)stream;
+    REQ(L->lambda_H);
+    if (which & 3) {
+        GLIDE_TRY(check_ssa_inputs(*L));
+        REQ(L->lambda_u); REQ(L->lambda_v);
+        if ((which & 1)) REQ(L->g_beta);
+        if ((which & 2)) REQ(L->g_bed);
+        SsaArgs<R> a = make_args(*L, dt, true);
+        HIP_TRY(launch_grad_beta_bed<R>(a, CP(L->lambda_u), CP(L->lambda_v), (which & 1) ? P(L->g_beta) : nullptr,
+                                        (which & 2) ? P(L->g_bed) : nullptr, st),
+                "gradients");
+    }
+    if (which & 4) {
+        REQ(L->g_H_prev);
+        if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+        HIP_TRY(launch_axpby<R>(P(L->g_H_prev), -1.0 / dt, CP(L->lambda_H), 0.0, CP(L->lambda_H), 0.0, nullptr,
+                                (long)nC(*L), st),
+                "grad H_prev");
+    }
+    if (which & 8) {
+        REQ(L->g_smb);
+        HIP_TRY(launch_axpby<R>(P(L->g_smb), -1.0, CP(L->lambda_H), 0.0, CP(L->lambda_H), 0.0, nullptr, (long)nC(*L),
+                                st),
+                "grad smb");
+    }
+    return GLIDE_OK;
+}
+
+}  // extern "C"

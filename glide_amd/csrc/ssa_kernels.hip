@@ -1,0 +1,743 @@
+// gfx950 kernels of the coupled SSA momentum + mass operator:
+//   grounded fraction, residual, additive Vanka smoothing (forward and
+//   adjoint), the transpose operator (VJP) and the parameter gradients.
+//
+// Layout (row-major, as the Python API exposes it): u-facets (ny, nx+1),
+// v-facets (ny+1, nx), cells (ny, nx).  One launch index (i, j) covers the
+// cell (i,j), the u-facet (i,j) and the v-facet (i,j); the index range is
+// [0, ny] x [0, nx] so the last facet column/row is included.
+//
+// Tiles are 64 wide in x so that one wavefront reads one contiguous row
+// segment of every array.  The viscosity of the tile's cells plus a one-cell
+// ring is computed once per block into LDS.  The transpose operator is a
+// deterministic two-phase gather (per-entity partial products in LDS, then one
+// owner per output): no floating-point atomics, so repeated runs are bitwise
+// identical.
+#include <hip/hip_runtime.h>
+
+#include "ssa_device.hpp"
+#include "ssa_kernels.hpp"
+
+namespace glide {
+
+constexpr int TX = 64;  // tile width (one wavefront per row)
+constexpr int TY = 4;   // tile height
+constexpr int NT = TX * TY;
+
+template <typename T>
+__device__ __forceinline__ T cell_viscosity(const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ B,
+                                            int ci, int cj, int ny, int nx, T dxi, const Phys<T>& P) {
+    CellVel<T> c = load_cell_vel(u, v, ci, cj, ny, nx);
+    T e2 = strain_invariant(c, ci, cj, ny, nx, dxi, P.eps_reg);
+    return T(0.5) * B[(long)ci * nx + cj] * pow(e2, P.glen_p);
+}
+
+// ------------------------------------------------------------------ grounded
+template <typename T>
+__global__ __launch_bounds__(256) void k_grounded(T* __restrict__ phi, const T* __restrict__ H,
+                                                  const T* __restrict__ bed, T c, T relax, long n) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    T g = grounded_fraction(H[k], bed[k], c);
+    phi[k] = relax == T(0) ? g : (T(1) - relax) * g + relax * phi[k];
+}
+
+// ------------------------------------------------------------------ residual
+template <typename T>
+__global__ __launch_bounds__(NT) void k_residual(SsaArgs<T> a, T* __restrict__ out_u, T* __restrict__ out_v,
+                                                 T* __restrict__ out_H, int use_forcing, int use_mask,
+                                                 T calving_rate) {
+    __shared__ T s_eta[TY + 2][TX + 2];
+    const int ny = a.ny, nx = a.nx;
+    const T dxi = a.dxi;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        int ci = i0 - 1 + k / (TX + 2), cj = j0 - 1 + k % (TX + 2);
+        T e = T(0);
+        if (ci >= 0 && ci < ny && cj >= 0 && cj < nx) e = cell_viscosity(a.u, a.v, a.B, ci, cj, ny, nx, dxi, a.P);
+        s_eta[k / (TX + 2)][k % (TX + 2)] = e;
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    const int li = threadIdx.y + 1, lj = threadIdx.x + 1;
+    const T* __restrict__ u = a.u;
+    const T* __restrict__ v = a.v;
+    const T* __restrict__ H = a.H;
+    const T* __restrict__ ph = a.phi;
+
+    if (i < ny && j < nx) {
+        const long c = (long)i * nx + j;
+        T Hc = H[c], pc = ph[c];
+        T Hl = H[offC(i, j - 1, ny, nx)], Hr = H[offC(i, j + 1, ny, nx)];
+        T Ht = H[offC(i - 1, j, ny, nx)], Hb = H[offC(i + 1, j, ny, nx)];
+        T rH = Hc / a.dt;
+        if (use_forcing) rH -= a.f_H[c];
+        rH -= flux_x(u[offU(i, j, ny, nx)], Hl, Hc, j > 0).res * dxi;
+        rH += (T(1) - pc) * (T(1) - ph[offC(i, j - 1, ny, nx)]) * calving_rate * Hc * dxi;
+        rH += flux_x(u[offU(i, j + 1, ny, nx)], Hc, Hr, j + 1 < nx).res * dxi;
+        rH += (T(1) - pc) * (T(1) - ph[offC(i, j + 1, ny, nx)]) * calving_rate * Hc * dxi;
+        rH += flux_y(v[offV(i, j, ny, nx)], Ht, Hc, i > 0).res * dxi;
+        rH += (T(1) - pc) * (T(1) - ph[offC(i - 1, j, ny, nx)]) * calving_rate * Hc * dxi;
+        rH -= flux_y(v[offV(i + 1, j, ny, nx)], Hc, Hb, i + 1 < ny).res * dxi;
+        rH += (T(1) - pc) * (T(1) - ph[offC(i + 1, j, ny, nx)]) * calving_rate * Hc * dxi;
+        T mk = use_mask ? a.mask[c] : T(0);
+        out_H[c] = (T(1) - mk) * rH + mk * (Hc - a.gamma[c]);
+    }
+    if (i < ny) {
+        const long o = (long)i * (nx + 1) + j;
+        T ru;
+        if (j == 0 || j == nx) {
+            ru = u[o];
+        } else {
+            ru = use_forcing ? -a.f_u[o] : T(0);
+            T ul = u[o];
+            T Hc = H[(long)i * nx + j], Hl = H[(long)i * nx + j - 1];
+            T vt = v[(long)i * nx + j], vb = v[(long)(i + 1) * nx + j];
+            T vlt = v[(long)i * nx + j - 1], vlb = v[(long)(i + 1) * nx + j - 1];
+            ru += sxx(ul, u[o + 1], vt, vb, s_eta[li][lj] * Hc, dxi, true).res * dxi;
+            ru -= sxx(u[o - 1], ul, vlt, vlb, s_eta[li][lj - 1] * Hl, dxi, true).res * dxi;
+            if (i > 0) {
+                T eH = T(0.25) * (s_eta[li - 1][lj - 1] * H[(long)(i - 1) * nx + j - 1] +
+                                  s_eta[li - 1][lj] * H[(long)(i - 1) * nx + j] + s_eta[li][lj - 1] * Hl +
+                                  s_eta[li][lj] * Hc);
+                ru += sxy(u[o - (nx + 1)], ul, vlt, vt, eH, dxi, true).res * dxi;
+            }
+            if (i + 1 < ny) {
+                T eH = T(0.25) * (s_eta[li][lj - 1] * Hl + s_eta[li][lj] * Hc +
+                                  s_eta[li + 1][lj - 1] * H[(long)(i + 1) * nx + j - 1] +
+                                  s_eta[li + 1][lj] * H[(long)(i + 1) * nx + j]);
+                ru -= sxy(ul, u[o + (nx + 1)], vlb, vb, eH, dxi, true).res * dxi;
+            }
+            T pl = ph[(long)i * nx + j - 1], pc = ph[(long)i * nx + j];
+            ru += basal_drag(ul, vlt, vt, vlb, vb, pl, pc, a.beta[(long)i * nx + j - 1], a.beta[(long)i * nx + j], a.P)
+                      .res;
+            ru -= drive_x(Hl, Hc, a.bed[(long)i * nx + j - 1], a.bed[(long)i * nx + j], pl, pc, dxi, true).res;
+        }
+        out_u[o] = ru;
+    }
+    if (j < nx) {
+        const long o = (long)i * nx + j;
+        T rv;
+        if (i == 0 || i == ny) {
+            rv = v[o];
+        } else {
+            rv = use_forcing ? -a.f_v[o] : T(0);
+            T vt = v[o];
+            T Hc = H[o], Ht = H[o - nx];
+            T ul = u[(long)i * (nx + 1) + j], ur = u[(long)i * (nx + 1) + j + 1];
+            T utl = u[(long)(i - 1) * (nx + 1) + j], utr = u[(long)(i - 1) * (nx + 1) + j + 1];
+            rv += syy(utl, utr, v[o - nx], vt, s_eta[li - 1][lj] * Ht, dxi, true).res * dxi;
+            rv -= syy(ul, ur, vt, v[o + nx], s_eta[li][lj] * Hc, dxi, true).res * dxi;
+            if (j > 0) {
+                T eH = T(0.25) * (s_eta[li - 1][lj - 1] * H[o - nx - 1] + s_eta[li - 1][lj] * Ht +
+                                  s_eta[li][lj - 1] * H[o - 1] + s_eta[li][lj] * Hc);
+                rv -= sxy(utl, ul, v[o - 1], vt, eH, dxi, true).res * dxi;
+            }
+            if (j + 1 < nx) {
+                T eH = T(0.25) * (s_eta[li - 1][lj] * Ht + s_eta[li - 1][lj + 1] * H[o - nx + 1] +
+                                  s_eta[li][lj] * Hc + s_eta[li][lj + 1] * H[o + 1]);
+                rv += sxy(utr, ur, vt, v[o + 1], eH, dxi, true).res * dxi;
+            }
+            T pt = ph[o - nx], pc = ph[o];
+            rv += basal_drag(vt, utl, utr, ul, ur, pt, pc, a.beta[o - nx], a.beta[o], a.P).res;
+            rv -= drive_y(Ht, Hc, a.bed[o - nx], a.bed[o], pt, pc, dxi, true).res;
+        }
+        out_v[o] = rv;
+    }
+}
+
+// ------------------------------------------------------------------ Vanka
+// Gather the patch of cell (i,j); viscosity from the LDS tile (li, lj are the
+// cell's tile coordinates including the one-cell ring).
+template <typename T>
+__device__ __forceinline__ void load_patch(Patch<T>& p, const SsaArgs<T>& a, const T (*s_eta)[TX + 2], int li, int lj,
+                                           int i, int j, const T* __restrict__ fu, const T* __restrict__ fv,
+                                           const T* __restrict__ fH, const T* __restrict__ phi) {
+    const int ny = a.ny, nx = a.nx;
+    p.e_c = s_eta[li][lj]; p.e_l = s_eta[li][lj - 1]; p.e_r = s_eta[li][lj + 1];
+    p.e_t = s_eta[li - 1][lj]; p.e_b = s_eta[li + 1][lj];
+    p.e_tl = s_eta[li - 1][lj - 1]; p.e_tr = s_eta[li - 1][lj + 1];
+    p.e_bl = s_eta[li + 1][lj - 1]; p.e_br = s_eta[li + 1][lj + 1];
+    const T* __restrict__ H = a.H;
+    p.H_l = H[offC(i, j - 1, ny, nx)]; p.H_r = H[offC(i, j + 1, ny, nx)];
+    p.H_t = H[offC(i - 1, j, ny, nx)]; p.H_b = H[offC(i + 1, j, ny, nx)];
+    p.H_tl = H[offC(i - 1, j - 1, ny, nx)]; p.H_tr = H[offC(i - 1, j + 1, ny, nx)];
+    p.H_bl = H[offC(i + 1, j - 1, ny, nx)]; p.H_br = H[offC(i + 1, j + 1, ny, nx)];
+    p.p_c = phi[offC(i, j, ny, nx)]; p.p_l = phi[offC(i, j - 1, ny, nx)]; p.p_r = phi[offC(i, j + 1, ny, nx)];
+    p.p_t = phi[offC(i - 1, j, ny, nx)]; p.p_b = phi[offC(i + 1, j, ny, nx)];
+    p.b_c = a.beta[offC(i, j, ny, nx)]; p.b_l = a.beta[offC(i, j - 1, ny, nx)]; p.b_r = a.beta[offC(i, j + 1, ny, nx)];
+    p.b_t = a.beta[offC(i - 1, j, ny, nx)]; p.b_b = a.beta[offC(i + 1, j, ny, nx)];
+    p.z_c = a.bed[offC(i, j, ny, nx)]; p.z_l = a.bed[offC(i, j - 1, ny, nx)]; p.z_r = a.bed[offC(i, j + 1, ny, nx)];
+    p.z_t = a.bed[offC(i - 1, j, ny, nx)]; p.z_b = a.bed[offC(i + 1, j, ny, nx)];
+    const T* __restrict__ u = a.u;
+    const T* __restrict__ v = a.v;
+    p.u_ll = u[offU(i, j - 1, ny, nx)]; p.u_rr = u[offU(i, j + 2, ny, nx)];
+    p.u_tl = u[offU(i - 1, j, ny, nx)]; p.u_tr = u[offU(i - 1, j + 1, ny, nx)];
+    p.u_bl = u[offU(i + 1, j, ny, nx)]; p.u_br = u[offU(i + 1, j + 1, ny, nx)];
+    p.v_lt = v[offV(i, j - 1, ny, nx)]; p.v_lb = v[offV(i + 1, j - 1, ny, nx)];
+    p.v_rt = v[offV(i, j + 1, ny, nx)]; p.v_rb = v[offV(i + 1, j + 1, ny, nx)];
+    p.v_tt = v[offV(i - 1, j, ny, nx)]; p.v_bb = v[offV(i + 2, j, ny, nx)];
+    if (fu) {
+        p.f0 = fu[offU(i, j, ny, nx)]; p.f1 = fu[offU(i, j + 1, ny, nx)];
+        p.f2 = fv[offV(i, j, ny, nx)]; p.f3 = fv[offV(i + 1, j, ny, nx)];
+        p.f4 = fH[offC(i, j, ny, nx)];
+    } else {
+        p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = T(0);
+    }
+    p.gam = a.gamma ? a.gamma[offC(i, j, ny, nx)] : T(0);
+    p.in_l = j > 0; p.in_r = j + 1 < nx; p.in_t = i > 0; p.in_b = i + 1 < ny;
+    p.cl = j > 0; p.cr = j + 1 < nx; p.ct = i > 0; p.cb = i + 1 < ny;
+    p.vtl = i > 0 && j > 0; p.vtr = i > 0 && j + 1 < nx; p.vbl = i + 1 < ny && j > 0; p.vbr = i + 1 < ny && j + 1 < nx;
+}
+
+// Viscosity tile (+ ring) of cells, optionally refreshing the grounded
+// fraction of the tile's own cells first (the ring recomputes it from H, bed).
+template <typename T>
+__device__ __forceinline__ void fill_eta_tile(T (*s_eta)[TX + 2], const SsaArgs<T>& a, int i0, int j0, int tid) {
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        int ci = i0 - 1 + k / (TX + 2), cj = j0 - 1 + k % (TX + 2);
+        T e = T(0);
+        if (ci >= 0 && ci < a.ny && cj >= 0 && cj < a.nx) e = cell_viscosity(a.u, a.v, a.B, ci, cj, a.ny, a.nx, a.dxi, a.P);
+        s_eta[k / (TX + 2)][k % (TX + 2)] = e;
+    }
+}
+
+// Forward smoothing: every cell runs the local damped Newton iteration on its
+// patch against the state at the start of the pass and writes its increments
+// (half-weighted for facets) into the workspace planes
+// work[0..4] = (du_l, du_r, dv_t, dv_b, dH); the active-set flag goes to mask.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__ work, T* __restrict__ mask,
+                                                   VankaArgs<T> vk) {
+    __shared__ T s_eta[TY + 2][TX + 2];
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    fill_eta_tile(s_eta, a, i0, j0, tid);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    const int ny = a.ny, nx = a.nx;
+    if (i >= ny || j >= nx) return;
+    Patch<T> p;
+    load_patch(p, a, s_eta, threadIdx.y + 1, threadIdx.x + 1, i, j, a.f_u, a.f_v, a.f_H, a.phi);
+    const long c = (long)i * nx + j;
+    T x0[5] = {a.u[(long)i * (nx + 1) + j], a.u[(long)i * (nx + 1) + j + 1], a.v[c], a.v[c + nx], a.H[c]};
+    T x[5], cmp[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) { x[k] = x0[k]; cmp[k] = T(0); }
+    T masked = mask[c];
+    T rnorm = T(1);
+    const T tol = T(1e-6);
+    const T thk = p.gam;
+    int it = 0;
+    while (it < vk.newton_steps && rnorm > tol) {
+        T J[25], r[5];
+        local_system(p, x, J, r, a.P, vk.calving_rate, a.dxi, a.dt);
+        r[0] -= p.f0; r[1] -= p.f1; r[2] -= p.f2; r[3] -= p.f3; r[4] -= p.f4;
+        J[0] -= vk.ssa_damping; J[6] -= vk.ssa_damping; J[12] -= vk.ssa_damping; J[18] -= vk.ssa_damping;
+        J[24] += vk.mc_damping;
+        if (j == 0) { pin_row(J, 0); r[0] = x[0]; }
+        if (j == nx - 1) { pin_row(J, 1); r[1] = x[1]; }
+        if (i == 0) { pin_row(J, 2); r[2] = x[2]; }
+        if (i == ny - 1) { pin_row(J, 3); r[3] = x[3]; }
+        if ((x[4] - a.dt * r[4]) <= thk) {
+            masked = T(1);
+            pin_row(J, 4);
+            r[4] = x[4] - thk;
+        } else {
+            masked = T(0);
+        }
+        T d[5];
+        lu5_solve(J, r, d);
+        rnorm = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3] + r[4] * r[4];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {  // compensated relaxed update, local relaxation 0.5
+            T y = T(-0.5) * d[k] - cmp[k];
+            T t = x[k] + y;
+            cmp[k] = (t - x[k]) - y;
+            x[k] = t;
+        }
+        x[4] = fmax(x[4], thk);
+        ++it;
+    }
+    const long n = (long)ny * nx;
+    work[c] = T(0.5) * (x[0] - x0[0]);
+    work[n + c] = T(0.5) * (x[1] - x0[1]);
+    work[2 * n + c] = T(0.5) * (x[2] - x0[2]);
+    work[3 * n + c] = T(0.5) * (x[3] - x0[3]);
+    work[4 * n + c] = x[4] - x0[4];
+    mask[c] = masked;
+    if (vk.iters) vk.iters[c] = it;
+}
+
+// x += sign * omega * (sum of the cell increments touching each unknown).
+template <typename T>
+__global__ __launch_bounds__(256) void k_vanka_apply(T* __restrict__ u, T* __restrict__ v, T* __restrict__ H,
+                                                     const T* __restrict__ work, int ny, int nx, T omega) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j > nx) return;
+    const long n = (long)ny * nx;
+    if (i < ny) {
+        T d = T(0);
+        if (j < nx) d = work[(long)i * nx + j];
+        if (j > 0) d = d + work[n + (long)i * nx + j - 1];
+        u[(long)i * (nx + 1) + j] += omega * d;
+    }
+    if (j < nx) {
+        T d = T(0);
+        if (i < ny) d = work[2 * n + (long)i * nx + j];
+        if (i > 0) d = d + work[3 * n + (long)(i - 1) * nx + j];
+        v[(long)i * nx + j] += omega * d;
+        if (i < ny) H[(long)i * nx + j] += omega * work[4 * n + (long)i * nx + j];
+    }
+}
+
+// Local systems at the current state (debug / parity dump).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vanka_dump(SsaArgs<T> a, T* __restrict__ Jout, T* __restrict__ rout,
+                                                   T calving_rate) {
+    __shared__ T s_eta[TY + 2][TX + 2];
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    fill_eta_tile(s_eta, a, i0, j0, tid);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    const int ny = a.ny, nx = a.nx;
+    if (i >= ny || j >= nx) return;
+    Patch<T> p;
+    load_patch(p, a, s_eta, threadIdx.y + 1, threadIdx.x + 1, i, j, a.f_u, a.f_v, a.f_H, a.phi);
+    const long c = (long)i * nx + j;
+    T x[5] = {a.u[(long)i * (nx + 1) + j], a.u[(long)i * (nx + 1) + j + 1], a.v[c], a.v[c + nx], a.H[c]};
+    T J[25], r[5];
+    local_system(p, x, J, r, a.P, calving_rate, a.dxi, a.dt);
+    r[0] -= p.f0; r[1] -= p.f1; r[2] -= p.f2; r[3] -= p.f3; r[4] -= p.f4;
+    if (j == 0) { pin_row(J, 0); r[0] = x[0]; }
+    if (j == nx - 1) { pin_row(J, 1); r[1] = x[1]; }
+    if (i == 0) { pin_row(J, 2); r[2] = x[2]; }
+    if (i == ny - 1) { pin_row(J, 3); r[3] = x[3]; }
+    if ((x[4] - a.dt * r[4]) <= p.gam) { pin_row(J, 4); r[4] = x[4] - p.gam; }
+    for (int k = 0; k < 25; ++k) Jout[25 * c + k] = J[k];
+    for (int k = 0; k < 5; ++k) rout[5 * c + k] = r[k];
+}
+
+// Adjoint smoothing: J_locᵀ d = r_adj at the forward state, increments into
+// the workspace planes (glide/cuda/vanka.cu:662-797).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vanka_adjoint_cell(SsaArgs<T> a, const T* __restrict__ ru,
+                                                           const T* __restrict__ rv, const T* __restrict__ rH,
+                                                           T* __restrict__ work, VankaArgs<T> vk) {
+    __shared__ T s_eta[TY + 2][TX + 2];
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    fill_eta_tile(s_eta, a, i0, j0, tid);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    const int ny = a.ny, nx = a.nx;
+    if (i >= ny || j >= nx) return;
+    Patch<T> p;
+    load_patch(p, a, s_eta, threadIdx.y + 1, threadIdx.x + 1, i, j, (const T*)nullptr, (const T*)nullptr,
+               (const T*)nullptr, a.phi);
+    const long c = (long)i * nx + j;
+    T x[5] = {a.u[(long)i * (nx + 1) + j], a.u[(long)i * (nx + 1) + j + 1], a.v[c], a.v[c + nx], a.H[c]};
+    T J[25], r[5];
+    local_system(p, x, J, r, a.P, vk.calving_rate, a.dxi, a.dt);
+    J[0] -= vk.ssa_damping; J[6] -= vk.ssa_damping; J[12] -= vk.ssa_damping; J[18] -= vk.ssa_damping;
+    J[24] += vk.mc_damping;
+    T b[5] = {ru[(long)i * (nx + 1) + j], ru[(long)i * (nx + 1) + j + 1], rv[c], rv[c + nx], rH[c]};
+    if (j == 0) { pin_row(J, 0); b[0] = T(0); }
+    if (j == nx - 1) { pin_row(J, 1); b[1] = T(0); }
+    if (i == 0) { pin_row(J, 2); b[2] = T(0); }
+    if (i == ny - 1) { pin_row(J, 3); b[3] = T(0); }
+    if (a.mask[c] > T(0.5)) { pin_row(J, 4); b[4] = T(0); }
+    T JT[25];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) JT[q * 5 + s] = J[s * 5 + q];
+    T d[5];
+    lu5_solve(JT, b, d);
+    const long n = (long)ny * nx;
+    work[c] = T(0.5) * d[0];
+    work[n + c] = T(0.5) * d[1];
+    work[2 * n + c] = T(0.5) * d[2];
+    work[3 * n + c] = T(0.5) * d[3];
+    work[4 * n + c] = d[4];
+}
+
+// ------------------------------------------------------------------ transpose operator
+// Jᵀλ (minus the adjoint forcing when use_forcing) as a gather.  Phase 1
+// computes, for the cells of the tile plus a one-cell ring, the masked
+// thickness multiplier, viscosity, its derivative along λ and the normal
+// stress rows applied to λ; phase 2 the shear-stress rows of the vertices and
+// the drag cross-terms of the facets; phase 3 gives every output its sum in a
+// fixed order.  Velocity rows use the symmetry of the velocity block as the
+// reference does (glide/cuda/residuals.cu:797-817); Dirichlet facets and
+// masked cells return zero (residuals.cu:1122-1160).
+template <typename T>
+struct VjpTile {
+    T lHm[TY + 2][TX + 2];
+    T eta[TY + 2][TX + 2];
+    T etad[TY + 2][TX + 2];
+    T Hs[TY + 2][TX + 2];
+    T Lxx[TY + 2][TX + 2];
+    T Lyy[TY + 2][TX + 2];
+    T Kxx[TY + 2][TX + 2];   // d sigma_xx / d(eta H) at the cell
+    T Kyy[TY + 2][TX + 2];
+    T Lxy[TY + 1][TX + 1];   // vertices [i0, i0+TY] x [j0, j0+TX]
+    T Gxy[TY + 1][TX + 1];   // d sigma_xy / d(eta H) * 0.25 * (λ combination) * dxi
+    T Pu[4][TY + 1][TX + 1]; // λu * drag cross-derivatives, u facets rows [i0-1, i0+TY-1], cols [j0, j0+TX]
+    T Pv[4][TY + 1][TX + 1]; // λv * drag cross-derivatives, v facets rows [i0, i0+TY], cols [j0-1, j0+TX-1]
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vjp(SsaArgs<T> a, const T* __restrict__ lu, const T* __restrict__ lv,
+                                            const T* __restrict__ lH, const T* __restrict__ fu,
+                                            const T* __restrict__ fv, const T* __restrict__ fH, T* __restrict__ out_u,
+                                            T* __restrict__ out_v, T* __restrict__ out_H, T calving_rate) {
+    __shared__ VjpTile<T> s;
+    const int ny = a.ny, nx = a.nx;
+    const T dxi = a.dxi;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    const T* __restrict__ u = a.u;
+    const T* __restrict__ v = a.v;
+    const T* __restrict__ H = a.H;
+
+    // phase 1: cells
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        const int ri = k / (TX + 2), rj = k % (TX + 2);
+        const int ci = i0 - 1 + ri, cj = j0 - 1 + rj;
+        T lHm = 0, e = 0, ed = 0, Hh = 0, Lx = 0, Ly = 0, Kx = 0, Ky = 0;
+        if (ci >= 0 && ci < ny && cj >= 0 && cj < nx) {
+            const long c = (long)ci * nx + cj;
+            CellVel<T> cv = load_cell_vel(u, v, ci, cj, ny, nx);
+            CellVel<T> cl = load_cell_vel(lu, lv, ci, cj, ny, nx);
+            T e2 = strain_invariant(cv, ci, cj, ny, nx, dxi, a.P.eps_reg);
+            T de2 = strain_invariant_dir(cv, cl, ci, cj, ny, nx, dxi);
+            T b = a.B[c];
+            e = T(0.5) * b * pow(e2, a.P.glen_p);
+            ed = T(0.5) * b * a.P.glen_p * pow(e2, a.P.glen_p - T(1)) * de2;
+            Hh = H[c];
+            lHm = lH[c] * (T(1) - a.mask[c]);
+            NormalStress<T> sx = sxx(cv.ul, cv.ur, cv.vt, cv.vb, e * Hh, dxi, true);
+            Lx = sx.d_ul * cl.ul + sx.d_ur * cl.ur + sx.d_vt * cl.vt + sx.d_vb * cl.vb + sx.d_eH * (Hh * ed);
+            Kx = sx.d_eH;
+            NormalStress<T> sy = syy(cv.ul, cv.ur, cv.vt, cv.vb, e * Hh, dxi, true);
+            Ly = sy.d_ul * cl.ul + sy.d_ur * cl.ur + sy.d_vt * cl.vt + sy.d_vb * cl.vb + sy.d_eH * (Hh * ed);
+            Ky = sy.d_eH;
+        }
+        s.lHm[ri][rj] = lHm; s.eta[ri][rj] = e; s.etad[ri][rj] = ed; s.Hs[ri][rj] = Hh;
+        s.Lxx[ri][rj] = Lx; s.Lyy[ri][rj] = Ly; s.Kxx[ri][rj] = Kx; s.Kyy[ri][rj] = Ky;
+    }
+    __syncthreads();
+    // phase 2a: vertices (vi, vj) in [i0, i0+TY] x [j0, j0+TX]
+    for (int k = tid; k < (TY + 1) * (TX + 1); k += NT) {
+        const int ri = k / (TX + 1), rj = k % (TX + 1);
+        const int vi = i0 + ri, vj = j0 + rj;
+        T L = 0, G = 0;
+        if (vi > 0 && vi < ny && vj > 0 && vj < nx) {
+            // cells (vi-1,vj-1), (vi-1,vj), (vi,vj-1), (vi,vj) at tile rows ri, ri+1 and cols rj, rj+1
+            T eH = T(0.25) * (s.eta[ri][rj] * s.Hs[ri][rj] + s.eta[ri][rj + 1] * s.Hs[ri][rj + 1] +
+                              s.eta[ri + 1][rj] * s.Hs[ri + 1][rj] + s.eta[ri + 1][rj + 1] * s.Hs[ri + 1][rj + 1]);
+            T eHd = T(0.25) * (s.etad[ri][rj] * s.Hs[ri][rj] + s.etad[ri][rj + 1] * s.Hs[ri][rj + 1] +
+                               s.etad[ri + 1][rj] * s.Hs[ri + 1][rj] + s.etad[ri + 1][rj + 1] * s.Hs[ri + 1][rj + 1]);
+            const long ut = (long)(vi - 1) * (nx + 1) + vj, ub = (long)vi * (nx + 1) + vj;
+            const long vl = (long)vi * nx + vj - 1, vr = (long)vi * nx + vj;
+            ShearStress<T> q = sxy(u[ut], u[ub], v[vl], v[vr], eH, dxi, true);
+            L = q.d_ut * lu[ut] + q.d_ub * lu[ub] + q.d_vl * lv[vl] + q.d_vr * lv[vr] + q.d_eH * eHd;
+            G = q.d_eH * T(0.25) * (lu[ub] - lu[ut] + lv[vl] - lv[vr]) * dxi;
+        }
+        s.Lxy[ri][rj] = L; s.Gxy[ri][rj] = G;
+    }
+    // phase 2b: drag cross-terms of u facets (rows i0-1.., cols j0..) and v facets (rows i0.., cols j0-1..)
+    for (int k = tid; k < (TY + 1) * (TX + 1); k += NT) {
+        const int ri = k / (TX + 1), rj = k % (TX + 1);
+        {
+            const int fi = i0 - 1 + ri, fj = j0 + rj;
+            T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+            if (fi >= 0 && fi < ny && fj >= 0 && fj <= nx && !a.P.linear_drag) {
+                Drag<T> t = basal_drag(u[(long)fi * (nx + 1) + fj], v[offV(fi, fj - 1, ny, nx)], v[offV(fi, fj, ny, nx)],
+                                       v[offV(fi + 1, fj - 1, ny, nx)], v[offV(fi + 1, fj, ny, nx)],
+                                       a.phi[offC(fi, fj - 1, ny, nx)], a.phi[offC(fi, fj, ny, nx)],
+                                       a.beta[offC(fi, fj - 1, ny, nx)], a.beta[offC(fi, fj, ny, nx)], a.P);
+                T l = lu[(long)fi * (nx + 1) + fj];
+                q0 = l * t.c0; q1 = l * t.c1; q2 = l * t.c2; q3 = l * t.c3;
+            }
+            s.Pu[0][ri][rj] = q0; s.Pu[1][ri][rj] = q1; s.Pu[2][ri][rj] = q2; s.Pu[3][ri][rj] = q3;
+        }
+        {
+            const int fi = i0 + ri, fj = j0 - 1 + rj;
+            T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+            if (fi >= 0 && fi <= ny && fj >= 0 && fj < nx && !a.P.linear_drag) {
+                Drag<T> t = basal_drag(v[(long)fi * nx + fj], u[offU(fi - 1, fj, ny, nx)], u[offU(fi - 1, fj + 1, ny, nx)],
+                                       u[offU(fi, fj, ny, nx)], u[offU(fi, fj + 1, ny, nx)],
+                                       a.phi[offC(fi - 1, fj, ny, nx)], a.phi[offC(fi, fj, ny, nx)],
+                                       a.beta[offC(fi - 1, fj, ny, nx)], a.beta[offC(fi, fj, ny, nx)], a.P);
+                T l = lv[(long)fi * nx + fj];
+                q0 = l * t.c0; q1 = l * t.c1; q2 = l * t.c2; q3 = l * t.c3;
+            }
+            s.Pv[0][ri][rj] = q0; s.Pv[1][ri][rj] = q1; s.Pv[2][ri][rj] = q2; s.Pv[3][ri][rj] = q3;
+        }
+    }
+    __syncthreads();
+
+    // phase 3: outputs
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    const int ci = threadIdx.y + 1, cj = threadIdx.x + 1;  // cell (i,j) in the ring tile
+    const int vi = threadIdx.y, vj = threadIdx.x;          // vertex (i,j) in the vertex tile
+    if (i < ny) {  // u facet (i,j)
+        const long o = (long)i * (nx + 1) + j;
+        T acc = T(0);
+        if (j > 0 && j < nx) {
+            acc = (s.Lxx[ci][cj] - s.Lxx[ci][cj - 1]) * dxi + (s.Lxy[vi][vj] - s.Lxy[vi + 1][vj]) * dxi;
+            const long cc = (long)i * nx + j;
+            T pl = a.phi[cc - 1], pc = a.phi[cc];
+            Drag<T> t = basal_drag(u[o], v[cc - 1], v[cc], v[cc + nx - 1], v[cc + nx], pl, pc, a.beta[cc - 1],
+                                   a.beta[cc], a.P);
+            acc += lu[o] * t.d_w;
+            // cross terms from v facets (i,j) [ubl], (i,j-1) [ubr], (i+1,j) [utl], (i+1,j-1) [utr]
+            acc += s.Pv[2][vi][cj] + s.Pv[3][vi][cj - 1] + s.Pv[0][vi + 1][cj] + s.Pv[1][vi + 1][cj - 1];
+            Flux<T> f = flux_x(u[o], H[cc - 1], H[cc], true);
+            acc += f.d_w * (s.lHm[ci][cj - 1] - s.lHm[ci][cj]) * dxi;
+            if (fu) acc -= fu[o];
+        }
+        out_u[o] = acc;
+    }
+    if (j < nx) {  // v facet (i,j)
+        const long o = (long)i * nx + j;
+        T acc = T(0);
+        if (i > 0 && i < ny) {
+            acc = (s.Lyy[ci - 1][cj] - s.Lyy[ci][cj]) * dxi + (s.Lxy[vi][vj + 1] - s.Lxy[vi][vj]) * dxi;
+            const long ul = (long)i * (nx + 1) + j, utl = ul - (nx + 1);
+            T pt = a.phi[o - nx], pc = a.phi[o];
+            Drag<T> t = basal_drag(v[o], u[utl], u[utl + 1], u[ul], u[ul + 1], pt, pc, a.beta[o - nx], a.beta[o], a.P);
+            acc += lv[o] * t.d_w;
+            // cross terms from u facets (i,j) [vtr], (i,j+1) [vtl], (i-1,j) [vbr], (i-1,j+1) [vbl]
+            acc += s.Pu[1][vi + 1][vj] + s.Pu[0][vi + 1][vj + 1] + s.Pu[3][vi][vj] + s.Pu[2][vi][vj + 1];
+            Flux<T> f = flux_y(v[o], H[o - nx], H[o], true);
+            acc += f.d_w * (s.lHm[ci][cj] - s.lHm[ci - 1][cj]) * dxi;
+            if (fv) acc -= fv[o];
+        }
+        out_v[o] = acc;
+    }
+    if (i < ny && j < nx) {  // cell (i,j)
+        const long c = (long)i * nx + j;
+        T acc = T(0);
+        if (!(a.mask[c] > T(0))) {
+            const T lc = s.lHm[ci][cj];
+            const T Hc = s.Hs[ci][cj], ec = s.eta[ci][cj];
+            const long ol = (long)i * (nx + 1) + j;
+            acc = lc / a.dt;
+            // thickness fluxes through the four faces
+            {
+                Flux<T> f = flux_x(u[ol], H[offC(i, j - 1, ny, nx)], Hc, j > 0);
+                acc += (s.lHm[ci][cj - 1] - lc) * f.d_HB * dxi;
+            }
+            {
+                Flux<T> f = flux_x(u[ol + 1], Hc, H[offC(i, j + 1, ny, nx)], j + 1 < nx);
+                acc += (lc - s.lHm[ci][cj + 1]) * f.d_HA * dxi;
+            }
+            {
+                Flux<T> f = flux_y(v[c], H[offC(i - 1, j, ny, nx)], Hc, i > 0);
+                acc += (lc - s.lHm[ci - 1][cj]) * f.d_HB * dxi;
+            }
+            {
+                Flux<T> f = flux_y(v[c + nx], Hc, H[offC(i + 1, j, ny, nx)], i + 1 < ny);
+                acc += (s.lHm[ci + 1][cj] - lc) * f.d_HA * dxi;
+            }
+            if (calving_rate != T(0)) {
+                const T pc = a.phi[c];
+                T k = (T(1) - pc) * ((T(1) - a.phi[offC(i, j - 1, ny, nx)]) + (T(1) - a.phi[offC(i, j + 1, ny, nx)]) +
+                                     (T(1) - a.phi[offC(i - 1, j, ny, nx)]) + (T(1) - a.phi[offC(i + 1, j, ny, nx)]));
+                acc += lc * k * calving_rate * dxi;
+            }
+            // viscosity-thickness products of the cell's normal stresses
+            acc += s.Kxx[ci][cj] * ec * (lu[ol] - lu[ol + 1]) * dxi;
+            acc += s.Kyy[ci][cj] * ec * (lv[c + nx] - lv[c]) * dxi;
+            // four corner vertices
+            acc += (s.Gxy[vi][vj] + s.Gxy[vi][vj + 1] + s.Gxy[vi + 1][vj] + s.Gxy[vi + 1][vj + 1]) * ec;
+            // driving stress on the four faces
+            {
+                const T pc = a.phi[c], zc = a.bed[c];
+                if (j > 0) {
+                    Drive<T> d = drive_x(H[c - 1], Hc, a.bed[c - 1], zc, a.phi[c - 1], pc, dxi, true);
+                    acc -= lu[ol] * d.d_HB;
+                }
+                if (j + 1 < nx) {
+                    Drive<T> d = drive_x(Hc, H[c + 1], zc, a.bed[c + 1], pc, a.phi[c + 1], dxi, true);
+                    acc -= lu[ol + 1] * d.d_HA;
+                }
+                if (i > 0) {
+                    Drive<T> d = drive_y(H[c - nx], Hc, a.bed[c - nx], zc, a.phi[c - nx], pc, dxi, true);
+                    acc -= lv[c] * d.d_HB;
+                }
+                if (i + 1 < ny) {
+                    Drive<T> d = drive_y(Hc, H[c + nx], zc, a.bed[c + nx], pc, a.phi[c + nx], dxi, true);
+                    acc -= lv[c + nx] * d.d_HA;
+                }
+            }
+            if (fH) acc -= fH[c];
+        }
+        out_H[c] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ gradients
+// dJ/dbeta and dJ/dbed (glide/cuda/grad.cu:1-169) as a per-cell gather,
+// keeping the reference's facet guards (j < nx-1, i < ny-1) on the
+// right/bottom-cell contributions.
+template <typename T>
+__global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __restrict__ lu,
+                                                       const T* __restrict__ lv, T* __restrict__ gbeta,
+                                                       T* __restrict__ gbed) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    const int ny = a.ny, nx = a.nx;
+    if (j >= nx) return;
+    const T* __restrict__ u = a.u;
+    const T* __restrict__ v = a.v;
+    const T* __restrict__ H = a.H;
+    const T* __restrict__ ph = a.phi;
+    const T* __restrict__ bt = a.beta;
+    const T* __restrict__ z = a.bed;
+    T gb = T(0), gz = T(0);
+    const long c = (long)i * nx + j;
+    {  // u facet (i, j+1): this cell is its left cell (always counted)
+        const int fj = j + 1;
+        const long o = (long)i * (nx + 1) + fj;
+        if (gbeta) {
+            Drag<T> t = basal_drag(u[o], v[offV(i, fj - 1, ny, nx)], v[offV(i, fj, ny, nx)], v[offV(i + 1, fj - 1, ny, nx)],
+                                   v[offV(i + 1, fj, ny, nx)], ph[offC(i, fj - 1, ny, nx)], ph[offC(i, fj, ny, nx)],
+                                   bt[offC(i, fj - 1, ny, nx)], bt[offC(i, fj, ny, nx)], a.P);
+            gb += lu[o] * t.d_bA;
+        }
+        if (gbed) {
+            Drive<T> d = drive_x(H[c], H[offC(i, fj, ny, nx)], z[c], z[offC(i, fj, ny, nx)], ph[c],
+                                 ph[offC(i, fj, ny, nx)], a.dxi, fj < nx);
+            gz -= lu[o] * d.d_zA;
+        }
+    }
+    if (j < nx - 1) {  // u facet (i, j): this cell is its right cell
+        const long o = (long)i * (nx + 1) + j;
+        if (gbeta) {
+            Drag<T> t = basal_drag(u[o], v[offV(i, j - 1, ny, nx)], v[offV(i, j, ny, nx)], v[offV(i + 1, j - 1, ny, nx)],
+                                   v[offV(i + 1, j, ny, nx)], ph[offC(i, j - 1, ny, nx)], ph[c],
+                                   bt[offC(i, j - 1, ny, nx)], bt[c], a.P);
+            gb += lu[o] * t.d_bB;
+        }
+        if (gbed) {
+            Drive<T> d = drive_x(H[offC(i, j - 1, ny, nx)], H[c], z[offC(i, j - 1, ny, nx)], z[c],
+                                 ph[offC(i, j - 1, ny, nx)], ph[c], a.dxi, j > 0);
+            gz -= lu[o] * d.d_zB;
+        }
+    }
+    {  // v facet (i+1, j): this cell is its top cell
+        const int fi = i + 1;
+        const long o = (long)fi * nx + j;
+        if (gbeta) {
+            Drag<T> t = basal_drag(v[o], u[offU(fi - 1, j, ny, nx)], u[offU(fi - 1, j + 1, ny, nx)], u[offU(fi, j, ny, nx)],
+                                   u[offU(fi, j + 1, ny, nx)], ph[c], ph[offC(fi, j, ny, nx)], bt[c],
+                                   bt[offC(fi, j, ny, nx)], a.P);
+            gb += lv[o] * t.d_bA;
+        }
+        if (gbed) {
+            Drive<T> d = drive_y(H[c], H[offC(fi, j, ny, nx)], z[c], z[offC(fi, j, ny, nx)], ph[c],
+                                 ph[offC(fi, j, ny, nx)], a.dxi, fi < ny);
+            gz -= lv[o] * d.d_zA;
+        }
+    }
+    if (i < ny - 1) {  // v facet (i, j): this cell is its bottom cell
+        const long o = (long)i * nx + j;
+        if (gbeta) {
+            Drag<T> t = basal_drag(v[o], u[offU(i - 1, j, ny, nx)], u[offU(i - 1, j + 1, ny, nx)], u[offU(i, j, ny, nx)],
+                                   u[offU(i, j + 1, ny, nx)], ph[offC(i - 1, j, ny, nx)], ph[c],
+                                   bt[offC(i - 1, j, ny, nx)], bt[c], a.P);
+            gb += lv[o] * t.d_bB;
+        }
+        if (gbed) {
+            Drive<T> d = drive_y(H[offC(i - 1, j, ny, nx)], H[c], z[offC(i - 1, j, ny, nx)], z[c],
+                                 ph[offC(i - 1, j, ny, nx)], ph[c], a.dxi, i > 0);
+            gz -= lv[o] * d.d_zB;
+        }
+    }
+    if (gbeta) gbeta[c] = gb;
+    if (gbed) gbed[c] = gz;
+}
+
+// ------------------------------------------------------------------ launchers
+static inline dim3 tile_grid(int ny, int nx, bool with_facets) {
+    int gx = with_facets ? (nx + 1 + TX - 1) / TX : (nx + TX - 1) / TX;
+    int gy = with_facets ? (ny + 1 + TY - 1) / TY : (ny + TY - 1) / TY;
+    return dim3(gx, gy);
+}
+
+template <typename T>
+hipError_t launch_grounded(T* phi, const T* H, const T* bed, T c, T relax, long n, hipStream_t st) {
+    k_grounded<T><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(phi, H, bed, c, relax, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_residual(const SsaArgs<T>& a, T* ou, T* ov, T* oH, int use_forcing, int use_mask, T calving_rate,
+                           hipStream_t st) {
+    k_residual<T><<<tile_grid(a.ny, a.nx, true), dim3(TX, TY), 0, st>>>(a, ou, ov, oH, use_forcing, use_mask,
+                                                                         calving_rate);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* mask, const VankaArgs<T>& vk, T omega,
+                             hipStream_t st) {
+    k_vanka_cell<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, work, mask, vk);
+    dim3 g((a.nx + 1 + 255) / 256, a.ny + 1);
+    k_vanka_apply<T><<<g, 256, 0, st>>>(u, v, H, work, a.ny, a.nx, omega);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st) {
+    k_vanka_dump<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, J, r, calving_rate);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* rv, const T* rH, T* lu, T* lv, T* lH,
+                                     T* work, const VankaArgs<T>& vk, T omega, hipStream_t st) {
+    k_vanka_adjoint_cell<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, ru, rv, rH, work, vk);
+    dim3 g((a.nx + 1 + 255) / 256, a.ny + 1);
+    k_vanka_apply<T><<<g, 256, 0, st>>>(lu, lv, lH, work, a.ny, a.nx, -omega);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH, const T* fu, const T* fv,
+                      const T* fH, T* ou, T* ov, T* oH, T calving_rate, hipStream_t st) {
+    k_vjp<T><<<tile_grid(a.ny, a.nx, true), dim3(TX, TY), 0, st>>>(a, lu, lv, lH, fu, fv, fH, ou, ov, oH,
+                                                                    calving_rate);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st) {
+    dim3 g((a.nx + 255) / 256, a.ny);
+    k_grad_beta_bed<T><<<g, 256, 0, st>>>(a, lu, lv, gbeta, gbed);
+    return hipGetLastError();
+}
+
+#define GLIDE_INSTANTIATE(T)                                                                                         \
+    template hipError_t launch_grounded<T>(T*, const T*, const T*, T, T, long, hipStream_t);                          \
+    template hipError_t launch_residual<T>(const SsaArgs<T>&, T*, T*, T*, int, int, T, hipStream_t);                  \
+    template hipError_t launch_vanka_pass<T>(const SsaArgs<T>&, T*, T*, T*, T*, T*, const VankaArgs<T>&, T,           \
+                                             hipStream_t);                                                           \
+    template hipError_t launch_vanka_dump<T>(const SsaArgs<T>&, T*, T*, T, hipStream_t);                              \
+    template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
+                                                     const VankaArgs<T>&, T, hipStream_t);                           \
+    template hipError_t launch_vjp<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, const T*, const T*, \
+                                      T*, T*, T*, T, hipStream_t);                                                   \
+    template hipError_t launch_grad_beta_bed<T>(const SsaArgs<T>&, const T*, const T*, T*, T*, hipStream_t);
+
+GLIDE_INSTANTIATE(double)
+
+}  // namespace glide

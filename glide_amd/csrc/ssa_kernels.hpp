@@ -1,0 +1,72 @@
+// Host-visible argument packs and launchers of the SSA kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ssa_device.hpp"
+
+namespace glide {
+
+// Read-only view of one level for the SSA kernels.
+template <typename T>
+struct SsaArgs {
+    int ny, nx;
+    T dxi, dt;
+    Phys<T> P;
+    const T *u, *v, *H, *phi, *mask, *bed, *B, *beta, *gamma;
+    const T *f_u, *f_v, *f_H;
+};
+
+template <typename T>
+struct VankaArgs {
+    int newton_steps;
+    T ssa_damping, mc_damping, calving_rate;
+    int* iters;  // optional per-cell Newton iteration count
+};
+
+template <typename T>
+hipError_t launch_grounded(T* phi, const T* H, const T* bed, T c, T relax, long n, hipStream_t st);
+template <typename T>
+hipError_t launch_residual(const SsaArgs<T>& a, T* ou, T* ov, T* oH, int use_forcing, int use_mask, T calving_rate,
+                           hipStream_t st);
+template <typename T>
+hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* mask, const VankaArgs<T>& vk, T omega,
+                             hipStream_t st);
+template <typename T>
+hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st);
+template <typename T>
+hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* rv, const T* rH, T* lu, T* lv, T* lH,
+                                     T* work, const VankaArgs<T>& vk, T omega, hipStream_t st);
+template <typename T>
+hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH, const T* fu, const T* fv,
+                      const T* fH, T* ou, T* ov, T* oH, T calving_rate, hipStream_t st);
+template <typename T>
+hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st);
+
+// transfers and element-wise helpers (transfer_kernels.hip)
+template <typename T>
+hipError_t launch_restrict_vfacet(const T* f, T* c, int nyc, int nxc, hipStream_t st);
+template <typename T>
+hipError_t launch_restrict_hfacet(const T* f, T* c, int nyc, int nxc, hipStream_t st);
+template <typename T>
+hipError_t launch_restrict_cell(const T* f, T* c, int nyc, int nxc, int use_max, hipStream_t st);
+template <typename T>
+hipError_t launch_prolong_vfacet_bilinear(const T* c, T* f, int nyf, int nxf, hipStream_t st);
+template <typename T>
+hipError_t launch_prolong_hfacet_bilinear(const T* c, T* f, int nyf, int nxf, hipStream_t st);
+template <typename T>
+hipError_t launch_prolong_cell(const T* c, T* f, int nyf, int nxf, int bilinear, hipStream_t st);
+
+// out = a*x + b*y (+ c*z when z != nullptr), element-wise over n values.
+template <typename T>
+hipError_t launch_axpby(T* out, T a, const T* x, T b, const T* y, T c, const T* z, long n, hipStream_t st);
+template <typename T>
+hipError_t launch_fill(T* out, T val, long n, hipStream_t st);
+// partial sums of squares of up to three arrays into `partials` (nblk per
+// array), then a single-block final pass into out3 (device).
+template <typename T>
+hipError_t launch_sumsq3(const T* a, long na, const T* b, long nb, const T* c, long nc, double* partials,
+                         double* out3, hipStream_t st);
+constexpr int kSumsqBlocks = 1024;
+
+}  // namespace glide

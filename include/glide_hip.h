@@ -1,0 +1,173 @@
+/*
+ * glide_hip.h — C ABI of the MI355X (gfx950) implementation of GLIDE's coupled
+ * SSA momentum + mass step (FAS constrained-descent multigrid with additive 5x5
+ * Vanka smoothing), its adjoint, and the parameter gradients.
+ *
+ * Every buffer is caller-owned device memory (row-major, contiguous, FP64):
+ *   u-facets (ny, nx+1), v-facets (ny+1, nx), cells (ny, nx).
+ * Pointers are plain device addresses; `stream` is a hipStream_t passed as
+ * void* (NULL = the null stream).  Every entry point returns 0 on success and
+ * a negative glide_status on failure; glide_last_error() gives the text.
+ * No entry point keeps a pointer past its return.  Calls are stream-ordered
+ * and do not synchronise the host unless stated.
+ *
+ * Reference interfaces replaced (JacobDowns/glide):
+ *   glide_compute_grounded     <- ForwardOperators.compute_phi / kernel compute_grounded
+ *                                 (glide/operators.py:173-192, glide/cuda/viscosity.cu:16-36)
+ *   glide_ssa_residual         <- ForwardOperators.compute_residual / kernel compute_residual
+ *                                 (glide/operators.py:71-127, glide/cuda/residuals.cu:5-333)
+ *   glide_vanka_sweeps         <- ForwardOperators.vanka_sweep / kernel vanka_smooth
+ *                                 (glide/operators.py:194-251, glide/cuda/vanka.cu:477-660)
+ *   glide_vanka_dump           <- ForwardOperators.vanka_dump (glide/operators.py:253-273,
+ *                                 glide/cuda/vanka.cu:800-908)
+ *   glide_restrict_* / glide_prolong_*
+ *                              <- Multigrid.restrict_* / prolongate_* (glide/multigrid.py:118-259,
+ *                                 glide/cuda/transfer.cu)
+ *   glide_fascd_solve          <- FASCDSolver.solve / vcycle (glide/multigrid.py:502-645)
+ *   glide_ssa_vjp              <- AdjointOperators.compute_residual / compute_vjp / kernel compute_vjp
+ *                                 (glide/operators.py:379-472, glide/cuda/residuals.cu:666-1163)
+ *   glide_vanka_adjoint_sweeps <- AdjointOperators.vanka_sweep / kernel vanka_smooth_adjoint
+ *                                 (glide/operators.py:475-524, glide/cuda/vanka.cu:662-797)
+ *   glide_fas_adjoint_solve    <- FASAdjointSolver.solve / vcycle (glide/multigrid.py:873-973)
+ *   glide_gradients            <- AdjointOperators.compute_gradient_{beta,bed,H_prev,smb}
+ *                                 (glide/operators.py:526-593, glide/cuda/grad.cu)
+ */
+#ifndef GLIDE_HIP_H
+#define GLIDE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum glide_status {
+    GLIDE_OK = 0,
+    GLIDE_ERR_INVALID_ARG = -1,
+    GLIDE_ERR_SHAPE = -2,
+    GLIDE_ERR_HIP = -3,
+    GLIDE_ERR_NONFINITE = -4
+} glide_status;
+
+/* Flags for the residual / VJP entry points. */
+#define GLIDE_USE_FORCING 1   /* subtract the level's forcing f (residual) / adjoint forcing (VJP) */
+#define GLIDE_USE_MASK 2      /* active-set rows H - gamma where mask is set (residual) */
+#define GLIDE_FREEZE_CALVING 4 /* calving rate treated as zero */
+#define GLIDE_FREEZE_PHI 8     /* do not refresh the grounded fraction before smoothing */
+
+/* Scalar parameters of one level (glide/grid.py:29-145). */
+typedef struct glide_physics {
+    double n, eps_reg, flotation_reg_driving;
+    double m, u_reg, water_drag, flotation_reg_sliding;
+    double calving_rate, flotation_reg_calving;
+    double thklim;
+} glide_physics;
+
+/* One multigrid level: shape, spacing, and the device buffers the solver reads
+ * and writes (glide/grid.py:147-372, glide/operators.py:31-56,
+ * glide/multigrid.py:647-664,975-985).  Unused pointers may be NULL for entry
+ * points that do not touch them. */
+typedef struct glide_level {
+    int32_t ny, nx;
+    double dx;
+    glide_physics phys;
+    /* state */
+    void *u, *v, *H, *H_prev, *phi, *mask;
+    /* parameters and forcing */
+    void *bed, *B, *beta, *smb;
+    /* forward operator buffers */
+    void *f_u, *f_v, *f_H, *r_u, *r_v, *r_H, *F_u, *F_v, *F_H, *gamma;
+    /* FAS-CD scratch */
+    void *w_u, *w_v, *w_H, *y_u, *y_v, *y_H, *z_u, *z_v, *z_H, *chi, *cphi;
+    /* adjoint state and buffers */
+    void *lambda_u, *lambda_v, *lambda_H;
+    void *af_u, *af_v, *af_H, *ar_u, *ar_v, *ar_H, *vjp_u, *vjp_v, *vjp_H;
+    void *wl_u, *wl_v, *wl_H, *zl_u, *zl_v, *zl_H;
+    /* gradients */
+    void *g_beta, *g_bed, *g_H_prev, *g_smb;
+    /* smoother workspace: 5 * ny * nx values */
+    void *work;
+} glide_level;
+
+/* VankaConfig + NewtonConfig (glide/operators.py:314-327).  The forward
+ * smoother uses a local Newton relaxation of 0.5, as the reference kernel does
+ * (glide/cuda/vanka.cu:615); `relaxation` is accepted for API parity. */
+typedef struct glide_vanka_config {
+    double omega, relax_phi;
+    int32_t newton_steps;
+    double relaxation, ssa_damping, mc_damping;
+} glide_vanka_config;
+
+/* FASCDConfig / FASAdjointConfig (glide/multigrid.py:671-682, 992-1001).
+ * compat_vprolong = 1 reproduces the reference's v-correction prolongation
+ * through the u-facet routine (glide/multigrid.py:212-235; square grids only);
+ * 0 uses the v-facet bilinear operator. */
+typedef struct glide_fas_config {
+    int32_t freeze_coarse_calving, freeze_coarse_phi;
+    int32_t coarsest_steps, pre_steps, post_steps, finest_steps, maximum_vcycles;
+    double relative_tolerance, absolute_tolerance;
+    int32_t compat_vprolong;
+} glide_fas_config;
+
+typedef struct glide_solve_report {
+    int32_t vcycles;
+    double initial_norm, final_norm;
+    double norms[64];     /* |r| after each V-cycle */
+    double norms3[3];     /* |r_u|, |r_v|, |r_H| at exit */
+} glide_solve_report;
+
+const char* glide_version(void);
+/* Copies the calling thread's last error message into buf (NUL-terminated);
+ * returns its full length. */
+int glide_last_error(char* buf, size_t len);
+
+int glide_compute_grounded(const glide_level* L, double relaxation, void* stream);
+int glide_ssa_residual(const glide_level* L, double dt, int flags, void* out_u, void* out_v, void* out_H,
+                       void* stream);
+/* |a|_2, |b|_2, |c|_2 into host memory out3 (synchronises the stream). */
+int glide_norms3(const void* a, size_t na, const void* b, size_t nb, const void* c, size_t nc, double* out3,
+                 void* stream);
+int glide_vanka_sweeps(const glide_level* L, double dt, int n_iter, const glide_vanka_config* vc, int flags,
+                       void* stream);
+/* Local 5x5 systems at the current state: J (ny*nx, 25), r (ny*nx, 5). */
+int glide_vanka_dump(const glide_level* L, double dt, void* J, void* r, void* stream);
+
+int glide_restrict_vfacet(const void* fine, void* coarse, int ny_coarse, int nx_coarse, void* stream);
+int glide_restrict_hfacet(const void* fine, void* coarse, int ny_coarse, int nx_coarse, void* stream);
+int glide_restrict_cell(const void* fine, void* coarse, int ny_coarse, int nx_coarse, int use_max, void* stream);
+int glide_prolong_vfacet_bilinear(const void* coarse, void* fine, int ny_fine, int nx_fine, void* stream);
+int glide_prolong_hfacet_bilinear(const void* coarse, void* fine, int ny_fine, int nx_fine, void* stream);
+int glide_prolong_cell(const void* coarse, void* fine, int ny_fine, int nx_fine, int bilinear, void* stream);
+
+/* Full FAS-CD solve on levels[start_level..n_levels-1]; vc holds one config
+ * per level.  Synchronises the stream once per V-cycle for the stopping test
+ * unless both tolerances are <= 0 (then it runs exactly maximum_vcycles
+ * V-cycles and synchronises once at the end). */
+int glide_fascd_solve(const glide_level* levels, int n_levels, int start_level, double dt,
+                      const glide_fas_config* cfg, const glide_vanka_config* vc, glide_solve_report* report,
+                      void* stream);
+
+/* One V-cycle from `level` down (FASCDSolver.vcycle, glide/multigrid.py:536-645);
+ * finest != 0 stores w and chi first and adds the finest-level sweeps. */
+int glide_fascd_vcycle(const glide_level* levels, int n_levels, int level, int finest, double dt,
+                       const glide_fas_config* cfg, const glide_vanka_config* vc, void* stream);
+
+int glide_ssa_vjp(const glide_level* L, double dt, int flags, const void* lambda_u, const void* lambda_v,
+                  const void* lambda_H, void* out_u, void* out_v, void* out_H, void* stream);
+int glide_vanka_adjoint_sweeps(const glide_level* L, double dt, int n_iter, const glide_vanka_config* vc, int flags,
+                               void* stream);
+int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_level, double dt,
+                            const glide_fas_config* cfg, const glide_vanka_config* vc, glide_solve_report* report,
+                            void* stream);
+/* One adjoint V-cycle (FASAdjointSolver.vcycle, glide/multigrid.py:905-973). */
+int glide_fas_adjoint_vcycle(const glide_level* levels, int n_levels, int level, int finest, double dt,
+                             const glide_fas_config* cfg, const glide_vanka_config* vc, void* stream);
+/* which: bit 0 beta, bit 1 bed, bit 2 H_prev, bit 3 smb. */
+int glide_gradients(const glide_level* L, double dt, int which, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLIDE_HIP_H */

@@ -1,0 +1,280 @@
+"""Pins of the FP64 CPU oracle of the SSA path (CPU only).
+
+The reference ships no golden vectors for this path (its tests need CuPy on an
+NVIDIA GPU), so the oracle is pinned by closed forms, invariants, independent
+numpy/scipy re-computations, finite differences and a dot-product test —
+each chosen so that a dropped term, a wrong sign or a transposed index fails.
+"""
+import numpy as np
+import pytest
+
+from tests.util import dome_fields, rel_err
+
+DX, DT = 1500.0, 10.0
+
+
+def test_lu5_matches_numpy(orc):
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        A = rng.standard_normal((5, 5)) + 8.0 * np.eye(5)
+        b = rng.standard_normal(5)
+        assert np.allclose(orc.lu5(A, b), np.linalg.solve(A, b), rtol=1e-13, atol=1e-14)
+
+
+def test_restrictions_match_numpy(orc):
+    rng = np.random.default_rng(1)
+    nyc, nxc = 6, 9
+    fu = rng.random((2 * nyc, 2 * nxc + 1))
+    fv = rng.random((2 * nyc + 1, 2 * nxc))
+    fc = rng.random((2 * nyc, 2 * nxc))
+    # u-facets: mean of the two fine faces stacked in y on every other column
+    ref_u = 0.5 * (fu[0::2, 0::2] + fu[1::2, 0::2])
+    ref_v = 0.5 * (fv[0::2, 0::2] + fv[0::2, 1::2])
+    blocks = fc.reshape(nyc, 2, nxc, 2)
+    assert np.allclose(orc.restrict_vfacet(fu), ref_u, rtol=0, atol=1e-15)
+    assert np.allclose(orc.restrict_hfacet(fv), ref_v, rtol=0, atol=1e-15)
+    assert np.allclose(orc.restrict_cell(fc), blocks.mean(axis=(1, 3)), rtol=0, atol=1e-15)
+    assert np.array_equal(orc.restrict_cell(fc, use_max=True), blocks.max(axis=(1, 3)))
+
+
+def _clamped_bilinear(c, ys, xs):
+    """Independent bilinear interpolation of c at fractional coordinates
+    (ys, xs) clamped to the array, via scipy."""
+    from scipy.interpolate import RegularGridInterpolator
+    ny, nx = c.shape
+    f = RegularGridInterpolator((np.arange(ny), np.arange(nx)), c, method="linear")
+    Y, X = np.meshgrid(np.clip(ys, 0, ny - 1), np.clip(xs, 0, nx - 1), indexing="ij")
+    return f(np.stack([Y.ravel(), X.ravel()], axis=-1)).reshape(Y.shape)
+
+
+def test_prolongations_match_scipy(orc):
+    rng = np.random.default_rng(2)
+    nyc, nxc = 5, 7
+    nyf, nxf = 2 * nyc, 2 * nxc
+    cu = rng.random((nyc, nxc + 1))
+    ref = _clamped_bilinear(cu, (np.arange(nyf) - 0.5) / 2, np.arange(nxf + 1) / 2)
+    assert np.allclose(orc.prolong_vfacet_bilinear(cu, nyf, nxf), ref, atol=1e-14)
+    cv = rng.random((nyc + 1, nxc))
+    ref = _clamped_bilinear(cv, np.arange(nyf + 1) / 2, (np.arange(nxf) - 0.5) / 2)
+    assert np.allclose(orc.prolong_hfacet_bilinear(cv, nyf, nxf), ref, atol=1e-14)
+    cc = rng.random((nyc, nxc))
+    ref = _clamped_bilinear(cc, (np.arange(nyf) - 0.5) / 2, (np.arange(nxf) - 0.5) / 2)
+    assert np.allclose(orc.prolong_cell(cc, nyf, nxf, bilinear=True), ref, atol=1e-14)
+    assert np.array_equal(orc.prolong_cell(cc, nyf, nxf), np.kron(cc, np.ones((2, 2))))
+
+
+def _uniform_strain(ny=12, nx=14, a=2e-3, b=-1.5e-3, H=800.0):
+    """u = a x, v = b * (height above the bottom face): exx = a, eyy = b, exy = 0."""
+    F = dome_fields(ny=ny, nx=nx, seed=0, with_velocity=False)
+    F["u"] = np.tile(a * DX * np.arange(nx + 1), (ny, 1))
+    F["v"] = np.tile((b * DX * (ny - np.arange(ny + 1)))[:, None], (1, nx))
+    F["H"] = np.full((ny, nx), H)
+    F["bed"] = np.zeros((ny, nx))
+    F["beta"] = np.zeros((ny, nx))
+    F["phi"] = np.ones((ny, nx))
+    F["mask"] = np.zeros((ny, nx))
+    F["f_H"] = F["H"] / DT
+    return F
+
+
+def test_viscosity_closed_form(orc):
+    F = _uniform_strain()
+    a, b = 2e-3, -1.5e-3
+    eta = orc.viscosity(F["u"], F["v"], F["B"], 3.0, 1e-6, DX)
+    ref = 0.5 * F["B"] * (a * a + b * b + a * b + 1e-6) ** (-1.0 / 3.0)
+    assert rel_err(eta, ref) < 1e-13
+
+
+def test_residual_closed_form_uniform_strain(orc):
+    F = _uniform_strain()
+    a, b, H = 2e-3, -1.5e-3, 800.0
+    P = orc.phys()
+    ru, rv, rH = orc.residual(F, P, DX, DT, use_forcing=False, use_mask=False)
+    ny, nx = F["H"].shape
+    # interior momentum rows: stresses are uniform, only the water drag remains
+    assert rel_err(ru[1:-1, 1:-1], -P.water_drag * F["u"][1:-1, 1:-1]) < 1e-9
+    assert rel_err(rv[1:-1, 1:-1], -P.water_drag * F["v"][1:-1, 1:-1]) < 1e-9
+    # Dirichlet rows return the facet velocity
+    assert np.array_equal(ru[:, 0], F["u"][:, 0]) and np.array_equal(ru[:, -1], F["u"][:, -1])
+    assert np.array_equal(rv[0], F["v"][0]) and np.array_equal(rv[-1], F["v"][-1])
+    # thickness rows away from the edge: H/dt + H (a + b)
+    assert rel_err(rH[1:-1, 1:-1], H / DT + H * (a + b) * np.ones((ny - 2, nx - 2))) < 1e-12
+
+
+def test_mass_residual_telescopes(orc):
+    F = dome_fields(n=20, seed=3)
+    P = orc.phys()
+    _, _, rH = orc.residual(F, P, DX, DT, use_forcing=False, use_mask=False)
+    div = rH - F["H"] / DT
+    assert abs(div.sum()) < 1e-10 * np.abs(div).sum()
+
+
+def test_local_system_matches_global_residual(orc):
+    """Rows of every interior cell's local 5x5 system at the current state equal
+    the rows of the global operator (two independent code paths)."""
+    F = dome_fields(ny=10, nx=12, seed=4)
+    P = orc.phys(m=1.0 / 3.0, calving_rate=0.3)
+    Fu, Fv, FH = orc.residual(F, P, DX, DT, use_forcing=False, use_mask=False)
+    ny, nx = F["H"].shape
+    for i in range(1, ny - 1):
+        for j in range(1, nx - 1):
+            x = [F["u"][i, j], F["u"][i, j + 1], F["v"][i, j], F["v"][i + 1, j], F["H"][i, j]]
+            _, r = orc.local_system(x, F, P, DX, DT, i, j)
+            ref = [Fu[i, j], Fu[i, j + 1], Fv[i, j], Fv[i + 1, j], FH[i, j]]
+            assert np.allclose(r, ref, rtol=1e-12, atol=1e-10)
+
+
+# entries of the reference's local Jacobian that are complete derivatives of
+# the local residual (the others drop cross-couplings by design)
+COMPLETE = [(0, 0), (0, 1), (0, 4), (1, 0), (1, 1), (1, 4), (2, 2), (2, 4), (3, 3), (3, 4),
+            (4, 0), (4, 1), (4, 2), (4, 3), (4, 4)]
+
+
+@pytest.mark.parametrize("cell", [(1, 1), (4, 5), (0, 3), (7, 10)])
+@pytest.mark.parametrize("m", [1.0, 1.0 / 3.0])
+def test_local_jacobian_vs_central_fd(orc, cell, m):
+    F = dome_fields(ny=8, nx=11, seed=5)
+    P = orc.phys(m=m, calving_rate=0.2)
+    i, j = cell
+    x0 = np.array([F["u"][i, j], F["u"][i, j + 1], F["v"][i, j], F["v"][i + 1, j], F["H"][i, j]])
+    J, _ = orc.local_system(x0, F, P, DX, DT, i, j)
+    for row, col in COMPLETE:
+        h = 1e-4 * max(1.0, abs(x0[col]))
+        xp, xm = x0.copy(), x0.copy()
+        xp[col] += h
+        xm[col] -= h
+        fd = (orc.local_system(xp, F, P, DX, DT, i, j)[1][row] - orc.local_system(xm, F, P, DX, DT, i, j)[1][row]) / (2 * h)
+        assert abs(fd - J[row, col]) <= 1e-6 * max(1.0, abs(J[row, col])), (row, col, fd, J[row, col])
+
+
+def _fd_jvp(orc, F, P, d, eps):
+    Fp, Fm = dict(F), dict(F)
+    for k, dk in zip(("u", "v", "H"), d):
+        Fp[k] = F[k] + eps * dk
+        Fm[k] = F[k] - eps * dk
+    rp = orc.residual(Fp, P, DX, DT, use_forcing=False, use_mask=False)
+    rm = orc.residual(Fm, P, DX, DT, use_forcing=False, use_mask=False)
+    return [(a - b) / (2 * eps) for a, b in zip(rp, rm)]
+
+
+@pytest.mark.parametrize("m", [1.0, 1.0 / 3.0])
+def test_vjp_dot_product(orc, m):
+    """<J^T lam, d> == <lam, J d> with J d by central differences of the
+    residual; multipliers and directions vanish on Dirichlet facets and masked
+    cells (adjoint_test.py restated with an assertion)."""
+    F = dome_fields(ny=14, nx=18, seed=6, mask_frac=0.2)
+    P = orc.phys(m=m)
+    rng = np.random.default_rng(7)
+    ny, nx = F["H"].shape
+
+    def pinned(shape_u, shape_v, shape_h):
+        a, b, c = rng.standard_normal(shape_u), rng.standard_normal(shape_v), rng.standard_normal(shape_h)
+        a[:, 0] = a[:, -1] = 0.0
+        b[0] = b[-1] = 0.0
+        c[F["mask"] > 0.5] = 0.0
+        return a, b, c
+
+    lam = pinned((ny, nx + 1), (ny + 1, nx), (ny, nx))
+    d = pinned((ny, nx + 1), (ny + 1, nx), (ny, nx))
+    vu, vv, vH = orc.vjp(F, *lam, P, DX, DT)
+    Jd = _fd_jvp(orc, F, P, d, 1e-4)
+    lhs = (vu * d[0]).sum() + (vv * d[1]).sum() + (vH * d[2]).sum()
+    rhs = sum((l * j).sum() for l, j in zip(lam, Jd))
+    assert abs(lhs - rhs) <= 1e-7 * abs(rhs)
+
+
+def test_vjp_forcing_is_subtracted(orc):
+    F = dome_fields(n=10, seed=8)
+    P = orc.phys()
+    rng = np.random.default_rng(9)
+    lam = [rng.standard_normal(F[k].shape) for k in ("u", "v", "H")]
+    af = [rng.standard_normal(F[k].shape) for k in ("u", "v", "H")]
+    a = orc.vjp(F, *lam, P, DX, DT, use_forcing=False, af=af)
+    b = orc.vjp(F, *lam, P, DX, DT, use_forcing=True, af=af)
+    assert np.allclose(b[0][:, 1:-1], a[0][:, 1:-1] - af[0][:, 1:-1], atol=1e-12)
+    assert np.allclose(b[1][1:-1], a[1][1:-1] - af[1][1:-1], atol=1e-12)
+    assert np.allclose(b[2], a[2] - af[2], atol=1e-12)
+
+
+@pytest.mark.parametrize("m", [1.0, 1.0 / 3.0])
+def test_gradients_vs_fd(orc, m):
+    """<g_beta, db> == <lam, dR/dbeta db> (and bed), with lam zero on the facets
+    whose right/bottom-cell contribution the reference's guards skip."""
+    F = dome_fields(ny=12, nx=15, seed=10)
+    P = orc.phys(m=m)
+    rng = np.random.default_rng(11)
+    ny, nx = F["H"].shape
+    lu, lv = rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx))
+    lu[:, 0] = lu[:, nx - 1] = lu[:, nx] = 0.0
+    lv[0] = lv[ny - 1] = lv[ny] = 0.0
+    for name, g in (("beta", orc.grad_beta(F, lu, lv, P)), ("bed", orc.grad_bed(F, lu, lv, DX))):
+        d = rng.standard_normal((ny, nx))
+        eps = 1e-3 * np.abs(F[name]).max()
+        Fp, Fm = dict(F), dict(F)
+        Fp[name] = F[name] + eps * d
+        Fm[name] = F[name] - eps * d
+        rp = orc.residual(Fp, P, DX, DT, use_forcing=False, use_mask=False)
+        rm = orc.residual(Fm, P, DX, DT, use_forcing=False, use_mask=False)
+        rhs = ((rp[0] - rm[0]) * lu).sum() / (2 * eps) + ((rp[1] - rm[1]) * lv).sum() / (2 * eps)
+        lhs = (g * d).sum()
+        assert abs(lhs - rhs) <= 1e-7 * max(abs(rhs), 1e-30), name
+
+
+def test_gradient_guard_quirk(orc):
+    """The reference drops the right-cell term of facet nx-1 (grad.cu:62)."""
+    F = dome_fields(ny=6, nx=8, seed=12)
+    P = orc.phys()
+    lu = np.zeros((6, 9))
+    lv = np.zeros((7, 8))
+    lu[2, 7] = 1.0  # facet nx-1: only its left cell (2, 6) receives a term
+    g = orc.grad_beta(F, lu, lv, P)
+    assert g[2, 7] == 0.0 and g[2, 6] != 0.0
+    lu[2, 7], lu[2, 6] = 0.0, 1.0  # an ordinary facet feeds both neighbours
+    g = orc.grad_beta(F, lu, lv, P)
+    assert g[2, 5] != 0.0 and g[2, 6] != 0.0
+
+
+def _dome_mg(orc, n, L, seed=0):
+    from glide_amd.synthetic import dome_problem
+    p = dome_problem(n, DX, seed=seed)
+    om = orc.OracleMG(n, n, DX, L)
+    for k in ("bed", "B", "beta", "smb"):
+        om.set(k, getattr(p, k))
+    om.set("H", p.H)
+    om.set("H_prev", p.H)
+    return p, om
+
+
+def test_fascd_contracts(orc):
+    _, om = _dome_mg(orc, 32, 3)
+    rep = om.fascd_solve(DT, orc.OracleMG.fas_config(vcycles=4, rtol=0.0, atol=0.0))
+    norms = [rep.initial_norm] + [rep.norms[k] for k in range(rep.vcycles)]
+    assert rep.vcycles == 4
+    for a, b in zip(norms[:-1], norms[1:]):
+        assert b < 0.5 * a
+    assert norms[3] < 1e-4 * norms[0]
+    assert om.get("H").min() >= 0.1 - 1e-12  # thickness constraint
+
+
+def test_vanka_fixed_point_at_solution(orc):
+    """At a converged state an additive pass is (nearly) the identity."""
+    _, om = _dome_mg(orc, 16, 2, seed=1)
+    om.fascd_solve(DT, orc.OracleMG.fas_config(vcycles=12, rtol=0.0, atol=0.0))
+    F = {k: om.get(k) for k in ("u", "v", "H", "phi", "mask", "bed", "B", "beta")}
+    F["f_u"], F["f_v"] = np.zeros_like(F["u"]), np.zeros_like(F["v"])
+    F["f_H"] = om.get("H_prev") / DT + om.get("smb")
+    F["gamma"] = np.full_like(F["H"], 0.1)
+    u, v, H, mask, _ = orc.vanka_sweep(F, orc.phys(), DX, DT, newton_steps=60)
+    assert rel_err(u, F["u"]) < 1e-5 and rel_err(H, F["H"]) < 1e-6
+
+
+def test_adjoint_solve_contracts(orc):
+    _, om = _dome_mg(orc, 32, 3, seed=2)
+    cfg = orc.OracleMG.fas_config(vcycles=3, rtol=0.0, atol=0.0)
+    om.fascd_solve(DT, cfg)
+    rng = np.random.default_rng(3)
+    om.put("af_u", rng.standard_normal((32, 33)))
+    om.put("af_v", rng.standard_normal((33, 32)))
+    om.put("af_H", np.zeros((32, 32)))
+    rep = om.fas_adjoint_solve(DT, cfg)
+    assert rep.norms[rep.vcycles - 1] < 1e-2 * rep.initial_norm

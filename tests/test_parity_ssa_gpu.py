@@ -1,0 +1,353 @@
+"""Parity of the gfx950 SSA kernels and drivers with the FP64 CPU oracle.
+
+Every test runs the HIP path through the C ABI (via the glide_amd API) and the
+oracle on the same seeded inputs.  Bar: FP64 results agree to 1e-12 relative
+for single kernels and 1e-10 relative for whole solves (u, v, H, lambda,
+gradients); active-set masks agree exactly.
+"""
+import numpy as np
+import pytest
+import torch
+
+from tests.util import dome_fields, rel_err
+
+pytestmark = pytest.mark.gpu
+
+KTOL = 1e-12   # one kernel
+STOL = 1e-10   # whole FAS solves
+
+
+def T(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=torch.float64)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def grid_from(F, dx=1500.0):
+    from glide_amd.grid import Grid
+    ny, nx = F["H"].shape
+    g = Grid(ny, nx, dx, device="cuda")
+    for k in ("u", "v", "H", "H_prev", "phi", "mask"):
+        getattr(g.state, k).set(F[k])
+    g.geometry.bed.set(F["bed"])
+    g.rheology.B.set(F["B"])
+    g.sliding.beta.set(F["beta"])
+    g.forcing.smb.set(F["smb"])
+    fo = g.forward_operators
+    fo.f_u.copy_(T(F["f_u"]))
+    fo.f_v.copy_(T(F["f_v"]))
+    fo.f_H.copy_(T(F["f_H"]))
+    fo.gamma.copy_(T(F["gamma"]))
+    return g
+
+
+SHAPES = [(32, 32), (24, 40), (66, 130)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("use_forcing,use_mask", [(True, True), (True, False), (False, False)])
+def test_residual(orc, shape, use_forcing, use_mask):
+    ny, nx = shape
+    print(f"residual shape={shape} forcing={use_forcing} mask={use_mask}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=1, mask_frac=0.2)
+    P = orc.phys()
+    g = grid_from(F)
+    fo = g.forward_operators
+    fo.compute_residual(10.0, use_mask=use_mask, operator_only=not use_forcing, freeze_phi=True)
+    torch.cuda.synchronize()
+    ru, rv, rH = orc.residual(F, P, 1500.0, 10.0, use_forcing=use_forcing, use_mask=use_mask)
+    out = (fo.r_u, fo.r_v, fo.r_H) if use_forcing else (fo.F_u, fo.F_v, fo.F_H)
+    for a, b in zip(out, (ru, rv, rH)):
+        assert rel_err(N(a), b) < KTOL
+
+
+def test_residual_nonlinear_drag(orc):
+    F = dome_fields(n=40, seed=2, mask_frac=0.1)
+    P = orc.phys(m=1.0 / 3.0, u_reg=0.5, calving_rate=0.2)
+    g = grid_from(F)
+    g.sliding.m.set(1.0 / 3.0)
+    g.sliding.u_reg.set(0.5)
+    g.calving.calving_rate.set(0.2)
+    fo = g.forward_operators
+    fo.compute_residual(5.0, freeze_phi=True)
+    ru, rv, rH = orc.residual(F, P, 1500.0, 5.0)
+    for a, b in zip((fo.r_u, fo.r_v, fo.r_H), (ru, rv, rH)):
+        assert rel_err(N(a), b) < KTOL
+
+
+def test_grounded(orc):
+    F = dome_fields(n=48, seed=3)
+    g = grid_from(F)
+    g.forward_operators.compute_phi()
+    ref = orc.grounded(F["H"], F["bed"], 0.1)
+    assert rel_err(N(g.state.phi.data), ref) < KTOL
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_vanka_dump(orc, shape):
+    ny, nx = shape
+    F = dome_fields(ny=ny, nx=nx, seed=4)
+    g = grid_from(F)
+    J, r = g.forward_operators.vanka_dump(10.0)
+    Jo, ro = orc.vanka_dump(F, orc.phys(), 1500.0, 10.0)
+    assert rel_err(N(J), Jo) < KTOL
+    assert rel_err(N(r), ro) < KTOL
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_vanka_single_pass(orc, shape):
+    ny, nx = shape
+    print(f"vanka pass shape={shape}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=5)
+    g = grid_from(F)
+    g.forward_operators.vanka_sweep(10.0, 1, freeze_phi=True)
+    u, v, H, mask, _ = orc.vanka_sweep(F, orc.phys(), 1500.0, 10.0)
+    assert rel_err(N(g.state.u.data), u) < KTOL
+    assert rel_err(N(g.state.v.data), v) < KTOL
+    assert rel_err(N(g.state.H.data), H) < KTOL
+    assert np.array_equal(N(g.state.mask.data), mask)
+
+
+def test_vanka_passes_with_phi_refresh(orc):
+    F = dome_fields(n=32, seed=6)
+    g = grid_from(F)
+    g.forward_operators.vanka_sweep(10.0, 6)
+    P = orc.phys()
+    for _ in range(6):
+        F["phi"] = orc.grounded(F["H"], F["bed"], 0.1)
+        F["u"], F["v"], F["H"], F["mask"], _ = orc.vanka_sweep(F, P, 1500.0, 10.0)
+    assert rel_err(N(g.state.u.data), F["u"]) < 1e-11
+    assert rel_err(N(g.state.v.data), F["v"]) < 1e-11
+    assert rel_err(N(g.state.H.data), F["H"]) < 1e-11
+    assert rel_err(N(g.state.phi.data), F["phi"]) < 1e-11
+
+
+def test_transfers(orc):
+    from glide_amd.multigrid import Multigrid
+    rng = np.random.default_rng(7)
+    mg = Multigrid(2, ny=16, nx=16, dx=100.0, device="cuda")
+    for nyc, nxc in ((8, 8), (6, 10)):
+        nyf, nxf = 2 * nyc, 2 * nxc
+        fu, fv, fc = rng.random((nyf, nxf + 1)), rng.random((nyf + 1, nxf)), rng.random((nyf, nxf))
+        assert rel_err(N(mg.restrict_vfacet(T(fu))), orc.restrict_vfacet(fu)) < KTOL
+        assert rel_err(N(mg.restrict_hfacet(T(fv))), orc.restrict_hfacet(fv)) < KTOL
+        assert rel_err(N(mg.restrict_cell(T(fc))), orc.restrict_cell(fc)) < KTOL
+        assert rel_err(N(mg.restrict_cell(T(fc), method="max")), orc.restrict_cell(fc, True)) < KTOL
+        cu, cv, cc = rng.random((nyc, nxc + 1)), rng.random((nyc + 1, nxc)), rng.random((nyc, nxc))
+        assert rel_err(N(mg.prolongate_vfacet(T(cu))), orc.prolong_vfacet_bilinear(cu, nyf, nxf)) < KTOL
+        assert rel_err(N(mg.prolongate_hfacet(T(cv), compat=False)), orc.prolong_hfacet_bilinear(cv, nyf, nxf)) < KTOL
+        assert rel_err(N(mg.prolongate_cell(T(cc))), orc.prolong_cell(cc, nyf, nxf)) < KTOL
+        assert rel_err(N(mg.prolongate_cell(T(cc), method="bilinear")), orc.prolong_cell(cc, nyf, nxf, True)) < KTOL
+    # the reference's v path through the u routine (square grids)
+    cv = rng.random((9, 8))
+    got = N(mg.prolongate_hfacet(T(cv), compat=True))
+    ref = orc.prolong_vfacet_bilinear(np.ascontiguousarray(cv.reshape(8, 9)), 16, 16).reshape(17, 16)
+    assert rel_err(got, ref) < KTOL
+
+
+def _lam(F, seed):
+    rng = np.random.default_rng(seed)
+    ny, nx = F["H"].shape
+    return (rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx)), rng.standard_normal((ny, nx)))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("m", [1.0, 1.0 / 3.0])
+@pytest.mark.parametrize("use_forcing", [False, True])
+def test_vjp(orc, shape, m, use_forcing):
+    ny, nx = shape
+    print(f"vjp shape={shape} m={m} forcing={use_forcing}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=8, mask_frac=0.15)
+    P = orc.phys(m=m)
+    lu, lv, lH = _lam(F, 9)
+    g = grid_from(F)
+    g.sliding.m.set(m)
+    a = g.adjoint
+    a.lambda_u.set(lu)
+    a.lambda_v.set(lv)
+    a.lambda_H.set(lH)
+    ao = g.adjoint_operators
+    rng = np.random.default_rng(10)
+    af = (rng.standard_normal(lu.shape), rng.standard_normal(lv.shape), rng.standard_normal(lH.shape))
+    ao.f_u.copy_(T(af[0]))
+    ao.f_v.copy_(T(af[1]))
+    ao.f_H.copy_(T(af[2]))
+    if use_forcing:
+        ao.compute_residual(10.0)
+        out = (ao.r_u, ao.r_v, ao.r_H)
+    else:
+        ao.compute_vjp(10.0)
+        out = (ao.vjp_u, ao.vjp_v, ao.vjp_H)
+    ref = orc.vjp(F, lu, lv, lH, P, 1500.0, 10.0, use_forcing=use_forcing, af=af)
+    for x, y in zip(out, ref):
+        assert rel_err(N(x), y) < KTOL
+
+
+def test_vanka_adjoint_pass(orc):
+    F = dome_fields(n=40, seed=11, mask_frac=0.1)
+    P = orc.phys()
+    lam = _lam(F, 12)
+    g = grid_from(F)
+    a = g.adjoint
+    a.lambda_u.set(lam[0])
+    a.lambda_v.set(lam[1])
+    a.lambda_H.set(lam[2])
+    ao = g.adjoint_operators
+    rng = np.random.default_rng(13)
+    af = (rng.standard_normal(lam[0].shape), rng.standard_normal(lam[1].shape), rng.standard_normal(lam[2].shape))
+    ao.f_u.copy_(T(af[0]))
+    ao.f_v.copy_(T(af[1]))
+    ao.f_H.copy_(T(af[2]))
+    ao.vanka_sweep(10.0, 1)
+    r = orc.vjp(F, *lam, P, 1500.0, 10.0, use_forcing=True, af=af)
+    ref = orc.vanka_adjoint_sweep(F, lam, r, P, 1500.0, 10.0)
+    for x, y in zip((a.lambda_u.data, a.lambda_v.data, a.lambda_H.data), ref):
+        assert rel_err(N(x), y) < KTOL
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_gradients(orc, shape):
+    ny, nx = shape
+    F = dome_fields(ny=ny, nx=nx, seed=14)
+    P = orc.phys(m=1.0 / 3.0)
+    lu, lv, lH = _lam(F, 15)
+    g = grid_from(F)
+    g.sliding.m.set(1.0 / 3.0)
+    a = g.adjoint
+    a.lambda_u.set(lu)
+    a.lambda_v.set(lv)
+    a.lambda_H.set(lH)
+    ao = g.adjoint_operators
+    ao.compute_gradient_beta()
+    ao.compute_gradient_bed()
+    ao.compute_gradient_H_prev(10.0)
+    ao.compute_gradient_smb()
+    assert rel_err(N(g.sliding.beta.grad), orc.grad_beta(F, lu, lv, P)) < KTOL
+    assert rel_err(N(g.geometry.bed.grad), orc.grad_bed(F, lu, lv, 1500.0)) < KTOL
+    assert rel_err(N(g.state.H_prev.grad), -lH / 10.0) < KTOL
+    assert rel_err(N(g.forcing.smb.grad), -lH) < KTOL
+
+
+def _dome_models(orc, n, L, seed=0, dx=1500.0):
+    from glide_amd import IceDynamics
+    from glide_amd.synthetic import dome_problem, install
+    p = dome_problem(n, dx, seed=seed)
+    model = IceDynamics(ny=n, nx=n, dx=dx, n_levels=L, device="cuda")
+    install(model.mg, p)
+    om = orc.OracleMG(n, n, dx, L)
+    for k in ("bed", "B", "beta", "smb"):
+        om.set(k, getattr(p, k))
+    om.set("H", p.H)
+    om.set("H_prev", p.H)
+    return p, model, om
+
+
+@pytest.mark.parametrize("n,L", [(64, 4), (32, 3)])
+def test_fascd_solve(orc, n, L):
+    print(f"fascd solve n={n} L={L}", flush=True)
+    p, model, om = _dome_models(orc, n, L)
+    fs = model.forward_solver
+    fs.fas_options.set(maximum_vcycles=3, relative_tolerance=0.0, absolute_tolerance=0.0, report_norms=False)
+    rep = fs.solve(10.0)
+    torch.cuda.synchronize()
+    orep = om.fascd_solve(10.0, orc.OracleMG.fas_config(vcycles=3, rtol=0.0, atol=0.0))
+    assert rep.vcycles == orep.vcycles == 3
+    g = model.mg[0]
+    for k in ("u", "v", "H"):
+        assert rel_err(N(getattr(g.state, k).data), om.get(k)) < STOL, k
+    assert np.array_equal(N(g.state.mask.data), om.get("mask"))
+    for k in range(3):
+        assert abs(rep.norms[k] - orep.norms[k]) <= 1e-9 * orep.norms[0]
+    assert rep.norms[2] < 1e-2 * rep.initial_norm
+
+
+def test_fascd_solve_tolerance_stop(orc):
+    p, model, om = _dome_models(orc, 32, 3, seed=3)
+    fs = model.forward_solver
+    fs.fas_options.set(maximum_vcycles=10, relative_tolerance=1e-3, absolute_tolerance=5.0, report_norms=False)
+    rep = fs.solve(10.0)
+    orep = om.fascd_solve(10.0, orc.OracleMG.fas_config(vcycles=10, rtol=1e-3, atol=5.0))
+    assert rep.vcycles == orep.vcycles
+    assert rel_err(N(model.mg[0].state.u.data), om.get("u")) < STOL
+
+
+@pytest.mark.parametrize("n,L", [(64, 4)])
+def test_forward_adjoint_gradients(orc, n, L):
+    print(f"forward+adjoint n={n} L={L}", flush=True)
+    p, model, om = _dome_models(orc, n, L, seed=5)
+    model.forward_solver.fas_options.set(maximum_vcycles=3, relative_tolerance=0.0, absolute_tolerance=0.0,
+                                         report_norms=False)
+    model.adjoint_solver.fas_options.set(maximum_vcycles=3, relative_tolerance=0.0, absolute_tolerance=0.0,
+                                         report_norms=False)
+    model.forward(0.0, 10.0, update_geometry=False)
+    cfg = orc.OracleMG.fas_config(vcycles=3, rtol=0.0, atol=0.0)
+    om.fascd_solve(10.0, cfg)
+    g = model.mg[0]
+    u, v = N(g.state.u.data), N(g.state.v.data)
+    rng = np.random.default_rng(17)
+    u_obs = om.get("u") + 5.0 * rng.standard_normal(u.shape)
+    v_obs = om.get("v") + 5.0 * rng.standard_normal(v.shape)
+    model.backward(0.0, 10.0, dJdu=T(u - u_obs), dJdv=T(v - v_obs), dJdH=None)
+    torch.cuda.synchronize()
+    om.put("af_u", -(om.get("u") - u_obs))
+    om.put("af_v", -(om.get("v") - v_obs))
+    om.put("af_H", np.zeros((n, n)))
+    om.fas_adjoint_solve(10.0, cfg)
+    om.gradients(10.0)
+    a = g.adjoint
+    for name, t in (("lambda_u", a.lambda_u.data), ("lambda_v", a.lambda_v.data), ("lambda_H", a.lambda_H.data)):
+        assert rel_err(N(t), om.get(name)) < STOL, name
+    assert rel_err(N(g.sliding.beta.grad), om.get("g_beta")) < STOL
+    assert rel_err(N(g.geometry.bed.grad), om.get("g_bed")) < STOL
+    assert rel_err(N(g.state.H_prev.grad), om.get("g_H_prev")) < STOL
+    assert rel_err(N(g.forcing.smb.grad), om.get("g_smb")) < STOL
+
+
+def test_glidestep_autograd(orc):
+    from glide_amd.torch import GlideStep
+    p, model, om = _dome_models(orc, 32, 3, seed=9)
+    for s in (model.forward_solver, model.adjoint_solver):
+        s.fas_options.set(maximum_vcycles=2, relative_tolerance=0.0, absolute_tolerance=0.0, report_norms=False)
+    H_prev = T(p.H).requires_grad_(True)
+    bed = T(p.bed).requires_grad_(True)
+    beta = T(p.beta).requires_grad_(True)
+    smb = T(p.smb).requires_grad_(True)
+    u, v, H = GlideStep.apply(0.0, 10.0, model, 0, H_prev, bed, beta, smb)
+    loss = 0.5 * (u ** 2).sum() + 0.5 * (v ** 2).sum()
+    loss.backward()
+    cfg = orc.OracleMG.fas_config(vcycles=2, rtol=0.0, atol=0.0)
+    om.fascd_solve(10.0, cfg)
+    assert rel_err(N(u), om.get("u")) < STOL
+    om.put("af_u", -om.get("u"))
+    om.put("af_v", -om.get("v"))
+    om.put("af_H", np.zeros((32, 32)))
+    om.fas_adjoint_solve(10.0, cfg)
+    om.gradients(10.0)
+    assert rel_err(N(beta.grad), om.get("g_beta")) < STOL
+    assert rel_err(N(bed.grad), om.get("g_bed")) < STOL
+    assert rel_err(N(H_prev.grad), om.get("g_H_prev")) < STOL
+    assert rel_err(N(smb.grad), om.get("g_smb")) < STOL
+
+
+def test_full_size_kernels_512(orc):
+    """At the benchmark size: one residual, one smoothing pass and one VJP
+    against the oracle, plus run-to-run bitwise determinism of the pass."""
+    print("512^2 kernels", flush=True)
+    F = dome_fields(n=512, seed=21, mask_frac=0.05)
+    P = orc.phys()
+    g = grid_from(F)
+    fo = g.forward_operators
+    fo.compute_residual(10.0, freeze_phi=True)
+    ru, rv, rH = orc.residual(F, P, 1500.0, 10.0)
+    for a, b in zip((fo.r_u, fo.r_v, fo.r_H), (ru, rv, rH)):
+        assert rel_err(N(a), b) < KTOL
+    fo.vanka_sweep(10.0, 1, freeze_phi=True)
+    u1 = N(g.state.u.data).copy()
+    ref = orc.vanka_sweep(F, P, 1500.0, 10.0)
+    assert rel_err(u1, ref[0]) < KTOL
+    assert rel_err(N(g.state.H.data), ref[2]) < KTOL
+    g2 = grid_from(F)
+    g2.forward_operators.vanka_sweep(10.0, 1, freeze_phi=True)
+    assert np.array_equal(N(g2.state.u.data), u1)
