@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark of the coupled SSA step on one MI355X.
+
+Workload: the dome on a Gaussian-random-field bed at 512 x 512 cells,
+dx = 1500 m, 5 multigrid levels, dt = 10 a (glide_amd.synthetic).  One step is
+a forward FAS-CD solve of exactly 3 V-cycles (advancing the geometry), the
+adjoint FAS solve of exactly 3 V-cycles for the loss 0.5 * |u|^2 + 0.5 * |v|^2,
+and the four parameter gradients.  Inputs live in HBM before timing starts.
+
+Prints one JSON line.  Besides the contract fields it carries
+  roofline     - the dominant kernel's algorithmic bytes / measured duration
+                 (HIP events around each launch in the timed region) vs HBM peak
+  cpu_baseline - the FP64 oracle on the host cores for a bounded sample of the
+                 same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "forward+adjoint steps/sec at 512² MAC grid, mixed vs FP64 at equal 1e-8 error"
+HBM_PEAK_GBS = 8000.0
+WORD = 8  # bytes per FP64 value
+
+# Algorithmic bytes per cell of one launch (each field counted once at 8 B):
+#   forward smoothing pass (patch solve + increment apply): 12 inputs + 4 outputs
+#   transpose operator: 15 inputs + 3 outputs
+#   adjoint smoothing pass: 12 inputs + 3 outputs
+#   residual: 12 inputs + 3 outputs
+ALG_FIELDS = {"vanka_sweep": 16, "vjp": 18, "adjoint_sweep": 15, "residual": 15}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--vcycles", type=int, default=3)
+    ap.add_argument("--dt", type=float, default=10.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-n", type=int, default=128,
+                    help="grid side of the oracle sample (same level count and cycle counts)")
+    return ap.parse_args()
+
+
+def build_model(args, torch):
+    from glide_amd import IceDynamics
+    from glide_amd.synthetic import dome_problem, install
+    p = dome_problem(args.n, 1500.0, seed=args.seed)
+    model = IceDynamics(ny=args.n, nx=args.n, dx=1500.0, n_levels=args.levels, device="cuda")
+    install(model.mg, p)
+    for s in (model.forward_solver, model.adjoint_solver):
+        s.fas_options.set(maximum_vcycles=args.vcycles, relative_tolerance=0.0, absolute_tolerance=0.0,
+                          report_norms=False)
+    return model
+
+
+def one_step(model, t, dt):
+    model.forward(t, dt, update_geometry=True)
+    g = model.mg[0]
+    model.backward(t, dt, dJdu=g.state.u.data, dJdv=g.state.v.data, dJdH=None)
+
+
+def cpu_baseline(args):
+    """FP64 oracle on the host: one forward and one adjoint V-cycle of the same
+    dome workload at cpu_sample_n^2 with the same level count, scaled to a full
+    step (x vcycles) and to 512^2 by the cell ratio."""
+    import numpy as np
+    from glide_amd.synthetic import dome_problem
+    from oracle import oracle as O
+    n = args.cpu_sample_n
+    p = dome_problem(n, 1500.0, seed=args.seed)
+    om = O.OracleMG(n, n, 1500.0, args.levels)
+    for k in ("bed", "B", "beta", "smb"):
+        om.set(k, getattr(p, k))
+    om.set("H", p.H)
+    om.set("H_prev", p.H)
+    cfg = O.OracleMG.fas_config(vcycles=1, rtol=0.0, atol=0.0)
+    t0 = time.perf_counter()
+    om.fascd_solve(args.dt, cfg)
+    t_fwd = time.perf_counter() - t0
+    om.put("af_u", -om.get("u"))
+    om.put("af_v", -om.get("v"))
+    om.put("af_H", np.zeros((n, n)))
+    t0 = time.perf_counter()
+    om.fas_adjoint_solve(args.dt, cfg)
+    om.gradients(args.dt)
+    t_adj = time.perf_counter() - t0
+    scale = (args.n / n) ** 2
+    step_s = args.vcycles * (t_fwd + t_adj) * scale
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": (f"FP64 oracle, 1 forward + 1 adjoint V-cycle of the dome workload at {n}^2 "
+                       f"({args.levels} levels): {t_fwd:.2f} s + {t_adj:.2f} s; x{args.vcycles} V-cycles per step, "
+                       f"x{scale:g} cell ratio to {args.n}^2"),
+            "sample_seconds": round(t_fwd + t_adj, 3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from glide_amd import _lib
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm device", file=sys.stderr)
+        sys.exit(1)
+    _lib.load()
+    model = build_model(args, torch)
+    t = 0.0
+    for _ in range(args.warmup):
+        one_step(model, t, args.dt)
+        t += args.dt
+    torch.cuda.synchronize()
+    _lib.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_step(model, t, args.dt)
+        t += args.dt
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern = {}
+    for name in ("vanka_cell", "vanka_apply", "adjoint_cell", "adjoint_apply", "vjp", "residual"):
+        kern[name] = _lib.timing_read(name)
+    _lib.timing_enable(False)
+
+    ms_step = 1000.0 * elapsed / args.steps
+    value = args.steps / elapsed
+
+    def bw(fields, ms, cells):
+        return fields * WORD * cells / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+
+    # a smoothing pass is the patch-solve launch plus the increment-apply launch
+    passes = {
+        "vanka_sweep": (kern["vanka_cell"][0], kern["vanka_cell"][1] + kern["vanka_apply"][1], kern["vanka_cell"][2]),
+        "adjoint_sweep": (kern["adjoint_cell"][0], kern["adjoint_cell"][1] + kern["adjoint_apply"][1],
+                          kern["adjoint_cell"][2]),
+        "vjp": kern["vjp"],
+    }
+    dominant = max(passes, key=lambda k: passes[k][1])
+    launches, dur_ms, cells = passes[dominant]
+    achieved = bw(ALG_FIELDS[dominant], dur_ms, cells)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": dominant, "launches": launches, "avg_launch_us": round(1000.0 * dur_ms / max(launches, 1), 3),
+                "alg_bytes_per_cell": ALG_FIELDS[dominant] * WORD}
+    kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "cells": v[2]} for k, v in kern.items()}
+    for k, (c, ms, cells) in passes.items():
+        kernels[k] = {"launches": c, "total_ms": round(ms, 3), "alg_GBps": round(bw(ALG_FIELDS[k], ms, cells), 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(args)
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"error": repr(e)}
+
+    line = {"metric": METRIC, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": (f"dome on GRF bed, {args.n}x{args.n} MAC grid, dx=1500 m, {args.levels} levels, "
+                                    f"{args.vcycles}+{args.vcycles} V-cycles forward+adjoint per step, dt={args.dt} a"),
+                       "precision": "plain FP64 form", "grid": f"{args.n}x{args.n}", "levels": args.levels,
+                       "vcycles": args.vcycles, "dt_years": args.dt, "parallelism": "single GPU"},
+            "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels,
+            "omp_threads": os.environ.get("OMP_NUM_THREADS")}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

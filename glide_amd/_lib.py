@@ -97,7 +97,22 @@ SIGNATURES = {
     "glide_fas_adjoint_vcycle": (_I, [_LP, _I, _I, _I, _D, ctypes.POINTER(FasConfigC),
                                       ctypes.POINTER(VankaConfigC), _P]),
     "glide_gradients": (_I, [_LP, _D, _I, _P]),
+    "glide_timing_enable": (_I, [_I]),
+    "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
 }
+
+TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3, "residual": 4, "adjoint_apply": 5}
+
+
+def timing_enable(on: bool) -> None:
+    call("glide_timing_enable", int(bool(on)))
+
+
+def timing_read(kernel: str):
+    """(launch count, summed ms, summed cells) of a timed kernel since enable."""
+    c, ms, cells = ctypes.c_long(0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+    call("glide_timing_read", TIMED_KERNELS[kernel], ctypes.byref(c), ctypes.byref(ms), ctypes.byref(cells))
+    return c.value, ms.value, cells.value
 
 _lock = threading.Lock()
 _lib = None

@@ -610,6 +610,17 @@ int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_l
     return solve_loop(X, start_level, report, resid, cycle);
 }
 
+int glide_timing_enable(int on) {
+    timing_enable(on != 0);
+    return GLIDE_OK;
+}
+
+int glide_timing_read(int kernel, long* count, double* total_ms, double* total_cells) {
+    if (!count || !total_ms || !total_cells) return fail(GLIDE_ERR_INVALID_ARG, "NULL output");
+    HIP_TRY(timing_read(kernel, count, total_ms, total_cells), "timing_read");
+    return GLIDE_OK;
+}
+
 int glide_gradients(const glide_level* L, double dt, int which, void* stream) {
     GLIDE_TRY(check_level(L));
     hipStream_t st = (hipStream_t)stream;

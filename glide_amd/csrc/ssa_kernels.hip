@@ -15,6 +15,8 @@
 // identical.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "ssa_device.hpp"
 #include "ssa_kernels.hpp"
 
@@ -666,6 +668,80 @@ __global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __
     if (gbed) gbed[c] = gz;
 }
 
+// ------------------------------------------------------------------ launch timing
+// Optional HIP-event pairs around the hot launches (enabled by the host for
+// measurement); each record keeps the level's cell count so that per-launch
+// algorithmic bytes can be computed.
+struct TimingRec {
+    hipEvent_t start, stop;
+    long cells;
+};
+static bool g_timing_on = false;
+static std::vector<TimingRec> g_timing[kTimedKernels];
+static std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t pooled_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct TimedLaunch {
+    int id;
+    long cells;
+    hipStream_t st;
+    hipEvent_t start = nullptr, stop = nullptr;
+    TimedLaunch(int id_, long cells_, hipStream_t st_) : id(id_), cells(cells_), st(st_) {
+        if (!g_timing_on) return;
+        start = pooled_event();
+        stop = pooled_event();
+        if (start) hipEventRecord(start, st);
+    }
+    ~TimedLaunch() {
+        if (!start || !stop) return;
+        hipEventRecord(stop, st);
+        g_timing[id].push_back({start, stop, cells});
+    }
+};
+
+void timing_enable(bool on) {
+    timing_clear();
+    g_timing_on = on;
+}
+
+void timing_clear() {
+    for (auto& v : g_timing) {
+        for (auto& r : v) {
+            g_event_pool.push_back(r.start);
+            g_event_pool.push_back(r.stop);
+        }
+        v.clear();
+    }
+}
+
+hipError_t timing_read(int id, long* count, double* total_ms, double* total_cells) {
+    *count = 0;
+    *total_ms = 0.0;
+    *total_cells = 0.0;
+    if (id < 0 || id >= kTimedKernels) return hipErrorInvalidValue;
+    for (auto& r : g_timing[id]) {
+        hipError_t e = hipEventSynchronize(r.stop);
+        if (e != hipSuccess) return e;
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, r.start, r.stop);
+        if (e != hipSuccess) return e;
+        *total_ms += ms;
+        *total_cells += (double)r.cells;
+        ++*count;
+    }
+    return hipSuccess;
+}
+
 // ------------------------------------------------------------------ launchers
 static inline dim3 tile_grid(int ny, int nx, bool with_facets) {
     int gx = with_facets ? (nx + 1 + TX - 1) / TX : (nx + TX - 1) / TX;
@@ -682,6 +758,7 @@ hipError_t launch_grounded(T* phi, const T* H, const T* bed, T c, T relax, long 
 template <typename T>
 hipError_t launch_residual(const SsaArgs<T>& a, T* ou, T* ov, T* oH, int use_forcing, int use_mask, T calving_rate,
                            hipStream_t st) {
+    TimedLaunch tl(kTimeResidual, (long)a.ny * a.nx, st);
     k_residual<T><<<tile_grid(a.ny, a.nx, true), dim3(TX, TY), 0, st>>>(a, ou, ov, oH, use_forcing, use_mask,
                                                                          calving_rate);
     return hipGetLastError();
@@ -690,8 +767,12 @@ hipError_t launch_residual(const SsaArgs<T>& a, T* ou, T* ov, T* oH, int use_for
 template <typename T>
 hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* mask, const VankaArgs<T>& vk, T omega,
                              hipStream_t st) {
-    k_vanka_cell<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, work, mask, vk);
+    {
+        TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx, st);
+        k_vanka_cell<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, work, mask, vk);
+    }
     dim3 g((a.nx + 1 + 255) / 256, a.ny + 1);
+    TimedLaunch tl(kTimeVankaApply, (long)a.ny * a.nx, st);
     k_vanka_apply<T><<<g, 256, 0, st>>>(u, v, H, work, a.ny, a.nx, omega);
     return hipGetLastError();
 }
@@ -705,8 +786,12 @@ hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hi
 template <typename T>
 hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* rv, const T* rH, T* lu, T* lv, T* lH,
                                      T* work, const VankaArgs<T>& vk, T omega, hipStream_t st) {
-    k_vanka_adjoint_cell<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, ru, rv, rH, work, vk);
+    {
+        TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx, st);
+        k_vanka_adjoint_cell<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, ru, rv, rH, work, vk);
+    }
     dim3 g((a.nx + 1 + 255) / 256, a.ny + 1);
+    TimedLaunch tl(kTimeAdjointApply, (long)a.ny * a.nx, st);
     k_vanka_apply<T><<<g, 256, 0, st>>>(lu, lv, lH, work, a.ny, a.nx, -omega);
     return hipGetLastError();
 }
@@ -714,6 +799,7 @@ hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* 
 template <typename T>
 hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH, const T* fu, const T* fv,
                       const T* fH, T* ou, T* ov, T* oH, T calving_rate, hipStream_t st) {
+    TimedLaunch tl(kTimeVjp, (long)a.ny * a.nx, st);
     k_vjp<T><<<tile_grid(a.ny, a.nx, true), dim3(TX, TY), 0, st>>>(a, lu, lv, lH, fu, fv, fH, ou, ov, oH,
                                                                     calving_rate);
     return hipGetLastError();

@@ -43,6 +43,14 @@ hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH
 template <typename T>
 hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st);
 
+// Opt-in launch timing of the hot kernels (HIP events around each launch).
+enum TimedKernel {
+    kTimeVankaCell = 0, kTimeVankaApply, kTimeAdjointCell, kTimeVjp, kTimeResidual, kTimeAdjointApply, kTimedKernels
+};
+void timing_enable(bool on);
+void timing_clear();
+hipError_t timing_read(int id, long* count, double* total_ms, double* total_cells);
+
 // transfers and element-wise helpers (transfer_kernels.hip)
 template <typename T>
 hipError_t launch_restrict_vfacet(const T* f, T* c, int nyc, int nxc, hipStream_t st);

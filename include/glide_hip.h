@@ -166,6 +166,15 @@ int glide_fas_adjoint_vcycle(const glide_level* levels, int n_levels, int level,
 /* which: bit 0 beta, bit 1 bed, bit 2 H_prev, bit 3 smb. */
 int glide_gradients(const glide_level* L, double dt, int which, void* stream);
 
+/* Measurement support (no counterpart in the reference): when enabled, HIP
+ * event pairs are recorded around every launch of the hot kernels
+ * (0 forward patch solve, 1 patch increment apply, 2 adjoint patch solve,
+ * 3 transpose operator, 4 residual, 5 adjoint increment apply).  glide_timing_read synchronises those
+ * events and returns the launch count, summed duration and summed cell count
+ * of kernel `kernel` since the last enable. */
+int glide_timing_enable(int on);
+int glide_timing_read(int kernel, long* count, double* total_ms, double* total_cells);
+
 #ifdef __cplusplus
 }
 #endif

@@ -612,7 +612,12 @@ void vjp(const Fields& F, const double* lu, const double* lv, const double* lH, 
     auto ETA = [&](int a, int b) { return eta[(size_t)clampi(a, 0, ny - 1) * nx + clampi(b, 0, nx - 1)]; };
     auto ETAD = [&](int a, int b) { return etad[(size_t)clampi(a, 0, ny - 1) * nx + clampi(b, 0, nx - 1)]; };
     auto Hc_ = [&](int a, int b) { return A.C(F.H, a, b); };
-    for (int i = 0; i <= ny; ++i) {
+    // Rows scatter into rows i-1..i+1 only: rows of one colour (i mod 3) never
+    // touch the same targets, so each colour runs in parallel and the
+    // summation order is fixed.
+    for (int colour = 0; colour < 3; ++colour)
+#pragma omp parallel for schedule(static)
+    for (int i = colour; i <= ny; i += 3) {
         for (int j = 0; j <= nx; ++j) {
             if (i < ny && j < nx) {
                 double Hc = Hc_(i, j), pc = A.C(F.phi, i, j), dH;
