@@ -300,3 +300,171 @@ class OracleMG:
 
     def gradients(self, dt, level=0):
         lib().orc_gradients(self.h, level, _D(dt))
+
+
+# ---------------------------------------------------------------- enthalpy
+class EnthParams(ctypes.Structure):
+    _fields_ = [("dx", _D), ("dt", _D), ("drain_rate", _D), ("h_thin", _D), ("lf_c", _D), ("term_flags", _I)]
+
+
+RHO_I_E, RHO_W_E, C_I, K_I, L_HEAT = 910.0, 1000.0, 2009.0, 2.1, 3.34e5
+T_REF, T_MELT, BETA_CC, GRAVITY = 223.15, 273.15, 7.9e-8, 9.81
+K_COLD = K_I / C_I
+K_TEMP = K_COLD * 0.1
+E_SCALE = C_I * (T_MELT - T_REF)
+SEC_PER_YR = 365.25 * 86400.0
+
+
+def _enth_lib():
+    L = lib()
+    if not getattr(L, "_enth_typed", False):
+        L.orc_enth_residual.restype = _D
+        L.orc_enth_column_sweep.restype = _I
+        for f in ("orc_E_pmp", "orc_K", "orc_water"):
+            getattr(L, f).restype = _D
+            getattr(L, f).argtypes = [_D, _D]
+        L._enth_typed = True
+    return L
+
+
+class OracleThermal:
+    """FP64 mirror of the reference's EnthalpyOperators for one grid
+    (glide/enthalpy.py:205-643); arrays are float64 numpy."""
+
+    def __init__(self, ny, nx, nz, dx, H):
+        self.ny, self.nx, self.nz, self.dx = ny, nx, nz, float(dx)
+        self.sigma = np.linspace(0.0, 1.0, nz)
+        self.H = np.ascontiguousarray(H, dtype=np.float64).copy()
+        self.H_prev = self.H.copy()
+        z3 = lambda: np.zeros((ny, nx, nz))
+        self.E, self.E_prev, self.f_E, self.omega = z3(), z3(), z3(), z3()
+        self.u3d = np.zeros((nz, ny, nx + 1))
+        self.v3d = np.zeros((nz, ny + 1, nx))
+        self.E_surface = np.zeros((ny, nx))
+        self.Q_geo = np.zeros((ny, nx))
+        self.Q_fh = np.zeros((ny, nx))
+        self.phi_strain = z3()
+        self.drain_rate = 0.01 / SEC_PER_YR
+        self.h_thin = 25.0
+        self.lf_c = 1e-12
+        self.flags = 7
+        self.omega_relax, self.n_newton, self.relaxation = 1.0, 3, 1.0
+        self.rtol, self.atol = 1e-5, 1e-2
+
+    def params(self, dt):
+        return EnthParams(self.dx, float(dt), self.drain_rate, self.h_thin, self.lf_c, int(self.flags))
+
+    def T_pmp(self):
+        depth = (1.0 - self.sigma)[None, None, :] * self.H[:, :, None]
+        return T_MELT - BETA_CC * RHO_I_E * GRAVITY * depth
+
+    def initialize_from_temperature(self, T):
+        T = np.broadcast_to(np.asarray(T, dtype=np.float64) if np.ndim(T) != 2 else np.asarray(T)[:, :, None],
+                            (self.ny, self.nx, self.nz))
+        T = np.minimum(T, self.T_pmp())
+        self.E = np.ascontiguousarray(C_I * (T - T_REF) / E_SCALE)
+        self.E_prev = self.E.copy()
+        self.H_prev = self.H.copy()
+
+    def set_surface_enthalpy_from_temperature(self, Ts):
+        Ts = np.minimum(np.broadcast_to(np.asarray(Ts, dtype=np.float64), (self.ny, self.nx)), T_MELT)
+        self.E_surface = np.ascontiguousarray(C_I * (Ts - T_REF) / E_SCALE)
+
+    def set_rhs(self, snapshot=True):
+        if snapshot:
+            self.E_prev = self.E.copy()
+            self.H_prev = self.H.copy()
+        dsig_half = 0.5 / (self.nz - 1)
+        f = self.H[:, :, None] * self.phi_strain / E_SCALE
+        f[:, :, 0] += (self.Q_geo + self.Q_fh) / (dsig_half * E_SCALE)
+        f[:, :, self.nz - 1] = 0.0
+        self.f_E = np.ascontiguousarray(f)
+
+    def _args(self):
+        return (_p(self.E_prev), _p(self.f_E), _p(self.u3d), _p(self.v3d), _p(self.omega), _p(self.H),
+                _p(self.H_prev), _p(self.E_surface))
+
+    def compute_residual(self, dt, use_forcing=True):
+        r = np.zeros_like(self.E)
+        P = self.params(dt)
+        mx = _enth_lib().orc_enth_residual(_p(r), _p(self.E), *self._args(), ctypes.byref(P), int(use_forcing),
+                                           self.ny, self.nx, self.nz)
+        self.r_E = r
+        return mx
+
+    def column_smooth(self, dt):
+        d = np.zeros_like(self.E)
+        P = self.params(dt)
+        _enth_lib().orc_enth_column_smooth(_p(d), _p(self.E), *self._args(), ctypes.byref(P), self.n_newton,
+                                           _D(self.relaxation), self.ny, self.nx, self.nz)
+        self.delta_E = d
+        return d
+
+    def layer_smooth(self, dt):
+        d = np.zeros_like(self.E)
+        P = self.params(dt)
+        _enth_lib().orc_enth_layer_smooth(_p(d), _p(self.E), *self._args(), ctypes.byref(P), self.ny, self.nx,
+                                          self.nz)
+        self.delta_E = d
+        return d
+
+    def column_sweep(self, dt, n_iter, alternating=True):
+        P = self.params(dt)
+        r0, r = _D(0.0), _D(0.0)
+        it = _enth_lib().orc_enth_column_sweep(_p(self.E), *self._args(), ctypes.byref(P), int(n_iter),
+                                               int(alternating), _D(self.omega_relax), self.n_newton,
+                                               _D(self.relaxation), _D(self.rtol), _D(self.atol), ctypes.byref(r0),
+                                               ctypes.byref(r), self.ny, self.nx, self.nz)
+        return it, r0.value, r.value
+
+    def compute_omega(self, dh_dt, bmb=None):
+        bmb = np.zeros((self.ny, self.nx)) if bmb is None else np.ascontiguousarray(bmb, dtype=np.float64)
+        dh = np.ascontiguousarray(np.broadcast_to(np.asarray(dh_dt, dtype=np.float64), (self.ny, self.nx)))
+        om = np.zeros_like(self.E)
+        _enth_lib().orc_enth_omega(_p(om), _p(self.u3d), _p(self.v3d), _p(self.H), _p(dh), _p(bmb), _D(self.dx),
+                                   self.ny, self.nx, self.nz)
+        self.omega = om
+        return om
+
+    def get_temperature(self):
+        return np.minimum(self.E * E_SCALE / C_I + T_REF, self.T_pmp())
+
+    def get_water_content(self):
+        E_pmp = C_I * (self.T_pmp() - T_REF)
+        return np.maximum(self.E * E_SCALE - E_pmp, 0.0) / L_HEAT
+
+    def get_arrhenius_factor(self, n_glen=3.0):
+        B = np.zeros((self.ny, self.nx))
+        _enth_lib().orc_enth_arrhenius(_p(B), _p(self.E), _p(self.H), _D(n_glen), self.ny, self.nx, self.nz)
+        return B
+
+
+def thermal_step(th: OracleThermal, u, v, H_new, beta, dt, n_smooth=10, m=1.0, u_reg=1.0, update_rheology=True,
+                 frictional_heating=True, rho_i=917.0, n_glen=3.0, strain_heating=None):
+    """ThermalModel.step (glide/model.py:167-254) after pre_momentum (E_prev,
+    H_prev snapshot) and the momentum solve that produced u, v (m/a) and H_new.
+    Returns B_glide when update_rheology else None."""
+    th.H = np.ascontiguousarray(H_new, dtype=np.float64).copy()
+    for k in range(th.nz):
+        th.u3d[k] = u
+        th.v3d[k] = v
+    dt_yr = dt / SEC_PER_YR
+    dh_dt_yr = (th.H - th.H_prev) / dt_yr
+    th.compute_omega(dh_dt_yr)
+    th.omega /= SEC_PER_YR
+    th.u3d /= SEC_PER_YR
+    th.v3d /= SEC_PER_YR
+    if frictional_heating:
+        ub, vb = th.u3d[0], th.v3d[0]
+        uc = 0.5 * (ub[:, 1:] + ub[:, :-1])
+        vc = 0.5 * (vb[1:, :] + vb[:-1, :])
+        speed = np.sqrt(uc ** 2 + vc ** 2 + u_reg ** 2)
+        th.Q_fh = np.ascontiguousarray(beta * speed ** (m + 1.0))
+    if strain_heating is not None:
+        th.phi_strain = np.ascontiguousarray(strain_heating)
+    th.set_rhs(snapshot=False)
+    th.column_sweep(dt, n_smooth)
+    if update_rheology:
+        scale = rho_i * GRAVITY * SEC_PER_YR ** (1.0 / n_glen)
+        return th.get_arrhenius_factor(n_glen) / scale
+    return None
