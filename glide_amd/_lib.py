@@ -1,4 +1,5 @@
-"""ctypes binding of ``libglide_hip.so`` (declared in ``include/glide_hip.h``).
+"""ctypes binding of ``libglide_hip.so`` (declared in ``include/glide_hip.h`` and
+``include/glide_enthalpy.h``).
 
 The shared library is loaded on first use.  There is no fallback: when the
 library is missing, or a call is made with tensors that are not on a ROCm
@@ -66,11 +67,42 @@ class SolveReport(ctypes.Structure):
                 ("norms", ctypes.c_double * 64), ("norms3", ctypes.c_double * 3)]
 
 
+ENTH_POINTERS = ("E", "E_prev", "f_E", "omega", "phi_strain", "r_E", "delta_E", "E_work", "u3d", "v3d",
+                 "H", "H_prev", "E_surface", "Q_geo", "Q_fh", "dh_dt")
+ENTH_MAX_SWEEPS = 256
+
+
+class EnthColumn(ctypes.Structure):
+    _fields_ = [("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("term_flags", ctypes.c_int32), ("dx", ctypes.c_double), ("drain_rate", ctypes.c_double),
+                ("h_thin", ctypes.c_double), ("lf_c", ctypes.c_double)] + [(n, ctypes.c_void_p)
+                                                                           for n in ENTH_POINTERS]
+
+
+class EnthSweepConfigC(ctypes.Structure):
+    _fields_ = [("omega", ctypes.c_double), ("relaxation", ctypes.c_double),
+                ("relative_tolerance", ctypes.c_double), ("absolute_tolerance", ctypes.c_double),
+                ("n_newton", ctypes.c_int32), ("alternating", ctypes.c_int32)]
+
+
+class EnthReport(ctypes.Structure):
+    _fields_ = [("sweeps", ctypes.c_int32), ("pad", ctypes.c_int32), ("r0", ctypes.c_double),
+                ("r", ctypes.c_double), ("norms", ctypes.c_double * ENTH_MAX_SWEEPS)]
+
+
+class ThermalConfigC(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("m", ctypes.c_double), ("u_reg", ctypes.c_double),
+                ("rho_i", ctypes.c_double), ("n_glen", ctypes.c_double), ("n_smooth", ctypes.c_int32),
+                ("frictional_heating", ctypes.c_int32), ("update_rheology", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
 _S = ctypes.c_size_t
 _LP = ctypes.POINTER(Level)
+_CP = ctypes.POINTER(EnthColumn)
 
 # name -> (restype, argtypes); every symbol the header declares.
 SIGNATURES = {
@@ -99,6 +131,23 @@ SIGNATURES = {
     "glide_gradients": (_I, [_LP, _D, _I, _P]),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    # include/glide_enthalpy.h
+    "glide_enth_residual": (_I, [_CP, _D, _I, ctypes.POINTER(_D), _P]),
+    "glide_enth_column_smooth": (_I, [_CP, _D, _I, _D, _P]),
+    "glide_enth_layer_smooth": (_I, [_CP, _D, _P]),
+    "glide_enth_apply": (_I, [_CP, _D, _P]),
+    "glide_enth_column_sweep": (_I, [_CP, _D, _I, ctypes.POINTER(EnthSweepConfigC), ctypes.POINTER(EnthReport),
+                                     _P]),
+    "glide_enth_omega": (_I, [_CP, _P, _P, _D, _P]),
+    "glide_enth_broadcast_velocity": (_I, [_CP, _P, _P, _D, _P]),
+    "glide_enth_frictional_heating": (_I, [_CP, _P, _D, _D, _P]),
+    "glide_enth_set_rhs": (_I, [_CP, _P]),
+    "glide_enth_arrhenius": (_I, [_CP, _D, _D, _P, _P]),
+    "glide_enth_diagnostics": (_I, [_CP, _P, _P, _P]),
+    "glide_enth_from_temperature": (_I, [_CP, _P, _P, _P]),
+    "glide_enth_surface_from_temperature": (_I, [_CP, _P, _P, _P]),
+    "glide_thermal_step": (_I, [_CP, _P, _P, _P, _P, ctypes.POINTER(ThermalConfigC),
+                                ctypes.POINTER(EnthSweepConfigC), ctypes.POINTER(EnthReport), _P]),
 }
 
 TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3, "residual": 4, "adjoint_apply": 5}

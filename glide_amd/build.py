@@ -16,8 +16,8 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libglide_hip.so"
 ARCH = os.environ.get("GLIDE_OFFLOAD_ARCH", "gfx950")
 
-DEVICE_SOURCES = ["ssa_kernels.hip", "transfer_kernels.hip"]
-HOST_SOURCES = ["solver.cpp"]
+DEVICE_SOURCES = ["ssa_kernels.hip", "transfer_kernels.hip", "enthalpy_kernels.hip"]
+HOST_SOURCES = ["solver.cpp", "enthalpy_abi.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1,0 +1,364 @@
+// C ABI of the enthalpy column solver and the fused thermal step.
+//
+// EnthalpyOperators.column_sweep (glide/enthalpy.py:426-471) runs here as one
+// stream-ordered sequence: the initial residual, then per iteration the
+// column smoother (E -> E_work), the layer smoother (E_work -> E) and the
+// residual, each followed by a one-block reduction that records the norm and
+// raises a device flag once the stopping test passes; every later launch
+// reads the flag and returns at once.  The host never waits inside the
+// sweep.  Buffers belong to the caller; the library owns a small per-device
+// reduction scratch and status record (allocated on first use).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/glide_enthalpy.h"
+#include "enthalpy_kernels.hpp"
+
+namespace glide {
+void set_last_error(const std::string& msg);
+}
+
+using namespace glide;
+
+namespace {
+
+constexpr double kSecPerYear = 365.25 * 86400.0;
+constexpr double kGravity = 9.81;
+
+int fail(int code, const std::string& msg) {
+    set_last_error(msg);
+    return code;
+}
+
+#define HIP_TRY(expr, where)                                                                  \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) return fail(GLIDE_ERR_HIP, std::string(where) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define GLIDE_TRY(expr)                \
+    do {                               \
+        int _r = (expr);               \
+        if (_r != GLIDE_OK) return _r; \
+    } while (0)
+
+struct EnthScratch {
+    double* partials = nullptr;   // kEnthResBlocks
+    double* max_out = nullptr;    // 1
+    EnthStatus* status = nullptr;
+    EnthStatus* host = nullptr;   // pinned mirror
+};
+
+std::mutex g_mutex;
+std::vector<EnthScratch> g_scratch;
+
+int scratch(EnthScratch** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+    EnthScratch& s = g_scratch[dev];
+    if (!s.partials) {
+        HIP_TRY(hipMalloc(&s.partials, sizeof(double) * kEnthResBlocks), "hipMalloc(enthalpy partials)");
+        HIP_TRY(hipMalloc(&s.max_out, sizeof(double)), "hipMalloc(enthalpy max)");
+        HIP_TRY(hipMalloc(&s.status, sizeof(EnthStatus)), "hipMalloc(enthalpy status)");
+        HIP_TRY(hipHostMalloc(&s.host, sizeof(EnthStatus), hipHostMallocDefault), "hipHostMalloc(enthalpy status)");
+    }
+    *out = &s;
+    return GLIDE_OK;
+}
+
+typedef double R;
+inline R* P(void* p) { return static_cast<R*>(p); }
+inline const R* CP(const void* p) { return static_cast<const R*>(p); }
+inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+int require(const void* p, const char* name) {
+    if (!p) return fail(GLIDE_ERR_INVALID_ARG, std::string("required buffer is NULL: ") + name);
+    return GLIDE_OK;
+}
+#define REQ(ptr) GLIDE_TRY(require((ptr), #ptr))
+
+int check_column(const glide_enth_column* C) {
+    if (!C) return fail(GLIDE_ERR_INVALID_ARG, "column grid is NULL");
+    if (C->ny < 1 || C->nx < 1) return fail(GLIDE_ERR_SHAPE, "enthalpy grid must have ny >= 1 and nx >= 1");
+    if (C->nz < 2 || C->nz > 64) return fail(GLIDE_ERR_SHAPE, "enthalpy columns need 2 <= nz <= 64");
+    if (!(C->dx > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dx must be positive");
+    if (C->term_flags & ~7) return fail(GLIDE_ERR_INVALID_ARG, "unknown enthalpy term flag");
+    return GLIDE_OK;
+}
+
+// Operator arguments; `needs_velocity` for kernels that read u3d/v3d.
+int make_args(const glide_enth_column* C, double dt, bool needs_state, EnthArgs<R>* a) {
+    GLIDE_TRY(check_column(C));
+    REQ(C->H);
+    if (needs_state) {
+        if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+        REQ(C->E);
+        REQ(C->E_prev);
+        REQ(C->f_E);
+        REQ(C->H_prev);
+        REQ(C->E_surface);
+        if (C->term_flags & GLIDE_ENTH_HORIZONTAL) {
+            REQ(C->u3d);
+            REQ(C->v3d);
+        }
+        if (C->term_flags & GLIDE_ENTH_OMEGA) REQ(C->omega);
+    }
+    a->ny = C->ny;
+    a->nx = C->nx;
+    a->nz = C->nz;
+    a->flags = C->term_flags;
+    a->dxi = 1.0 / C->dx;
+    a->dt = dt;
+    a->drain_rate = C->drain_rate;
+    a->h_thin = C->h_thin;
+    a->lf_sq = std::sqrt(1.0 + C->lf_c);
+    a->E = CP(C->E);
+    a->E_prev = CP(C->E_prev);
+    a->f_E = CP(C->f_E);
+    a->u3d = CP(C->u3d);
+    a->v3d = CP(C->v3d);
+    a->omega = (C->term_flags & GLIDE_ENTH_OMEGA) ? CP(C->omega) : nullptr;
+    a->H = CP(C->H);
+    a->H_prev = CP(C->H_prev);
+    a->E_surface = CP(C->E_surface);
+    a->Q_geo = CP(C->Q_geo);
+    a->phi_strain = CP(C->phi_strain);
+    a->skip = nullptr;
+    return GLIDE_OK;
+}
+
+int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
+               glide_enth_report* report, hipStream_t st) {
+    if (!cfg) return fail(GLIDE_ERR_INVALID_ARG, "sweep config is NULL");
+    if (n_iter < 0 || n_iter > GLIDE_ENTH_MAX_SWEEPS)
+        return fail(GLIDE_ERR_INVALID_ARG, "n_iter must be in [0, GLIDE_ENTH_MAX_SWEEPS]");
+    if (cfg->n_newton < 0) return fail(GLIDE_ERR_INVALID_ARG, "n_newton must be non-negative");
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, dt, true, &A));
+    REQ(C->r_E);
+    if (cfg->alternating) REQ(C->E_work);
+    else REQ(C->delta_E);
+    EnthScratch* s = nullptr;
+    GLIDE_TRY(scratch(&s));
+    const double rtol = cfg->relative_tolerance, atol = cfg->absolute_tolerance;
+    HIP_TRY(launch_enth_residual<R>(A, P(C->r_E), 1, s->partials, nullptr, s->status, 0, rtol, atol, st),
+            "enthalpy residual");
+    EnthArgs<R> Ak = A;
+    Ak.skip = &s->status->done;
+    for (int it = 0; it < n_iter; ++it) {
+        if (cfg->alternating) {
+            HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, P(C->E_work), cfg->omega, cfg->n_newton,
+                                                 cfg->relaxation, st),
+                    "enthalpy column smoother");
+            EnthArgs<R> Aw = Ak;
+            Aw.E = CP(C->E_work);
+            HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
+        } else {
+            HIP_TRY(launch_enth_column_smooth<R>(Ak, P(C->delta_E), nullptr, 0.0, cfg->n_newton, cfg->relaxation,
+                                                 st),
+                    "enthalpy column smoother");
+            HIP_TRY(launch_enth_apply<R>(Ak, P(C->E), CP(C->delta_E), cfg->omega, st), "enthalpy update");
+        }
+        HIP_TRY(launch_enth_residual<R>(Ak, P(C->r_E), 1, s->partials, nullptr, s->status, 1, rtol, atol, st),
+                "enthalpy residual");
+    }
+    if (report) {
+        HIP_TRY(hipMemcpyAsync(s->host, s->status, sizeof(EnthStatus), hipMemcpyDeviceToHost, st), "copy status");
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        report->sweeps = s->host->sweeps;
+        report->pad = 0;
+        report->r0 = s->host->r0;
+        report->r = s->host->r;
+        std::memset(report->norms, 0, sizeof(report->norms));
+        for (int q = 0; q < s->host->sweeps && q < GLIDE_ENTH_MAX_SWEEPS; ++q) report->norms[q] = s->host->norms[q];
+    }
+    return GLIDE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glide_enth_residual(const glide_enth_column* C, double dt, int use_forcing, double* max_abs, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, dt, true, &A));
+    REQ(C->r_E);
+    EnthScratch* s = nullptr;
+    GLIDE_TRY(scratch(&s));
+    hipStream_t st = S(stream);
+    HIP_TRY(launch_enth_residual<R>(A, P(C->r_E), use_forcing ? 1 : 0, s->partials, max_abs ? s->max_out : nullptr,
+                                    nullptr, 0, 0.0, 0.0, st),
+            "enthalpy residual");
+    if (max_abs) {
+        HIP_TRY(hipMemcpyAsync(&s->host->r, s->max_out, sizeof(double), hipMemcpyDeviceToHost, st), "copy max");
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        *max_abs = s->host->r;
+    }
+    return GLIDE_OK;
+}
+
+int glide_enth_column_smooth(const glide_enth_column* C, double dt, int n_newton, double relaxation, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, dt, true, &A));
+    REQ(C->delta_E);
+    if (n_newton < 0) return fail(GLIDE_ERR_INVALID_ARG, "n_newton must be non-negative");
+    HIP_TRY(launch_enth_column_smooth<R>(A, P(C->delta_E), nullptr, 0.0, n_newton, relaxation, S(stream)),
+            "enthalpy column smoother");
+    return GLIDE_OK;
+}
+
+int glide_enth_layer_smooth(const glide_enth_column* C, double dt, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, dt, true, &A));
+    REQ(C->delta_E);
+    HIP_TRY(launch_enth_layer_smooth<R>(A, P(C->delta_E), nullptr, 0.0, S(stream)), "enthalpy layer smoother");
+    return GLIDE_OK;
+}
+
+int glide_enth_apply(const glide_enth_column* C, double omega, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, 1.0, false, &A));
+    REQ(C->E);
+    REQ(C->delta_E);
+    HIP_TRY(launch_enth_apply<R>(A, P(C->E), CP(C->delta_E), omega, S(stream)), "enthalpy update");
+    return GLIDE_OK;
+}
+
+int glide_enth_column_sweep(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
+                            glide_enth_report* report, void* stream) {
+    return sweep_impl(C, dt, n_iter, cfg, report, S(stream));
+}
+
+int glide_enth_omega(const glide_enth_column* C, const void* dh_dt, const void* bmb, double divide_by, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, 1.0, false, &A));
+    REQ(C->u3d);
+    REQ(C->v3d);
+    REQ(C->omega);
+    REQ(dh_dt);
+    if (divide_by == 0.0) return fail(GLIDE_ERR_INVALID_ARG, "divide_by must be non-zero");
+    HIP_TRY(launch_enth_omega<R>(A, CP(dh_dt), CP(bmb), divide_by, P(C->omega), S(stream)), "enthalpy omega");
+    return GLIDE_OK;
+}
+
+int glide_enth_broadcast_velocity(const glide_enth_column* C, const void* u2d, const void* v2d, double divide_by,
+                                  void* stream) {
+    GLIDE_TRY(check_column(C));
+    REQ(C->u3d);
+    REQ(C->v3d);
+    REQ(u2d);
+    REQ(v2d);
+    if (divide_by == 0.0) return fail(GLIDE_ERR_INVALID_ARG, "divide_by must be non-zero");
+    const long nu = (long)C->ny * (C->nx + 1), nv = (long)(C->ny + 1) * C->nx;
+    HIP_TRY(launch_enth_broadcast<R>(CP(u2d), P(C->u3d), nu, C->nz, divide_by, S(stream)), "broadcast u");
+    HIP_TRY(launch_enth_broadcast<R>(CP(v2d), P(C->v3d), nv, C->nz, divide_by, S(stream)), "broadcast v");
+    return GLIDE_OK;
+}
+
+int glide_enth_frictional_heating(const glide_enth_column* C, const void* beta, double m, double u_reg,
+                                  void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, 1.0, false, &A));
+    REQ(C->u3d);
+    REQ(C->v3d);
+    REQ(C->Q_fh);
+    REQ(beta);
+    HIP_TRY(launch_enth_rhs<R>(A, CP(beta), m, u_reg, 1, P(C->Q_fh), nullptr, S(stream)), "frictional heating");
+    return GLIDE_OK;
+}
+
+int glide_enth_set_rhs(const glide_enth_column* C, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, 1.0, false, &A));
+    REQ(C->f_E);
+    REQ(C->Q_geo);
+    REQ(C->Q_fh);
+    HIP_TRY(launch_enth_rhs<R>(A, nullptr, 0.0, 0.0, 0, P(C->Q_fh), P(C->f_E), S(stream)), "enthalpy forcing");
+    return GLIDE_OK;
+}
+
+int glide_enth_arrhenius(const glide_enth_column* C, double n_glen, double scale, void* B, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, 1.0, false, &A));
+    REQ(C->E);
+    REQ(B);
+    if (!(n_glen > 0.0) || scale == 0.0) return fail(GLIDE_ERR_INVALID_ARG, "n_glen must be positive, scale non-zero");
+    HIP_TRY(launch_enth_arrhenius<R>(A, n_glen, scale, P(B), S(stream)), "rate factor");
+    return GLIDE_OK;
+}
+
+int glide_enth_diagnostics(const glide_enth_column* C, void* temperature, void* water, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, 1.0, false, &A));
+    REQ(C->E);
+    HIP_TRY(launch_enth_diagnostics<R>(A, P(temperature), P(water), S(stream)), "enthalpy diagnostics");
+    return GLIDE_OK;
+}
+
+int glide_enth_from_temperature(const glide_enth_column* C, const void* temperature, void* E_out, void* stream) {
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, 1.0, false, &A));
+    REQ(temperature);
+    REQ(E_out);
+    HIP_TRY(launch_enth_from_temperature<R>(A, CP(temperature), P(E_out), S(stream)), "enthalpy from temperature");
+    return GLIDE_OK;
+}
+
+int glide_enth_surface_from_temperature(const glide_enth_column* C, const void* T_surface, void* E_surface_out,
+                                        void* stream) {
+    GLIDE_TRY(check_column(C));
+    REQ(T_surface);
+    REQ(E_surface_out);
+    HIP_TRY(launch_enth_surface_from_temperature<R>(CP(T_surface), P(E_surface_out), (long)C->ny * C->nx, S(stream)),
+            "surface enthalpy");
+    return GLIDE_OK;
+}
+
+int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta, void* B_out,
+                       const glide_thermal_config* tc, const glide_enth_sweep_config* sc, glide_enth_report* report,
+                       void* stream) {
+    if (!tc || !sc) return fail(GLIDE_ERR_INVALID_ARG, "thermal or sweep config is NULL");
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, tc->dt, true, &A));
+    REQ(C->u3d);
+    REQ(C->v3d);
+    REQ(C->omega);
+    REQ(C->dh_dt);
+    REQ(C->Q_geo);
+    REQ(C->Q_fh);
+    if (tc->frictional_heating) REQ(beta);
+    if (tc->update_rheology) {
+        REQ(B_out);
+        if (!(tc->n_glen > 0.0) || !(tc->rho_i > 0.0))
+            return fail(GLIDE_ERR_INVALID_ARG, "n_glen and rho_i must be positive");
+    }
+    hipStream_t st = S(stream);
+    // glide/model.py:180-197: layer velocities in m/a, omega from the realised
+    // thickness change in m/a, then both converted to m/s.
+    GLIDE_TRY(glide_enth_broadcast_velocity(C, u2d, v2d, 1.0, stream));
+    const double dt_yr = tc->dt / kSecPerYear;
+    HIP_TRY(launch_enth_tendency<R>(CP(C->H), CP(C->H_prev), dt_yr, P(C->dh_dt), (long)C->ny * C->nx, st),
+            "thickness tendency");
+    HIP_TRY(launch_enth_omega<R>(A, CP(C->dh_dt), nullptr, kSecPerYear, P(C->omega), st), "enthalpy omega");
+    GLIDE_TRY(glide_enth_broadcast_velocity(C, u2d, v2d, kSecPerYear, stream));
+    // glide/model.py:199-203, glide/enthalpy.py:473-519
+    HIP_TRY(launch_enth_rhs<R>(A, CP(beta), tc->m, tc->u_reg, tc->frictional_heating ? 1 : 0, P(C->Q_fh), P(C->f_E),
+                               st),
+            "enthalpy forcing");
+    GLIDE_TRY(sweep_impl(C, tc->dt, tc->n_smooth, sc, report, st));
+    // glide/model.py:205-213
+    if (tc->update_rheology) {
+        const double scale = tc->rho_i * kGravity * std::pow(kSecPerYear, 1.0 / tc->n_glen);
+        HIP_TRY(launch_enth_arrhenius<R>(A, tc->n_glen, scale, P(B_out), st), "rate factor");
+    }
+    return GLIDE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,667 @@
+// gfx950 kernels of the conservative sigma-coordinate enthalpy solver.
+//
+// Discretisation follows glide/cuda/enthalpy.cu (time term, vertical
+// diffusion with smoothed K(E), upwind sigma advection, Lax-Friedrichs
+// horizontal advection with the momentum solver's mass flux, drainage) and
+// glide/enthalpy.py (forcing, diagnostics, rate factor).
+//
+// Work decomposition: one wavefront per column, lane k = sigma level k
+// (nz <= 64).  In the (ny, nx, nz) layout a column is contiguous, so every
+// per-level load and store of a wavefront is one coalesced segment, and the
+// horizontal neighbours of a level are contiguous columns too.  Vertical
+// neighbours come from cross-lane shuffles.  The column's tridiagonal Newton
+// system is solved with parallel cyclic reduction across the 64 lanes (rows
+// >= nz are identity rows) instead of a serial per-thread Thomas sweep, so no
+// per-thread arrays of length nz are needed.
+#include <hip/hip_runtime.h>
+
+#include "enthalpy_kernels.hpp"
+
+namespace glide {
+
+// glide/enthalpy.py:46-65
+constexpr double kRhoI = 910.0, kRhoW = 1000.0, kCI = 2009.0, kKI = 2.1, kLHeat = 3.34e5;
+constexpr double kTRef = 223.15, kTMelt = 273.15, kBetaCC = 7.9e-8, kGrav = 9.81;
+constexpr double kKCold = kKI / kCI, kKTempFactor = 0.1;
+constexpr double kEScale = kCI * (kTMelt - kTRef);
+constexpr double kSecPerYr = 365.25 * 86400.0;
+constexpr double kMassFluxReg = 10.0 / (kSecPerYr * kSecPerYr);
+constexpr double kMassFluxRegYr = 10.0;
+constexpr double kDrainSharpness = 0.01;
+constexpr double kClausius = kBetaCC * kRhoI * kGrav;  // K per m of ice
+constexpr int kColsPerBlock = 4;                        // wavefronts (columns) per 256-thread block
+
+template <typename T>
+__device__ __forceinline__ T t_pmp(T sigma, T H) {
+    return T(kTMelt) - T(kClausius) * ((T(1) - sigma) * H);
+}
+template <typename T>
+__device__ __forceinline__ T e_pmp(T sigma, T H) {
+    return T(kCI) * (t_pmp(sigma, H) - T(kTRef)) / T(kEScale);
+}
+// glide/cuda/enthalpy.cu:48-98
+template <typename T>
+__device__ __forceinline__ T diffusivity(T E, T Ep) {
+    T z = (E - Ep) * T(kEScale) * T(0.01);
+    z = fmin(fmax(z, T(-20)), T(20));
+    T s = T(1) / (T(1) + exp(-z));
+    return T(kKCold) * (T(1) - s + T(kKTempFactor) * s);
+}
+template <typename T>
+__device__ __forceinline__ T water_content(T E, T Ep) {
+    T x = (E - Ep) * T(kEScale);
+    T ax = fmin(fmax(T(kDrainSharpness) * x, T(-20)), T(20));
+    T sp = (ax > T(10)) ? x : log(T(1) + exp(ax)) / T(kDrainSharpness);
+    return sp / T(kLHeat);
+}
+template <typename T>
+__device__ __forceinline__ T dwater(T E, T Ep) {
+    T z = fmin(fmax(T(kDrainSharpness) * (E - Ep) * T(kEScale), T(-20)), T(20));
+    return (T(1) / (T(1) + exp(-z))) / T(kLHeat);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+template <typename T>
+__device__ __forceinline__ T lane_get(T x, int src) {
+    return __shfl(x, src, 64);
+}
+
+__device__ __forceinline__ bool skipped(const int* skip) { return skip && *skip; }
+
+// Horizontal advection pieces of one level that do not depend on the
+// level's own enthalpy: face mass fluxes (zero on the domain edge) and the
+// neighbouring columns' enthalpy (glide/cuda/enthalpy.cu:671-763).
+template <typename T>
+struct Horiz {
+    T Hu_l, Hu_r, Hv_t, Hv_b;
+    T E_xm, E_xp, E_ym, E_yp;
+};
+
+template <typename T>
+__device__ __forceinline__ Horiz<T> load_horiz(const EnthArgs<T>& A, int i, int j, int k) {
+    const int ny = A.ny, nx = A.nx, nz = A.nz;
+    Horiz<T> h;
+    const T Hh = A.H[(long)i * nx + j];
+    const T* u = A.u3d + (long)k * ny * (nx + 1);
+    const T* v = A.v3d + (long)k * (ny + 1) * nx;
+    h.Hu_l = h.Hu_r = h.Hv_t = h.Hv_b = T(0);
+    if (j > 0) {
+        T ul = u[(long)i * (nx + 1) + j], Hm = A.H[(long)i * nx + j - 1];
+        h.Hu_l = T(0.5) * (Hm + Hh) * ul - T(0.5) * sqrt(ul * ul + T(kMassFluxReg)) * (Hh - Hm);
+    }
+    if (j < nx - 1) {
+        T ur = u[(long)i * (nx + 1) + j + 1], Hp = A.H[(long)i * nx + j + 1];
+        h.Hu_r = T(0.5) * (Hh + Hp) * ur - T(0.5) * sqrt(ur * ur + T(kMassFluxReg)) * (Hp - Hh);
+    }
+    if (i > 0) {
+        T vt = v[(long)i * nx + j], Hm = A.H[(long)(i - 1) * nx + j];
+        h.Hv_t = T(0.5) * (Hm + Hh) * vt - T(0.5) * sqrt(vt * vt + T(kMassFluxReg)) * (Hm - Hh);
+    }
+    if (i < ny - 1) {
+        T vb = v[(long)(i + 1) * nx + j], Hp = A.H[(long)(i + 1) * nx + j];
+        h.Hv_b = T(0.5) * (Hh + Hp) * vb - T(0.5) * sqrt(vb * vb + T(kMassFluxReg)) * (Hh - Hp);
+    }
+    auto col = [&](int ii, int jj) {
+        ii = min(max(ii, 0), ny - 1);
+        jj = min(max(jj, 0), nx - 1);
+        return A.E[((long)ii * nx + jj) * nz + k];
+    };
+    h.E_xm = col(i, j - 1);
+    h.E_xp = col(i, j + 1);
+    h.E_ym = col(i - 1, j);
+    h.E_yp = col(i + 1, j);
+    return h;
+}
+
+// Lax-Friedrichs enthalpy flux 0.5 U (El + Er) - 0.5 |U| sq (Er - El) and its
+// derivatives with respect to El and Er.
+template <typename T>
+__device__ __forceinline__ T lf_flux(T U, T El, T Er, T sq, T& dl, T& dr) {
+    T al = fabs(U) * sq;
+    dl = T(0.5) * (U + al);
+    dr = T(0.5) * (U - al);
+    return T(0.5) * U * (El + Er) - T(0.5) * al * (Er - El);
+}
+
+// rho_i div(H u E) at a level whose own enthalpy is Eh: residual and diagonal.
+template <typename T>
+__device__ __forceinline__ void horiz_adv(const Horiz<T>& h, T Eh, T sq, T dxi, T& res, T& diag) {
+    T dll, dlr, drl, drr, dtl, dtr, dbl, dbr;
+    T fl = lf_flux(h.Hu_l, h.E_xm, Eh, sq, dll, dlr);
+    T fr = lf_flux(h.Hu_r, Eh, h.E_xp, sq, drl, drr);
+    T ft = lf_flux(h.Hv_t, Eh, h.E_ym, sq, dtl, dtr);
+    T fb = lf_flux(h.Hv_b, h.E_yp, Eh, sq, dbl, dbr);
+    res = T(kRhoI) * ((fr - fl) + (ft - fb)) * dxi;
+    diag = T(kRhoI) * ((drl - dlr) + (dtl - dbr)) * dxi;
+}
+
+// One level's residual (without forcing) and tridiagonal Jacobian row
+// (glide/cuda/enthalpy.cu:783-905).  Ekm1/Ekp1 and om_m/om_p are the
+// vertical neighbours (meaningless where the stencil does not use them).
+template <typename T>
+struct Row {
+    T r, a, b, c;
+};
+
+template <typename T>
+__device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, int k, T h, T hinv, T hp, T Ek, T Ekm1, T Ekp1,
+                                            T Epk, T om_m, T om_k, T om_p, const Horiz<T>& hz) {
+    const int nz = A.nz;
+    const T dsig = T(1) / T(nz - 1);
+    const T invdt = T(1) / A.dt;
+    Row<T> w;
+    w.r = T(kRhoI) * (h * Ek - hp * Epk) * invdt;
+    w.b = T(kRhoI) * h * invdt;
+    w.a = T(0);
+    w.c = T(0);
+    if (A.flags & 1) {
+        T hr, hd;
+        horiz_adv(hz, Ek, A.lf_sq, A.dxi, hr, hd);
+        w.r += hr;
+        w.b += hd;
+    }
+    const T Ep_k = e_pmp(T(k) * dsig, h);
+    if (k == 0) {
+        if (A.flags & 2) {
+            T wh = T(0.5) * (om_k + om_p);
+            T wp = fmax(wh, T(0)), wn = fmin(wh, T(0));
+            T hdi = T(1) / (T(0.5) * dsig);
+            w.r += T(kRhoI) * ((wp * Ek + wn * Ekp1) - om_k * Ek) * hdi;
+            w.b += T(kRhoI) * (wp - om_k) * hdi;
+            w.c += T(kRhoI) * wn * hdi;
+        }
+        T Kh = diffusivity(T(0.5) * (Ek + Ekp1), T(0.5) * (Ep_k + e_pmp(dsig, h)));
+        T dh = T(0.5) * dsig;
+        w.r += (-hinv * Kh * (Ekp1 - Ek) / dsig) / dh;
+        T co = hinv * Kh / (dsig * dh);
+        w.b += co;
+        w.c -= co;
+    } else {
+        if (A.flags & 2) {
+            T wu = T(0.5) * (om_k + om_p), wl = T(0.5) * (om_m + om_k);
+            T up = fmax(wu, T(0)), un = fmin(wu, T(0)), lp = fmax(wl, T(0)), ln = fmin(wl, T(0));
+            T di = T(1) / dsig;
+            w.r += T(kRhoI) * ((up * Ek + un * Ekp1) - (lp * Ekm1 + ln * Ek)) * di;
+            w.a -= T(kRhoI) * lp * di;
+            w.c += T(kRhoI) * un * di;
+            w.b += T(kRhoI) * (up - ln) * di;
+        }
+        T Ep_m = e_pmp(T(k - 1) * dsig, h), Ep_p = e_pmp(T(min(k + 1, nz - 1)) * dsig, h);
+        T Ku = diffusivity(T(0.5) * (Ek + Ekp1), T(0.5) * (Ep_k + Ep_p));
+        T Kl = diffusivity(T(0.5) * (Ek + Ekm1), T(0.5) * (Ep_k + Ep_m));
+        T d2 = T(1) / (dsig * dsig);
+        w.r += -hinv * d2 * (Ku * (Ekp1 - Ek) - Kl * (Ek - Ekm1));
+        T dl = hinv * d2 * Kl, du = hinv * d2 * Ku;
+        w.a -= dl;
+        w.c -= du;
+        w.b += dl + du;
+    }
+    if (A.flags & 4) {
+        w.r += h * T(kRhoW) * T(kLHeat) * (A.drain_rate * water_content(Ek, Ep_k)) / T(kEScale);
+        w.b += h * T(kRhoW) * T(kLHeat) * A.drain_rate * dwater(Ek, Ep_k);
+    }
+    return w;
+}
+
+// Parallel cyclic reduction over the wavefront: lane k holds row k,
+// a x_{k-1} + b x_k + c x_{k+1} = d, with identity rows beyond the column;
+// returns x_k.  Six doubling steps cover 64 rows.
+template <typename T>
+__device__ __forceinline__ T pcr_solve(T a, T b, T c, T d) {
+    const int k = lane_id();
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        const bool lo = k - s >= 0, hi = k + s < 64;
+        const int km = lo ? k - s : k, kp = hi ? k + s : k;
+        T am = lane_get(a, km), bm = lane_get(b, km), cm = lane_get(c, km), dm = lane_get(d, km);
+        T ap = lane_get(a, kp), bp = lane_get(b, kp), cp = lane_get(c, kp), dp = lane_get(d, kp);
+        T al = lo ? -a / bm : T(0);
+        T ga = hi ? -c / bp : T(0);
+        T na = al * am, nc = ga * cp;
+        T nb = b + al * cm + ga * ap;
+        T nd = d + al * dm + ga * dp;
+        a = na;
+        b = nb;
+        c = nc;
+        d = nd;
+    }
+    return d / b;
+}
+
+// Column handled by a wavefront.
+struct ColIdx {
+    long col;
+    int i, j, k;
+    bool valid;
+};
+
+__device__ __forceinline__ ColIdx col_index(long col, int nx, long ncol) {
+    ColIdx c;
+    c.col = col;
+    c.valid = col < ncol;
+    c.i = c.valid ? (int)(col / nx) : 0;
+    c.j = c.valid ? (int)(col % nx) : 0;
+    c.k = lane_id();
+    return c;
+}
+__device__ __forceinline__ long wave_column() { return (long)blockIdx.x * kColsPerBlock + (threadIdx.x >> 6); }
+
+// Residual r = F(E) - f_E (glide/cuda/enthalpy.cu:783-905) on a grid-stride
+// loop over columns, and per-block max |r| into partials[blockIdx.x].
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_residual(EnthArgs<T> A, T* __restrict__ r_out, int use_forcing,
+                                                       double* __restrict__ partials) {
+    __shared__ double smax[kColsPerBlock];
+    if (skipped(A.skip)) return;
+    const int nz = A.nz, k = lane_id();
+    const long ncol = (long)A.ny * A.nx;
+    const bool in = k < nz;
+    double m = 0.0;
+    for (long col = wave_column(); col < ncol; col += (long)gridDim.x * kColsPerBlock) {
+        const ColIdx q = col_index(col, A.nx, ncol);
+        const long base = col * nz;
+        const T h = A.H[col];
+        const T Ek = in ? A.E[base + k] : T(0);
+        const T omk = (in && A.omega) ? A.omega[base + k] : T(0);
+        const T Ekm1 = lane_get(Ek, max(k - 1, 0)), Ekp1 = lane_get(Ek, min(k + 1, 63));
+        const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
+        T r = T(0);
+        if (in && k < nz - 1 && !(h < A.h_thin)) {
+            Horiz<T> hz{};
+            if (A.flags & 1) hz = load_horiz(A, q.i, q.j, k);
+            Row<T> w = level_row(A, k, h, T(1) / h, A.H_prev[col], Ek, Ekm1, Ekp1, A.E_prev[base + k], omm, omk,
+                                 omp, hz);
+            r = use_forcing ? w.r - A.f_E[base + k] : w.r;
+        }
+        if (in && r_out) r_out[base + k] = r;
+        m = fmax(m, fabs((double)r));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if (k == 0) smax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+}
+
+// Final max over the partials.  With a status record: mode 0 starts a sweep
+// (r0, zero count, clear the skip flag); mode 1 closes one sweep iteration
+// (count, norm, stopping test of glide/enthalpy.py:426-471) unless the
+// sweep already stopped.
+__global__ __launch_bounds__(256) void k_enth_reduce(const double* __restrict__ part, int n, double* __restrict__ out,
+                                                     EnthStatus* status, int mode, double rtol, double atol) {
+    __shared__ double sh[4];
+    if (status && mode == 1 && status->done) return;
+    double m = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, part[q]);
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    if (out) *out = r;
+    if (!status) return;
+    if (mode == 0) {
+        status->r0 = r;
+        status->r = r;
+        status->sweeps = 0;
+        status->done = 0;
+    } else {
+        const int it = status->sweeps;
+        if (it < kEnthMaxSweeps) status->norms[it] = r;
+        status->sweeps = it + 1;
+        status->r = r;
+        const double r0 = status->r0;
+        const double rel = r0 > 0.0 ? r / r0 : 0.0;
+        if (rel < rtol || r < atol) status->done = 1;
+    }
+}
+
+// Column Newton smoother (glide/cuda/enthalpy.cu:932-1112): Newton on the
+// column with horizontal neighbours frozen at E; writes delta = E_col - E
+// and/or E_out = E + omega_relax * delta.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __restrict__ delta, T* __restrict__ E_out,
+                                                            T omega_relax, int n_newton, T relaxation) {
+    if (skipped(A.skip)) return;
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k;
+    const bool in = k < nz;
+    const long base = q.col * nz;
+    const T h = A.H[q.col];
+    const T E0 = in ? A.E[base + k] : T(0);
+    const T Es = A.E_surface[q.col];
+    T El = E0;
+    if (h < A.h_thin) {
+        El = fmin(Es, e_pmp(T(k) * (T(1) / T(nz - 1)), h));
+    } else {
+        const T hinv = T(1) / h, hp = A.H_prev[q.col];
+        const bool interior = in && k < nz - 1;
+        const T Epk = in ? A.E_prev[base + k] : T(0);
+        const T fEk = in ? A.f_E[base + k] : T(0);
+        const T omk = (in && A.omega) ? A.omega[base + k] : T(0);
+        const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
+        Horiz<T> hz{};
+        if ((A.flags & 1) && interior) hz = load_horiz(A, q.i, q.j, k);
+        for (int it = 0; it < n_newton; ++it) {
+            const T Ekm1 = lane_get(El, max(k - 1, 0)), Ekp1 = lane_get(El, min(k + 1, 63));
+            T a = T(0), b = T(1), c = T(0), d = T(0);
+            if (interior) {
+                Row<T> w = level_row(A, k, h, hinv, hp, El, Ekm1, Ekp1, Epk, omm, omk, omp, hz);
+                a = k == 0 ? T(0) : w.a;
+                b = w.b;
+                c = w.c;
+                d = -(w.r - fEk);
+            } else if (k == nz - 1) {
+                d = -(El - Es);  // Dirichlet surface row
+            }
+            const T dx = pcr_solve(a, b, c, d);
+            if (in) El += relaxation * dx;
+        }
+    }
+    if (!in) return;
+    const T dE = El - E0;
+    if (delta) delta[base + k] = dE;
+    if (E_out) E_out[base + k] = E0 + omega_relax * dE;
+}
+
+// Pointwise Jacobi layer smoother (glide/cuda/enthalpy.cu:1128-1255):
+// delta = -r / J_kk at every interior node, zero on the surface row and in
+// thin columns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_layer_smooth(EnthArgs<T> A, T* __restrict__ delta, T* __restrict__ E_out,
+                                                           T omega_relax) {
+    if (skipped(A.skip)) return;
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k;
+    const bool in = k < nz;
+    const long base = q.col * nz;
+    const T h = A.H[q.col];
+    const T Ek = in ? A.E[base + k] : T(0);
+    const T omk = (in && A.omega) ? A.omega[base + k] : T(0);
+    const T Ekm1 = lane_get(Ek, max(k - 1, 0)), Ekp1 = lane_get(Ek, min(k + 1, 63));
+    const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
+    if (!in) return;
+    T out = T(0);
+    if (k < nz - 1 && !(h < A.h_thin)) {
+        Horiz<T> hz{};
+        if (A.flags & 1) hz = load_horiz(A, q.i, q.j, k);
+        Row<T> w = level_row(A, k, h, T(1) / h, A.H_prev[q.col], Ek, Ekm1, Ekp1, A.E_prev[base + k], omm, omk, omp,
+                             hz);
+        const T r = w.r - A.f_E[base + k];
+        out = (fabs(w.b) > T(1e-30)) ? -r / w.b : T(0);
+    }
+    if (delta) delta[base + k] = out;
+    if (E_out) E_out[base + k] = Ek + omega_relax * out;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_apply(const int* __restrict__ skip, T* __restrict__ E,
+                                                    const T* __restrict__ delta, T omega_relax, long n) {
+    if (skipped(skip)) return;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        E[q] += omega_relax * delta[q];
+}
+
+// omega = H sigma_dot by upward integration of the sigma-space continuity
+// equation (glide/cuda/enthalpy.cu:1276-1372): lanes form the per-level
+// flux divergence, lane 0 integrates in level order; the result is divided
+// by out_div.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_omega(EnthArgs<T> A, const T* __restrict__ dhdt,
+                                                    const T* __restrict__ bmb, T out_div, T* __restrict__ omega) {
+    __shared__ T sdiv[kColsPerBlock][64];
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k, w = threadIdx.x >> 6;
+    const int ny = A.ny, nx = A.nx, i = q.i, j = q.j;
+    const long base = q.col * nz;
+    const T h = A.H[q.col];
+    if (h < T(1e-3)) {
+        if (k < nz) omega[base + k] = T(0);
+        return;
+    }
+    if (k < nz) {
+        const T* u = A.u3d + (long)k * ny * (nx + 1);
+        const T* v = A.v3d + (long)k * (ny + 1) * nx;
+        T fr = 0, fl = 0, fb = 0, ft = 0;
+        if (j < nx - 1) {
+            T ur = u[(long)i * (nx + 1) + j + 1], Hp = A.H[q.col + 1];
+            fr = T(0.5) * (h + Hp) * ur - T(0.5) * sqrt(ur * ur + T(kMassFluxRegYr)) * (Hp - h);
+        }
+        if (j > 0) {
+            T ul = u[(long)i * (nx + 1) + j], Hm = A.H[q.col - 1];
+            fl = T(0.5) * (Hm + h) * ul - T(0.5) * sqrt(ul * ul + T(kMassFluxRegYr)) * (h - Hm);
+        }
+        if (i < ny - 1) {
+            T vb = v[(long)(i + 1) * nx + j], Hp = A.H[q.col + nx];
+            fb = T(0.5) * (h + Hp) * vb - T(0.5) * sqrt(vb * vb + T(kMassFluxRegYr)) * (h - Hp);
+        }
+        if (i > 0) {
+            T vt = v[(long)i * nx + j], Hm = A.H[q.col - nx];
+            ft = T(0.5) * (Hm + h) * vt - T(0.5) * sqrt(vt * vt + T(kMassFluxRegYr)) * (Hm - h);
+        }
+        sdiv[w][k] = (fr - fl + ft - fb) * A.dxi;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (k == 0) {
+        const T dsig = T(1) / T(nz - 1), dh = dhdt[q.col];
+        T o = bmb ? -bmb[q.col] : T(0);
+        omega[base] = o / out_div;
+        for (int m = 1; m < nz; ++m) {
+            o = o - dsig * (dh + T(0.5) * (sdiv[w][m - 1] + sdiv[w][m]));
+            omega[base + m] = o / out_div;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_broadcast(const T* __restrict__ src, T* __restrict__ dst, long n2d,
+                                                        long n, T div) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        dst[q] = src[q % n2d] / div;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_tendency(const T* __restrict__ a, const T* __restrict__ b, T div,
+                                                       T* __restrict__ out, long n) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        out[q] = (a[q] - b[q]) / div;
+}
+
+// Basal frictional heating Q_fh = beta |u_b|^(m+1) from the bed-layer
+// velocities averaged to cell centres (glide/model.py:215-228), then the
+// forcing f_E of glide/enthalpy.py:473-519.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_rhs(EnthArgs<T> A, const T* __restrict__ beta, T m, T u_reg,
+                                                  int frictional, T* __restrict__ Q_fh, T* __restrict__ f_E) {
+    // frictional: Q_fh from the velocities (else read); f_E == nullptr: Q_fh only.
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k, nx = A.nx;
+    const long base = q.col * nz;
+    T qfh;
+    if (frictional) {
+        const T* ub = A.u3d;
+        const T* vb = A.v3d;
+        T uc = T(0.5) * (ub[(long)q.i * (nx + 1) + q.j + 1] + ub[(long)q.i * (nx + 1) + q.j]);
+        T vc = T(0.5) * (vb[(long)(q.i + 1) * nx + q.j] + vb[(long)q.i * nx + q.j]);
+        T speed = sqrt(uc * uc + vc * vc + u_reg * u_reg);
+        qfh = beta[q.col] * pow(speed, m + T(1));
+        if (k == 0) Q_fh[q.col] = qfh;
+    } else {
+        qfh = Q_fh[q.col];
+    }
+    if (k >= nz || !f_E) return;
+    const T h = A.H[q.col];
+    const T dsig_half = T(0.5) * (T(1) / T(nz - 1));
+    T f = A.phi_strain ? h * A.phi_strain[base + k] / T(kEScale) : T(0);
+    if (k == 0) f += (A.Q_geo[q.col] + qfh) / (dsig_half * T(kEScale));
+    if (k == nz - 1) f = T(0);
+    f_E[base + k] = f;
+}
+
+// Paterson-Budd rate factor A(T) (1 + 181.25 w) averaged over the levels and
+// returned as B = <A>^(-1/n) / scale (glide/enthalpy.py:600-643,
+// glide/model.py:209-213).
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_arrhenius(EnthArgs<T> A, T n_glen, T scale, T* __restrict__ B) {
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k;
+    T val = T(0);
+    if (k < nz) {
+        const T h = A.H[q.col];
+        const T Tp = t_pmp(T(k) * (T(1) / T(nz - 1)), h);
+        const T Ek = A.E[q.col * nz + k];
+        const T Tk = fmin(Ek * T(kEScale) / T(kCI) + T(kTRef), Tp);
+        const T Af = Tk < T(263.15) ? T(3.985e-13) : T(1.916e3);
+        const T Qf = Tk < T(263.15) ? T(60e3) : T(139e3);
+        const T Ak = Af * exp(-Qf / (T(8.314) * fmax(Tk, T(1))));
+        const T wk = fmax(Ek * T(kEScale) - T(kCI) * (Tp - T(kTRef)), T(0)) / T(kLHeat);
+        val = Ak * (T(1) + T(181.25) * wk);
+    }
+    for (int off = 32; off > 0; off >>= 1) val += __shfl_down(val, off, 64);
+    if (k == 0) B[q.col] = pow(val / T(nz), T(-1) / n_glen) / scale;
+}
+
+// Temperature min(E E_SCALE / C_I + T_REF, T_pmp) and water content
+// max(E E_SCALE - E_pmp, 0) / L (glide/enthalpy.py:565-597).
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_diagnostics(EnthArgs<T> A, T* __restrict__ temperature,
+                                                          T* __restrict__ water) {
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid || q.k >= A.nz) return;
+    const int nz = A.nz, k = q.k;
+    const T Tp = t_pmp(T(k) * (T(1) / T(nz - 1)), A.H[q.col]);
+    const T Ek = A.E[q.col * nz + k];
+    if (temperature) temperature[q.col * nz + k] = fmin(Ek * T(kEScale) / T(kCI) + T(kTRef), Tp);
+    if (water) water[q.col * nz + k] = fmax(Ek * T(kEScale) - T(kCI) * (Tp - T(kTRef)), T(0)) / T(kLHeat);
+}
+
+// glide/enthalpy.py:534-563
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_from_temperature(EnthArgs<T> A, const T* __restrict__ temperature,
+                                                               T* __restrict__ E) {
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid || q.k >= A.nz) return;
+    const int nz = A.nz, k = q.k;
+    const T Tp = t_pmp(T(k) * (T(1) / T(nz - 1)), A.H[q.col]);
+    const T Tk = fmin(temperature[q.col * nz + k], Tp);
+    E[q.col * nz + k] = T(kCI) * (Tk - T(kTRef)) / T(kEScale);
+}
+
+// glide/enthalpy.py:521-532
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_surface_from_temperature(const T* __restrict__ Ts, T* __restrict__ Es,
+                                                                       long n) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+        Es[q] = T(kCI) * (fmin(Ts[q], T(kTMelt)) - T(kTRef)) / T(kEScale);
+}
+
+// ------------------------------------------------------------------ launchers
+static inline unsigned col_blocks(int ny, int nx) {
+    return (unsigned)(((long)ny * nx + kColsPerBlock - 1) / kColsPerBlock);
+}
+static inline unsigned flat_blocks(long n) {
+    long b = (n + 255) / 256;
+    return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+
+template <typename T>
+hipError_t launch_enth_residual(const EnthArgs<T>& A, T* r_out, int use_forcing, double* partials, double* max_out,
+                                EnthStatus* status, int status_mode, double rtol, double atol, hipStream_t st) {
+    unsigned g = col_blocks(A.ny, A.nx);
+    if (g > (unsigned)kEnthResBlocks) g = kEnthResBlocks;
+    k_enth_residual<T><<<g, 256, 0, st>>>(A, r_out, use_forcing, partials);
+    if (max_out || status) k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)g, max_out, status, status_mode, rtol, atol);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_column_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, int n_newton,
+                                     T relaxation, hipStream_t st) {
+    k_enth_column_smooth<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, delta, E_out, omega_relax, n_newton, relaxation);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_layer_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, hipStream_t st) {
+    k_enth_layer_smooth<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, delta, E_out, omega_relax);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_apply(const EnthArgs<T>& A, T* E, const T* delta, T omega_relax, hipStream_t st) {
+    long n = (long)A.ny * A.nx * A.nz;
+    k_enth_apply<T><<<flat_blocks(n), 256, 0, st>>>(A.skip, E, delta, omega_relax, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_omega(const EnthArgs<T>& A, const T* dhdt, const T* bmb, T out_div, T* omega, hipStream_t st) {
+    k_enth_omega<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, dhdt, bmb, out_div, omega);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_broadcast(const T* src, T* dst, long n2d, int nz, T div, hipStream_t st) {
+    long n = n2d * nz;
+    k_enth_broadcast<T><<<flat_blocks(n), 256, 0, st>>>(src, dst, n2d, n, div);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_tendency(const T* a, const T* b, T div, T* out, long n, hipStream_t st) {
+    k_enth_tendency<T><<<flat_blocks(n), 256, 0, st>>>(a, b, div, out, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_rhs(const EnthArgs<T>& A, const T* beta, T m, T u_reg, int frictional, T* Q_fh, T* f_E,
+                           hipStream_t st) {
+    k_enth_rhs<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, beta, m, u_reg, frictional, Q_fh, f_E);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_arrhenius(const EnthArgs<T>& A, T n_glen, T scale, T* B, hipStream_t st) {
+    k_enth_arrhenius<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, n_glen, scale, B);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_diagnostics(const EnthArgs<T>& A, T* temperature, T* water, hipStream_t st) {
+    k_enth_diagnostics<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, temperature, water);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_from_temperature(const EnthArgs<T>& A, const T* temperature, T* E, hipStream_t st) {
+    k_enth_from_temperature<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, temperature, E);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_surface_from_temperature(const T* Ts, T* Es, long n, hipStream_t st) {
+    k_enth_surface_from_temperature<T><<<flat_blocks(n), 256, 0, st>>>(Ts, Es, n);
+    return hipGetLastError();
+}
+
+#define INST(T)                                                                                                      \
+    template hipError_t launch_enth_residual<T>(const EnthArgs<T>&, T*, int, double*, double*, EnthStatus*, int,     \
+                                                double, double, hipStream_t);                                        \
+    template hipError_t launch_enth_column_smooth<T>(const EnthArgs<T>&, T*, T*, T, int, T, hipStream_t);            \
+    template hipError_t launch_enth_layer_smooth<T>(const EnthArgs<T>&, T*, T*, T, hipStream_t);                     \
+    template hipError_t launch_enth_apply<T>(const EnthArgs<T>&, T*, const T*, T, hipStream_t);                      \
+    template hipError_t launch_enth_omega<T>(const EnthArgs<T>&, const T*, const T*, T, T*, hipStream_t);            \
+    template hipError_t launch_enth_broadcast<T>(const T*, T*, long, int, T, hipStream_t);                           \
+    template hipError_t launch_enth_tendency<T>(const T*, const T*, T, T*, long, hipStream_t);                       \
+    template hipError_t launch_enth_rhs<T>(const EnthArgs<T>&, const T*, T, T, int, T*, T*, hipStream_t);            \
+    template hipError_t launch_enth_arrhenius<T>(const EnthArgs<T>&, T, T, T*, hipStream_t);                         \
+    template hipError_t launch_enth_diagnostics<T>(const EnthArgs<T>&, T*, T*, hipStream_t);                         \
+    template hipError_t launch_enth_from_temperature<T>(const EnthArgs<T>&, const T*, T*, hipStream_t);              \
+    template hipError_t launch_enth_surface_from_temperature<T>(const T*, T*, long, hipStream_t);
+INST(double)
+#undef INST
+
+}  // namespace glide

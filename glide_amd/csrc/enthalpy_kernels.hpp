@@ -1,0 +1,72 @@
+// Launch interface of the enthalpy kernels (enthalpy_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace glide {
+
+// Everything a column kernel reads.  Layouts: E, E_prev, f_E, omega,
+// phi_strain (ny, nx, nz); u3d (nz, ny, nx+1); v3d (nz, ny+1, nx); H, H_prev,
+// E_surface, Q_geo (ny, nx).
+template <typename T>
+struct EnthArgs {
+    int ny, nx, nz, flags;  // flags: 1 horizontal advection, 2 sigma advection, 4 drainage
+    T dxi, dt, drain_rate, h_thin;
+    T lf_sq;  // sqrt(1 + lf_c), the Lax-Friedrichs dissipation factor
+    const T *E, *E_prev, *f_E, *u3d, *v3d, *omega, *H, *H_prev, *E_surface, *Q_geo, *phi_strain;
+    const int* skip;  // nullable: a launch does nothing while *skip != 0
+};
+
+// Device record of a fused column sweep (EnthalpyOperators.column_sweep).
+constexpr int kEnthMaxSweeps = 256;
+struct EnthStatus {
+    double r0, r;
+    int sweeps, done;  // `done` is the skip flag of the remaining launches
+    double norms[kEnthMaxSweeps];
+};
+
+constexpr int kEnthResBlocks = 2048;  // residual grid (grid-stride over columns)
+
+// max |r| of the residual over the grid, into `max_out` (device), or, with
+// `status`, the sweep bookkeeping: mode 0 starts a sweep (r0), mode 1 ends an
+// iteration (count, norm, stopping test).
+template <typename T>
+hipError_t launch_enth_residual(const EnthArgs<T>& A, T* r_out, int use_forcing, double* partials, double* max_out,
+                                EnthStatus* status, int status_mode, double rtol, double atol, hipStream_t st);
+// Column Newton smoother; writes delta = E_col - E (if non-null) and/or
+// E_out = E + omega_relax * delta (if non-null).
+template <typename T>
+hipError_t launch_enth_column_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, int n_newton,
+                                     T relaxation, hipStream_t st);
+// Pointwise Jacobi layer smoother; delta = -r / J_kk and/or E_out = E + omega_relax * delta.
+template <typename T>
+hipError_t launch_enth_layer_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, hipStream_t st);
+// E += omega_relax * delta (skips with A.skip).
+template <typename T>
+hipError_t launch_enth_apply(const EnthArgs<T>& A, T* E, const T* delta, T omega_relax, hipStream_t st);
+// omega = H sigma_dot from the sigma-space continuity equation, divided by out_div.
+template <typename T>
+hipError_t launch_enth_omega(const EnthArgs<T>& A, const T* dhdt, const T* bmb, T out_div, T* omega, hipStream_t st);
+// dst[k] = src / div for every layer k of a (nz, n2d) array.
+template <typename T>
+hipError_t launch_enth_broadcast(const T* src, T* dst, long n2d, int nz, T div, hipStream_t st);
+// out = (a - b) / div elementwise (the column thickness tendency).
+template <typename T>
+hipError_t launch_enth_tendency(const T* a, const T* b, T div, T* out, long n, hipStream_t st);
+// Basal frictional heating (if `frictional`) and the forcing f_E.
+template <typename T>
+hipError_t launch_enth_rhs(const EnthArgs<T>& A, const T* beta, T m, T u_reg, int frictional, T* Q_fh, T* f_E,
+                           hipStream_t st);
+// B = mean_k(A_k (1 + 181.25 w_k))^(-1/n) / scale.
+template <typename T>
+hipError_t launch_enth_arrhenius(const EnthArgs<T>& A, T n_glen, T scale, T* B, hipStream_t st);
+// Temperature (clipped at the pressure melting point) and water content.
+template <typename T>
+hipError_t launch_enth_diagnostics(const EnthArgs<T>& A, T* temperature, T* water, hipStream_t st);
+// E = C_I (min(T, T_pmp) - T_REF) / E_SCALE for a (ny, nx, nz) temperature.
+template <typename T>
+hipError_t launch_enth_from_temperature(const EnthArgs<T>& A, const T* temperature, T* E, hipStream_t st);
+// E_surface = C_I (min(T_s, T_MELT) - T_REF) / E_SCALE.
+template <typename T>
+hipError_t launch_enth_surface_from_temperature(const T* Ts, T* Es, long n, hipStream_t st);
+
+}  // namespace glide

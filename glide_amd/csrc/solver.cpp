@@ -436,6 +436,10 @@ int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, 
 
 }  // namespace
 
+namespace glide {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace glide
+
 // ====================================================================== C ABI
 extern "C" {
 

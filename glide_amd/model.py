@@ -71,3 +71,116 @@ class IceDynamics:
                  (4 if compute_H_prev_grad else 0) | (8 if compute_smb_grad else 0))
         if which:
             ao._grad(which, dt)
+
+
+class ThermalModel:
+    """Enthalpy solver coupled to the momentum step (glide/model.py:97-280).
+
+    ``pre_momentum()`` before ``IceDynamics.forward``, ``step(dt)`` after it:
+    the step broadcasts the SSA velocity to the sigma levels, derives omega
+    from the realised thickness change, adds basal frictional heating, runs
+    the column sweep and feeds the depth-averaged rate factor back into
+    ``grid.rheology.B`` (head units).  Without a smoother hook the whole step
+    is one native call."""
+
+    SEC_PER_YR = 365.25 * 86400.0
+
+    def __init__(self, grid, nz=21, n_smooth=10, update_rheology=True, frictional_heating=True, rho_i=917.0):
+        from .enthalpy import EnthalpyOperators
+        self.ops = EnthalpyOperators(grid, nz=nz)
+        self.n_smooth = n_smooth
+        self.update_rheology = update_rheology
+        self.frictional_heating = frictional_heating
+        self.rho_i = rho_i
+        self.g = 9.81
+        self.last_report = None
+
+    def initialize(self, T_surface, T_field=None, Q_geo=None):
+        """Initial temperature (K), surface temperature (Dirichlet) and
+        geothermal flux (W/m^2) (glide/model.py:134-154)."""
+        if T_field is None:
+            T_field = T_surface
+        self.ops.initialize_from_temperature(T_field)
+        self.ops.set_surface_enthalpy_from_temperature(T_surface)
+        if Q_geo is not None:
+            self.ops.enthalpy_forcing.Q_geo.copy_(torch.as_tensor(Q_geo, dtype=torch.float64).expand(
+                self.ops.grid.ny, self.ops.grid.nx))
+
+    def pre_momentum(self):
+        """Snapshot E and H before the momentum step (glide/model.py:156-165)."""
+        ops = self.ops
+        ops.enthalpy_state.E_prev.copy_(ops.enthalpy_state.E)
+        ops.H_prev.copy_(ops.grid.state.H.data)
+
+    def step(self, dt, report=False):
+        """Advance the enthalpy one step (glide/model.py:167-213)."""
+        import ctypes
+
+        from . import _lib
+        from ._lib import call, ptr
+        ops = self.ops
+        if ops.smoother_config.hook_func is not None:
+            return self._step_host(dt)
+        grid = ops.grid
+        n = float(grid.rheology.n.value)
+        tc = _lib.ThermalConfigC(dt=float(dt), m=float(grid.sliding.m.value), u_reg=float(grid.sliding.u_reg.value),
+                                 rho_i=float(self.rho_i), n_glen=n, n_smooth=int(self.n_smooth),
+                                 frictional_heating=1 if self.frictional_heating else 0,
+                                 update_rheology=1 if self.update_rheology else 0)
+        want = report or ops.smoother_config.report_norms
+        rep = _lib.EnthReport() if want else None
+        C = ops._column()
+        call("glide_thermal_step", ctypes.byref(C), ptr(grid.state.u.data), ptr(grid.state.v.data),
+             ptr(grid.sliding.beta.data), ptr(grid.rheology.B.data) if self.update_rheology else None,
+             ctypes.byref(tc), ctypes.byref(ops.smoother_config.to_c(True)),
+             ctypes.byref(rep) if rep is not None else None, ops._stream())
+        self.last_report = rep
+        return rep
+
+    def _step_host(self, dt):
+        ops = self.ops
+        ops.broadcast_velocity()
+        self._compute_omega(dt)
+        ops.enthalpy_velocity.u3d /= self.SEC_PER_YR
+        ops.enthalpy_velocity.v3d /= self.SEC_PER_YR
+        if self.frictional_heating:
+            self._compute_frictional_heating()
+        ops.set_rhs(snapshot=False)
+        ops.column_sweep(dt, self.n_smooth)
+        if self.update_rheology:
+            ops.grid.rheology.B.data.copy_(ops.get_arrhenius_factor() / self.B_scale)
+        return None
+
+    def _compute_frictional_heating(self):
+        """glide/model.py:215-228."""
+        s = self.ops.grid.sliding
+        self.ops.compute_frictional_heating(s.m.value, s.u_reg.value)
+
+    def _compute_omega(self, dt):
+        """omega from (H - H_prev) / dt in m/a, converted to m/s (glide/model.py:230-254)."""
+        ops = self.ops
+        dt_yr = dt / self.SEC_PER_YR
+        ops.compute_omega((ops.grid.state.H.data - ops.H_prev) / dt_yr)
+        ops.enthalpy_velocity.omega /= self.SEC_PER_YR
+
+    @property
+    def B_scale(self):
+        """B_glide = B_SI / B_scale (glide/model.py:256-260)."""
+        n = float(self.ops.grid.rheology.n.value)
+        return self.rho_i * self.g * self.SEC_PER_YR ** (1.0 / n)
+
+    @property
+    def temperature(self):
+        return self.ops.get_temperature()
+
+    @property
+    def water_content(self):
+        return self.ops.get_water_content()
+
+    @property
+    def enthalpy(self):
+        return self.ops.enthalpy_state.E
+
+    @property
+    def sigma(self):
+        return self.ops.sigma

@@ -1,0 +1,145 @@
+/*
+ * glide_enthalpy.h — C ABI of the MI355X (gfx950) enthalpy column solver
+ * coupled to the SSA step: residual, column Newton smoother, layer smoother,
+ * the fused column sweep, the omega integration, forcing, the rate-factor
+ * feedback to the momentum solver, and one fused thermal step.
+ *
+ * Same conventions as glide_hip.h: caller-owned FP64 device buffers,
+ * row-major and contiguous; stream passed as void*; 0 on success, a negative
+ * glide_status on failure (text from glide_last_error()).  Layouts:
+ *   E, E_prev, f_E, omega, phi_strain, r_E, delta_E, E_work: (ny, nx, nz)
+ *   u3d (nz, ny, nx+1), v3d (nz, ny+1, nx)
+ *   H, H_prev, E_surface, Q_geo, Q_fh, dh_dt: (ny, nx)
+ * Enthalpy is scaled by E_SCALE = C_I (T_MELT - T_REF); 2 <= nz <= 64.
+ *
+ * Reference interfaces replaced (JacobDowns/glide):
+ *   glide_enth_residual          <- EnthalpyOperators.compute_residual / kernel enthalpy_compute_residual
+ *                                   (glide/enthalpy.py:341-364, glide/cuda/enthalpy.cu:783-905)
+ *   glide_enth_column_smooth     <- EnthalpyOperators.column_smooth / kernel enthalpy_column_smooth
+ *                                   (glide/enthalpy.py:366-393, glide/cuda/enthalpy.cu:932-1112)
+ *   glide_enth_layer_smooth      <- EnthalpyOperators.layer_smooth / kernel enthalpy_layer_smooth
+ *                                   (glide/enthalpy.py:395-424, glide/cuda/enthalpy.cu:1128-1255)
+ *   glide_enth_apply             <- the update E += omega delta_E in column_sweep (glide/enthalpy.py:452-457)
+ *   glide_enth_column_sweep      <- EnthalpyOperators.column_sweep (glide/enthalpy.py:426-471)
+ *   glide_enth_omega             <- EnthalpyOperators.compute_omega / kernel compute_omega
+ *                                   (glide/enthalpy.py:303-339, glide/cuda/enthalpy.cu:1276-1372)
+ *   glide_enth_broadcast_velocity <- EnthalpyOperators.broadcast_velocity and the m/a -> m/s
+ *                                   conversion of ThermalModel.step (glide/enthalpy.py:289-301,
+ *                                   glide/model.py:180-197)
+ *   glide_enth_frictional_heating <- ThermalModel._compute_frictional_heating (glide/model.py:215-228)
+ *   glide_enth_set_rhs           <- EnthalpyOperators.set_rhs after its optional snapshot
+ *                                   (glide/enthalpy.py:473-519)
+ *   glide_enth_arrhenius         <- EnthalpyOperators.get_arrhenius_factor and the head-unit scale of
+ *                                   ThermalModel.step (glide/enthalpy.py:600-643, glide/model.py:205-213)
+ *   glide_enth_diagnostics       <- EnthalpyOperators.get_temperature / get_water_content
+ *                                   (glide/enthalpy.py:565-598)
+ *   glide_enth_from_temperature  <- EnthalpyOperators.initialize_from_temperature (glide/enthalpy.py:534-563)
+ *   glide_enth_surface_from_temperature
+ *                                <- EnthalpyOperators.set_surface_enthalpy_from_temperature
+ *                                   (glide/enthalpy.py:521-532)
+ *   glide_thermal_step           <- ThermalModel.step (glide/model.py:167-213)
+ */
+#ifndef GLIDE_ENTHALPY_H
+#define GLIDE_ENTHALPY_H
+
+#include "glide_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLIDE_ENTH_HORIZONTAL 1 /* horizontal Lax-Friedrichs advection */
+#define GLIDE_ENTH_OMEGA 2      /* sigma advection by omega */
+#define GLIDE_ENTH_DRAINAGE 4   /* temperate-ice drainage */
+#define GLIDE_ENTH_MAX_SWEEPS 256
+
+/* One grid of columns (EnthalpyOperators state, forcing and work arrays). */
+typedef struct glide_enth_column {
+    int32_t ny, nx, nz;
+    int32_t term_flags; /* GLIDE_ENTH_* bits (EnthalpyTermFlags.bitmask) */
+    double dx;          /* m */
+    double drain_rate;  /* s^-1 (EnthalpyForcing.drain_rate) */
+    double h_thin;      /* m: thinner columns are clamped to the surface enthalpy */
+    double lf_c;        /* Lax-Friedrichs dissipation parameter (ColumnSmootherConfig.lf_c) */
+    void *E, *E_prev, *f_E, *omega, *phi_strain, *r_E, *delta_E, *E_work;
+    void *u3d, *v3d;
+    void *H, *H_prev, *E_surface, *Q_geo, *Q_fh, *dh_dt;
+} glide_enth_column;
+
+/* ColumnSmootherConfig (glide/enthalpy.py:183-198) minus the Python hook. */
+typedef struct glide_enth_sweep_config {
+    double omega;      /* relaxation of each sweep's update */
+    double relaxation; /* Newton step relaxation */
+    double relative_tolerance, absolute_tolerance;
+    int32_t n_newton;
+    int32_t alternating; /* follow each column sweep by a layer sweep */
+} glide_enth_sweep_config;
+
+typedef struct glide_enth_report {
+    int32_t sweeps; /* sweeps taken */
+    int32_t pad;
+    double r0, r;   /* max |r| before the first and after the last sweep */
+    double norms[GLIDE_ENTH_MAX_SWEEPS];
+} glide_enth_report;
+
+/* ThermalModel settings (glide/model.py:97-132). */
+typedef struct glide_thermal_config {
+    double dt;      /* s */
+    double m, u_reg; /* sliding exponent and regularisation for Q_fh */
+    double rho_i;   /* ice density of the B unit conversion */
+    double n_glen;
+    int32_t n_smooth;
+    int32_t frictional_heating;
+    int32_t update_rheology;
+    int32_t pad;
+} glide_thermal_config;
+
+/* r_E = F(E) - f_E (use_forcing != 0) or F(E); when max_abs is non-NULL the
+ * stream is synchronised and max |r_E| is stored there. */
+int glide_enth_residual(const glide_enth_column* C, double dt, int use_forcing, double* max_abs, void* stream);
+/* delta_E = column Newton solve (n_newton steps, relaxation) minus E. */
+int glide_enth_column_smooth(const glide_enth_column* C, double dt, int n_newton, double relaxation, void* stream);
+/* delta_E = -r / J_kk. */
+int glide_enth_layer_smooth(const glide_enth_column* C, double dt, void* stream);
+/* E += omega delta_E. */
+int glide_enth_apply(const glide_enth_column* C, double omega, void* stream);
+/* Up to n_iter sweeps (column smoother, then layer smoother when alternating),
+ * stopping after the first sweep whose max |r| is below the relative or the
+ * absolute tolerance.  The stopping test runs on the device: no host
+ * synchronisation unless `report` is non-NULL (then one, at the end).
+ * Uses r_E and E_work (alternating) or delta_E (not alternating). */
+int glide_enth_column_sweep(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
+                            glide_enth_report* report, void* stream);
+/* omega = H sigma_dot from dh_dt and the layer velocities (same units), with
+ * basal melt bmb (NULL = 0), divided by divide_by. */
+int glide_enth_omega(const glide_enth_column* C, const void* dh_dt, const void* bmb, double divide_by, void* stream);
+/* u3d[k] = u2d / divide_by, v3d[k] = v2d / divide_by for every level k. */
+int glide_enth_broadcast_velocity(const glide_enth_column* C, const void* u2d, const void* v2d, double divide_by,
+                                  void* stream);
+/* Q_fh = beta (|u_b|^2 + u_reg^2)^((m+1)/2) from the bed-level velocities. */
+int glide_enth_frictional_heating(const glide_enth_column* C, const void* beta, double m, double u_reg,
+                                  void* stream);
+/* f_E from phi_strain (NULL = 0), Q_geo and Q_fh. */
+int glide_enth_set_rhs(const glide_enth_column* C, void* stream);
+/* B = (mean over levels of A(T)(1 + 181.25 w))^(-1/n_glen) / scale. */
+int glide_enth_arrhenius(const glide_enth_column* C, double n_glen, double scale, void* B, void* stream);
+/* Temperature and/or water content (either may be NULL), (ny, nx, nz). */
+int glide_enth_diagnostics(const glide_enth_column* C, void* temperature, void* water, void* stream);
+/* E_out = scaled enthalpy of the (ny, nx, nz) temperature capped at T_pmp. */
+int glide_enth_from_temperature(const glide_enth_column* C, const void* temperature, void* E_out, void* stream);
+/* E_surface_out = scaled enthalpy of the (ny, nx) temperature capped at T_MELT. */
+int glide_enth_surface_from_temperature(const glide_enth_column* C, const void* T_surface, void* E_surface_out,
+                                        void* stream);
+/* ThermalModel.step after pre_momentum: velocities u2d, v2d in m/a and H the
+ * new thickness; fills u3d, v3d (m/s), dh_dt, omega, Q_fh (when
+ * frictional_heating), f_E, runs the column sweep and, when
+ * update_rheology, writes B (head units) into B_out. */
+int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta, void* B_out,
+                       const glide_thermal_config* tc, const glide_enth_sweep_config* sc, glide_enth_report* report,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLIDE_ENTHALPY_H */

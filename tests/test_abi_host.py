@@ -11,9 +11,11 @@ ROOT = Path(__file__).resolve().parents[1]
 
 
 def declared_symbols():
-    text = (ROOT / "include" / "glide_hip.h").read_text()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(glide_[a-z0-9_]+)\s*\(", text)))
+    syms = set()
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        syms.update(re.findall(r"\b(glide_[a-z0-9_]+)\s*\(", text))
+    return sorted(syms)
 
 
 def test_library_exports_every_declared_symbol():
@@ -39,6 +41,14 @@ def test_struct_layouts_match_header():
     ptrs = re.findall(r"\*(\w+)", body)
     assert tuple(ptrs) == _lib.LEVEL_POINTERS
     assert ctypes.sizeof(_lib.SolveReport) == 8 + 8 + 8 + 64 * 8 + 3 * 8
+    eh = (ROOT / "include" / "glide_enthalpy.h").read_text()
+    body = eh[eh.index("typedef struct glide_enth_column"):eh.index("} glide_enth_column;")]
+    assert tuple(re.findall(r"\*(\w+)", body)) == _lib.ENTH_POINTERS
+    assert ctypes.sizeof(_lib.EnthColumn) == 16 + 32 + 8 * len(_lib.ENTH_POINTERS)
+    assert ctypes.sizeof(_lib.EnthSweepConfigC) == 4 * 8 + 2 * 4
+    assert ctypes.sizeof(_lib.EnthReport) == 8 + 16 + 8 * _lib.ENTH_MAX_SWEEPS
+    assert ctypes.sizeof(_lib.ThermalConfigC) == 5 * 8 + 4 * 4
+    assert f"GLIDE_ENTH_MAX_SWEEPS {_lib.ENTH_MAX_SWEEPS}" in eh
 
 
 def test_version_and_error_text():
@@ -68,6 +78,45 @@ def test_hierarchy_checks_on_host():
     cfg.maximum_vcycles = 65
     assert lib.glide_fascd_solve(levels, 2, 0, 1.0, ctypes.byref(cfg), vc, None, None) == -1
     assert "maximum_vcycles" in _lib.last_error()
+
+
+def test_enthalpy_argument_checks_on_host():
+    from glide_amd import _lib
+    lib = _lib.load()
+    C = _lib.EnthColumn(ny=4, nx=4, nz=65, term_flags=7, dx=100.0)
+    assert lib.glide_enth_residual(ctypes.byref(C), 1.0, 1, None, None) == -2
+    assert "nz" in _lib.last_error()
+    C.nz = 11
+    assert lib.glide_enth_residual(ctypes.byref(C), 1.0, 1, None, None) == -1
+    assert "NULL" in _lib.last_error()
+    C.term_flags = 8
+    assert lib.glide_enth_set_rhs(ctypes.byref(C), None) == -1
+    assert "flag" in _lib.last_error()
+    C.term_flags = 7
+    cfg = _lib.EnthSweepConfigC(1.0, 1.0, 1e-5, 1e-2, 3, 1)
+    assert lib.glide_enth_column_sweep(ctypes.byref(C), 1.0, 257, ctypes.byref(cfg), None, None) == -1
+    assert "n_iter" in _lib.last_error()
+    assert lib.glide_thermal_step(ctypes.byref(C), None, None, None, None, None, None, None, None) == -1
+
+
+def test_enthalpy_host_objects_on_cpu():
+    from glide_amd import Grid, ThermalModel
+    from glide_amd._lib import GlideError
+    from glide_amd.enthalpy import ColumnSmootherConfig, EnthalpyTermFlags, make_sigma
+    g = Grid(8, 8, 1000.0, device="cpu")
+    tm = ThermalModel(g, nz=5)
+    ops = tm.ops
+    assert ops.enthalpy_state.E.shape == (8, 8, 5)
+    assert ops.enthalpy_velocity.u3d.shape == (5, 8, 9)
+    assert ops.enthalpy_velocity.v3d.shape == (5, 9, 8)
+    assert torch.equal(make_sigma(5), torch.tensor([0.0, 0.25, 0.5, 0.75, 1.0], dtype=torch.float64))
+    assert EnthalpyTermFlags().bitmask == 7
+    assert EnthalpyTermFlags(horizontal_advection=False, drainage=False).bitmask == 2
+    c = ColumnSmootherConfig().to_c(alternating=False)
+    assert (c.n_newton, c.alternating, c.omega) == (3, 0, 1.0)
+    assert tm.B_scale == pytest.approx(917.0 * 9.81 * (365.25 * 86400.0) ** (1.0 / 3.0))
+    with pytest.raises(GlideError):
+        ops.compute_residual(1.0)  # CPU tensors are refused, nothing runs on the host
 
 
 def test_ptr_rejects_cpu_and_wrong_dtype():

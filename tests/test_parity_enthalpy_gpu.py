@@ -1,0 +1,284 @@
+"""Parity of the gfx950 enthalpy kernels, the fused column sweep and the fused
+thermal step with the FP64 CPU oracle.
+
+Every test runs the HIP path through the C ABI (glide_amd.enthalpy /
+glide_amd.model.ThermalModel) and the oracle on the same seeded columns:
+cold and temperate ice, thin and ice-free columns, depth-varying velocities.
+Bar: single kernels agree to 1e-12 relative to the field's scale, whole
+sweeps and thermal steps to 1e-10 relative (enthalpy, omega, forcing, B).
+"""
+import numpy as np
+import pytest
+import torch
+
+from tests.util import rel_err
+
+pytestmark = pytest.mark.gpu
+
+KTOL = 1e-12   # one kernel
+STOL = 1e-10   # sweeps / thermal steps
+DT = 10.0 * 365.25 * 86400.0
+DX = 1500.0
+
+
+def T(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=torch.float64)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def enth_case(ny, nx, nz, seed, temperate_frac=0.3):
+    """Seeded enthalpy state in the reference's layouts (numpy, SI, scaled E)."""
+    from oracle.oracle import BETA_CC, C_I, E_SCALE, GRAVITY, RHO_I_E, SEC_PER_YR, T_MELT, T_REF
+    rng = np.random.default_rng(seed)
+    H = 200.0 + 1800.0 * rng.random((ny, nx))
+    r = rng.random((ny, nx))
+    H[r < 0.08] = 20.0 * rng.random(int((r < 0.08).sum()))      # thin columns (< h_thin = 25 m)
+    H[r < 0.03] = 0.0                                           # ice-free
+    sigma = np.linspace(0.0, 1.0, nz)
+    T_pmp = T_MELT - BETA_CC * RHO_I_E * GRAVITY * (1.0 - sigma)[None, None, :] * H[:, :, None]
+    Temp = T_pmp - 35.0 * rng.random((ny, nx, nz)) * sigma[None, None, :] - 0.5 * rng.random((ny, nx, nz))
+    E = C_I * (Temp - T_REF) / E_SCALE
+    warm = rng.random((ny, nx)) < temperate_frac
+    E[warm, :max(2, nz // 4)] += 0.02 * rng.random((int(warm.sum()), max(2, nz // 4)))
+    C = {"H": H, "H_prev": H + 5.0 * rng.standard_normal((ny, nx)) * (H > 0), "E": E,
+         "E_prev": E + 0.01 * rng.standard_normal(E.shape)}
+    shape_u = (1.0 + sigma)[:, None, None] / 2.0
+    C["u3d"] = shape_u * (60.0 * rng.standard_normal((1, ny, nx + 1))) / SEC_PER_YR
+    C["v3d"] = shape_u * (45.0 * rng.standard_normal((1, ny + 1, nx))) / SEC_PER_YR
+    C["omega"] = 0.3 * rng.standard_normal((ny, nx, nz)) / SEC_PER_YR
+    Ts = 235.0 + 30.0 * rng.random((ny, nx))
+    C["E_surface"] = C_I * (np.minimum(Ts, T_MELT) - T_REF) / E_SCALE
+    C["Q_geo"] = 0.05 + 0.02 * rng.random((ny, nx))
+    C["Q_fh"] = 0.02 * rng.random((ny, nx))
+    C["phi_strain"] = 2e-6 * rng.random((ny, nx, nz))
+    return C
+
+
+def make_pair(C, nz, flags=7):
+    """(oracle, EnthalpyOperators) holding the same state, f_E from the oracle's
+    set_rhs copied to the device."""
+    from glide_amd.enthalpy import EnthalpyOperators
+    from glide_amd.grid import Grid
+    from oracle.oracle import OracleThermal
+    ny, nx = C["H"].shape
+    th = OracleThermal(ny, nx, nz, DX, C["H"])
+    th.flags = flags
+    for k in ("E", "E_prev", "u3d", "v3d", "omega", "E_surface", "Q_geo", "Q_fh", "phi_strain", "H_prev"):
+        setattr(th, k, np.ascontiguousarray(C[k], dtype=np.float64).copy())
+    th.set_rhs(snapshot=False)
+    g = Grid(ny, nx, DX, device="cuda")
+    g.state.H.set(C["H"])
+    ops = EnthalpyOperators(g, nz=nz)
+    ops.term_flags.horizontal_advection = bool(flags & 1)
+    ops.term_flags.omega = bool(flags & 2)
+    ops.term_flags.drainage = bool(flags & 4)
+    st, vel, frc = ops.enthalpy_state, ops.enthalpy_velocity, ops.enthalpy_forcing
+    st.E.copy_(T(C["E"]))
+    st.E_prev.copy_(T(C["E_prev"]))
+    vel.u3d.copy_(T(C["u3d"]))
+    vel.v3d.copy_(T(C["v3d"]))
+    vel.omega.copy_(T(C["omega"]))
+    for k in ("E_surface", "Q_geo", "Q_fh", "phi_strain"):
+        getattr(frc, k).copy_(T(C[k]))
+    ops.H_prev.copy_(T(C["H_prev"]))
+    ops.set_rhs(snapshot=False)
+    return th, ops
+
+
+CASES = [(12, 10, 11, 0), (9, 14, 41, 1), (6, 5, 64, 2), (5, 7, 2, 3), (1, 3, 21, 4)]
+
+
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES)
+def test_set_rhs(ny, nx, nz, seed):
+    print(f"set_rhs {ny}x{nx}x{nz} seed={seed}", flush=True)
+    th, ops = make_pair(enth_case(ny, nx, nz, seed), nz)
+    assert rel_err(N(ops.f_E), th.f_E) < KTOL
+
+
+@pytest.mark.parametrize("flags", [7, 6, 5, 3, 1, 0])
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES)
+def test_residual(ny, nx, nz, seed, flags):
+    print(f"residual {ny}x{nx}x{nz} seed={seed} flags={flags}", flush=True)
+    th, ops = make_pair(enth_case(ny, nx, nz, seed), nz, flags)
+    m_gpu = ops.compute_residual(DT)
+    m_orc = th.compute_residual(DT)
+    assert rel_err(N(ops.r_E), th.r_E) < KTOL
+    assert abs(m_gpu - m_orc) <= KTOL * m_orc
+
+
+@pytest.mark.parametrize("flags", [7, 2, 0])
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES)
+def test_column_smooth(ny, nx, nz, seed, flags):
+    print(f"column_smooth {ny}x{nx}x{nz} seed={seed} flags={flags}", flush=True)
+    th, ops = make_pair(enth_case(ny, nx, nz, seed), nz, flags)
+    ops.column_smooth(DT)
+    d = th.column_smooth(DT)
+    # scale: the enthalpy itself (the correction solves a diagonally dominant system)
+    assert np.max(np.abs(N(ops.delta_E) - d)) < 1e-11 * max(np.max(np.abs(th.E)), 1.0)
+
+
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES)
+def test_layer_smooth(ny, nx, nz, seed):
+    print(f"layer_smooth {ny}x{nx}x{nz} seed={seed}", flush=True)
+    th, ops = make_pair(enth_case(ny, nx, nz, seed), nz)
+    ops.layer_smooth(DT)
+    d = th.layer_smooth(DT)
+    assert rel_err(N(ops.delta_E), d) < KTOL
+
+
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES)
+def test_omega(ny, nx, nz, seed):
+    print(f"omega {ny}x{nx}x{nz} seed={seed}", flush=True)
+    from oracle.oracle import SEC_PER_YR
+    C = enth_case(ny, nx, nz, seed)
+    th, ops = make_pair(C, nz)
+    rng = np.random.default_rng(seed + 7)
+    dh = 0.5 * rng.standard_normal((ny, nx))
+    bmb = 0.01 * rng.random((ny, nx))
+    # velocities in m/a, as in the coupled step
+    ops.enthalpy_velocity.u3d.mul_(SEC_PER_YR)
+    ops.enthalpy_velocity.v3d.mul_(SEC_PER_YR)
+    th.u3d = np.ascontiguousarray(N(ops.enthalpy_velocity.u3d))
+    th.v3d = np.ascontiguousarray(N(ops.enthalpy_velocity.v3d))
+    ops.compute_omega(T(dh), T(bmb))
+    om = th.compute_omega(dh, bmb)
+    assert rel_err(N(ops.enthalpy_velocity.omega), om) < KTOL
+    ops.compute_omega(T(dh))
+    om0 = th.compute_omega(dh)
+    assert rel_err(N(ops.enthalpy_velocity.omega), om0) < KTOL
+
+
+@pytest.mark.parametrize("alternating", [True, False])
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES)
+def test_column_sweep(ny, nx, nz, seed, alternating):
+    print(f"column_sweep {ny}x{nx}x{nz} seed={seed} alternating={alternating}", flush=True)
+    th, ops = make_pair(enth_case(ny, nx, nz, seed), nz)
+    ops.smoother_config.report_norms = True
+    ops.smoother_config.absolute_tolerance = 0.0   # the default 1e-2 stops every case after one sweep
+    th.atol = 0.0
+    ops.column_sweep(DT, 6, alternating=alternating)
+    rep = ops.last_report
+    it, r0, r = th.column_sweep(DT, 6, alternating=alternating)
+    assert rep.sweeps == it
+    assert abs(rep.r0 - r0) <= KTOL * r0
+    assert abs(rep.r - r) <= 1e-8 * r0
+    assert rel_err(N(ops.enthalpy_state.E), th.E) < STOL
+
+
+def test_column_sweep_stops_early_on_device():
+    """A loose tolerance stops after the first sweep; later launches are skipped
+    on the device and leave E untouched."""
+    print("column_sweep early stop", flush=True)
+    th, ops = make_pair(enth_case(10, 12, 21, 5), 21)
+    ops.smoother_config.relative_tolerance = 0.9
+    th.rtol = 0.9
+    ops.smoother_config.report_norms = True
+    ops.column_sweep(DT, 8)
+    it, _, _ = th.column_sweep(DT, 8)
+    assert ops.last_report.sweeps == it < 8
+    assert rel_err(N(ops.enthalpy_state.E), th.E) < STOL
+
+
+def test_column_sweep_host_loop_with_hook_matches_fused():
+    print("column_sweep hook path", flush=True)
+    C = enth_case(8, 9, 15, 6)
+    _, ops_a = make_pair(C, 15)
+    th, ops_b = make_pair(C, 15)
+    calls = []
+    ops_b.smoother_config.hook_func = calls.append
+    ops_a.column_sweep(DT, 4)
+    ops_b.column_sweep(DT, 4)
+    th.column_sweep(DT, 4)
+    assert calls == list(range(len(calls))) and len(calls) >= 1
+    assert rel_err(N(ops_a.enthalpy_state.E), th.E) < STOL
+    assert rel_err(N(ops_b.enthalpy_state.E), th.E) < STOL
+
+
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES)
+def test_diagnostics_and_rate_factor(ny, nx, nz, seed):
+    print(f"diagnostics {ny}x{nx}x{nz} seed={seed}", flush=True)
+    th, ops = make_pair(enth_case(ny, nx, nz, seed), nz)
+    assert rel_err(N(ops.get_temperature()), th.get_temperature()) < KTOL
+    w_orc = th.get_water_content()
+    assert np.max(np.abs(N(ops.get_water_content()) - w_orc)) < KTOL * max(np.max(w_orc), 1e-3)
+    assert rel_err(N(ops.get_arrhenius_factor()), th.get_arrhenius_factor(3.0)) < KTOL
+
+
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES[:3])
+def test_initialisation_from_temperature(ny, nx, nz, seed):
+    print(f"initialise {ny}x{nx}x{nz} seed={seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    C = enth_case(ny, nx, nz, seed)
+    th, ops = make_pair(C, nz)
+    T3 = 230.0 + 45.0 * rng.random((ny, nx, nz))   # partly above T_pmp: capped
+    Ts = 240.0 + 40.0 * rng.random((ny, nx))       # partly above T_MELT: capped
+    ops.initialize_from_temperature(T(T3))
+    th.initialize_from_temperature(T3)
+    assert rel_err(N(ops.enthalpy_state.E), th.E) < KTOL
+    assert rel_err(N(ops.enthalpy_state.E_prev), th.E) < KTOL
+    ops.initialize_from_temperature(T(Ts))         # 2-D field broadcast over the levels
+    th.initialize_from_temperature(Ts)
+    assert rel_err(N(ops.enthalpy_state.E), th.E) < KTOL
+    ops.set_surface_enthalpy_from_temperature(T(Ts))
+    th.set_surface_enthalpy_from_temperature(Ts)
+    assert rel_err(N(ops.enthalpy_forcing.E_surface), th.E_surface) < KTOL
+
+
+def _thermal_pair(n, nz, seed, frictional=True):
+    """ThermalModel on a dome state after a momentum step, and the oracle."""
+    from glide_amd.grid import Grid
+    from glide_amd.model import ThermalModel
+    from oracle.oracle import OracleThermal
+    from tests.util import dome_fields
+    F = dome_fields(n=n, seed=seed)
+    ny, nx = F["H"].shape
+    g = Grid(ny, nx, DX, device="cuda")
+    g.state.H.set(F["H_prev"])
+    g.state.u.set(F["u"])
+    g.state.v.set(F["v"])
+    g.sliding.beta.set(F["beta"])
+    tm = ThermalModel(g, nz=nz, n_smooth=10, frictional_heating=frictional)
+    rng = np.random.default_rng(seed + 3)
+    Ts = 243.15 + 6.5e-3 * np.maximum(2000.0 - (F["bed"] + F["H"]), 0.0) * 0.2
+    Qg = 0.05 + 0.01 * rng.random((ny, nx))
+    tm.initialize(T(Ts), Q_geo=T(Qg))
+    th = OracleThermal(ny, nx, nz, DX, F["H_prev"])
+    th.initialize_from_temperature(Ts)
+    th.set_surface_enthalpy_from_temperature(Ts)
+    th.Q_geo = Qg.copy()
+    # momentum step realised: H_prev snapshot, new H
+    tm.pre_momentum()
+    g.state.H.set(F["H"])
+    return F, g, tm, th
+
+
+@pytest.mark.parametrize("n,nz,seed,frictional", [(16, 11, 0, True), (24, 41, 1, True), (20, 21, 2, False)])
+def test_thermal_step(n, nz, seed, frictional):
+    print(f"thermal_step n={n} nz={nz} seed={seed} frictional={frictional}", flush=True)
+    from oracle.oracle import thermal_step
+    F, g, tm, th = _thermal_pair(n, nz, seed, frictional)
+    rep = tm.step(DT, report=True)
+    B_orc = thermal_step(th, F["u"], F["v"], F["H"], F["beta"], DT, n_smooth=10, frictional_heating=frictional)
+    ops = tm.ops
+    assert rel_err(N(ops.enthalpy_velocity.omega), th.omega) < KTOL
+    assert rel_err(N(ops.enthalpy_velocity.u3d), th.u3d) < KTOL
+    if frictional:
+        assert rel_err(N(ops.enthalpy_forcing.Q_fh), th.Q_fh) < KTOL
+    assert rel_err(N(ops.f_E), th.f_E) < KTOL
+    assert rel_err(N(ops.enthalpy_state.E), th.E) < STOL
+    assert rel_err(N(g.rheology.B.data), B_orc) < STOL
+    assert rep.sweeps >= 1
+
+
+def test_thermal_step_host_path_matches_fused():
+    print("thermal_step host path", flush=True)
+    F, g, tm, th = _thermal_pair(16, 11, 4)
+    F2, g2, tm2, _ = _thermal_pair(16, 11, 4)
+    tm2.ops.smoother_config.hook_func = lambda i: None
+    tm.step(DT)
+    tm2.step(DT)
+    assert rel_err(N(tm2.ops.enthalpy_state.E), N(tm.ops.enthalpy_state.E)) < STOL
+    assert rel_err(N(g2.rheology.B.data), N(g.rheology.B.data)) < STOL
