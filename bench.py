@@ -2,10 +2,15 @@
 """Benchmark of the coupled SSA step on one MI355X.
 
 Workload: the dome on a Gaussian-random-field bed at 512 x 512 cells,
-dx = 1500 m, 5 multigrid levels, dt = 10 a (glide_amd.synthetic).  One step is
-a forward FAS-CD solve of exactly 3 V-cycles (advancing the geometry), the
-adjoint FAS solve of exactly 3 V-cycles for the loss 0.5 * |u|^2 + 0.5 * |v|^2,
-and the four parameter gradients.  Inputs live in HBM before timing starts.
+dx = 1500 m, 5 multigrid levels, dt = 10 a (glide_amd.synthetic), coupled to
+the enthalpy solve on its 512 x 512 columns with 41 sigma levels (surface
+temperature -30 C + 6.5e-3 K/m (2000 m - s), geothermal flux 0.05 W/m^2,
+strain heating from the SSA velocities, frictional heating, rate factor fed
+back to the momentum solver).  One step is a forward FAS-CD solve of exactly
+3 V-cycles (advancing the geometry), the thermal step (column Newton sweeps
+with the device-side stopping test), the adjoint FAS solve of exactly 3
+V-cycles for the loss 0.5 * |u|^2 + 0.5 * |v|^2, and the four parameter
+gradients.  Inputs live in HBM before timing starts.
 
 Prints one JSON line.  Besides the contract fields it carries
   roofline     - the dominant kernel's algorithmic bytes / measured duration
@@ -34,7 +39,11 @@ WORD = 8  # bytes per FP64 value
 #   transpose operator: 15 inputs + 3 outputs
 #   adjoint smoothing pass: 12 inputs + 3 outputs
 #   residual: 12 inputs + 3 outputs
-ALG_FIELDS = {"vanka_sweep": 16, "vjp": 18, "adjoint_sweep": 15, "residual": 15}
+#   enthalpy column / layer smoother and residual, per column node: E, E_prev,
+#   f_E, omega, u3d, v3d in + one 3-D field out
+ALG_FIELDS = {"vanka_sweep": 16, "vjp": 18, "adjoint_sweep": 15, "residual": 15,
+              "enth_column": 7, "enth_layer": 7, "enth_residual": 7}
+SEC_PER_YR = 365.25 * 86400.0
 
 
 def parse():
@@ -47,6 +56,8 @@ def parse():
     ap.add_argument("--vcycles", type=int, default=3)
     ap.add_argument("--dt", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--nz", type=int, default=41, help="sigma levels of the enthalpy columns (0: no thermal step)")
+    ap.add_argument("--n-smooth", type=int, default=10, help="maximum enthalpy sweeps per step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=128,
                     help="grid side of the oracle sample (same level count and cycle counts)")
@@ -54,19 +65,28 @@ def parse():
 
 
 def build_model(args, torch):
-    from glide_amd import IceDynamics
-    from glide_amd.synthetic import dome_problem, install
+    from glide_amd import IceDynamics, ThermalModel
+    from glide_amd.synthetic import dome_problem, install, thermal_forcing
     p = dome_problem(args.n, 1500.0, seed=args.seed)
     model = IceDynamics(ny=args.n, nx=args.n, dx=1500.0, n_levels=args.levels, device="cuda")
     install(model.mg, p)
     for s in (model.forward_solver, model.adjoint_solver):
         s.fas_options.set(maximum_vcycles=args.vcycles, relative_tolerance=0.0, absolute_tolerance=0.0,
                           report_norms=False)
-    return model
+    thermal = None
+    if args.nz > 0:
+        thermal = ThermalModel(model.mg[0], nz=args.nz, n_smooth=args.n_smooth, strain_heating=True)
+        Ts, qg = thermal_forcing(p)
+        thermal.initialize(torch.as_tensor(Ts, device="cuda"), Q_geo=torch.as_tensor(qg, device="cuda"))
+    return model, thermal
 
 
-def one_step(model, t, dt):
+def one_step(model, thermal, t, dt):
+    if thermal is not None:
+        thermal.pre_momentum()
     model.forward(t, dt, update_geometry=True)
+    if thermal is not None:
+        thermal.step(dt * SEC_PER_YR)
     g = model.mg[0]
     model.backward(t, dt, dJdu=g.state.u.data, dJdv=g.state.v.data, dJdH=None)
 
@@ -96,13 +116,29 @@ def cpu_baseline(args):
     om.fas_adjoint_solve(args.dt, cfg)
     om.gradients(args.dt)
     t_adj = time.perf_counter() - t0
+    t_th = 0.0
+    if args.nz > 0:
+        from glide_amd.synthetic import thermal_forcing
+        Ts, qg = thermal_forcing(p)
+        th = O.OracleThermal(n, n, args.nz, 1500.0, p.H)
+        th.initialize_from_temperature(Ts)
+        th.set_surface_enthalpy_from_temperature(Ts)
+        th.Q_geo = qg
+        B = om.get("B")
+        u, v = om.get("u"), om.get("v")
+        phi = np.repeat(O.strain_heating(u, v, B, 3.0, 1e-6, 1500.0)[:, :, None], args.nz, axis=2)
+        t0 = time.perf_counter()
+        O.thermal_step(th, u, v, om.get("H"), p.beta, args.dt * SEC_PER_YR, n_smooth=args.n_smooth,
+                       strain_heating=phi)
+        t_th = time.perf_counter() - t0
     scale = (args.n / n) ** 2
-    step_s = args.vcycles * (t_fwd + t_adj) * scale
+    step_s = (args.vcycles * (t_fwd + t_adj) + t_th) * scale
     return {"value": 1.0 / step_s, "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
             "sample": (f"FP64 oracle, 1 forward + 1 adjoint V-cycle of the dome workload at {n}^2 "
-                       f"({args.levels} levels): {t_fwd:.2f} s + {t_adj:.2f} s; x{args.vcycles} V-cycles per step, "
+                       f"({args.levels} levels): {t_fwd:.2f} s + {t_adj:.2f} s, x{args.vcycles} V-cycles per step; "
+                       f"one thermal step on {n}^2 x {args.nz} columns: {t_th:.2f} s; "
                        f"x{scale:g} cell ratio to {args.n}^2"),
-            "sample_seconds": round(t_fwd + t_adj, 3)}
+            "sample_seconds": round(t_fwd + t_adj + t_th, 3)}
 
 
 def main():
@@ -117,22 +153,23 @@ def main():
         print("bench.py needs a ROCm device", file=sys.stderr)
         sys.exit(1)
     _lib.load()
-    model = build_model(args, torch)
+    model, thermal = build_model(args, torch)
     t = 0.0
     for _ in range(args.warmup):
-        one_step(model, t, args.dt)
+        one_step(model, thermal, t, args.dt)
         t += args.dt
     torch.cuda.synchronize()
     _lib.timing_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        one_step(model, t, args.dt)
+        one_step(model, thermal, t, args.dt)
         t += args.dt
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     kern = {}
-    for name in ("vanka_cell", "vanka_apply", "adjoint_cell", "adjoint_apply", "vjp", "residual"):
+    for name in ("vanka_cell", "vanka_apply", "adjoint_cell", "adjoint_apply", "vjp", "residual", "enth_column",
+                 "enth_layer", "enth_residual"):
         kern[name] = _lib.timing_read(name)
     _lib.timing_enable(False)
 
@@ -159,6 +196,9 @@ def main():
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "cells": v[2]} for k, v in kern.items()}
     for k, (c, ms, cells) in passes.items():
         kernels[k] = {"launches": c, "total_ms": round(ms, 3), "alg_GBps": round(bw(ALG_FIELDS[k], ms, cells), 2)}
+    for k in ("enth_column", "enth_layer", "enth_residual"):
+        c, ms, units = kern[k]
+        kernels[k]["alg_GBps_incl_skipped"] = round(bw(ALG_FIELDS[k], ms, units), 2)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -171,7 +211,11 @@ def main():
             "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": (f"dome on GRF bed, {args.n}x{args.n} MAC grid, dx=1500 m, {args.levels} levels, "
-                                    f"{args.vcycles}+{args.vcycles} V-cycles forward+adjoint per step, dt={args.dt} a"),
+                                    f"{args.vcycles}+{args.vcycles} V-cycles forward+adjoint per step, dt={args.dt} a"
+                                    + (f", coupled enthalpy on {args.n}x{args.n}x{args.nz} columns "
+                                       f"(<= {args.n_smooth} sweeps, strain + frictional heating)"
+                                       if args.nz > 0 else "")),
+                       "sigma_levels": args.nz,
                        "precision": "plain FP64 form", "grid": f"{args.n}x{args.n}", "levels": args.levels,
                        "vcycles": args.vcycles, "dt_years": args.dt, "parallelism": "single GPU"},
             "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels,

@@ -92,9 +92,9 @@ class EnthReport(ctypes.Structure):
 
 class ThermalConfigC(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("m", ctypes.c_double), ("u_reg", ctypes.c_double),
-                ("rho_i", ctypes.c_double), ("n_glen", ctypes.c_double), ("n_smooth", ctypes.c_int32),
-                ("frictional_heating", ctypes.c_int32), ("update_rheology", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("rho_i", ctypes.c_double), ("n_glen", ctypes.c_double), ("eps_reg", ctypes.c_double),
+                ("n_smooth", ctypes.c_int32), ("frictional_heating", ctypes.c_int32),
+                ("update_rheology", ctypes.c_int32), ("strain_heating", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p
@@ -141,6 +141,7 @@ SIGNATURES = {
     "glide_enth_omega": (_I, [_CP, _P, _P, _D, _P]),
     "glide_enth_broadcast_velocity": (_I, [_CP, _P, _P, _D, _P]),
     "glide_enth_frictional_heating": (_I, [_CP, _P, _D, _D, _P]),
+    "glide_enth_strain_heating": (_I, [_CP, _P, _P, _P, _D, _D, _D, _P]),
     "glide_enth_set_rhs": (_I, [_CP, _P]),
     "glide_enth_arrhenius": (_I, [_CP, _D, _D, _P, _P]),
     "glide_enth_diagnostics": (_I, [_CP, _P, _P, _P]),
@@ -150,7 +151,8 @@ SIGNATURES = {
                                 ctypes.POINTER(EnthSweepConfigC), ctypes.POINTER(EnthReport), _P]),
 }
 
-TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3, "residual": 4, "adjoint_apply": 5}
+TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3, "residual": 4, "adjoint_apply": 5,
+                 "enth_column": 6, "enth_layer": 7, "enth_residual": 8}
 
 
 def timing_enable(on: bool) -> None:

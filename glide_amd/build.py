@@ -17,7 +17,7 @@ LIB = PKG / "libglide_hip.so"
 ARCH = os.environ.get("GLIDE_OFFLOAD_ARCH", "gfx950")
 
 DEVICE_SOURCES = ["ssa_kernels.hip", "transfer_kernels.hip", "enthalpy_kernels.hip"]
-HOST_SOURCES = ["solver.cpp", "enthalpy_abi.cpp"]
+HOST_SOURCES = ["solver.cpp", "enthalpy_abi.cpp", "timing.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

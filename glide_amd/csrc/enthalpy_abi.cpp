@@ -274,6 +274,20 @@ int glide_enth_frictional_heating(const glide_enth_column* C, const void* beta, 
     return GLIDE_OK;
 }
 
+int glide_enth_strain_heating(const glide_enth_column* C, const void* u2d, const void* v2d, const void* B,
+                              double n_glen, double eps_reg, double rho_i, void* stream) {
+    GLIDE_TRY(check_column(C));
+    REQ(C->phi_strain);
+    REQ(u2d);
+    REQ(v2d);
+    REQ(B);
+    if (!(n_glen > 0.0) || !(eps_reg >= 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "n_glen > 0 and eps_reg >= 0 required");
+    HIP_TRY(launch_enth_strain_heating<R>(CP(u2d), CP(v2d), CP(B), C->ny, C->nx, C->nz, 1.0 / C->dx, n_glen, eps_reg,
+                                          rho_i * kGravity, P(C->phi_strain), S(stream)),
+            "strain heating");
+    return GLIDE_OK;
+}
+
 int glide_enth_set_rhs(const glide_enth_column* C, void* stream) {
     EnthArgs<R> A;
     GLIDE_TRY(make_args(C, 1.0, false, &A));
@@ -321,7 +335,7 @@ int glide_enth_surface_from_temperature(const glide_enth_column* C, const void* 
     return GLIDE_OK;
 }
 
-int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta, void* B_out,
+int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta, void* B,
                        const glide_thermal_config* tc, const glide_enth_sweep_config* sc, glide_enth_report* report,
                        void* stream) {
     if (!tc || !sc) return fail(GLIDE_ERR_INVALID_ARG, "thermal or sweep config is NULL");
@@ -334,8 +348,8 @@ int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* 
     REQ(C->Q_geo);
     REQ(C->Q_fh);
     if (tc->frictional_heating) REQ(beta);
-    if (tc->update_rheology) {
-        REQ(B_out);
+    if (tc->update_rheology || tc->strain_heating) {
+        REQ(B);
         if (!(tc->n_glen > 0.0) || !(tc->rho_i > 0.0))
             return fail(GLIDE_ERR_INVALID_ARG, "n_glen and rho_i must be positive");
     }
@@ -348,6 +362,8 @@ int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* 
             "thickness tendency");
     HIP_TRY(launch_enth_omega<R>(A, CP(C->dh_dt), nullptr, kSecPerYear, P(C->omega), st), "enthalpy omega");
     GLIDE_TRY(glide_enth_broadcast_velocity(C, u2d, v2d, kSecPerYear, stream));
+    if (tc->strain_heating)
+        GLIDE_TRY(glide_enth_strain_heating(C, u2d, v2d, B, tc->n_glen, tc->eps_reg, tc->rho_i, stream));
     // glide/model.py:199-203, glide/enthalpy.py:473-519
     HIP_TRY(launch_enth_rhs<R>(A, CP(beta), tc->m, tc->u_reg, tc->frictional_heating ? 1 : 0, P(C->Q_fh), P(C->f_E),
                                st),
@@ -356,7 +372,7 @@ int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* 
     // glide/model.py:205-213
     if (tc->update_rheology) {
         const double scale = tc->rho_i * kGravity * std::pow(kSecPerYear, 1.0 / tc->n_glen);
-        HIP_TRY(launch_enth_arrhenius<R>(A, tc->n_glen, scale, P(B_out), st), "rate factor");
+        HIP_TRY(launch_enth_arrhenius<R>(A, tc->n_glen, scale, P(B), st), "rate factor");
     }
     return GLIDE_OK;
 }

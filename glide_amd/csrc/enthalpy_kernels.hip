@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "enthalpy_kernels.hpp"
+#include "ssa_device.hpp"
 
 namespace glide {
 
@@ -508,6 +509,24 @@ __global__ __launch_bounds__(256) void k_enth_rhs(EnthArgs<T> A, const T* __rest
     f_E[base + k] = f;
 }
 
+// SSA strain heating, uniform over the column (no counterpart in the
+// reference, which leaves phi_strain to the caller): 4 eta eps_e^2 rho_i g /
+// SEC_PER_YR in W/m^3, with eps_e^2 the momentum solver's cell strain-rate
+// invariant of the velocities (m/a) and eta = B/2 (eps_e^2 + eps_reg)^p its
+// Glen viscosity (B in head units).
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_strain_heating(const T* __restrict__ u, const T* __restrict__ v,
+                                                             const T* __restrict__ B, int ny, int nx, int nz, T dxi,
+                                                             T glen_p, T eps_reg, T rho_g, T* __restrict__ phi) {
+    const long ncol = (long)ny * nx;
+    const ColIdx q = col_index(wave_column(), nx, ncol);
+    if (!q.valid || q.k >= nz) return;
+    const CellVel<T> c = load_cell_vel(u, v, q.i, q.j, ny, nx);
+    const T e2 = strain_invariant(c, q.i, q.j, ny, nx, dxi, T(0));
+    const T eta = T(0.5) * B[q.col] * pow(e2 + eps_reg, glen_p);
+    phi[q.col * nz + q.k] = T(4) * eta * e2 * rho_g / T(kSecPerYr);
+}
+
 // Paterson-Budd rate factor A(T) (1 + 181.25 w) averaged over the levels and
 // returned as B = <A>^(-1/n) / scale (glide/enthalpy.py:600-643,
 // glide/model.py:209-213).
@@ -583,18 +602,23 @@ hipError_t launch_enth_residual(const EnthArgs<T>& A, T* r_out, int use_forcing,
                                 EnthStatus* status, int status_mode, double rtol, double atol, hipStream_t st) {
     unsigned g = col_blocks(A.ny, A.nx);
     if (g > (unsigned)kEnthResBlocks) g = kEnthResBlocks;
-    k_enth_residual<T><<<g, 256, 0, st>>>(A, r_out, use_forcing, partials);
+    {
+        TimedLaunch tl(kTimeEnthResidual, (long)A.ny * A.nx * A.nz, st);
+        k_enth_residual<T><<<g, 256, 0, st>>>(A, r_out, use_forcing, partials);
+    }
     if (max_out || status) k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)g, max_out, status, status_mode, rtol, atol);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_enth_column_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, int n_newton,
                                      T relaxation, hipStream_t st) {
+    TimedLaunch tl(kTimeEnthColumn, (long)A.ny * A.nx * A.nz, st);
     k_enth_column_smooth<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, delta, E_out, omega_relax, n_newton, relaxation);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_enth_layer_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, hipStream_t st) {
+    TimedLaunch tl(kTimeEnthLayer, (long)A.ny * A.nx * A.nz, st);
     k_enth_layer_smooth<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, delta, E_out, omega_relax);
     return hipGetLastError();
 }
@@ -624,6 +648,13 @@ template <typename T>
 hipError_t launch_enth_rhs(const EnthArgs<T>& A, const T* beta, T m, T u_reg, int frictional, T* Q_fh, T* f_E,
                            hipStream_t st) {
     k_enth_rhs<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, beta, m, u_reg, frictional, Q_fh, f_E);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_strain_heating(const T* u, const T* v, const T* B, int ny, int nx, int nz, T dxi, T n_glen,
+                                     T eps_reg, T rho_g, T* phi, hipStream_t st) {
+    const T glen_p = (T(1) - n_glen) / (T(2) * n_glen);
+    k_enth_strain_heating<T><<<col_blocks(ny, nx), 256, 0, st>>>(u, v, B, ny, nx, nz, dxi, glen_p, eps_reg, rho_g, phi);
     return hipGetLastError();
 }
 template <typename T>
@@ -658,6 +689,8 @@ hipError_t launch_enth_surface_from_temperature(const T* Ts, T* Es, long n, hipS
     template hipError_t launch_enth_tendency<T>(const T*, const T*, T, T*, long, hipStream_t);                       \
     template hipError_t launch_enth_rhs<T>(const EnthArgs<T>&, const T*, T, T, int, T*, T*, hipStream_t);            \
     template hipError_t launch_enth_arrhenius<T>(const EnthArgs<T>&, T, T, T*, hipStream_t);                         \
+    template hipError_t launch_enth_strain_heating<T>(const T*, const T*, const T*, int, int, int, T, T, T, T, T*,   \
+                                                      hipStream_t);                                                  \
     template hipError_t launch_enth_diagnostics<T>(const EnthArgs<T>&, T*, T*, hipStream_t);                         \
     template hipError_t launch_enth_from_temperature<T>(const EnthArgs<T>&, const T*, T*, hipStream_t);              \
     template hipError_t launch_enth_surface_from_temperature<T>(const T*, T*, long, hipStream_t);

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "timing.hpp"
+
 namespace glide {
 
 // Everything a column kernel reads.  Layouts: E, E_prev, f_E, omega,
@@ -59,6 +61,10 @@ hipError_t launch_enth_rhs(const EnthArgs<T>& A, const T* beta, T m, T u_reg, in
 // B = mean_k(A_k (1 + 181.25 w_k))^(-1/n) / scale.
 template <typename T>
 hipError_t launch_enth_arrhenius(const EnthArgs<T>& A, T n_glen, T scale, T* B, hipStream_t st);
+// SSA strain heating 4 eta eps_e^2 rho_g / SEC_PER_YR written to every level.
+template <typename T>
+hipError_t launch_enth_strain_heating(const T* u, const T* v, const T* B, int ny, int nx, int nz, T dxi, T n_glen,
+                                     T eps_reg, T rho_g, T* phi, hipStream_t st);
 // Temperature (clipped at the pressure melting point) and water content.
 template <typename T>
 hipError_t launch_enth_diagnostics(const EnthArgs<T>& A, T* temperature, T* water, hipStream_t st);

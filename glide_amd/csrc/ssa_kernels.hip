@@ -668,80 +668,6 @@ __global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __
     if (gbed) gbed[c] = gz;
 }
 
-// ------------------------------------------------------------------ launch timing
-// Optional HIP-event pairs around the hot launches (enabled by the host for
-// measurement); each record keeps the level's cell count so that per-launch
-// algorithmic bytes can be computed.
-struct TimingRec {
-    hipEvent_t start, stop;
-    long cells;
-};
-static bool g_timing_on = false;
-static std::vector<TimingRec> g_timing[kTimedKernels];
-static std::vector<hipEvent_t> g_event_pool;
-
-static hipEvent_t pooled_event() {
-    if (!g_event_pool.empty()) {
-        hipEvent_t e = g_event_pool.back();
-        g_event_pool.pop_back();
-        return e;
-    }
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
-}
-
-struct TimedLaunch {
-    int id;
-    long cells;
-    hipStream_t st;
-    hipEvent_t start = nullptr, stop = nullptr;
-    TimedLaunch(int id_, long cells_, hipStream_t st_) : id(id_), cells(cells_), st(st_) {
-        if (!g_timing_on) return;
-        start = pooled_event();
-        stop = pooled_event();
-        if (start) hipEventRecord(start, st);
-    }
-    ~TimedLaunch() {
-        if (!start || !stop) return;
-        hipEventRecord(stop, st);
-        g_timing[id].push_back({start, stop, cells});
-    }
-};
-
-void timing_enable(bool on) {
-    timing_clear();
-    g_timing_on = on;
-}
-
-void timing_clear() {
-    for (auto& v : g_timing) {
-        for (auto& r : v) {
-            g_event_pool.push_back(r.start);
-            g_event_pool.push_back(r.stop);
-        }
-        v.clear();
-    }
-}
-
-hipError_t timing_read(int id, long* count, double* total_ms, double* total_cells) {
-    *count = 0;
-    *total_ms = 0.0;
-    *total_cells = 0.0;
-    if (id < 0 || id >= kTimedKernels) return hipErrorInvalidValue;
-    for (auto& r : g_timing[id]) {
-        hipError_t e = hipEventSynchronize(r.stop);
-        if (e != hipSuccess) return e;
-        float ms = 0.f;
-        e = hipEventElapsedTime(&ms, r.start, r.stop);
-        if (e != hipSuccess) return e;
-        *total_ms += ms;
-        *total_cells += (double)r.cells;
-        ++*count;
-    }
-    return hipSuccess;
-}
-
 // ------------------------------------------------------------------ launchers
 static inline dim3 tile_grid(int ny, int nx, bool with_facets) {
     int gx = with_facets ? (nx + 1 + TX - 1) / TX : (nx + TX - 1) / TX;

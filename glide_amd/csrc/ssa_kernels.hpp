@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ssa_device.hpp"
+#include "timing.hpp"
 
 namespace glide {
 
@@ -43,13 +44,6 @@ hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH
 template <typename T>
 hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st);
 
-// Opt-in launch timing of the hot kernels (HIP events around each launch).
-enum TimedKernel {
-    kTimeVankaCell = 0, kTimeVankaApply, kTimeAdjointCell, kTimeVjp, kTimeResidual, kTimeAdjointApply, kTimedKernels
-};
-void timing_enable(bool on);
-void timing_clear();
-hipError_t timing_read(int id, long* count, double* total_ms, double* total_cells);
 
 // transfers and element-wise helpers (transfer_kernels.hip)
 template <typename T>

@@ -1,0 +1,36 @@
+// Opt-in launch timing of the hot kernels: HIP event pairs recorded on the
+// launch stream around each launch while enabled; each record keeps the
+// launch's unit count (cells, or column nodes) so that per-launch
+// algorithmic bytes can be computed by the caller.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace glide {
+
+enum TimedKernel {
+    kTimeVankaCell = 0,
+    kTimeVankaApply,
+    kTimeAdjointCell,
+    kTimeVjp,
+    kTimeResidual,
+    kTimeAdjointApply,
+    kTimeEnthColumn,
+    kTimeEnthLayer,
+    kTimeEnthResidual,
+    kTimedKernels
+};
+
+void timing_enable(bool on);
+void timing_clear();
+hipError_t timing_read(int id, long* count, double* total_ms, double* total_units);
+
+struct TimedLaunch {
+    int id;
+    long units;
+    hipStream_t st;
+    hipEvent_t start = nullptr, stop = nullptr;
+    TimedLaunch(int id, long units, hipStream_t st);
+    ~TimedLaunch();
+};
+
+}  // namespace glide

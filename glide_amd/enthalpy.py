@@ -252,6 +252,18 @@ class EnthalpyOperators:
         call("glide_enth_frictional_heating", ctypes.byref(C), ptr(self.grid.sliding.beta.data), float(m),
              float(u_reg), self._stream())
 
+    def compute_strain_heating(self, rho_i=917.0):
+        """phi_strain = 4 eta eps_e^2 rho_i g / a on every level from the grid's
+        depth-averaged velocity and rate factor B (head units), eta and eps_e^2
+        being the momentum solver's cell viscosity and strain-rate invariant.
+        The reference leaves phi_strain to the caller; this supplies the SSA
+        dissipation for the coupled step."""
+        g = self.grid
+        C = self._column()
+        call("glide_enth_strain_heating", ctypes.byref(C), ptr(g.state.u.data), ptr(g.state.v.data),
+             ptr(g.rheology.B.data), float(g.rheology.n.value), float(g.rheology.eps_reg.value), float(rho_i),
+             self._stream())
+
     def set_surface_enthalpy_from_temperature(self, T_surface):
         """glide/enthalpy.py:521-532."""
         Ts = _as_field(T_surface, (self.grid.ny, self.grid.nx), self.grid.device)

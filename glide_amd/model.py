@@ -85,13 +85,17 @@ class ThermalModel:
 
     SEC_PER_YR = 365.25 * 86400.0
 
-    def __init__(self, grid, nz=21, n_smooth=10, update_rheology=True, frictional_heating=True, rho_i=917.0):
+    def __init__(self, grid, nz=21, n_smooth=10, update_rheology=True, frictional_heating=True, rho_i=917.0,
+                 strain_heating=False):
         from .enthalpy import EnthalpyOperators
         self.ops = EnthalpyOperators(grid, nz=nz)
         self.n_smooth = n_smooth
         self.update_rheology = update_rheology
         self.frictional_heating = frictional_heating
         self.rho_i = rho_i
+        # SSA strain heating from the momentum solution before each sweep
+        # (an addition: the reference's phi_strain is caller-supplied)
+        self.strain_heating = strain_heating
         self.g = 9.81
         self.last_report = None
 
@@ -124,14 +128,16 @@ class ThermalModel:
         grid = ops.grid
         n = float(grid.rheology.n.value)
         tc = _lib.ThermalConfigC(dt=float(dt), m=float(grid.sliding.m.value), u_reg=float(grid.sliding.u_reg.value),
-                                 rho_i=float(self.rho_i), n_glen=n, n_smooth=int(self.n_smooth),
-                                 frictional_heating=1 if self.frictional_heating else 0,
-                                 update_rheology=1 if self.update_rheology else 0)
+                                 rho_i=float(self.rho_i), n_glen=n, eps_reg=float(grid.rheology.eps_reg.value),
+                                 n_smooth=int(self.n_smooth), frictional_heating=1 if self.frictional_heating else 0,
+                                 update_rheology=1 if self.update_rheology else 0,
+                                 strain_heating=1 if self.strain_heating else 0)
         want = report or ops.smoother_config.report_norms
         rep = _lib.EnthReport() if want else None
         C = ops._column()
         call("glide_thermal_step", ctypes.byref(C), ptr(grid.state.u.data), ptr(grid.state.v.data),
-             ptr(grid.sliding.beta.data), ptr(grid.rheology.B.data) if self.update_rheology else None,
+             ptr(grid.sliding.beta.data),
+             ptr(grid.rheology.B.data) if (self.update_rheology or self.strain_heating) else None,
              ctypes.byref(tc), ctypes.byref(ops.smoother_config.to_c(True)),
              ctypes.byref(rep) if rep is not None else None, ops._stream())
         self.last_report = rep
@@ -145,6 +151,8 @@ class ThermalModel:
         ops.enthalpy_velocity.v3d /= self.SEC_PER_YR
         if self.frictional_heating:
             self._compute_frictional_heating()
+        if self.strain_heating:
+            ops.compute_strain_heating(self.rho_i)
         ops.set_rhs(snapshot=False)
         ops.column_sweep(dt, self.n_smooth)
         if self.update_rheology:

@@ -87,3 +87,13 @@ def install(mg, prob: DomeProblem, level: int = 0) -> None:
     mg.state.u.set(0.0, start_level=level)
     mg.state.v.set(0.0, start_level=level)
     mg.state.mask.set(0.0, start_level=level)
+
+
+def thermal_forcing(prob: DomeProblem, lapse: float = 6.5e-3, T_ref_surface: float = 273.15 - 30.0,
+                    z_ref: float = 2000.0, q_geo: float = 0.05):
+    """Surface temperature T_s = -30 C + 6.5e-3 K/m (2000 m - s) on the
+    flotation surface s (K) and a uniform geothermal flux (W/m^2) of the
+    coupled enthalpy workload."""
+    s = flotation_surface(prob.bed, prob.H)
+    Ts = T_ref_surface + lapse * (z_ref - s)
+    return Ts, np.full_like(Ts, q_geo)

@@ -37,6 +37,8 @@
  *   glide_enth_surface_from_temperature
  *                                <- EnthalpyOperators.set_surface_enthalpy_from_temperature
  *                                   (glide/enthalpy.py:521-532)
+ *   glide_enth_strain_heating    <- no counterpart (the reference's phi_strain is caller-supplied,
+ *                                   glide/enthalpy.py:108-131); SSA dissipation for the coupled step
  *   glide_thermal_step           <- ThermalModel.step (glide/model.py:167-213)
  */
 #ifndef GLIDE_ENTHALPY_H
@@ -82,16 +84,18 @@ typedef struct glide_enth_report {
     double norms[GLIDE_ENTH_MAX_SWEEPS];
 } glide_enth_report;
 
-/* ThermalModel settings (glide/model.py:97-132). */
+/* ThermalModel settings (glide/model.py:97-132); strain_heating and eps_reg
+ * have no counterpart in the reference (see glide_enth_strain_heating). */
 typedef struct glide_thermal_config {
-    double dt;      /* s */
+    double dt;       /* s */
     double m, u_reg; /* sliding exponent and regularisation for Q_fh */
-    double rho_i;   /* ice density of the B unit conversion */
+    double rho_i;    /* ice density of the B unit conversion and strain heating */
     double n_glen;
+    double eps_reg;  /* strain-rate regularisation of the momentum solver's viscosity (a^-2) */
     int32_t n_smooth;
     int32_t frictional_heating;
     int32_t update_rheology;
-    int32_t pad;
+    int32_t strain_heating; /* phi_strain from the SSA velocities and B before the sweep */
 } glide_thermal_config;
 
 /* r_E = F(E) - f_E (use_forcing != 0) or F(E); when max_abs is non-NULL the
@@ -119,6 +123,13 @@ int glide_enth_broadcast_velocity(const glide_enth_column* C, const void* u2d, c
 /* Q_fh = beta (|u_b|^2 + u_reg^2)^((m+1)/2) from the bed-level velocities. */
 int glide_enth_frictional_heating(const glide_enth_column* C, const void* beta, double m, double u_reg,
                                   void* stream);
+/* phi_strain[k] = 4 eta eps_e^2 rho_i g / a (W/m^3) on every level from the
+ * depth-averaged velocities u2d, v2d (m/a) and B (head units), with eps_e^2
+ * and eta = B/2 (eps_e^2 + eps_reg)^((1-n)/(2n)) the momentum solver's cell
+ * strain-rate invariant and viscosity.  No counterpart in the reference,
+ * which leaves phi_strain to the caller. */
+int glide_enth_strain_heating(const glide_enth_column* C, const void* u2d, const void* v2d, const void* B,
+                              double n_glen, double eps_reg, double rho_i, void* stream);
 /* f_E from phi_strain (NULL = 0), Q_geo and Q_fh. */
 int glide_enth_set_rhs(const glide_enth_column* C, void* stream);
 /* B = (mean over levels of A(T)(1 + 181.25 w))^(-1/n_glen) / scale. */
@@ -132,9 +143,10 @@ int glide_enth_surface_from_temperature(const glide_enth_column* C, const void* 
                                         void* stream);
 /* ThermalModel.step after pre_momentum: velocities u2d, v2d in m/a and H the
  * new thickness; fills u3d, v3d (m/s), dh_dt, omega, Q_fh (when
- * frictional_heating), f_E, runs the column sweep and, when
- * update_rheology, writes B (head units) into B_out. */
-int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta, void* B_out,
+ * frictional_heating), phi_strain (when strain_heating, from the B given),
+ * f_E, runs the column sweep and, when update_rheology, overwrites B with
+ * the new rate factor (head units). */
+int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta, void* B,
                        const glide_thermal_config* tc, const glide_enth_sweep_config* sc, glide_enth_report* report,
                        void* stream);
 

@@ -169,9 +169,11 @@ int glide_gradients(const glide_level* L, double dt, int which, void* stream);
 /* Measurement support (no counterpart in the reference): when enabled, HIP
  * event pairs are recorded around every launch of the hot kernels
  * (0 forward patch solve, 1 patch increment apply, 2 adjoint patch solve,
- * 3 transpose operator, 4 residual, 5 adjoint increment apply).  glide_timing_read synchronises those
- * events and returns the launch count, summed duration and summed cell count
- * of kernel `kernel` since the last enable. */
+ * 3 transpose operator, 4 residual, 5 adjoint increment apply, 6 enthalpy
+ * column smoother, 7 enthalpy layer smoother, 8 enthalpy residual).
+ * glide_timing_read synchronises those events and returns the launch count,
+ * summed duration and summed unit count (cells; column nodes for 6-8) of
+ * kernel `kernel` since the last enable. */
 int glide_timing_enable(int on);
 int glide_timing_read(int kernel, long* count, double* total_ms, double* total_cells);
 

@@ -125,6 +125,19 @@ def viscosity(u, v, B, n, eps_reg, dx, du=None, dv=None):
     return (eta, etad) if du is not None else eta
 
 
+def strain_heating(u, v, B, n, eps_reg, dx, rho_i=917.0, g=9.81):
+    """SSA strain heating 4 eta eps_e^2 rho_i g / SEC_PER_YR (W/m^3), (ny, nx).
+    Parity pinned by the closed forms of uniform stretching and simple shear
+    (tests/test_oracle_enthalpy.py)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    ny, nx = B.shape
+    phi = np.zeros((ny, nx))
+    lib().orc_strain_heating(_p(phi), _p(u), _p(v), _p(B), _D(n), _D(eps_reg), _D(dx), _D(rho_i * g), ny, nx)
+    return phi
+
+
 def vanka_dump(F, P: Phys, dx, dt):
     ny, nx = F["H"].shape
     J, r = np.zeros((ny * nx, 25)), np.zeros((ny * nx, 5))

@@ -123,6 +123,26 @@ EtaD cell_viscosity(const Acc& A, const double* u, const double* v, const double
     return r;
 }
 
+// Strain-rate invariant of cell (i,j) without regularisation: the same
+// terms, in the same order, as cell_viscosity's eII.
+double strain_invariant0(const Acc& A, const double* u, const double* v, int i, int j, double dx) {
+    const double dxi = 1.0 / dx;
+    const int ny = A.ny, nx = A.nx;
+    double ul = A.U(u, i, j), ur = A.U(u, i, j + 1), vt = A.V(v, i, j), vb = A.V(v, i + 1, j);
+    double exx = (ur - ul) * dxi, eyy = (vt - vb) * dxi;
+    double s2 = 0;
+    double e = (i > 0 && j > 0) ? 0.5 * ((A.U(u, i - 1, j) - ul) * dxi + (vt - A.V(v, i, j - 1)) * dxi) : 0.0;
+    s2 += e * e;
+    e = (i > 0 && j < nx - 1) ? 0.5 * ((A.U(u, i - 1, j + 1) - ur) * dxi + (A.V(v, i, j + 1) - vt) * dxi) : 0.0;
+    s2 += e * e;
+    e = (i < ny - 1 && j > 0) ? 0.5 * ((ul - A.U(u, i + 1, j)) * dxi + (vb - A.V(v, i + 1, j - 1)) * dxi) : 0.0;
+    s2 += e * e;
+    e = (i < ny - 1 && j < nx - 1) ? 0.5 * ((ur - A.U(u, i + 1, j + 1)) * dxi + (A.V(v, i + 1, j + 1) - vb) * dxi)
+                                   : 0.0;
+    s2 += e * e;
+    return exx * exx + eyy * eyy + exx * eyy + 0.25 * s2;
+}
+
 // Normal stresses on cell ci/cj (glide/cuda/stress.cu:43-112).
 struct Normal { double res, d_ul, d_ur, d_vt, d_vb, d_etaH; };
 Normal sigma_xx(double ul, double ur, double vt, double vb, double etaH, double dxi, int cj, int nx) {
@@ -1265,6 +1285,23 @@ void orc_residual(double* r_u, double* r_v, double* r_H, const double* u, const 
                   int use_mask, const orc_phys* P, double dx, double dt, int ny, int nx) {
     Fields F{u, v, H, phi, mask, f_u, f_v, f_H, bed, B, beta, gamma};
     residual(F, r_u, r_v, r_H, use_forcing != 0, use_mask != 0, *P, dx, dt, ny, nx);
+}
+
+// SSA strain heating (no counterpart in the reference, which leaves
+// phi_strain to the caller): Phi = 4 eta eps_e^2 rho_i g / SEC_PER_YR in
+// W/m^3 from the depth-averaged velocities (m/a) and B (head units), with
+// eta = 0.5 B (eps_e^2 + eps_reg)^((1-n)/(2n)) the momentum solver's cell
+// viscosity and eps_e^2 its strain-rate invariant without regularisation.
+void orc_strain_heating(double* phi, const double* u, const double* v, const double* B, double n, double eps_reg,
+                        double dx, double rho_g, int ny, int nx) {
+    Acc A{ny, nx};
+    const double sec = 365.25 * 86400.0;
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+            double e2 = strain_invariant0(A, u, v, i, j, dx);
+            double eta = cell_viscosity(A, u, v, nullptr, nullptr, B, i, j, n, eps_reg, dx).v;
+            phi[(size_t)i * nx + j] = 4.0 * eta * e2 * rho_g / sec;
+        }
 }
 
 void orc_viscosity(double* eta, double* etad, const double* u, const double* v, const double* du, const double* dv,

@@ -47,7 +47,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.EnthColumn) == 16 + 32 + 8 * len(_lib.ENTH_POINTERS)
     assert ctypes.sizeof(_lib.EnthSweepConfigC) == 4 * 8 + 2 * 4
     assert ctypes.sizeof(_lib.EnthReport) == 8 + 16 + 8 * _lib.ENTH_MAX_SWEEPS
-    assert ctypes.sizeof(_lib.ThermalConfigC) == 5 * 8 + 4 * 4
+    assert ctypes.sizeof(_lib.ThermalConfigC) == 6 * 8 + 4 * 4
     assert f"GLIDE_ENTH_MAX_SWEEPS {_lib.ENTH_MAX_SWEEPS}" in eh
 
 

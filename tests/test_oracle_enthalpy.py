@@ -258,3 +258,33 @@ def test_arrhenius_factor_closed_form():
     B = th.get_arrhenius_factor(3.0)
     A = 3.985e-13 * np.exp(-60e3 / (8.314 * 250.0))
     assert np.allclose(B, A ** (-1.0 / 3.0), rtol=1e-12)
+
+
+def test_strain_heating_closed_forms(orc):
+    """SSA strain heating Phi = 4 eta eps_e^2 rho_i g / a with
+    eta = B/2 (eps_e^2 + eps_reg)^((1-n)/(2n)): uniform stretching u = a x gives
+    eps_e^2 = a^2 in every cell; simple shear u = c y gives eps_e^2 = c^2/4 in
+    interior cells (four unmasked corners) and Phi = 0 for rigid translation."""
+    ny, nx, dx, n, eps = 12, 10, 1000.0, 3.0, 1e-6
+    sec = 365.25 * 86400.0
+    rho_g = 917.0 * 9.81
+    B = np.full((ny, nx), 0.5) + 0.1 * np.arange(nx)[None, :]
+    a = 2e-3
+    x_u = dx * np.arange(nx + 1)
+    u = np.broadcast_to(a * x_u, (ny, nx + 1)).copy()
+    v = np.zeros((ny + 1, nx))
+    phi = orc.strain_heating(u, v, B, n, eps, dx)
+    p = (1 - n) / (2 * n)
+    expect = 4.0 * (0.5 * B * (a * a + eps) ** p) * a * a * rho_g / sec
+    np.testing.assert_allclose(phi, expect, rtol=1e-13)
+    c = 1.5e-3
+    y_u = -dx * np.arange(ny)             # rows run towards -y
+    u = np.broadcast_to((c * y_u)[:, None], (ny, nx + 1)).copy()
+    phi = orc.strain_heating(u, v, B, n, eps, dx)
+    e2 = c * c / 4.0
+    expect = 4.0 * (0.5 * B * (e2 + eps) ** p) * e2 * rho_g / sec
+    np.testing.assert_allclose(phi[1:-1, 1:-1], expect[1:-1, 1:-1], rtol=1e-12)
+    assert np.all(phi[0, 1:-1] < expect[0, 1:-1])   # boundary rows lose masked corners
+    u = np.full((ny, nx + 1), 37.0)
+    v = np.full((ny + 1, nx), -12.0)
+    assert np.max(np.abs(orc.strain_heating(u, v, B, n, eps, dx))) == 0.0

@@ -227,7 +227,28 @@ def test_initialisation_from_temperature(ny, nx, nz, seed):
     assert rel_err(N(ops.enthalpy_forcing.E_surface), th.E_surface) < KTOL
 
 
-def _thermal_pair(n, nz, seed, frictional=True):
+@pytest.mark.parametrize("n,nz,seed", [(16, 11, 0), (24, 41, 1), (13, 2, 2)])
+def test_strain_heating(n, nz, seed):
+    print(f"strain_heating n={n} nz={nz} seed={seed}", flush=True)
+    from glide_amd.enthalpy import EnthalpyOperators
+    from glide_amd.grid import Grid
+    from oracle.oracle import strain_heating
+    from tests.util import dome_fields
+    F = dome_fields(n=n, seed=seed)
+    g = Grid(n, n, DX, device="cuda")
+    g.state.H.set(F["H"])
+    g.state.u.set(F["u"])
+    g.state.v.set(F["v"])
+    g.rheology.B.set(F["B"])
+    ops = EnthalpyOperators(g, nz=nz)
+    ops.compute_strain_heating(917.0)
+    phi = strain_heating(F["u"], F["v"], F["B"], 3.0, 1e-6, DX, 917.0)
+    got = N(ops.enthalpy_forcing.phi_strain)
+    for k in range(nz):
+        assert rel_err(got[:, :, k], phi) < KTOL
+
+
+def _thermal_pair(n, nz, seed, frictional=True, strain=False):
     """ThermalModel on a dome state after a momentum step, and the oracle."""
     from glide_amd.grid import Grid
     from glide_amd.model import ThermalModel
@@ -240,7 +261,8 @@ def _thermal_pair(n, nz, seed, frictional=True):
     g.state.u.set(F["u"])
     g.state.v.set(F["v"])
     g.sliding.beta.set(F["beta"])
-    tm = ThermalModel(g, nz=nz, n_smooth=10, frictional_heating=frictional)
+    g.rheology.B.set(F["B"])
+    tm = ThermalModel(g, nz=nz, n_smooth=10, frictional_heating=frictional, strain_heating=strain)
     rng = np.random.default_rng(seed + 3)
     Ts = 243.15 + 6.5e-3 * np.maximum(2000.0 - (F["bed"] + F["H"]), 0.0) * 0.2
     Qg = 0.05 + 0.01 * rng.random((ny, nx))
@@ -255,13 +277,18 @@ def _thermal_pair(n, nz, seed, frictional=True):
     return F, g, tm, th
 
 
-@pytest.mark.parametrize("n,nz,seed,frictional", [(16, 11, 0, True), (24, 41, 1, True), (20, 21, 2, False)])
-def test_thermal_step(n, nz, seed, frictional):
-    print(f"thermal_step n={n} nz={nz} seed={seed} frictional={frictional}", flush=True)
-    from oracle.oracle import thermal_step
-    F, g, tm, th = _thermal_pair(n, nz, seed, frictional)
+@pytest.mark.parametrize("n,nz,seed,frictional,strain", [(16, 11, 0, True, False), (24, 41, 1, True, True),
+                                                          (20, 21, 2, False, False), (18, 41, 3, False, True)])
+def test_thermal_step(n, nz, seed, frictional, strain):
+    print(f"thermal_step n={n} nz={nz} seed={seed} frictional={frictional} strain={strain}", flush=True)
+    from oracle.oracle import strain_heating, thermal_step
+    F, g, tm, th = _thermal_pair(n, nz, seed, frictional, strain)
     rep = tm.step(DT, report=True)
-    B_orc = thermal_step(th, F["u"], F["v"], F["H"], F["beta"], DT, n_smooth=10, frictional_heating=frictional)
+    phi = None
+    if strain:
+        phi = np.repeat(strain_heating(F["u"], F["v"], F["B"], 3.0, 1e-6, DX, 917.0)[:, :, None], nz, axis=2)
+    B_orc = thermal_step(th, F["u"], F["v"], F["H"], F["beta"], DT, n_smooth=10, frictional_heating=frictional,
+                         strain_heating=phi)
     ops = tm.ops
     assert rel_err(N(ops.enthalpy_velocity.omega), th.omega) < KTOL
     assert rel_err(N(ops.enthalpy_velocity.u3d), th.u3d) < KTOL
@@ -275,8 +302,8 @@ def test_thermal_step(n, nz, seed, frictional):
 
 def test_thermal_step_host_path_matches_fused():
     print("thermal_step host path", flush=True)
-    F, g, tm, th = _thermal_pair(16, 11, 4)
-    F2, g2, tm2, _ = _thermal_pair(16, 11, 4)
+    F, g, tm, th = _thermal_pair(16, 11, 4, strain=True)
+    F2, g2, tm2, _ = _thermal_pair(16, 11, 4, strain=True)
     tm2.ops.smoother_config.hook_func = lambda i: None
     tm.step(DT)
     tm2.step(DT)
