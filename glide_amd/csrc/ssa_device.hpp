@@ -401,6 +401,231 @@ __device__ __forceinline__ void local_system(const Patch<T>& p, const T x[5], T 
     }
 }
 
+// ----------------------------------------------------------- folded cell system
+// During a smoothing pass the viscosity, the parameters and every neighbour
+// value of a cell's patch are frozen, so the patch system of local_system is
+// a low-order polynomial in the cell's unknowns x = (u_l, u_r, v_t, v_b, H_c)
+// apart from the four facet flux speeds sqrt(w^2 + 10) and, for m != 1, the
+// drag power.  CellSys holds the patch folded into those polynomial
+// coefficients once per cell, and cell_system evaluates J(x) and r(x) - f
+// (with the Vanka dampings) from it: the same matrix and residual as
+// local_system term by term, at about a quarter of the FP64 instructions per
+// Newton iteration.  Boundary facets, cells and vertices enter as zero
+// coefficients instead of branches.
+template <typename T>
+struct CellSys {
+    T D, hD;                            // 1/dx, 0.5/dx
+    // thickness row
+    T H_l, H_r, H_t, H_b;               // neighbour thickness of the four facet fluxes
+    T dL, dR, dT, dB;                   // D on interior facets, 0 on the domain edge
+    T r4h;                              // 1/dt + calving coefficient * D (coefficient of H_c in r4)
+    T j24c;                             // r4h + mc_damping
+    T f4;
+    // velocity block: centre stresses K = kH * H_c in a fixed +-1/+-2 pattern,
+    // diagonal a_k + h_k * H_c
+    T kH;                               // 2 e_c D^2
+    T a0, a1, a2, a3, h0, h1, h2, h3;
+    // residual rows 0-3: constant parts (neighbour stresses minus forcing) and
+    // coefficients of the row's own velocity (neighbour normal stress + linear drag)
+    T c0, c1, c2, c3, o0, o1, o2, o3;
+    T eD;                               // 2 e_c D (centre strain -> stress)
+    // shear vertices tl, tr, bl, br: eta*H = C + q H_c, strain = s + hD * (own velocities)
+    T C_tl, C_tr, C_bl, C_br, q_tl, q_tr, q_bl, q_br;
+    T s_tl, s_tr, s_bl, s_br;
+    // driving stress: surface S_c = Zc + gc H_c against the neighbours' S
+    T S_l, S_r, S_t, S_b, Zc, gc;
+    // nonlinear drag: effective beta of the four facets and the cross velocities
+    T be_l, be_r, be_t, be_b;
+    T u_tl, u_tr, u_bl, u_br, v_lt, v_lb, v_rt, v_rb;
+};
+
+template <typename T>
+__device__ __forceinline__ void make_cell_sys(CellSys<T>& s, const Patch<T>& p, const Phys<T>& P, T calving_rate,
+                                              T dxi, T dt, T ssa_damping, T mc_damping) {
+    const T D = dxi, D2 = dxi * dxi;
+    s.D = D;
+    s.hD = T(0.5) * D;
+    s.H_l = p.H_l; s.H_r = p.H_r; s.H_t = p.H_t; s.H_b = p.H_b;
+    s.dL = p.in_l ? D : T(0); s.dR = p.in_r ? D : T(0); s.dT = p.in_t ? D : T(0); s.dB = p.in_b ? D : T(0);
+    const T kcal = (T(1) - p.p_c) * calving_rate *
+                   ((T(1) - p.p_l) + (T(1) - p.p_r) + (T(1) - p.p_t) + (T(1) - p.p_b));
+    s.r4h = T(1) / dt + kcal * D;
+    s.j24c = s.r4h + mc_damping;
+    s.f4 = p.f4;
+    s.kH = T(2) * p.e_c * D2;
+    // shear vertices (glide/cuda/stress.cu sigma_xy): eta*H averaged over the four cells
+    const T qc = T(0.25) * p.e_c;
+    s.C_tl = p.vtl ? T(0.25) * (p.e_tl * p.H_tl + p.e_t * p.H_t + p.e_l * p.H_l) : T(0);
+    s.C_tr = p.vtr ? T(0.25) * (p.e_t * p.H_t + p.e_tr * p.H_tr + p.e_r * p.H_r) : T(0);
+    s.C_bl = p.vbl ? T(0.25) * (p.e_l * p.H_l + p.e_bl * p.H_bl + p.e_b * p.H_b) : T(0);
+    s.C_br = p.vbr ? T(0.25) * (p.e_r * p.H_r + p.e_b * p.H_b + p.e_br * p.H_br) : T(0);
+    s.q_tl = p.vtl ? qc : T(0); s.q_tr = p.vtr ? qc : T(0); s.q_bl = p.vbl ? qc : T(0); s.q_br = p.vbr ? qc : T(0);
+    s.s_tl = s.hD * (p.u_tl - p.v_lt);
+    s.s_tr = s.hD * (p.u_tr + p.v_rt);
+    s.s_bl = -s.hD * (p.u_bl + p.v_lb);
+    s.s_br = s.hD * (p.v_rb - p.u_br);
+    // neighbour normal stresses (sigma_xx of the left/right cell, sigma_yy of the top/bottom cell)
+    const T nl = p.cl ? T(2) * p.e_l * p.H_l * D2 : T(0);
+    const T nr = p.cr ? T(2) * p.e_r * p.H_r * D2 : T(0);
+    const T nt = p.ct ? T(2) * p.e_t * p.H_t * D2 : T(0);
+    const T nb = p.cb ? T(2) * p.e_b * p.H_b * D2 : T(0);
+    s.c0 = -nl * (p.v_lt - p.v_lb - T(2) * p.u_ll) - p.f0;
+    s.c1 = nr * (T(2) * p.u_rr + p.v_rt - p.v_rb) - p.f1;
+    s.c2 = nt * (p.u_tr - p.u_tl + T(2) * p.v_tt) - p.f2;
+    s.c3 = -nb * (p.u_br - p.u_bl - T(2) * p.v_bb) - p.f3;
+    // drag: effective beta per facet; the linear law contributes -(beta_eff + water_drag) w
+    s.be_l = T(0.5) * (p.b_l * p.p_l + p.b_c * p.p_c);
+    s.be_r = T(0.5) * (p.b_c * p.p_c + p.b_r * p.p_r);
+    s.be_t = T(0.5) * (p.b_t * p.p_t + p.b_c * p.p_c);
+    s.be_b = T(0.5) * (p.b_c * p.p_c + p.b_b * p.p_b);
+    const T lin = P.linear_drag ? T(1) : T(0);
+    s.o0 = -T(2) * nl - lin * (s.be_l + P.water_drag);
+    s.o1 = -T(2) * nr - lin * (s.be_r + P.water_drag);
+    s.o2 = -T(2) * nt - lin * (s.be_t + P.water_drag);
+    s.o3 = -T(2) * nb - lin * (s.be_b + P.water_drag);
+    // diagonal: neighbour normal stress, the two shear vertices touching the facet, linear drag, damping
+    s.a0 = s.o0 - (s.C_tl + s.C_bl) * D2 - ssa_damping;
+    s.a1 = s.o1 - (s.C_tr + s.C_br) * D2 - ssa_damping;
+    s.a2 = s.o2 - (s.C_tl + s.C_tr) * D2 - ssa_damping;
+    s.a3 = s.o3 - (s.C_bl + s.C_br) * D2 - ssa_damping;
+    s.h0 = -(s.q_tl + s.q_bl) * D2;
+    s.h1 = -(s.q_tr + s.q_br) * D2;
+    s.h2 = -(s.q_tl + s.q_tr) * D2;
+    s.h3 = -(s.q_bl + s.q_br) * D2;
+    s.eD = T(2) * p.e_c * D;
+    // driving stress surfaces (glide/cuda/stress.cu driving terms)
+    s.S_l = p.p_l * p.z_l - (T(1) - p.p_l) * T(0.917) * p.H_l + p.H_l;
+    s.S_r = p.p_r * p.z_r - (T(1) - p.p_r) * T(0.917) * p.H_r + p.H_r;
+    s.S_t = p.p_t * p.z_t - (T(1) - p.p_t) * T(0.917) * p.H_t + p.H_t;
+    s.S_b = p.p_b * p.z_b - (T(1) - p.p_b) * T(0.917) * p.H_b + p.H_b;
+    s.Zc = p.p_c * p.z_c;
+    s.gc = T(1) - (T(1) - p.p_c) * T(0.917);
+    s.u_tl = p.u_tl; s.u_tr = p.u_tr; s.u_bl = p.u_bl; s.u_br = p.u_br;
+    s.v_lt = p.v_lt; s.v_lb = p.v_lb; s.v_rt = p.v_rt; s.v_rb = p.v_rb;
+}
+
+// Drag power of the nonlinear law at facet velocity w with cross velocities c0..c3.
+template <typename T>
+__device__ __forceinline__ void drag_power(T w, T c0, T c1, T c2, T c3, const Phys<T>& P, T& pw, T& dpw) {
+    T s2 = w * w + T(0.25) * (c0 * c0 + c1 * c1 + c2 * c2 + c3 * c3);
+    pw = pow(s2 + P.u_reg, P.drag_q);
+    dpw = P.drag_q * pow(s2 + P.u_reg, P.drag_q - T(1));
+}
+
+// J(x) and r(x) - f of the cell with the Vanka dampings (before boundary /
+// active-set pinning).  LINEAR: the drag law is linear (m == 1, u_reg > 0).
+template <typename T, bool LINEAR>
+__device__ __forceinline__ void cell_system(const CellSys<T>& s, const T x[5], T J[25], T r[5], const Phys<T>& P) {
+    const T ul = x[0], ur = x[1], vt = x[2], vb = x[3], Hc = x[4];
+    const T D = s.D;
+    // centre cell normal stresses
+    const T K = s.kH * Hc;
+    const T du = ur - ul, dv = vt - vb;
+    const T exx = (T(2) * du + dv) * D, eyy = (du + T(2) * dv) * D;
+    const T sx = s.eD * Hc * exx, sy = s.eD * Hc * eyy;    // 2 eta_c H_c e D
+    const T jx = s.eD * exx, jy = s.eD * eyy;               // their H_c derivatives
+    // shear vertices
+    const T eH_tl = s.C_tl + s.q_tl * Hc, eH_tr = s.C_tr + s.q_tr * Hc;
+    const T eH_bl = s.C_bl + s.q_bl * Hc, eH_br = s.C_br + s.q_br * Hc;
+    const T e_tl = s.s_tl + s.hD * (vt - ul), e_tr = s.s_tr - s.hD * (ur + vt);
+    const T e_bl = s.s_bl + s.hD * (ul + vb), e_br = s.s_br + s.hD * (ur - vb);
+    const T t_tl = T(2) * D * eH_tl * e_tl, t_tr = T(2) * D * eH_tr * e_tr;
+    const T t_bl = T(2) * D * eH_bl * e_bl, t_br = T(2) * D * eH_br * e_br;
+    const T g_tl = T(2) * D * s.q_tl * e_tl, g_tr = T(2) * D * s.q_tr * e_tr;
+    const T g_bl = T(2) * D * s.q_bl * e_bl, g_br = T(2) * D * s.q_br * e_br;
+    // driving stress
+    const T Sc = s.Zc + s.gc * Hc;
+    const T Hm_l = T(0.5) * (s.H_l + Hc), Hm_r = T(0.5) * (Hc + s.H_r);
+    const T Hm_t = T(0.5) * (s.H_t + Hc), Hm_b = T(0.5) * (Hc + s.H_b);
+    const T dS_l = Sc - s.S_l, dS_r = s.S_r - Sc, dS_t = s.S_t - Sc, dS_b = Sc - s.S_b;
+
+    r[0] = sx + s.c0 + s.o0 * ul + t_tl - t_bl - Hm_l * dS_l * s.dL;
+    r[1] = -sx + s.c1 + s.o1 * ur + t_tr - t_br - Hm_r * dS_r * s.dR;
+    r[2] = -sy + s.c2 + s.o2 * vt - t_tl + t_tr - Hm_t * dS_t * s.dT;
+    r[3] = sy + s.c3 + s.o3 * vb - t_bl + t_br - Hm_b * dS_b * s.dB;
+
+    const T K2 = T(2) * K;
+    J[0] = s.a0 + s.h0 * Hc - K2; J[1] = K2;  J[2] = K;   J[3] = -K;
+    J[5] = K2;  J[6] = s.a1 + s.h1 * Hc - K2; J[7] = -K;  J[8] = K;
+    J[10] = K;  J[11] = -K; J[12] = s.a2 + s.h2 * Hc - K2; J[13] = K2;
+    J[15] = -K; J[16] = K;  J[17] = K2; J[18] = s.a3 + s.h3 * Hc - K2;
+    J[4] = jx + g_tl - g_bl - (T(0.5) * dS_l + Hm_l * s.gc) * s.dL;
+    J[9] = -jx + g_tr - g_br - (T(0.5) * dS_r - Hm_r * s.gc) * s.dR;
+    J[14] = -jy - g_tl + g_tr - (T(0.5) * dS_t - Hm_t * s.gc) * s.dT;
+    J[19] = jy - g_bl + g_br - (T(0.5) * dS_b + Hm_b * s.gc) * s.dB;
+
+    if (!LINEAR) {
+        // glide/cuda/stress.cu basal drag with the 4-neighbour cross velocity
+        T pw, dpw, k;
+        drag_power(ul, s.v_lt, vt, s.v_lb, vb, P, pw, dpw);
+        r[0] -= (s.be_l * pw + P.water_drag) * ul;
+        J[0] -= s.be_l * (T(2) * dpw * ul * ul + pw) + P.water_drag;
+        k = -s.be_l * T(0.5) * dpw * ul;
+        J[2] += k * vt; J[3] += k * vb;
+        drag_power(ur, vt, s.v_rt, vb, s.v_rb, P, pw, dpw);
+        r[1] -= (s.be_r * pw + P.water_drag) * ur;
+        J[6] -= s.be_r * (T(2) * dpw * ur * ur + pw) + P.water_drag;
+        k = -s.be_r * T(0.5) * dpw * ur;
+        J[7] += k * vt; J[8] += k * vb;
+        drag_power(vt, s.u_tl, s.u_tr, ul, ur, P, pw, dpw);
+        r[2] -= (s.be_t * pw + P.water_drag) * vt;
+        J[12] -= s.be_t * (T(2) * dpw * vt * vt + pw) + P.water_drag;
+        k = -s.be_t * T(0.5) * dpw * vt;
+        J[10] += k * ul; J[11] += k * ur;
+        drag_power(vb, ul, ur, s.u_bl, s.u_br, P, pw, dpw);
+        r[3] -= (s.be_b * pw + P.water_drag) * vb;
+        J[18] -= s.be_b * (T(2) * dpw * vb * vb + pw) + P.water_drag;
+        k = -s.be_b * T(0.5) * dpw * vb;
+        J[15] += k * ul; J[16] += k * ur;
+    }
+
+    // thickness row: dH/dt + div(q) + calving with Lax-Friedrichs facet fluxes
+    const T a_l = sqrt(ul * ul + T(10)), a_r = sqrt(ur * ur + T(10));
+    const T a_t = sqrt(vt * vt + T(10)), a_b = sqrt(vb * vb + T(10));
+    const T Fl = Hm_l * ul - T(0.5) * a_l * (Hc - s.H_l);
+    const T Fr = Hm_r * ur - T(0.5) * a_r * (s.H_r - Hc);
+    const T Ft = Hm_t * vt - T(0.5) * a_t * (s.H_t - Hc);
+    const T Fb = Hm_b * vb - T(0.5) * a_b * (Hc - s.H_b);
+    r[4] = s.r4h * Hc + (Fr * s.dR - Fl * s.dL) + (Ft * s.dT - Fb * s.dB) - s.f4;
+    J[20] = -(Hm_l - T(0.5) * (ul / a_l) * (Hc - s.H_l)) * s.dL;
+    J[21] = (Hm_r - T(0.5) * (ur / a_r) * (s.H_r - Hc)) * s.dR;
+    J[22] = (Hm_t - T(0.5) * (vt / a_t) * (s.H_t - Hc)) * s.dT;
+    J[23] = -(Hm_b - T(0.5) * (vb / a_b) * (Hc - s.H_b)) * s.dB;
+    J[24] = s.j24c + T(0.5) * ((ur + a_r) * s.dR - (ul - a_l) * s.dL + (vt + a_t) * s.dT - (vb - a_b) * s.dB);
+}
+
+// lu5_solve with the back substitution multiplying by the pivot reciprocals
+// of the factorisation (5 instead of 10 divisions).
+template <typename T>
+__device__ __forceinline__ void lu5_solve_rcp(T A[25], const T b[5], T x[5]) {
+    T inv[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        inv[k] = T(1) / A[k * 5 + k];
+#pragma unroll
+        for (int i = k + 1; i < 5; ++i) {
+            A[i * 5 + k] *= inv[k];
+#pragma unroll
+            for (int j = k + 1; j < 5; ++j) A[i * 5 + j] -= A[i * 5 + k] * A[k * 5 + j];
+        }
+    }
+    T y[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        T s = b[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) s -= A[i * 5 + j] * y[j];
+        y[i] = s;
+    }
+#pragma unroll
+    for (int i = 4; i >= 0; --i) {
+        T s = y[i];
+#pragma unroll
+        for (int j = i + 1; j < 5; ++j) s -= A[i * 5 + j] * x[j];
+        x[i] = s * inv[i];
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void pin_row(T J[25], int row) {
 #pragma unroll

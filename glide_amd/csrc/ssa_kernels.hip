@@ -210,7 +210,7 @@ __device__ __forceinline__ void fill_eta_tile(T (*s_eta)[TX + 2], const SsaArgs<
 // patch against the state at the start of the pass and writes its increments
 // (half-weighted for facets) into the workspace planes
 // work[0..4] = (du_l, du_r, dv_t, dv_b, dH); the active-set flag goes to mask.
-template <typename T>
+template <typename T, bool LINEAR>
 __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__ work, T* __restrict__ mask,
                                                    VankaArgs<T> vk) {
     __shared__ T s_eta[TY + 2][TX + 2];
@@ -221,28 +221,31 @@ __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__
     const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
     const int ny = a.ny, nx = a.nx;
     if (i >= ny || j >= nx) return;
-    Patch<T> p;
-    load_patch(p, a, s_eta, threadIdx.y + 1, threadIdx.x + 1, i, j, a.f_u, a.f_v, a.f_H, a.phi);
+    CellSys<T> cs;
+    T thk;
+    {
+        Patch<T> p;
+        load_patch(p, a, s_eta, threadIdx.y + 1, threadIdx.x + 1, i, j, a.f_u, a.f_v, a.f_H, a.phi);
+        make_cell_sys(cs, p, a.P, vk.calving_rate, a.dxi, a.dt, vk.ssa_damping, vk.mc_damping);
+        thk = p.gam;
+    }
     const long c = (long)i * nx + j;
-    T x0[5] = {a.u[(long)i * (nx + 1) + j], a.u[(long)i * (nx + 1) + j + 1], a.v[c], a.v[c + nx], a.H[c]};
-    T x[5], cmp[5];
+    T x[5] = {a.u[(long)i * (nx + 1) + j], a.u[(long)i * (nx + 1) + j + 1], a.v[c], a.v[c + nx], a.H[c]};
+    T cmp[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) { x[k] = x0[k]; cmp[k] = T(0); }
+    for (int k = 0; k < 5; ++k) cmp[k] = T(0);
     T masked = mask[c];
     T rnorm = T(1);
     const T tol = T(1e-6);
-    const T thk = p.gam;
+    const bool pin0 = j == 0, pin1 = j == nx - 1, pin2 = i == 0, pin3 = i == ny - 1;
     int it = 0;
     while (it < vk.newton_steps && rnorm > tol) {
         T J[25], r[5];
-        local_system(p, x, J, r, a.P, vk.calving_rate, a.dxi, a.dt);
-        r[0] -= p.f0; r[1] -= p.f1; r[2] -= p.f2; r[3] -= p.f3; r[4] -= p.f4;
-        J[0] -= vk.ssa_damping; J[6] -= vk.ssa_damping; J[12] -= vk.ssa_damping; J[18] -= vk.ssa_damping;
-        J[24] += vk.mc_damping;
-        if (j == 0) { pin_row(J, 0); r[0] = x[0]; }
-        if (j == nx - 1) { pin_row(J, 1); r[1] = x[1]; }
-        if (i == 0) { pin_row(J, 2); r[2] = x[2]; }
-        if (i == ny - 1) { pin_row(J, 3); r[3] = x[3]; }
+        cell_system<T, LINEAR>(cs, x, J, r, a.P);
+        if (pin0) { pin_row(J, 0); r[0] = x[0]; }
+        if (pin1) { pin_row(J, 1); r[1] = x[1]; }
+        if (pin2) { pin_row(J, 2); r[2] = x[2]; }
+        if (pin3) { pin_row(J, 3); r[3] = x[3]; }
         if ((x[4] - a.dt * r[4]) <= thk) {
             masked = T(1);
             pin_row(J, 4);
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__
             masked = T(0);
         }
         T d[5];
-        lu5_solve(J, r, d);
+        lu5_solve_rcp(J, r, d);
         rnorm = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3] + r[4] * r[4];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {  // compensated relaxed update, local relaxation 0.5
@@ -264,11 +267,11 @@ __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__
         ++it;
     }
     const long n = (long)ny * nx;
-    work[c] = T(0.5) * (x[0] - x0[0]);
-    work[n + c] = T(0.5) * (x[1] - x0[1]);
-    work[2 * n + c] = T(0.5) * (x[2] - x0[2]);
-    work[3 * n + c] = T(0.5) * (x[3] - x0[3]);
-    work[4 * n + c] = x[4] - x0[4];
+    work[c] = T(0.5) * (x[0] - a.u[(long)i * (nx + 1) + j]);
+    work[n + c] = T(0.5) * (x[1] - a.u[(long)i * (nx + 1) + j + 1]);
+    work[2 * n + c] = T(0.5) * (x[2] - a.v[c]);
+    work[3 * n + c] = T(0.5) * (x[3] - a.v[c + nx]);
+    work[4 * n + c] = x[4] - a.H[c];
     mask[c] = masked;
     if (vk.iters) vk.iters[c] = it;
 }
@@ -695,7 +698,10 @@ hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* 
                              hipStream_t st) {
     {
         TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx, st);
-        k_vanka_cell<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, work, mask, vk);
+        if (a.P.linear_drag)
+            k_vanka_cell<T, true><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, work, mask, vk);
+        else
+            k_vanka_cell<T, false><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, work, mask, vk);
     }
     dim3 g((a.nx + 1 + 255) / 256, a.ny + 1);
     TimedLaunch tl(kTimeVankaApply, (long)a.ny * a.nx, st);

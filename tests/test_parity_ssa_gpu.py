@@ -110,6 +110,24 @@ def test_vanka_single_pass(orc, shape):
     assert np.array_equal(N(g.state.mask.data), mask)
 
 
+@pytest.mark.parametrize("shape", SHAPES[:2])
+def test_vanka_single_pass_nonlinear_drag_calving(orc, shape):
+    ny, nx = shape
+    print(f"vanka pass nonlinear drag shape={shape}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=9, mask_frac=0.1)
+    P = orc.phys(m=1.0 / 3.0, u_reg=0.5, calving_rate=0.2)
+    g = grid_from(F)
+    g.sliding.m.set(1.0 / 3.0)
+    g.sliding.u_reg.set(0.5)
+    g.calving.calving_rate.set(0.2)
+    g.forward_operators.vanka_sweep(5.0, 1, freeze_phi=True)
+    u, v, H, mask, _ = orc.vanka_sweep(F, P, 1500.0, 5.0)
+    assert rel_err(N(g.state.u.data), u) < KTOL
+    assert rel_err(N(g.state.v.data), v) < KTOL
+    assert rel_err(N(g.state.H.data), H) < KTOL
+    assert np.array_equal(N(g.state.mask.data), mask)
+
+
 def test_vanka_passes_with_phi_refresh(orc):
     F = dome_fields(n=32, seed=6)
     g = grid_from(F)
