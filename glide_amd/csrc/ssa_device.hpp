@@ -504,6 +504,23 @@ __device__ __forceinline__ void make_cell_sys(CellSys<T>& s, const Patch<T>& p, 
     s.v_lt = p.v_lt; s.v_lb = p.v_lb; s.v_rt = p.v_rt; s.v_rb = p.v_rb;
 }
 
+// 1/sqrt(s) and 1/a to within an ulp or two: the hardware estimates refined
+// by two Newton steps each (for the flux speeds, s >= 10, and the LU pivots,
+// which are diagonally dominated and far from denormal).
+__device__ __forceinline__ double rsqrt_refined(double s) {
+    double y = __builtin_amdgcn_rsq(s);
+    const double h = 0.5 * s;
+    y = y * (1.5 - h * y * y);
+    y = y * (1.5 - h * y * y);
+    return y;
+}
+__device__ __forceinline__ double rcp_refined(double a) {
+    double y = __builtin_amdgcn_rcp(a);
+    y = fma(y, fma(-a, y, 1.0), y);
+    y = fma(y, fma(-a, y, 1.0), y);
+    return y;
+}
+
 // Drag power of the nonlinear law at facet velocity w with cross velocities c0..c3.
 template <typename T>
 __device__ __forceinline__ void drag_power(T w, T c0, T c1, T c2, T c3, const Phys<T>& P, T& pw, T& dpw) {
@@ -580,28 +597,29 @@ __device__ __forceinline__ void cell_system(const CellSys<T>& s, const T x[5], T
     }
 
     // thickness row: dH/dt + div(q) + calving with Lax-Friedrichs facet fluxes
-    const T a_l = sqrt(ul * ul + T(10)), a_r = sqrt(ur * ur + T(10));
-    const T a_t = sqrt(vt * vt + T(10)), a_b = sqrt(vb * vb + T(10));
+    const T sl = ul * ul + T(10), sr = ur * ur + T(10), st = vt * vt + T(10), sb = vb * vb + T(10);
+    const T il = rsqrt_refined(sl), ir = rsqrt_refined(sr), it = rsqrt_refined(st), ib = rsqrt_refined(sb);
+    const T a_l = sl * il, a_r = sr * ir, a_t = st * it, a_b = sb * ib;   // sqrt(w^2 + 10)
     const T Fl = Hm_l * ul - T(0.5) * a_l * (Hc - s.H_l);
     const T Fr = Hm_r * ur - T(0.5) * a_r * (s.H_r - Hc);
     const T Ft = Hm_t * vt - T(0.5) * a_t * (s.H_t - Hc);
     const T Fb = Hm_b * vb - T(0.5) * a_b * (Hc - s.H_b);
     r[4] = s.r4h * Hc + (Fr * s.dR - Fl * s.dL) + (Ft * s.dT - Fb * s.dB) - s.f4;
-    J[20] = -(Hm_l - T(0.5) * (ul / a_l) * (Hc - s.H_l)) * s.dL;
-    J[21] = (Hm_r - T(0.5) * (ur / a_r) * (s.H_r - Hc)) * s.dR;
-    J[22] = (Hm_t - T(0.5) * (vt / a_t) * (s.H_t - Hc)) * s.dT;
-    J[23] = -(Hm_b - T(0.5) * (vb / a_b) * (Hc - s.H_b)) * s.dB;
+    J[20] = -(Hm_l - T(0.5) * (ul * il) * (Hc - s.H_l)) * s.dL;
+    J[21] = (Hm_r - T(0.5) * (ur * ir) * (s.H_r - Hc)) * s.dR;
+    J[22] = (Hm_t - T(0.5) * (vt * it) * (s.H_t - Hc)) * s.dT;
+    J[23] = -(Hm_b - T(0.5) * (vb * ib) * (Hc - s.H_b)) * s.dB;
     J[24] = s.j24c + T(0.5) * ((ur + a_r) * s.dR - (ul - a_l) * s.dL + (vt + a_t) * s.dT - (vb - a_b) * s.dB);
 }
 
-// lu5_solve with the back substitution multiplying by the pivot reciprocals
-// of the factorisation (5 instead of 10 divisions).
+// lu5_solve with refined hardware reciprocals of the pivots, used for both
+// the elimination and the back substitution (no divisions).
 template <typename T>
 __device__ __forceinline__ void lu5_solve_rcp(T A[25], const T b[5], T x[5]) {
     T inv[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        inv[k] = T(1) / A[k * 5 + k];
+        inv[k] = rcp_refined(A[k * 5 + k]);
 #pragma unroll
         for (int i = k + 1; i < 5; ++i) {
             A[i * 5 + k] *= inv[k];
