@@ -129,6 +129,8 @@ SIGNATURES = {
     "glide_fas_adjoint_vcycle": (_I, [_LP, _I, _I, _I, _D, ctypes.POINTER(FasConfigC),
                                       ctypes.POINTER(VankaConfigC), _P]),
     "glide_gradients": (_I, [_LP, _D, _I, _P]),
+    "glide_graphs_enable": (_I, [_I]),
+    "glide_graphs_clear": (_I, []),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     # include/glide_enthalpy.h
@@ -153,6 +155,15 @@ SIGNATURES = {
 
 TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3, "residual": 4, "adjoint_apply": 5,
                  "enth_column": 6, "enth_layer": 7, "enth_residual": 8}
+
+
+def graphs_enable(on: bool) -> None:
+    """Replay fixed-cycle FAS solves from cached HIP graphs (default on)."""
+    call("glide_graphs_enable", int(bool(on)))
+
+
+def graphs_clear() -> None:
+    call("glide_graphs_clear")
 
 
 def timing_enable(on: bool) -> None:

@@ -5,8 +5,9 @@
 // their kernels on the caller's stream; the only host synchronisation is the
 // per-V-cycle residual norm the stopping test needs (none when both
 // tolerances are non-positive: a fixed number of V-cycles, norms read once at
-// the end).  All field buffers belong to the caller; the library owns one
-// small per-device reduction scratch (allocated on first use).
+// the end, and the whole solve replayed from a cached HIP graph).  All field
+// buffers belong to the caller; the library owns one small per-device
+// reduction scratch (allocated on first use) and the graph cache.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -212,7 +213,7 @@ int axpby(void* out, double a, const void* x, double b, const void* y, size_t n,
     return GLIDE_OK;
 }
 int copy(void* dst, const void* src, size_t n, hipStream_t st) {
-    HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(R), hipMemcpyDeviceToDevice, st), "copy");
+    HIP_TRY(launch_copy<R>(P(dst), CP(src), (long)n, st), "copy");
     return GLIDE_OK;
 }
 int fill(void* dst, double val, size_t n, hipStream_t st) {
@@ -376,16 +377,105 @@ int check_hierarchy(const glide_level* levels, int n_levels, int start, const gl
     return GLIDE_OK;
 }
 
+// ---------------------------------------------------------------- graph replay
+// A fixed-cycle solve enqueues the same few thousand launches every call.
+// Those launches are captured once into a HIP graph on a private stream and
+// replayed on the caller's stream afterwards, which removes most of the
+// per-launch dispatch gap between the many short dependent kernels of the
+// coarse levels.  The cache key is every byte the launches depend on (the
+// level descriptors with their device pointers, the configs, dt, the timing
+// switch); buffer contents are read at replay time as in eager execution.
+// Captured graphs hold no memset nodes and no host work.  Elapsed times of
+// events recorded inside a graph cannot be queried, so solves run eagerly
+// while launch timing is on.
+struct GraphEntry {
+    std::string key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long last_use = 0;
+};
+struct DeviceGraphs {
+    hipStream_t cap = nullptr;
+    std::vector<GraphEntry> entries;
+    unsigned long clock = 0;
+};
+constexpr size_t kMaxGraphs = 8;
+bool g_graphs_on = true;
+std::mutex g_graph_mutex;
+std::vector<DeviceGraphs> g_graphs;
+
+template <typename T>
+void key_append(std::string& k, const T& v) {
+    k.append(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+
+std::string solve_key(char tag, const FasCtx& X, int start) {
+    std::string k(1, tag);
+    key_append(k, X.n_levels);
+    key_append(k, start);
+    key_append(k, X.dt);
+    key_append(k, X.cfg);
+    for (int l = start; l < X.n_levels; ++l) {
+        key_append(k, X.L[l]);
+        key_append(k, X.vc[l]);
+    }
+    return k;
+}
+
+void drop_entry(GraphEntry& e) {
+    if (e.exec) hipGraphExecDestroy(e.exec);
+    e.exec = nullptr;
+}
+
+// Replays (capturing on first use) the launches `body` enqueues on the stream
+// it is given; on return the replay is enqueued on `st`.
+template <typename Body>
+int replay(const std::string& key, hipStream_t st, Body&& body) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_graph_mutex);
+    if ((int)g_graphs.size() <= dev) g_graphs.resize(dev + 1);
+    DeviceGraphs& G = g_graphs[dev];
+    GraphEntry* e = nullptr;
+    for (auto& x : G.entries)
+        if (x.key == key) e = &x;
+    if (!e) {
+        if (!G.cap) HIP_TRY(hipStreamCreateWithFlags(&G.cap, hipStreamNonBlocking), "hipStreamCreate(capture)");
+        HIP_TRY(hipStreamBeginCapture(G.cap, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        int rc = body(G.cap);
+        hipGraph_t graph = nullptr;
+        hipError_t ee = hipStreamEndCapture(G.cap, &graph);
+        if (rc != GLIDE_OK || ee != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            if (rc != GLIDE_OK) return rc;
+            return hip_fail(ee, "hipStreamEndCapture");
+        }
+        hipGraphExec_t exec = nullptr;
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
+        if (G.entries.size() >= kMaxGraphs) {
+            size_t lru = 0;
+            for (size_t q = 1; q < G.entries.size(); ++q)
+                if (G.entries[q].last_use < G.entries[lru].last_use) lru = q;
+            drop_entry(G.entries[lru]);
+            G.entries.erase(G.entries.begin() + lru);
+        }
+        G.entries.push_back(GraphEntry{key, exec, 0});
+        e = &G.entries.back();
+    }
+    e->last_use = ++G.clock;
+    HIP_TRY(hipGraphLaunch(e->exec, st), "hipGraphLaunch");
+    return GLIDE_OK;
+}
+
 // Shared solve loop of the forward and adjoint FAS drivers
 // (glide/multigrid.py:502-533, 873-903).
 template <typename Residual, typename Cycle>
-int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, Cycle&& cycle) {
+int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, Cycle&& cycle, char tag) {
     DeviceScratch* s = nullptr;
     GLIDE_TRY(scratch_for_current_device(&s));
     hipStream_t st = X.st;
     const bool fixed = !(X.cfg.relative_tolerance > 0.0) && !(X.cfg.absolute_tolerance > 0.0);
-    // initial residual (slot 0)
-    GLIDE_TRY(resid(true, s->norms));
     double r0 = 0.0, absn = 0.0, rel = 1.0;
     double n3[3] = {0, 0, 0};
     auto read_slots = [&](int nslots) -> int {
@@ -395,10 +485,20 @@ int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, 
     };
     int it = 0;
     if (fixed) {
-        for (; it < X.cfg.maximum_vcycles; ++it) {
-            GLIDE_TRY(cycle());
-            GLIDE_TRY(resid(false, s->norms + 3 * (it + 1)));
-        }
+        // initial residual (slot 0), the cycles, the residual after each (slots 1..)
+        auto body = [&](hipStream_t cs) -> int {
+            X.st = cs;
+            int rc = resid(true, s->norms);
+            for (int k = 0; rc == GLIDE_OK && k < X.cfg.maximum_vcycles; ++k) {
+                rc = cycle();
+                if (rc == GLIDE_OK) rc = resid(false, s->norms + 3 * (k + 1));
+            }
+            X.st = st;
+            return rc;
+        };
+        it = X.cfg.maximum_vcycles;
+        if (g_graphs_on && !timing_on()) GLIDE_TRY(replay(solve_key(tag, X, start), st, body));
+        else GLIDE_TRY(body(st));
         GLIDE_TRY(read_slots(it + 1));
         r0 = std::sqrt(s->host[0] + s->host[1] + s->host[2]);
         absn = r0;
@@ -410,6 +510,7 @@ int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, 
         const double* q = s->host + 3 * it;
         n3[0] = q[0]; n3[1] = q[1]; n3[2] = q[2];
     } else {
+        GLIDE_TRY(resid(true, s->norms));
         GLIDE_TRY(read_slots(1));
         r0 = std::sqrt(s->host[0] + s->host[1] + s->host[2]);
         n3[0] = s->host[0]; n3[1] = s->host[1]; n3[2] = s->host[2];
@@ -547,16 +648,18 @@ int glide_fascd_solve(const glide_level* levels, int n_levels, int start_level, 
     if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
     FasCtx X{levels, n_levels, dt, *cfg, vc, (hipStream_t)stream};
     const glide_level& L = levels[start_level];
-    hipStream_t st = X.st;
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));  // allocated before any capture
     // set_rhs: f_H = H_prev/dt + smb (glide/operators.py:275-276)
-    HIP_TRY(launch_axpby<R>(P(L.f_H), 1.0 / dt, CP(L.H_prev), 1.0, CP(L.smb), 0.0, nullptr, (long)nC(L), st), "rhs");
+    HIP_TRY(launch_axpby<R>(P(L.f_H), 1.0 / dt, CP(L.H_prev), 1.0, CP(L.smb), 0.0, nullptr, (long)nC(L), X.st),
+            "rhs");
     auto resid = [&](bool initial, double* slot) -> int {
-        if (initial) GLIDE_TRY(grounded(L, 0.0, st));
-        GLIDE_TRY(residual(L, dt, GLIDE_USE_FORCING | GLIDE_USE_MASK, P(L.r_u), P(L.r_v), P(L.r_H), st));
-        return sumsq3_async(CP(L.r_u), nU(L), CP(L.r_v), nV(L), CP(L.r_H), nC(L), slot, st);
+        if (initial) GLIDE_TRY(grounded(L, 0.0, X.st));
+        GLIDE_TRY(residual(L, dt, GLIDE_USE_FORCING | GLIDE_USE_MASK, P(L.r_u), P(L.r_v), P(L.r_H), X.st));
+        return sumsq3_async(CP(L.r_u), nU(L), CP(L.r_v), nV(L), CP(L.r_H), nC(L), slot, X.st);
     };
     auto cycle = [&]() -> int { return fas_vcycle(X, start_level, true); };
-    return solve_loop(X, start_level, report, resid, cycle);
+    return solve_loop(X, start_level, report, resid, cycle, 'f');
 }
 
 int glide_fascd_vcycle(const glide_level* levels, int n_levels, int level, int finest, double dt,
@@ -604,14 +707,29 @@ int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_l
     if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
     FasCtx X{levels, n_levels, dt, *cfg, vc, (hipStream_t)stream};
     const glide_level& L = levels[start_level];
-    hipStream_t st = X.st;
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));  // allocated before any capture
     auto resid = [&](bool, double* slot) -> int {
         GLIDE_TRY(vjp(L, dt, GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u), P(L.ar_v),
-                      P(L.ar_H), st));
-        return sumsq3_async(CP(L.ar_u), nU(L), CP(L.ar_v), nV(L), CP(L.ar_H), nC(L), slot, st);
+                      P(L.ar_H), X.st));
+        return sumsq3_async(CP(L.ar_u), nU(L), CP(L.ar_v), nV(L), CP(L.ar_H), nC(L), slot, X.st);
     };
     auto cycle = [&]() -> int { return adj_vcycle(X, start_level, true); };
-    return solve_loop(X, start_level, report, resid, cycle);
+    return solve_loop(X, start_level, report, resid, cycle, 'a');
+}
+
+int glide_graphs_enable(int on) {
+    g_graphs_on = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_graphs_clear(void) {
+    std::lock_guard<std::mutex> lk(g_graph_mutex);
+    for (auto& G : g_graphs) {
+        for (auto& e : G.entries) drop_entry(e);
+        G.entries.clear();
+    }
+    return GLIDE_OK;
 }
 
 int glide_timing_enable(int on) {

@@ -63,6 +63,8 @@ hipError_t launch_prolong_cell(const T* c, T* f, int nyf, int nxf, int bilinear,
 template <typename T>
 hipError_t launch_axpby(T* out, T a, const T* x, T b, const T* y, T c, const T* z, long n, hipStream_t st);
 template <typename T>
+hipError_t launch_copy(T* out, const T* x, long n, hipStream_t st);
+template <typename T>
 hipError_t launch_fill(T* out, T val, long n, hipStream_t st);
 // partial sums of squares of up to three arrays into `partials` (nblk per
 // array), then a single-block final pass into out3 (device).

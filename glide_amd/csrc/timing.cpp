@@ -1,20 +1,18 @@
-// Launch timing records (see timing.hpp).  Host code only; the records are
-// read back, and the events returned to a pool, by timing_read/timing_clear.
+// Launch timing records (see timing.hpp).  Host code only.
 #include "timing.hpp"
-
-#include <vector>
 
 namespace glide {
 
 namespace {
 
-struct TimingRec {
-    hipEvent_t start, stop;
-    long units;
+struct Totals {
+    long count = 0;
+    double ms = 0.0, units = 0.0;
 };
 
 bool g_timing_on = false;
-std::vector<TimingRec> g_timing[kTimedKernels];
+std::vector<TimingPair> g_pending;  // launches not yet read
+Totals g_totals[kTimedKernels];
 std::vector<hipEvent_t> g_event_pool;
 
 hipEvent_t pooled_event() {
@@ -26,6 +24,19 @@ hipEvent_t pooled_event() {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
+}
+
+hipError_t add_pair(const TimingPair& r) {
+    hipError_t e = hipEventSynchronize(r.stop);
+    if (e != hipSuccess) return e;
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, r.start, r.stop);
+    if (e != hipSuccess) return e;
+    Totals& t = g_totals[r.id];
+    t.ms += ms;
+    t.units += (double)r.units;
+    ++t.count;
+    return hipSuccess;
 }
 
 }  // namespace
@@ -40,8 +51,10 @@ TimedLaunch::TimedLaunch(int id_, long units_, hipStream_t st_) : id(id_), units
 TimedLaunch::~TimedLaunch() {
     if (!start || !stop) return;
     hipEventRecord(stop, st);
-    g_timing[id].push_back({start, stop, units});
+    g_pending.push_back({id, units, start, stop});
 }
+
+bool timing_on() { return g_timing_on; }
 
 void timing_enable(bool on) {
     timing_clear();
@@ -49,13 +62,12 @@ void timing_enable(bool on) {
 }
 
 void timing_clear() {
-    for (auto& v : g_timing) {
-        for (auto& r : v) {
-            g_event_pool.push_back(r.start);
-            g_event_pool.push_back(r.stop);
-        }
-        v.clear();
+    for (auto& r : g_pending) {
+        g_event_pool.push_back(r.start);
+        g_event_pool.push_back(r.stop);
     }
+    g_pending.clear();
+    for (auto& t : g_totals) t = Totals{};
 }
 
 hipError_t timing_read(int id, long* count, double* total_ms, double* total_units) {
@@ -63,16 +75,16 @@ hipError_t timing_read(int id, long* count, double* total_ms, double* total_unit
     *total_ms = 0.0;
     *total_units = 0.0;
     if (id < 0 || id >= kTimedKernels) return hipErrorInvalidValue;
-    for (auto& r : g_timing[id]) {
-        hipError_t e = hipEventSynchronize(r.stop);
+    for (auto& r : g_pending) {
+        hipError_t e = add_pair(r);
         if (e != hipSuccess) return e;
-        float ms = 0.f;
-        e = hipEventElapsedTime(&ms, r.start, r.stop);
-        if (e != hipSuccess) return e;
-        *total_ms += ms;
-        *total_units += (double)r.units;
-        ++*count;
+        g_event_pool.push_back(r.start);
+        g_event_pool.push_back(r.stop);
     }
+    g_pending.clear();
+    *count = g_totals[id].count;
+    *total_ms = g_totals[id].ms;
+    *total_units = g_totals[id].units;
     return hipSuccess;
 }
 

@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace glide {
 
 enum TimedKernel {
@@ -20,10 +22,16 @@ enum TimedKernel {
     kTimedKernels
 };
 
+struct TimingPair {
+    int id;
+    long units;
+    hipEvent_t start, stop;
+};
+
+bool timing_on();
 void timing_enable(bool on);
 void timing_clear();
 hipError_t timing_read(int id, long* count, double* total_ms, double* total_units);
-
 struct TimedLaunch {
     int id;
     long units;

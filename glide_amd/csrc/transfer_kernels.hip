@@ -111,6 +111,12 @@ __global__ __launch_bounds__(256) void k_axpby(T* __restrict__ out, T a, const T
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_copy(T* __restrict__ out, const T* __restrict__ x, long n) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = x[k];
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_fill(T* __restrict__ out, T val, long n) {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) out[k] = val;
@@ -191,6 +197,12 @@ hipError_t launch_axpby(T* out, T a, const T* x, T b, const T* y, T c, const T* 
     return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_copy(T* out, const T* x, long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_copy<T><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(out, x, n);
+    return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_fill(T* out, T val, long n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_fill<T><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(out, val, n);
@@ -218,6 +230,7 @@ hipError_t launch_sumsq3(const T* a, long na, const T* b, long nb, const T* c, l
     template hipError_t launch_prolong_cell<T>(const T*, T*, int, int, int, hipStream_t);                  \
     template hipError_t launch_axpby<T>(T*, T, const T*, T, const T*, T, const T*, long, hipStream_t);     \
     template hipError_t launch_fill<T>(T*, T, long, hipStream_t);                                          \
+    template hipError_t launch_copy<T>(T*, const T*, long, hipStream_t);                                   \
     template hipError_t launch_sumsq3<T>(const T*, long, const T*, long, const T*, long, double*, double*, \
                                          hipStream_t);
 

@@ -166,6 +166,17 @@ int glide_fas_adjoint_vcycle(const glide_level* levels, int n_levels, int level,
 /* which: bit 0 beta, bit 1 bed, bit 2 H_prev, bit 3 smb. */
 int glide_gradients(const glide_level* L, double dt, int which, void* stream);
 
+/* Graph replay of fixed-cycle solves (no counterpart in the reference; the
+ * reference launches every kernel from Python): when both tolerances are <= 0,
+ * glide_fascd_solve / glide_fas_adjoint_solve capture their launches into a
+ * HIP graph on first use and replay it on later calls with identical level
+ * descriptors, configs and dt (buffer contents may change freely).  On by
+ * default; glide_graphs_clear drops the cached graphs (call it before freeing
+ * buffers that a cached graph refers to if the addresses could be reused by
+ * unrelated data). */
+int glide_graphs_enable(int on);
+int glide_graphs_clear(void);
+
 /* Measurement support (no counterpart in the reference): when enabled, HIP
  * event pairs are recorded around every launch of the hot kernels
  * (0 forward patch solve, 1 patch increment apply, 2 adjoint patch solve,
