@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--nz", type=int, default=41, help="sigma levels of the enthalpy columns (0: no thermal step)")
     ap.add_argument("--n-smooth", type=int, default=10, help="maximum enthalpy sweeps per step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", dest="mixed", action="store_false",
+                    help="skip the mixed-precision run (FP32 patch Jacobian) and its error against FP64")
     ap.add_argument("--cpu-sample-n", type=int, default=128,
                     help="grid side of the oracle sample (same level count and cycle counts)")
     return ap.parse_args()
@@ -80,6 +82,49 @@ def build_model(args, torch):
         Ts, qg = thermal_forcing(p)
         thermal.initialize(torch.as_tensor(Ts, device="cuda"), Q_geo=torch.as_tensor(qg, device="cuda"))
     return model, thermal
+
+
+def run_timed(model, thermal, args, torch):
+    """Warm-up steps, then the timed steps on the production path (fixed-cycle
+    solves replayed from the library's cached HIP graphs); returns (t, seconds)."""
+    t = 0.0
+    for _ in range(args.warmup):
+        one_step(model, thermal, t, args.dt)
+        t += args.dt
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_step(model, thermal, t, args.dt)
+        t += args.dt
+    torch.cuda.synchronize()
+    return t, time.perf_counter() - t0
+
+
+def snapshot(model, thermal):
+    g = model.mg[0]
+    out = {"u": g.state.u.data, "v": g.state.v.data, "H": g.state.H.data, "lambda_u": g.adjoint.lambda_u.data,
+           "lambda_v": g.adjoint.lambda_v.data, "lambda_H": g.adjoint.lambda_H.data, "grad_beta": g.sliding.beta.grad}
+    if thermal is not None:
+        out["E"] = thermal.ops.enthalpy_state.E
+    return {k: v.detach().clone() for k, v in out.items()}
+
+
+def rel_l2(a, b):
+    den = float(b.norm())
+    return float((a - b).norm()) / den if den > 0 else float((a - b).norm())
+
+
+def read_traffic(args, kernel):
+    """HBM bytes per finest-level launch of the dominant pass from the committed
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated); None when the
+    summary does not cover this workload."""
+    path = ROOT / "profiles" / "vanka_pass_512_hbm_traffic.md"
+    if kernel != "vanka_sweep" or args.n != 512 or not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def one_step(model, thermal, t, dt):
@@ -155,19 +200,21 @@ def main():
         sys.exit(1)
     _lib.load()
     model, thermal = build_model(args, torch)
-    t = 0.0
-    for _ in range(args.warmup):
-        one_step(model, thermal, t, args.dt)
-        t += args.dt
-    torch.cuda.synchronize()
-    # throughput: the production path (fixed-cycle solves replayed from the
-    # library's cached HIP graphs)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_step(model, thermal, t, args.dt)
-        t += args.dt
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    t, elapsed = run_timed(model, thermal, args, torch)
+    fp64_state = snapshot(model, thermal)
+    mixed = None
+    if args.mixed:
+        # the mixed form (FP32 patch Jacobian/LU in the forward smoother) on a
+        # fresh copy of the workload, same steps; error against the FP64 state
+        m2, th2 = build_model(args, torch)
+        m2.forward_solver.vanka_options.set(local_fp32=True)
+        _, el2 = run_timed(m2, th2, args, torch)
+        errs = {k: rel_l2(a, b) for (k, a), b in zip(snapshot(m2, th2).items(), fp64_state.values())}
+        mixed = {"value": round(args.steps / el2, 4), "ms_per_step": round(1000.0 * el2 / args.steps, 3),
+                 "speedup_vs_fp64": round(elapsed / el2, 4), "rel_l2_err_vs_fp64": errs,
+                 "form": "forward patch Jacobian and LU in FP32; residuals, stopping tests, updates, transfers, "
+                         "adjoint and enthalpy in FP64"}
+        del m2, th2
     # per-kernel durations: the same number of steps again with HIP events
     # around every hot launch (events inside graphs cannot be timed, so these
     # steps launch eagerly; kernel bodies are identical)
@@ -199,12 +246,20 @@ def main():
         "vjp": kern["vjp"],
     }
     dominant = max(passes, key=lambda k: passes[k][1])
-    launches, dur_ms, cells = passes[dominant]
-    achieved = bw(ALG_FIELDS[dominant], dur_ms, cells)
+    # roofline of the dominant pass over the finest-level launches (args.n^2 cells each)
+    fine = args.n * args.n
+    pair = {"vanka_sweep": ("vanka_cell", "vanka_apply"), "adjoint_sweep": ("adjoint_cell", "adjoint_apply"),
+            "vjp": ("vjp",)}[dominant]
+    f_launches = _lib.timing_read_units(pair[0], fine)[0]
+    f_ms = sum(_lib.timing_read_units(k, fine)[1] for k in pair)
+    avg_ms = f_ms / max(f_launches, 1)
+    alg_bytes = ALG_FIELDS[dominant] * WORD * fine
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": dominant, "launches": launches, "avg_launch_us": round(1000.0 * dur_ms / max(launches, 1), 3),
-                "alg_bytes_per_cell": ALG_FIELDS[dominant] * WORD}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(args, dominant),
+                "kernel": dominant, "level": f"{args.n}x{args.n}", "launches": f_launches,
+                "avg_launch_us": round(1000.0 * avg_ms, 3), "alg_bytes_per_launch": alg_bytes,
+                "all_levels_alg_GBps": round(bw(ALG_FIELDS[dominant], passes[dominant][1], passes[dominant][2]), 2)}
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "cells": v[2]} for k, v in kern.items()}
     for k, (c, ms, cells) in passes.items():
         kernels[k] = {"launches": c, "total_ms": round(ms, 3), "alg_GBps": round(bw(ALG_FIELDS[k], ms, cells), 2)}
@@ -230,7 +285,7 @@ def main():
                        "sigma_levels": args.nz,
                        "precision": "plain FP64 form", "grid": f"{args.n}x{args.n}", "levels": args.levels,
                        "vcycles": args.vcycles, "dt_years": args.dt, "parallelism": "single GPU"},
-            "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels,
+            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "kernels": kernels,
             "ms_per_step_eager_timed": round(1000.0 * elapsed_eager / args.steps, 3),
             "omp_threads": os.environ.get("OMP_NUM_THREADS")}
     print(json.dumps(line), flush=True)

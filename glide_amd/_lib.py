@@ -51,7 +51,8 @@ class Level(ctypes.Structure):
 
 class VankaConfigC(ctypes.Structure):
     _fields_ = [("omega", ctypes.c_double), ("relax_phi", ctypes.c_double), ("newton_steps", ctypes.c_int32),
-                ("relaxation", ctypes.c_double), ("ssa_damping", ctypes.c_double), ("mc_damping", ctypes.c_double)]
+                ("relaxation", ctypes.c_double), ("ssa_damping", ctypes.c_double), ("mc_damping", ctypes.c_double),
+                ("local_fp32", ctypes.c_int32)]
 
 
 class FasConfigC(ctypes.Structure):
@@ -133,6 +134,7 @@ SIGNATURES = {
     "glide_graphs_clear": (_I, []),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "glide_timing_read_units": (_I, [_I, ctypes.c_long, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D)]),
     # include/glide_enthalpy.h
     "glide_enth_residual": (_I, [_CP, _D, _I, ctypes.POINTER(_D), _P]),
     "glide_enth_column_smooth": (_I, [_CP, _D, _I, _D, _P]),
@@ -168,6 +170,13 @@ def graphs_clear() -> None:
 
 def timing_enable(on: bool) -> None:
     call("glide_timing_enable", int(bool(on)))
+
+
+def timing_read_units(kernel: str, units: int):
+    """(launch count, summed ms) of a timed kernel's launches over exactly `units` cells."""
+    c, ms = ctypes.c_long(0), ctypes.c_double(0.0)
+    call("glide_timing_read_units", TIMED_KERNELS[kernel], int(units), ctypes.byref(c), ctypes.byref(ms))
+    return c.value, ms.value
 
 
 def timing_read(kernel: str):

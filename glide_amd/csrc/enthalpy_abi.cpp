@@ -131,17 +131,24 @@ int make_args(const glide_enth_column* C, double dt, bool needs_state, EnthArgs<
     a->Q_geo = CP(C->Q_geo);
     a->phi_strain = CP(C->phi_strain);
     a->skip = nullptr;
+    a->u2d = nullptr;
+    a->v2d = nullptr;
+    a->vel_div = 1.0;
     return GLIDE_OK;
 }
 
 int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
-               glide_enth_report* report, hipStream_t st) {
+               glide_enth_report* report, hipStream_t st, const R* u2d = nullptr, const R* v2d = nullptr,
+               double vel_div = 1.0) {
     if (!cfg) return fail(GLIDE_ERR_INVALID_ARG, "sweep config is NULL");
     if (n_iter < 0 || n_iter > GLIDE_ENTH_MAX_SWEEPS)
         return fail(GLIDE_ERR_INVALID_ARG, "n_iter must be in [0, GLIDE_ENTH_MAX_SWEEPS]");
     if (cfg->n_newton < 0) return fail(GLIDE_ERR_INVALID_ARG, "n_newton must be non-negative");
     EnthArgs<R> A;
     GLIDE_TRY(make_args(C, dt, true, &A));
+    A.u2d = u2d;
+    A.v2d = v2d;
+    A.vel_div = vel_div;
     REQ(C->r_E);
     if (cfg->alternating) REQ(C->E_work);
     else REQ(C->delta_E);
@@ -355,20 +362,26 @@ int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* 
     }
     hipStream_t st = S(stream);
     // glide/model.py:180-197: layer velocities in m/a, omega from the realised
-    // thickness change in m/a, then both converted to m/s.
-    GLIDE_TRY(glide_enth_broadcast_velocity(C, u2d, v2d, 1.0, stream));
+    // thickness change in m/a, then both converted to m/s.  The velocity is
+    // the same on every level, so the kernels read the SSA planes (divided as
+    // the broadcast is); u3d / v3d are still filled in m/s for the caller.
     const double dt_yr = tc->dt / kSecPerYear;
     HIP_TRY(launch_enth_tendency<R>(CP(C->H), CP(C->H_prev), dt_yr, P(C->dh_dt), (long)C->ny * C->nx, st),
             "thickness tendency");
-    HIP_TRY(launch_enth_omega<R>(A, CP(C->dh_dt), nullptr, kSecPerYear, P(C->omega), st), "enthalpy omega");
+    EnthArgs<R> A2 = A;
+    A2.u2d = CP(u2d);
+    A2.v2d = CP(v2d);
+    A2.vel_div = 1.0;
+    HIP_TRY(launch_enth_omega<R>(A2, CP(C->dh_dt), nullptr, kSecPerYear, P(C->omega), st), "enthalpy omega");
     GLIDE_TRY(glide_enth_broadcast_velocity(C, u2d, v2d, kSecPerYear, stream));
+    A2.vel_div = kSecPerYear;
     if (tc->strain_heating)
         GLIDE_TRY(glide_enth_strain_heating(C, u2d, v2d, B, tc->n_glen, tc->eps_reg, tc->rho_i, stream));
     // glide/model.py:199-203, glide/enthalpy.py:473-519
-    HIP_TRY(launch_enth_rhs<R>(A, CP(beta), tc->m, tc->u_reg, tc->frictional_heating ? 1 : 0, P(C->Q_fh), P(C->f_E),
-                               st),
+    HIP_TRY(launch_enth_rhs<R>(A2, CP(beta), tc->m, tc->u_reg, tc->frictional_heating ? 1 : 0, P(C->Q_fh),
+                               P(C->f_E), st),
             "enthalpy forcing");
-    GLIDE_TRY(sweep_impl(C, tc->dt, tc->n_smooth, sc, report, st));
+    GLIDE_TRY(sweep_impl(C, tc->dt, tc->n_smooth, sc, report, st, CP(u2d), CP(v2d), kSecPerYear));
     // glide/model.py:205-213
     if (tc->update_rheology) {
         const double scale = tc->rho_i * kGravity * std::pow(kSecPerYear, 1.0 / tc->n_glen);

@@ -84,23 +84,24 @@ __device__ __forceinline__ Horiz<T> load_horiz(const EnthArgs<T>& A, int i, int 
     const int ny = A.ny, nx = A.nx, nz = A.nz;
     Horiz<T> h;
     const T Hh = A.H[(long)i * nx + j];
-    const T* u = A.u3d + (long)k * ny * (nx + 1);
-    const T* v = A.v3d + (long)k * (ny + 1) * nx;
+    const T* u = A.u2d ? A.u2d : A.u3d + (long)k * ny * (nx + 1);
+    const T* v = A.u2d ? A.v2d : A.v3d + (long)k * (ny + 1) * nx;
+    const T div = A.u2d ? A.vel_div : T(1);
     h.Hu_l = h.Hu_r = h.Hv_t = h.Hv_b = T(0);
     if (j > 0) {
-        T ul = u[(long)i * (nx + 1) + j], Hm = A.H[(long)i * nx + j - 1];
+        T ul = u[(long)i * (nx + 1) + j] / div, Hm = A.H[(long)i * nx + j - 1];
         h.Hu_l = T(0.5) * (Hm + Hh) * ul - T(0.5) * sqrt(ul * ul + T(kMassFluxReg)) * (Hh - Hm);
     }
     if (j < nx - 1) {
-        T ur = u[(long)i * (nx + 1) + j + 1], Hp = A.H[(long)i * nx + j + 1];
+        T ur = u[(long)i * (nx + 1) + j + 1] / div, Hp = A.H[(long)i * nx + j + 1];
         h.Hu_r = T(0.5) * (Hh + Hp) * ur - T(0.5) * sqrt(ur * ur + T(kMassFluxReg)) * (Hp - Hh);
     }
     if (i > 0) {
-        T vt = v[(long)i * nx + j], Hm = A.H[(long)(i - 1) * nx + j];
+        T vt = v[(long)i * nx + j] / div, Hm = A.H[(long)(i - 1) * nx + j];
         h.Hv_t = T(0.5) * (Hm + Hh) * vt - T(0.5) * sqrt(vt * vt + T(kMassFluxReg)) * (Hm - Hh);
     }
     if (i < ny - 1) {
-        T vb = v[(long)(i + 1) * nx + j], Hp = A.H[(long)(i + 1) * nx + j];
+        T vb = v[(long)(i + 1) * nx + j] / div, Hp = A.H[(long)(i + 1) * nx + j];
         h.Hv_b = T(0.5) * (Hh + Hp) * vb - T(0.5) * sqrt(vb * vb + T(kMassFluxReg)) * (Hh - Hp);
     }
     auto col = [&](int ii, int jj) {
@@ -427,8 +428,8 @@ __global__ __launch_bounds__(256) void k_enth_omega(EnthArgs<T> A, const T* __re
         return;
     }
     if (k < nz) {
-        const T* u = A.u3d + (long)k * ny * (nx + 1);
-        const T* v = A.v3d + (long)k * (ny + 1) * nx;
+        const T* u = A.u2d ? A.u2d : A.u3d + (long)k * ny * (nx + 1);
+        const T* v = A.u2d ? A.v2d : A.v3d + (long)k * (ny + 1) * nx;
         T fr = 0, fl = 0, fb = 0, ft = 0;
         if (j < nx - 1) {
             T ur = u[(long)i * (nx + 1) + j + 1], Hp = A.H[q.col + 1];
@@ -451,15 +452,22 @@ __global__ __launch_bounds__(256) void k_enth_omega(EnthArgs<T> A, const T* __re
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (k == 0) {
+    if (k == 0) {  // the upward sum in level order, in place over the divergences
         const T dsig = T(1) / T(nz - 1), dh = dhdt[q.col];
         T o = bmb ? -bmb[q.col] : T(0);
-        omega[base] = o / out_div;
+        T prev = sdiv[w][0];
+        sdiv[w][0] = o;
         for (int m = 1; m < nz; ++m) {
-            o = o - dsig * (dh + T(0.5) * (sdiv[w][m - 1] + sdiv[w][m]));
-            omega[base + m] = o / out_div;
+            const T cur = sdiv[w][m];
+            o = o - dsig * (dh + T(0.5) * (prev + cur));
+            sdiv[w][m] = o;
+            prev = cur;
         }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (k < nz) omega[base + k] = sdiv[w][k] / out_div;
 }
 
 template <typename T>
@@ -490,10 +498,11 @@ __global__ __launch_bounds__(256) void k_enth_rhs(EnthArgs<T> A, const T* __rest
     const long base = q.col * nz;
     T qfh;
     if (frictional) {
-        const T* ub = A.u3d;
-        const T* vb = A.v3d;
-        T uc = T(0.5) * (ub[(long)q.i * (nx + 1) + q.j + 1] + ub[(long)q.i * (nx + 1) + q.j]);
-        T vc = T(0.5) * (vb[(long)(q.i + 1) * nx + q.j] + vb[(long)q.i * nx + q.j]);
+        const T* ub = A.u2d ? A.u2d : A.u3d;
+        const T* vb = A.u2d ? A.v2d : A.v3d;
+        const T div = A.u2d ? A.vel_div : T(1);
+        T uc = T(0.5) * (ub[(long)q.i * (nx + 1) + q.j + 1] / div + ub[(long)q.i * (nx + 1) + q.j] / div);
+        T vc = T(0.5) * (vb[(long)(q.i + 1) * nx + q.j] / div + vb[(long)q.i * nx + q.j] / div);
         T speed = sqrt(uc * uc + vc * vc + u_reg * u_reg);
         qfh = beta[q.col] * pow(speed, m + T(1));
         if (k == 0) Q_fh[q.col] = qfh;

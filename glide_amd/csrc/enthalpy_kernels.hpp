@@ -16,6 +16,11 @@ struct EnthArgs {
     T lf_sq;  // sqrt(1 + lf_c), the Lax-Friedrichs dissipation factor
     const T *E, *E_prev, *f_E, *u3d, *v3d, *omega, *H, *H_prev, *E_surface, *Q_geo, *phi_strain;
     const int* skip;  // nullable: a launch does nothing while *skip != 0
+    // Depth-constant velocity (the coupled step's broadcast SSA velocity):
+    // when u2d is set, level k's velocity is u2d / vel_div, the same value the
+    // broadcast u3d[k] holds, read from one (ny, nx+1) / (ny+1, nx) plane.
+    const T *u2d, *v2d;
+    T vel_div;
 };
 
 // Device record of a fused column sweep (EnthalpyOperators.column_sweep).

@@ -158,6 +158,7 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
     vk.mc_damping = vc.mc_damping;
     vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
     vk.iters = nullptr;
+    vk.local_fp32 = vc.local_fp32;
     for (int it = 0; it < n_iter; ++it) {
         if (!freeze_phi) GLIDE_TRY(grounded(L, vc.relax_phi, st));
         SsaArgs<R> a = make_args(L, dt, false);
@@ -185,6 +186,7 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
     vk.mc_damping = vc.mc_damping;
     vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
     vk.iters = nullptr;
+    vk.local_fp32 = 0;
     for (int it = 0; it < n_iter; ++it) {
         GLIDE_TRY(vjp(L, dt, flags | GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u),
                       P(L.ar_v), P(L.ar_H), st));
@@ -740,6 +742,13 @@ int glide_timing_enable(int on) {
 int glide_timing_read(int kernel, long* count, double* total_ms, double* total_cells) {
     if (!count || !total_ms || !total_cells) return fail(GLIDE_ERR_INVALID_ARG, "NULL output");
     HIP_TRY(timing_read(kernel, count, total_ms, total_cells), "timing_read");
+    return GLIDE_OK;
+}
+
+int glide_timing_read_units(int kernel, long units, long* count, double* total_ms) {
+    if (!count || !total_ms) return fail(GLIDE_ERR_INVALID_ARG, "NULL output");
+    double u = 0.0;
+    HIP_TRY(timing_read(kernel, count, total_ms, &u, units), "timing_read");
     return GLIDE_OK;
 }
 

@@ -507,19 +507,22 @@ __device__ __forceinline__ void make_cell_sys(CellSys<T>& s, const Patch<T>& p, 
 // 1/sqrt(s) and 1/a to within an ulp or two: the hardware estimates refined
 // by two Newton steps each (for the flux speeds, s >= 10, and the LU pivots,
 // which are diagonally dominated and far from denormal).
-__device__ __forceinline__ double rsqrt_refined(double s) {
+__device__ __forceinline__ double fast_rsqrt(double s) {
     double y = __builtin_amdgcn_rsq(s);
     const double h = 0.5 * s;
     y = y * (1.5 - h * y * y);
     y = y * (1.5 - h * y * y);
     return y;
 }
-__device__ __forceinline__ double rcp_refined(double a) {
+__device__ __forceinline__ double fast_rcp(double a) {
     double y = __builtin_amdgcn_rcp(a);
     y = fma(y, fma(-a, y, 1.0), y);
     y = fma(y, fma(-a, y, 1.0), y);
     return y;
 }
+// FP32: the hardware estimates are already within an ulp or so of FP32.
+__device__ __forceinline__ float fast_rsqrt(float s) { return __builtin_amdgcn_rsqf(s); }
+__device__ __forceinline__ float fast_rcp(float a) { return __builtin_amdgcn_rcpf(a); }
 
 // Drag power of the nonlinear law at facet velocity w with cross velocities c0..c3.
 template <typename T>
@@ -529,87 +532,111 @@ __device__ __forceinline__ void drag_power(T w, T c0, T c1, T c2, T c3, const Ph
     dpw = P.drag_q * pow(s2 + P.u_reg, P.drag_q - T(1));
 }
 
-// J(x) and r(x) - f of the cell with the Vanka dampings (before boundary /
-// active-set pinning).  LINEAR: the drag law is linear (m == 1, u_reg > 0).
-template <typename T, bool LINEAR>
-__device__ __forceinline__ void cell_system(const CellSys<T>& s, const T x[5], T J[25], T r[5], const Phys<T>& P) {
-    const T ul = x[0], ur = x[1], vt = x[2], vb = x[3], Hc = x[4];
-    const T D = s.D;
-    // centre cell normal stresses
-    const T K = s.kH * Hc;
-    const T du = ur - ul, dv = vt - vb;
-    const T exx = (T(2) * du + dv) * D, eyy = (du + T(2) * dv) * D;
-    const T sx = s.eD * Hc * exx, sy = s.eD * Hc * eyy;    // 2 eta_c H_c e D
-    const T jx = s.eD * exx, jy = s.eD * eyy;               // their H_c derivatives
-    // shear vertices
-    const T eH_tl = s.C_tl + s.q_tl * Hc, eH_tr = s.C_tr + s.q_tr * Hc;
-    const T eH_bl = s.C_bl + s.q_bl * Hc, eH_br = s.C_br + s.q_br * Hc;
-    const T e_tl = s.s_tl + s.hD * (vt - ul), e_tr = s.s_tr - s.hD * (ur + vt);
-    const T e_bl = s.s_bl + s.hD * (ul + vb), e_br = s.s_br + s.hD * (ur - vb);
-    const T t_tl = T(2) * D * eH_tl * e_tl, t_tr = T(2) * D * eH_tr * e_tr;
-    const T t_bl = T(2) * D * eH_bl * e_bl, t_br = T(2) * D * eH_br * e_br;
-    const T g_tl = T(2) * D * s.q_tl * e_tl, g_tr = T(2) * D * s.q_tr * e_tr;
-    const T g_bl = T(2) * D * s.q_bl * e_bl, g_br = T(2) * D * s.q_br * e_br;
-    // driving stress
-    const T Sc = s.Zc + s.gc * Hc;
-    const T Hm_l = T(0.5) * (s.H_l + Hc), Hm_r = T(0.5) * (Hc + s.H_r);
-    const T Hm_t = T(0.5) * (s.H_t + Hc), Hm_b = T(0.5) * (Hc + s.H_b);
-    const T dS_l = Sc - s.S_l, dS_r = s.S_r - Sc, dS_t = s.S_t - Sc, dS_b = Sc - s.S_b;
+// The coefficients of a cell system in another precision (the Jacobian of
+// the mixed form is evaluated and factorised in FP32).
+template <typename TJ, typename T>
+__device__ __forceinline__ void cast_cell_sys(CellSys<TJ>& q, const CellSys<T>& s) {
+    const T* src = &s.D;
+    TJ* dst = &q.D;
+    constexpr int n = (int)(sizeof(CellSys<T>) / sizeof(T));
+#pragma unroll
+    for (int k = 0; k < n; ++k) dst[k] = TJ(src[k]);
+}
 
-    r[0] = sx + s.c0 + s.o0 * ul + t_tl - t_bl - Hm_l * dS_l * s.dL;
-    r[1] = -sx + s.c1 + s.o1 * ur + t_tr - t_br - Hm_r * dS_r * s.dR;
-    r[2] = -sy + s.c2 + s.o2 * vt - t_tl + t_tr - Hm_t * dS_t * s.dT;
-    r[3] = sy + s.c3 + s.o3 * vb - t_bl + t_br - Hm_b * dS_b * s.dB;
-
-    const T K2 = T(2) * K;
-    J[0] = s.a0 + s.h0 * Hc - K2; J[1] = K2;  J[2] = K;   J[3] = -K;
-    J[5] = K2;  J[6] = s.a1 + s.h1 * Hc - K2; J[7] = -K;  J[8] = K;
-    J[10] = K;  J[11] = -K; J[12] = s.a2 + s.h2 * Hc - K2; J[13] = K2;
-    J[15] = -K; J[16] = K;  J[17] = K2; J[18] = s.a3 + s.h3 * Hc - K2;
-    J[4] = jx + g_tl - g_bl - (T(0.5) * dS_l + Hm_l * s.gc) * s.dL;
-    J[9] = -jx + g_tr - g_br - (T(0.5) * dS_r - Hm_r * s.gc) * s.dR;
-    J[14] = -jy - g_tl + g_tr - (T(0.5) * dS_t - Hm_t * s.gc) * s.dT;
-    J[19] = jy - g_bl + g_br - (T(0.5) * dS_b + Hm_b * s.gc) * s.dB;
+// r(x) - f (precision T) and J(x) (precision TJ) of the cell with the Vanka
+// dampings, before boundary / active-set pinning.  q holds the coefficients
+// of s in TJ (the same object when TJ == T).  LINEAR: the drag law is
+// linear (m == 1, u_reg > 0).
+template <typename T, typename TJ, bool LINEAR>
+__device__ __forceinline__ void cell_system(const CellSys<T>& s, const CellSys<TJ>& q, const T x[5], TJ J[25],
+                                            T r[5], const Phys<T>& P) {
+    // ------------------------------------------------------------ residual
+    {
+        const T ul = x[0], ur = x[1], vt = x[2], vb = x[3], Hc = x[4];
+        const T D = s.D;
+        const T du = ur - ul, dv = vt - vb;
+        const T exx = (T(2) * du + dv) * D, eyy = (du + T(2) * dv) * D;
+        const T sx = s.eD * Hc * exx, sy = s.eD * Hc * eyy;    // 2 eta_c H_c e D
+        const T eH_tl = s.C_tl + s.q_tl * Hc, eH_tr = s.C_tr + s.q_tr * Hc;
+        const T eH_bl = s.C_bl + s.q_bl * Hc, eH_br = s.C_br + s.q_br * Hc;
+        const T e_tl = s.s_tl + s.hD * (vt - ul), e_tr = s.s_tr - s.hD * (ur + vt);
+        const T e_bl = s.s_bl + s.hD * (ul + vb), e_br = s.s_br + s.hD * (ur - vb);
+        const T t_tl = T(2) * D * eH_tl * e_tl, t_tr = T(2) * D * eH_tr * e_tr;
+        const T t_bl = T(2) * D * eH_bl * e_bl, t_br = T(2) * D * eH_br * e_br;
+        const T Sc = s.Zc + s.gc * Hc;
+        const T Hm_l = T(0.5) * (s.H_l + Hc), Hm_r = T(0.5) * (Hc + s.H_r);
+        const T Hm_t = T(0.5) * (s.H_t + Hc), Hm_b = T(0.5) * (Hc + s.H_b);
+        r[0] = sx + s.c0 + s.o0 * ul + t_tl - t_bl - Hm_l * (Sc - s.S_l) * s.dL;
+        r[1] = -sx + s.c1 + s.o1 * ur + t_tr - t_br - Hm_r * (s.S_r - Sc) * s.dR;
+        r[2] = -sy + s.c2 + s.o2 * vt - t_tl + t_tr - Hm_t * (s.S_t - Sc) * s.dT;
+        r[3] = sy + s.c3 + s.o3 * vb - t_bl + t_br - Hm_b * (Sc - s.S_b) * s.dB;
+        const T sl = ul * ul + T(10), sr = ur * ur + T(10), st = vt * vt + T(10), sb = vb * vb + T(10);
+        const T a_l = sl * fast_rsqrt(sl), a_r = sr * fast_rsqrt(sr);   // sqrt(w^2 + 10)
+        const T a_t = st * fast_rsqrt(st), a_b = sb * fast_rsqrt(sb);
+        const T Fl = Hm_l * ul - T(0.5) * a_l * (Hc - s.H_l);
+        const T Fr = Hm_r * ur - T(0.5) * a_r * (s.H_r - Hc);
+        const T Ft = Hm_t * vt - T(0.5) * a_t * (s.H_t - Hc);
+        const T Fb = Hm_b * vb - T(0.5) * a_b * (Hc - s.H_b);
+        r[4] = s.r4h * Hc + (Fr * s.dR - Fl * s.dL) + (Ft * s.dT - Fb * s.dB) - s.f4;
+    }
+    // ------------------------------------------------------------ Jacobian
+    const TJ ul = TJ(x[0]), ur = TJ(x[1]), vt = TJ(x[2]), vb = TJ(x[3]), Hc = TJ(x[4]);
+    const TJ D = q.D;
+    const TJ K = q.kH * Hc;
+    const TJ du = ur - ul, dv = vt - vb;
+    const TJ jx = q.eD * ((TJ(2) * du + dv) * D), jy = q.eD * ((du + TJ(2) * dv) * D);
+    const TJ e_tl = q.s_tl + q.hD * (vt - ul), e_tr = q.s_tr - q.hD * (ur + vt);
+    const TJ e_bl = q.s_bl + q.hD * (ul + vb), e_br = q.s_br + q.hD * (ur - vb);
+    const TJ g_tl = TJ(2) * D * q.q_tl * e_tl, g_tr = TJ(2) * D * q.q_tr * e_tr;
+    const TJ g_bl = TJ(2) * D * q.q_bl * e_bl, g_br = TJ(2) * D * q.q_br * e_br;
+    const TJ Sc = q.Zc + q.gc * Hc;
+    const TJ Hm_l = TJ(0.5) * (q.H_l + Hc), Hm_r = TJ(0.5) * (Hc + q.H_r);
+    const TJ Hm_t = TJ(0.5) * (q.H_t + Hc), Hm_b = TJ(0.5) * (Hc + q.H_b);
+    const TJ dS_l = Sc - q.S_l, dS_r = q.S_r - Sc, dS_t = q.S_t - Sc, dS_b = Sc - q.S_b;
+    const TJ K2 = TJ(2) * K;
+    J[0] = q.a0 + q.h0 * Hc - K2; J[1] = K2;  J[2] = K;   J[3] = -K;
+    J[5] = K2;  J[6] = q.a1 + q.h1 * Hc - K2; J[7] = -K;  J[8] = K;
+    J[10] = K;  J[11] = -K; J[12] = q.a2 + q.h2 * Hc - K2; J[13] = K2;
+    J[15] = -K; J[16] = K;  J[17] = K2; J[18] = q.a3 + q.h3 * Hc - K2;
+    J[4] = jx + g_tl - g_bl - (TJ(0.5) * dS_l + Hm_l * q.gc) * q.dL;
+    J[9] = -jx + g_tr - g_br - (TJ(0.5) * dS_r - Hm_r * q.gc) * q.dR;
+    J[14] = -jy - g_tl + g_tr - (TJ(0.5) * dS_t - Hm_t * q.gc) * q.dT;
+    J[19] = jy - g_bl + g_br - (TJ(0.5) * dS_b + Hm_b * q.gc) * q.dB;
+    const TJ sl = ul * ul + TJ(10), sr = ur * ur + TJ(10), st = vt * vt + TJ(10), sb = vb * vb + TJ(10);
+    const TJ il = fast_rsqrt(sl), ir = fast_rsqrt(sr), it = fast_rsqrt(st), ib = fast_rsqrt(sb);
+    const TJ a_l = sl * il, a_r = sr * ir, a_t = st * it, a_b = sb * ib;
+    J[20] = -(Hm_l - TJ(0.5) * (ul * il) * (Hc - q.H_l)) * q.dL;
+    J[21] = (Hm_r - TJ(0.5) * (ur * ir) * (q.H_r - Hc)) * q.dR;
+    J[22] = (Hm_t - TJ(0.5) * (vt * it) * (q.H_t - Hc)) * q.dT;
+    J[23] = -(Hm_b - TJ(0.5) * (vb * ib) * (Hc - q.H_b)) * q.dB;
+    J[24] = q.j24c + TJ(0.5) * ((ur + a_r) * q.dR - (ul - a_l) * q.dL + (vt + a_t) * q.dT - (vb - a_b) * q.dB);
 
     if (!LINEAR) {
-        // glide/cuda/stress.cu basal drag with the 4-neighbour cross velocity
-        T pw, dpw, k;
-        drag_power(ul, s.v_lt, vt, s.v_lb, vb, P, pw, dpw);
-        r[0] -= (s.be_l * pw + P.water_drag) * ul;
-        J[0] -= s.be_l * (T(2) * dpw * ul * ul + pw) + P.water_drag;
-        k = -s.be_l * T(0.5) * dpw * ul;
+        // glide/cuda/stress.cu basal drag with the 4-neighbour cross velocity:
+        // residual terms in T, their derivatives in TJ
+        const T xl = x[0], xr = x[1], xt = x[2], xb = x[3];
+        T pw, dpw;
+        drag_power(xl, s.v_lt, xt, s.v_lb, xb, P, pw, dpw);
+        r[0] -= (s.be_l * pw + P.water_drag) * xl;
+        J[0] -= TJ(s.be_l * (T(2) * dpw * xl * xl + pw) + P.water_drag);
+        TJ k = TJ(-s.be_l * T(0.5) * dpw * xl);
         J[2] += k * vt; J[3] += k * vb;
-        drag_power(ur, vt, s.v_rt, vb, s.v_rb, P, pw, dpw);
-        r[1] -= (s.be_r * pw + P.water_drag) * ur;
-        J[6] -= s.be_r * (T(2) * dpw * ur * ur + pw) + P.water_drag;
-        k = -s.be_r * T(0.5) * dpw * ur;
+        drag_power(xr, xt, s.v_rt, xb, s.v_rb, P, pw, dpw);
+        r[1] -= (s.be_r * pw + P.water_drag) * xr;
+        J[6] -= TJ(s.be_r * (T(2) * dpw * xr * xr + pw) + P.water_drag);
+        k = TJ(-s.be_r * T(0.5) * dpw * xr);
         J[7] += k * vt; J[8] += k * vb;
-        drag_power(vt, s.u_tl, s.u_tr, ul, ur, P, pw, dpw);
-        r[2] -= (s.be_t * pw + P.water_drag) * vt;
-        J[12] -= s.be_t * (T(2) * dpw * vt * vt + pw) + P.water_drag;
-        k = -s.be_t * T(0.5) * dpw * vt;
+        drag_power(xt, s.u_tl, s.u_tr, xl, xr, P, pw, dpw);
+        r[2] -= (s.be_t * pw + P.water_drag) * xt;
+        J[12] -= TJ(s.be_t * (T(2) * dpw * xt * xt + pw) + P.water_drag);
+        k = TJ(-s.be_t * T(0.5) * dpw * xt);
         J[10] += k * ul; J[11] += k * ur;
-        drag_power(vb, ul, ur, s.u_bl, s.u_br, P, pw, dpw);
-        r[3] -= (s.be_b * pw + P.water_drag) * vb;
-        J[18] -= s.be_b * (T(2) * dpw * vb * vb + pw) + P.water_drag;
-        k = -s.be_b * T(0.5) * dpw * vb;
+        drag_power(xb, xl, xr, s.u_bl, s.u_br, P, pw, dpw);
+        r[3] -= (s.be_b * pw + P.water_drag) * xb;
+        J[18] -= TJ(s.be_b * (T(2) * dpw * xb * xb + pw) + P.water_drag);
+        k = TJ(-s.be_b * T(0.5) * dpw * xb);
         J[15] += k * ul; J[16] += k * ur;
     }
-
-    // thickness row: dH/dt + div(q) + calving with Lax-Friedrichs facet fluxes
-    const T sl = ul * ul + T(10), sr = ur * ur + T(10), st = vt * vt + T(10), sb = vb * vb + T(10);
-    const T il = rsqrt_refined(sl), ir = rsqrt_refined(sr), it = rsqrt_refined(st), ib = rsqrt_refined(sb);
-    const T a_l = sl * il, a_r = sr * ir, a_t = st * it, a_b = sb * ib;   // sqrt(w^2 + 10)
-    const T Fl = Hm_l * ul - T(0.5) * a_l * (Hc - s.H_l);
-    const T Fr = Hm_r * ur - T(0.5) * a_r * (s.H_r - Hc);
-    const T Ft = Hm_t * vt - T(0.5) * a_t * (s.H_t - Hc);
-    const T Fb = Hm_b * vb - T(0.5) * a_b * (Hc - s.H_b);
-    r[4] = s.r4h * Hc + (Fr * s.dR - Fl * s.dL) + (Ft * s.dT - Fb * s.dB) - s.f4;
-    J[20] = -(Hm_l - T(0.5) * (ul * il) * (Hc - s.H_l)) * s.dL;
-    J[21] = (Hm_r - T(0.5) * (ur * ir) * (s.H_r - Hc)) * s.dR;
-    J[22] = (Hm_t - T(0.5) * (vt * it) * (s.H_t - Hc)) * s.dT;
-    J[23] = -(Hm_b - T(0.5) * (vb * ib) * (Hc - s.H_b)) * s.dB;
-    J[24] = s.j24c + T(0.5) * ((ur + a_r) * s.dR - (ul - a_l) * s.dL + (vt + a_t) * s.dT - (vb - a_b) * s.dB);
 }
 
 // lu5_solve with refined hardware reciprocals of the pivots, used for both
@@ -619,7 +646,7 @@ __device__ __forceinline__ void lu5_solve_rcp(T A[25], const T b[5], T x[5]) {
     T inv[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        inv[k] = rcp_refined(A[k * 5 + k]);
+        inv[k] = fast_rcp(A[k * 5 + k]);
 #pragma unroll
         for (int i = k + 1; i < 5; ++i) {
             A[i * 5 + k] *= inv[k];

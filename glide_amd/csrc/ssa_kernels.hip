@@ -15,6 +15,7 @@
 // identical.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "ssa_device.hpp"
@@ -210,7 +211,7 @@ __device__ __forceinline__ void fill_eta_tile(T (*s_eta)[TX + 2], const SsaArgs<
 // patch against the state at the start of the pass and writes its increments
 // (half-weighted for facets) into the workspace planes
 // work[0..4] = (du_l, du_r, dv_t, dv_b, dH); the active-set flag goes to mask.
-template <typename T, bool LINEAR>
+template <typename T, bool LINEAR, typename TJ>
 __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__ work, T* __restrict__ mask,
                                                    VankaArgs<T> vk) {
     __shared__ T s_eta[TY + 2][TX + 2];
@@ -229,6 +230,11 @@ __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__
         make_cell_sys(cs, p, a.P, vk.calving_rate, a.dxi, a.dt, vk.ssa_damping, vk.mc_damping);
         thk = p.gam;
     }
+    CellSys<TJ> cq_own;
+    if constexpr (!std::is_same<T, TJ>::value) cast_cell_sys(cq_own, cs);
+    const CellSys<TJ>* cq;
+    if constexpr (std::is_same<T, TJ>::value) cq = &cs;
+    else cq = &cq_own;
     const long c = (long)i * nx + j;
     T x[5] = {a.u[(long)i * (nx + 1) + j], a.u[(long)i * (nx + 1) + j + 1], a.v[c], a.v[c + nx], a.H[c]};
     T cmp[5];
@@ -240,8 +246,9 @@ __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__
     const bool pin0 = j == 0, pin1 = j == nx - 1, pin2 = i == 0, pin3 = i == ny - 1;
     int it = 0;
     while (it < vk.newton_steps && rnorm > tol) {
-        T J[25], r[5];
-        cell_system<T, LINEAR>(cs, x, J, r, a.P);
+        TJ J[25];
+        T r[5];
+        cell_system<T, TJ, LINEAR>(cs, *cq, x, J, r, a.P);
         if (pin0) { pin_row(J, 0); r[0] = x[0]; }
         if (pin1) { pin_row(J, 1); r[1] = x[1]; }
         if (pin2) { pin_row(J, 2); r[2] = x[2]; }
@@ -253,12 +260,13 @@ __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__
         } else {
             masked = T(0);
         }
-        T d[5];
-        lu5_solve_rcp(J, r, d);
+        TJ rj[5] = {TJ(r[0]), TJ(r[1]), TJ(r[2]), TJ(r[3]), TJ(r[4])};
+        TJ d[5];
+        lu5_solve_rcp(J, rj, d);
         rnorm = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3] + r[4] * r[4];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {  // compensated relaxed update, local relaxation 0.5
-            T y = T(-0.5) * d[k] - cmp[k];
+            T y = T(-0.5) * T(d[k]) - cmp[k];
             T t = x[k] + y;
             cmp[k] = (t - x[k]) - y;
             x[k] = t;
@@ -698,10 +706,14 @@ hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* 
                              hipStream_t st) {
     {
         TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx, st);
-        if (a.P.linear_drag)
-            k_vanka_cell<T, true><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, work, mask, vk);
-        else
-            k_vanka_cell<T, false><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, work, mask, vk);
+        const dim3 g = tile_grid(a.ny, a.nx, false), b(TX, TY);
+        if (vk.local_fp32) {
+            if (a.P.linear_drag) k_vanka_cell<T, true, float><<<g, b, 0, st>>>(a, work, mask, vk);
+            else k_vanka_cell<T, false, float><<<g, b, 0, st>>>(a, work, mask, vk);
+        } else {
+            if (a.P.linear_drag) k_vanka_cell<T, true, T><<<g, b, 0, st>>>(a, work, mask, vk);
+            else k_vanka_cell<T, false, T><<<g, b, 0, st>>>(a, work, mask, vk);
+        }
     }
     dim3 g((a.nx + 1 + 255) / 256, a.ny + 1);
     TimedLaunch tl(kTimeVankaApply, (long)a.ny * a.nx, st);

@@ -22,7 +22,8 @@ template <typename T>
 struct VankaArgs {
     int newton_steps;
     T ssa_damping, mc_damping, calving_rate;
-    int* iters;  // optional per-cell Newton iteration count
+    int* iters;      // optional per-cell Newton iteration count
+    int local_fp32;  // mixed form: patch Jacobian and its LU in FP32, residual and update in T
 };
 
 template <typename T>

@@ -1,6 +1,8 @@
 // Launch timing records (see timing.hpp).  Host code only.
 #include "timing.hpp"
 
+#include <map>
+
 namespace glide {
 
 namespace {
@@ -12,7 +14,7 @@ struct Totals {
 
 bool g_timing_on = false;
 std::vector<TimingPair> g_pending;  // launches not yet read
-Totals g_totals[kTimedKernels];
+std::map<long, Totals> g_totals[kTimedKernels];  // per kernel, per launch unit count
 std::vector<hipEvent_t> g_event_pool;
 
 hipEvent_t pooled_event() {
@@ -32,7 +34,7 @@ hipError_t add_pair(const TimingPair& r) {
     float ms = 0.f;
     e = hipEventElapsedTime(&ms, r.start, r.stop);
     if (e != hipSuccess) return e;
-    Totals& t = g_totals[r.id];
+    Totals& t = g_totals[r.id][r.units];
     t.ms += ms;
     t.units += (double)r.units;
     ++t.count;
@@ -67,10 +69,10 @@ void timing_clear() {
         g_event_pool.push_back(r.stop);
     }
     g_pending.clear();
-    for (auto& t : g_totals) t = Totals{};
+    for (auto& t : g_totals) t.clear();
 }
 
-hipError_t timing_read(int id, long* count, double* total_ms, double* total_units) {
+hipError_t timing_read(int id, long* count, double* total_ms, double* total_units, long units_filter) {
     *count = 0;
     *total_ms = 0.0;
     *total_units = 0.0;
@@ -82,9 +84,12 @@ hipError_t timing_read(int id, long* count, double* total_ms, double* total_unit
         g_event_pool.push_back(r.stop);
     }
     g_pending.clear();
-    *count = g_totals[id].count;
-    *total_ms = g_totals[id].ms;
-    *total_units = g_totals[id].units;
+    for (auto& kv : g_totals[id]) {
+        if (units_filter > 0 && kv.first != units_filter) continue;
+        *count += kv.second.count;
+        *total_ms += kv.second.ms;
+        *total_units += kv.second.units;
+    }
     return hipSuccess;
 }
 

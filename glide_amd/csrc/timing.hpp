@@ -31,7 +31,9 @@ struct TimingPair {
 bool timing_on();
 void timing_enable(bool on);
 void timing_clear();
-hipError_t timing_read(int id, long* count, double* total_ms, double* total_units);
+// Totals of kernel `id` since the last enable; units_filter > 0 restricts them
+// to launches of exactly that unit count (one grid level).
+hipError_t timing_read(int id, long* count, double* total_ms, double* total_units, long units_filter = 0);
 struct TimedLaunch {
     int id;
     long units;

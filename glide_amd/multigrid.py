@@ -411,9 +411,10 @@ class VankaOptions:
     def __init__(self, levels, getter):
         self._levels = levels
         self._getter = getter
-        self.options = ["omega", "relax_phi"]
+        self.options = ["omega", "relax_phi", "local_fp32"]
         self.omega = BroadcastOption(levels, getter, "omega")
         self.relax_phi = BroadcastOption(levels, getter, "relax_phi")
+        self.local_fp32 = BroadcastOption(levels, getter, "local_fp32")
         self.newton_options = NewtonOptions(levels, getter=lambda lev: getter(lev).newton_config)
 
     def set(self, **kwargs):

@@ -32,18 +32,23 @@ class NewtonConfig:
 
 @dataclass
 class VankaConfig:
-    """glide/operators.py:321-327."""
+    """glide/operators.py:321-327.  ``local_fp32`` (not in the reference)
+    selects the mixed form of the forward smoother: each cell's patch Jacobian
+    is evaluated and factorised in FP32 while the residual, the local stopping
+    test and the Newton update stay FP64."""
     omega: float = 0.5
     newton_config: NewtonConfig = field(default_factory=NewtonConfig)
     relax_phi: float = 0.0
     hook_interval: int = 1
     hook_func: Callable[[int], None] | None = None
+    local_fp32: bool = False
 
     def to_c(self) -> _lib.VankaConfigC:
         n = self.newton_config
         return _lib.VankaConfigC(omega=float(self.omega), relax_phi=float(self.relax_phi),
                                  newton_steps=int(n.steps), relaxation=float(n.relaxation),
-                                 ssa_damping=float(n.ssa_damping), mc_damping=float(n.mc_damping))
+                                 ssa_damping=float(n.ssa_damping), mc_damping=float(n.mc_damping),
+                                 local_fp32=1 if self.local_fp32 else 0)
 
 
 def _norms(a, b, c, stream):

@@ -97,6 +97,10 @@ typedef struct glide_vanka_config {
     double omega, relax_phi;
     int32_t newton_steps;
     double relaxation, ssa_damping, mc_damping;
+    /* Mixed form (no counterpart in the reference): 1 evaluates and factorises
+     * each cell's patch Jacobian in FP32 while the residual, the stopping test
+     * and the Newton update stay FP64; 0 is the plain FP64 form. */
+    int32_t local_fp32;
 } glide_vanka_config;
 
 /* FASCDConfig / FASAdjointConfig (glide/multigrid.py:671-682, 992-1001).
@@ -187,6 +191,8 @@ int glide_graphs_clear(void);
  * kernel `kernel` since the last enable. */
 int glide_timing_enable(int on);
 int glide_timing_read(int kernel, long* count, double* total_ms, double* total_cells);
+/* The same restricted to launches over exactly `units` cells (one grid level). */
+int glide_timing_read_units(int kernel, long units, long* count, double* total_ms);
 
 #ifdef __cplusplus
 }

@@ -281,6 +281,27 @@ def test_fascd_solve(orc, n, L):
     assert rep.norms[2] < 1e-2 * rep.initial_norm
 
 
+@pytest.mark.parametrize("n,L", [(64, 4)])
+def test_mixed_form_solve_against_fp64_oracle(orc, n, L):
+    """Mixed form (FP32 patch Jacobian and LU, FP64 residual/stopping/update):
+    three V-cycles stay within the stated distance of the FP64 oracle's, and
+    converge as the FP64 form does."""
+    print(f"mixed fascd solve n={n} L={L}", flush=True)
+    p, model, om = _dome_models(orc, n, L)
+    fs = model.forward_solver
+    fs.fas_options.set(maximum_vcycles=3, relative_tolerance=0.0, absolute_tolerance=0.0, report_norms=False)
+    fs.vanka_options.set(local_fp32=True)
+    rep = fs.solve(10.0)
+    torch.cuda.synchronize()
+    orep = om.fascd_solve(10.0, orc.OracleMG.fas_config(vcycles=3, rtol=0.0, atol=0.0))
+    g = model.mg[0]
+    errs = {k: rel_err(N(getattr(g.state, k).data), om.get(k)) for k in ("u", "v", "H")}
+    print("mixed vs FP64 oracle max rel err:", errs, "norms", list(rep.norms[:3]), list(orep.norms[:3]), flush=True)
+    for k, e in errs.items():
+        assert e < 1e-4, (k, e)
+    assert rep.norms[2] < 1e-2 * rep.initial_norm
+
+
 def test_fascd_solve_tolerance_stop(orc):
     p, model, om = _dome_models(orc, 32, 3, seed=3)
     fs = model.forward_solver
