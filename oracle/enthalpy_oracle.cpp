@@ -215,6 +215,108 @@ void node_row(const Col& G, const double* E, const double* Ecol, double Es, cons
     *c_out = c;
 }
 
+// ------------------------------------------------------------ exact Jacobian
+// Derivatives used by the adjoint of the converged enthalpy solve (no
+// counterpart in the reference).  Unlike the Newton rows above (frozen K, the
+// drainage sigmoid), these differentiate every term: dK/dE of the smoothed
+// diffusivity and the exact derivative of the drainage softplus branches.
+inline double dK_dE(double E, double Ep) {
+    double zr = (E - Ep) * E_SCALE * 0.01;
+    if (zr < -20.0 || zr > 20.0) return 0.0;
+    double s = 1.0 / (1.0 + std::exp(-zr));
+    return -0.9 * K_COLD * s * (1.0 - s) * E_SCALE * 0.01;
+}
+inline double dsoftplus_dx(double E, double Ep) {
+    double ax = DRAIN_SHARPNESS * (E - Ep) * E_SCALE;
+    if (ax > 10.0) return 1.0;
+    if (ax < -20.0) return 0.0;
+    return 1.0 / (1.0 + std::exp(-ax));
+}
+
+// Row of the exact Jacobian of node (i,j,k) (interior, non-thin): column
+// coefficients a (E_{k-1}), b (E_k), c (E_{k+1}) and horizontal couplings
+// xm, xp, ym, yp (E of the neighbouring columns at level k).
+void exact_row(const Col& G, const double* E, const double* u3d, const double* v3d, const double* om,
+               const double* H, const double* Hp, const Params& P, int i, int j, int k, double* row) {
+    const int nx = G.nx, nz = G.nz;
+    const size_t col = ((size_t)i * nx + j) * nz;
+    const double h = H[(size_t)i * nx + j], hinv = 1.0 / h;
+    (void)Hp;
+    const double dsig = 1.0 / (double)(nz - 1);
+    const double Ek = E[col + k];
+    const double Epk = E_pmp(k * dsig, h);
+    double a = 0, b = RHO_I * h / P.dt, c = 0, xm = 0, xp = 0, ym = 0, yp = 0;
+    if (P.flags & TERM_HORIZ) {
+        const int ny = G.ny;
+        const double sq = std::sqrt(1.0 + P.lf_c), dxi = 1.0 / P.dx;
+        double Hh = h;
+        auto U3 = [&](int kk, int ii, int jj) { return G.U3(u3d, kk, ii, jj); };
+        auto V3 = [&](int kk, int ii, int jj) { return G.V3(v3d, kk, ii, jj); };
+        auto dl = [&](double U) { return 0.5 * (U + std::fabs(U) * sq); };
+        auto dr = [&](double U) { return 0.5 * (U - std::fabs(U) * sq); };
+        double Ul = 0, Ur = 0, Ut = 0, Ub = 0;
+        if (j > 0) {
+            double ul = U3(k, i, j), Hm = G.C(H, i, j - 1);
+            Ul = 0.5 * (Hm + Hh) * ul - 0.5 * std::sqrt(ul * ul + MASS_FLUX_REG) * (Hh - Hm);
+        }
+        if (j < nx - 1) {
+            double ur = U3(k, i, j + 1), Hq = G.C(H, i, j + 1);
+            Ur = 0.5 * (Hh + Hq) * ur - 0.5 * std::sqrt(ur * ur + MASS_FLUX_REG) * (Hq - Hh);
+        }
+        if (i > 0) {
+            double vt = V3(k, i, j), Hm = G.C(H, i - 1, j);
+            Ut = 0.5 * (Hm + Hh) * vt - 0.5 * std::sqrt(vt * vt + MASS_FLUX_REG) * (Hm - Hh);
+        }
+        if (i < ny - 1) {
+            double vb = V3(k, i + 1, j), Hq = G.C(H, i + 1, j);
+            Ub = 0.5 * (Hh + Hq) * vb - 0.5 * std::sqrt(vb * vb + MASS_FLUX_REG) * (Hh - Hq);
+        }
+        xm = -RHO_I * dxi * dl(Ul);
+        xp = RHO_I * dxi * dr(Ur);
+        ym = RHO_I * dxi * dr(Ut);
+        yp = -RHO_I * dxi * dl(Ub);
+        b += RHO_I * dxi * ((dl(Ur) - dr(Ul)) + (dl(Ut) - dr(Ub)));
+    }
+    if (k == 0) {
+        const double E1 = E[col + 1];
+        if (P.flags & TERM_SIGMA) {
+            double w0 = om[col], w1 = om[col + 1];
+            double wh = 0.5 * (w0 + w1);
+            double wp = std::fmax(wh, 0.0), wn = std::fmin(wh, 0.0);
+            double hdi = 1.0 / (0.5 * dsig);
+            b += RHO_I * (wp - w0) * hdi;
+            c += RHO_I * wn * hdi;
+        }
+        const double Em = 0.5 * (Ek + E1), Epm = 0.5 * (Epk + E_pmp(dsig, h));
+        const double Kh = K_of(Em, Epm), dKh = dK_dE(Em, Epm);
+        const double f = -hinv / (dsig * 0.5 * dsig);
+        b += f * (dKh * 0.5 * (E1 - Ek) - Kh);
+        c += f * (dKh * 0.5 * (E1 - Ek) + Kh);
+    } else {
+        const double Ekm1 = E[col + k - 1], Ekp1 = E[col + k + 1];
+        if (P.flags & TERM_SIGMA) {
+            double wm = om[col + k - 1], w0 = om[col + k], wp_ = om[col + std::min(k + 1, nz - 1)];
+            double wu = 0.5 * (w0 + wp_), wl = 0.5 * (wm + w0);
+            double up = std::fmax(wu, 0.0), un = std::fmin(wu, 0.0);
+            double lp = std::fmax(wl, 0.0), ln = std::fmin(wl, 0.0);
+            double di = 1.0 / dsig;
+            a += -RHO_I * lp * di;
+            c += RHO_I * un * di;
+            b += RHO_I * (up - ln) * di;
+        }
+        const double Epm = E_pmp(std::max(k - 1, 0) * dsig, h), Epp = E_pmp(std::min(k + 1, nz - 1) * dsig, h);
+        const double Eu = 0.5 * (Ek + Ekp1), Epu = 0.5 * (Epk + Epp);
+        const double El = 0.5 * (Ek + Ekm1), Epl = 0.5 * (Epk + Epm);
+        const double Ku = K_of(Eu, Epu), dKu = dK_dE(Eu, Epu), Kl = K_of(El, Epl), dKl = dK_dE(El, Epl);
+        const double f = -hinv / (dsig * dsig);
+        c += f * (dKu * 0.5 * (Ekp1 - Ek) + Ku);
+        b += f * (dKu * 0.5 * (Ekp1 - Ek) - Ku - dKl * 0.5 * (Ek - Ekm1) - Kl);
+        a += f * (-dKl * 0.5 * (Ek - Ekm1) + Kl);
+    }
+    if (P.flags & TERM_DRAIN) b += h * RHO_W * P.drain_rate * dsoftplus_dx(Ek, Epk);
+    row[0] = a; row[1] = b; row[2] = c; row[3] = xm; row[4] = xp; row[5] = ym; row[6] = yp;
+}
+
 }  // namespace
 
 extern "C" {
@@ -401,6 +503,79 @@ void orc_enth_arrhenius(double* B, const double* E, const double* H, double n_gl
             s += A * (1.0 + 181.25 * w);
         }
         B[c] = std::pow(s / nz, -1.0 / n_glen);
+    }
+}
+
+// System G(E) = 0 whose solution the column sweep converges to: the residual
+// on interior nodes of non-thin columns, E - E_surface on their surface node,
+// E - min(E_surface, E_pmp) on thin columns (no counterpart in the reference;
+// the adjoint's system).
+void orc_enth_system_residual(double* out, const double* E, const double* Ep, const double* fE, const double* u3d,
+                              const double* v3d, const double* om, const double* H, const double* Hp,
+                              const double* Es, const orc_enth_params* pp, int ny, int nx, int nz) {
+    orc_enth_residual(out, E, Ep, fE, u3d, v3d, om, H, Hp, Es, pp, 1, ny, nx, nz);
+    const double dsig = 1.0 / (double)(nz - 1);
+    for (int c = 0; c < ny * nx; ++c) {
+        size_t col = (size_t)c * nz;
+        if (H[c] < pp->h_thin) {
+            for (int k = 0; k < nz; ++k) out[col + k] = E[col + k] - std::fmin(Es[c], E_pmp(k * dsig, H[c]));
+        } else {
+            out[col + nz - 1] = E[col + nz - 1] - Es[c];
+        }
+    }
+}
+
+// Rows (a, b, c, xm, xp, ym, yp) of dG/dE at E, (ny, nx, nz, 7); identity
+// rows on surface and thin-column nodes.
+void orc_enth_jacobian_rows(double* rows, const double* E, const double* u3d, const double* v3d, const double* om,
+                            const double* H, const double* Hp, const orc_enth_params* pp, int ny, int nx, int nz) {
+    Col G{ny, nx, nz};
+    Params P{pp->dx, pp->dt, pp->drain_rate, pp->h_thin, pp->lf_c, pp->term_flags};
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < ny * nx; ++c) {
+        int i = c / nx, j = c % nx;
+        for (int k = 0; k < nz; ++k) {
+            double* row = rows + ((size_t)c * nz + k) * 7;
+            if (H[c] < P.h_thin || k == nz - 1) {
+                for (int q = 0; q < 7; ++q) row[q] = 0.0;
+                row[1] = 1.0;
+                continue;
+            }
+            exact_row(G, E, u3d, v3d, om, H, Hp, P, i, j, k, row);
+        }
+    }
+}
+
+// dB/dE^T gB: the rate factor's derivative (B as orc_enth_arrhenius divided by
+// `scale`) applied to the cotangent gB (ny, nx), into gE (ny, nx, nz).
+void orc_enth_arrhenius_vjp(double* gE, const double* gB, const double* E, const double* H, double n_glen,
+                            double scale, int ny, int nx, int nz) {
+    const double dsig = 1.0 / (double)(nz - 1);
+    for (int c = 0; c < ny * nx; ++c) {
+        double s = 0.0;
+        std::vector<double> dA(nz);
+        for (int k = 0; k < nz; ++k) {
+            double depth = (1.0 - k * dsig) * H[c];
+            double Tp = T_MELT - BETA_CC * RHO_I * GRAVITY * depth;
+            double Ek = E[(size_t)c * nz + k];
+            double Tu = Ek * E_SCALE / C_I + T_REF;
+            double T = std::fmin(Tu, Tp);
+            double Af = T < 263.15 ? 3.985e-13 : 1.916e3;
+            double Qf = T < 263.15 ? 60e3 : 139e3;
+            double Ts = std::fmax(T, 1.0);
+            double A = Af * std::exp(-Qf / (8.314 * Ts));
+            double Epmp = C_I * (Tp - T_REF);
+            double wraw = Ek * E_SCALE - Epmp;
+            double w = std::fmax(wraw, 0.0) / L_HEAT;
+            s += A * (1.0 + 181.25 * w);
+            double dT = (Tu < Tp && T > 1.0) ? E_SCALE / C_I : 0.0;
+            double dAdE = A * Qf / (8.314 * Ts * Ts) * dT;
+            double dwdE = wraw > 0.0 ? E_SCALE / L_HEAT : 0.0;
+            dA[k] = dAdE * (1.0 + 181.25 * w) + A * 181.25 * dwdE;
+        }
+        double mean = s / nz;
+        double dBdmean = (-1.0 / n_glen) * std::pow(mean, -1.0 / n_glen - 1.0) / scale;
+        for (int k = 0; k < nz; ++k) gE[(size_t)c * nz + k] = gB[c] * dBdmean * dA[k] / nz;
     }
 }
 

@@ -446,6 +446,73 @@ class OracleThermal:
         E_pmp = C_I * (self.T_pmp() - T_REF)
         return np.maximum(self.E * E_SCALE - E_pmp, 0.0) / L_HEAT
 
+    # ---------------------------------------------------------- adjoint pieces
+    def system_residual(self, dt):
+        """G(E): the residual on interior nodes, E - E_surface on the surface
+        node, E - min(E_surface, E_pmp) on thin columns."""
+        out = np.zeros_like(self.E)
+        P = self.params(dt)
+        _enth_lib().orc_enth_system_residual(_p(out), _p(self.E), *self._args(), ctypes.byref(P), self.ny, self.nx,
+                                             self.nz)
+        return out
+
+    def jacobian_rows(self, dt):
+        """Exact dG/dE rows (a, b, c, xm, xp, ym, yp) per node."""
+        rows = np.zeros((self.ny, self.nx, self.nz, 7))
+        P = self.params(dt)
+        _enth_lib().orc_enth_jacobian_rows(_p(rows), _p(self.E), _p(self.u3d), _p(self.v3d), _p(self.omega),
+                                           _p(self.H), _p(self.H_prev), ctypes.byref(P), self.ny, self.nx, self.nz)
+        return rows
+
+    def jacobian(self, dt):
+        """dG/dE as a scipy CSR matrix over the (ny, nx, nz) node order."""
+        import scipy.sparse as sp
+        ny, nx, nz = self.ny, self.nx, self.nz
+        rows = self.jacobian_rows(dt)
+        idx = np.arange(ny * nx * nz).reshape(ny, nx, nz)
+        ii, jj, kk = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
+        targets = [(ii, jj, kk - 1), (ii, jj, kk), (ii, jj, kk + 1), (ii, jj - 1, kk), (ii, jj + 1, kk),
+                   (ii - 1, jj, kk), (ii + 1, jj, kk)]
+        R, C, V = [], [], []
+        for q, (ti, tj, tk) in enumerate(targets):
+            val = rows[..., q]
+            ok = (val != 0.0) & (ti >= 0) & (ti < ny) & (tj >= 0) & (tj < nx) & (tk >= 0) & (tk < nz)
+            R.append(idx[ok]); C.append(idx[ti[ok], tj[ok], tk[ok]]); V.append(val[ok])
+        n = ny * nx * nz
+        return sp.csr_matrix((np.concatenate(V), (np.concatenate(R), np.concatenate(C))), shape=(n, n))
+
+    def adjoint_solve(self, dt, g):
+        """lambda with J^T lambda = g (direct sparse solve)."""
+        import scipy.sparse.linalg as spla
+        J = self.jacobian(dt)
+        lam = spla.spsolve(J.T.tocsc(), np.ascontiguousarray(g).ravel())
+        return lam.reshape(self.E.shape)
+
+    def adjoint_gradients(self, dt, lam):
+        """dL/dp = -lambda^T dG/dp for E_prev, Q_geo, Q_fh, phi_strain, E_surface."""
+        nz = self.nz
+        dsig = 1.0 / (nz - 1)
+        thin = self.H < self.h_thin
+        live = ~thin[:, :, None] & (np.arange(nz) < nz - 1)[None, None, :]
+        g = {}
+        g["E_prev"] = np.where(live, lam * RHO_I_E * self.H_prev[:, :, None] / dt, 0.0)
+        g["Q_geo"] = np.where(~thin, lam[:, :, 0] / (0.5 * dsig * E_SCALE), 0.0)
+        g["Q_fh"] = g["Q_geo"].copy()
+        g["phi_strain"] = np.where(live, lam * self.H[:, :, None] / E_SCALE, 0.0)
+        sig = np.linspace(0.0, 1.0, nz)
+        Epmp = C_I * (T_MELT - BETA_CC * RHO_I_E * GRAVITY * (1.0 - sig)[None, None, :] * self.H[:, :, None]
+                      - T_REF) / E_SCALE
+        below = self.E_surface[:, :, None] < Epmp
+        g["E_surface"] = np.where(thin, np.sum(np.where(below, lam, 0.0), axis=2), lam[:, :, nz - 1])
+        return g
+
+    def arrhenius_vjp(self, gB, n_glen=3.0, scale=1.0):
+        gE = np.zeros_like(self.E)
+        gB = np.ascontiguousarray(gB, dtype=np.float64)
+        _enth_lib().orc_enth_arrhenius_vjp(_p(gE), _p(gB), _p(self.E), _p(self.H), _D(n_glen), _D(scale), self.ny,
+                                           self.nx, self.nz)
+        return gE
+
     def get_arrhenius_factor(self, n_glen=3.0):
         B = np.zeros((self.ny, self.nx))
         _enth_lib().orc_enth_arrhenius(_p(B), _p(self.E), _p(self.H), _D(n_glen), self.ny, self.nx, self.nz)

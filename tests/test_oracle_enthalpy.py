@@ -288,3 +288,106 @@ def test_strain_heating_closed_forms(orc):
     u = np.full((ny, nx + 1), 37.0)
     v = np.full((ny + 1, nx), -12.0)
     assert np.max(np.abs(orc.strain_heating(u, v, B, n, eps, dx))) == 0.0
+
+
+# ---------------------------------------------------------------- adjoint pins
+def _adjoint_case(orc, ny=6, nx=5, nz=9, seed=3):
+    from tests.util import enth_case
+    C = enth_case(ny, nx, nz, seed)
+    th = orc.OracleThermal(ny, nx, nz, 1500.0, C["H"])
+    for k in ("E", "E_prev", "u3d", "v3d", "omega", "E_surface", "Q_geo", "Q_fh", "phi_strain", "H_prev"):
+        setattr(th, k, np.ascontiguousarray(C[k], dtype=np.float64).copy())
+    th.set_rhs(snapshot=False)
+    return th
+
+
+DT_ADJ = 10.0 * 365.25 * 86400.0
+
+
+def test_exact_jacobian_matches_central_differences(orc):
+    """dG/dE rows (diffusivity and drainage derivatives included) against
+    central differences of the system residual in a random direction."""
+    th = _adjoint_case(orc)
+    rng = np.random.default_rng(0)
+    J = th.jacobian(DT_ADJ)
+    E0 = th.E.copy()
+    for trial in range(3):
+        v = rng.standard_normal(E0.shape)
+        eps = 1e-6
+        th.E = E0 + eps * v
+        gp = th.system_residual(DT_ADJ)
+        th.E = E0 - eps * v
+        gm = th.system_residual(DT_ADJ)
+        th.E = E0.copy()
+        fd = (gp - gm) / (2 * eps)
+        jv = (J @ v.ravel()).reshape(E0.shape)
+        assert np.max(np.abs(fd - jv)) < 1e-6 * np.max(np.abs(jv)), trial
+
+
+def _solve(th, n_iter=400):
+    th.rtol, th.atol = 1e-15, 0.0
+    th.column_sweep(DT_ADJ, n_iter)
+    return th
+
+
+def test_adjoint_gradients_match_finite_differences_of_the_solve(orc):
+    """Gradients of L = sum(w * E) through the converged solve: the adjoint
+    (J^T lambda = w) against central differences of complete solves in
+    E_prev, Q_geo, phi_strain and E_surface (thick and thin columns)."""
+    base = _adjoint_case(orc, seed=5)
+    E_init = base.E.copy()
+    rng = np.random.default_rng(1)
+    w = rng.standard_normal(E_init.shape)
+    th = _solve(_adjoint_case(orc, seed=5))
+    G = th.system_residual(DT_ADJ)
+    assert np.max(np.abs(G)) < 1e-10
+    lam = th.adjoint_solve(DT_ADJ, w)
+    grads = th.adjoint_gradients(DT_ADJ, lam)
+
+    def L_after(mutate):
+        t = _adjoint_case(orc, seed=5)
+        mutate(t)
+        t.set_rhs(snapshot=False)
+        t.E = E_init.copy()
+        _solve(t)
+        return float(np.sum(w * t.E))
+
+    thin = np.argwhere(th.H < th.h_thin)
+    thick = np.argwhere(th.H >= th.h_thin)
+    i, j = thick[0]
+    checks = []
+    for name, (ii, jj, kk), size in (("E_prev", (i, j, 2), 1e-3), ("phi_strain", (i, j, 3), 1e-9),
+                                    ("Q_geo", (i, j, None), 1e-3), ("E_surface", (i, j, None), 1e-3)):
+        def mut(t, s, name=name, ii=ii, jj=jj, kk=kk):
+            arr = getattr(t, name)
+            if kk is None:
+                arr[ii, jj] += s
+            else:
+                arr[ii, jj, kk] += s
+        fd = (L_after(lambda t: mut(t, size)) - L_after(lambda t: mut(t, -size))) / (2 * size)
+        ad = grads[name][ii, jj] if kk is None else grads[name][ii, jj, kk]
+        checks.append((name, fd, ad))
+        assert abs(fd - ad) <= 1e-5 * max(abs(fd), abs(ad), 1e-30), (name, fd, ad)
+    if len(thin):
+        ti, tj = thin[0]
+        fd = (L_after(lambda t: t.E_surface.__setitem__((ti, tj), t.E_surface[ti, tj] + 1e-3))
+              - L_after(lambda t: t.E_surface.__setitem__((ti, tj), t.E_surface[ti, tj] - 1e-3))) / 2e-3
+        assert abs(fd - grads["E_surface"][ti, tj]) <= 1e-5 * max(abs(fd), 1e-30)
+
+
+def test_arrhenius_vjp_matches_finite_differences(orc):
+    th = _adjoint_case(orc, seed=7)
+    rng = np.random.default_rng(2)
+    gB = rng.standard_normal((th.ny, th.nx))
+    scale = 917.0 * 9.81 * (365.25 * 86400.0) ** (1.0 / 3.0)
+    gE = th.arrhenius_vjp(gB, 3.0, scale)
+    v = rng.standard_normal(th.E.shape)
+    E0 = th.E.copy()
+    eps = 1e-7
+    th.E = E0 + eps * v
+    Bp = th.get_arrhenius_factor(3.0) / scale
+    th.E = E0 - eps * v
+    Bm = th.get_arrhenius_factor(3.0) / scale
+    th.E = E0
+    fd = np.sum(gB * (Bp - Bm)) / (2 * eps)
+    assert abs(fd - np.sum(gE * v)) <= 1e-6 * abs(fd)

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 import torch
 
-from tests.util import rel_err
+from tests.util import enth_case, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -27,34 +27,6 @@ def T(a):
 
 def N(t):
     return t.detach().cpu().numpy()
-
-
-def enth_case(ny, nx, nz, seed, temperate_frac=0.3):
-    """Seeded enthalpy state in the reference's layouts (numpy, SI, scaled E)."""
-    from oracle.oracle import BETA_CC, C_I, E_SCALE, GRAVITY, RHO_I_E, SEC_PER_YR, T_MELT, T_REF
-    rng = np.random.default_rng(seed)
-    H = 200.0 + 1800.0 * rng.random((ny, nx))
-    r = rng.random((ny, nx))
-    H[r < 0.08] = 20.0 * rng.random(int((r < 0.08).sum()))      # thin columns (< h_thin = 25 m)
-    H[r < 0.03] = 0.0                                           # ice-free
-    sigma = np.linspace(0.0, 1.0, nz)
-    T_pmp = T_MELT - BETA_CC * RHO_I_E * GRAVITY * (1.0 - sigma)[None, None, :] * H[:, :, None]
-    Temp = T_pmp - 35.0 * rng.random((ny, nx, nz)) * sigma[None, None, :] - 0.5 * rng.random((ny, nx, nz))
-    E = C_I * (Temp - T_REF) / E_SCALE
-    warm = rng.random((ny, nx)) < temperate_frac
-    E[warm, :max(2, nz // 4)] += 0.02 * rng.random((int(warm.sum()), max(2, nz // 4)))
-    C = {"H": H, "H_prev": H + 5.0 * rng.standard_normal((ny, nx)) * (H > 0), "E": E,
-         "E_prev": E + 0.01 * rng.standard_normal(E.shape)}
-    shape_u = (1.0 + sigma)[:, None, None] / 2.0
-    C["u3d"] = shape_u * (60.0 * rng.standard_normal((1, ny, nx + 1))) / SEC_PER_YR
-    C["v3d"] = shape_u * (45.0 * rng.standard_normal((1, ny + 1, nx))) / SEC_PER_YR
-    C["omega"] = 0.3 * rng.standard_normal((ny, nx, nz)) / SEC_PER_YR
-    Ts = 235.0 + 30.0 * rng.random((ny, nx))
-    C["E_surface"] = C_I * (np.minimum(Ts, T_MELT) - T_REF) / E_SCALE
-    C["Q_geo"] = 0.05 + 0.02 * rng.random((ny, nx))
-    C["Q_fh"] = 0.02 * rng.random((ny, nx))
-    C["phi_strain"] = 2e-6 * rng.random((ny, nx, nz))
-    return C
 
 
 def make_pair(C, nz, flags=7):
