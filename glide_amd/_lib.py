@@ -91,6 +91,14 @@ class EnthReport(ctypes.Structure):
                 ("r", ctypes.c_double), ("norms", ctypes.c_double * ENTH_MAX_SWEEPS)]
 
 
+ENTH_ADJOINT_POINTERS = ("rows", "g", "lambda_", "lambda_work", "r", "g_E_prev", "g_phi_strain", "g_Q_geo",
+                         "g_E_surface")
+
+
+class EnthAdjointC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ENTH_ADJOINT_POINTERS]
+
+
 class ThermalConfigC(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("m", ctypes.c_double), ("u_reg", ctypes.c_double),
                 ("rho_i", ctypes.c_double), ("n_glen", ctypes.c_double), ("eps_reg", ctypes.c_double),
@@ -151,6 +159,8 @@ SIGNATURES = {
     "glide_enth_diagnostics": (_I, [_CP, _P, _P, _P]),
     "glide_enth_from_temperature": (_I, [_CP, _P, _P, _P]),
     "glide_enth_surface_from_temperature": (_I, [_CP, _P, _P, _P]),
+    "glide_enth_adjoint_solve": (_I, [_CP, ctypes.POINTER(EnthAdjointC), _D, _P, _P, _D, _D, _I,
+                                      ctypes.POINTER(EnthSweepConfigC), ctypes.POINTER(EnthReport), _P]),
     "glide_thermal_step": (_I, [_CP, _P, _P, _P, _P, ctypes.POINTER(ThermalConfigC),
                                 ctypes.POINTER(EnthSweepConfigC), ctypes.POINTER(EnthReport), _P]),
 }

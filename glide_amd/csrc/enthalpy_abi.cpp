@@ -342,6 +342,59 @@ int glide_enth_surface_from_temperature(const glide_enth_column* C, const void* 
     return GLIDE_OK;
 }
 
+int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoint* X, double dt, const void* dL_dE,
+                             const void* dL_dB, double n_glen, double B_scale, int n_iter,
+                             const glide_enth_sweep_config* cfg, glide_enth_report* report, void* stream) {
+    if (!X || !cfg) return fail(GLIDE_ERR_INVALID_ARG, "adjoint buffers or config is NULL");
+    if (n_iter < 0 || n_iter > GLIDE_ENTH_MAX_SWEEPS)
+        return fail(GLIDE_ERR_INVALID_ARG, "n_iter must be in [0, GLIDE_ENTH_MAX_SWEEPS]");
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, dt, true, &A));
+    REQ(X->rows);
+    REQ(X->g);
+    REQ(X->lambda);
+    REQ(X->lambda_work);
+    REQ(X->r);
+    if (dL_dB && (!(n_glen > 0.0) || B_scale == 0.0))
+        return fail(GLIDE_ERR_INVALID_ARG, "n_glen must be positive and B_scale non-zero");
+    EnthScratch* s = nullptr;
+    GLIDE_TRY(scratch(&s));
+    hipStream_t st = S(stream);
+    const int ny = C->ny, nx = C->nx, nz = C->nz;
+    HIP_TRY(launch_enth_adj_rows<R>(A, P(X->rows), st), "adjoint rows");
+    HIP_TRY(launch_enth_adj_rhs<R>(A, CP(dL_dE), CP(dL_dB), n_glen, B_scale, P(X->g), st), "adjoint rhs");
+    const double rtol = cfg->relative_tolerance, atol = cfg->absolute_tolerance;
+    HIP_TRY(launch_enth_adj_residual<R>(nullptr, ny, nx, nz, CP(X->rows), CP(X->g), CP(X->lambda), P(X->r),
+                                        s->partials, s->status, 0, rtol, atol, st),
+            "adjoint residual");
+    const int* skip = &s->status->done;
+    for (int it = 0; it < n_iter; ++it) {
+        HIP_TRY(launch_enth_adj_column<R>(skip, ny, nx, nz, CP(X->rows), CP(X->g), CP(X->lambda), P(X->lambda_work),
+                                          cfg->omega, st),
+                "adjoint column pass");
+        HIP_TRY(launch_enth_adj_layer<R>(skip, ny, nx, nz, CP(X->rows), CP(X->g), CP(X->lambda_work), P(X->lambda),
+                                         cfg->omega, st),
+                "adjoint layer pass");
+        HIP_TRY(launch_enth_adj_residual<R>(skip, ny, nx, nz, CP(X->rows), CP(X->g), CP(X->lambda), P(X->r),
+                                            s->partials, s->status, 1, rtol, atol, st),
+                "adjoint residual");
+    }
+    HIP_TRY(launch_enth_adj_grads<R>(A, CP(X->lambda), P(X->g_E_prev), P(X->g_phi_strain), P(X->g_Q_geo),
+                                     P(X->g_E_surface), st),
+            "adjoint gradients");
+    if (report) {
+        HIP_TRY(hipMemcpyAsync(s->host, s->status, sizeof(EnthStatus), hipMemcpyDeviceToHost, st), "copy status");
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        report->sweeps = s->host->sweeps;
+        report->pad = 0;
+        report->r0 = s->host->r0;
+        report->r = s->host->r;
+        std::memset(report->norms, 0, sizeof(report->norms));
+        for (int q = 0; q < s->host->sweeps && q < GLIDE_ENTH_MAX_SWEEPS; ++q) report->norms[q] = s->host->norms[q];
+    }
+    return GLIDE_OK;
+}
+
 int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta, void* B,
                        const glide_thermal_config* tc, const glide_enth_sweep_config* sc, glide_enth_report* report,
                        void* stream) {

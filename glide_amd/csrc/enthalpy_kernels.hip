@@ -597,6 +597,281 @@ __global__ __launch_bounds__(256) void k_enth_surface_from_temperature(const T* 
         Es[q] = T(kCI) * (fmin(Ts[q], T(kTMelt)) - T(kTRef)) / T(kEScale);
 }
 
+// ------------------------------------------------------------------ adjoint
+// Adjoint of the converged solve (no counterpart in the reference): the
+// system G(E) = 0 solved by the sweeps has rows R (interior nodes of thick
+// columns), E - E_surface (their surface node) and E - min(E_surface, E_pmp)
+// (thin columns).  Its exact Jacobian differentiates every term, including
+// dK/dE of the smoothed diffusivity and the drainage softplus branches.
+template <typename T>
+__device__ __forceinline__ T dK_dE(T E, T Ep) {
+    T zr = (E - Ep) * T(kEScale) * T(0.01);
+    if (zr < T(-20) || zr > T(20)) return T(0);
+    T s = T(1) / (T(1) + exp(-zr));
+    return T(-0.9) * T(kKCold) * s * (T(1) - s) * T(kEScale) * T(0.01);
+}
+template <typename T>
+__device__ __forceinline__ T dsoftplus_dx(T E, T Ep) {
+    T ax = T(kDrainSharpness) * (E - Ep) * T(kEScale);
+    if (ax > T(10)) return T(1);
+    if (ax < T(-20)) return T(0);
+    return T(1) / (T(1) + exp(-ax));
+}
+
+template <typename T>
+struct JRow {
+    T a, b, c, xm, xp, ym, yp;
+};
+
+// Exact Jacobian row of an interior node of a thick column; Ekm1 / Ekp1 the
+// column neighbours' enthalpy.
+template <typename T>
+__device__ __forceinline__ JRow<T> exact_row(const EnthArgs<T>& A, int i, int j, int k, T h, T Ek, T Ekm1, T Ekp1,
+                                             T om_m, T om_k, T om_p) {
+    const int nz = A.nz;
+    const T dsig = T(1) / T(nz - 1);
+    const T hinv = T(1) / h;
+    JRow<T> w{T(0), T(kRhoI) * h / A.dt, T(0), T(0), T(0), T(0), T(0)};
+    if (A.flags & 1) {
+        const Horiz<T> hz = load_horiz(A, i, j, k);
+        const T sq = A.lf_sq, f = T(kRhoI) * A.dxi;
+        auto dl = [&](T U) { return T(0.5) * (U + fabs(U) * sq); };
+        auto dr = [&](T U) { return T(0.5) * (U - fabs(U) * sq); };
+        w.xm = -f * dl(hz.Hu_l);
+        w.xp = f * dr(hz.Hu_r);
+        w.ym = f * dr(hz.Hv_t);
+        w.yp = -f * dl(hz.Hv_b);
+        w.b += f * ((dl(hz.Hu_r) - dr(hz.Hu_l)) + (dl(hz.Hv_t) - dr(hz.Hv_b)));
+    }
+    const T Ep_k = e_pmp(T(k) * dsig, h);
+    if (k == 0) {
+        if (A.flags & 2) {
+            T wh = T(0.5) * (om_k + om_p);
+            T wp = fmax(wh, T(0)), wn = fmin(wh, T(0));
+            T hdi = T(1) / (T(0.5) * dsig);
+            w.b += T(kRhoI) * (wp - om_k) * hdi;
+            w.c += T(kRhoI) * wn * hdi;
+        }
+        const T Em = T(0.5) * (Ek + Ekp1), Epm = T(0.5) * (Ep_k + e_pmp(dsig, h));
+        const T Kh = diffusivity(Em, Epm), dKh = dK_dE(Em, Epm);
+        const T f = -hinv / (dsig * (T(0.5) * dsig));
+        w.b += f * (dKh * T(0.5) * (Ekp1 - Ek) - Kh);
+        w.c += f * (dKh * T(0.5) * (Ekp1 - Ek) + Kh);
+    } else {
+        if (A.flags & 2) {
+            T wu = T(0.5) * (om_k + om_p), wl = T(0.5) * (om_m + om_k);
+            T up = fmax(wu, T(0)), un = fmin(wu, T(0)), lp = fmax(wl, T(0)), ln = fmin(wl, T(0));
+            T di = T(1) / dsig;
+            w.a -= T(kRhoI) * lp * di;
+            w.c += T(kRhoI) * un * di;
+            w.b += T(kRhoI) * (up - ln) * di;
+        }
+        const T Ep_m = e_pmp(T(k - 1) * dsig, h), Ep_p = e_pmp(T(min(k + 1, nz - 1)) * dsig, h);
+        const T Eu = T(0.5) * (Ek + Ekp1), Epu = T(0.5) * (Ep_k + Ep_p);
+        const T El = T(0.5) * (Ek + Ekm1), Epl = T(0.5) * (Ep_k + Ep_m);
+        const T Ku = diffusivity(Eu, Epu), dKu = dK_dE(Eu, Epu), Kl = diffusivity(El, Epl), dKl = dK_dE(El, Epl);
+        const T f = -hinv / (dsig * dsig);
+        w.c += f * (dKu * T(0.5) * (Ekp1 - Ek) + Ku);
+        w.b += f * (dKu * T(0.5) * (Ekp1 - Ek) - Ku - dKl * T(0.5) * (Ek - Ekm1) - Kl);
+        w.a += f * (-dKl * T(0.5) * (Ek - Ekm1) + Kl);
+    }
+    if (A.flags & 4) w.b += h * T(kRhoW) * A.drain_rate * dsoftplus_dx(Ek, Ep_k);
+    return w;
+}
+
+// Rows (a, b, c, xm, xp, ym, yp) at the converged E into seven (ny, nx, nz)
+// planes; identity rows on surface nodes and thin columns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_adj_rows(EnthArgs<T> A, T* __restrict__ rows) {
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k;
+    const bool in = k < nz;
+    const long base = q.col * nz, n3 = ncol * nz;
+    const T h = A.H[q.col];
+    const T Ek = in ? A.E[base + k] : T(0);
+    const T omk = (in && A.omega) ? A.omega[base + k] : T(0);
+    const T Ekm1 = lane_get(Ek, max(k - 1, 0)), Ekp1 = lane_get(Ek, min(k + 1, 63));
+    const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
+    if (!in) return;
+    JRow<T> w{T(0), T(1), T(0), T(0), T(0), T(0), T(0)};
+    if (k < nz - 1 && !(h < A.h_thin)) w = exact_row(A, q.i, q.j, k, h, Ek, Ekm1, Ekp1, omm, omk, omp);
+    const long n = base + k;
+    rows[n] = w.a;
+    rows[n3 + n] = w.b;
+    rows[2 * n3 + n] = w.c;
+    rows[3 * n3 + n] = w.xm;
+    rows[4 * n3 + n] = w.xp;
+    rows[5 * n3 + n] = w.ym;
+    rows[6 * n3 + n] = w.yp;
+}
+
+// Horizontal part of (J^T lam) at node (i, j, k): the neighbouring rows'
+// couplings to this node's enthalpy.
+template <typename T>
+__device__ __forceinline__ T jt_horizontal(const T* __restrict__ rows, const T* __restrict__ lam, int ny, int nx,
+                                           int nz, long n3, int i, int j, int k) {
+    T s = T(0);
+    if (j > 0) { const long m = ((long)i * nx + j - 1) * nz + k; s += rows[4 * n3 + m] * lam[m]; }
+    if (j < nx - 1) { const long m = ((long)i * nx + j + 1) * nz + k; s += rows[3 * n3 + m] * lam[m]; }
+    if (i > 0) { const long m = ((long)(i - 1) * nx + j) * nz + k; s += rows[6 * n3 + m] * lam[m]; }
+    if (i < ny - 1) { const long m = ((long)(i + 1) * nx + j) * nz + k; s += rows[5 * n3 + m] * lam[m]; }
+    return s;
+}
+
+// Column pass on J^T lam = g: each column's transposed tridiagonal block
+// solved by PCR against g minus the horizontal coupling of lam_in;
+// lam_out = lam_in + omega_relax (lam_col - lam_in).
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_adj_column(const int* __restrict__ skip, int ny, int nx, int nz,
+                                                         const T* __restrict__ rows, const T* __restrict__ g,
+                                                         const T* __restrict__ lam_in, T* __restrict__ lam_out,
+                                                         T omega_relax) {
+    if (skipped(skip)) return;
+    const long ncol = (long)ny * nx;
+    const ColIdx q = col_index(wave_column(), nx, ncol);
+    if (!q.valid) return;
+    const int k = q.k;
+    const bool in = k < nz;
+    const long n3 = ncol * nz, n = q.col * nz + k;
+    const T a = in ? rows[n] : T(0), b = in ? rows[n3 + n] : T(1), c = in ? rows[2 * n3 + n] : T(0);
+    const T cm = lane_get(c, max(k - 1, 0)), ap = lane_get(a, min(k + 1, 63));
+    const T aT = k > 0 ? cm : T(0);        // (J^T)_{k,k-1} = J_{k-1,k}
+    const T cT = (k + 1 < nz) ? ap : T(0);  // (J^T)_{k,k+1} = J_{k+1,k}
+    T rhs = T(0), l0 = T(0);
+    if (in) {
+        l0 = lam_in[n];
+        rhs = g[n] - jt_horizontal(rows, lam_in, ny, nx, nz, n3, q.i, q.j, k);
+    }
+    const T lc = pcr_solve(in ? aT : T(0), b, in ? cT : T(0), rhs);
+    if (in) lam_out[n] = l0 + omega_relax * (lc - l0);
+}
+
+// (J^T lam)_n at one node.
+template <typename T>
+__device__ __forceinline__ T jt_apply(const T* __restrict__ rows, const T* __restrict__ lam, int ny, int nx, int nz,
+                                      long n3, int i, int j, int k, long n) {
+    T s = rows[n3 + n] * lam[n];
+    if (k > 0) s += rows[2 * n3 + n - 1] * lam[n - 1];
+    if (k < nz - 1) s += rows[n + 1] * lam[n + 1];
+    return s + jt_horizontal(rows, lam, ny, nx, nz, n3, i, j, k);
+}
+
+// Layer pass: lam_out = lam_in + omega_relax (g - J^T lam_in) / J_nn.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_adj_layer(const int* __restrict__ skip, int ny, int nx, int nz,
+                                                        const T* __restrict__ rows, const T* __restrict__ g,
+                                                        const T* __restrict__ lam_in, T* __restrict__ lam_out,
+                                                        T omega_relax) {
+    if (skipped(skip)) return;
+    const long n3 = (long)ny * nx * nz;
+    for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < n3; n += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(n % nz);
+        const long col = n / nz;
+        const int i = (int)(col / nx), j = (int)(col % nx);
+        const T r = g[n] - jt_apply(rows, lam_in, ny, nx, nz, n3, i, j, k, n);
+        const T d = rows[n3 + n];
+        lam_out[n] = lam_in[n] + (fabs(d) > T(1e-30) ? omega_relax * r / d : T(0));
+    }
+}
+
+// Adjoint residual g - J^T lam and per-block max |.|.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_adj_residual(const int* __restrict__ skip, int ny, int nx, int nz,
+                                                           const T* __restrict__ rows, const T* __restrict__ g,
+                                                           const T* __restrict__ lam, T* __restrict__ r_out,
+                                                           double* __restrict__ partials) {
+    __shared__ double smax[4];
+    if (skipped(skip)) return;
+    const long n3 = (long)ny * nx * nz;
+    double m = 0.0;
+    for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < n3; n += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(n % nz);
+        const long col = n / nz;
+        const T r = g[n] - jt_apply(rows, lam, ny, nx, nz, n3, (int)(col / nx), (int)(col % nx), k, n);
+        if (r_out) r_out[n] = r;
+        m = fmax(m, fabs((double)r));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+}
+
+// Right-hand side g = dL/dE + (dB/dE)^T dL/dB of the rate factor
+// B = mean_k(A_k (1 + 181.25 w_k))^(-1/n) / scale.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_adj_rhs(EnthArgs<T> A, const T* __restrict__ gE,
+                                                      const T* __restrict__ gB, T n_glen, T scale,
+                                                      T* __restrict__ g) {
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k;
+    const bool in = k < nz;
+    const long n = q.col * nz + k;
+    T val = T(0), dA = T(0);
+    if (in && gB) {
+        const T h = A.H[q.col];
+        const T Tp = t_pmp(T(k) * (T(1) / T(nz - 1)), h);
+        const T Ek = A.E[n];
+        const T Tu = Ek * T(kEScale) / T(kCI) + T(kTRef);
+        const T Tk = fmin(Tu, Tp);
+        const T Af = Tk < T(263.15) ? T(3.985e-13) : T(1.916e3);
+        const T Qf = Tk < T(263.15) ? T(60e3) : T(139e3);
+        const T Ts = fmax(Tk, T(1));
+        const T Ak = Af * exp(-Qf / (T(8.314) * Ts));
+        const T wraw = Ek * T(kEScale) - T(kCI) * (Tp - T(kTRef));
+        const T wk = fmax(wraw, T(0)) / T(kLHeat);
+        val = Ak * (T(1) + T(181.25) * wk);
+        const T dT = (Tu < Tp && Tk > T(1)) ? T(kEScale) / T(kCI) : T(0);
+        const T dw = wraw > T(0) ? T(kEScale) / T(kLHeat) : T(0);
+        dA = Ak * Qf / (T(8.314) * Ts * Ts) * dT * (T(1) + T(181.25) * wk) + Ak * T(181.25) * dw;
+    }
+    T sum = val;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if (!in) return;
+    T out = gE ? gE[n] : T(0);
+    if (gB) {
+        const T mean = sum / T(nz);
+        const T dBdmean = (T(-1) / n_glen) * pow(mean, T(-1) / n_glen - T(1)) / scale;
+        out += gB[q.col] * dBdmean * dA / T(nz);
+    }
+    g[n] = out;
+}
+
+// Parameter gradients dL/dp = -lam^T dG/dp: E_prev, phi_strain (3-D), Q_geo
+// (equal to the Q_fh gradient), E_surface (2-D).
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_adj_grads(EnthArgs<T> A, const T* __restrict__ lam,
+                                                        T* __restrict__ g_E_prev, T* __restrict__ g_phi,
+                                                        T* __restrict__ g_Q, T* __restrict__ g_Es) {
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k;
+    const bool in = k < nz;
+    const long n = q.col * nz + k;
+    const T h = A.H[q.col];
+    const bool thin = h < A.h_thin;
+    const T l = in ? lam[n] : T(0);
+    const bool live = in && !thin && k < nz - 1;
+    if (in && g_E_prev) g_E_prev[n] = live ? l * T(kRhoI) * A.H_prev[q.col] / A.dt : T(0);
+    if (in && g_phi) g_phi[n] = live ? l * h / T(kEScale) : T(0);
+    const T l0 = lane_get(l, 0), ltop = lane_get(l, nz - 1);
+    T contrib = T(0);
+    if (in && thin) {
+        const T Es = A.E_surface[q.col];
+        contrib = (Es < e_pmp(T(k) * (T(1) / T(nz - 1)), h)) ? l : T(0);
+    }
+    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off, 64);
+    if (k == 0) {
+        if (g_Q) g_Q[q.col] = thin ? T(0) : l0 / (T(0.5) * (T(1) / T(nz - 1)) * T(kEScale));
+        if (g_Es) g_Es[q.col] = thin ? contrib : ltop;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 static inline unsigned col_blocks(int ny, int nx) {
     return (unsigned)(((long)ny * nx + kColsPerBlock - 1) / kColsPerBlock);
@@ -687,6 +962,48 @@ hipError_t launch_enth_surface_from_temperature(const T* Ts, T* Es, long n, hipS
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_enth_adj_rows(const EnthArgs<T>& A, T* rows, hipStream_t st) {
+    k_enth_adj_rows<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, rows);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_adj_rhs(const EnthArgs<T>& A, const T* gE, const T* gB, T n_glen, T scale, T* g,
+                               hipStream_t st) {
+    k_enth_adj_rhs<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, gE, gB, n_glen, scale, g);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_adj_column(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
+                                  const T* lam_in, T* lam_out, T omega_relax, hipStream_t st) {
+    k_enth_adj_column<T><<<col_blocks(ny, nx), 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam_in, lam_out,
+                                                              omega_relax);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_adj_layer(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
+                                 const T* lam_in, T* lam_out, T omega_relax, hipStream_t st) {
+    k_enth_adj_layer<T><<<flat_blocks((long)ny * nx * nz), 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam_in, lam_out,
+                                                                          omega_relax);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_adj_residual(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
+                                    const T* lam, T* r_out, double* partials, EnthStatus* status, int status_mode,
+                                    double rtol, double atol, hipStream_t st) {
+    long nb = ((long)ny * nx * nz + 255) / 256;
+    unsigned g_ = (unsigned)(nb < kEnthResBlocks ? (nb > 0 ? nb : 1) : kEnthResBlocks);
+    k_enth_adj_residual<T><<<g_, 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam, r_out, partials);
+    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)g_, nullptr, status, status_mode, rtol, atol);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev, T* g_phi, T* g_Q, T* g_Es,
+                                 hipStream_t st) {
+    k_enth_adj_grads<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, lam, g_E_prev, g_phi, g_Q, g_Es);
+    return hipGetLastError();
+}
+
 #define INST(T)                                                                                                      \
     template hipError_t launch_enth_residual<T>(const EnthArgs<T>&, T*, int, double*, double*, EnthStatus*, int,     \
                                                 double, double, hipStream_t);                                        \
@@ -702,7 +1019,16 @@ hipError_t launch_enth_surface_from_temperature(const T* Ts, T* Es, long n, hipS
                                                       hipStream_t);                                                  \
     template hipError_t launch_enth_diagnostics<T>(const EnthArgs<T>&, T*, T*, hipStream_t);                         \
     template hipError_t launch_enth_from_temperature<T>(const EnthArgs<T>&, const T*, T*, hipStream_t);              \
-    template hipError_t launch_enth_surface_from_temperature<T>(const T*, T*, long, hipStream_t);
+    template hipError_t launch_enth_surface_from_temperature<T>(const T*, T*, long, hipStream_t);              \
+    template hipError_t launch_enth_adj_rows<T>(const EnthArgs<T>&, T*, hipStream_t);                                \
+    template hipError_t launch_enth_adj_rhs<T>(const EnthArgs<T>&, const T*, const T*, T, T, T*, hipStream_t);       \
+    template hipError_t launch_enth_adj_column<T>(const int*, int, int, int, const T*, const T*, const T*, T*, T,    \
+                                                  hipStream_t);                                                      \
+    template hipError_t launch_enth_adj_layer<T>(const int*, int, int, int, const T*, const T*, const T*, T*, T,     \
+                                                 hipStream_t);                                                       \
+    template hipError_t launch_enth_adj_residual<T>(const int*, int, int, int, const T*, const T*, const T*, T*,     \
+                                                    double*, EnthStatus*, int, double, double, hipStream_t);         \
+    template hipError_t launch_enth_adj_grads<T>(const EnthArgs<T>&, const T*, T*, T*, T*, T*, hipStream_t);
 INST(double)
 #undef INST
 

@@ -80,4 +80,26 @@ hipError_t launch_enth_from_temperature(const EnthArgs<T>& A, const T* temperatu
 template <typename T>
 hipError_t launch_enth_surface_from_temperature(const T* Ts, T* Es, long n, hipStream_t st);
 
+// Adjoint of the converged solve: exact Jacobian rows (7 planes of
+// (ny, nx, nz)), right-hand side with the rate-factor term, transposed
+// column / layer passes, residual with the sweep bookkeeping, gradients.
+template <typename T>
+hipError_t launch_enth_adj_rows(const EnthArgs<T>& A, T* rows, hipStream_t st);
+template <typename T>
+hipError_t launch_enth_adj_rhs(const EnthArgs<T>& A, const T* gE, const T* gB, T n_glen, T scale, T* g,
+                               hipStream_t st);
+template <typename T>
+hipError_t launch_enth_adj_column(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
+                                  const T* lam_in, T* lam_out, T omega_relax, hipStream_t st);
+template <typename T>
+hipError_t launch_enth_adj_layer(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
+                                 const T* lam_in, T* lam_out, T omega_relax, hipStream_t st);
+template <typename T>
+hipError_t launch_enth_adj_residual(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
+                                    const T* lam, T* r_out, double* partials, EnthStatus* status, int status_mode,
+                                    double rtol, double atol, hipStream_t st);
+template <typename T>
+hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev, T* g_phi, T* g_Q, T* g_Es,
+                                 hipStream_t st);
+
 }  // namespace glide

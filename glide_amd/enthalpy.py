@@ -298,6 +298,41 @@ class EnthalpyOperators:
         call("glide_enth_diagnostics", ctypes.byref(C), None, ptr(w), self._stream())
         return w
 
+    def adjoint_solve(self, dt, dL_dE, dL_dB=None, B_scale=1.0, n_iter=200, relative_tolerance=1e-12,
+                      absolute_tolerance=0.0, omega=1.0, report=False):
+        """Adjoint of the converged solve at the current E (no counterpart in
+        the reference): solves J^T lambda = dL/dE + (dB/dE)^T dL/dB with J the
+        exact Jacobian of the system the sweeps solve, B the rate factor
+        divided by ``B_scale``, and returns ``{"lambda", "E_prev",
+        "phi_strain", "Q_geo", "Q_fh", "E_surface"}`` gradients (``Q_fh`` and
+        ``Q_geo`` share one).  Exact for a converged forward solve."""
+        g = self.grid
+        ny, nx, nz = g.ny, g.nx, self.nz
+        if not hasattr(self, "_adj"):
+            z3 = lambda: g.zeros((ny, nx, nz))  # noqa: E731
+            self._adj = {"rows": g.zeros((7, ny, nx, nz)), "g": z3(), "lambda_work": z3(), "r": z3()}
+        W = self._adj
+        lam = g.zeros((ny, nx, nz))
+        out = {"E_prev": g.zeros((ny, nx, nz)), "phi_strain": g.zeros((ny, nx, nz)), "Q_geo": g.zeros((ny, nx)),
+               "E_surface": g.zeros((ny, nx))}
+        X = _lib.EnthAdjointC(rows=ptr(W["rows"]), g=ptr(W["g"]), lambda_=ptr(lam), lambda_work=ptr(W["lambda_work"]),
+                              r=ptr(W["r"]), g_E_prev=ptr(out["E_prev"]), g_phi_strain=ptr(out["phi_strain"]),
+                              g_Q_geo=ptr(out["Q_geo"]), g_E_surface=ptr(out["E_surface"]))
+        gE = _as_field(dL_dE, (ny, nx, nz), g.device)
+        gB = None if dL_dB is None else _as_field(dL_dB, (ny, nx), g.device)
+        cfg = ColumnSmootherConfig(omega=omega, relative_tolerance=relative_tolerance,
+                                   absolute_tolerance=absolute_tolerance).to_c(True)
+        rep = _lib.EnthReport() if report else None
+        C = self._column()
+        call("glide_enth_adjoint_solve", ctypes.byref(C), ctypes.byref(X), float(dt), ptr(gE), ptr(gB),
+             float(g.rheology.n.value), float(B_scale), int(n_iter), ctypes.byref(cfg),
+             ctypes.byref(rep) if rep is not None else None, self._stream())
+        out["lambda"] = lam
+        out["Q_fh"] = out["Q_geo"]
+        if rep is not None:
+            out["report"] = rep
+        return out
+
     def get_arrhenius_factor(self):
         """Depth-averaged B = A^(-1/n) (SI) from the Paterson-Budd law with the
         water-content enhancement (glide/enthalpy.py:600-643)."""

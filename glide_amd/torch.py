@@ -1,4 +1,5 @@
-"""torch.autograd surface of one SSA step (glide/torch.py:4-59).
+"""torch.autograd surfaces: one SSA step (glide/torch.py:4-59) and one
+thermal step.
 
 ``GlideStep.apply(t, dt, model, level, H_prev, bed, beta, smb) -> (u, v, H)``
 runs the native forward solve; its backward runs the native adjoint solve and
@@ -50,3 +51,42 @@ class GlideStep(torch.autograd.Function):
         g = mg[level]
         return (None, None, None, None, g.state.H_prev.grad.clone(), g.geometry.bed.grad.clone(),
                 g.sliding.beta.grad.clone(), g.forcing.smb.grad.clone())
+
+
+class ThermalStep(torch.autograd.Function):
+    """torch.autograd surface of one thermal step (no counterpart in the
+    reference, whose enthalpy solve has no adjoint).
+
+    ``ThermalStep.apply(thermal, dt, E_prev, Q_geo, E_surface) -> (E, B)`` sets
+    the previous enthalpy (also the initial iterate), the geothermal flux and
+    the surface enthalpy of ``thermal`` (a ``ThermalModel`` after its momentum
+    step and ``pre_momentum`` snapshot of H), runs ``thermal.step(dt)`` and
+    returns the new enthalpy and rate factor (head units; zeros when
+    ``update_rheology`` is off).  The backward solves the adjoint of the
+    converged solve at the returned state, so it is exact when the forward
+    sweeps converge (tighten ``thermal.ops.smoother_config`` tolerances and
+    ``thermal.n_smooth`` for gradient work); the thermal model must not step
+    again between the two passes."""
+
+    @staticmethod
+    def forward(ctx, thermal, dt, E_prev, Q_geo, E_surface):
+        ops = thermal.ops
+        ops.enthalpy_state.E_prev.copy_(E_prev.detach())
+        ops.enthalpy_state.E.copy_(E_prev.detach())
+        ops.enthalpy_forcing.Q_geo.copy_(Q_geo.detach())
+        ops.enthalpy_forcing.E_surface.copy_(E_surface.detach())
+        thermal.step(dt)
+        E = ops.enthalpy_state.E.clone()
+        B = ops.grid.rheology.B.data.clone() if thermal.update_rheology else torch.zeros_like(Q_geo)
+        ctx.thermal, ctx.dt = thermal, dt
+        return E, B
+
+    @staticmethod
+    def backward(ctx, gE, gB):
+        thermal = ctx.thermal
+        ops = thermal.ops
+        if gE is None:
+            gE = torch.zeros_like(ops.enthalpy_state.E)
+        use_B = gB is not None and thermal.update_rheology
+        g = ops.adjoint_solve(ctx.dt, gE.contiguous(), gB.contiguous() if use_B else None, B_scale=thermal.B_scale)
+        return None, None, g["E_prev"], g["Q_geo"], g["E_surface"]

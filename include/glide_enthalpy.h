@@ -40,6 +40,7 @@
  *   glide_enth_strain_heating    <- no counterpart (the reference's phi_strain is caller-supplied,
  *                                   glide/enthalpy.py:108-131); SSA dissipation for the coupled step
  *   glide_thermal_step           <- ThermalModel.step (glide/model.py:167-213)
+ *   glide_enth_adjoint_solve     <- no counterpart: adjoint of the converged enthalpy solve
  */
 #ifndef GLIDE_ENTHALPY_H
 #define GLIDE_ENTHALPY_H
@@ -141,6 +142,28 @@ int glide_enth_from_temperature(const glide_enth_column* C, const void* temperat
 /* E_surface_out = scaled enthalpy of the (ny, nx) temperature capped at T_MELT. */
 int glide_enth_surface_from_temperature(const glide_enth_column* C, const void* T_surface, void* E_surface_out,
                                         void* stream);
+/* Adjoint of the converged solve (no counterpart in the reference): buffers. */
+typedef struct glide_enth_adjoint {
+    void* rows;        /* workspace (7, ny, nx, nz): exact Jacobian rows */
+    void* g;           /* workspace (ny, nx, nz): right-hand side */
+    void* lambda;      /* (ny, nx, nz): initial guess in, adjoint state out */
+    void* lambda_work; /* workspace (ny, nx, nz) */
+    void* r;           /* workspace (ny, nx, nz): adjoint residual */
+    void *g_E_prev, *g_phi_strain; /* (ny, nx, nz) gradients, either may be NULL */
+    void *g_Q_geo, *g_E_surface;   /* (ny, nx) gradients (g_Q_geo is also dL/dQ_fh), either may be NULL */
+} glide_enth_adjoint;
+
+/* Adjoint of the thermal solve at the converged E of C: solves J^T lambda = g
+ * with J the exact Jacobian of the system the sweeps solve (interior rows,
+ * Dirichlet surface node, clamped thin columns) and g = dL/dE + (dB/dE)^T
+ * dL/dB (dL_dB may be NULL; B = rate factor / B_scale), by alternating
+ * transposed column and layer passes (relaxation cfg->omega) for up to n_iter
+ * sweeps with the device-side stopping test on max |g - J^T lambda|
+ * (cfg tolerances); then writes the parameter gradients. */
+int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoint* X, double dt, const void* dL_dE,
+                             const void* dL_dB, double n_glen, double B_scale, int n_iter,
+                             const glide_enth_sweep_config* cfg, glide_enth_report* report, void* stream);
+
 /* ThermalModel.step after pre_momentum: velocities u2d, v2d in m/a and H the
  * new thickness; fills u3d, v3d (m/s), dh_dt, omega, Q_fh (when
  * frictional_heating), phi_strain (when strain_heating, from the B given),

@@ -281,3 +281,72 @@ def test_thermal_step_host_path_matches_fused():
     tm2.step(DT)
     assert rel_err(N(tm2.ops.enthalpy_state.E), N(tm.ops.enthalpy_state.E)) < STOL
     assert rel_err(N(g2.rheology.B.data), N(g.rheology.B.data)) < STOL
+
+
+# ---------------------------------------------------------------- adjoint
+B_SCALE = 917.0 * 9.81 * (365.25 * 86400.0) ** (1.0 / 3.0)
+
+
+@pytest.mark.parametrize("ny,nx,nz,seed", CASES)
+def test_adjoint_solve_matches_oracle(ny, nx, nz, seed):
+    """GPU adjoint (iterative transposed column/layer passes) against the
+    oracle's direct sparse solve of J^T lambda = g at the same converged E."""
+    print(f"adjoint {ny}x{nx}x{nz} seed={seed}", flush=True)
+    th, ops = make_pair(enth_case(ny, nx, nz, seed), nz)
+    ops.smoother_config.relative_tolerance = 1e-15
+    ops.smoother_config.absolute_tolerance = 0.0
+    ops.column_sweep(DT, 200)
+    th.E = np.ascontiguousarray(N(ops.enthalpy_state.E))
+    rng = np.random.default_rng(seed + 11)
+    w = rng.standard_normal((ny, nx, nz))
+    gB = rng.standard_normal((ny, nx))
+    out = ops.adjoint_solve(DT, T(w), T(gB), B_scale=B_SCALE, relative_tolerance=1e-14, n_iter=250, report=True)
+    rep = out["report"]
+    g = w + th.arrhenius_vjp(gB, 3.0, B_SCALE)
+    lam = th.adjoint_solve(DT, g)
+    grads = th.adjoint_gradients(DT, lam)
+    print(f"  adjoint sweeps {rep.sweeps}, |r0| {rep.r0:.3e} -> {rep.r:.3e}", flush=True)
+    assert rel_err(N(out["lambda"]), lam) < 1e-9
+    for k in ("E_prev", "phi_strain", "Q_geo", "E_surface"):
+        assert rel_err(N(out[k]), grads[k]) < 1e-9, k
+
+
+def test_thermal_step_autograd_matches_finite_differences():
+    """ThermalStep backward (adjoint of the converged solve) against central
+    differences of complete forward steps in Q_geo, E_prev and E_surface,
+    with the rate-factor output in the loss."""
+    print("thermal step autograd", flush=True)
+    from glide_amd.torch import ThermalStep
+    F, g, tm, th = _thermal_pair(16, 11, 6)
+    tm.n_smooth = 200
+    cfg = tm.ops.smoother_config
+    cfg.relative_tolerance, cfg.absolute_tolerance = 1e-15, 0.0
+    ops = tm.ops
+    E_prev0 = ops.enthalpy_state.E.clone()
+    Q0 = ops.enthalpy_forcing.Q_geo.clone()
+    Es0 = ops.enthalpy_forcing.E_surface.clone()
+    rng = np.random.default_rng(4)
+    w, wB = T(rng.standard_normal(tuple(E_prev0.shape))), T(rng.standard_normal(tuple(Q0.shape)))
+
+    def loss(Ep, Q, Es):
+        E, B = ThermalStep.apply(tm, DT, Ep, Q, Es)
+        return (w * E).sum() + 1e-3 * (wB * B).sum()
+
+    Ep, Q, Es = (t.clone().requires_grad_(True) for t in (E_prev0, Q0, Es0))
+    L = loss(Ep, Q, Es)
+    L.backward()
+    H = N(g.state.H.data)
+    ti, tj = [int(v) for v in np.argwhere(H > 500.0)[0]]
+    checks = [("Q_geo", Q, Q0, (ti, tj), 1e-4), ("E_prev", Ep, E_prev0, (ti, tj, 3), 1e-5),
+              ("E_surface", Es, Es0, (ti, tj), 1e-5)]
+    for name, leaf, base, idx, h in checks:
+        def L_at(delta):
+            args = [E_prev0.clone(), Q0.clone(), Es0.clone()]
+            pos = {"E_prev": 0, "Q_geo": 1, "E_surface": 2}[name]
+            args[pos][idx] += delta
+            with torch.no_grad():
+                return float(loss(*args))
+        fd = (L_at(h) - L_at(-h)) / (2 * h)
+        ad = float(leaf.grad[idx])
+        print(f"  {name}{idx}: adjoint {ad:.10e}  fd {fd:.10e}", flush=True)
+        assert abs(fd - ad) <= 1e-5 * max(abs(fd), abs(ad)), name
