@@ -40,10 +40,10 @@ WORD = 8  # bytes per FP64 value
 #   transpose operator: 15 inputs + 3 outputs
 #   adjoint smoothing pass: 12 inputs + 3 outputs
 #   residual: 12 inputs + 3 outputs
-#   enthalpy column / layer smoother and residual, per column node: E, E_prev,
-#   f_E, omega, u3d, v3d in + one 3-D field out
-ALG_FIELDS = {"vanka_sweep": 16, "vjp": 18, "adjoint_sweep": 15, "residual": 15,
-              "enth_column": 7, "enth_layer": 7, "enth_residual": 7}
+#   enthalpy column / layer smoother, per column of nz nodes: E, E_prev, f_E,
+#   omega in and E out (5 nz) + five 2-D fields (u, v, H, H_prev, E_surface);
+#   enthalpy residual + max-norm: 4 nz + 5 per column (set in main from nz)
+ALG_FIELDS = {"vanka_sweep": 16, "vjp": 18, "adjoint_sweep": 15, "residual": 15}
 SEC_PER_YR = 365.25 * 86400.0
 
 
@@ -230,6 +230,10 @@ def main():
     for name in ("vanka_cell", "vanka_apply", "adjoint_cell", "adjoint_apply", "vjp", "residual", "enth_column",
                  "enth_layer", "enth_residual"):
         kern[name] = _lib.timing_read(name)
+    # finest-level launches only (args.n^2 cells each), read before disabling clears the totals
+    fine = args.n * args.n
+    fine_kern = {name: _lib.timing_read_units(name, fine)
+                 for name in ("vanka_cell", "vanka_apply", "adjoint_cell", "adjoint_apply", "vjp")}
     _lib.timing_enable(False)
 
     ms_step = 1000.0 * elapsed / args.steps
@@ -247,11 +251,10 @@ def main():
     }
     dominant = max(passes, key=lambda k: passes[k][1])
     # roofline of the dominant pass over the finest-level launches (args.n^2 cells each)
-    fine = args.n * args.n
     pair = {"vanka_sweep": ("vanka_cell", "vanka_apply"), "adjoint_sweep": ("adjoint_cell", "adjoint_apply"),
             "vjp": ("vjp",)}[dominant]
-    f_launches = _lib.timing_read_units(pair[0], fine)[0]
-    f_ms = sum(_lib.timing_read_units(k, fine)[1] for k in pair)
+    f_launches = fine_kern[pair[0]][0]
+    f_ms = sum(fine_kern[k][1] for k in pair)
     avg_ms = f_ms / max(f_launches, 1)
     alg_bytes = ALG_FIELDS[dominant] * WORD * fine
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -263,7 +266,10 @@ def main():
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "cells": v[2]} for k, v in kern.items()}
     for k, (c, ms, cells) in passes.items():
         kernels[k] = {"launches": c, "total_ms": round(ms, 3), "alg_GBps": round(bw(ALG_FIELDS[k], ms, cells), 2)}
-    for k in ("enth_column", "enth_layer", "enth_residual"):
+    if args.nz > 0:
+        ALG_FIELDS.update({"enth_column": 5 + 5 / args.nz, "enth_layer": 5 + 5 / args.nz,
+                           "enth_residual": 4 + 5 / args.nz})
+    for k in ("enth_column", "enth_layer", "enth_residual") if args.nz > 0 else ():
         c, ms, units = kern[k]
         kernels[k]["alg_GBps_incl_skipped"] = round(bw(ALG_FIELDS[k], ms, units), 2)
 
