@@ -119,6 +119,7 @@ SIGNATURES = {
     "glide_last_error": (_I, [ctypes.c_char_p, _S]),
     "glide_compute_grounded": (_I, [_LP, _D, _P]),
     "glide_ssa_residual": (_I, [_LP, _D, _I, _P, _P, _P, _P]),
+    "glide_ssa_jvp": (_I, [_LP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
     "glide_norms3": (_I, [_P, _S, _P, _S, _P, _S, ctypes.POINTER(_D), _P]),
     "glide_vanka_sweeps": (_I, [_LP, _D, _I, ctypes.POINTER(VankaConfigC), _I, _P]),
     "glide_vanka_dump": (_I, [_LP, _D, _P, _P, _P]),

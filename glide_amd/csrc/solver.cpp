@@ -575,6 +575,20 @@ int glide_ssa_residual(const glide_level* L, double dt, int flags, void* out_u, 
     return residual(*L, dt, flags, P(out_u), P(out_v), P(out_H), (hipStream_t)stream);
 }
 
+int glide_ssa_jvp(const glide_level* L, double dt, int flags, const void* d_u, const void* d_v, const void* d_H,
+                  void* out_u, void* out_v, void* out_H, void* stream) {
+    GLIDE_TRY(check_level(L));
+    GLIDE_TRY(check_ssa_inputs(*L));
+    REQ(d_u); REQ(d_v); REQ(d_H); REQ(out_u); REQ(out_v); REQ(out_H);
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    SsaArgs<R> a = make_args(*L, dt, false);
+    const double cr = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L->phys.calving_rate;
+    HIP_TRY(launch_jvp<R>(a, CP(d_u), CP(d_v), CP(d_H), P(out_u), P(out_v), P(out_H), (flags & GLIDE_USE_MASK) ? 1 : 0,
+                          cr, (hipStream_t)stream),
+            "jvp");
+    return GLIDE_OK;
+}
+
 int glide_norms3(const void* a, size_t na, const void* b, size_t nb, const void* c, size_t nc, double* out3,
                  void* stream) {
     if (!out3) return fail(GLIDE_ERR_INVALID_ARG, "out3 is NULL");

@@ -151,6 +151,146 @@ __global__ __launch_bounds__(NT) void k_residual(SsaArgs<T> a, T* __restrict__ o
     }
 }
 
+// ------------------------------------------------------------------ JVP
+// J(x) d of the coupled residual along d = (du, dv, dH) with the grounded
+// fraction, mask, bed, B and beta frozen (glide/cuda/residuals.cu:340-660):
+// the derivative of k_residual term by term.  Dirichlet facet rows are zero
+// and the thickness row is (1 - mask) times the mass-balance derivative, as
+// in the reference.  The tile holds each cell's viscosity and its derivative
+// along d.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_jvp(SsaArgs<T> a, const T* __restrict__ du, const T* __restrict__ dv,
+                                            const T* __restrict__ dH, T* __restrict__ out_u, T* __restrict__ out_v,
+                                            T* __restrict__ out_H, int use_mask, T calving_rate) {
+    __shared__ T s_eta[TY + 2][TX + 2];
+    __shared__ T s_etd[TY + 2][TX + 2];
+    const int ny = a.ny, nx = a.nx;
+    const T dxi = a.dxi;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        int ci = i0 - 1 + k / (TX + 2), cj = j0 - 1 + k % (TX + 2);
+        T e = T(0), ed = T(0);
+        if (ci >= 0 && ci < ny && cj >= 0 && cj < nx) {
+            CellVel<T> c = load_cell_vel(a.u, a.v, ci, cj, ny, nx);
+            CellVel<T> d = load_cell_vel(du, dv, ci, cj, ny, nx);
+            T e2 = strain_invariant(c, ci, cj, ny, nx, dxi, a.P.eps_reg);
+            T de2 = strain_invariant_dir(c, d, ci, cj, ny, nx, dxi);
+            e = T(0.5) * a.B[(long)ci * nx + cj] * pow(e2, a.P.glen_p);
+            ed = e * a.P.glen_p * de2 / e2;
+        }
+        s_eta[k / (TX + 2)][k % (TX + 2)] = e;
+        s_etd[k / (TX + 2)][k % (TX + 2)] = ed;
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    const int li = threadIdx.y + 1, lj = threadIdx.x + 1;
+    const T* __restrict__ u = a.u;
+    const T* __restrict__ v = a.v;
+    const T* __restrict__ H = a.H;
+    const T* __restrict__ ph = a.phi;
+    // eta H of tile cell (ti, tj) at grid cell c and its derivative along d
+    auto eH = [&](int ti, int tj, long c) { return s_eta[ti][tj] * H[c]; };
+    auto deH = [&](int ti, int tj, long c) { return s_etd[ti][tj] * H[c] + s_eta[ti][tj] * dH[c]; };
+
+    if (i < ny && j < nx) {
+        const long c = (long)i * nx + j;
+        T Hc = H[c], pc = ph[c], dHc = dH[c];
+        const long cl = offC(i, j - 1, ny, nx), cr = offC(i, j + 1, ny, nx);
+        const long ct = offC(i - 1, j, ny, nx), cb = offC(i + 1, j, ny, nx);
+        T d = dHc / a.dt;
+        Flux<T> f = flux_x(u[offU(i, j, ny, nx)], H[cl], Hc, j > 0);
+        d -= (f.d_w * du[offU(i, j, ny, nx)] + f.d_HA * dH[cl] + f.d_HB * dHc) * dxi;
+        d += (T(1) - pc) * (T(1) - ph[cl]) * calving_rate * dHc * dxi;
+        f = flux_x(u[offU(i, j + 1, ny, nx)], Hc, H[cr], j + 1 < nx);
+        d += (f.d_w * du[offU(i, j + 1, ny, nx)] + f.d_HA * dHc + f.d_HB * dH[cr]) * dxi;
+        d += (T(1) - pc) * (T(1) - ph[cr]) * calving_rate * dHc * dxi;
+        f = flux_y(v[offV(i, j, ny, nx)], H[ct], Hc, i > 0);
+        d += (f.d_w * dv[offV(i, j, ny, nx)] + f.d_HA * dH[ct] + f.d_HB * dHc) * dxi;
+        d += (T(1) - pc) * (T(1) - ph[ct]) * calving_rate * dHc * dxi;
+        f = flux_y(v[offV(i + 1, j, ny, nx)], Hc, H[cb], i + 1 < ny);
+        d -= (f.d_w * dv[offV(i + 1, j, ny, nx)] + f.d_HA * dHc + f.d_HB * dH[cb]) * dxi;
+        d += (T(1) - pc) * (T(1) - ph[cb]) * calving_rate * dHc * dxi;
+        T mk = use_mask ? a.mask[c] : T(0);
+        out_H[c] = (T(1) - mk) * d;
+    }
+    if (i < ny) {
+        const long o = (long)i * (nx + 1) + j;
+        T d = T(0);
+        if (j > 0 && j < nx) {
+            const long c = (long)i * nx + j;
+            T ul = u[o], dul = du[o];
+            T vt = v[c], vb = v[c + nx], vlt = v[c - 1], vlb = v[c + nx - 1];
+            T dvt = dv[c], dvb = dv[c + nx], dvlt = dv[c - 1], dvlb = dv[c + nx - 1];
+            NormalStress<T> s = sxx(ul, u[o + 1], vt, vb, eH(li, lj, c), dxi, true);
+            d += (s.d_ul * dul + s.d_ur * du[o + 1] + s.d_vt * dvt + s.d_vb * dvb + s.d_eH * deH(li, lj, c)) * dxi;
+            s = sxx(u[o - 1], ul, vlt, vlb, eH(li, lj - 1, c - 1), dxi, true);
+            d -= (s.d_ul * du[o - 1] + s.d_ur * dul + s.d_vt * dvlt + s.d_vb * dvlb +
+                  s.d_eH * deH(li, lj - 1, c - 1)) * dxi;
+            if (i > 0) {
+                T e = T(0.25) * (eH(li - 1, lj - 1, c - nx - 1) + eH(li - 1, lj, c - nx) + eH(li, lj - 1, c - 1) +
+                                 eH(li, lj, c));
+                T de = T(0.25) * (deH(li - 1, lj - 1, c - nx - 1) + deH(li - 1, lj, c - nx) +
+                                  deH(li, lj - 1, c - 1) + deH(li, lj, c));
+                ShearStress<T> t = sxy(u[o - (nx + 1)], ul, vlt, vt, e, dxi, true);
+                d += (t.d_ut * du[o - (nx + 1)] + t.d_ub * dul + t.d_vl * dvlt + t.d_vr * dvt + t.d_eH * de) * dxi;
+            }
+            if (i + 1 < ny) {
+                T e = T(0.25) * (eH(li, lj - 1, c - 1) + eH(li, lj, c) + eH(li + 1, lj - 1, c + nx - 1) +
+                                 eH(li + 1, lj, c + nx));
+                T de = T(0.25) * (deH(li, lj - 1, c - 1) + deH(li, lj, c) + deH(li + 1, lj - 1, c + nx - 1) +
+                                  deH(li + 1, lj, c + nx));
+                ShearStress<T> t = sxy(ul, u[o + (nx + 1)], vlb, vb, e, dxi, true);
+                d -= (t.d_ut * dul + t.d_ub * du[o + (nx + 1)] + t.d_vl * dvlb + t.d_vr * dvb + t.d_eH * de) * dxi;
+            }
+            T pl = ph[c - 1], pc = ph[c];
+            Drag<T> g = basal_drag(ul, vlt, vt, vlb, vb, pl, pc, a.beta[c - 1], a.beta[c], a.P);
+            d += g.d_w * dul + g.c0 * dvlt + g.c1 * dvt + g.c2 * dvlb + g.c3 * dvb;
+            Drive<T> r = drive_x(H[c - 1], H[c], a.bed[c - 1], a.bed[c], pl, pc, dxi, true);
+            d -= r.d_HA * dH[c - 1] + r.d_HB * dH[c];
+        }
+        out_u[o] = d;
+    }
+    if (j < nx) {
+        const long o = (long)i * nx + j;
+        T d = T(0);
+        if (i > 0 && i < ny) {
+            const long ou = (long)i * (nx + 1) + j, ot = ou - (nx + 1);
+            T vt = v[o], dvt = dv[o];
+            T ul = u[ou], ur = u[ou + 1], utl = u[ot], utr = u[ot + 1];
+            T dul = du[ou], dur = du[ou + 1], dutl = du[ot], dutr = du[ot + 1];
+            NormalStress<T> s = syy(utl, utr, v[o - nx], vt, eH(li - 1, lj, o - nx), dxi, true);
+            d += (s.d_ul * dutl + s.d_ur * dutr + s.d_vt * dv[o - nx] + s.d_vb * dvt +
+                  s.d_eH * deH(li - 1, lj, o - nx)) * dxi;
+            s = syy(ul, ur, vt, v[o + nx], eH(li, lj, o), dxi, true);
+            d -= (s.d_ul * dul + s.d_ur * dur + s.d_vt * dvt + s.d_vb * dv[o + nx] + s.d_eH * deH(li, lj, o)) * dxi;
+            if (j > 0) {
+                T e = T(0.25) * (eH(li - 1, lj - 1, o - nx - 1) + eH(li - 1, lj, o - nx) + eH(li, lj - 1, o - 1) +
+                                 eH(li, lj, o));
+                T de = T(0.25) * (deH(li - 1, lj - 1, o - nx - 1) + deH(li - 1, lj, o - nx) +
+                                  deH(li, lj - 1, o - 1) + deH(li, lj, o));
+                ShearStress<T> t = sxy(utl, ul, v[o - 1], vt, e, dxi, true);
+                d -= (t.d_ut * dutl + t.d_ub * dul + t.d_vl * dv[o - 1] + t.d_vr * dvt + t.d_eH * de) * dxi;
+            }
+            if (j + 1 < nx) {
+                T e = T(0.25) * (eH(li - 1, lj, o - nx) + eH(li - 1, lj + 1, o - nx + 1) + eH(li, lj, o) +
+                                 eH(li, lj + 1, o + 1));
+                T de = T(0.25) * (deH(li - 1, lj, o - nx) + deH(li - 1, lj + 1, o - nx + 1) + deH(li, lj, o) +
+                                  deH(li, lj + 1, o + 1));
+                ShearStress<T> t = sxy(utr, ur, vt, v[o + 1], e, dxi, true);
+                d += (t.d_ut * dutr + t.d_ub * dur + t.d_vl * dvt + t.d_vr * dv[o + 1] + t.d_eH * de) * dxi;
+            }
+            T pt = ph[o - nx], pc = ph[o];
+            Drag<T> g = basal_drag(vt, utl, utr, ul, ur, pt, pc, a.beta[o - nx], a.beta[o], a.P);
+            d += g.d_w * dvt + g.c0 * dutl + g.c1 * dutr + g.c2 * dul + g.c3 * dur;
+            Drive<T> r = drive_y(H[o - nx], H[o], a.bed[o - nx], a.bed[o], pt, pc, dxi, true);
+            d -= r.d_HA * dH[o - nx] + r.d_HB * dH[o];
+        }
+        out_v[o] = d;
+    }
+}
+
 // ------------------------------------------------------------------ Vanka
 // Gather the patch of cell (i,j); viscosity from the LDS tile (li, lj are the
 // cell's tile coordinates including the one-cell ring).
@@ -702,6 +842,14 @@ hipError_t launch_residual(const SsaArgs<T>& a, T* ou, T* ov, T* oH, int use_for
 }
 
 template <typename T>
+hipError_t launch_jvp(const SsaArgs<T>& a, const T* du, const T* dv, const T* dH, T* ou, T* ov, T* oH, int use_mask,
+                      T calving_rate, hipStream_t st) {
+    k_jvp<T><<<tile_grid(a.ny, a.nx, true), dim3(TX, TY), 0, st>>>(a, du, dv, dH, ou, ov, oH, use_mask,
+                                                                     calving_rate);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* mask, const VankaArgs<T>& vk, T omega,
                              hipStream_t st) {
     {
@@ -759,6 +907,8 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
 #define GLIDE_INSTANTIATE(T)                                                                                         \
     template hipError_t launch_grounded<T>(T*, const T*, const T*, T, T, long, hipStream_t);                          \
     template hipError_t launch_residual<T>(const SsaArgs<T>&, T*, T*, T*, int, int, T, hipStream_t);                  \
+    template hipError_t launch_jvp<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, int, T,      \
+                                      hipStream_t);                                                           \
     template hipError_t launch_vanka_pass<T>(const SsaArgs<T>&, T*, T*, T*, T*, T*, const VankaArgs<T>&, T,           \
                                              hipStream_t);                                                           \
     template hipError_t launch_vanka_dump<T>(const SsaArgs<T>&, T*, T*, T, hipStream_t);                              \

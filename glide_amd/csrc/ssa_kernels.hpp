@@ -32,6 +32,9 @@ template <typename T>
 hipError_t launch_residual(const SsaArgs<T>& a, T* ou, T* ov, T* oH, int use_forcing, int use_mask, T calving_rate,
                            hipStream_t st);
 template <typename T>
+hipError_t launch_jvp(const SsaArgs<T>& a, const T* du, const T* dv, const T* dH, T* ou, T* ov, T* oH, int use_mask,
+                      T calving_rate, hipStream_t st);
+template <typename T>
 hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* mask, const VankaArgs<T>& vk, T omega,
                              hipStream_t st);
 template <typename T>

@@ -69,9 +69,62 @@ class ForwardOperators:
         self.F_u, self.F_v, self.F_H = z((ny, nx + 1)), z((ny + 1, nx)), z((ny, nx))
         self.gamma = z((ny, nx)).fill_(grid.geometry.thklim.value)
         self.vanka_config = VankaConfig()
+        self._var = None  # direction and product of compute_jvp, allocated on first use
+        self._jvp = None
 
     def _level(self):
         return self.grid.level_struct()
+
+    def _pair(self, which):
+        g = self.grid
+        if getattr(self, which) is None:
+            setattr(self, which, (g.zeros((g.ny, g.nx + 1)), g.zeros((g.ny + 1, g.nx)), g.zeros((g.ny, g.nx))))
+        return getattr(self, which)
+
+    @property
+    def var_u(self):
+        """Direction of compute_jvp, u-facets (glide/operators.py:278-282)."""
+        return self._pair("_var")[0]
+
+    @property
+    def var_v(self):
+        return self._pair("_var")[1]
+
+    @property
+    def var_H(self):
+        return self._pair("_var")[2]
+
+    @property
+    def jvp_u(self):
+        """J var of the last compute_jvp, u-facets (glide/operators.py:296-312)."""
+        return self._pair("_jvp")[0]
+
+    @property
+    def jvp_v(self):
+        return self._pair("_jvp")[1]
+
+    @property
+    def jvp_H(self):
+        return self._pair("_jvp")[2]
+
+    def compute_jvp(self, dt, use_mask=True, freeze_calving=False, freeze_phi=False, return_norms=False):
+        """jvp = J(x) var, the residual linearised at the current state along
+        (var_u, var_v, var_H) with phi, mask, bed, B and beta frozen
+        (glide/operators.py:129-171, glide/cuda/residuals.cu:340-660).  Rows of
+        Dirichlet facets are zero and the thickness row is (1 - mask) times
+        the mass-balance derivative, as in the reference."""
+        st = stream_handle(self.grid.device)
+        if not freeze_phi:
+            self.compute_phi()
+        L = self._level()
+        flags = (USE_MASK if use_mask else 0) | (FREEZE_CALVING if freeze_calving else 0)
+        d = self._pair("_var")
+        out = self._pair("_jvp")
+        call("glide_ssa_jvp", ctypes.byref(L), float(dt), flags, ptr(d[0]), ptr(d[1]), ptr(d[2]), ptr(out[0]),
+             ptr(out[1]), ptr(out[2]), st)
+        if return_norms:
+            return _norms(out[0], out[1], out[2], st)
+        return None
 
     def compute_residual(self, dt, use_mask=True, operator_only=False, freeze_calving=False, freeze_phi=False,
                          return_norms=False):

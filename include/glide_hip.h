@@ -16,6 +16,8 @@
  *                                 (glide/operators.py:173-192, glide/cuda/viscosity.cu:16-36)
  *   glide_ssa_residual         <- ForwardOperators.compute_residual / kernel compute_residual
  *                                 (glide/operators.py:71-127, glide/cuda/residuals.cu:5-333)
+ *   glide_ssa_jvp              <- ForwardOperators.compute_jvp / kernel compute_jvp
+ *                                 (glide/operators.py:129-171, glide/cuda/residuals.cu:340-660)
  *   glide_vanka_sweeps         <- ForwardOperators.vanka_sweep / kernel vanka_smooth
  *                                 (glide/operators.py:194-251, glide/cuda/vanka.cu:477-660)
  *   glide_vanka_dump           <- ForwardOperators.vanka_dump (glide/operators.py:253-273,
@@ -129,6 +131,12 @@ int glide_last_error(char* buf, size_t len);
 int glide_compute_grounded(const glide_level* L, double relaxation, void* stream);
 int glide_ssa_residual(const glide_level* L, double dt, int flags, void* out_u, void* out_v, void* out_H,
                        void* stream);
+/* J(x) d of the residual along d = (d_u, d_v, d_H) with phi, mask, bed, B and
+ * beta frozen.  As in the reference: Dirichlet facet rows are zero and the
+ * thickness row is (1 - mask) times the mass-balance derivative when
+ * GLIDE_USE_MASK is set; GLIDE_FREEZE_CALVING drops the calving term. */
+int glide_ssa_jvp(const glide_level* L, double dt, int flags, const void* d_u, const void* d_v, const void* d_H,
+                  void* out_u, void* out_v, void* out_H, void* stream);
 /* |a|_2, |b|_2, |c|_2 into host memory out3 (synchronises the stream). */
 int glide_norms3(const void* a, size_t na, const void* b, size_t nb, const void* c, size_t nc, double* out3,
                  void* stream);

@@ -117,6 +117,22 @@ def residual(F, P: Phys, dx, dt, use_forcing=True, use_mask=True):
     return ru, rv, rH
 
 
+def jvp(F, du, dv, dH, P: Phys, dx, dt, use_mask=True):
+    """J d of the coupled residual with phi, mask, bed, B, beta frozen; zero
+    rows on Dirichlet facets and (1 - mask) times the mass-balance derivative
+    (glide/cuda/residuals.cu:340-660).  Pinned by central differences of
+    residual() and by the dot product with vjp() (tests/test_oracle_ssa.py)."""
+    ny, nx = F["H"].shape
+    du = np.ascontiguousarray(du, dtype=np.float64)
+    dv = np.ascontiguousarray(dv, dtype=np.float64)
+    dH = np.ascontiguousarray(dH, dtype=np.float64)
+    ou, ov, oH = np.zeros((ny, nx + 1)), np.zeros((ny + 1, nx)), np.zeros((ny, nx))
+    lib().orc_jvp(_p(ou), _p(ov), _p(oH), _p(F["u"]), _p(F["v"]), _p(F["H"]), _p(du), _p(dv), _p(dH), _p(F["phi"]),
+                  _p(F["mask"]), _p(F["bed"]), _p(F["B"]), _p(F["beta"]), _I(int(use_mask)), ctypes.byref(P),
+                  _D(dx), _D(dt), ny, nx)
+    return ou, ov, oH
+
+
 def viscosity(u, v, B, n, eps_reg, dx, du=None, dv=None):
     ny, nx = B.shape
     eta, etad = np.zeros((ny, nx)), np.zeros((ny, nx))

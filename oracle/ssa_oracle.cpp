@@ -361,6 +361,125 @@ void residual(const Fields& F, double* r_u, double* r_v, double* r_H, bool use_f
     }
 }
 
+// ------------------------------------------------------------------ JVP
+// Directional derivative J(x) d of the coupled residual along d = (du, dv, dH)
+// with the grounded fraction, the mask, bed, B and beta frozen
+// (glide/cuda/residuals.cu:340-660).  Term by term the derivative of
+// residual() above, with two reference conventions: rows of Dirichlet facets
+// are zero (not d itself, residuals.cu:567,659), and the thickness row is
+// (1 - mask) times the derivative of the mass balance, without the mask's
+// H - gamma term (residuals.cu:437).
+void jvp(const Fields& F, const double* du, const double* dv, const double* dH, double* o_u, double* o_v,
+         double* o_H, bool use_mask, const orc_phys& P, double dx, double dt, int ny, int nx) {
+    Acc A{ny, nx};
+    const double dxi = 1.0 / dx;
+    std::vector<double> eta, etd;
+    viscosity_field(A, F.u, F.v, du, dv, F.B, P.n, P.eps_reg, dx, eta, &etd);
+    auto ETA = [&](int i, int j) { return eta[(size_t)clampi(i, 0, ny - 1) * nx + clampi(j, 0, nx - 1)]; };
+    auto ETD = [&](int i, int j) { return etd[(size_t)clampi(i, 0, ny - 1) * nx + clampi(j, 0, nx - 1)]; };
+    // eta H of a cell and its derivative
+    auto eH = [&](int i, int j) { return ETA(i, j) * A.C(F.H, i, j); };
+    auto deH = [&](int i, int j) { return ETD(i, j) * A.C(F.H, i, j) + ETA(i, j) * A.C(dH, i, j); };
+    // vertex (vi,vj) average over cells (vi-1,vj-1),(vi-1,vj),(vi,vj-1),(vi,vj)
+    auto eHv = [&](int vi, int vj) {
+        return 0.25 * (eH(vi - 1, vj - 1) + eH(vi - 1, vj) + eH(vi, vj - 1) + eH(vi, vj));
+    };
+    auto deHv = [&](int vi, int vj) {
+        return 0.25 * (deH(vi - 1, vj - 1) + deH(vi - 1, vj) + deH(vi, vj - 1) + deH(vi, vj));
+    };
+    const double cr = P.calving_rate;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i <= ny; ++i) {
+        for (int j = 0; j <= nx; ++j) {
+            if (i < ny && j < nx) {
+                double Hc = A.C(F.H, i, j), pc = A.C(F.phi, i, j), k;
+                double dHc = A.C(dH, i, j);
+                double d = dHc / dt;
+                Flux f = flux_x(A.U(F.u, i, j), A.C(F.H, i, j - 1), Hc, j > 0 && j < nx);
+                d -= (f.d_w * A.U(du, i, j) + f.d_HA * A.C(dH, i, j - 1) + f.d_HB * dHc) * dxi;
+                calving(Hc, pc, A.C(F.phi, i, j - 1), cr, &k);
+                d += k * dHc * dxi;
+                f = flux_x(A.U(F.u, i, j + 1), Hc, A.C(F.H, i, j + 1), j + 1 > 0 && j + 1 < nx);
+                d += (f.d_w * A.U(du, i, j + 1) + f.d_HA * dHc + f.d_HB * A.C(dH, i, j + 1)) * dxi;
+                calving(Hc, pc, A.C(F.phi, i, j + 1), cr, &k);
+                d += k * dHc * dxi;
+                f = flux_y(A.V(F.v, i, j), A.C(F.H, i - 1, j), Hc, i > 0 && i < ny);
+                d += (f.d_w * A.V(dv, i, j) + f.d_HA * A.C(dH, i - 1, j) + f.d_HB * dHc) * dxi;
+                calving(Hc, pc, A.C(F.phi, i - 1, j), cr, &k);
+                d += k * dHc * dxi;
+                f = flux_y(A.V(F.v, i + 1, j), Hc, A.C(F.H, i + 1, j), i + 1 > 0 && i + 1 < ny);
+                d -= (f.d_w * A.V(dv, i + 1, j) + f.d_HA * dHc + f.d_HB * A.C(dH, i + 1, j)) * dxi;
+                calving(Hc, pc, A.C(F.phi, i + 1, j), cr, &k);
+                d += k * dHc * dxi;
+                double mk = use_mask ? A.C(F.mask, i, j) : 0.0;
+                o_H[(size_t)i * nx + j] = (1.0 - mk) * d;
+            }
+            if (i < ny) {  // u facet (i,j)
+                double d = 0.0;
+                if (j > 0 && j < nx) {
+                    Normal s = sigma_xx(A.U(F.u, i, j), A.U(F.u, i, j + 1), A.V(F.v, i, j), A.V(F.v, i + 1, j), eH(i, j),
+                                        dxi, j, nx);
+                    d += (s.d_ul * A.U(du, i, j) + s.d_ur * A.U(du, i, j + 1) + s.d_vt * A.V(dv, i, j) +
+                          s.d_vb * A.V(dv, i + 1, j) + s.d_etaH * deH(i, j)) * dxi;
+                    s = sigma_xx(A.U(F.u, i, j - 1), A.U(F.u, i, j), A.V(F.v, i, j - 1), A.V(F.v, i + 1, j - 1),
+                                 eH(i, j - 1), dxi, j - 1, nx);
+                    d -= (s.d_ul * A.U(du, i, j - 1) + s.d_ur * A.U(du, i, j) + s.d_vt * A.V(dv, i, j - 1) +
+                          s.d_vb * A.V(dv, i + 1, j - 1) + s.d_etaH * deH(i, j - 1)) * dxi;
+                    Shear t = sigma_xy(A.U(F.u, i - 1, j), A.U(F.u, i, j), A.V(F.v, i, j - 1), A.V(F.v, i, j), eHv(i, j),
+                                       dxi, i, j, ny, nx);
+                    d += (t.d_ut * A.U(du, i - 1, j) + t.d_ub * A.U(du, i, j) + t.d_vl * A.V(dv, i, j - 1) +
+                          t.d_vr * A.V(dv, i, j) + t.d_etaH * deHv(i, j)) * dxi;
+                    t = sigma_xy(A.U(F.u, i, j), A.U(F.u, i + 1, j), A.V(F.v, i + 1, j - 1), A.V(F.v, i + 1, j),
+                                 eHv(i + 1, j), dxi, i + 1, j, ny, nx);
+                    d -= (t.d_ut * A.U(du, i, j) + t.d_ub * A.U(du, i + 1, j) + t.d_vl * A.V(dv, i + 1, j - 1) +
+                          t.d_vr * A.V(dv, i + 1, j) + t.d_etaH * deHv(i + 1, j)) * dxi;
+                    double c[4] = {A.V(F.v, i, j - 1), A.V(F.v, i, j), A.V(F.v, i + 1, j - 1), A.V(F.v, i + 1, j)};
+                    double dc[4] = {A.V(dv, i, j - 1), A.V(dv, i, j), A.V(dv, i + 1, j - 1), A.V(dv, i + 1, j)};
+                    Drag g = basal_drag(A.U(F.u, i, j), c, A.C(F.phi, i, j - 1), A.C(F.phi, i, j),
+                                        A.C(F.beta, i, j - 1), A.C(F.beta, i, j), P);
+                    d += g.d_w * A.U(du, i, j) + g.d_c[0] * dc[0] + g.d_c[1] * dc[1] + g.d_c[2] * dc[2] +
+                         g.d_c[3] * dc[3];
+                    Drive r = driving_x(A.C(F.H, i, j - 1), A.C(F.H, i, j), A.C(F.bed, i, j - 1), A.C(F.bed, i, j),
+                                        A.C(F.phi, i, j - 1), A.C(F.phi, i, j), dxi, true);
+                    d -= r.d_HA * A.C(dH, i, j - 1) + r.d_HB * A.C(dH, i, j);
+                }
+                o_u[(size_t)i * (nx + 1) + j] = d;
+            }
+            if (j < nx) {  // v facet (i,j)
+                double d = 0.0;
+                if (i > 0 && i < ny) {
+                    Normal s = sigma_yy(A.U(F.u, i - 1, j), A.U(F.u, i - 1, j + 1), A.V(F.v, i - 1, j), A.V(F.v, i, j),
+                                        eH(i - 1, j), dxi, i - 1, ny);
+                    d += (s.d_ul * A.U(du, i - 1, j) + s.d_ur * A.U(du, i - 1, j + 1) + s.d_vt * A.V(dv, i - 1, j) +
+                          s.d_vb * A.V(dv, i, j) + s.d_etaH * deH(i - 1, j)) * dxi;
+                    s = sigma_yy(A.U(F.u, i, j), A.U(F.u, i, j + 1), A.V(F.v, i, j), A.V(F.v, i + 1, j), eH(i, j), dxi,
+                                 i, ny);
+                    d -= (s.d_ul * A.U(du, i, j) + s.d_ur * A.U(du, i, j + 1) + s.d_vt * A.V(dv, i, j) +
+                          s.d_vb * A.V(dv, i + 1, j) + s.d_etaH * deH(i, j)) * dxi;
+                    Shear t = sigma_xy(A.U(F.u, i - 1, j), A.U(F.u, i, j), A.V(F.v, i, j - 1), A.V(F.v, i, j), eHv(i, j),
+                                       dxi, i, j, ny, nx);
+                    d -= (t.d_ut * A.U(du, i - 1, j) + t.d_ub * A.U(du, i, j) + t.d_vl * A.V(dv, i, j - 1) +
+                          t.d_vr * A.V(dv, i, j) + t.d_etaH * deHv(i, j)) * dxi;
+                    t = sigma_xy(A.U(F.u, i - 1, j + 1), A.U(F.u, i, j + 1), A.V(F.v, i, j), A.V(F.v, i, j + 1),
+                                 eHv(i, j + 1), dxi, i, j + 1, ny, nx);
+                    d += (t.d_ut * A.U(du, i - 1, j + 1) + t.d_ub * A.U(du, i, j + 1) + t.d_vl * A.V(dv, i, j) +
+                          t.d_vr * A.V(dv, i, j + 1) + t.d_etaH * deHv(i, j + 1)) * dxi;
+                    double c[4] = {A.U(F.u, i - 1, j), A.U(F.u, i - 1, j + 1), A.U(F.u, i, j), A.U(F.u, i, j + 1)};
+                    double dc[4] = {A.U(du, i - 1, j), A.U(du, i - 1, j + 1), A.U(du, i, j), A.U(du, i, j + 1)};
+                    Drag g = basal_drag(A.V(F.v, i, j), c, A.C(F.phi, i - 1, j), A.C(F.phi, i, j),
+                                        A.C(F.beta, i - 1, j), A.C(F.beta, i, j), P);
+                    d += g.d_w * A.V(dv, i, j) + g.d_c[0] * dc[0] + g.d_c[1] * dc[1] + g.d_c[2] * dc[2] +
+                         g.d_c[3] * dc[3];
+                    Drive r = driving_y(A.C(F.H, i - 1, j), A.C(F.H, i, j), A.C(F.bed, i - 1, j), A.C(F.bed, i, j),
+                                        A.C(F.phi, i - 1, j), A.C(F.phi, i, j), dxi, true);
+                    d -= r.d_HA * A.C(dH, i - 1, j) + r.d_HB * A.C(dH, i, j);
+                }
+                o_v[(size_t)i * nx + j] = d;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ local 5x5
 // Local patch system of cell (i,j) for the unknowns
 // x = (u_l, u_r, v_t, v_b, H_c) with viscosity frozen at `eta`
@@ -1285,6 +1404,14 @@ void orc_residual(double* r_u, double* r_v, double* r_H, const double* u, const 
                   int use_mask, const orc_phys* P, double dx, double dt, int ny, int nx) {
     Fields F{u, v, H, phi, mask, f_u, f_v, f_H, bed, B, beta, gamma};
     residual(F, r_u, r_v, r_H, use_forcing != 0, use_mask != 0, *P, dx, dt, ny, nx);
+}
+
+void orc_jvp(double* o_u, double* o_v, double* o_H, const double* u, const double* v, const double* H,
+             const double* du, const double* dv, const double* dH, const double* phi, const double* mask,
+             const double* bed, const double* B, const double* beta, int use_mask, const orc_phys* P, double dx,
+             double dt, int ny, int nx) {
+    Fields F{u, v, H, phi, mask, nullptr, nullptr, nullptr, bed, B, beta, nullptr};
+    jvp(F, du, dv, dH, o_u, o_v, o_H, use_mask != 0, *P, dx, dt, ny, nx);
 }
 
 // SSA strain heating (no counterpart in the reference, which leaves

@@ -59,6 +59,7 @@ def test_version_and_error_text():
     st = lib.glide_ssa_residual(None, 1.0, 0, None, None, None, None)
     assert st == -1
     assert "NULL" in _lib.last_error()
+    assert lib.glide_ssa_jvp(None, 1.0, 0, None, None, None, None, None, None, None) == -1
     L = _lib.Level()
     L.ny, L.nx, L.dx = 1, 8, 100.0
     assert lib.glide_compute_grounded(ctypes.byref(L), 0.0, None) == -2

@@ -183,6 +183,68 @@ def test_vjp_dot_product(orc, m):
     assert abs(lhs - rhs) <= 1e-7 * abs(rhs)
 
 
+@pytest.mark.parametrize("m", [1.0, 1.0 / 3.0])
+@pytest.mark.parametrize("calving", [0.0, 0.7])
+def test_jvp_vs_central_fd(orc, m, calving):
+    """J d against central differences of the residual with phi frozen
+    (jvp_test.py restated with an assertion): interior facet rows and every
+    thickness row; the error of the difference quotient is O(eps^2)."""
+    F = dome_fields(ny=13, nx=17, seed=21)
+    P = orc.phys(m=m, calving_rate=calving)
+    rng = np.random.default_rng(22)
+    ny, nx = F["H"].shape
+    d = (rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx)), rng.standard_normal((ny, nx)))
+    ju, jv, jH = orc.jvp(F, *d, P, DX, DT, use_mask=False)
+    fu, fv, fH = _fd_jvp(orc, F, P, d, 1e-3)
+    assert rel_err(ju[:, 1:-1], fu[:, 1:-1]) < 1e-7
+    assert rel_err(jv[1:-1], fv[1:-1]) < 1e-7
+    assert rel_err(jH, fH) < 1e-7
+    # each term family moves the result: dropping the thickness direction changes every row kind
+    ju0, jv0, jH0 = orc.jvp(F, d[0], d[1], np.zeros_like(d[2]), P, DX, DT, use_mask=False)
+    assert rel_err(ju0[:, 1:-1], ju[:, 1:-1]) > 1e-3 and rel_err(jH0, jH) > 1e-3
+
+
+def test_jvp_reference_conventions(orc):
+    """Dirichlet facet rows are zero (not d), and a masked cell's thickness row
+    is zero even where d_H is not (glide/cuda/residuals.cu:437,567,659)."""
+    F = dome_fields(ny=12, nx=11, seed=23, mask_frac=0.3)
+    P = orc.phys()
+    rng = np.random.default_rng(24)
+    ny, nx = F["H"].shape
+    d = (rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx)), rng.standard_normal((ny, nx)))
+    ju, jv, jH = orc.jvp(F, *d, P, DX, DT, use_mask=True)
+    assert np.all(ju[:, 0] == 0) and np.all(ju[:, -1] == 0)
+    assert np.all(jv[0] == 0) and np.all(jv[-1] == 0)
+    m = F["mask"] > 0.5
+    assert m.any() and np.all(jH[m] == 0)
+    _, _, jH_nomask = orc.jvp(F, *d, P, DX, DT, use_mask=False)
+    assert np.array_equal(jH[~m], jH_nomask[~m])
+
+
+@pytest.mark.parametrize("m", [1.0, 1.0 / 3.0])
+def test_jvp_vjp_dot_product(orc, m):
+    """<lam, J d> == <J^T lam, d> between the oracle's own JVP and VJP
+    (adjoint_test.py restated with an assertion)."""
+    F = dome_fields(ny=14, nx=18, seed=25, mask_frac=0.2)
+    P = orc.phys(m=m, calving_rate=0.4)
+    rng = np.random.default_rng(26)
+    ny, nx = F["H"].shape
+
+    def pinned():
+        a, b, c = rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx)), rng.standard_normal((ny, nx))
+        a[:, 0] = a[:, -1] = 0.0
+        b[0] = b[-1] = 0.0
+        c[F["mask"] > 0.5] = 0.0
+        return a, b, c
+
+    lam, d = pinned(), pinned()
+    ju, jv, jH = orc.jvp(F, *d, P, DX, DT, use_mask=True)
+    vu, vv, vH = orc.vjp(F, *lam, P, DX, DT)
+    lhs = (lam[0] * ju).sum() + (lam[1] * jv).sum() + (lam[2] * jH).sum()
+    rhs = (vu * d[0]).sum() + (vv * d[1]).sum() + (vH * d[2]).sum()
+    assert abs(lhs - rhs) <= 1e-11 * (abs(lhs) + abs(rhs))
+
+
 def test_vjp_forcing_is_subtracted(orc):
     F = dome_fields(n=10, seed=8)
     P = orc.phys()

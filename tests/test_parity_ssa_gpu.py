@@ -77,6 +77,53 @@ def test_residual_nonlinear_drag(orc):
         assert rel_err(N(a), b) < KTOL
 
 
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("m,calving,use_mask", [(1.0, 0.0, True), (1.0 / 3.0, 0.3, True), (1.0, 0.2, False)])
+def test_jvp(orc, shape, m, calving, use_mask):
+    """ForwardOperators.compute_jvp against the oracle's J d (pinned by
+    central differences and by the dot product with the VJP)."""
+    ny, nx = shape
+    print(f"jvp shape={shape} m={m:.3f} calving={calving} mask={use_mask}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=31, mask_frac=0.2)
+    P = orc.phys(m=m, calving_rate=calving)
+    g = grid_from(F)
+    g.sliding.m.set(m)
+    g.calving.calving_rate.set(calving)
+    rng = np.random.default_rng(32)
+    d = (rng.standard_normal((ny, nx + 1)), rng.standard_normal((ny + 1, nx)), rng.standard_normal((ny, nx)))
+    fo = g.forward_operators
+    fo.var_u.copy_(T(d[0]))
+    fo.var_v.copy_(T(d[1]))
+    fo.var_H.copy_(T(d[2]))
+    fo.compute_jvp(10.0, use_mask=use_mask, freeze_phi=True)
+    torch.cuda.synchronize()
+    ref = orc.jvp(F, *d, P, 1500.0, 10.0, use_mask=use_mask)
+    for a, b in zip((fo.jvp_u, fo.jvp_v, fo.jvp_H), ref):
+        assert rel_err(N(a), b) < KTOL
+    assert np.all(N(fo.jvp_u)[:, [0, -1]] == 0) and np.all(N(fo.jvp_v)[[0, -1]] == 0)
+
+
+def test_jvp_freeze_calving_and_phi_refresh(orc):
+    """freeze_calving drops the calving term; without freeze_phi the grounded
+    fraction is recomputed from H and bed first."""
+    F = dome_fields(ny=20, nx=28, seed=33)
+    P0 = orc.phys(calving_rate=0.0)
+    g = grid_from(F)
+    g.calving.calving_rate.set(0.5)
+    rng = np.random.default_rng(34)
+    d = (rng.standard_normal((20, 29)), rng.standard_normal((21, 28)), rng.standard_normal((20, 28)))
+    fo = g.forward_operators
+    for t, a in zip((fo.var_u, fo.var_v, fo.var_H), d):
+        t.copy_(T(a))
+    fo.compute_jvp(10.0, freeze_calving=True)
+    torch.cuda.synchronize()
+    F2 = dict(F)
+    F2["phi"] = orc.grounded(F["H"], F["bed"], 0.1)
+    ref = orc.jvp(F2, *d, P0, 1500.0, 10.0)
+    for a, b in zip((fo.jvp_u, fo.jvp_v, fo.jvp_H), ref):
+        assert rel_err(N(a), b) < KTOL
+
+
 def test_grounded(orc):
     F = dome_fields(n=48, seed=3)
     g = grid_from(F)
