@@ -40,7 +40,7 @@ LEVEL_POINTERS = (
     "af_u", "af_v", "af_H", "ar_u", "ar_v", "ar_H", "vjp_u", "vjp_v", "vjp_H",
     "wl_u", "wl_v", "wl_H", "zl_u", "zl_v", "zl_H",
     "g_beta", "g_bed", "g_H_prev", "g_smb",
-    "work",
+    "work", "adj_cache",
 )
 
 
@@ -167,7 +167,7 @@ SIGNATURES = {
 }
 
 TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3, "residual": 4, "adjoint_apply": 5,
-                 "enth_column": 6, "enth_layer": 7, "enth_residual": 8}
+                 "enth_column": 6, "enth_layer": 7, "enth_residual": 8, "adjoint_factor": 9}
 
 
 def graphs_enable(on: bool) -> None:

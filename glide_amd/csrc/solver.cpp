@@ -167,19 +167,25 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
     return GLIDE_OK;
 }
 
+// The adjoint smoother cache of a level (its transposed patch factors and
+// cell viscosity) when the level has one and adjoint_prepare has filled it for
+// the current state; nullptr otherwise.
+const R* adj_cache(const glide_level& L, bool prepared) { return prepared ? CP(L.adj_cache) : nullptr; }
+
 int vjp(const glide_level& L, double dt, int flags, const R* lu, const R* lv, const R* lH, R* ou, R* ov, R* oH,
-        hipStream_t st) {
+        hipStream_t st, bool prepared = false) {
     SsaArgs<R> a = make_args(L, dt, true);
     const bool forcing = (flags & GLIDE_USE_FORCING) != 0;
     const double cr = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
+    const R* cache = adj_cache(L, prepared);
     HIP_TRY(launch_vjp<R>(a, lu, lv, lH, forcing ? CP(L.af_u) : nullptr, forcing ? CP(L.af_v) : nullptr,
-                          forcing ? CP(L.af_H) : nullptr, ou, ov, oH, cr, st),
+                          forcing ? CP(L.af_H) : nullptr, ou, ov, oH, cr,
+                          cache ? cache + (size_t)kAdjointCacheEta * nC(L) : nullptr, st),
             "vjp");
     return GLIDE_OK;
 }
 
-int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
-                         hipStream_t st) {
+VankaArgs<R> adjoint_vanka_args(const glide_level& L, const glide_vanka_config& vc, int flags) {
     VankaArgs<R> vk;
     vk.newton_steps = 1;
     vk.ssa_damping = vc.ssa_damping;
@@ -187,13 +193,37 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
     vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
     vk.iters = nullptr;
     vk.local_fp32 = 0;
+    return vk;
+}
+
+// Fills the level's adjoint smoother cache for its current state (no-op
+// without a cache).  The state must not change until the adjoint passes that
+// use it have run.
+int adjoint_prepare(const glide_level& L, double dt, const glide_vanka_config& vc, int flags, hipStream_t st) {
+    if (!L.adj_cache) return GLIDE_OK;
+    SsaArgs<R> a = make_args(L, dt, true);
+    HIP_TRY(launch_adjoint_factor<R>(a, P(L.adj_cache), adjoint_vanka_args(L, vc, flags), st), "adjoint factor");
+    return GLIDE_OK;
+}
+
+int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
+                         hipStream_t st, bool prepared = false) {
+    const VankaArgs<R> vk = adjoint_vanka_args(L, vc, flags);
+    const R* cache = adj_cache(L, prepared);
     for (int it = 0; it < n_iter; ++it) {
         GLIDE_TRY(vjp(L, dt, flags | GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u),
-                      P(L.ar_v), P(L.ar_H), st));
-        SsaArgs<R> a = make_args(L, dt, true);
-        HIP_TRY(launch_vanka_adjoint_pass<R>(a, CP(L.ar_u), CP(L.ar_v), CP(L.ar_H), P(L.lambda_u), P(L.lambda_v),
-                                             P(L.lambda_H), P(L.work), vk, vc.omega, st),
-                "vanka adjoint");
+                      P(L.ar_v), P(L.ar_H), st, prepared));
+        if (cache) {
+            HIP_TRY(launch_vanka_adjoint_pass_cached<R>(L.ny, L.nx, cache, CP(L.ar_u), CP(L.ar_v), CP(L.ar_H),
+                                                        P(L.lambda_u), P(L.lambda_v), P(L.lambda_H), P(L.work),
+                                                        vc.omega, st),
+                    "vanka adjoint");
+        } else {
+            SsaArgs<R> a = make_args(L, dt, true);
+            HIP_TRY(launch_vanka_adjoint_pass<R>(a, CP(L.ar_u), CP(L.ar_v), CP(L.ar_H), P(L.lambda_u), P(L.lambda_v),
+                                                 P(L.lambda_H), P(L.work), vk, vc.omega, st),
+                    "vanka adjoint");
+        }
     }
     return GLIDE_OK;
 }
@@ -325,10 +355,13 @@ int fas_vcycle(FasCtx& X, int l, bool finest) {
 int adj_vcycle(FasCtx& X, int l, bool finest) {
     const glide_level& L = X.L[l];
     hipStream_t st = X.st;
-    if (l == X.n_levels - 1) return vanka_adjoint_sweeps(L, X.dt, X.cfg.coarsest_steps, X.vc[l], 0, st);
+    // every level's cache is filled for its current state before its first pass
+    // (the start level by the caller, coarser levels after their restriction)
+    if (l == X.n_levels - 1) return vanka_adjoint_sweeps(L, X.dt, X.cfg.coarsest_steps, X.vc[l], 0, st, true);
     const glide_level& N = X.L[l + 1];
-    GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.pre_steps, X.vc[l], 0, st));
+    GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.pre_steps, X.vc[l], 0, st, true));
     GLIDE_TRY(restrict_state(L, N, st));
+    GLIDE_TRY(adjoint_prepare(N, X.dt, X.vc[l + 1], 0, st));
     GLIDE_TRY(restrict_u(L.lambda_u, N.lambda_u, N, st));
     GLIDE_TRY(restrict_v(L.lambda_v, N.lambda_v, N, st));
     GLIDE_TRY(restrict_c(L.lambda_H, N.lambda_H, N, 0, st));
@@ -336,11 +369,12 @@ int adj_vcycle(FasCtx& X, int l, bool finest) {
     GLIDE_TRY(copy(N.wl_v, N.lambda_v, nV(N), st));
     GLIDE_TRY(copy(N.wl_H, N.lambda_H, nC(N), st));
     GLIDE_TRY(vjp(L, X.dt, GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u), P(L.ar_v),
-                  P(L.ar_H), st));
+                  P(L.ar_H), st, true));
     GLIDE_TRY(restrict_u(L.ar_u, N.ar_u, N, st));
     GLIDE_TRY(restrict_v(L.ar_v, N.ar_v, N, st));
     GLIDE_TRY(restrict_c(L.ar_H, N.ar_H, N, 0, st));
-    GLIDE_TRY(vjp(N, X.dt, 0, CP(N.lambda_u), CP(N.lambda_v), CP(N.lambda_H), P(N.vjp_u), P(N.vjp_v), P(N.vjp_H), st));
+    GLIDE_TRY(vjp(N, X.dt, 0, CP(N.lambda_u), CP(N.lambda_v), CP(N.lambda_H), P(N.vjp_u), P(N.vjp_v), P(N.vjp_H), st,
+                  true));
     GLIDE_TRY(axpby(N.af_u, 1.0, N.vjp_u, -1.0, N.ar_u, nU(N), st));
     GLIDE_TRY(axpby(N.af_v, 1.0, N.vjp_v, -1.0, N.ar_v, nV(N), st));
     GLIDE_TRY(axpby(N.af_H, 1.0, N.vjp_H, -1.0, N.ar_H, nC(N), st));
@@ -354,8 +388,8 @@ int adj_vcycle(FasCtx& X, int l, bool finest) {
     GLIDE_TRY(axpby(L.lambda_u, 1.0, L.lambda_u, 1.0, L.zl_u, nU(L), st));
     GLIDE_TRY(axpby(L.lambda_v, 1.0, L.lambda_v, 1.0, L.zl_v, nV(L), st));
     GLIDE_TRY(axpby(L.lambda_H, 1.0, L.lambda_H, 1.0, L.zl_H, nC(L), st));
-    GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.post_steps, X.vc[l], 0, st));
-    if (finest) GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.finest_steps, X.vc[l], 0, st));
+    GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.post_steps, X.vc[l], 0, st, true));
+    if (finest) GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.finest_steps, X.vc[l], 0, st, true));
     return GLIDE_OK;
 }
 
@@ -691,6 +725,7 @@ int glide_fas_adjoint_vcycle(const glide_level* levels, int n_levels, int level,
     GLIDE_TRY(check_hierarchy(levels, n_levels, level, cfg, vc));
     if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
     FasCtx X{levels, n_levels, dt, *cfg, vc, (hipStream_t)stream};
+    GLIDE_TRY(adjoint_prepare(levels[level], dt, vc[level], 0, X.st));
     return adj_vcycle(X, level, finest != 0);
 }
 
@@ -713,7 +748,8 @@ int glide_vanka_adjoint_sweeps(const glide_level* L, double dt, int n_iter, cons
     REQ(L->lambda_u); REQ(L->lambda_v); REQ(L->lambda_H); REQ(L->af_u); REQ(L->af_v); REQ(L->af_H);
     REQ(L->ar_u); REQ(L->ar_v); REQ(L->ar_H); REQ(L->work);
     if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
-    return vanka_adjoint_sweeps(*L, dt, n_iter, *vc, flags, (hipStream_t)stream);
+    GLIDE_TRY(adjoint_prepare(*L, dt, *vc, flags, (hipStream_t)stream));
+    return vanka_adjoint_sweeps(*L, dt, n_iter, *vc, flags, (hipStream_t)stream, true);
 }
 
 int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_level, double dt,
@@ -725,9 +761,10 @@ int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_l
     const glide_level& L = levels[start_level];
     DeviceScratch* s = nullptr;
     GLIDE_TRY(scratch_for_current_device(&s));  // allocated before any capture
+    GLIDE_TRY(adjoint_prepare(L, dt, vc[start_level], 0, X.st));
     auto resid = [&](bool, double* slot) -> int {
         GLIDE_TRY(vjp(L, dt, GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u), P(L.ar_v),
-                      P(L.ar_H), X.st));
+                      P(L.ar_H), X.st, true));
         return sumsq3_async(CP(L.ar_u), nU(L), CP(L.ar_v), nV(L), CP(L.ar_H), nC(L), slot, X.st);
     };
     auto cycle = [&]() -> int { return adj_vcycle(X, start_level, true); };

@@ -519,6 +519,98 @@ __global__ __launch_bounds__(NT) void k_vanka_adjoint_cell(SsaArgs<T> a, const T
     work[4 * n + c] = d[4];
 }
 
+// Adjoint smoother cache.  The forward state is fixed during an adjoint
+// solve, so each cell's transposed, damped and pinned patch Jacobian is the
+// same in every adjoint pass: k_adjoint_factor factorises it once per level
+// and state, and k_vanka_adjoint_cell_cached only substitutes.  Layout
+// (planar, n = ny * nx cells): cache[q * n + c], q = 0..24, the in-place
+// no-pivot Doolittle factors of J^T (row-major) with the reciprocal pivot on
+// the diagonal, 0 for a pinned row (boundary facet or active-set cell), so
+// that its increment is exactly zero as when its right-hand side is cleared;
+// cache[25 * n + c] the cell viscosity, reused by the transpose operator.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adjoint_factor(SsaArgs<T> a, T* __restrict__ cache, VankaArgs<T> vk) {
+    __shared__ T s_eta[TY + 2][TX + 2];
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    fill_eta_tile(s_eta, a, i0, j0, tid);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    const int ny = a.ny, nx = a.nx;
+    if (i >= ny || j >= nx) return;
+    Patch<T> p;
+    load_patch(p, a, s_eta, threadIdx.y + 1, threadIdx.x + 1, i, j, (const T*)nullptr, (const T*)nullptr,
+               (const T*)nullptr, a.phi);
+    const long c = (long)i * nx + j;
+    T x[5] = {a.u[(long)i * (nx + 1) + j], a.u[(long)i * (nx + 1) + j + 1], a.v[c], a.v[c + nx], a.H[c]};
+    T J[25], r[5];
+    local_system(p, x, J, r, a.P, vk.calving_rate, a.dxi, a.dt);
+    J[0] -= vk.ssa_damping; J[6] -= vk.ssa_damping; J[12] -= vk.ssa_damping; J[18] -= vk.ssa_damping;
+    J[24] += vk.mc_damping;
+    const bool pin[5] = {j == 0, j == nx - 1, i == 0, i == ny - 1, a.mask[c] > T(0.5)};
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        if (pin[k]) pin_row(J, k);
+    T A[25];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) A[q * 5 + s] = J[s * 5 + q];
+    T inv[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        inv[k] = T(1) / A[k * 5 + k];
+#pragma unroll
+        for (int q = k + 1; q < 5; ++q) {
+            A[q * 5 + k] *= inv[k];
+#pragma unroll
+            for (int s = k + 1; s < 5; ++s) A[q * 5 + s] -= A[q * 5 + k] * A[k * 5 + s];
+        }
+    }
+    const long n = (long)ny * nx;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) A[k * 6] = pin[k] ? T(0) : inv[k];
+#pragma unroll
+    for (int q = 0; q < 25; ++q) cache[q * n + c] = A[q];
+    cache[25 * n + c] = s_eta[threadIdx.y + 1][threadIdx.x + 1];
+}
+
+// Adjoint patch solve from the cached factors: the increments of
+// k_vanka_adjoint_cell (half-weighted facets) into the workspace planes.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vanka_adjoint_cell_cached(const T* __restrict__ cache,
+                                                                  const T* __restrict__ ru, const T* __restrict__ rv,
+                                                                  const T* __restrict__ rH, T* __restrict__ work,
+                                                                  int ny, int nx) {
+    const int i = blockIdx.y * TY + threadIdx.y, j = blockIdx.x * TX + threadIdx.x;
+    if (i >= ny || j >= nx) return;
+    const long n = (long)ny * nx, c = (long)i * nx + j;
+    T A[25];
+#pragma unroll
+    for (int q = 0; q < 25; ++q) A[q] = cache[q * n + c];
+    const T b[5] = {ru[(long)i * (nx + 1) + j], ru[(long)i * (nx + 1) + j + 1], rv[c], rv[c + nx], rH[c]};
+    T y[5], d[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        T s = b[q];
+#pragma unroll
+        for (int k = 0; k < q; ++k) s -= A[q * 5 + k] * y[k];
+        y[q] = s;
+    }
+#pragma unroll
+    for (int q = 4; q >= 0; --q) {
+        T s = y[q];
+#pragma unroll
+        for (int k = q + 1; k < 5; ++k) s -= A[q * 5 + k] * d[k];
+        d[q] = s * A[q * 6];
+    }
+    work[c] = T(0.5) * d[0];
+    work[n + c] = T(0.5) * d[1];
+    work[2 * n + c] = T(0.5) * d[2];
+    work[3 * n + c] = T(0.5) * d[3];
+    work[4 * n + c] = d[4];
+}
+
 // ------------------------------------------------------------------ transpose operator
 // Jᵀλ (minus the adjoint forcing when use_forcing) as a gather.  Phase 1
 // computes, for the cells of the tile plus a one-cell ring, the masked
@@ -548,7 +640,8 @@ template <typename T>
 __global__ __launch_bounds__(NT) void k_vjp(SsaArgs<T> a, const T* __restrict__ lu, const T* __restrict__ lv,
                                             const T* __restrict__ lH, const T* __restrict__ fu,
                                             const T* __restrict__ fv, const T* __restrict__ fH, T* __restrict__ out_u,
-                                            T* __restrict__ out_v, T* __restrict__ out_H, T calving_rate) {
+                                            T* __restrict__ out_v, T* __restrict__ out_H, T calving_rate,
+                                            const T* __restrict__ eta_cache) {
     __shared__ VjpTile<T> s;
     const int ny = a.ny, nx = a.nx;
     const T dxi = a.dxi;
@@ -569,9 +662,14 @@ __global__ __launch_bounds__(NT) void k_vjp(SsaArgs<T> a, const T* __restrict__ 
             CellVel<T> cl = load_cell_vel(lu, lv, ci, cj, ny, nx);
             T e2 = strain_invariant(cv, ci, cj, ny, nx, dxi, a.P.eps_reg);
             T de2 = strain_invariant_dir(cv, cl, ci, cj, ny, nx, dxi);
-            T b = a.B[c];
-            e = T(0.5) * b * pow(e2, a.P.glen_p);
-            ed = T(0.5) * b * a.P.glen_p * pow(e2, a.P.glen_p - T(1)) * de2;
+            if (eta_cache) {
+                e = eta_cache[c];
+                ed = e * a.P.glen_p * de2 / e2;
+            } else {
+                T b = a.B[c];
+                e = T(0.5) * b * pow(e2, a.P.glen_p);
+                ed = T(0.5) * b * a.P.glen_p * pow(e2, a.P.glen_p - T(1)) * de2;
+            }
             Hh = H[c];
             lHm = lH[c] * (T(1) - a.mask[c]);
             NormalStress<T> sx = sxx(cv.ul, cv.ur, cv.vt, cv.vb, e * Hh, dxi, true);
@@ -890,10 +988,31 @@ hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* 
 
 template <typename T>
 hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH, const T* fu, const T* fv,
-                      const T* fH, T* ou, T* ov, T* oH, T calving_rate, hipStream_t st) {
+                      const T* fH, T* ou, T* ov, T* oH, T calving_rate, const T* eta_cache, hipStream_t st) {
     TimedLaunch tl(kTimeVjp, (long)a.ny * a.nx, st);
     k_vjp<T><<<tile_grid(a.ny, a.nx, true), dim3(TX, TY), 0, st>>>(a, lu, lv, lH, fu, fv, fH, ou, ov, oH,
-                                                                    calving_rate);
+                                                                    calving_rate, eta_cache);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_adjoint_factor(const SsaArgs<T>& a, T* cache, const VankaArgs<T>& vk, hipStream_t st) {
+    TimedLaunch tl(kTimeAdjointFactor, (long)a.ny * a.nx, st);
+    k_adjoint_factor<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, cache, vk);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_vanka_adjoint_pass_cached(int ny, int nx, const T* cache, const T* ru, const T* rv, const T* rH,
+                                            T* lu, T* lv, T* lH, T* work, T omega, hipStream_t st) {
+    {
+        TimedLaunch tl(kTimeAdjointCell, (long)ny * nx, st);
+        k_vanka_adjoint_cell_cached<T><<<tile_grid(ny, nx, false), dim3(TX, TY), 0, st>>>(cache, ru, rv, rH, work,
+                                                                                         ny, nx);
+    }
+    dim3 g((nx + 1 + 255) / 256, ny + 1);
+    TimedLaunch tl(kTimeAdjointApply, (long)ny * nx, st);
+    k_vanka_apply<T><<<g, 256, 0, st>>>(lu, lv, lH, work, ny, nx, -omega);
     return hipGetLastError();
 }
 
@@ -915,7 +1034,10 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
     template hipError_t launch_vjp<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, const T*, const T*, \
-                                      T*, T*, T*, T, hipStream_t);                                                   \
+                                      T*, T*, T*, T, const T*, hipStream_t);                                         \
+    template hipError_t launch_adjoint_factor<T>(const SsaArgs<T>&, T*, const VankaArgs<T>&, hipStream_t);          \
+    template hipError_t launch_vanka_adjoint_pass_cached<T>(int, int, const T*, const T*, const T*, const T*, T*, T*, \
+                                                            T*, T*, T, hipStream_t);                                 \
     template hipError_t launch_grad_beta_bed<T>(const SsaArgs<T>&, const T*, const T*, T*, T*, hipStream_t);
 
 GLIDE_INSTANTIATE(double)

@@ -42,9 +42,20 @@ hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hi
 template <typename T>
 hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* rv, const T* rH, T* lu, T* lv, T* lH,
                                      T* work, const VankaArgs<T>& vk, T omega, hipStream_t st);
+// eta_cache: the cell viscosity cached by launch_adjoint_factor for the
+// current state, or nullptr to compute it.
 template <typename T>
 hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH, const T* fu, const T* fv,
-                      const T* fH, T* ou, T* ov, T* oH, T calving_rate, hipStream_t st);
+                      const T* fH, T* ou, T* ov, T* oH, T calving_rate, const T* eta_cache, hipStream_t st);
+// Adjoint smoother cache of one level and state: kAdjointCacheValues * ny * nx
+// values (transposed patch LU factors and the cell viscosity).
+constexpr int kAdjointCacheValues = 26;
+constexpr int kAdjointCacheEta = 25;  // plane of the cell viscosity
+template <typename T>
+hipError_t launch_adjoint_factor(const SsaArgs<T>& a, T* cache, const VankaArgs<T>& vk, hipStream_t st);
+template <typename T>
+hipError_t launch_vanka_adjoint_pass_cached(int ny, int nx, const T* cache, const T* ru, const T* rv, const T* rH,
+                                            T* lu, T* lv, T* lH, T* work, T omega, hipStream_t st);
 template <typename T>
 hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st);
 

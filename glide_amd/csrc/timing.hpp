@@ -19,6 +19,7 @@ enum TimedKernel {
     kTimeEnthColumn,
     kTimeEnthLayer,
     kTimeEnthResidual,
+    kTimeAdjointFactor,
     kTimedKernels
 };
 

@@ -197,6 +197,8 @@ class Grid:
             L.af_u, L.af_v, L.af_H = ptr(ao.f_u), ptr(ao.f_v), ptr(ao.f_H)
             L.ar_u, L.ar_v, L.ar_H = ptr(ao.r_u), ptr(ao.r_v), ptr(ao.r_H)
             L.vjp_u, L.vjp_v, L.vjp_H = ptr(ao.vjp_u), ptr(ao.vjp_v), ptr(ao.vjp_H)
+            if ao.use_cache:
+                L.adj_cache = ptr(ao.cache)
         if adj_scratch is not None:
             L.wl_u, L.wl_v, L.wl_H = ptr(adj_scratch.w_lambda_u), ptr(adj_scratch.w_lambda_v), ptr(
                 adj_scratch.w_lambda_H)

@@ -194,6 +194,11 @@ class AdjointOperators:
         self.vjp_u, self.vjp_v, self.vjp_H = z((ny, nx + 1)), z((ny + 1, nx)), z((ny, nx))
         self.gamma = z((ny, nx)).fill_(grid.geometry.thklim.value)
         self.vanka_config = VankaConfig()
+        # Adjoint smoother cache (transposed patch LU factors and cell
+        # viscosity, refreshed by every adjoint solve or sweep call for the
+        # state at that moment); use_cache = False recomputes them per pass.
+        self.use_cache = True
+        self.cache = z((26, ny, nx))
         grid.adjoint  # the multipliers exist whenever the operators do
 
     def _level(self):

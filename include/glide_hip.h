@@ -90,6 +90,12 @@ typedef struct glide_level {
     void *g_beta, *g_bed, *g_H_prev, *g_smb;
     /* smoother workspace: 5 * ny * nx values */
     void *work;
+    /* Optional adjoint smoother cache, 26 * ny * nx values (no counterpart in
+     * the reference).  When set, adjoint solves and sweeps factorise each
+     * cell's transposed patch Jacobian once per level and state (the forward
+     * state is fixed during an adjoint solve) and reuse the factors and the
+     * cell viscosity in every pass; NULL recomputes them in every pass. */
+    void *adj_cache;
 } glide_level;
 
 /* VankaConfig + NewtonConfig (glide/operators.py:314-327).  The forward
@@ -193,7 +199,8 @@ int glide_graphs_clear(void);
  * event pairs are recorded around every launch of the hot kernels
  * (0 forward patch solve, 1 patch increment apply, 2 adjoint patch solve,
  * 3 transpose operator, 4 residual, 5 adjoint increment apply, 6 enthalpy
- * column smoother, 7 enthalpy layer smoother, 8 enthalpy residual).
+ * column smoother, 7 enthalpy layer smoother, 8 enthalpy residual, 9 adjoint
+ * patch factorisation into the cache).
  * glide_timing_read synchronises those events and returns the launch count,
  * summed duration and summed unit count (cells; column nodes for 6-8) of
  * kernel `kernel` since the last enable. */

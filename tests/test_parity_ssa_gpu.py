@@ -250,7 +250,9 @@ def test_vjp(orc, shape, m, use_forcing):
         assert rel_err(N(x), y) < KTOL
 
 
-def test_vanka_adjoint_pass(orc):
+@pytest.mark.parametrize("use_cache", [True, False])
+def test_vanka_adjoint_pass(orc, use_cache):
+    print(f"adjoint pass cache={use_cache}", flush=True)
     F = dome_fields(n=40, seed=11, mask_frac=0.1)
     P = orc.phys()
     lam = _lam(F, 12)
@@ -260,6 +262,7 @@ def test_vanka_adjoint_pass(orc):
     a.lambda_v.set(lam[1])
     a.lambda_H.set(lam[2])
     ao = g.adjoint_operators
+    ao.use_cache = use_cache
     rng = np.random.default_rng(13)
     af = (rng.standard_normal(lam[0].shape), rng.standard_normal(lam[1].shape), rng.standard_normal(lam[2].shape))
     ao.f_u.copy_(T(af[0]))
@@ -270,6 +273,31 @@ def test_vanka_adjoint_pass(orc):
     ref = orc.vanka_adjoint_sweep(F, lam, r, P, 1500.0, 10.0)
     for x, y in zip((a.lambda_u.data, a.lambda_v.data, a.lambda_H.data), ref):
         assert rel_err(N(x), y) < KTOL
+
+
+def test_adjoint_cache_matches_uncached_passes(orc):
+    """Several adjoint passes from the cached patch factors agree with passes
+    that rebuild and factorise every patch (mask, calving and nonlinear drag
+    on, non-square level); the cache is filled by the sweep call itself."""
+    F = dome_fields(ny=36, nx=52, seed=14, mask_frac=0.15)
+    out = []
+    for use_cache in (True, False):
+        g = grid_from(F)
+        g.sliding.m.set(1.0 / 3.0)
+        g.calving.calving_rate.set(0.3)
+        lam = _lam(F, 15)
+        for t, x in zip((g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H), lam):
+            t.set(x)
+        ao = g.adjoint_operators
+        ao.use_cache = use_cache
+        rng = np.random.default_rng(16)
+        for t in (ao.f_u, ao.f_v, ao.f_H):
+            t.copy_(T(rng.standard_normal(tuple(t.shape))))
+        ao.vanka_sweep(10.0, 6)
+        torch.cuda.synchronize()
+        out.append([N(t.data) for t in (g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H)])
+    for x, y in zip(*out):
+        assert rel_err(x, y) < 1e-13
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -359,10 +387,12 @@ def test_fascd_solve_tolerance_stop(orc):
     assert rel_err(N(model.mg[0].state.u.data), om.get("u")) < STOL
 
 
-@pytest.mark.parametrize("n,L", [(64, 4)])
-def test_forward_adjoint_gradients(orc, n, L):
-    print(f"forward+adjoint n={n} L={L}", flush=True)
+@pytest.mark.parametrize("n,L,use_cache", [(64, 4, True), (64, 4, False)])
+def test_forward_adjoint_gradients(orc, n, L, use_cache):
+    print(f"forward+adjoint n={n} L={L} cache={use_cache}", flush=True)
     p, model, om = _dome_models(orc, n, L, seed=5)
+    for lev in range(L):
+        model.mg[lev].adjoint_operators.use_cache = use_cache
     model.forward_solver.fas_options.set(maximum_vcycles=3, relative_tolerance=0.0, absolute_tolerance=0.0,
                                          report_norms=False)
     model.adjoint_solver.fas_options.set(maximum_vcycles=3, relative_tolerance=0.0, absolute_tolerance=0.0,
