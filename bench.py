@@ -33,6 +33,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "forward+adjoint steps/sec at 512² MAC grid, mixed vs FP64 at equal 1e-8 error"
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFS = 78.6  # MI355X vector FP64
 WORD = 8  # bytes per FP64 value
 
 # Algorithmic bytes per cell of one launch (each field counted once at 8 B):
@@ -124,6 +125,19 @@ def read_traffic(args, kernel):
     for line in path.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":")[1])
+    return None
+
+
+def read_fp64_flops(args):
+    """FP64 operations per finest-level forward patch-solve launch from the
+    committed VALU instruction-count summary (64 lanes x (add + mul + trans +
+    2 fma) per wave instruction, masked lanes included); None when absent."""
+    path = ROOT / "profiles" / "vanka_cell_512_fp64_valu.md"
+    if args.n != 512 or not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("fp64_flops_per_launch:"):
+            return float(line.split(":")[1])
     return None
 
 
@@ -263,6 +277,18 @@ def main():
                 "kernel": dominant, "level": f"{args.n}x{args.n}", "launches": f_launches,
                 "avg_launch_us": round(1000.0 * avg_ms, 3), "alg_bytes_per_launch": alg_bytes,
                 "all_levels_alg_GBps": round(bw(ALG_FIELDS[dominant], passes[dominant][1], passes[dominant][2]), 2)}
+    # the patch solve's own bound: FP64 VALU issue (per-launch FP64 operations
+    # from the committed counter summary over its measured finest-level duration)
+    flops = read_fp64_flops(args)
+    c_launches, c_ms = fine_kern["vanka_cell"]
+    valu = None
+    if flops is not None and c_launches > 0 and c_ms > 0:
+        tf = flops / (c_ms / c_launches * 1e-3) / 1e12
+        valu = {"bound": "fp64 valu", "kernel": "vanka_cell", "level": f"{args.n}x{args.n}",
+                "achieved": round(tf, 2), "peak": FP64_VALU_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": round(tf / FP64_VALU_PEAK_TFS, 4), "fp64_flops_per_launch": flops,
+                "avg_launch_us": round(1000.0 * c_ms / c_launches, 3)}
+    roofline["valu"] = valu
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "cells": v[2]} for k, v in kern.items()}
     for k, (c, ms, cells) in passes.items():
         kernels[k] = {"launches": c, "total_ms": round(ms, 3), "alg_GBps": round(bw(ALG_FIELDS[k], ms, cells), 2)}

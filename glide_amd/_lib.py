@@ -140,6 +140,7 @@ SIGNATURES = {
                                       ctypes.POINTER(VankaConfigC), _P]),
     "glide_gradients": (_I, [_LP, _D, _I, _P]),
     "glide_graphs_enable": (_I, [_I]),
+    "glide_adjoint_fused_enable": (_I, [_I]),
     "glide_graphs_clear": (_I, []),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -173,6 +174,11 @@ TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3,
 def graphs_enable(on: bool) -> None:
     """Replay fixed-cycle FAS solves from cached HIP graphs (default on)."""
     call("glide_graphs_enable", int(bool(on)))
+
+
+def adjoint_fused_enable(on: bool) -> None:
+    """Cached adjoint passes as one fused launch (default) or solve + apply."""
+    call("glide_adjoint_fused_enable", int(bool(on)))
 
 
 def graphs_clear() -> None:

@@ -159,13 +159,22 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
     vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
     vk.iters = nullptr;
     vk.local_fp32 = vc.local_fp32;
+    // the grounded fraction is refreshed before every pass: before the first
+    // by itself, before the others by the previous pass's increment apply
+    if (!freeze_phi && n_iter > 0) GLIDE_TRY(grounded(L, vc.relax_phi, st));
     for (int it = 0; it < n_iter; ++it) {
-        if (!freeze_phi) GLIDE_TRY(grounded(L, vc.relax_phi, st));
         SsaArgs<R> a = make_args(L, dt, false);
-        HIP_TRY(launch_vanka_pass<R>(a, P(L.u), P(L.v), P(L.H), P(L.work), P(L.mask), vk, vc.omega, st), "vanka");
+        R* phi_next = (!freeze_phi && it + 1 < n_iter) ? P(L.phi) : nullptr;
+        HIP_TRY(launch_vanka_pass<R>(a, P(L.u), P(L.v), P(L.H), P(L.work), P(L.mask), vk, vc.omega, phi_next,
+                                     vc.relax_phi, st),
+                "vanka");
     }
     return GLIDE_OK;
 }
+
+// Adjoint passes from the cache run as one fused launch (default) or as
+// solve + apply (glide_adjoint_fused_enable).
+bool g_fused_adjoint = true;
 
 // The adjoint smoother cache of a level (its transposed patch factors and
 // cell viscosity) when the level has one and adjoint_prepare has filled it for
@@ -216,7 +225,7 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
         if (cache) {
             HIP_TRY(launch_vanka_adjoint_pass_cached<R>(L.ny, L.nx, cache, CP(L.ar_u), CP(L.ar_v), CP(L.ar_H),
                                                         P(L.lambda_u), P(L.lambda_v), P(L.lambda_H), P(L.work),
-                                                        vc.omega, st),
+                                                        vc.omega, g_fused_adjoint, st),
                     "vanka adjoint");
         } else {
             SsaArgs<R> a = make_args(L, dt, true);
@@ -769,6 +778,11 @@ int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_l
     };
     auto cycle = [&]() -> int { return adj_vcycle(X, start_level, true); };
     return solve_loop(X, start_level, report, resid, cycle, 'a');
+}
+
+int glide_adjoint_fused_enable(int on) {
+    g_fused_adjoint = on != 0;
+    return GLIDE_OK;
 }
 
 int glide_graphs_enable(int on) {

@@ -385,35 +385,46 @@ __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__
     const T tol = T(1e-6);
     const bool pin0 = j == 0, pin1 = j == nx - 1, pin2 = i == 0, pin3 = i == ny - 1;
     int it = 0;
-    while (it < vk.newton_steps && rnorm > tol) {
-        TJ J[25];
-        T r[5];
-        cell_system<T, TJ, LINEAR>(cs, *cq, x, J, r, a.P);
-        if (pin0) { pin_row(J, 0); r[0] = x[0]; }
-        if (pin1) { pin_row(J, 1); r[1] = x[1]; }
-        if (pin2) { pin_row(J, 2); r[2] = x[2]; }
-        if (pin3) { pin_row(J, 3); r[3] = x[3]; }
-        if ((x[4] - a.dt * r[4]) <= thk) {
-            masked = T(1);
-            pin_row(J, 4);
-            r[4] = x[4] - thk;
-        } else {
-            masked = T(0);
-        }
-        TJ rj[5] = {TJ(r[0]), TJ(r[1]), TJ(r[2]), TJ(r[3]), TJ(r[4])};
-        TJ d[5];
-        lu5_solve_rcp(J, rj, d);
-        rnorm = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3] + r[4] * r[4];
+    // The Newton loop in two copies: waves on the domain's edge test the
+    // Dirichlet pins in every iteration, interior waves (a wave is one tile
+    // row, so the test is wave-uniform) carry no pin code at all.
+    auto newton = [&](auto edge_tag) {
+        constexpr bool EDGE = decltype(edge_tag)::value;
+        while (it < vk.newton_steps && rnorm > tol) {
+            TJ J[25];
+            T r[5];
+            cell_system<T, TJ, LINEAR>(cs, *cq, x, J, r, a.P);
+            if constexpr (EDGE) {
+                if (pin0) { pin_row(J, 0); r[0] = x[0]; }
+                if (pin1) { pin_row(J, 1); r[1] = x[1]; }
+                if (pin2) { pin_row(J, 2); r[2] = x[2]; }
+                if (pin3) { pin_row(J, 3); r[3] = x[3]; }
+            }
+            if ((x[4] - a.dt * r[4]) <= thk) {
+                masked = T(1);
+                pin_row(J, 4);
+                r[4] = x[4] - thk;
+            } else {
+                masked = T(0);
+            }
+            TJ rj[5] = {TJ(r[0]), TJ(r[1]), TJ(r[2]), TJ(r[3]), TJ(r[4])};
+            TJ d[5];
+            lu5_solve_rcp(J, rj, d);
+            rnorm = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3] + r[4] * r[4];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {  // compensated relaxed update, local relaxation 0.5
-            T y = T(-0.5) * T(d[k]) - cmp[k];
-            T t = x[k] + y;
-            cmp[k] = (t - x[k]) - y;
-            x[k] = t;
+            for (int k = 0; k < 5; ++k) {  // compensated relaxed update, local relaxation 0.5
+                T y = T(-0.5) * T(d[k]) - cmp[k];
+                T t = x[k] + y;
+                cmp[k] = (t - x[k]) - y;
+                x[k] = t;
+            }
+            x[4] = fmax(x[4], thk);
+            ++it;
         }
-        x[4] = fmax(x[4], thk);
-        ++it;
-    }
+    };
+    const bool edge_wave = i == 0 || i == ny - 1 || j0 == 0 || j0 + TX >= nx;
+    if (edge_wave) newton(std::true_type{});
+    else newton(std::false_type{});
     const long n = (long)ny * nx;
     work[c] = T(0.5) * (x[0] - a.u[(long)i * (nx + 1) + j]);
     work[n + c] = T(0.5) * (x[1] - a.u[(long)i * (nx + 1) + j + 1]);
@@ -425,9 +436,13 @@ __global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__
 }
 
 // x += sign * omega * (sum of the cell increments touching each unknown).
+// With phi != nullptr the grounded fraction of the next pass is computed from
+// the updated thickness in the same launch (what k_grounded would give).
 template <typename T>
 __global__ __launch_bounds__(256) void k_vanka_apply(T* __restrict__ u, T* __restrict__ v, T* __restrict__ H,
-                                                     const T* __restrict__ work, int ny, int nx, T omega) {
+                                                     const T* __restrict__ work, int ny, int nx, T omega,
+                                                     T* __restrict__ phi = nullptr, const T* __restrict__ bed = nullptr,
+                                                     T gc = T(0), T relax = T(0)) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
     if (j > nx) return;
@@ -443,7 +458,15 @@ __global__ __launch_bounds__(256) void k_vanka_apply(T* __restrict__ u, T* __res
         if (i < ny) d = work[2 * n + (long)i * nx + j];
         if (i > 0) d = d + work[3 * n + (long)(i - 1) * nx + j];
         v[(long)i * nx + j] += omega * d;
-        if (i < ny) H[(long)i * nx + j] += omega * work[4 * n + (long)i * nx + j];
+        if (i < ny) {
+            const long c = (long)i * nx + j;
+            const T h = H[c] + omega * work[4 * n + c];
+            H[c] = h;
+            if (phi) {
+                T g = grounded_fraction(h, bed[c], gc);
+                phi[c] = relax == T(0) ? g : (T(1) - relax) * g + relax * phi[c];
+            }
+        }
     }
 }
 
@@ -575,15 +598,12 @@ __global__ __launch_bounds__(NT) void k_adjoint_factor(SsaArgs<T> a, T* __restri
     cache[25 * n + c] = s_eta[threadIdx.y + 1][threadIdx.x + 1];
 }
 
-// Adjoint patch solve from the cached factors: the increments of
-// k_vanka_adjoint_cell (half-weighted facets) into the workspace planes.
+// Increments of cell (i,j) from its cached factors and the residual r:
+// w = (d0/2, d1/2, d2/2, d3/2, d4) with J^T d = r (half-weighted facets).
 template <typename T>
-__global__ __launch_bounds__(NT) void k_vanka_adjoint_cell_cached(const T* __restrict__ cache,
-                                                                  const T* __restrict__ ru, const T* __restrict__ rv,
-                                                                  const T* __restrict__ rH, T* __restrict__ work,
-                                                                  int ny, int nx) {
-    const int i = blockIdx.y * TY + threadIdx.y, j = blockIdx.x * TX + threadIdx.x;
-    if (i >= ny || j >= nx) return;
+__device__ __forceinline__ void adjoint_cached_increments(const T* __restrict__ cache, const T* __restrict__ ru,
+                                                          const T* __restrict__ rv, const T* __restrict__ rH, int i,
+                                                          int j, int ny, int nx, T w[5]) {
     const long n = (long)ny * nx, c = (long)i * nx + j;
     T A[25];
 #pragma unroll
@@ -604,11 +624,89 @@ __global__ __launch_bounds__(NT) void k_vanka_adjoint_cell_cached(const T* __res
         for (int k = q + 1; k < 5; ++k) s -= A[q * 5 + k] * d[k];
         d[q] = s * A[q * 6];
     }
-    work[c] = T(0.5) * d[0];
-    work[n + c] = T(0.5) * d[1];
-    work[2 * n + c] = T(0.5) * d[2];
-    work[3 * n + c] = T(0.5) * d[3];
-    work[4 * n + c] = d[4];
+    w[0] = T(0.5) * d[0];
+    w[1] = T(0.5) * d[1];
+    w[2] = T(0.5) * d[2];
+    w[3] = T(0.5) * d[3];
+    w[4] = d[4];
+}
+
+// Adjoint patch solve from the cached factors: the increments of
+// k_vanka_adjoint_cell into the workspace planes.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vanka_adjoint_cell_cached(const T* __restrict__ cache,
+                                                                  const T* __restrict__ ru, const T* __restrict__ rv,
+                                                                  const T* __restrict__ rH, T* __restrict__ work,
+                                                                  int ny, int nx) {
+    const int i = blockIdx.y * TY + threadIdx.y, j = blockIdx.x * TX + threadIdx.x;
+    if (i >= ny || j >= nx) return;
+    const long n = (long)ny * nx, c = (long)i * nx + j;
+    T w[5];
+    adjoint_cached_increments(cache, ru, rv, rH, i, j, ny, nx, w);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) work[q * n + c] = w[q];
+}
+
+// A whole adjoint smoothing pass after the transpose operator in one launch:
+// the cached patch solves of the tile's cells and of the cells left of and
+// above the tile (whose right / bottom facet increments the tile's facets
+// also receive), then lambda += omega * (sum of the increments) on the facets
+// and cells the tile owns, with the arithmetic of k_vanka_apply.  Each
+// multiplier is read and written by its owner only, so the update is in place.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_vanka_adjoint_fused(const T* __restrict__ cache, const T* __restrict__ ru,
+                                                            const T* __restrict__ rv, const T* __restrict__ rH,
+                                                            T* __restrict__ lu, T* __restrict__ lv,
+                                                            T* __restrict__ lH, int ny, int nx, T omega) {
+    __shared__ T s_w1[TY][TX + 1];  // right-facet increment of cells (i0 + a, j0 - 1 + b)
+    __shared__ T s_w3[TY + 1][TX];  // bottom-facet increment of cells (i0 - 1 + a, j0 + b)
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int ty = threadIdx.y, tx = threadIdx.x, tid = ty * TX + tx;
+    const int i = i0 + ty, j = j0 + tx;
+    const bool own = i < ny && j < nx;
+    T w[5] = {T(0), T(0), T(0), T(0), T(0)};
+    if (own) {
+        adjoint_cached_increments(cache, ru, rv, rH, i, j, ny, nx, w);
+        s_w1[ty][tx + 1] = w[1];
+        s_w3[ty + 1][tx] = w[3];
+    }
+    if (tid < TY) {
+        const int ci = i0 + tid, cj = j0 - 1;
+        if (cj >= 0 && ci < ny) {
+            T wr[5];
+            adjoint_cached_increments(cache, ru, rv, rH, ci, cj, ny, nx, wr);
+            s_w1[tid][0] = wr[1];
+        }
+    } else if (tid < TY + TX) {
+        const int ci = i0 - 1, cj = j0 + (tid - TY);
+        if (ci >= 0 && cj < nx) {
+            T wr[5];
+            adjoint_cached_increments(cache, ru, rv, rH, ci, cj, ny, nx, wr);
+            s_w3[0][tid - TY] = wr[3];
+        }
+    }
+    __syncthreads();
+    if (!own) return;
+    const long c = (long)i * nx + j;
+    {
+        T d = w[0];
+        if (j > 0) d = d + s_w1[ty][tx];
+        lu[(long)i * (nx + 1) + j] += omega * d;
+        if (j == nx - 1) {
+            T e = T(0) + w[1];
+            lu[(long)i * (nx + 1) + nx] += omega * e;
+        }
+    }
+    {
+        T d = w[2];
+        if (i > 0) d = d + s_w3[ty][tx];
+        lv[c] += omega * d;
+        if (i == ny - 1) {
+            T e = T(0) + w[3];
+            lv[(long)ny * nx + j] += omega * e;
+        }
+    }
+    lH[c] += omega * w[4];
 }
 
 // ------------------------------------------------------------------ transpose operator
@@ -949,7 +1047,7 @@ hipError_t launch_jvp(const SsaArgs<T>& a, const T* du, const T* dv, const T* dH
 
 template <typename T>
 hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* mask, const VankaArgs<T>& vk, T omega,
-                             hipStream_t st) {
+                             T* phi_next, T relax_phi, hipStream_t st) {
     {
         TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx, st);
         const dim3 g = tile_grid(a.ny, a.nx, false), b(TX, TY);
@@ -963,7 +1061,7 @@ hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* 
     }
     dim3 g((a.nx + 1 + 255) / 256, a.ny + 1);
     TimedLaunch tl(kTimeVankaApply, (long)a.ny * a.nx, st);
-    k_vanka_apply<T><<<g, 256, 0, st>>>(u, v, H, work, a.ny, a.nx, omega);
+    k_vanka_apply<T><<<g, 256, 0, st>>>(u, v, H, work, a.ny, a.nx, omega, phi_next, a.bed, a.P.frd, relax_phi);
     return hipGetLastError();
 }
 
@@ -1004,7 +1102,13 @@ hipError_t launch_adjoint_factor(const SsaArgs<T>& a, T* cache, const VankaArgs<
 
 template <typename T>
 hipError_t launch_vanka_adjoint_pass_cached(int ny, int nx, const T* cache, const T* ru, const T* rv, const T* rH,
-                                            T* lu, T* lv, T* lH, T* work, T omega, hipStream_t st) {
+                                            T* lu, T* lv, T* lH, T* work, T omega, bool fused, hipStream_t st) {
+    if (fused) {
+        TimedLaunch tl(kTimeAdjointCell, (long)ny * nx, st);
+        k_vanka_adjoint_fused<T><<<tile_grid(ny, nx, false), dim3(TX, TY), 0, st>>>(cache, ru, rv, rH, lu, lv, lH, ny,
+                                                                                   nx, -omega);
+        return hipGetLastError();
+    }
     {
         TimedLaunch tl(kTimeAdjointCell, (long)ny * nx, st);
         k_vanka_adjoint_cell_cached<T><<<tile_grid(ny, nx, false), dim3(TX, TY), 0, st>>>(cache, ru, rv, rH, work,
@@ -1028,7 +1132,7 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_residual<T>(const SsaArgs<T>&, T*, T*, T*, int, int, T, hipStream_t);                  \
     template hipError_t launch_jvp<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, int, T,      \
                                       hipStream_t);                                                           \
-    template hipError_t launch_vanka_pass<T>(const SsaArgs<T>&, T*, T*, T*, T*, T*, const VankaArgs<T>&, T,           \
+    template hipError_t launch_vanka_pass<T>(const SsaArgs<T>&, T*, T*, T*, T*, T*, const VankaArgs<T>&, T, T*, T,   \
                                              hipStream_t);                                                           \
     template hipError_t launch_vanka_dump<T>(const SsaArgs<T>&, T*, T*, T, hipStream_t);                              \
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
@@ -1037,7 +1141,7 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
                                       T*, T*, T*, T, const T*, hipStream_t);                                         \
     template hipError_t launch_adjoint_factor<T>(const SsaArgs<T>&, T*, const VankaArgs<T>&, hipStream_t);          \
     template hipError_t launch_vanka_adjoint_pass_cached<T>(int, int, const T*, const T*, const T*, const T*, T*, T*, \
-                                                            T*, T*, T, hipStream_t);                                 \
+                                                            T*, T*, T, bool, hipStream_t);                           \
     template hipError_t launch_grad_beta_bed<T>(const SsaArgs<T>&, const T*, const T*, T*, T*, hipStream_t);
 
 GLIDE_INSTANTIATE(double)

@@ -34,9 +34,11 @@ hipError_t launch_residual(const SsaArgs<T>& a, T* ou, T* ov, T* oH, int use_for
 template <typename T>
 hipError_t launch_jvp(const SsaArgs<T>& a, const T* du, const T* dv, const T* dH, T* ou, T* ov, T* oH, int use_mask,
                       T calving_rate, hipStream_t st);
+// One forward smoothing pass; phi_next != nullptr also refreshes the grounded
+// fraction from the updated thickness for the next pass (relaxation relax_phi).
 template <typename T>
 hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* mask, const VankaArgs<T>& vk, T omega,
-                             hipStream_t st);
+                             T* phi_next, T relax_phi, hipStream_t st);
 template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st);
 template <typename T>
@@ -53,9 +55,11 @@ constexpr int kAdjointCacheValues = 26;
 constexpr int kAdjointCacheEta = 25;  // plane of the cell viscosity
 template <typename T>
 hipError_t launch_adjoint_factor(const SsaArgs<T>& a, T* cache, const VankaArgs<T>& vk, hipStream_t st);
+// fused: one launch (patch solves of the tile and its left/top neighbours,
+// then the in-place multiplier update); otherwise solve into work + apply.
 template <typename T>
 hipError_t launch_vanka_adjoint_pass_cached(int ny, int nx, const T* cache, const T* ru, const T* rv, const T* rH,
-                                            T* lu, T* lv, T* lH, T* work, T omega, hipStream_t st);
+                                            T* lu, T* lv, T* lH, T* work, T omega, bool fused, hipStream_t st);
 template <typename T>
 hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st);
 

@@ -194,6 +194,12 @@ int glide_gradients(const glide_level* L, double dt, int which, void* stream);
  * unrelated data). */
 int glide_graphs_enable(int on);
 int glide_graphs_clear(void);
+/* Adjoint passes of levels with an adj_cache (no counterpart in the
+ * reference): 1 (default) runs each pass after the transpose operator as one
+ * launch (patch solves of the tile and its left/top neighbours, then the
+ * in-place multiplier update); 0 solves into the workspace and applies in a
+ * second launch.  Both give the same multipliers. */
+int glide_adjoint_fused_enable(int on);
 
 /* Measurement support (no counterpart in the reference): when enabled, HIP
  * event pairs are recorded around every launch of the hot kernels

@@ -275,13 +275,19 @@ def test_vanka_adjoint_pass(orc, use_cache):
         assert rel_err(N(x), y) < KTOL
 
 
-def test_adjoint_cache_matches_uncached_passes(orc):
-    """Several adjoint passes from the cached patch factors agree with passes
-    that rebuild and factorise every patch (mask, calving and nonlinear drag
-    on, non-square level); the cache is filled by the sweep call itself."""
-    F = dome_fields(ny=36, nx=52, seed=14, mask_frac=0.15)
+@pytest.mark.parametrize("shape", [(36, 52), (70, 130)])
+def test_adjoint_cache_matches_uncached_passes(orc, shape):
+    """Several adjoint passes from the cached patch factors, fused into one
+    launch per pass or as solve + apply, agree with passes that rebuild and
+    factorise every patch (mask, calving and nonlinear drag on, non-square
+    levels, partial edge tiles); the cache is filled by the sweep call."""
+    from glide_amd import _lib
+    ny, nx = shape
+    print(f"adjoint cache shape={shape}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=14, mask_frac=0.15)
     out = []
-    for use_cache in (True, False):
+    for use_cache, fused in ((True, True), (True, False), (False, True)):
+        _lib.adjoint_fused_enable(fused)
         g = grid_from(F)
         g.sliding.m.set(1.0 / 3.0)
         g.calving.calving_rate.set(0.3)
@@ -296,7 +302,10 @@ def test_adjoint_cache_matches_uncached_passes(orc):
         ao.vanka_sweep(10.0, 6)
         torch.cuda.synchronize()
         out.append([N(t.data) for t in (g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H)])
-    for x, y in zip(*out):
+    _lib.adjoint_fused_enable(True)
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+    for x, y in zip(out[0], out[2]):
         assert rel_err(x, y) < 1e-13
 
 
