@@ -88,6 +88,7 @@ Phys<R> make_phys(const glide_physics& g) {
     p.glen_p = (1.0 - g.n) / (2.0 * g.n);
     p.drag_q = (g.m - 1.0) / 2.0;
     p.linear_drag = (g.m == 1.0 && g.u_reg > 0.0) ? 1 : 0;
+    p.glen_cube = g.n == 3.0 ? 1 : 0;
     return p;
 }
 

@@ -19,7 +19,28 @@ struct Phys {
     T glen_p;        // (1-n)/(2n)
     T drag_q;        // (m-1)/2
     int linear_drag; // m == 1 and u_reg > 0: the drag power is exactly 1 and its derivative 0
+    int glen_cube;   // n == 3: the viscosity power e2^((1-n)/(2n)) is e2^(-1/3)
 };
+
+// e2^(-1/3) for the n = 3 Glen law without the FP64 pow: an FP32 log2/exp2
+// estimate (relative error ~1e-6) refined by two division-free Newton steps
+// on g(y) = y^-3 - e2, y <- y (4 - e2 y^3) / 3, whose relative error maps
+// e -> -2 e^2 (1e-6 -> 2e-12 -> below FP64 rounding).  Outside the range the
+// FP32 estimate covers the library pow is used.
+__device__ __forceinline__ double inv_cbrt(double x) {
+    if (!(x > 1e-30 && x < 1e30)) return pow(x, -1.0 / 3.0);
+    double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)x) * (-1.0f / 3.0f));
+    y = y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
+    y = y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
+    return y;
+}
+__device__ __forceinline__ float inv_cbrt(float x) { return pow(x, -1.0f / 3.0f); }
+
+// The Glen-law viscosity power e2^((1-n)/(2n)).
+template <typename T>
+__device__ __forceinline__ T glen_power(T e2, const Phys<T>& P) {
+    return P.glen_cube ? inv_cbrt(e2) : pow(e2, P.glen_p);
+}
 
 __device__ __forceinline__ int clampi(int a, int lo, int hi) { return min(max(a, lo), hi); }
 

@@ -32,7 +32,7 @@ __device__ __forceinline__ T cell_viscosity(const T* __restrict__ u, const T* __
                                             int ci, int cj, int ny, int nx, T dxi, const Phys<T>& P) {
     CellVel<T> c = load_cell_vel(u, v, ci, cj, ny, nx);
     T e2 = strain_invariant(c, ci, cj, ny, nx, dxi, P.eps_reg);
-    return T(0.5) * B[(long)ci * nx + cj] * pow(e2, P.glen_p);
+    return T(0.5) * B[(long)ci * nx + cj] * glen_power(e2, P);
 }
 
 // ------------------------------------------------------------------ grounded
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(NT) void k_jvp(SsaArgs<T> a, const T* __restrict__ 
             CellVel<T> d = load_cell_vel(du, dv, ci, cj, ny, nx);
             T e2 = strain_invariant(c, ci, cj, ny, nx, dxi, a.P.eps_reg);
             T de2 = strain_invariant_dir(c, d, ci, cj, ny, nx, dxi);
-            e = T(0.5) * a.B[(long)ci * nx + cj] * pow(e2, a.P.glen_p);
+            e = T(0.5) * a.B[(long)ci * nx + cj] * glen_power(e2, a.P);
             ed = e * a.P.glen_p * de2 / e2;
         }
         s_eta[k / (TX + 2)][k % (TX + 2)] = e;
@@ -760,14 +760,8 @@ __global__ __launch_bounds__(NT) void k_vjp(SsaArgs<T> a, const T* __restrict__ 
             CellVel<T> cl = load_cell_vel(lu, lv, ci, cj, ny, nx);
             T e2 = strain_invariant(cv, ci, cj, ny, nx, dxi, a.P.eps_reg);
             T de2 = strain_invariant_dir(cv, cl, ci, cj, ny, nx, dxi);
-            if (eta_cache) {
-                e = eta_cache[c];
-                ed = e * a.P.glen_p * de2 / e2;
-            } else {
-                T b = a.B[c];
-                e = T(0.5) * b * pow(e2, a.P.glen_p);
-                ed = T(0.5) * b * a.P.glen_p * pow(e2, a.P.glen_p - T(1)) * de2;
-            }
+            e = eta_cache ? eta_cache[c] : T(0.5) * a.B[c] * glen_power(e2, a.P);
+            ed = e * a.P.glen_p * de2 / e2;
             Hh = H[c];
             lHm = lH[c] * (T(1) - a.mask[c]);
             NormalStress<T> sx = sxx(cv.ul, cv.ur, cv.vt, cv.vb, e * Hh, dxi, true);
