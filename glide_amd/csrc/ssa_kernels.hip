@@ -26,6 +26,16 @@ namespace glide {
 constexpr int TX = 64;  // tile width (one wavefront per row)
 constexpr int TY = 4;   // tile height
 constexpr int NT = TX * TY;
+// Tile height of the forward patch solve (a compile-time tuning constant)
+#ifndef GLIDE_VANKA_ROWS
+#define GLIDE_VANKA_ROWS 1
+#endif
+constexpr int kVankaRows = GLIDE_VANKA_ROWS;
+// Tile height of the transpose operator
+#ifndef GLIDE_VJP_ROWS
+#define GLIDE_VJP_ROWS 4
+#endif
+constexpr int kVjpRows = GLIDE_VJP_ROWS;
 
 template <typename T>
 __device__ __forceinline__ T cell_viscosity(const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ B,
@@ -337,9 +347,9 @@ __device__ __forceinline__ void load_patch(Patch<T>& p, const SsaArgs<T>& a, con
 
 // Viscosity tile (+ ring) of cells, optionally refreshing the grounded
 // fraction of the tile's own cells first (the ring recomputes it from H, bed).
-template <typename T>
+template <typename T, int ROWS = TY>
 __device__ __forceinline__ void fill_eta_tile(T (*s_eta)[TX + 2], const SsaArgs<T>& a, int i0, int j0, int tid) {
-    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+    for (int k = tid; k < (ROWS + 2) * (TX + 2); k += TX * ROWS) {
         int ci = i0 - 1 + k / (TX + 2), cj = j0 - 1 + k % (TX + 2);
         T e = T(0);
         if (ci >= 0 && ci < a.ny && cj >= 0 && cj < a.nx) e = cell_viscosity(a.u, a.v, a.B, ci, cj, a.ny, a.nx, a.dxi, a.P);
@@ -351,13 +361,13 @@ __device__ __forceinline__ void fill_eta_tile(T (*s_eta)[TX + 2], const SsaArgs<
 // patch against the state at the start of the pass and writes its increments
 // (half-weighted for facets) into the workspace planes
 // work[0..4] = (du_l, du_r, dv_t, dv_b, dH); the active-set flag goes to mask.
-template <typename T, bool LINEAR, typename TJ>
-__global__ __launch_bounds__(NT) void k_vanka_cell(SsaArgs<T> a, T* __restrict__ work, T* __restrict__ mask,
-                                                   VankaArgs<T> vk) {
-    __shared__ T s_eta[TY + 2][TX + 2];
-    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+template <typename T, bool LINEAR, typename TJ, int ROWS>
+__global__ __launch_bounds__(TX * ROWS) void k_vanka_cell(SsaArgs<T> a, T* __restrict__ work, T* __restrict__ mask,
+                                                          VankaArgs<T> vk) {
+    __shared__ T s_eta[ROWS + 2][TX + 2];
+    const int i0 = blockIdx.y * ROWS, j0 = blockIdx.x * TX;
     const int tid = threadIdx.y * TX + threadIdx.x;
-    fill_eta_tile(s_eta, a, i0, j0, tid);
+    fill_eta_tile<T, ROWS>(s_eta, a, i0, j0, tid);
     __syncthreads();
     const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
     const int ny = a.ny, nx = a.nx;
@@ -718,39 +728,39 @@ __global__ __launch_bounds__(NT) void k_vanka_adjoint_fused(const T* __restrict_
 // fixed order.  Velocity rows use the symmetry of the velocity block as the
 // reference does (glide/cuda/residuals.cu:797-817); Dirichlet facets and
 // masked cells return zero (residuals.cu:1122-1160).
-template <typename T>
+template <typename T, int ROWS>
 struct VjpTile {
-    T lHm[TY + 2][TX + 2];
-    T eta[TY + 2][TX + 2];
-    T etad[TY + 2][TX + 2];
-    T Hs[TY + 2][TX + 2];
-    T Lxx[TY + 2][TX + 2];
-    T Lyy[TY + 2][TX + 2];
-    T Kxx[TY + 2][TX + 2];   // d sigma_xx / d(eta H) at the cell
-    T Kyy[TY + 2][TX + 2];
-    T Lxy[TY + 1][TX + 1];   // vertices [i0, i0+TY] x [j0, j0+TX]
-    T Gxy[TY + 1][TX + 1];   // d sigma_xy / d(eta H) * 0.25 * (λ combination) * dxi
-    T Pu[4][TY + 1][TX + 1]; // λu * drag cross-derivatives, u facets rows [i0-1, i0+TY-1], cols [j0, j0+TX]
-    T Pv[4][TY + 1][TX + 1]; // λv * drag cross-derivatives, v facets rows [i0, i0+TY], cols [j0-1, j0+TX-1]
+    T lHm[ROWS + 2][TX + 2];
+    T eta[ROWS + 2][TX + 2];
+    T etad[ROWS + 2][TX + 2];
+    T Hs[ROWS + 2][TX + 2];
+    T Lxx[ROWS + 2][TX + 2];
+    T Lyy[ROWS + 2][TX + 2];
+    T Kxx[ROWS + 2][TX + 2];   // d sigma_xx / d(eta H) at the cell
+    T Kyy[ROWS + 2][TX + 2];
+    T Lxy[ROWS + 1][TX + 1];   // vertices [i0, i0+ROWS] x [j0, j0+TX]
+    T Gxy[ROWS + 1][TX + 1];   // d sigma_xy / d(eta H) * 0.25 * (λ combination) * dxi
+    T Pu[4][ROWS + 1][TX + 1]; // λu * drag cross-derivatives, u facets rows [i0-1, i0+ROWS-1], cols [j0, j0+TX]
+    T Pv[4][ROWS + 1][TX + 1]; // λv * drag cross-derivatives, v facets rows [i0, i0+ROWS], cols [j0-1, j0+TX-1]
 };
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_vjp(SsaArgs<T> a, const T* __restrict__ lu, const T* __restrict__ lv,
+template <typename T, int ROWS>
+__global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __restrict__ lu, const T* __restrict__ lv,
                                             const T* __restrict__ lH, const T* __restrict__ fu,
                                             const T* __restrict__ fv, const T* __restrict__ fH, T* __restrict__ out_u,
                                             T* __restrict__ out_v, T* __restrict__ out_H, T calving_rate,
                                             const T* __restrict__ eta_cache) {
-    __shared__ VjpTile<T> s;
+    __shared__ VjpTile<T, ROWS> s;
     const int ny = a.ny, nx = a.nx;
     const T dxi = a.dxi;
-    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int i0 = blockIdx.y * ROWS, j0 = blockIdx.x * TX;
     const int tid = threadIdx.y * TX + threadIdx.x;
     const T* __restrict__ u = a.u;
     const T* __restrict__ v = a.v;
     const T* __restrict__ H = a.H;
 
     // phase 1: cells
-    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+    for (int k = tid; k < (ROWS + 2) * (TX + 2); k += (TX * ROWS)) {
         const int ri = k / (TX + 2), rj = k % (TX + 2);
         const int ci = i0 - 1 + ri, cj = j0 - 1 + rj;
         T lHm = 0, e = 0, ed = 0, Hh = 0, Lx = 0, Ly = 0, Kx = 0, Ky = 0;
@@ -775,8 +785,8 @@ __global__ __launch_bounds__(NT) void k_vjp(SsaArgs<T> a, const T* __restrict__ 
         s.Lxx[ri][rj] = Lx; s.Lyy[ri][rj] = Ly; s.Kxx[ri][rj] = Kx; s.Kyy[ri][rj] = Ky;
     }
     __syncthreads();
-    // phase 2a: vertices (vi, vj) in [i0, i0+TY] x [j0, j0+TX]
-    for (int k = tid; k < (TY + 1) * (TX + 1); k += NT) {
+    // phase 2a: vertices (vi, vj) in [i0, i0+ROWS] x [j0, j0+TX]
+    for (int k = tid; k < (ROWS + 1) * (TX + 1); k += (TX * ROWS)) {
         const int ri = k / (TX + 1), rj = k % (TX + 1);
         const int vi = i0 + ri, vj = j0 + rj;
         T L = 0, G = 0;
@@ -795,7 +805,7 @@ __global__ __launch_bounds__(NT) void k_vjp(SsaArgs<T> a, const T* __restrict__ 
         s.Lxy[ri][rj] = L; s.Gxy[ri][rj] = G;
     }
     // phase 2b: drag cross-terms of u facets (rows i0-1.., cols j0..) and v facets (rows i0.., cols j0-1..)
-    for (int k = tid; k < (TY + 1) * (TX + 1); k += NT) {
+    for (int k = tid; k < (ROWS + 1) * (TX + 1); k += (TX * ROWS)) {
         const int ri = k / (TX + 1), rj = k % (TX + 1);
         {
             const int fi = i0 - 1 + ri, fj = j0 + rj;
@@ -1044,13 +1054,14 @@ hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* 
                              T* phi_next, T relax_phi, hipStream_t st) {
     {
         TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx, st);
-        const dim3 g = tile_grid(a.ny, a.nx, false), b(TX, TY);
+        constexpr int R = kVankaRows;
+        const dim3 g((a.nx + TX - 1) / TX, (a.ny + R - 1) / R), b(TX, R);
         if (vk.local_fp32) {
-            if (a.P.linear_drag) k_vanka_cell<T, true, float><<<g, b, 0, st>>>(a, work, mask, vk);
-            else k_vanka_cell<T, false, float><<<g, b, 0, st>>>(a, work, mask, vk);
+            if (a.P.linear_drag) k_vanka_cell<T, true, float, R><<<g, b, 0, st>>>(a, work, mask, vk);
+            else k_vanka_cell<T, false, float, R><<<g, b, 0, st>>>(a, work, mask, vk);
         } else {
-            if (a.P.linear_drag) k_vanka_cell<T, true, T><<<g, b, 0, st>>>(a, work, mask, vk);
-            else k_vanka_cell<T, false, T><<<g, b, 0, st>>>(a, work, mask, vk);
+            if (a.P.linear_drag) k_vanka_cell<T, true, T, R><<<g, b, 0, st>>>(a, work, mask, vk);
+            else k_vanka_cell<T, false, T, R><<<g, b, 0, st>>>(a, work, mask, vk);
         }
     }
     dim3 g((a.nx + 1 + 255) / 256, a.ny + 1);
@@ -1082,8 +1093,9 @@ template <typename T>
 hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH, const T* fu, const T* fv,
                       const T* fH, T* ou, T* ov, T* oH, T calving_rate, const T* eta_cache, hipStream_t st) {
     TimedLaunch tl(kTimeVjp, (long)a.ny * a.nx, st);
-    k_vjp<T><<<tile_grid(a.ny, a.nx, true), dim3(TX, TY), 0, st>>>(a, lu, lv, lH, fu, fv, fH, ou, ov, oH,
-                                                                    calving_rate, eta_cache);
+    constexpr int R = kVjpRows;
+    const dim3 g((a.nx + 1 + TX - 1) / TX, (a.ny + 1 + R - 1) / R);
+    k_vjp<T, R><<<g, dim3(TX, R), 0, st>>>(a, lu, lv, lH, fu, fv, fH, ou, ov, oH, calving_rate, eta_cache);
     return hipGetLastError();
 }
 
