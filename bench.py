@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--nz", type=int, default=41, help="sigma levels of the enthalpy columns (0: no thermal step)")
     ap.add_argument("--n-smooth", type=int, default=10, help="maximum enthalpy sweeps per step")
+    ap.add_argument("--enth-adjoint-rtol", type=float, default=1e-8,
+                    help="stopping test of the enthalpy adjoint sweeps (relative to the first residual)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false",
                     help="skip the mixed-precision run (FP32 patch Jacobian) and its error against FP64")
@@ -85,17 +87,18 @@ def build_model(args, torch):
     return model, thermal
 
 
-def run_timed(model, thermal, args, torch):
+def run_timed(stepper, args, torch):
     """Warm-up steps, then the timed steps on the production path (fixed-cycle
-    solves replayed from the library's cached HIP graphs); returns (t, seconds)."""
+    solves replayed from the library's cached HIP graphs, enthalpy work on its
+    own stream); returns (t, seconds)."""
     t = 0.0
     for _ in range(args.warmup):
-        one_step(model, thermal, t, args.dt)
+        stepper(t, args.dt)
         t += args.dt
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        one_step(model, thermal, t, args.dt)
+        stepper(t, args.dt)
         t += args.dt
     torch.cuda.synchronize()
     return t, time.perf_counter() - t0
@@ -141,14 +144,59 @@ def read_fp64_flops(args):
     return None
 
 
-def one_step(model, thermal, t, dt):
-    if thermal is not None:
-        thermal.pre_momentum()
-    model.forward(t, dt, update_geometry=True)
-    if thermal is not None:
-        thermal.step(dt * SEC_PER_YR)
-    g = model.mg[0]
-    model.backward(t, dt, dJdu=g.state.u.data, dJdv=g.state.v.data, dJdH=None)
+class Stepper:
+    """One coupled step: momentum + mass forward solve, enthalpy step and its
+    adjoint, momentum adjoint + gradients, then the rate-factor feedback.
+
+    The enthalpy step and its adjoint depend only on the forward solution, and
+    the momentum adjoint only on the forward state and the rate factor the
+    forward used, so with ``concurrent`` the enthalpy work runs on a second
+    stream while the adjoint's latency-bound coarse levels leave the GPU
+    mostly idle; the new rate factor is written to the grid after both."""
+
+    def __init__(self, model, thermal, torch, adj_rtol=1e-8):
+        self.model, self.thermal, self.torch = model, thermal, torch
+        self.adj_rtol = adj_rtol
+        if thermal is not None:
+            g = model.mg[0]
+            self.stream = torch.cuda.Stream(device=g.state.u.data.device)
+            self.B_next = torch.empty_like(g.rheology.B.data)
+            self.dL_dB = torch.ones_like(g.rheology.B.data)
+            self.ev_fwd, self.ev_thermal = torch.cuda.Event(), torch.cuda.Event()
+        self.adjoint_sweeps = None
+
+    def _thermal(self, dt, report):
+        th = self.thermal
+        th.step(dt * SEC_PER_YR, B_out=self.B_next)
+        # objective 0.5 |E|^2 + sum(B): the seeds only set the right-hand side;
+        # a report synchronises the host, so only the sequential steps ask for one
+        res = th.ops.adjoint_solve(dt * SEC_PER_YR, dL_dE=th.ops.enthalpy_state.E, dL_dB=self.dL_dB,
+                                   B_scale=th.B_scale, report=report, relative_tolerance=self.adj_rtol,
+                                   n_iter=256)
+        if report:
+            self.adjoint_sweeps = res["report"].sweeps
+
+    def __call__(self, t, dt, concurrent=True):
+        torch, model, thermal = self.torch, self.model, self.thermal
+        if thermal is not None:
+            thermal.pre_momentum()
+        model.forward(t, dt, update_geometry=True)
+        g = model.mg[0]
+        main = torch.cuda.current_stream()
+        if thermal is not None:
+            if concurrent:
+                self.ev_fwd.record(main)
+                self.stream.wait_event(self.ev_fwd)
+                with torch.cuda.stream(self.stream):
+                    self._thermal(dt, False)
+                    self.ev_thermal.record(self.stream)
+            else:
+                self._thermal(dt, True)
+        model.backward(t, dt, dJdu=g.state.u.data, dJdv=g.state.v.data, dJdH=None)
+        if thermal is not None:
+            if concurrent:
+                main.wait_event(self.ev_thermal)
+            g.rheology.B.data.copy_(self.B_next)
 
 
 def cpu_baseline(args):
@@ -176,7 +224,7 @@ def cpu_baseline(args):
     om.fas_adjoint_solve(args.dt, cfg)
     om.gradients(args.dt)
     t_adj = time.perf_counter() - t0
-    t_th = 0.0
+    t_th = t_tha = 0.0
     if args.nz > 0:
         from glide_amd.synthetic import thermal_forcing
         Ts, qg = thermal_forcing(p)
@@ -191,14 +239,34 @@ def cpu_baseline(args):
         O.thermal_step(th, u, v, om.get("H"), p.beta, args.dt * SEC_PER_YR, n_smooth=args.n_smooth,
                        strain_heating=phi)
         t_th = time.perf_counter() - t0
+        # the enthalpy adjoint with the GPU step's seeds (dL/dE = E, dL/dB = 1):
+        # the oracle solves J^T lambda = g directly (sparse LU), whose fill-in
+        # grows faster than the cell count, so it is timed on a na^2 sample and
+        # scaled by the cell ratio (which favours the CPU)
+        na = min(n, 24)
+        pa = dome_problem(na, 1500.0, seed=args.seed)
+        Tsa, qga = thermal_forcing(pa)
+        tha = O.OracleThermal(na, na, args.nz, 1500.0, pa.H)
+        tha.initialize_from_temperature(Tsa)
+        tha.set_surface_enthalpy_from_temperature(Tsa)
+        tha.Q_geo = qga
+        O.thermal_step(tha, np.zeros((na, na + 1)), np.zeros((na + 1, na)), pa.H, pa.beta, args.dt * SEC_PER_YR,
+                       n_smooth=args.n_smooth)
+        t0 = time.perf_counter()
+        bs = 917.0 * 9.81 * SEC_PER_YR ** (1.0 / 3.0)
+        gE = tha.E + tha.arrhenius_vjp(np.ones((na, na)), 3.0, bs)
+        lam = tha.adjoint_solve(args.dt * SEC_PER_YR, gE)
+        tha.adjoint_gradients(args.dt * SEC_PER_YR, lam)
+        t_tha = (time.perf_counter() - t0) * (n / na) ** 2
     scale = (args.n / n) ** 2
-    step_s = (args.vcycles * (t_fwd + t_adj) + t_th) * scale
+    step_s = (args.vcycles * (t_fwd + t_adj) + t_th + t_tha) * scale
     return {"value": 1.0 / step_s, "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
             "sample": (f"FP64 oracle, 1 forward + 1 adjoint V-cycle of the dome workload at {n}^2 "
                        f"({args.levels} levels): {t_fwd:.2f} s + {t_adj:.2f} s, x{args.vcycles} V-cycles per step; "
-                       f"one thermal step on {n}^2 x {args.nz} columns: {t_th:.2f} s; "
+                       f"one thermal step on {n}^2 x {args.nz} columns: {t_th:.2f} s, its adjoint "
+                       f"(sparse direct, 24^2 sample scaled to {n}^2): {t_tha:.2f} s; "
                        f"x{scale:g} cell ratio to {args.n}^2"),
-            "sample_seconds": round(t_fwd + t_adj + t_th, 3)}
+            "sample_seconds": round(t_fwd + t_adj + t_th + t_tha, 3)}
 
 
 def main():
@@ -214,7 +282,8 @@ def main():
         sys.exit(1)
     _lib.load()
     model, thermal = build_model(args, torch)
-    t, elapsed = run_timed(model, thermal, args, torch)
+    stepper = Stepper(model, thermal, torch, args.enth_adjoint_rtol)
+    t, elapsed = run_timed(stepper, args, torch)
     fp64_state = snapshot(model, thermal)
     mixed = None
     if args.mixed:
@@ -222,7 +291,7 @@ def main():
         # fresh copy of the workload, same steps; error against the FP64 state
         m2, th2 = build_model(args, torch)
         m2.forward_solver.vanka_options.set(local_fp32=True)
-        _, el2 = run_timed(m2, th2, args, torch)
+        _, el2 = run_timed(Stepper(m2, th2, torch, args.enth_adjoint_rtol), args, torch)
         errs = {k: rel_l2(a, b) for (k, a), b in zip(snapshot(m2, th2).items(), fp64_state.values())}
         mixed = {"value": round(args.steps / el2, 4), "ms_per_step": round(1000.0 * el2 / args.steps, 3),
                  "speedup_vs_fp64": round(elapsed / el2, 4), "rel_l2_err_vs_fp64": errs,
@@ -231,12 +300,13 @@ def main():
         del m2, th2
     # per-kernel durations: the same number of steps again with HIP events
     # around every hot launch (events inside graphs cannot be timed, so these
-    # steps launch eagerly; kernel bodies are identical)
+    # steps launch eagerly, and on one stream so that kernels do not share the
+    # GPU while timed; kernel bodies are identical)
     _lib.timing_enable(True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        one_step(model, thermal, t, args.dt)
+        stepper(t, args.dt, concurrent=False)
         t += args.dt
     torch.cuda.synchronize()
     elapsed_eager = time.perf_counter() - t1
@@ -319,6 +389,7 @@ def main():
                        "vcycles": args.vcycles, "dt_years": args.dt, "parallelism": "single GPU"},
             "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "kernels": kernels,
             "ms_per_step_eager_timed": round(1000.0 * elapsed_eager / args.steps, 3),
+            "enthalpy_adjoint_sweeps": stepper.adjoint_sweeps,
             "omp_threads": os.environ.get("OMP_NUM_THREADS")}
     print(json.dumps(line), flush=True)
 

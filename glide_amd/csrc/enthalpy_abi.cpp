@@ -368,17 +368,12 @@ int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoin
                                         s->partials, s->status, 0, rtol, atol, st),
             "adjoint residual");
     const int* skip = &s->status->done;
-    for (int it = 0; it < n_iter; ++it) {
-        HIP_TRY(launch_enth_adj_column<R>(skip, ny, nx, nz, CP(X->rows), CP(X->g), CP(X->lambda), P(X->lambda_work),
-                                          cfg->omega, st),
-                "adjoint column pass");
-        HIP_TRY(launch_enth_adj_layer<R>(skip, ny, nx, nz, CP(X->rows), CP(X->g), CP(X->lambda_work), P(X->lambda),
-                                         cfg->omega, st),
-                "adjoint layer pass");
-        HIP_TRY(launch_enth_adj_residual<R>(skip, ny, nx, nz, CP(X->rows), CP(X->g), CP(X->lambda), P(X->r),
-                                            s->partials, s->status, 1, rtol, atol, st),
-                "adjoint residual");
-    }
+    // red-black column Gauss-Seidel sweeps; each records the residual of the
+    // columns before their update and stops (device-side) at the tolerance
+    for (int it = 0; it < n_iter; ++it)
+        HIP_TRY(launch_enth_adj_sweep_rb<R>(skip, ny, nx, nz, CP(X->rows), CP(X->g), P(X->lambda), cfg->omega,
+                                            s->partials, s->status, rtol, atol, st),
+                "adjoint sweep");
     HIP_TRY(launch_enth_adj_grads<R>(A, CP(X->lambda), P(X->g_E_prev), P(X->g_phi_strain), P(X->g_Q_geo),
                                      P(X->g_E_surface), st),
             "adjoint gradients");

@@ -748,6 +748,66 @@ __global__ __launch_bounds__(256) void k_enth_adj_column(const int* __restrict__
     if (in) lam_out[n] = l0 + omega_relax * (lc - l0);
 }
 
+// Column of colour `color` ((i + j) % 2 == color) with colour index w:
+// rows alternate between the counts ce (even rows) and co (odd rows).
+__device__ __forceinline__ ColIdx color_col_index(long w, int color, int ny, int nx) {
+    const int ce = (nx + 1 - (color & 1)) / 2;  // columns of this colour in an even row
+    const long p = w / nx, r = w % nx;          // a pair of rows holds nx of them
+    const int i = (int)(r < ce ? 2 * p : 2 * p + 1);
+    const int t = (int)(r < ce ? r : r - ce);
+    const int j = 2 * t + ((i + color) & 1);
+    ColIdx c;
+    c.valid = i < ny && j < nx;
+    c.i = c.valid ? i : 0;
+    c.j = c.valid ? j : 0;
+    c.col = (long)c.i * nx + c.j;
+    c.k = lane_id();
+    return c;
+}
+
+// Red-black column Gauss-Seidel pass on J^T lam = g, in place: the columns of
+// one colour solve their transposed tridiagonal block (PCR) against g minus
+// the horizontal coupling of the other colour's current multipliers, which
+// this pass does not change; lam = lam + omega_relax (lam_col - lam).  The
+// max |g - J^T lam| of the columns before their update goes to
+// partials[blockIdx.x] (the residual the sweep's stopping test reads).
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_adj_column_rb(const int* __restrict__ skip, int ny, int nx, int nz,
+                                                            const T* __restrict__ rows, const T* __restrict__ g,
+                                                            T* __restrict__ lam, T omega_relax, int color,
+                                                            double* __restrict__ partials) {
+    __shared__ double smax[kColsPerBlock];
+    if (skipped(skip)) return;
+    const long ncol = (long)ny * nx, n3 = ncol * nz;
+    const long nc = (long)((ny + 1) / 2) * nx;  // colour indices covered (pairs of rows)
+    const int k = lane_id();
+    const bool in = k < nz;
+    double m = 0.0;
+    for (long w = wave_column(); w < nc; w += (long)gridDim.x * kColsPerBlock) {
+        const ColIdx q = color_col_index(w, color, ny, nx);
+        if (!q.valid) continue;
+        const long n = q.col * nz + k;
+        const T a = in ? rows[n] : T(0), b = in ? rows[n3 + n] : T(1), c = in ? rows[2 * n3 + n] : T(0);
+        const T cm = lane_get(c, max(k - 1, 0)), ap = lane_get(a, min(k + 1, 63));
+        const T aT = (in && k > 0) ? cm : T(0);        // (J^T)_{k,k-1} = J_{k-1,k}
+        const T cT = (in && k + 1 < nz) ? ap : T(0);   // (J^T)_{k,k+1} = J_{k+1,k}
+        T rhs = T(0), l0 = T(0);
+        if (in) {
+            l0 = lam[n];
+            rhs = g[n] - jt_horizontal(rows, lam, ny, nx, nz, n3, q.i, q.j, k);
+        }
+        const T lm = lane_get(l0, max(k - 1, 0)), lp = lane_get(l0, min(k + 1, 63));
+        const T res = in ? rhs - (aT * lm + b * l0 + cT * lp) : T(0);
+        m = fmax(m, fabs((double)res));
+        const T lc = pcr_solve(aT, b, cT, rhs);
+        if (in) lam[n] = l0 + omega_relax * (lc - l0);
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if (k == 0) smax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+}
+
 // (J^T lam)_n at one node.
 template <typename T>
 __device__ __forceinline__ T jt_apply(const T* __restrict__ rows, const T* __restrict__ lam, int ny, int nx, int nz,
@@ -981,6 +1041,24 @@ hipError_t launch_enth_adj_column(const int* skip, int ny, int nx, int nz, const
     return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_enth_adj_sweep_rb(const int* skip, int ny, int nx, int nz, const T* rows, const T* g, T* lam,
+                                    T omega_relax, double* partials, EnthStatus* status, double rtol, double atol,
+                                    hipStream_t st) {
+    const long nc = (long)((ny + 1) / 2) * nx;
+    long nb = (nc + kColsPerBlock - 1) / kColsPerBlock;
+    const unsigned g_ = (unsigned)(nb < kEnthResBlocks / 2 ? (nb > 0 ? nb : 1) : kEnthResBlocks / 2);
+    {
+        TimedLaunch tl(kTimeEnthColumn, (long)ny * nx * nz / 2, st);
+        k_enth_adj_column_rb<T><<<g_, 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam, omega_relax, 0, partials);
+    }
+    {
+        TimedLaunch tl(kTimeEnthColumn, (long)ny * nx * nz / 2, st);
+        k_enth_adj_column_rb<T><<<g_, 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam, omega_relax, 1, partials + g_);
+    }
+    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)(2 * g_), nullptr, status, 1, rtol, atol);
+    return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_enth_adj_layer(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
                                  const T* lam_in, T* lam_out, T omega_relax, hipStream_t st) {
     k_enth_adj_layer<T><<<flat_blocks((long)ny * nx * nz), 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam_in, lam_out,
@@ -1024,6 +1102,8 @@ hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev
     template hipError_t launch_enth_adj_rhs<T>(const EnthArgs<T>&, const T*, const T*, T, T, T*, hipStream_t);       \
     template hipError_t launch_enth_adj_column<T>(const int*, int, int, int, const T*, const T*, const T*, T*, T,    \
                                                   hipStream_t);                                                      \
+    template hipError_t launch_enth_adj_sweep_rb<T>(const int*, int, int, int, const T*, const T*, T*, T, double*,   \
+                                                    EnthStatus*, double, double, hipStream_t);                       \
     template hipError_t launch_enth_adj_layer<T>(const int*, int, int, int, const T*, const T*, const T*, T*, T,     \
                                                  hipStream_t);                                                       \
     template hipError_t launch_enth_adj_residual<T>(const int*, int, int, int, const T*, const T*, const T*, T*,     \

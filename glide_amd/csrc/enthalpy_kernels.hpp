@@ -91,6 +91,13 @@ hipError_t launch_enth_adj_rhs(const EnthArgs<T>& A, const T* gE, const T* gB, T
 template <typename T>
 hipError_t launch_enth_adj_column(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
                                   const T* lam_in, T* lam_out, T omega_relax, hipStream_t st);
+// One adjoint sweep: red and black column Gauss-Seidel passes (in place)
+// and the stopping test on the max residual of the columns before their
+// update (status mode 1).
+template <typename T>
+hipError_t launch_enth_adj_sweep_rb(const int* skip, int ny, int nx, int nz, const T* rows, const T* g, T* lam,
+                                    T omega_relax, double* partials, EnthStatus* status, double rtol, double atol,
+                                    hipStream_t st);
 template <typename T>
 hipError_t launch_enth_adj_layer(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
                                  const T* lam_in, T* lam_out, T omega_relax, hipStream_t st);

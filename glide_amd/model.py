@@ -116,16 +116,29 @@ class ThermalModel:
         ops.enthalpy_state.E_prev.copy_(ops.enthalpy_state.E)
         ops.H_prev.copy_(ops.grid.state.H.data)
 
-    def step(self, dt, report=False):
-        """Advance the enthalpy one step (glide/model.py:167-213)."""
+    def step(self, dt, report=False, B_out=None):
+        """Advance the enthalpy one step (glide/model.py:167-213).
+
+        B_out (optional, (ny, nx) float64 tensor): receive the new rate factor
+        there instead of overwriting ``grid.rheology.B``, which then stays the
+        one the momentum solve used (an addition for callers that run the
+        momentum adjoint concurrently and apply the feedback afterwards)."""
         import ctypes
 
         from . import _lib
         from ._lib import call, ptr
         ops = self.ops
         if ops.smoother_config.hook_func is not None:
+            if B_out is not None:
+                raise ValueError("B_out needs the fused native step (no smoother hook)")
             return self._step_host(dt)
         grid = ops.grid
+        B = grid.rheology.B.data
+        if B_out is not None:
+            if tuple(B_out.shape) != tuple(B.shape) or B_out.dtype != B.dtype or B_out.device != B.device:
+                raise ValueError("B_out must match grid.rheology.B in shape, dtype and device")
+            B_out.copy_(B)  # strain heating reads the current rate factor
+            B = B_out
         n = float(grid.rheology.n.value)
         tc = _lib.ThermalConfigC(dt=float(dt), m=float(grid.sliding.m.value), u_reg=float(grid.sliding.u_reg.value),
                                  rho_i=float(self.rho_i), n_glen=n, eps_reg=float(grid.rheology.eps_reg.value),
@@ -137,7 +150,7 @@ class ThermalModel:
         C = ops._column()
         call("glide_thermal_step", ctypes.byref(C), ptr(grid.state.u.data), ptr(grid.state.v.data),
              ptr(grid.sliding.beta.data),
-             ptr(grid.rheology.B.data) if (self.update_rheology or self.strain_heating) else None,
+             ptr(B) if (self.update_rheology or self.strain_heating) else None,
              ctypes.byref(tc), ctypes.byref(ops.smoother_config.to_c(True)),
              ctypes.byref(rep) if rep is not None else None, ops._stream())
         self.last_report = rep

@@ -147,7 +147,7 @@ typedef struct glide_enth_adjoint {
     void* rows;        /* workspace (7, ny, nx, nz): exact Jacobian rows */
     void* g;           /* workspace (ny, nx, nz): right-hand side */
     void* lambda;      /* (ny, nx, nz): initial guess in, adjoint state out */
-    void* lambda_work; /* workspace (ny, nx, nz) */
+    void* lambda_work; /* workspace (ny, nx, nz), reserved (the in-place sweeps do not use it) */
     void* r;           /* workspace (ny, nx, nz): adjoint residual */
     void *g_E_prev, *g_phi_strain; /* (ny, nx, nz) gradients, either may be NULL */
     void *g_Q_geo, *g_E_surface;   /* (ny, nx) gradients (g_Q_geo is also dL/dQ_fh), either may be NULL */
@@ -156,9 +156,10 @@ typedef struct glide_enth_adjoint {
 /* Adjoint of the thermal solve at the converged E of C: solves J^T lambda = g
  * with J the exact Jacobian of the system the sweeps solve (interior rows,
  * Dirichlet surface node, clamped thin columns) and g = dL/dE + (dB/dE)^T
- * dL/dB (dL_dB may be NULL; B = rate factor / B_scale), by alternating
- * transposed column and layer passes (relaxation cfg->omega) for up to n_iter
- * sweeps with the device-side stopping test on max |g - J^T lambda|
+ * dL/dB (dL_dB may be NULL; B = rate factor / B_scale), by red-black column
+ * Gauss-Seidel sweeps (each column's transposed tridiagonal block solved
+ * exactly, relaxation cfg->omega) for up to n_iter sweeps with the device-side
+ * stopping test on max |g - J^T lambda| of the columns before their update
  * (cfg tolerances); then writes the parameter gradients. */
 int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoint* X, double dt, const void* dL_dE,
                              const void* dL_dB, double n_glen, double B_scale, int n_iter,

@@ -283,6 +283,30 @@ def test_thermal_step_host_path_matches_fused():
     assert rel_err(N(g2.rheology.B.data), N(g.rheology.B.data)) < STOL
 
 
+def test_thermal_step_deferred_rate_factor():
+    """step(B_out=...) leaves grid.rheology.B as the momentum solve used it and
+    returns the same new rate factor (and enthalpy) as the in-place step, also
+    when run on a second stream."""
+    print("thermal_step deferred B", flush=True)
+    F, g, tm, _ = _thermal_pair(16, 21, 5, strain=True)
+    F2, g2, tm2, _ = _thermal_pair(16, 21, 5, strain=True)
+    B0 = g2.rheology.B.data.clone()
+    B_out = torch.empty_like(B0)
+    tm.step(DT)
+    s = torch.cuda.Stream()
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream())
+    s.wait_event(ev)
+    with torch.cuda.stream(s):
+        tm2.step(DT, B_out=B_out)
+    torch.cuda.synchronize()
+    assert torch.equal(g2.rheology.B.data, B0)
+    assert torch.equal(B_out, g.rheology.B.data)
+    assert torch.equal(tm2.ops.enthalpy_state.E, tm.ops.enthalpy_state.E)
+    with pytest.raises(ValueError):
+        tm2.step(DT, B_out=torch.empty(3, device="cuda", dtype=torch.float64))
+
+
 # ---------------------------------------------------------------- adjoint
 B_SCALE = 917.0 * 9.81 * (365.25 * 86400.0) ** (1.0 / 3.0)
 
