@@ -141,6 +141,7 @@ SIGNATURES = {
     "glide_gradients": (_I, [_LP, _D, _I, _P]),
     "glide_graphs_enable": (_I, [_I]),
     "glide_adjoint_fused_enable": (_I, [_I]),
+    "glide_forward_fused_enable": (_I, [_I]),
     "glide_graphs_clear": (_I, []),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -179,6 +180,11 @@ def graphs_enable(on: bool) -> None:
 def adjoint_fused_enable(on: bool) -> None:
     """Cached adjoint passes as one fused launch (default) or solve + apply."""
     call("glide_adjoint_fused_enable", int(bool(on)))
+
+
+def forward_fused_enable(on: bool) -> None:
+    """Forward passes of small levels as one fused launch (default) or solve + apply."""
+    call("glide_forward_fused_enable", int(bool(on)))
 
 
 def graphs_clear() -> None:

@@ -150,6 +150,12 @@ int sumsq3_async(const R* a, size_t na, const R* b, size_t nb, const R* c, size_
     return GLIDE_OK;
 }
 
+// Adjoint passes from the cache run as one fused launch (default) or as
+// solve + apply (glide_adjoint_fused_enable).
+bool g_fused_adjoint = true;
+// Forward passes of small levels as one fused launch (glide_forward_fused_enable).
+bool g_fused_forward = true;
+
 int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
                  hipStream_t st) {
     const bool freeze_phi = (flags & GLIDE_FREEZE_PHI) != 0;
@@ -160,6 +166,28 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
     vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
     vk.iters = nullptr;
     vk.local_fp32 = vc.local_fp32;
+    // small levels with a frozen grounded fraction: one fused launch per pass,
+    // the state alternating between the level's buffers and the workspace
+    // (5 planes hold u, v and H); an odd pass count copies it back at the end
+    if (freeze_phi && !vk.local_fp32 && g_fused_forward && (long)L.ny * L.nx <= kFusedVankaMaxCells) {
+        R* alt_u = P(L.work);
+        R* alt_v = alt_u + nU(L);
+        R* alt_H = alt_v + nV(L);
+        for (int it = 0; it < n_iter; ++it) {
+            const bool from_alt = (it & 1) != 0;
+            SsaArgs<R> a = make_args(L, dt, false);
+            if (from_alt) { a.u = alt_u; a.v = alt_v; a.H = alt_H; }
+            HIP_TRY(launch_vanka_fused<R>(a, from_alt ? P(L.u) : alt_u, from_alt ? P(L.v) : alt_v,
+                                          from_alt ? P(L.H) : alt_H, P(L.mask), vk, vc.omega, st),
+                    "vanka fused");
+        }
+        if (n_iter & 1) {
+            HIP_TRY(launch_copy<R>(P(L.u), alt_u, (long)nU(L), st), "copy u");
+            HIP_TRY(launch_copy<R>(P(L.v), alt_v, (long)nV(L), st), "copy v");
+            HIP_TRY(launch_copy<R>(P(L.H), alt_H, (long)nC(L), st), "copy H");
+        }
+        return GLIDE_OK;
+    }
     // the grounded fraction is refreshed before every pass: before the first
     // by itself, before the others by the previous pass's increment apply
     if (!freeze_phi && n_iter > 0) GLIDE_TRY(grounded(L, vc.relax_phi, st));
@@ -173,9 +201,6 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
     return GLIDE_OK;
 }
 
-// Adjoint passes from the cache run as one fused launch (default) or as
-// solve + apply (glide_adjoint_fused_enable).
-bool g_fused_adjoint = true;
 
 // The adjoint smoother cache of a level (its transposed patch factors and
 // cell viscosity) when the level has one and adjoint_prepare has filled it for
@@ -783,6 +808,11 @@ int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_l
 
 int glide_adjoint_fused_enable(int on) {
     g_fused_adjoint = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_forward_fused_enable(int on) {
+    g_fused_forward = on != 0;
     return GLIDE_OK;
 }
 

@@ -304,8 +304,8 @@ __global__ __launch_bounds__(NT) void k_jvp(SsaArgs<T> a, const T* __restrict__ 
 // ------------------------------------------------------------------ Vanka
 // Gather the patch of cell (i,j); viscosity from the LDS tile (li, lj are the
 // cell's tile coordinates including the one-cell ring).
-template <typename T>
-__device__ __forceinline__ void load_patch(Patch<T>& p, const SsaArgs<T>& a, const T (*s_eta)[TX + 2], int li, int lj,
+template <typename T, int W = TX + 2>
+__device__ __forceinline__ void load_patch(Patch<T>& p, const SsaArgs<T>& a, const T (*s_eta)[W], int li, int lj,
                                            int i, int j, const T* __restrict__ fu, const T* __restrict__ fv,
                                            const T* __restrict__ fH, const T* __restrict__ phi) {
     const int ny = a.ny, nx = a.nx;
@@ -361,22 +361,21 @@ __device__ __forceinline__ void fill_eta_tile(T (*s_eta)[TX + 2], const SsaArgs<
 // patch against the state at the start of the pass and writes its increments
 // (half-weighted for facets) into the workspace planes
 // work[0..4] = (du_l, du_r, dv_t, dv_b, dH); the active-set flag goes to mask.
-template <typename T, bool LINEAR, typename TJ, int ROWS>
-__global__ __launch_bounds__(TX * ROWS) void k_vanka_cell(SsaArgs<T> a, T* __restrict__ work, T* __restrict__ mask,
-                                                          VankaArgs<T> vk) {
-    __shared__ T s_eta[ROWS + 2][TX + 2];
-    const int i0 = blockIdx.y * ROWS, j0 = blockIdx.x * TX;
-    const int tid = threadIdx.y * TX + threadIdx.x;
-    fill_eta_tile<T, ROWS>(s_eta, a, i0, j0, tid);
-    __syncthreads();
-    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+// The damped local Newton iteration of cell (i,j) on its patch against the
+// state in a (viscosity from the tile s_eta, whose cell (li, lj) is (i,j)):
+// returns the increments w = ((u_l, u_r, v_t, v_b) change / 2, H change) and
+// the active-set flag.  edge: the calling wave may hold cells on the domain's
+// edge (the loop then tests the Dirichlet pins; interior waves carry no pin
+// code).  The wave-uniform edge test keeps one loop copy per wave.
+template <typename T, bool LINEAR, typename TJ, int W>
+__device__ __forceinline__ void cell_newton(const SsaArgs<T>& a, const T (*s_eta)[W], int li, int lj, int i, int j,
+                                            const VankaArgs<T>& vk, bool edge, T w[5], T& masked, int& iters) {
     const int ny = a.ny, nx = a.nx;
-    if (i >= ny || j >= nx) return;
     CellSys<T> cs;
     T thk;
     {
         Patch<T> p;
-        load_patch(p, a, s_eta, threadIdx.y + 1, threadIdx.x + 1, i, j, a.f_u, a.f_v, a.f_H, a.phi);
+        load_patch<T, W>(p, a, s_eta, li, lj, i, j, a.f_u, a.f_v, a.f_H, a.phi);
         make_cell_sys(cs, p, a.P, vk.calving_rate, a.dxi, a.dt, vk.ssa_damping, vk.mc_damping);
         thk = p.gam;
     }
@@ -390,14 +389,10 @@ __global__ __launch_bounds__(TX * ROWS) void k_vanka_cell(SsaArgs<T> a, T* __res
     T cmp[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) cmp[k] = T(0);
-    T masked = mask[c];
     T rnorm = T(1);
     const T tol = T(1e-6);
     const bool pin0 = j == 0, pin1 = j == nx - 1, pin2 = i == 0, pin3 = i == ny - 1;
     int it = 0;
-    // The Newton loop in two copies: waves on the domain's edge test the
-    // Dirichlet pins in every iteration, interior waves (a wave is one tile
-    // row, so the test is wave-uniform) carry no pin code at all.
     auto newton = [&](auto edge_tag) {
         constexpr bool EDGE = decltype(edge_tag)::value;
         while (it < vk.newton_steps && rnorm > tol) {
@@ -432,17 +427,115 @@ __global__ __launch_bounds__(TX * ROWS) void k_vanka_cell(SsaArgs<T> a, T* __res
             ++it;
         }
     };
-    const bool edge_wave = i == 0 || i == ny - 1 || j0 == 0 || j0 + TX >= nx;
-    if (edge_wave) newton(std::true_type{});
+    if (edge) newton(std::true_type{});
     else newton(std::false_type{});
+    w[0] = T(0.5) * (x[0] - a.u[(long)i * (nx + 1) + j]);
+    w[1] = T(0.5) * (x[1] - a.u[(long)i * (nx + 1) + j + 1]);
+    w[2] = T(0.5) * (x[2] - a.v[c]);
+    w[3] = T(0.5) * (x[3] - a.v[c + nx]);
+    w[4] = x[4] - a.H[c];
+    iters = it;
+}
+
+// Forward smoothing: every cell runs the local damped Newton iteration on its
+// patch against the state at the start of the pass and writes its increments
+// (half-weighted for facets) into the workspace planes
+// work[0..4] = (du_l, du_r, dv_t, dv_b, dH); the active-set flag goes to mask.
+template <typename T, bool LINEAR, typename TJ, int ROWS>
+__global__ __launch_bounds__(TX * ROWS) void k_vanka_cell(SsaArgs<T> a, T* __restrict__ work, T* __restrict__ mask,
+                                                          VankaArgs<T> vk) {
+    __shared__ T s_eta[ROWS + 2][TX + 2];
+    const int i0 = blockIdx.y * ROWS, j0 = blockIdx.x * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    fill_eta_tile<T, ROWS>(s_eta, a, i0, j0, tid);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    const int ny = a.ny, nx = a.nx;
+    if (i >= ny || j >= nx) return;
+    const long c = (long)i * nx + j;
+    T w[5], masked = mask[c];
+    int it = 0;
+    const bool edge_wave = i == 0 || i == ny - 1 || j0 == 0 || j0 + TX >= nx;
+    cell_newton<T, LINEAR, TJ, TX + 2>(a, s_eta, threadIdx.y + 1, threadIdx.x + 1, i, j, vk, edge_wave, w, masked, it);
     const long n = (long)ny * nx;
-    work[c] = T(0.5) * (x[0] - a.u[(long)i * (nx + 1) + j]);
-    work[n + c] = T(0.5) * (x[1] - a.u[(long)i * (nx + 1) + j + 1]);
-    work[2 * n + c] = T(0.5) * (x[2] - a.v[c]);
-    work[3 * n + c] = T(0.5) * (x[3] - a.v[c + nx]);
-    work[4 * n + c] = x[4] - a.H[c];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) work[q * n + c] = w[q];
     mask[c] = masked;
     if (vk.iters) vk.iters[c] = it;
+}
+
+// A whole forward smoothing pass in one launch, for levels small enough that
+// the patch solves are latency-bound: the block solves the cells of one tile
+// row (wave 0) together with the cells above them (wave 1) and the cell left
+// of them (wave 2), whose bottom / right facet increments the row's facets
+// also receive, then writes the updated state of the facets and cells the
+// row owns into a second buffer with the arithmetic of k_vanka_apply.  The
+// pass reads the state from a (unchanged during the pass, so no block sees
+// another block's update) and writes u_dst, v_dst, H_dst; the active-set flag
+// is written by the owner in place (no other cell reads it).
+template <typename T, bool LINEAR>
+__global__ __launch_bounds__(3 * TX) void k_vanka_fused(SsaArgs<T> a, T* __restrict__ u_dst, T* __restrict__ v_dst,
+                                                        T* __restrict__ H_dst, T* __restrict__ mask, VankaArgs<T> vk,
+                                                        T omega) {
+    constexpr int W = TX + 3;
+    __shared__ T s_eta[4][W];     // cells (i0 - 2 .. i0 + 1, j0 - 2 .. j0 + TX)
+    __shared__ T s_w1[TX + 1];    // right-facet increments of cells (i0, j0 - 1 + q)
+    __shared__ T s_w3[TX];        // bottom-facet increments of cells (i0 - 1, j0 + q)
+    const int ny = a.ny, nx = a.nx;
+    const int i0 = blockIdx.y, j0 = blockIdx.x * TX;
+    const int wv = threadIdx.y, lane = threadIdx.x, tid = wv * TX + lane;
+    for (int k = tid; k < 4 * W; k += 3 * TX) {
+        const int ci = i0 - 2 + k / W, cj = j0 - 2 + k % W;
+        T e = T(0);
+        if (ci >= 0 && ci < ny && cj >= 0 && cj < nx) e = cell_viscosity(a.u, a.v, a.B, ci, cj, ny, nx, a.dxi, a.P);
+        s_eta[k / W][k % W] = e;
+    }
+    __syncthreads();
+    int ci = i0, cj = j0 + lane;
+    bool active = cj < nx;
+    bool edge = i0 == 0 || i0 == ny - 1 || j0 == 0 || j0 + TX >= nx;
+    if (wv == 1) {
+        ci = i0 - 1;
+        active = ci >= 0 && cj < nx;
+        edge = ci <= 0 || ci == ny - 1 || j0 == 0 || j0 + TX >= nx;
+    } else if (wv == 2) {
+        cj = j0 - 1;
+        active = lane == 0 && cj >= 0;
+        edge = true;
+    }
+    T w[5] = {T(0), T(0), T(0), T(0), T(0)};
+    T masked = (wv == 0 && active) ? mask[(long)i0 * nx + cj] : T(0);
+    int it = 0;
+    if (active) {
+        cell_newton<T, LINEAR, T, W>(a, s_eta, ci - i0 + 2, cj - j0 + 2, ci, cj, vk, edge, w, masked, it);
+        if (wv == 0) s_w1[lane + 1] = w[1];
+        else if (wv == 1) s_w3[lane] = w[3];
+        else s_w1[0] = w[1];
+    }
+    __syncthreads();
+    if (wv != 0 || !active) return;
+    const int i = i0, j = cj;
+    const long c = (long)i * nx + j, ou = (long)i * (nx + 1) + j;
+    {
+        T d = w[0];
+        if (j > 0) d = d + s_w1[lane];
+        u_dst[ou] = a.u[ou] + omega * d;
+        if (j == nx - 1) {
+            const T e = T(0) + w[1];
+            u_dst[ou + 1] = a.u[ou + 1] + omega * e;
+        }
+    }
+    {
+        T d = w[2];
+        if (i > 0) d = d + s_w3[lane];
+        v_dst[c] = a.v[c] + omega * d;
+        if (i == ny - 1) {
+            const T e = T(0) + w[3];
+            v_dst[c + nx] = a.v[c + nx] + omega * e;
+        }
+    }
+    H_dst[c] = a.H[c] + omega * w[4];
+    mask[c] = masked;
 }
 
 // x += sign * omega * (sum of the cell increments touching each unknown).
@@ -1071,6 +1164,16 @@ hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* 
 }
 
 template <typename T>
+hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst, T* mask, const VankaArgs<T>& vk,
+                              T omega, hipStream_t st) {
+    TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx, st);
+    const dim3 g((a.nx + TX - 1) / TX, a.ny), b(TX, 3);
+    if (a.P.linear_drag) k_vanka_fused<T, true><<<g, b, 0, st>>>(a, u_dst, v_dst, H_dst, mask, vk, omega);
+    else k_vanka_fused<T, false><<<g, b, 0, st>>>(a, u_dst, v_dst, H_dst, mask, vk, omega);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st) {
     k_vanka_dump<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, J, r, calving_rate);
     return hipGetLastError();
@@ -1141,6 +1244,7 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_vanka_pass<T>(const SsaArgs<T>&, T*, T*, T*, T*, T*, const VankaArgs<T>&, T, T*, T,   \
                                              hipStream_t);                                                           \
     template hipError_t launch_vanka_dump<T>(const SsaArgs<T>&, T*, T*, T, hipStream_t);                              \
+    template hipError_t launch_vanka_fused<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, hipStream_t);\
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
     template hipError_t launch_vjp<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, const T*, const T*, \

@@ -39,6 +39,15 @@ hipError_t launch_jvp(const SsaArgs<T>& a, const T* du, const T* dv, const T* dH
 template <typename T>
 hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* mask, const VankaArgs<T>& vk, T omega,
                              T* phi_next, T relax_phi, hipStream_t st);
+// One forward smoothing pass in one launch (patch solves of each tile row and
+// of the cells above / left of it, then the update of the owned state): reads
+// the state of a, writes the updated u, v, H to u_dst, v_dst, H_dst (buffers
+// distinct from a's) and the active-set flags to mask.  Uses the plain FP64
+// form; for levels whose patch solves are latency-bound.
+template <typename T>
+hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst, T* mask, const VankaArgs<T>& vk,
+                              T omega, hipStream_t st);
+constexpr long kFusedVankaMaxCells = 128 * 128;
 template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st);
 template <typename T>

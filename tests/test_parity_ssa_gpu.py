@@ -175,6 +175,42 @@ def test_vanka_single_pass_nonlinear_drag_calving(orc, shape):
     assert np.array_equal(N(g.state.mask.data), mask)
 
 
+@pytest.mark.parametrize("shape,n_pass,m", [((32, 32), 6, 1.0), ((40, 70), 5, 1.0), ((33, 129), 3, 1.0 / 3.0),
+                                             ((128, 128), 2, 1.0)])
+def test_vanka_fused_passes_match_two_launch_passes(orc, shape, n_pass, m):
+    """Forward passes of a small level as one launch each (state alternating
+    with the workspace, odd counts copied back) give the state and active set
+    of the solve + apply passes (the same arithmetic; the compiler may contract
+    a product into a different FMA in the other kernel), and match the oracle."""
+    from glide_amd import _lib
+    ny, nx = shape
+    print(f"vanka fused shape={shape} passes={n_pass} m={m:.3f}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=41, mask_frac=0.1)
+    out = []
+    for fused in (True, False):
+        _lib.forward_fused_enable(fused)
+        g = grid_from(F)
+        g.sliding.m.set(m)
+        g.calving.calving_rate.set(0.2)
+        g.forward_operators.vanka_sweep(10.0, n_pass, freeze_phi=True)
+        torch.cuda.synchronize()
+        out.append([N(t.data) for t in (g.state.u, g.state.v, g.state.H, g.state.mask)])
+    _lib.forward_fused_enable(True)
+    for x, y in zip(out[0][:3], out[1][:3]):
+        print(f"  fused vs two-launch max rel diff {rel_err(x, y):.3e}", flush=True)
+        assert rel_err(x, y) < 1e-13
+    assert np.array_equal(out[0][3], out[1][3])
+    if n_pass <= 3:
+        P = orc.phys(m=m, calving_rate=0.2)
+        Fo = dict(F)
+        for _ in range(n_pass):
+            u, v, H, mask, _ = orc.vanka_sweep(Fo, P, 1500.0, 10.0)
+            Fo.update(u=u, v=v, H=H, mask=mask)
+        for x, y in zip(out[0][:3], (Fo["u"], Fo["v"], Fo["H"])):
+            assert rel_err(x, y) < 1e-11
+        assert np.array_equal(out[0][3], Fo["mask"])
+
+
 def test_vanka_passes_with_phi_refresh(orc):
     F = dome_fields(n=32, seed=6)
     g = grid_from(F)
