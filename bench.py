@@ -78,7 +78,7 @@ def build_model(args, torch):
     install(model.mg, p)
     for s in (model.forward_solver, model.adjoint_solver):
         s.fas_options.set(maximum_vcycles=args.vcycles, relative_tolerance=0.0, absolute_tolerance=0.0,
-                          report_norms=False)
+                          report_norms=False, asynchronous=True)
     thermal = None
     if args.nz > 0:
         thermal = ThermalModel(model.mg[0], nz=args.nz, n_smooth=args.n_smooth, strain_heating=True)
@@ -172,7 +172,7 @@ class Stepper:
         # a report synchronises the host, so only the sequential steps ask for one
         res = th.ops.adjoint_solve(dt * SEC_PER_YR, dL_dE=th.ops.enthalpy_state.E, dL_dB=self.dL_dB,
                                    B_scale=th.B_scale, report=report, relative_tolerance=self.adj_rtol,
-                                   n_iter=256)
+                                   n_iter=64)
         if report:
             self.adjoint_sweeps = res["report"].sweeps
 

@@ -570,6 +570,7 @@ int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, 
         it = X.cfg.maximum_vcycles;
         if (g_graphs_on && !timing_on()) GLIDE_TRY(replay(solve_key(tag, X, start), st, body));
         else GLIDE_TRY(body(st));
+        if (!rep) return GLIDE_OK;  // nothing to report: no host synchronisation
         GLIDE_TRY(read_slots(it + 1));
         r0 = std::sqrt(s->host[0] + s->host[1] + s->host[2]);
         absn = r0;

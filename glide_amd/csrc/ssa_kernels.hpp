@@ -47,7 +47,7 @@ hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* 
 template <typename T>
 hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst, T* mask, const VankaArgs<T>& vk,
                               T omega, hipStream_t st);
-constexpr long kFusedVankaMaxCells = 128 * 128;
+constexpr long kFusedVankaMaxCells = 48 * 48;
 template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st);
 template <typename T>

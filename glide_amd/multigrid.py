@@ -306,6 +306,9 @@ class FASCDConfig:
     absolute_tolerance: float = 5.0
     report_norms: bool = True
     compat_vprolong: bool = True
+    # (an addition) fixed-cycle solves (both tolerances <= 0) return at once,
+    # without waiting for the GPU or filling a report; solve() then returns None
+    asynchronous: bool = False
 
     def to_c(self) -> _lib.FasConfigC:
         return _lib.FasConfigC(
@@ -328,6 +331,9 @@ class FASAdjointConfig:
     absolute_tolerance: float = 5.0
     report_norms: bool = True
     compat_vprolong: bool = True
+    # (an addition) fixed-cycle solves (both tolerances <= 0) return at once,
+    # without waiting for the GPU or filling a report; solve() then returns None
+    asynchronous: bool = False
 
     def to_c(self) -> _lib.FasConfigC:
         return _lib.FasConfigC(
@@ -335,6 +341,12 @@ class FASAdjointConfig:
             pre_steps=int(self.pre_steps), post_steps=int(self.post_steps), finest_steps=int(self.finest_steps),
             maximum_vcycles=int(self.maximum_vcycles), relative_tolerance=float(self.relative_tolerance),
             absolute_tolerance=float(self.absolute_tolerance), compat_vprolong=int(bool(self.compat_vprolong)))
+
+
+def _no_report(cfg, report_norms) -> bool:
+    """An asynchronous fixed-cycle solve that prints nothing needs no report."""
+    fixed = not (cfg.relative_tolerance > 0.0) and not (cfg.absolute_tolerance > 0.0)
+    return bool(cfg.asynchronous) and fixed and not (cfg.report_norms and report_norms)
 
 
 def dataclass_field_names(cls):
@@ -510,12 +522,12 @@ class FASCDSolver:
         self._check_square(start_level)
         arr, vcs = self._structs()
         cfg = self._fas_config.to_c()
-        rep = _lib.SolveReport()
+        rep = None if _no_report(self._fas_config, report_norms) else _lib.SolveReport()
         st = stream_handle(self.multigrid.device)
         call("glide_fascd_solve", arr, self.n_levels, int(start_level), self.dt, ctypes.byref(cfg), vcs,
-             ctypes.byref(rep), st)
+             ctypes.byref(rep) if rep is not None else None, st)
         self.last_report = rep
-        if self._fas_config.report_norms and report_norms:
+        if rep is not None and self._fas_config.report_norms and report_norms:
             _print_report(rep, list(rep.norms3))
         return rep
 
@@ -561,11 +573,11 @@ class FASAdjointSolver:
                     raise ValueError("compat_vprolong needs square levels; set fas_options compat_vprolong=False")
         arr, vcs = self._structs()
         cfg = self._fas_config.to_c()
-        rep = _lib.SolveReport()
+        rep = None if _no_report(self._fas_config, report_norms) else _lib.SolveReport()
         call("glide_fas_adjoint_solve", arr, self.n_levels, int(start_level), self.dt, ctypes.byref(cfg), vcs,
-             ctypes.byref(rep), stream_handle(self.multigrid.device))
+             ctypes.byref(rep) if rep is not None else None, stream_handle(self.multigrid.device))
         self.last_report = rep
-        if self._fas_config.report_norms and report_norms:
+        if rep is not None and self._fas_config.report_norms and report_norms:
             _print_report(rep, list(rep.norms3))
         return rep
 

@@ -161,7 +161,8 @@ int glide_prolong_cell(const void* coarse, void* fine, int ny_fine, int nx_fine,
 /* Full FAS-CD solve on levels[start_level..n_levels-1]; vc holds one config
  * per level.  Synchronises the stream once per V-cycle for the stopping test
  * unless both tolerances are <= 0 (then it runs exactly maximum_vcycles
- * V-cycles and synchronises once at the end). */
+ * V-cycles and synchronises once at the end to fill the report, or not at
+ * all when report is NULL; glide_fas_adjoint_solve alike). */
 int glide_fascd_solve(const glide_level* levels, int n_levels, int start_level, double dt,
                       const glide_fas_config* cfg, const glide_vanka_config* vc, glide_solve_report* report,
                       void* stream);
@@ -200,7 +201,7 @@ int glide_graphs_clear(void);
  * in-place multiplier update); 0 solves into the workspace and applies in a
  * second launch.  Both give the same multipliers. */
 int glide_adjoint_fused_enable(int on);
-/* Forward smoothing of levels of at most 128 x 128 cells with a frozen
+/* Forward smoothing of levels of at most 48 x 48 cells with a frozen
  * grounded fraction (no counterpart in the reference): 1 (default) runs each
  * pass as one launch (the patch solves of each tile row and of the cells
  * above and left of it, then the owned state written to the other of two
