@@ -263,14 +263,6 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
     return GLIDE_OK;
 }
 
-int restrict_u(const void* f, void* c, const glide_level& C, hipStream_t st) {
-    HIP_TRY(launch_restrict_vfacet<R>(CP(f), P(c), C.ny, C.nx, st), "restrict u");
-    return GLIDE_OK;
-}
-int restrict_v(const void* f, void* c, const glide_level& C, hipStream_t st) {
-    HIP_TRY(launch_restrict_hfacet<R>(CP(f), P(c), C.ny, C.nx, st), "restrict v");
-    return GLIDE_OK;
-}
 int restrict_c(const void* f, void* c, const glide_level& C, int use_max, hipStream_t st) {
     HIP_TRY(launch_restrict_cell<R>(CP(f), P(c), C.ny, C.nx, use_max, st), "restrict cell");
     return GLIDE_OK;
@@ -289,16 +281,6 @@ int fill(void* dst, double val, size_t n, hipStream_t st) {
 }
 
 // Restrict the forward state of level l to level l+1 (glide/multigrid.py:81-87).
-int restrict_state(const glide_level& F, const glide_level& C, hipStream_t st) {
-    GLIDE_TRY(restrict_u(F.u, C.u, C, st));
-    GLIDE_TRY(restrict_v(F.v, C.v, C, st));
-    GLIDE_TRY(restrict_c(F.H, C.H, C, 0, st));
-    GLIDE_TRY(restrict_c(F.H_prev, C.H_prev, C, 0, st));
-    GLIDE_TRY(restrict_c(F.phi, C.phi, C, 0, st));
-    GLIDE_TRY(restrict_c(F.mask, C.mask, C, 1, st));
-    return GLIDE_OK;
-}
-
 int prolong_v(const void* c, void* f, const glide_level& F, int compat, hipStream_t st) {
     if (compat)
         HIP_TRY(launch_prolong_vfacet_bilinear<R>(CP(c), P(f), F.ny, F.nx, st), "prolong v (compat)");
@@ -325,14 +307,23 @@ int fas_vcycle(FasCtx& X, int l, bool finest) {
     if (coarse && X.cfg.freeze_coarse_phi) sflags |= GLIDE_FREEZE_PHI;
     const double thk = L.phys.thklim;
     hipStream_t st = X.st;
+    // independent element-wise jobs of a stage run as one batched launch
+    auto run = [&](const EwBatch<R>& b) -> int {
+        HIP_TRY(launch_ew_batch<R>(b, st), "element-wise batch");
+        return GLIDE_OK;
+    };
     if (finest) {
-        GLIDE_TRY(copy(L.w_u, L.u, nU(L), st));
-        GLIDE_TRY(copy(L.w_v, L.v, nV(L), st));
-        GLIDE_TRY(copy(L.w_H, L.H, nC(L), st));
-        // chi = thklim - H
-        HIP_TRY(launch_axpby<R>(P(L.chi), -1.0, CP(L.H), 0.0, CP(L.H), 0.0, nullptr, (long)nC(L), st), "chi");
-        HIP_TRY(launch_fill<R>(P(L.gamma), thk, (long)nC(L), st), "fill");
-        GLIDE_TRY(axpby(L.chi, 1.0, L.chi, 1.0, L.gamma, nC(L), st));
+        // w = x; chi = -H (then + thklim)
+        EwBatch<R> b;
+        b.copy(P(L.w_u), CP(L.u), (long)nU(L));
+        b.copy(P(L.w_v), CP(L.v), (long)nV(L));
+        b.copy(P(L.w_H), CP(L.H), (long)nC(L));
+        b.axpby(P(L.chi), -1.0, CP(L.H), 0.0, CP(L.H), (long)nC(L));
+        b.fill(P(L.gamma), thk, (long)nC(L));
+        GLIDE_TRY(run(b));
+        EwBatch<R> c;
+        c.axpby(P(L.chi), 1.0, CP(L.chi), 1.0, CP(L.gamma), (long)nC(L));
+        GLIDE_TRY(run(c));
     }
     if (l == X.n_levels - 1) {
         GLIDE_TRY(axpby(L.gamma, 1.0, L.w_H, 1.0, L.chi, nC(L), st));
@@ -347,39 +338,71 @@ int fas_vcycle(FasCtx& X, int l, bool finest) {
     GLIDE_TRY(axpby(L.cphi, -1.0, L.cphi, 1.0, L.chi, nC(L), st));
     GLIDE_TRY(axpby(L.gamma, 1.0, L.w_H, 1.0, L.cphi, nC(L), st));
     GLIDE_TRY(vanka_sweeps(L, X.dt, X.cfg.pre_steps, X.vc[l], sflags, st));
-    GLIDE_TRY(fill(L.gamma, thk, nC(L), st));
-    GLIDE_TRY(axpby(L.y_u, 1.0, L.u, -1.0, L.w_u, nU(L), st));
-    GLIDE_TRY(axpby(L.y_v, 1.0, L.v, -1.0, L.w_v, nV(L), st));
-    GLIDE_TRY(axpby(L.y_H, 1.0, L.H, -1.0, L.w_H, nC(L), st));
-    GLIDE_TRY(restrict_state(L, N, st));
-    GLIDE_TRY(copy(N.w_u, N.u, nU(N), st));
-    GLIDE_TRY(copy(N.w_v, N.v, nV(N), st));
-    GLIDE_TRY(copy(N.w_H, N.H, nC(N), st));
+    {
+        // gamma back to thklim; y = x - w
+        EwBatch<R> b;
+        b.fill(P(L.gamma), thk, (long)nC(L));
+        b.axpby(P(L.y_u), 1.0, CP(L.u), -1.0, CP(L.w_u), (long)nU(L));
+        b.axpby(P(L.y_v), 1.0, CP(L.v), -1.0, CP(L.w_v), (long)nV(L));
+        b.axpby(P(L.y_H), 1.0, CP(L.H), -1.0, CP(L.w_H), (long)nC(L));
+        GLIDE_TRY(run(b));
+    }
+    {
+        // restrict the state
+        RestrictBatch<R> rb;
+        rb.add(CP(L.u), P(N.u), kRestrictU);
+        rb.add(CP(L.v), P(N.v), kRestrictV);
+        rb.add(CP(L.H), P(N.H), kRestrictCellMean);
+        rb.add(CP(L.H_prev), P(N.H_prev), kRestrictCellMean);
+        rb.add(CP(L.phi), P(N.phi), kRestrictCellMean);
+        rb.add(CP(L.mask), P(N.mask), kRestrictCellMax);
+        HIP_TRY(launch_restrict_batch<R>(rb, N.ny, N.nx, st), "restrict state");
+    }
+    {
+        EwBatch<R> b;
+        b.copy(P(N.w_u), CP(N.u), (long)nU(N));
+        b.copy(P(N.w_v), CP(N.v), (long)nV(N));
+        b.copy(P(N.w_H), CP(N.H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
     GLIDE_TRY(residual(L, X.dt, GLIDE_USE_FORCING | (sflags & GLIDE_FREEZE_CALVING), P(L.r_u), P(L.r_v), P(L.r_H), st));
-    GLIDE_TRY(restrict_u(L.r_u, N.r_u, N, st));
-    GLIDE_TRY(restrict_v(L.r_v, N.r_v, N, st));
-    GLIDE_TRY(restrict_c(L.r_H, N.r_H, N, 0, st));
+    {
+        RestrictBatch<R> rb;
+        rb.add(CP(L.r_u), P(N.r_u), kRestrictU);
+        rb.add(CP(L.r_v), P(N.r_v), kRestrictV);
+        rb.add(CP(L.r_H), P(N.r_H), kRestrictCellMean);
+        HIP_TRY(launch_restrict_batch<R>(rb, N.ny, N.nx, st), "restrict residual");
+    }
     if (!X.cfg.freeze_coarse_phi) GLIDE_TRY(grounded(N, 0.0, st));
     GLIDE_TRY(residual(N, X.dt, X.cfg.freeze_coarse_calving ? GLIDE_FREEZE_CALVING : 0, P(N.F_u), P(N.F_v), P(N.F_H),
                        st));
-    GLIDE_TRY(axpby(N.f_u, 1.0, N.F_u, -1.0, N.r_u, nU(N), st));
-    GLIDE_TRY(axpby(N.f_v, 1.0, N.F_v, -1.0, N.r_v, nV(N), st));
-    GLIDE_TRY(axpby(N.f_H, 1.0, N.F_H, -1.0, N.r_H, nC(N), st));
+    {
+        EwBatch<R> b;
+        b.axpby(P(N.f_u), 1.0, CP(N.F_u), -1.0, CP(N.r_u), (long)nU(N));
+        b.axpby(P(N.f_v), 1.0, CP(N.F_v), -1.0, CP(N.r_v), (long)nV(N));
+        b.axpby(P(N.f_H), 1.0, CP(N.F_H), -1.0, CP(N.r_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
     GLIDE_TRY(fas_vcycle(X, l + 1, false));
-    GLIDE_TRY(axpby(N.z_u, 1.0, N.u, -1.0, N.w_u, nU(N), st));
-    GLIDE_TRY(axpby(N.z_v, 1.0, N.v, -1.0, N.w_v, nV(N), st));
-    GLIDE_TRY(axpby(N.z_H, 1.0, N.H, -1.0, N.w_H, nC(N), st));
+    {
+        EwBatch<R> b;
+        b.axpby(P(N.z_u), 1.0, CP(N.u), -1.0, CP(N.w_u), (long)nU(N));
+        b.axpby(P(N.z_v), 1.0, CP(N.v), -1.0, CP(N.w_v), (long)nV(N));
+        b.axpby(P(N.z_H), 1.0, CP(N.H), -1.0, CP(N.w_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
     HIP_TRY(launch_prolong_vfacet_bilinear<R>(CP(N.z_u), P(L.z_u), L.ny, L.nx, st), "prolong z_u");
     GLIDE_TRY(prolong_v(N.z_v, L.z_v, L, X.cfg.compat_vprolong, st));
     HIP_TRY(launch_prolong_cell<R>(CP(N.z_H), P(L.z_H), L.ny, L.nx, 0, st), "prolong z_H");
-    // z += y; x = w + z
-    GLIDE_TRY(axpby(L.z_u, 1.0, L.z_u, 1.0, L.y_u, nU(L), st));
-    GLIDE_TRY(axpby(L.z_v, 1.0, L.z_v, 1.0, L.y_v, nV(L), st));
-    GLIDE_TRY(axpby(L.z_H, 1.0, L.z_H, 1.0, L.y_H, nC(L), st));
-    GLIDE_TRY(axpby(L.u, 1.0, L.w_u, 1.0, L.z_u, nU(L), st));
-    GLIDE_TRY(axpby(L.v, 1.0, L.w_v, 1.0, L.z_v, nV(L), st));
-    GLIDE_TRY(axpby(L.H, 1.0, L.w_H, 1.0, L.z_H, nC(L), st));
-    GLIDE_TRY(axpby(L.gamma, 1.0, L.w_H, 1.0, L.chi, nC(L), st));
+    {
+        // z += y; x = w + z; gamma = w_H + chi
+        EwBatch<R> b;
+        b.correct(P(L.z_u), CP(L.y_u), P(L.u), CP(L.w_u), (long)nU(L));
+        b.correct(P(L.z_v), CP(L.y_v), P(L.v), CP(L.w_v), (long)nV(L));
+        b.correct(P(L.z_H), CP(L.y_H), P(L.H), CP(L.w_H), (long)nC(L));
+        b.axpby(P(L.gamma), 1.0, CP(L.w_H), 1.0, CP(L.chi), (long)nC(L));
+        GLIDE_TRY(run(b));
+    }
     GLIDE_TRY(vanka_sweeps(L, X.dt, X.cfg.post_steps, X.vc[l], sflags, st));
     GLIDE_TRY(fill(L.gamma, thk, nC(L), st));
     if (finest) GLIDE_TRY(vanka_sweeps(L, X.dt, X.cfg.finest_steps, X.vc[l], 0, st));
@@ -395,34 +418,68 @@ int adj_vcycle(FasCtx& X, int l, bool finest) {
     if (l == X.n_levels - 1) return vanka_adjoint_sweeps(L, X.dt, X.cfg.coarsest_steps, X.vc[l], 0, st, true);
     const glide_level& N = X.L[l + 1];
     GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.pre_steps, X.vc[l], 0, st, true));
-    GLIDE_TRY(restrict_state(L, N, st));
+    auto run = [&](const EwBatch<R>& b) -> int {
+        HIP_TRY(launch_ew_batch<R>(b, st), "element-wise batch");
+        return GLIDE_OK;
+    };
+    {
+        // restrict the state and the multipliers
+        RestrictBatch<R> rb;
+        rb.add(CP(L.u), P(N.u), kRestrictU);
+        rb.add(CP(L.v), P(N.v), kRestrictV);
+        rb.add(CP(L.H), P(N.H), kRestrictCellMean);
+        rb.add(CP(L.H_prev), P(N.H_prev), kRestrictCellMean);
+        rb.add(CP(L.phi), P(N.phi), kRestrictCellMean);
+        rb.add(CP(L.mask), P(N.mask), kRestrictCellMax);
+        rb.add(CP(L.lambda_u), P(N.lambda_u), kRestrictU);
+        rb.add(CP(L.lambda_v), P(N.lambda_v), kRestrictV);
+        rb.add(CP(L.lambda_H), P(N.lambda_H), kRestrictCellMean);
+        HIP_TRY(launch_restrict_batch<R>(rb, N.ny, N.nx, st), "restrict state and multipliers");
+    }
     GLIDE_TRY(adjoint_prepare(N, X.dt, X.vc[l + 1], 0, st));
-    GLIDE_TRY(restrict_u(L.lambda_u, N.lambda_u, N, st));
-    GLIDE_TRY(restrict_v(L.lambda_v, N.lambda_v, N, st));
-    GLIDE_TRY(restrict_c(L.lambda_H, N.lambda_H, N, 0, st));
-    GLIDE_TRY(copy(N.wl_u, N.lambda_u, nU(N), st));
-    GLIDE_TRY(copy(N.wl_v, N.lambda_v, nV(N), st));
-    GLIDE_TRY(copy(N.wl_H, N.lambda_H, nC(N), st));
+    {
+        EwBatch<R> b;
+        b.copy(P(N.wl_u), CP(N.lambda_u), (long)nU(N));
+        b.copy(P(N.wl_v), CP(N.lambda_v), (long)nV(N));
+        b.copy(P(N.wl_H), CP(N.lambda_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
     GLIDE_TRY(vjp(L, X.dt, GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u), P(L.ar_v),
                   P(L.ar_H), st, true));
-    GLIDE_TRY(restrict_u(L.ar_u, N.ar_u, N, st));
-    GLIDE_TRY(restrict_v(L.ar_v, N.ar_v, N, st));
-    GLIDE_TRY(restrict_c(L.ar_H, N.ar_H, N, 0, st));
+    {
+        RestrictBatch<R> rb;
+        rb.add(CP(L.ar_u), P(N.ar_u), kRestrictU);
+        rb.add(CP(L.ar_v), P(N.ar_v), kRestrictV);
+        rb.add(CP(L.ar_H), P(N.ar_H), kRestrictCellMean);
+        HIP_TRY(launch_restrict_batch<R>(rb, N.ny, N.nx, st), "restrict adjoint residual");
+    }
     GLIDE_TRY(vjp(N, X.dt, 0, CP(N.lambda_u), CP(N.lambda_v), CP(N.lambda_H), P(N.vjp_u), P(N.vjp_v), P(N.vjp_H), st,
                   true));
-    GLIDE_TRY(axpby(N.af_u, 1.0, N.vjp_u, -1.0, N.ar_u, nU(N), st));
-    GLIDE_TRY(axpby(N.af_v, 1.0, N.vjp_v, -1.0, N.ar_v, nV(N), st));
-    GLIDE_TRY(axpby(N.af_H, 1.0, N.vjp_H, -1.0, N.ar_H, nC(N), st));
+    {
+        EwBatch<R> b;
+        b.axpby(P(N.af_u), 1.0, CP(N.vjp_u), -1.0, CP(N.ar_u), (long)nU(N));
+        b.axpby(P(N.af_v), 1.0, CP(N.vjp_v), -1.0, CP(N.ar_v), (long)nV(N));
+        b.axpby(P(N.af_H), 1.0, CP(N.vjp_H), -1.0, CP(N.ar_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
     GLIDE_TRY(adj_vcycle(X, l + 1, false));
-    GLIDE_TRY(axpby(N.zl_u, 1.0, N.lambda_u, -1.0, N.wl_u, nU(N), st));
-    GLIDE_TRY(axpby(N.zl_v, 1.0, N.lambda_v, -1.0, N.wl_v, nV(N), st));
-    GLIDE_TRY(axpby(N.zl_H, 1.0, N.lambda_H, -1.0, N.wl_H, nC(N), st));
+    {
+        EwBatch<R> b;
+        b.axpby(P(N.zl_u), 1.0, CP(N.lambda_u), -1.0, CP(N.wl_u), (long)nU(N));
+        b.axpby(P(N.zl_v), 1.0, CP(N.lambda_v), -1.0, CP(N.wl_v), (long)nV(N));
+        b.axpby(P(N.zl_H), 1.0, CP(N.lambda_H), -1.0, CP(N.wl_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
     HIP_TRY(launch_prolong_vfacet_bilinear<R>(CP(N.zl_u), P(L.zl_u), L.ny, L.nx, st), "prolong zl_u");
     GLIDE_TRY(prolong_v(N.zl_v, L.zl_v, L, X.cfg.compat_vprolong, st));
     HIP_TRY(launch_prolong_cell<R>(CP(N.zl_H), P(L.zl_H), L.ny, L.nx, 1, st), "prolong zl_H");
-    GLIDE_TRY(axpby(L.lambda_u, 1.0, L.lambda_u, 1.0, L.zl_u, nU(L), st));
-    GLIDE_TRY(axpby(L.lambda_v, 1.0, L.lambda_v, 1.0, L.zl_v, nV(L), st));
-    GLIDE_TRY(axpby(L.lambda_H, 1.0, L.lambda_H, 1.0, L.zl_H, nC(L), st));
+    {
+        EwBatch<R> b;
+        b.axpby(P(L.lambda_u), 1.0, CP(L.lambda_u), 1.0, CP(L.zl_u), (long)nU(L));
+        b.axpby(P(L.lambda_v), 1.0, CP(L.lambda_v), 1.0, CP(L.zl_v), (long)nV(L));
+        b.axpby(P(L.lambda_H), 1.0, CP(L.lambda_H), 1.0, CP(L.zl_H), (long)nC(L));
+        GLIDE_TRY(run(b));
+    }
     GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.post_steps, X.vc[l], 0, st, true));
     if (finest) GLIDE_TRY(vanka_adjoint_sweeps(L, X.dt, X.cfg.finest_steps, X.vc[l], 0, st, true));
     return GLIDE_OK;

@@ -87,6 +87,51 @@ hipError_t launch_prolong_hfacet_bilinear(const T* c, T* f, int nyf, int nxf, hi
 template <typename T>
 hipError_t launch_prolong_cell(const T* c, T* f, int nyf, int nxf, int bilinear, hipStream_t st);
 
+// Batched element-wise jobs (one launch; the jobs must not read each other's
+// outputs): axpby out = a x + b y (+ c z), copy, fill out = a, and the
+// correction step out = x + y (z += y in place when out == x), out2 = z + out.
+enum EwOp { kEwAxpby = 0, kEwCopy = 1, kEwFill = 2, kEwCorrect = 3 };
+template <typename T>
+struct EwJob {
+    T* out;
+    T* out2;
+    const T *x, *y, *z;
+    T a, b, c;
+    long n;
+    int op;
+};
+constexpr int kEwMaxJobs = 8;
+template <typename T>
+struct EwBatch {
+    EwJob<T> job[kEwMaxJobs];
+    int count = 0;
+    void axpby(T* o, T a_, const T* x_, T b_, const T* y_, long n_) { job[count++] = {o, nullptr, x_, y_, nullptr, a_, b_, T(0), n_, kEwAxpby}; }
+    void copy(T* o, const T* x_, long n_) { job[count++] = {o, nullptr, x_, nullptr, nullptr, T(0), T(0), T(0), n_, kEwCopy}; }
+    void fill(T* o, T v, long n_) { job[count++] = {o, nullptr, nullptr, nullptr, nullptr, v, T(0), T(0), n_, kEwFill}; }
+    // z = z + y; x = w + z
+    void correct(T* z, const T* y_, T* x_, const T* w, long n_) { job[count++] = {z, x_, z, y_, w, T(1), T(1), T(1), n_, kEwCorrect}; }
+};
+template <typename T>
+hipError_t launch_ew_batch(const EwBatch<T>& b, hipStream_t st);
+
+// Batched restrictions onto one coarse level (nyc, nxc).
+enum RestrictKind { kRestrictU = 0, kRestrictV = 1, kRestrictCellMean = 2, kRestrictCellMax = 3 };
+template <typename T>
+struct RestrictJob {
+    const T* f;
+    T* c;
+    int kind;
+};
+constexpr int kRestrictMaxJobs = 12;
+template <typename T>
+struct RestrictBatch {
+    RestrictJob<T> job[kRestrictMaxJobs];
+    int count = 0;
+    void add(const T* f, T* c, int kind) { job[count++] = {f, c, kind}; }
+};
+template <typename T>
+hipError_t launch_restrict_batch(const RestrictBatch<T>& b, int nyc, int nxc, hipStream_t st);
+
 // out = a*x + b*y (+ c*z when z != nullptr), element-wise over n values.
 template <typename T>
 hipError_t launch_axpby(T* out, T a, const T* x, T b, const T* y, T c, const T* z, long n, hipStream_t st);

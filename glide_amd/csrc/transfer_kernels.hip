@@ -122,6 +122,57 @@ __global__ __launch_bounds__(256) void k_fill(T* __restrict__ out, T val, long n
     if (k < n) out[k] = val;
 }
 
+// Batched element-wise jobs of one V-cycle stage (independent of each other)
+// in one launch: blockIdx.y selects the job, a grid-stride loop covers it.
+// Each job keeps the arithmetic of its single-array kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ew_batch(EwBatch<T> b) {
+    const EwJob<T>& q = b.job[blockIdx.y];
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < q.n; k += (long)gridDim.x * blockDim.x) {
+        switch (q.op) {
+            case kEwAxpby: {
+                T r = q.a * q.x[k] + q.b * q.y[k];
+                if (q.z) r += q.c * q.z[k];
+                q.out[k] = r;
+                break;
+            }
+            case kEwCopy: q.out[k] = q.x[k]; break;
+            case kEwFill: q.out[k] = q.a; break;
+            default: {  // kEwCorrect: z = z + y, then x = w + z
+                const T t = q.x[k] + q.y[k];
+                q.out[k] = t;
+                q.out2[k] = q.z[k] + t;
+                break;
+            }
+        }
+    }
+}
+
+// Batched restrictions onto one coarse level (nyc, nxc): blockIdx.z selects
+// the job; the arithmetic of k_restrict_vfacet / _hfacet / _cell.
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict_batch(RestrictBatch<T> b, int nyc, int nxc) {
+    const RestrictJob<T>& q = b.job[blockIdx.z];
+    const int I = blockIdx.x * blockDim.x + threadIdx.x, J = blockIdx.y;
+    const T* __restrict__ f = q.f;
+    T* __restrict__ c = q.c;
+    if (q.kind == kRestrictU) {
+        if (I > nxc || J >= nyc) return;
+        const long sf = 2 * nxc + 1;
+        c[(long)J * (nxc + 1) + I] = T(0.5) * (f[2L * J * sf + 2 * I] + f[(2L * J + 1) * sf + 2 * I]);
+    } else if (q.kind == kRestrictV) {
+        if (I >= nxc || J > nyc) return;
+        const long sf = 2 * nxc;
+        c[(long)J * nxc + I] = T(0.5) * (f[2L * J * sf + 2 * I] + f[2L * J * sf + 2 * I + 1]);
+    } else {
+        if (I >= nxc || J >= nyc) return;
+        const long sf = 2 * nxc;
+        T a = f[2L * J * sf + 2 * I], bb = f[2L * J * sf + 2 * I + 1];
+        T d = f[(2L * J + 1) * sf + 2 * I], e = f[(2L * J + 1) * sf + 2 * I + 1];
+        c[(long)J * nxc + I] = q.kind == kRestrictCellMax ? fmax(fmax(a, bb), fmax(d, e)) : T(0.25) * (a + bb + d + e);
+    }
+}
+
 __device__ __forceinline__ double block_sum_256(double s, double* sh) {
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -221,6 +272,24 @@ hipError_t launch_sumsq3(const T* a, long na, const T* b, long nb, const T* c, l
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_ew_batch(const EwBatch<T>& b, hipStream_t st) {
+    if (b.count <= 0) return hipSuccess;
+    long nmax = 0;
+    for (int q = 0; q < b.count; ++q) nmax = b.job[q].n > nmax ? b.job[q].n : nmax;
+    if (nmax <= 0) return hipSuccess;
+    long gx = (nmax + 255) / 256;
+    if (gx > 2048) gx = 2048;
+    k_ew_batch<T><<<dim3((unsigned)gx, b.count), 256, 0, st>>>(b);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_restrict_batch(const RestrictBatch<T>& b, int nyc, int nxc, hipStream_t st) {
+    if (b.count <= 0) return hipSuccess;
+    k_restrict_batch<T><<<dim3((nxc + 1 + 255) / 256, nyc + 1, b.count), 256, 0, st>>>(b, nyc, nxc);
+    return hipGetLastError();
+}
+
 #define GLIDE_TRANSFER_INSTANTIATE(T)                                                                     \
     template hipError_t launch_restrict_vfacet<T>(const T*, T*, int, int, hipStream_t);                    \
     template hipError_t launch_restrict_hfacet<T>(const T*, T*, int, int, hipStream_t);                    \
@@ -232,7 +301,9 @@ hipError_t launch_sumsq3(const T* a, long na, const T* b, long nb, const T* c, l
     template hipError_t launch_fill<T>(T*, T, long, hipStream_t);                                          \
     template hipError_t launch_copy<T>(T*, const T*, long, hipStream_t);                                   \
     template hipError_t launch_sumsq3<T>(const T*, long, const T*, long, const T*, long, double*, double*, \
-                                         hipStream_t);
+                                         hipStream_t);                                                     \
+    template hipError_t launch_ew_batch<T>(const EwBatch<T>&, hipStream_t);                                \
+    template hipError_t launch_restrict_batch<T>(const RestrictBatch<T>&, int, int, hipStream_t);
 
 GLIDE_TRANSFER_INSTANTIATE(double)
 
