@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--n-smooth", type=int, default=10, help="maximum enthalpy sweeps per step")
     ap.add_argument("--enth-adjoint-rtol", type=float, default=1e-8,
                     help="stopping test of the enthalpy adjoint sweeps (relative to the first residual)")
+    ap.add_argument("--serial-enthalpy", dest="concurrent", action="store_false",
+                    help="run the enthalpy step and its adjoint on the momentum stream (no overlap)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false",
                     help="skip the mixed-precision run (FP32 patch Jacobian) and its error against FP64")
@@ -93,12 +95,12 @@ def run_timed(stepper, args, torch):
     own stream); returns (t, seconds)."""
     t = 0.0
     for _ in range(args.warmup):
-        stepper(t, args.dt)
+        stepper(t, args.dt, concurrent=args.concurrent)
         t += args.dt
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stepper(t, args.dt)
+        stepper(t, args.dt, concurrent=args.concurrent)
         t += args.dt
     torch.cuda.synchronize()
     return t, time.perf_counter() - t0
@@ -170,13 +172,14 @@ class Stepper:
         th.step(dt * SEC_PER_YR, B_out=self.B_next)
         # objective 0.5 |E|^2 + sum(B): the seeds only set the right-hand side;
         # a report synchronises the host, so only the sequential steps ask for one
+        g = self.model.mg[0]
         res = th.ops.adjoint_solve(dt * SEC_PER_YR, dL_dE=th.ops.enthalpy_state.E, dL_dB=self.dL_dB,
                                    B_scale=th.B_scale, report=report, relative_tolerance=self.adj_rtol,
-                                   n_iter=64)
+                                   n_iter=64, u2d=g.state.u.data, v2d=g.state.v.data, vel_div=SEC_PER_YR)
         if report:
             self.adjoint_sweeps = res["report"].sweeps
 
-    def __call__(self, t, dt, concurrent=True):
+    def __call__(self, t, dt, concurrent=True, report=False):
         torch, model, thermal = self.torch, self.model, self.thermal
         if thermal is not None:
             thermal.pre_momentum()
@@ -188,10 +191,10 @@ class Stepper:
                 self.ev_fwd.record(main)
                 self.stream.wait_event(self.ev_fwd)
                 with torch.cuda.stream(self.stream):
-                    self._thermal(dt, False)
+                    self._thermal(dt, report)
                     self.ev_thermal.record(self.stream)
             else:
-                self._thermal(dt, True)
+                self._thermal(dt, report)
         model.backward(t, dt, dJdu=g.state.u.data, dJdv=g.state.v.data, dJdH=None)
         if thermal is not None:
             if concurrent:
@@ -306,7 +309,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        stepper(t, args.dt, concurrent=False)
+        stepper(t, args.dt, concurrent=False, report=True)
         t += args.dt
     torch.cuda.synchronize()
     elapsed_eager = time.perf_counter() - t1

@@ -92,11 +92,11 @@ class EnthReport(ctypes.Structure):
 
 
 ENTH_ADJOINT_POINTERS = ("rows", "g", "lambda_", "lambda_work", "r", "g_E_prev", "g_phi_strain", "g_Q_geo",
-                         "g_E_surface")
+                         "g_E_surface", "u2d", "v2d")
 
 
 class EnthAdjointC(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_void_p) for n in ENTH_ADJOINT_POINTERS]
+    _fields_ = [(n, ctypes.c_void_p) for n in ENTH_ADJOINT_POINTERS] + [("vel_div", ctypes.c_double)]
 
 
 class ThermalConfigC(ctypes.Structure):

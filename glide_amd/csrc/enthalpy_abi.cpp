@@ -354,26 +354,37 @@ int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoin
     REQ(X->g);
     REQ(X->lambda);
     REQ(X->lambda_work);
-    REQ(X->r);
     if (dL_dB && (!(n_glen > 0.0) || B_scale == 0.0))
         return fail(GLIDE_ERR_INVALID_ARG, "n_glen must be positive and B_scale non-zero");
     EnthScratch* s = nullptr;
     GLIDE_TRY(scratch(&s));
     hipStream_t st = S(stream);
     const int ny = C->ny, nx = C->nx, nz = C->nz;
-    HIP_TRY(launch_enth_adj_rows<R>(A, P(X->rows), st), "adjoint rows");
+    const bool flat = X->u2d || X->v2d;
+    if (flat) {
+        REQ(X->u2d);
+        REQ(X->v2d);
+        if (!(X->vel_div != 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "vel_div must be non-zero");
+        A.u2d = CP(X->u2d);
+        A.v2d = CP(X->v2d);
+        A.vel_div = X->vel_div;
+    }
+    // factors, right-hand side and multipliers in the checkerboard layout
+    HIP_TRY(launch_enth_adj_factor<R>(A, P(X->rows), st), "adjoint factors");
     HIP_TRY(launch_enth_adj_rhs<R>(A, CP(dL_dE), CP(dL_dB), n_glen, B_scale, P(X->g), st), "adjoint rhs");
+    HIP_TRY(launch_enth_adj_layout<R>(ny, nx, nz, CP(X->lambda), P(X->lambda_work), true, st), "adjoint layout");
     const double rtol = cfg->relative_tolerance, atol = cfg->absolute_tolerance;
-    HIP_TRY(launch_enth_adj_residual<R>(nullptr, ny, nx, nz, CP(X->rows), CP(X->g), CP(X->lambda), P(X->r),
-                                        s->partials, s->status, 0, rtol, atol, st),
+    HIP_TRY(launch_enth_adj_residual<R>(nullptr, A, flat, CP(X->rows), CP(X->g), CP(X->lambda_work), s->partials,
+                                        s->status, 0, rtol, atol, st),
             "adjoint residual");
     const int* skip = &s->status->done;
     // red-black column Gauss-Seidel sweeps; each records the residual of the
     // columns before their update and stops (device-side) at the tolerance
     for (int it = 0; it < n_iter; ++it)
-        HIP_TRY(launch_enth_adj_sweep_rb<R>(skip, ny, nx, nz, CP(X->rows), CP(X->g), P(X->lambda), cfg->omega,
+        HIP_TRY(launch_enth_adj_sweep_rb<R>(skip, A, flat, CP(X->rows), CP(X->g), P(X->lambda_work), cfg->omega,
                                             s->partials, s->status, rtol, atol, st),
                 "adjoint sweep");
+    HIP_TRY(launch_enth_adj_layout<R>(ny, nx, nz, CP(X->lambda_work), P(X->lambda), false, st), "adjoint layout");
     HIP_TRY(launch_enth_adj_grads<R>(A, CP(X->lambda), P(X->g_E_prev), P(X->g_phi_strain), P(X->g_Q_geo),
                                      P(X->g_E_surface), st),
             "adjoint gradients");

@@ -12,7 +12,9 @@
 // neighbours come from cross-lane shuffles.  The column's tridiagonal Newton
 // system is solved with parallel cyclic reduction across the 64 lanes (rows
 // >= nz are identity rows) instead of a serial per-thread Thomas sweep, so no
-// per-thread arrays of length nz are needed.
+// per-thread arrays of length nz are needed.  The adjoint sweeps, whose
+// column blocks are fixed, instead factor them once and solve one column per
+// thread in a level-major checkerboard layout (see the adjoint layout below).
 #include <hip/hip_runtime.h>
 
 #include "enthalpy_kernels.hpp"
@@ -79,31 +81,38 @@ struct Horiz {
     T E_xm, E_xp, E_ym, E_yp;
 };
 
+// Lax-Friedrichs thickness flux through an x facet between columns A (left)
+// and B (right), and through a y facet between A (top) and B (bottom).
 template <typename T>
-__device__ __forceinline__ Horiz<T> load_horiz(const EnthArgs<T>& A, int i, int j, int k) {
-    const int ny = A.ny, nx = A.nx, nz = A.nz;
-    Horiz<T> h;
+__device__ __forceinline__ T lf_flux_x(T u, T HA, T HB) {
+    return T(0.5) * (HA + HB) * u - T(0.5) * sqrt(u * u + T(kMassFluxReg)) * (HB - HA);
+}
+template <typename T>
+__device__ __forceinline__ T lf_flux_y(T v, T HA, T HB) {
+    return T(0.5) * (HA + HB) * v - T(0.5) * sqrt(v * v + T(kMassFluxReg)) * (HA - HB);
+}
+
+// Face mass fluxes of column (i, j) at level k (zero on the domain edge);
+// the neighbour enthalpies are left zero.
+template <typename T>
+__device__ __forceinline__ Horiz<T> face_fluxes(const EnthArgs<T>& A, int i, int j, int k) {
+    const int ny = A.ny, nx = A.nx;
+    Horiz<T> h{};
     const T Hh = A.H[(long)i * nx + j];
     const T* u = A.u2d ? A.u2d : A.u3d + (long)k * ny * (nx + 1);
     const T* v = A.u2d ? A.v2d : A.v3d + (long)k * (ny + 1) * nx;
     const T div = A.u2d ? A.vel_div : T(1);
-    h.Hu_l = h.Hu_r = h.Hv_t = h.Hv_b = T(0);
-    if (j > 0) {
-        T ul = u[(long)i * (nx + 1) + j] / div, Hm = A.H[(long)i * nx + j - 1];
-        h.Hu_l = T(0.5) * (Hm + Hh) * ul - T(0.5) * sqrt(ul * ul + T(kMassFluxReg)) * (Hh - Hm);
-    }
-    if (j < nx - 1) {
-        T ur = u[(long)i * (nx + 1) + j + 1] / div, Hp = A.H[(long)i * nx + j + 1];
-        h.Hu_r = T(0.5) * (Hh + Hp) * ur - T(0.5) * sqrt(ur * ur + T(kMassFluxReg)) * (Hp - Hh);
-    }
-    if (i > 0) {
-        T vt = v[(long)i * nx + j] / div, Hm = A.H[(long)(i - 1) * nx + j];
-        h.Hv_t = T(0.5) * (Hm + Hh) * vt - T(0.5) * sqrt(vt * vt + T(kMassFluxReg)) * (Hm - Hh);
-    }
-    if (i < ny - 1) {
-        T vb = v[(long)(i + 1) * nx + j] / div, Hp = A.H[(long)(i + 1) * nx + j];
-        h.Hv_b = T(0.5) * (Hh + Hp) * vb - T(0.5) * sqrt(vb * vb + T(kMassFluxReg)) * (Hh - Hp);
-    }
+    if (j > 0) h.Hu_l = lf_flux_x(u[(long)i * (nx + 1) + j] / div, A.H[(long)i * nx + j - 1], Hh);
+    if (j < nx - 1) h.Hu_r = lf_flux_x(u[(long)i * (nx + 1) + j + 1] / div, Hh, A.H[(long)i * nx + j + 1]);
+    if (i > 0) h.Hv_t = lf_flux_y(v[(long)i * nx + j] / div, A.H[(long)(i - 1) * nx + j], Hh);
+    if (i < ny - 1) h.Hv_b = lf_flux_y(v[(long)(i + 1) * nx + j] / div, Hh, A.H[(long)(i + 1) * nx + j]);
+    return h;
+}
+
+template <typename T>
+__device__ __forceinline__ Horiz<T> load_horiz(const EnthArgs<T>& A, int i, int j, int k) {
+    const int ny = A.ny, nx = A.nx, nz = A.nz;
+    Horiz<T> h = face_fluxes(A, i, j, k);
     auto col = [&](int ii, int jj) {
         ii = min(max(ii, 0), ny - 1);
         jj = min(max(jj, 0), nx - 1);
@@ -623,17 +632,16 @@ struct JRow {
     T a, b, c, xm, xp, ym, yp;
 };
 
-// Exact Jacobian row of an interior node of a thick column; Ekm1 / Ekp1 the
-// column neighbours' enthalpy.
+// Exact Jacobian row of an interior node of a thick column; hz the column's
+// face fluxes at this level, Ekm1 / Ekp1 the column neighbours' enthalpy.
 template <typename T>
-__device__ __forceinline__ JRow<T> exact_row(const EnthArgs<T>& A, int i, int j, int k, T h, T Ek, T Ekm1, T Ekp1,
-                                             T om_m, T om_k, T om_p) {
+__device__ __forceinline__ JRow<T> exact_row(const EnthArgs<T>& A, const Horiz<T>& hz, int k, T h, T Ek, T Ekm1,
+                                             T Ekp1, T om_m, T om_k, T om_p) {
     const int nz = A.nz;
     const T dsig = T(1) / T(nz - 1);
     const T hinv = T(1) / h;
     JRow<T> w{T(0), T(kRhoI) * h / A.dt, T(0), T(0), T(0), T(0), T(0)};
     if (A.flags & 1) {
-        const Horiz<T> hz = load_horiz(A, i, j, k);
         const T sq = A.lf_sq, f = T(kRhoI) * A.dxi;
         auto dl = [&](T U) { return T(0.5) * (U + fabs(U) * sq); };
         auto dr = [&](T U) { return T(0.5) * (U - fabs(U) * sq); };
@@ -679,179 +687,190 @@ __device__ __forceinline__ JRow<T> exact_row(const EnthArgs<T>& A, int i, int j,
     return w;
 }
 
-// Rows (a, b, c, xm, xp, ym, yp) at the converged E into seven (ny, nx, nz)
-// planes; identity rows on surface nodes and thin columns.
-template <typename T>
-__global__ __launch_bounds__(256) void k_enth_adj_rows(EnthArgs<T> A, T* __restrict__ rows) {
-    const long ncol = (long)A.ny * A.nx;
-    const ColIdx q = col_index(wave_column(), A.nx, ncol);
-    if (!q.valid) return;
-    const int nz = A.nz, k = q.k;
-    const bool in = k < nz;
-    const long base = q.col * nz, n3 = ncol * nz;
-    const T h = A.H[q.col];
-    const T Ek = in ? A.E[base + k] : T(0);
-    const T omk = (in && A.omega) ? A.omega[base + k] : T(0);
-    const T Ekm1 = lane_get(Ek, max(k - 1, 0)), Ekp1 = lane_get(Ek, min(k + 1, 63));
-    const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
-    if (!in) return;
-    JRow<T> w{T(0), T(1), T(0), T(0), T(0), T(0), T(0)};
-    if (k < nz - 1 && !(h < A.h_thin)) w = exact_row(A, q.i, q.j, k, h, Ek, Ekm1, Ekp1, omm, omk, omp);
-    const long n = base + k;
-    rows[n] = w.a;
-    rows[n3 + n] = w.b;
-    rows[2 * n3 + n] = w.c;
-    rows[3 * n3 + n] = w.xm;
-    rows[4 * n3 + n] = w.xp;
-    rows[5 * n3 + n] = w.ym;
-    rows[6 * n3 + n] = w.yp;
+// ------------------------------------------------------------ adjoint layout
+// The adjoint sweeps work one thread per column (a serial Thomas solve down
+// the column with precomputed factors), so their planes are kept level-major
+// in a checkerboard layout: node (i, j, k) of colour c = (i + j) & 1 sits at
+// ((c * nz + k) * ny + i) * hw + (j >> 1), hw = (nx + 1) / 2.  A colour's
+// columns are then contiguous along a row at every level, and so are the
+// other colour's left, right, upper and lower neighbours of a run of them.
+__device__ __forceinline__ long cb_base(int c, int i, int t, int ny, int nz, int hw) {
+    return ((long)c * nz * ny + i) * hw + t;
 }
 
-// Horizontal part of (J^T lam) at node (i, j, k): the neighbouring rows'
-// couplings to this node's enthalpy.
-template <typename T>
-__device__ __forceinline__ T jt_horizontal(const T* __restrict__ rows, const T* __restrict__ lam, int ny, int nx,
-                                           int nz, long n3, int i, int j, int k) {
-    T s = T(0);
-    if (j > 0) { const long m = ((long)i * nx + j - 1) * nz + k; s += rows[4 * n3 + m] * lam[m]; }
-    if (j < nx - 1) { const long m = ((long)i * nx + j + 1) * nz + k; s += rows[3 * n3 + m] * lam[m]; }
-    if (i > 0) { const long m = ((long)(i - 1) * nx + j) * nz + k; s += rows[6 * n3 + m] * lam[m]; }
-    if (i < ny - 1) { const long m = ((long)(i + 1) * nx + j) * nz + k; s += rows[5 * n3 + m] * lam[m]; }
-    return s;
+// Virtual block vb of nvb: blocks go round-robin over the eight XCDs, so each
+// XCD is given one contiguous range of rows and finds the neighbouring rows a
+// pass rereads in its own L2.
+__device__ __forceinline__ long xcd_block(long vb, long nvb) {
+    const long q = nvb / 8, r = nvb % 8, x = vb % 8, l = vb / 8;
+    return x * q + (x < r ? x : r) + l;
 }
 
-// Column pass on J^T lam = g: each column's transposed tridiagonal block
-// solved by PCR against g minus the horizontal coupling of lam_in;
-// lam_out = lam_in + omega_relax (lam_col - lam_in).
+// Transposed horizontal couplings of node (i, j): the coefficient the
+// neighbour's row holds for this node's enthalpy (its xp / xm / yp / ym in
+// exact_row), from the shared facet's flux, which the node's own face_fluxes
+// hold; zero where the neighbour's row is an identity row (thin column; the
+// surface level is the caller's).
 template <typename T>
-__global__ __launch_bounds__(256) void k_enth_adj_column(const int* __restrict__ skip, int ny, int nx, int nz,
-                                                         const T* __restrict__ rows, const T* __restrict__ g,
-                                                         const T* __restrict__ lam_in, T* __restrict__ lam_out,
-                                                         T omega_relax) {
-    if (skipped(skip)) return;
-    const long ncol = (long)ny * nx;
-    const ColIdx q = col_index(wave_column(), nx, ncol);
-    if (!q.valid) return;
-    const int k = q.k;
-    const bool in = k < nz;
-    const long n3 = ncol * nz, n = q.col * nz + k;
-    const T a = in ? rows[n] : T(0), b = in ? rows[n3 + n] : T(1), c = in ? rows[2 * n3 + n] : T(0);
-    const T cm = lane_get(c, max(k - 1, 0)), ap = lane_get(a, min(k + 1, 63));
-    const T aT = k > 0 ? cm : T(0);        // (J^T)_{k,k-1} = J_{k-1,k}
-    const T cT = (k + 1 < nz) ? ap : T(0);  // (J^T)_{k,k+1} = J_{k+1,k}
-    T rhs = T(0), l0 = T(0);
-    if (in) {
-        l0 = lam_in[n];
-        rhs = g[n] - jt_horizontal(rows, lam_in, ny, nx, nz, n3, q.i, q.j, k);
-    }
-    const T lc = pcr_solve(in ? aT : T(0), b, in ? cT : T(0), rhs);
-    if (in) lam_out[n] = l0 + omega_relax * (lc - l0);
-}
-
-// Column of colour `color` ((i + j) % 2 == color) with colour index w:
-// rows alternate between the counts ce (even rows) and co (odd rows).
-__device__ __forceinline__ ColIdx color_col_index(long w, int color, int ny, int nx) {
-    const int ce = (nx + 1 - (color & 1)) / 2;  // columns of this colour in an even row
-    const long p = w / nx, r = w % nx;          // a pair of rows holds nx of them
-    const int i = (int)(r < ce ? 2 * p : 2 * p + 1);
-    const int t = (int)(r < ce ? r : r - ce);
-    const int j = 2 * t + ((i + color) & 1);
-    ColIdx c;
-    c.valid = i < ny && j < nx;
-    c.i = c.valid ? i : 0;
-    c.j = c.valid ? j : 0;
-    c.col = (long)c.i * nx + c.j;
-    c.k = lane_id();
+struct AdjHoriz {
+    T l, r, t, b;
+};
+template <typename T>
+__device__ __forceinline__ AdjHoriz<T> transposed_couplings(const EnthArgs<T>& A, const Horiz<T>& hz, int i, int j) {
+    AdjHoriz<T> c{T(0), T(0), T(0), T(0)};
+    if (!(A.flags & 1)) return c;
+    const int ny = A.ny, nx = A.nx;
+    const T sq = A.lf_sq, f = T(kRhoI) * A.dxi;
+    auto dl = [&](T U) { return T(0.5) * (U + fabs(U) * sq); };
+    auto dr = [&](T U) { return T(0.5) * (U - fabs(U) * sq); };
+    auto thick = [&](int ii, int jj) { return !(A.H[(long)ii * nx + jj] < A.h_thin); };
+    if (j > 0 && thick(i, j - 1)) c.l = f * dr(hz.Hu_l);
+    if (j < nx - 1 && thick(i, j + 1)) c.r = -f * dl(hz.Hu_r);
+    if (i > 0 && thick(i - 1, j)) c.t = -f * dl(hz.Hv_t);
+    if (i < ny - 1 && thick(i + 1, j)) c.b = f * dr(hz.Hv_b);
     return c;
 }
 
-// Red-black column Gauss-Seidel pass on J^T lam = g, in place: the columns of
-// one colour solve their transposed tridiagonal block (PCR) against g minus
-// the horizontal coupling of the other colour's current multipliers, which
-// this pass does not change; lam = lam + omega_relax (lam_col - lam).  The
-// max |g - J^T lam| of the columns before their update goes to
-// partials[blockIdx.x] (the residual the sweep's stopping test reads).
+// Factors of every column's transposed tridiagonal block J^T at the converged
+// E, one thread per column walking down it: plane 0 the sub-diagonal aT_k =
+// J_{k-1,k}, plane 1 the Thomas pivot w_k = b_k - aT_k cp_{k-1}, plane 2 cp_k =
+// J_{k+1,k} / w_k; planes 3-6 the transposed horizontal couplings (left,
+// right, upper, lower neighbour) per level, or, with a depth-constant
+// velocity, once per column on level 0.  Identity rows on surface nodes and
+// thin columns.
 template <typename T>
-__global__ __launch_bounds__(256) void k_enth_adj_column_rb(const int* __restrict__ skip, int ny, int nx, int nz,
-                                                            const T* __restrict__ rows, const T* __restrict__ g,
-                                                            T* __restrict__ lam, T omega_relax, int color,
-                                                            double* __restrict__ partials) {
-    __shared__ double smax[kColsPerBlock];
-    if (skipped(skip)) return;
-    const long ncol = (long)ny * nx, n3 = ncol * nz;
-    const long nc = (long)((ny + 1) / 2) * nx;  // colour indices covered (pairs of rows)
-    const int k = lane_id();
-    const bool in = k < nz;
-    double m = 0.0;
-    for (long w = wave_column(); w < nc; w += (long)gridDim.x * kColsPerBlock) {
-        const ColIdx q = color_col_index(w, color, ny, nx);
-        if (!q.valid) continue;
-        const long n = q.col * nz + k;
-        const T a = in ? rows[n] : T(0), b = in ? rows[n3 + n] : T(1), c = in ? rows[2 * n3 + n] : T(0);
-        const T cm = lane_get(c, max(k - 1, 0)), ap = lane_get(a, min(k + 1, 63));
-        const T aT = (in && k > 0) ? cm : T(0);        // (J^T)_{k,k-1} = J_{k-1,k}
-        const T cT = (in && k + 1 < nz) ? ap : T(0);   // (J^T)_{k,k+1} = J_{k+1,k}
-        T rhs = T(0), l0 = T(0);
-        if (in) {
-            l0 = lam[n];
-            rhs = g[n] - jt_horizontal(rows, lam, ny, nx, nz, n3, q.i, q.j, k);
-        }
-        const T lm = lane_get(l0, max(k - 1, 0)), lp = lane_get(l0, min(k + 1, 63));
-        const T res = in ? rhs - (aT * lm + b * l0 + cT * lp) : T(0);
-        m = fmax(m, fabs((double)res));
-        const T lc = pcr_solve(aT, b, cT, rhs);
-        if (in) lam[n] = l0 + omega_relax * (lc - l0);
-    }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
-    if (k == 0) smax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
-}
-
-// (J^T lam)_n at one node.
-template <typename T>
-__device__ __forceinline__ T jt_apply(const T* __restrict__ rows, const T* __restrict__ lam, int ny, int nx, int nz,
-                                      long n3, int i, int j, int k, long n) {
-    T s = rows[n3 + n] * lam[n];
-    if (k > 0) s += rows[2 * n3 + n - 1] * lam[n - 1];
-    if (k < nz - 1) s += rows[n + 1] * lam[n + 1];
-    return s + jt_horizontal(rows, lam, ny, nx, nz, n3, i, j, k);
-}
-
-// Layer pass: lam_out = lam_in + omega_relax (g - J^T lam_in) / J_nn.
-template <typename T>
-__global__ __launch_bounds__(256) void k_enth_adj_layer(const int* __restrict__ skip, int ny, int nx, int nz,
-                                                        const T* __restrict__ rows, const T* __restrict__ g,
-                                                        const T* __restrict__ lam_in, T* __restrict__ lam_out,
-                                                        T omega_relax) {
-    if (skipped(skip)) return;
-    const long n3 = (long)ny * nx * nz;
-    for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < n3; n += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(n % nz);
-        const long col = n / nz;
+__global__ __launch_bounds__(256) void k_enth_adj_factor(EnthArgs<T> A, T* __restrict__ rows) {
+    const int ny = A.ny, nx = A.nx, nz = A.nz, hw = (nx + 1) / 2;
+    const long ncol = (long)ny * nx, LS = (long)ny * hw, P = 2 * nz * LS;
+    const bool flat = A.u2d != nullptr;
+    for (long col = (long)blockIdx.x * 256 + threadIdx.x; col < ncol; col += (long)gridDim.x * 256) {
         const int i = (int)(col / nx), j = (int)(col % nx);
-        const T r = g[n] - jt_apply(rows, lam_in, ny, nx, nz, n3, i, j, k, n);
-        const T d = rows[n3 + n];
-        lam_out[n] = lam_in[n] + (fabs(d) > T(1e-30) ? omega_relax * r / d : T(0));
+        const long o = cb_base((i + j) & 1, i, j >> 1, ny, nz, hw), base = col * nz;
+        const T h = A.H[col];
+        const bool exact = !(h < A.h_thin);
+        Horiz<T> hz{};
+        if (A.flags & 1) hz = face_fluxes(A, i, j, 0);
+        if (flat) {
+            const AdjHoriz<T> c = transposed_couplings(A, hz, i, j);
+            rows[3 * P + o] = c.l;
+            rows[4 * P + o] = c.r;
+            rows[5 * P + o] = c.t;
+            rows[6 * P + o] = c.b;
+        }
+        T Em = T(0), E0 = A.E[base], Ep = nz > 1 ? A.E[base + 1] : T(0);
+        T om_m = T(0), om0 = A.omega ? A.omega[base] : T(0), om_p = (A.omega && nz > 1) ? A.omega[base + 1] : T(0);
+        T c_prev = T(0), w_prev = T(1), cp_prev = T(0);
+        for (int k = 0; k < nz; ++k) {
+            if (!flat && k > 0 && (A.flags & 1)) hz = face_fluxes(A, i, j, k);
+            JRow<T> w{T(0), T(1), T(0), T(0), T(0), T(0), T(0)};
+            if (k < nz - 1 && exact) w = exact_row(A, hz, k, h, E0, Em, Ep, om_m, om0, om_p);
+            const long ok = o + k * LS;
+            if (k > 0) {  // J_{k,k-1} completes column entry k - 1
+                cp_prev = w.a / w_prev;
+                rows[2 * P + ok - LS] = cp_prev;
+            }
+            const T aT = k > 0 ? c_prev : T(0);
+            const T wk = w.b - aT * cp_prev;
+            rows[ok] = aT;
+            rows[P + ok] = wk;
+            c_prev = w.c;
+            w_prev = wk;
+            if (!flat) {
+                AdjHoriz<T> c{T(0), T(0), T(0), T(0)};
+                if (k < nz - 1) c = transposed_couplings(A, hz, i, j);
+                rows[3 * P + ok] = c.l;
+                rows[4 * P + ok] = c.r;
+                rows[5 * P + ok] = c.t;
+                rows[6 * P + ok] = c.b;
+            }
+            Em = E0;
+            E0 = Ep;
+            Ep = k + 2 < nz ? A.E[base + k + 2] : T(0);
+            om_m = om0;
+            om0 = om_p;
+            om_p = (A.omega && k + 2 < nz) ? A.omega[base + k + 2] : om0;
+        }
+        rows[2 * P + o + (nz - 1) * LS] = T(0);
     }
 }
 
-// Adjoint residual g - J^T lam and per-block max |.|.
-template <typename T>
-__global__ __launch_bounds__(256) void k_enth_adj_residual(const int* __restrict__ skip, int ny, int nx, int nz,
-                                                           const T* __restrict__ rows, const T* __restrict__ g,
-                                                           const T* __restrict__ lam, T* __restrict__ r_out,
-                                                           double* __restrict__ partials) {
+// One colour's columns against J^T lam = g (lam in the checkerboard layout):
+// the residual g - J^T lam of each column before its update (max |.| to
+// partials[blockIdx.x]) and, with SOLVE, the column's transposed block solved
+// by the stored Thomas factors against g minus the horizontal coupling of the
+// other colour's multipliers, which this pass does not change; lam = lam +
+// omega_relax (lam_col - lam).  NZB >= nz bounds the per-thread forward
+// elimination kept in registers.
+template <typename T, int NZB, bool SOLVE>
+__global__ __launch_bounds__(256) void k_enth_adj_pass(const int* __restrict__ skip, int ny, int nx, int nz, int flat,
+                                                       const T* __restrict__ rows, const T* __restrict__ g,
+                                                       T* __restrict__ lam, T omega_relax, int color,
+                                                       double* __restrict__ partials) {
     __shared__ double smax[4];
     if (skipped(skip)) return;
-    const long n3 = (long)ny * nx * nz;
+    const int hw = (nx + 1) / 2;
+    const long LS = (long)ny * hw, P = 2 * nz * LS, n = LS, nvb = (n + 255) / 256;
     double m = 0.0;
-    for (long n = (long)blockIdx.x * blockDim.x + threadIdx.x; n < n3; n += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(n % nz);
-        const long col = n / nz;
-        const T r = g[n] - jt_apply(rows, lam, ny, nx, nz, n3, (int)(col / nx), (int)(col % nx), k, n);
-        if (r_out) r_out[n] = r;
-        m = fmax(m, fabs((double)r));
+    for (long vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
+        const long w = xcd_block(vb, nvb) * 256 + threadIdx.x;
+        const int i = (int)(w / hw), t = (int)(w % hw);
+        const int j = 2 * t + ((i + color) & 1);
+        if (w >= n || j >= nx) continue;
+        const long own = cb_base(color, i, t, ny, nz, hw), oth = cb_base(color ^ 1, i, 0, ny, nz, hw);
+        const bool hl = j > 0, hr = j + 1 < nx, hu = i > 0, hd = i + 1 < ny;
+        const long il = oth + ((j - 1) >> 1), ir = oth + ((j + 1) >> 1), iu = oth - hw + t, id = oth + hw + t;
+        T cl = T(0), cr = T(0), ct = T(0), cb = T(0);
+        if (flat) {
+            cl = rows[3 * P + own];
+            cr = rows[4 * P + own];
+            ct = rows[5 * P + own];
+            cb = rows[6 * P + own];
+        }
+        T dp[NZB];
+        T lm = T(0), l0 = lam[own], cpm = T(0), dprev = T(0);
+#pragma unroll
+        for (int k = 0; k < NZB; ++k) {
+            if (k < nz) {
+                const long ok = own + k * LS, kl = k * LS;
+                const T aT = rows[ok], wk = rows[P + ok], cpk = rows[2 * P + ok];
+                const T lp = k + 1 < nz ? lam[ok + LS] : T(0);
+                T hs = T(0);
+                if (k < nz - 1) {
+                    if (!flat) {
+                        cl = rows[3 * P + ok];
+                        cr = rows[4 * P + ok];
+                        ct = rows[5 * P + ok];
+                        cb = rows[6 * P + ok];
+                    }
+                    if (hl) hs += cl * lam[il + kl];
+                    if (hr) hs += cr * lam[ir + kl];
+                    if (hu) hs += ct * lam[iu + kl];
+                    if (hd) hs += cb * lam[id + kl];
+                }
+                const T rhs = g[ok] - hs;
+                const T b = wk + aT * cpm, cT = cpk * wk;
+                m = fmax(m, fabs((double)(rhs - (aT * lm + b * l0 + cT * lp))));
+                if (SOLVE) {
+                    dprev = (rhs - aT * dprev) / wk;
+                    dp[k] = dprev;
+                }
+                lm = l0;
+                l0 = lp;
+                cpm = cpk;
+            }
+        }
+        if (SOLVE) {
+            T x = T(0);
+#pragma unroll
+            for (int k = NZB - 1; k >= 0; --k) {
+                if (k < nz) {
+                    const long ok = own + k * LS;
+                    x = dp[k] - rows[2 * P + ok] * x;
+                    const T lo = lam[ok];
+                    lam[ok] = lo + omega_relax * (x - lo);
+                }
+            }
+        }
     }
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
@@ -859,46 +878,75 @@ __global__ __launch_bounds__(256) void k_enth_adj_residual(const int* __restrict
     if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
 }
 
+// lam between the (ny, nx, nz) layout and the checkerboard layout, one thread
+// per column.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_adj_layout(int ny, int nx, int nz, const T* __restrict__ src,
+                                                         T* __restrict__ dst, int to_cb) {
+    const int hw = (nx + 1) / 2;
+    const long ncol = (long)ny * nx, LS = (long)ny * hw;
+    for (long col = (long)blockIdx.x * 256 + threadIdx.x; col < ncol; col += (long)gridDim.x * 256) {
+        const int i = (int)(col / nx), j = (int)(col % nx);
+        const long o = cb_base((i + j) & 1, i, j >> 1, ny, nz, hw), base = col * nz;
+        for (int k = 0; k < nz; ++k) {
+            if (to_cb) dst[o + k * LS] = src[base + k];
+            else dst[base + k] = src[o + k * LS];
+        }
+    }
+}
+
+// Rate-factor term of level k: A_k (1 + 181.25 w_k) and its derivative with
+// respect to E_k.
+template <typename T>
+struct RateTerm {
+    T val, dA;
+};
+template <typename T>
+__device__ __forceinline__ RateTerm<T> rate_term(T Ek, T h, int k, int nz) {
+    const T Tp = t_pmp(T(k) * (T(1) / T(nz - 1)), h);
+    const T Tu = Ek * T(kEScale) / T(kCI) + T(kTRef);
+    const T Tk = fmin(Tu, Tp);
+    const T Af = Tk < T(263.15) ? T(3.985e-13) : T(1.916e3);
+    const T Qf = Tk < T(263.15) ? T(60e3) : T(139e3);
+    const T Ts = fmax(Tk, T(1));
+    const T Ak = Af * exp(-Qf / (T(8.314) * Ts));
+    const T wraw = Ek * T(kEScale) - T(kCI) * (Tp - T(kTRef));
+    const T wk = fmax(wraw, T(0)) / T(kLHeat);
+    const T dT = (Tu < Tp && Tk > T(1)) ? T(kEScale) / T(kCI) : T(0);
+    const T dw = wraw > T(0) ? T(kEScale) / T(kLHeat) : T(0);
+    RateTerm<T> r;
+    r.val = Ak * (T(1) + T(181.25) * wk);
+    r.dA = Ak * Qf / (T(8.314) * Ts * Ts) * dT * (T(1) + T(181.25) * wk) + Ak * T(181.25) * dw;
+    return r;
+}
+
 // Right-hand side g = dL/dE + (dB/dE)^T dL/dB of the rate factor
-// B = mean_k(A_k (1 + 181.25 w_k))^(-1/n) / scale.
+// B = mean_k(A_k (1 + 181.25 w_k))^(-1/n) / scale, into the checkerboard
+// layout, one thread per column.
 template <typename T>
 __global__ __launch_bounds__(256) void k_enth_adj_rhs(EnthArgs<T> A, const T* __restrict__ gE,
                                                       const T* __restrict__ gB, T n_glen, T scale,
                                                       T* __restrict__ g) {
-    const long ncol = (long)A.ny * A.nx;
-    const ColIdx q = col_index(wave_column(), A.nx, ncol);
-    if (!q.valid) return;
-    const int nz = A.nz, k = q.k;
-    const bool in = k < nz;
-    const long n = q.col * nz + k;
-    T val = T(0), dA = T(0);
-    if (in && gB) {
-        const T h = A.H[q.col];
-        const T Tp = t_pmp(T(k) * (T(1) / T(nz - 1)), h);
-        const T Ek = A.E[n];
-        const T Tu = Ek * T(kEScale) / T(kCI) + T(kTRef);
-        const T Tk = fmin(Tu, Tp);
-        const T Af = Tk < T(263.15) ? T(3.985e-13) : T(1.916e3);
-        const T Qf = Tk < T(263.15) ? T(60e3) : T(139e3);
-        const T Ts = fmax(Tk, T(1));
-        const T Ak = Af * exp(-Qf / (T(8.314) * Ts));
-        const T wraw = Ek * T(kEScale) - T(kCI) * (Tp - T(kTRef));
-        const T wk = fmax(wraw, T(0)) / T(kLHeat);
-        val = Ak * (T(1) + T(181.25) * wk);
-        const T dT = (Tu < Tp && Tk > T(1)) ? T(kEScale) / T(kCI) : T(0);
-        const T dw = wraw > T(0) ? T(kEScale) / T(kLHeat) : T(0);
-        dA = Ak * Qf / (T(8.314) * Ts * Ts) * dT * (T(1) + T(181.25) * wk) + Ak * T(181.25) * dw;
+    const int ny = A.ny, nx = A.nx, nz = A.nz, hw = (nx + 1) / 2;
+    const long ncol = (long)ny * nx, LS = (long)ny * hw;
+    for (long col = (long)blockIdx.x * 256 + threadIdx.x; col < ncol; col += (long)gridDim.x * 256) {
+        const int i = (int)(col / nx), j = (int)(col % nx);
+        const long o = cb_base((i + j) & 1, i, j >> 1, ny, nz, hw), base = col * nz;
+        const T h = A.H[col];
+        T fac = T(0);
+        if (gB) {
+            T sum = T(0);
+            for (int k = 0; k < nz; ++k) sum += rate_term(A.E[base + k], h, k, nz).val;
+            const T mean = sum / T(nz);
+            const T dBdmean = (T(-1) / n_glen) * pow(mean, T(-1) / n_glen - T(1)) / scale;
+            fac = gB[col] * dBdmean / T(nz);
+        }
+        for (int k = 0; k < nz; ++k) {
+            T out = gE ? gE[base + k] : T(0);
+            if (gB) out += fac * rate_term(A.E[base + k], h, k, nz).dA;
+            g[o + k * LS] = out;
+        }
     }
-    T sum = val;
-    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
-    if (!in) return;
-    T out = gE ? gE[n] : T(0);
-    if (gB) {
-        const T mean = sum / T(nz);
-        const T dBdmean = (T(-1) / n_glen) * pow(mean, T(-1) / n_glen - T(1)) / scale;
-        out += gB[q.col] * dBdmean * dA / T(nz);
-    }
-    g[n] = out;
 }
 
 // Parameter gradients dL/dp = -lam^T dG/dp: E_prev, phi_strain (3-D), Q_geo
@@ -1023,56 +1071,70 @@ hipError_t launch_enth_surface_from_temperature(const T* Ts, T* Es, long n, hipS
 }
 
 template <typename T>
-hipError_t launch_enth_adj_rows(const EnthArgs<T>& A, T* rows, hipStream_t st) {
-    k_enth_adj_rows<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, rows);
+hipError_t launch_enth_adj_factor(const EnthArgs<T>& A, T* rows, hipStream_t st) {
+    k_enth_adj_factor<T><<<flat_blocks((long)A.ny * A.nx), 256, 0, st>>>(A, rows);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_enth_adj_rhs(const EnthArgs<T>& A, const T* gE, const T* gB, T n_glen, T scale, T* g,
                                hipStream_t st) {
-    k_enth_adj_rhs<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, gE, gB, n_glen, scale, g);
+    k_enth_adj_rhs<T><<<flat_blocks((long)A.ny * A.nx), 256, 0, st>>>(A, gE, gB, n_glen, scale, g);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_enth_adj_column(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
-                                  const T* lam_in, T* lam_out, T omega_relax, hipStream_t st) {
-    k_enth_adj_column<T><<<col_blocks(ny, nx), 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam_in, lam_out,
-                                                              omega_relax);
+hipError_t launch_enth_adj_layout(int ny, int nx, int nz, const T* src, T* dst, bool to_cb, hipStream_t st) {
+    k_enth_adj_layout<T><<<flat_blocks((long)ny * nx), 256, 0, st>>>(ny, nx, nz, src, dst, to_cb ? 1 : 0);
     return hipGetLastError();
 }
+
+// Blocks of one colour pass (at most half the partials, so that both colours'
+// maxima fit).
+static inline unsigned adj_pass_blocks(int ny, int nx) {
+    const long nvb = ((long)ny * ((nx + 1) / 2) + 255) / 256;
+    return (unsigned)(nvb < kEnthResBlocks / 2 ? (nvb > 0 ? nvb : 1) : kEnthResBlocks / 2);
+}
+template <typename T, bool SOLVE>
+static void adj_pass(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g, T* lam,
+                     T omega_relax, int color, double* partials, hipStream_t st) {
+    const unsigned nb = adj_pass_blocks(A.ny, A.nx);
+    const int ny = A.ny, nx = A.nx, nz = A.nz, f = flat ? 1 : 0;
+    if (nz <= 16)
+        k_enth_adj_pass<T, 16, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, partials);
+    else if (nz <= 32)
+        k_enth_adj_pass<T, 32, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, partials);
+    else if (nz <= 48)
+        k_enth_adj_pass<T, 48, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, partials);
+    else
+        k_enth_adj_pass<T, 64, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, partials);
+}
 template <typename T>
-hipError_t launch_enth_adj_sweep_rb(const int* skip, int ny, int nx, int nz, const T* rows, const T* g, T* lam,
-                                    T omega_relax, double* partials, EnthStatus* status, double rtol, double atol,
-                                    hipStream_t st) {
-    const long nc = (long)((ny + 1) / 2) * nx;
-    long nb = (nc + kColsPerBlock - 1) / kColsPerBlock;
-    const unsigned g_ = (unsigned)(nb < kEnthResBlocks / 2 ? (nb > 0 ? nb : 1) : kEnthResBlocks / 2);
+hipError_t launch_enth_adj_sweep_rb(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
+                                    T* lam, T omega_relax, double* partials, EnthStatus* status, double rtol,
+                                    double atol, hipStream_t st) {
+    if (A.nz > 64) return hipErrorInvalidValue;
+    const unsigned nb = adj_pass_blocks(A.ny, A.nx);
+    const long half = (long)A.ny * A.nx * A.nz / 2;
     {
-        TimedLaunch tl(kTimeEnthColumn, (long)ny * nx * nz / 2, st);
-        k_enth_adj_column_rb<T><<<g_, 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam, omega_relax, 0, partials);
+        TimedLaunch tl(kTimeEnthColumn, half, st);
+        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 0, partials, st);
     }
     {
-        TimedLaunch tl(kTimeEnthColumn, (long)ny * nx * nz / 2, st);
-        k_enth_adj_column_rb<T><<<g_, 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam, omega_relax, 1, partials + g_);
+        TimedLaunch tl(kTimeEnthColumn, half, st);
+        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 1, partials + nb, st);
     }
-    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)(2 * g_), nullptr, status, 1, rtol, atol);
+    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)(2 * nb), nullptr, status, 1, rtol, atol);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_enth_adj_layer(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
-                                 const T* lam_in, T* lam_out, T omega_relax, hipStream_t st) {
-    k_enth_adj_layer<T><<<flat_blocks((long)ny * nx * nz), 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam_in, lam_out,
-                                                                          omega_relax);
-    return hipGetLastError();
-}
-template <typename T>
-hipError_t launch_enth_adj_residual(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
-                                    const T* lam, T* r_out, double* partials, EnthStatus* status, int status_mode,
-                                    double rtol, double atol, hipStream_t st) {
-    long nb = ((long)ny * nx * nz + 255) / 256;
-    unsigned g_ = (unsigned)(nb < kEnthResBlocks ? (nb > 0 ? nb : 1) : kEnthResBlocks);
-    k_enth_adj_residual<T><<<g_, 256, 0, st>>>(skip, ny, nx, nz, rows, g, lam, r_out, partials);
-    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)g_, nullptr, status, status_mode, rtol, atol);
+hipError_t launch_enth_adj_residual(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
+                                    const T* lam, double* partials, EnthStatus* status, int status_mode, double rtol,
+                                    double atol, hipStream_t st) {
+    if (A.nz > 64) return hipErrorInvalidValue;
+    const unsigned nb = adj_pass_blocks(A.ny, A.nx);
+    T* l = const_cast<T*>(lam);  // the residual pass does not write lam
+    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 0, partials, st);
+    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 1, partials + nb, st);
+    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)(2 * nb), nullptr, status, status_mode, rtol, atol);
     return hipGetLastError();
 }
 template <typename T>
@@ -1098,16 +1160,13 @@ hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev
     template hipError_t launch_enth_diagnostics<T>(const EnthArgs<T>&, T*, T*, hipStream_t);                         \
     template hipError_t launch_enth_from_temperature<T>(const EnthArgs<T>&, const T*, T*, hipStream_t);              \
     template hipError_t launch_enth_surface_from_temperature<T>(const T*, T*, long, hipStream_t);              \
-    template hipError_t launch_enth_adj_rows<T>(const EnthArgs<T>&, T*, hipStream_t);                                \
+    template hipError_t launch_enth_adj_factor<T>(const EnthArgs<T>&, T*, hipStream_t);                              \
     template hipError_t launch_enth_adj_rhs<T>(const EnthArgs<T>&, const T*, const T*, T, T, T*, hipStream_t);       \
-    template hipError_t launch_enth_adj_column<T>(const int*, int, int, int, const T*, const T*, const T*, T*, T,    \
-                                                  hipStream_t);                                                      \
-    template hipError_t launch_enth_adj_sweep_rb<T>(const int*, int, int, int, const T*, const T*, T*, T, double*,   \
-                                                    EnthStatus*, double, double, hipStream_t);                       \
-    template hipError_t launch_enth_adj_layer<T>(const int*, int, int, int, const T*, const T*, const T*, T*, T,     \
-                                                 hipStream_t);                                                       \
-    template hipError_t launch_enth_adj_residual<T>(const int*, int, int, int, const T*, const T*, const T*, T*,     \
-                                                    double*, EnthStatus*, int, double, double, hipStream_t);         \
+    template hipError_t launch_enth_adj_layout<T>(int, int, int, const T*, T*, bool, hipStream_t);                   \
+    template hipError_t launch_enth_adj_sweep_rb<T>(const int*, const EnthArgs<T>&, bool, const T*, const T*, T*, T, \
+                                                    double*, EnthStatus*, double, double, hipStream_t);              \
+    template hipError_t launch_enth_adj_residual<T>(const int*, const EnthArgs<T>&, bool, const T*, const T*,        \
+                                                    const T*, double*, EnthStatus*, int, double, double, hipStream_t); \
     template hipError_t launch_enth_adj_grads<T>(const EnthArgs<T>&, const T*, T*, T*, T*, T*, hipStream_t);
 INST(double)
 #undef INST

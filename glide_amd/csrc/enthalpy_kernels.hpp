@@ -80,31 +80,34 @@ hipError_t launch_enth_from_temperature(const EnthArgs<T>& A, const T* temperatu
 template <typename T>
 hipError_t launch_enth_surface_from_temperature(const T* Ts, T* Es, long n, hipStream_t st);
 
-// Adjoint of the converged solve: exact Jacobian rows (7 planes of
-// (ny, nx, nz)), right-hand side with the rate-factor term, transposed
-// column / layer passes, residual with the sweep bookkeeping, gradients.
+// Adjoint of the converged solve: factors of the exact Jacobian's transposed
+// column blocks, right-hand side with the rate-factor term, red-black column
+// sweeps, residual with the sweep bookkeeping, gradients.
+// Workspaces of the adjoint in the checkerboard layout (doubles per plane:
+// 2 * nz * ny * ((nx + 1) / 2)); rows holds seven planes: the Thomas factors
+// (aT, w, cp) of every column's transposed block and the transposed
+// horizontal couplings (per level, or on level 0 with A.u2d set).
 template <typename T>
-hipError_t launch_enth_adj_rows(const EnthArgs<T>& A, T* rows, hipStream_t st);
+hipError_t launch_enth_adj_factor(const EnthArgs<T>& A, T* rows, hipStream_t st);
 template <typename T>
 hipError_t launch_enth_adj_rhs(const EnthArgs<T>& A, const T* gE, const T* gB, T n_glen, T scale, T* g,
                                hipStream_t st);
+// lam between the (ny, nx, nz) and the checkerboard layouts.
 template <typename T>
-hipError_t launch_enth_adj_column(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
-                                  const T* lam_in, T* lam_out, T omega_relax, hipStream_t st);
-// One adjoint sweep: red and black column Gauss-Seidel passes (in place)
-// and the stopping test on the max residual of the columns before their
-// update (status mode 1).
+hipError_t launch_enth_adj_layout(int ny, int nx, int nz, const T* src, T* dst, bool to_cb, hipStream_t st);
+// One adjoint sweep: red and black column Gauss-Seidel passes (in place) and
+// the stopping test on the max residual of the columns before their update
+// (status mode 1).  flat: the couplings were stored for a depth-constant
+// velocity.
 template <typename T>
-hipError_t launch_enth_adj_sweep_rb(const int* skip, int ny, int nx, int nz, const T* rows, const T* g, T* lam,
-                                    T omega_relax, double* partials, EnthStatus* status, double rtol, double atol,
-                                    hipStream_t st);
+hipError_t launch_enth_adj_sweep_rb(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
+                                    T* lam, T omega_relax, double* partials, EnthStatus* status, double rtol,
+                                    double atol, hipStream_t st);
+// max |g - J^T lam| into the sweep bookkeeping (status mode 0 starts it).
 template <typename T>
-hipError_t launch_enth_adj_layer(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
-                                 const T* lam_in, T* lam_out, T omega_relax, hipStream_t st);
-template <typename T>
-hipError_t launch_enth_adj_residual(const int* skip, int ny, int nx, int nz, const T* rows, const T* g,
-                                    const T* lam, T* r_out, double* partials, EnthStatus* status, int status_mode,
-                                    double rtol, double atol, hipStream_t st);
+hipError_t launch_enth_adj_residual(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
+                                    const T* lam, double* partials, EnthStatus* status, int status_mode, double rtol,
+                                    double atol, hipStream_t st);
 template <typename T>
 hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev, T* g_phi, T* g_Q, T* g_Es,
                                  hipStream_t st);

@@ -536,8 +536,11 @@ void key_append(std::string& k, const T& v) {
     k.append(reinterpret_cast<const char*>(&v), sizeof(T));
 }
 
-std::string solve_key(char tag, const FasCtx& X, int start) {
+std::string solve_key(char tag, const FasCtx& X, int start, bool norms) {
     std::string k(1, tag);
+    key_append(k, norms);
+    key_append(k, g_fused_forward);
+    key_append(k, g_fused_adjoint);
     key_append(k, X.n_levels);
     key_append(k, start);
     key_append(k, X.dt);
@@ -613,19 +616,21 @@ int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, 
     };
     int it = 0;
     if (fixed) {
-        // initial residual (slot 0), the cycles, the residual after each (slots 1..)
+        // initial residual (slot 0), the cycles, the residual after each
+        // (slots 1..); without a report the norms are not computed at all
+        const bool norms = rep != nullptr;
         auto body = [&](hipStream_t cs) -> int {
             X.st = cs;
-            int rc = resid(true, s->norms);
+            int rc = norms ? resid(true, s->norms) : GLIDE_OK;
             for (int k = 0; rc == GLIDE_OK && k < X.cfg.maximum_vcycles; ++k) {
                 rc = cycle();
-                if (rc == GLIDE_OK) rc = resid(false, s->norms + 3 * (k + 1));
+                if (rc == GLIDE_OK && norms) rc = resid(false, s->norms + 3 * (k + 1));
             }
             X.st = st;
             return rc;
         };
         it = X.cfg.maximum_vcycles;
-        if (g_graphs_on && !timing_on()) GLIDE_TRY(replay(solve_key(tag, X, start), st, body));
+        if (g_graphs_on && !timing_on()) GLIDE_TRY(replay(solve_key(tag, X, start, norms), st, body));
         else GLIDE_TRY(body(st));
         if (!rep) return GLIDE_OK;  // nothing to report: no host synchronisation
         GLIDE_TRY(read_slots(it + 1));

@@ -299,25 +299,30 @@ class EnthalpyOperators:
         return w
 
     def adjoint_solve(self, dt, dL_dE, dL_dB=None, B_scale=1.0, n_iter=200, relative_tolerance=1e-12,
-                      absolute_tolerance=0.0, omega=1.0, report=False):
+                      absolute_tolerance=0.0, omega=1.0, report=False, u2d=None, v2d=None, vel_div=1.0):
         """Adjoint of the converged solve at the current E (no counterpart in
         the reference): solves J^T lambda = dL/dE + (dB/dE)^T dL/dB with J the
         exact Jacobian of the system the sweeps solve, B the rate factor
         divided by ``B_scale``, and returns ``{"lambda", "E_prev",
         "phi_strain", "Q_geo", "Q_fh", "E_surface"}`` gradients (``Q_fh`` and
-        ``Q_geo`` share one).  Exact for a converged forward solve."""
+        ``Q_geo`` share one).  Exact for a converged forward solve.  u2d, v2d
+        (optional): the depth-constant velocity whose quotient by vel_div the
+        velocity fields hold on every level (the coupled step's broadcast SSA
+        velocity); the sweeps then compute the horizontal couplings from it."""
         g = self.grid
         ny, nx, nz = g.ny, g.nx, self.nz
         if not hasattr(self, "_adj"):
-            z3 = lambda: g.zeros((ny, nx, nz))  # noqa: E731
-            self._adj = {"rows": g.zeros((7, ny, nx, nz)), "g": z3(), "lambda_work": z3(), "r": z3()}
+            plane = 2 * nz * ny * ((nx + 1) // 2)  # GLIDE_ENTH_ADJ_PLANE
+            self._adj = {"rows": g.zeros((7, plane)), "g": g.zeros((plane,)), "lambda_work": g.zeros((plane,))}
         W = self._adj
         lam = g.zeros((ny, nx, nz))
         out = {"E_prev": g.zeros((ny, nx, nz)), "phi_strain": g.zeros((ny, nx, nz)), "Q_geo": g.zeros((ny, nx)),
                "E_surface": g.zeros((ny, nx))}
         X = _lib.EnthAdjointC(rows=ptr(W["rows"]), g=ptr(W["g"]), lambda_=ptr(lam), lambda_work=ptr(W["lambda_work"]),
-                              r=ptr(W["r"]), g_E_prev=ptr(out["E_prev"]), g_phi_strain=ptr(out["phi_strain"]),
-                              g_Q_geo=ptr(out["Q_geo"]), g_E_surface=ptr(out["E_surface"]))
+                              r=None, g_E_prev=ptr(out["E_prev"]), g_phi_strain=ptr(out["phi_strain"]),
+                              g_Q_geo=ptr(out["Q_geo"]), g_E_surface=ptr(out["E_surface"]),
+                              u2d=ptr(u2d) if u2d is not None else None,
+                              v2d=ptr(v2d) if v2d is not None else None, vel_div=float(vel_div))
         gE = _as_field(dL_dE, (ny, nx, nz), g.device)
         gB = None if dL_dB is None else _as_field(dL_dB, (ny, nx), g.device)
         cfg = ColumnSmootherConfig(omega=omega, relative_tolerance=relative_tolerance,

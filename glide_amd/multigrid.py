@@ -307,7 +307,9 @@ class FASCDConfig:
     report_norms: bool = True
     compat_vprolong: bool = True
     # (an addition) fixed-cycle solves (both tolerances <= 0) return at once,
-    # without waiting for the GPU or filling a report; solve() then returns None
+    # without waiting for the GPU, computing residual norms or filling a
+    # report (the residual buffers then hold the last cycle's inner residual);
+    # solve() returns None
     asynchronous: bool = False
 
     def to_c(self) -> _lib.FasConfigC:
@@ -332,7 +334,9 @@ class FASAdjointConfig:
     report_norms: bool = True
     compat_vprolong: bool = True
     # (an addition) fixed-cycle solves (both tolerances <= 0) return at once,
-    # without waiting for the GPU or filling a report; solve() then returns None
+    # without waiting for the GPU, computing residual norms or filling a
+    # report (the residual buffers then hold the last cycle's inner residual);
+    # solve() returns None
     asynchronous: bool = False
 
     def to_c(self) -> _lib.FasConfigC:

@@ -142,15 +142,26 @@ int glide_enth_from_temperature(const glide_enth_column* C, const void* temperat
 /* E_surface_out = scaled enthalpy of the (ny, nx) temperature capped at T_MELT. */
 int glide_enth_surface_from_temperature(const glide_enth_column* C, const void* T_surface, void* E_surface_out,
                                         void* stream);
-/* Adjoint of the converged solve (no counterpart in the reference): buffers. */
+/* Adjoint of the converged solve (no counterpart in the reference): buffers.
+ * The workspaces hold checkerboard planes of GLIDE_ENTH_ADJ_PLANE doubles:
+ * node (i, j, k) of colour c = (i + j) % 2 at ((c * nz + k) * ny + i) *
+ * ((nx + 1) / 2) + j / 2. */
+#define GLIDE_ENTH_ADJ_PLANE(ny, nx, nz) (2L * (long)(nz) * (long)(ny) * (long)(((nx) + 1) / 2))
 typedef struct glide_enth_adjoint {
-    void* rows;        /* workspace (7, ny, nx, nz): exact Jacobian rows */
-    void* g;           /* workspace (ny, nx, nz): right-hand side */
+    void* rows;        /* workspace, 7 planes: Thomas factors of the transposed column blocks, couplings */
+    void* g;           /* workspace, 1 plane: right-hand side */
     void* lambda;      /* (ny, nx, nz): initial guess in, adjoint state out */
-    void* lambda_work; /* workspace (ny, nx, nz), reserved (the in-place sweeps do not use it) */
-    void* r;           /* workspace (ny, nx, nz): adjoint residual */
+    void* lambda_work; /* workspace, 1 plane: the adjoint state during the sweeps */
+    void* r;           /* reserved, may be NULL */
     void *g_E_prev, *g_phi_strain; /* (ny, nx, nz) gradients, either may be NULL */
     void *g_Q_geo, *g_E_surface;   /* (ny, nx) gradients (g_Q_geo is also dL/dQ_fh), either may be NULL */
+    /* Optional depth-constant velocity (no counterpart in the reference): when
+     * u2d (ny, nx+1) and v2d (ny+1, nx) are set, level k's velocity is
+     * u2d / vel_div (the values u3d[k] holds in the coupled step; u3d and v3d
+     * are then not read) and the horizontal couplings are stored once per
+     * column instead of per level. */
+    const void *u2d, *v2d;
+    double vel_div;
 } glide_enth_adjoint;
 
 /* Adjoint of the thermal solve at the converged E of C: solves J^T lambda = g
@@ -158,7 +169,7 @@ typedef struct glide_enth_adjoint {
  * Dirichlet surface node, clamped thin columns) and g = dL/dE + (dB/dE)^T
  * dL/dB (dL_dB may be NULL; B = rate factor / B_scale), by red-black column
  * Gauss-Seidel sweeps (each column's transposed tridiagonal block solved
- * exactly, relaxation cfg->omega) for up to n_iter sweeps with the device-side
+ * exactly by one thread from factors computed once, relaxation cfg->omega) for up to n_iter sweeps with the device-side
  * stopping test on max |g - J^T lambda| of the columns before their update
  * (cfg tolerances); then writes the parameter gradients. */
 int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoint* X, double dt, const void* dL_dE,

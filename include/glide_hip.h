@@ -161,8 +161,10 @@ int glide_prolong_cell(const void* coarse, void* fine, int ny_fine, int nx_fine,
 /* Full FAS-CD solve on levels[start_level..n_levels-1]; vc holds one config
  * per level.  Synchronises the stream once per V-cycle for the stopping test
  * unless both tolerances are <= 0 (then it runs exactly maximum_vcycles
- * V-cycles and synchronises once at the end to fill the report, or not at
- * all when report is NULL; glide_fas_adjoint_solve alike). */
+ * V-cycles and synchronises once at the end to fill the report; with report
+ * NULL it neither synchronises nor computes the per-cycle residual norms, and
+ * the level's residual buffers are not refreshed after the last cycle;
+ * glide_fas_adjoint_solve alike). */
 int glide_fascd_solve(const glide_level* levels, int n_levels, int start_level, double dt,
                       const glide_fas_config* cfg, const glide_vanka_config* vc, glide_solve_report* report,
                       void* stream);

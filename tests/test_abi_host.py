@@ -48,6 +48,13 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.EnthSweepConfigC) == 4 * 8 + 2 * 4
     assert ctypes.sizeof(_lib.EnthReport) == 8 + 16 + 8 * _lib.ENTH_MAX_SWEEPS
     assert ctypes.sizeof(_lib.ThermalConfigC) == 6 * 8 + 4 * 4
+    assert ctypes.sizeof(_lib.EnthAdjointC) == 11 * 8 + 8
+    body = eh[eh.index("typedef struct glide_enth_adjoint"):eh.index("} glide_enth_adjoint;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    # lambda is a Python keyword: the ctypes field is lambda_
+    assert tuple(re.findall(r"\*\s*(\w+)\s*[;,]", body)) == tuple(
+        f.rstrip("_") for f in _lib.ENTH_ADJOINT_POINTERS)
+    assert "double vel_div" in body
     assert f"GLIDE_ENTH_MAX_SWEEPS {_lib.ENTH_MAX_SWEEPS}" in eh
 
 

@@ -311,6 +311,27 @@ def test_thermal_step_deferred_rate_factor():
 B_SCALE = 917.0 * 9.81 * (365.25 * 86400.0) ** (1.0 / 3.0)
 
 
+@pytest.mark.parametrize("n,nz,seed", [(16, 21, 6), (23, 41, 7)])
+def test_adjoint_depth_constant_velocity_path(n, nz, seed):
+    """After a thermal step (whose velocity fields are the broadcast SSA
+    velocity over seconds per year), the adjoint with the 2-D velocity
+    (horizontal couplings computed in the sweeps) equals the one reading the
+    stored coupling planes."""
+    print(f"adjoint 2-D velocity n={n} nz={nz} seed={seed}", flush=True)
+    F, g, tm, _ = _thermal_pair(n, nz, seed, strain=True)
+    tm.step(DT)
+    ops = tm.ops
+    gE = ops.enthalpy_state.E.clone()
+    gB = torch.ones_like(g.rheology.B.data)
+    kw = dict(B_scale=B_SCALE, n_iter=200, relative_tolerance=1e-12, report=True)
+    a = ops.adjoint_solve(DT, gE, gB, **kw)
+    b = ops.adjoint_solve(DT, gE, gB, u2d=g.state.u.data, v2d=g.state.v.data, vel_div=365.25 * 86400.0, **kw)
+    torch.cuda.synchronize()
+    print(f"  sweeps {a['report'].sweeps} / {b['report'].sweeps}", flush=True)
+    for k in ("lambda", "E_prev", "phi_strain", "Q_geo", "E_surface"):
+        assert rel_err(N(b[k]), N(a[k])) < 1e-10, k
+
+
 @pytest.mark.parametrize("ny,nx,nz,seed", CASES)
 def test_adjoint_solve_matches_oracle(ny, nx, nz, seed):
     """GPU adjoint (iterative transposed column/layer passes) against the
