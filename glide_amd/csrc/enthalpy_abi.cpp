@@ -50,6 +50,7 @@ int fail(int code, const std::string& msg) {
 struct EnthScratch {
     double* partials = nullptr;   // kEnthResBlocks
     double* max_out = nullptr;    // 1
+    unsigned long long* acc = nullptr;  // adjoint passes' max accumulator, kept zero between uses
     EnthStatus* status = nullptr;
     EnthStatus* host = nullptr;   // pinned mirror
 };
@@ -66,6 +67,8 @@ int scratch(EnthScratch** out) {
     if (!s.partials) {
         HIP_TRY(hipMalloc(&s.partials, sizeof(double) * kEnthResBlocks), "hipMalloc(enthalpy partials)");
         HIP_TRY(hipMalloc(&s.max_out, sizeof(double)), "hipMalloc(enthalpy max)");
+        HIP_TRY(hipMalloc(&s.acc, sizeof(unsigned long long)), "hipMalloc(enthalpy accumulator)");
+        HIP_TRY(hipMemset(s.acc, 0, sizeof(unsigned long long)), "hipMemset(enthalpy accumulator)");
         HIP_TRY(hipMalloc(&s.status, sizeof(EnthStatus)), "hipMalloc(enthalpy status)");
         HIP_TRY(hipHostMalloc(&s.host, sizeof(EnthStatus), hipHostMallocDefault), "hipHostMalloc(enthalpy status)");
     }
@@ -374,7 +377,7 @@ int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoin
     HIP_TRY(launch_enth_adj_rhs<R>(A, CP(dL_dE), CP(dL_dB), n_glen, B_scale, P(X->g), st), "adjoint rhs");
     HIP_TRY(launch_enth_adj_layout<R>(ny, nx, nz, CP(X->lambda), P(X->lambda_work), true, st), "adjoint layout");
     const double rtol = cfg->relative_tolerance, atol = cfg->absolute_tolerance;
-    HIP_TRY(launch_enth_adj_residual<R>(nullptr, A, flat, CP(X->rows), CP(X->g), CP(X->lambda_work), s->partials,
+    HIP_TRY(launch_enth_adj_residual<R>(nullptr, A, flat, CP(X->rows), CP(X->g), CP(X->lambda_work), s->acc,
                                         s->status, 0, rtol, atol, st),
             "adjoint residual");
     const int* skip = &s->status->done;
@@ -382,7 +385,7 @@ int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoin
     // columns before their update and stops (device-side) at the tolerance
     for (int it = 0; it < n_iter; ++it)
         HIP_TRY(launch_enth_adj_sweep_rb<R>(skip, A, flat, CP(X->rows), CP(X->g), P(X->lambda_work), cfg->omega,
-                                            s->partials, s->status, rtol, atol, st),
+                                            s->acc, s->status, rtol, atol, st),
                 "adjoint sweep");
     HIP_TRY(launch_enth_adj_layout<R>(ny, nx, nz, CP(X->lambda_work), P(X->lambda), false, st), "adjoint layout");
     HIP_TRY(launch_enth_adj_grads<R>(A, CP(X->lambda), P(X->g_E_prev), P(X->g_phi_strain), P(X->g_Q_geo),

@@ -294,6 +294,26 @@ __global__ __launch_bounds__(256) void k_enth_residual(EnthArgs<T> A, T* __restr
     if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
 }
 
+// Sweep bookkeeping: mode 0 starts a sweep (r0, zero count, clear the skip
+// flag); mode 1 closes one sweep iteration (count, norm, stopping test of
+// glide/enthalpy.py:426-471).
+__device__ __forceinline__ void record_sweep(EnthStatus* status, int mode, double r, double rtol, double atol) {
+    if (mode == 0) {
+        status->r0 = r;
+        status->r = r;
+        status->sweeps = 0;
+        status->done = 0;
+    } else {
+        const int it = status->sweeps;
+        if (it < kEnthMaxSweeps) status->norms[it] = r;
+        status->sweeps = it + 1;
+        status->r = r;
+        const double r0 = status->r0;
+        const double rel = r0 > 0.0 ? r / r0 : 0.0;
+        if (rel < rtol || r < atol) status->done = 1;
+    }
+}
+
 // Final max over the partials.  With a status record: mode 0 starts a sweep
 // (r0, zero count, clear the skip flag); mode 1 closes one sweep iteration
 // (count, norm, stopping test of glide/enthalpy.py:426-471) unless the
@@ -310,22 +330,20 @@ __global__ __launch_bounds__(256) void k_enth_reduce(const double* __restrict__ 
     if (threadIdx.x != 0) return;
     const double r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
     if (out) *out = r;
-    if (!status) return;
-    if (mode == 0) {
-        status->r0 = r;
-        status->r = r;
-        status->sweeps = 0;
-        status->done = 0;
-    } else {
-        const int it = status->sweeps;
-        if (it < kEnthMaxSweeps) status->norms[it] = r;
-        status->sweeps = it + 1;
-        status->r = r;
-        const double r0 = status->r0;
-        const double rel = r0 > 0.0 ? r / r0 : 0.0;
-        if (rel < rtol || r < atol) status->done = 1;
-    }
+    if (status) record_sweep(status, mode, r, rtol, atol);
 }
+
+// The adjoint sweeps' maxima arrive in one accumulator (bit patterns of
+// non-negative doubles, combined by atomic max): read it into the
+// bookkeeping and clear it for the next pass.
+__global__ void k_enth_adj_close(unsigned long long* __restrict__ acc, EnthStatus* status, int mode, double rtol,
+                                 double atol) {
+    if (mode == 1 && status->done) return;  // the skipped passes left it zero
+    const double r = __longlong_as_double((long long)*acc);
+    *acc = 0ull;
+    record_sweep(status, mode, r, rtol, atol);
+}
+
 
 // Column Newton smoother (glide/cuda/enthalpy.cu:932-1112): Newton on the
 // column with horizontal neighbours frozen at E; writes delta = E_col - E
@@ -794,80 +812,128 @@ __global__ __launch_bounds__(256) void k_enth_adj_factor(EnthArgs<T> A, T* __res
     }
 }
 
+// Load at a byte offset from a wave-uniform base: the address is then a
+// scalar base plus a 32-bit per-lane offset, and the level stride moves into
+// the scalar base.
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* __restrict__ base, unsigned byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+constexpr int kAdjChunk = 4;  // levels whose loads a thread issues together
+
 // One colour's columns against J^T lam = g (lam in the checkerboard layout):
-// the residual g - J^T lam of each column before its update (max |.| to
-// partials[blockIdx.x]) and, with SOLVE, the column's transposed block solved
+// the residual g - J^T lam of each column before its update (max |.| into
+// *acc) and, with SOLVE, the column's transposed block solved
 // by the stored Thomas factors against g minus the horizontal coupling of the
 // other colour's multipliers, which this pass does not change; lam = lam +
-// omega_relax (lam_col - lam).  NZB >= nz bounds the per-thread forward
-// elimination kept in registers.
+// omega_relax (lam_col - lam).  NZB >= nz bounds the forward elimination kept
+// in registers.  A thread issues the loads of kAdjChunk levels at once
+// (branch-free: a missing neighbour reads the thread's own node against a zero
+// coupling), so that several memory latencies overlap down the column.  Byte
+// offsets within one checkerboard plane fit 32 bits (checked by the host).
 template <typename T, int NZB, bool SOLVE>
 __global__ __launch_bounds__(256) void k_enth_adj_pass(const int* __restrict__ skip, int ny, int nx, int nz, int flat,
                                                        const T* __restrict__ rows, const T* __restrict__ g,
                                                        T* __restrict__ lam, T omega_relax, int color,
-                                                       double* __restrict__ partials) {
+                                                       unsigned long long* __restrict__ acc) {
+    constexpr int KC = kAdjChunk;
     __shared__ double smax[4];
     if (skipped(skip)) return;
     const int hw = (nx + 1) / 2;
-    const long LS = (long)ny * hw, P = 2 * nz * LS, n = LS, nvb = (n + 255) / 256;
+    const long LS = (long)ny * hw, P = 2 * nz * LS, n = LS;
+    const T* __restrict__ rA = rows;
+    const T* __restrict__ rW = rows + P;
+    const T* __restrict__ rC = rows + 2 * P;
     double m = 0.0;
-    for (long vb = blockIdx.x; vb < nvb; vb += gridDim.x) {
-        const long w = xcd_block(vb, nvb) * 256 + threadIdx.x;
-        const int i = (int)(w / hw), t = (int)(w % hw);
-        const int j = 2 * t + ((i + color) & 1);
-        if (w >= n || j >= nx) continue;
+    const long w = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int i = (int)(w / hw), t = (int)(w % hw);
+    const int j = 2 * t + ((i + color) & 1);
+    if (w < n && j < nx) {
         const long own = cb_base(color, i, t, ny, nz, hw), oth = cb_base(color ^ 1, i, 0, ny, nz, hw);
-        const bool hl = j > 0, hr = j + 1 < nx, hu = i > 0, hd = i + 1 < ny;
-        const long il = oth + ((j - 1) >> 1), ir = oth + ((j + 1) >> 1), iu = oth - hw + t, id = oth + hw + t;
+        const unsigned b_own = (unsigned)own * sizeof(T);
+        const unsigned b_l = (unsigned)(j > 0 ? oth + ((j - 1) >> 1) : own) * sizeof(T);
+        const unsigned b_r = (unsigned)(j + 1 < nx ? oth + ((j + 1) >> 1) : own) * sizeof(T);
+        const unsigned b_u = (unsigned)(i > 0 ? oth - hw + t : own) * sizeof(T);
+        const unsigned b_d = (unsigned)(i + 1 < ny ? oth + hw + t : own) * sizeof(T);
         T cl = T(0), cr = T(0), ct = T(0), cb = T(0);
-        if (flat) {
-            cl = rows[3 * P + own];
-            cr = rows[4 * P + own];
-            ct = rows[5 * P + own];
-            cb = rows[6 * P + own];
+        if (flat) {  // zero on the domain edge, where the offsets above fall back to the own node
+            cl = ld_at(rows + 3 * P, b_own);
+            cr = ld_at(rows + 4 * P, b_own);
+            ct = ld_at(rows + 5 * P, b_own);
+            cb = ld_at(rows + 6 * P, b_own);
         }
         T dp[NZB];
-        T lm = T(0), l0 = lam[own], cpm = T(0), dprev = T(0);
+        T lm = T(0), l0 = ld_at(lam, b_own), cpm = T(0), dprev = T(0);
 #pragma unroll
-        for (int k = 0; k < NZB; ++k) {
-            if (k < nz) {
-                const long ok = own + k * LS, kl = k * LS;
-                const T aT = rows[ok], wk = rows[P + ok], cpk = rows[2 * P + ok];
-                const T lp = k + 1 < nz ? lam[ok + LS] : T(0);
-                T hs = T(0);
-                if (k < nz - 1) {
+        for (int k0 = 0; k0 < NZB; k0 += KC) {
+            if (k0 < nz) {
+                T a[KC], wv[KC], c[KC], gv[KC], lp[KC], xl[KC], xr[KC], xu[KC], xd[KC];
+                T ql[KC], qr[KC], qu[KC], qd[KC];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    const long lk = (long)min(k0 + q, nz - 1) * LS, lp1 = (long)min(k0 + q + 1, nz - 1) * LS;
+                    a[q] = ld_at(rA + lk, b_own);
+                    wv[q] = ld_at(rW + lk, b_own);
+                    c[q] = ld_at(rC + lk, b_own);
+                    gv[q] = ld_at(g + lk, b_own);
+                    lp[q] = ld_at(lam + lp1, b_own);
+                    xl[q] = ld_at(lam + lk, b_l);
+                    xr[q] = ld_at(lam + lk, b_r);
+                    xu[q] = ld_at(lam + lk, b_u);
+                    xd[q] = ld_at(lam + lk, b_d);
                     if (!flat) {
-                        cl = rows[3 * P + ok];
-                        cr = rows[4 * P + ok];
-                        ct = rows[5 * P + ok];
-                        cb = rows[6 * P + ok];
+                        ql[q] = ld_at(rows + 3 * P + lk, b_own);
+                        qr[q] = ld_at(rows + 4 * P + lk, b_own);
+                        qu[q] = ld_at(rows + 5 * P + lk, b_own);
+                        qd[q] = ld_at(rows + 6 * P + lk, b_own);
                     }
-                    if (hl) hs += cl * lam[il + kl];
-                    if (hr) hs += cr * lam[ir + kl];
-                    if (hu) hs += ct * lam[iu + kl];
-                    if (hd) hs += cb * lam[id + kl];
                 }
-                const T rhs = g[ok] - hs;
-                const T b = wk + aT * cpm, cT = cpk * wk;
-                m = fmax(m, fabs((double)(rhs - (aT * lm + b * l0 + cT * lp))));
-                if (SOLVE) {
-                    dprev = (rhs - aT * dprev) / wk;
-                    dp[k] = dprev;
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    const int k = k0 + q;
+                    if (k < nz) {
+                        const T lpk = k + 1 < nz ? lp[q] : T(0);
+                        T hs = T(0);
+                        if (k < nz - 1) {  // every neighbour's surface row is an identity row
+                            if (flat) hs = cl * xl[q] + cr * xr[q] + ct * xu[q] + cb * xd[q];
+                            else hs = ql[q] * xl[q] + qr[q] * xr[q] + qu[q] * xu[q] + qd[q] * xd[q];
+                        }
+                        const T rhs = gv[q] - hs;
+                        const T b = wv[q] + a[q] * cpm, cT = c[q] * wv[q];
+                        m = fmax(m, fabs((double)(rhs - (a[q] * lm + b * l0 + cT * lpk))));
+                        if (SOLVE) {
+                            dprev = (rhs - a[q] * dprev) / wv[q];
+                            dp[k] = dprev;
+                        }
+                        lm = l0;
+                        l0 = lpk;
+                        cpm = c[q];
+                    }
                 }
-                lm = l0;
-                l0 = lp;
-                cpm = cpk;
             }
         }
         if (SOLVE) {
             T x = T(0);
 #pragma unroll
-            for (int k = NZB - 1; k >= 0; --k) {
-                if (k < nz) {
-                    const long ok = own + k * LS;
-                    x = dp[k] - rows[2 * P + ok] * x;
-                    const T lo = lam[ok];
-                    lam[ok] = lo + omega_relax * (x - lo);
+            for (int k1 = NZB - 1; k1 >= 0; k1 -= KC) {
+                if (k1 - KC + 1 < nz) {
+                    T c[KC], lo[KC];
+#pragma unroll
+                    for (int q = 0; q < KC; ++q) {
+                        const long lk = (long)max(min(k1 - q, nz - 1), 0) * LS;
+                        c[q] = ld_at(rC + lk, b_own);
+                        lo[q] = ld_at(lam + lk, b_own);
+                    }
+#pragma unroll
+                    for (int q = 0; q < KC; ++q) {
+                        const int k = k1 - q;
+                        if (k >= 0 && k < nz) {
+                            x = dp[k] - c[q] * x;
+                            *reinterpret_cast<T*>(reinterpret_cast<char*>(lam + (long)k * LS) + b_own) =
+                                lo[q] + omega_relax * (x - lo[q]);
+                        }
+                    }
                 }
             }
         }
@@ -875,7 +941,9 @@ __global__ __launch_bounds__(256) void k_enth_adj_pass(const int* __restrict__ s
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+    // non-negative doubles order as their bit patterns
+    if (threadIdx.x == 0)
+        atomicMax(acc, (unsigned long long)__double_as_longlong(fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]))));
 }
 
 // lam between the (ny, nx, nz) layout and the checkerboard layout, one thread
@@ -1087,54 +1155,46 @@ hipError_t launch_enth_adj_layout(int ny, int nx, int nz, const T* src, T* dst, 
     return hipGetLastError();
 }
 
-// Blocks of one colour pass (at most half the partials, so that both colours'
-// maxima fit).
-static inline unsigned adj_pass_blocks(int ny, int nx) {
-    const long nvb = ((long)ny * ((nx + 1) / 2) + 255) / 256;
-    return (unsigned)(nvb < kEnthResBlocks / 2 ? (nvb > 0 ? nvb : 1) : kEnthResBlocks / 2);
-}
 template <typename T, bool SOLVE>
 static void adj_pass(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g, T* lam,
-                     T omega_relax, int color, double* partials, hipStream_t st) {
-    const unsigned nb = adj_pass_blocks(A.ny, A.nx);
+                     T omega_relax, int color, unsigned long long* acc, hipStream_t st) {
+    const unsigned nb = (unsigned)(((long)A.ny * ((A.nx + 1) / 2) + 255) / 256);
     const int ny = A.ny, nx = A.nx, nz = A.nz, f = flat ? 1 : 0;
     if (nz <= 16)
-        k_enth_adj_pass<T, 16, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, partials);
+        k_enth_adj_pass<T, 16, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc);
     else if (nz <= 32)
-        k_enth_adj_pass<T, 32, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, partials);
+        k_enth_adj_pass<T, 32, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc);
     else if (nz <= 48)
-        k_enth_adj_pass<T, 48, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, partials);
+        k_enth_adj_pass<T, 48, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc);
     else
-        k_enth_adj_pass<T, 64, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, partials);
+        k_enth_adj_pass<T, 64, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc);
 }
 template <typename T>
 hipError_t launch_enth_adj_sweep_rb(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
-                                    T* lam, T omega_relax, double* partials, EnthStatus* status, double rtol,
+                                    T* lam, T omega_relax, unsigned long long* acc, EnthStatus* status, double rtol,
                                     double atol, hipStream_t st) {
     if (A.nz > 64) return hipErrorInvalidValue;
-    const unsigned nb = adj_pass_blocks(A.ny, A.nx);
     const long half = (long)A.ny * A.nx * A.nz / 2;
     {
         TimedLaunch tl(kTimeEnthColumn, half, st);
-        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 0, partials, st);
+        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 0, acc, st);
     }
     {
         TimedLaunch tl(kTimeEnthColumn, half, st);
-        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 1, partials + nb, st);
+        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 1, acc, st);
     }
-    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)(2 * nb), nullptr, status, 1, rtol, atol);
+    k_enth_adj_close<<<1, 1, 0, st>>>(acc, status, 1, rtol, atol);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_enth_adj_residual(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
-                                    const T* lam, double* partials, EnthStatus* status, int status_mode, double rtol,
-                                    double atol, hipStream_t st) {
+                                    const T* lam, unsigned long long* acc, EnthStatus* status, int status_mode,
+                                    double rtol, double atol, hipStream_t st) {
     if (A.nz > 64) return hipErrorInvalidValue;
-    const unsigned nb = adj_pass_blocks(A.ny, A.nx);
     T* l = const_cast<T*>(lam);  // the residual pass does not write lam
-    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 0, partials, st);
-    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 1, partials + nb, st);
-    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)(2 * nb), nullptr, status, status_mode, rtol, atol);
+    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 0, acc, st);
+    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 1, acc, st);
+    k_enth_adj_close<<<1, 1, 0, st>>>(acc, status, status_mode, rtol, atol);
     return hipGetLastError();
 }
 template <typename T>
@@ -1164,9 +1224,10 @@ hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev
     template hipError_t launch_enth_adj_rhs<T>(const EnthArgs<T>&, const T*, const T*, T, T, T*, hipStream_t);       \
     template hipError_t launch_enth_adj_layout<T>(int, int, int, const T*, T*, bool, hipStream_t);                   \
     template hipError_t launch_enth_adj_sweep_rb<T>(const int*, const EnthArgs<T>&, bool, const T*, const T*, T*, T, \
-                                                    double*, EnthStatus*, double, double, hipStream_t);              \
+                                                    unsigned long long*, EnthStatus*, double, double, hipStream_t);  \
     template hipError_t launch_enth_adj_residual<T>(const int*, const EnthArgs<T>&, bool, const T*, const T*,        \
-                                                    const T*, double*, EnthStatus*, int, double, double, hipStream_t); \
+                                                    const T*, unsigned long long*, EnthStatus*, int, double, double, \
+                                                    hipStream_t);                                                    \
     template hipError_t launch_enth_adj_grads<T>(const EnthArgs<T>&, const T*, T*, T*, T*, T*, hipStream_t);
 INST(double)
 #undef INST

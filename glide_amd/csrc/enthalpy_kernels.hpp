@@ -101,13 +101,14 @@ hipError_t launch_enth_adj_layout(int ny, int nx, int nz, const T* src, T* dst, 
 // velocity.
 template <typename T>
 hipError_t launch_enth_adj_sweep_rb(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
-                                    T* lam, T omega_relax, double* partials, EnthStatus* status, double rtol,
+                                    T* lam, T omega_relax, unsigned long long* acc, EnthStatus* status, double rtol,
                                     double atol, hipStream_t st);
 // max |g - J^T lam| into the sweep bookkeeping (status mode 0 starts it).
+// acc: the passes' max accumulator, zero on entry and left zero.
 template <typename T>
 hipError_t launch_enth_adj_residual(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
-                                    const T* lam, double* partials, EnthStatus* status, int status_mode, double rtol,
-                                    double atol, hipStream_t st);
+                                    const T* lam, unsigned long long* acc, EnthStatus* status, int status_mode,
+                                    double rtol, double atol, hipStream_t st);
 template <typename T>
 hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev, T* g_phi, T* g_Q, T* g_Es,
                                  hipStream_t st);
