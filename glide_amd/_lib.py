@@ -142,6 +142,7 @@ SIGNATURES = {
     "glide_graphs_enable": (_I, [_I]),
     "glide_adjoint_fused_enable": (_I, [_I]),
     "glide_forward_fused_enable": (_I, [_I]),
+    "glide_adjoint_coarse_enable": (_I, [_I]),
     "glide_graphs_clear": (_I, []),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -185,6 +186,12 @@ def adjoint_fused_enable(on: bool) -> None:
 def forward_fused_enable(on: bool) -> None:
     """Forward passes of small levels as one fused launch (default) or solve + apply."""
     call("glide_forward_fused_enable", int(bool(on)))
+
+
+def adjoint_coarse_enable(on: bool) -> None:
+    """Cached adjoint passes of levels of at most 512 cells (linear drag) in
+    one workgroup (default) or one launch pair per pass."""
+    call("glide_adjoint_coarse_enable", int(bool(on)))
 
 
 def graphs_clear() -> None:

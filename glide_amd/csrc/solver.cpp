@@ -155,6 +155,9 @@ int sumsq3_async(const R* a, size_t na, const R* b, size_t nb, const R* c, size_
 bool g_fused_adjoint = true;
 // Forward passes of small levels as one fused launch (glide_forward_fused_enable).
 bool g_fused_forward = true;
+// Cached adjoint passes of levels of at most 512 cells as one workgroup
+// (glide_adjoint_coarse_enable).
+bool g_coarse_adjoint = true;
 
 int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
                  hipStream_t st) {
@@ -245,6 +248,15 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
                          hipStream_t st, bool prepared = false) {
     const VankaArgs<R> vk = adjoint_vanka_args(L, vc, flags);
     const R* cache = adj_cache(L, prepared);
+    if (cache && g_coarse_adjoint && n_iter > 0) {
+        SsaArgs<R> a = make_args(L, dt, true);
+        if (adjoint_coarse_fits<R>(a)) {
+            HIP_TRY(launch_adjoint_coarse<R>(a, cache, P(L.lambda_u), P(L.lambda_v), P(L.lambda_H), CP(L.af_u),
+                                             CP(L.af_v), CP(L.af_H), vk.calving_rate, vc.omega, n_iter, st),
+                    "adjoint passes in one workgroup");
+            return GLIDE_OK;
+        }
+    }
     for (int it = 0; it < n_iter; ++it) {
         GLIDE_TRY(vjp(L, dt, flags | GLIDE_USE_FORCING, CP(L.lambda_u), CP(L.lambda_v), CP(L.lambda_H), P(L.ar_u),
                       P(L.ar_v), P(L.ar_H), st, prepared));
@@ -541,6 +553,7 @@ std::string solve_key(char tag, const FasCtx& X, int start, bool norms) {
     key_append(k, norms);
     key_append(k, g_fused_forward);
     key_append(k, g_fused_adjoint);
+    key_append(k, g_coarse_adjoint);
     key_append(k, X.n_levels);
     key_append(k, start);
     key_append(k, X.dt);
@@ -876,6 +889,11 @@ int glide_adjoint_fused_enable(int on) {
 
 int glide_forward_fused_enable(int on) {
     g_fused_forward = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_adjoint_coarse_enable(int on) {
+    g_coarse_adjoint = on != 0;
     return GLIDE_OK;
 }
 

@@ -837,24 +837,26 @@ struct VjpTile {
     T Pv[4][ROWS + 1][TX + 1]; // λv * drag cross-derivatives, v facets rows [i0, i0+ROWS], cols [j0-1, j0+TX-1]
 };
 
-template <typename T, int ROWS>
-__global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __restrict__ lu, const T* __restrict__ lv,
-                                            const T* __restrict__ lH, const T* __restrict__ fu,
-                                            const T* __restrict__ fv, const T* __restrict__ fH, T* __restrict__ out_u,
-                                            T* __restrict__ out_v, T* __restrict__ out_H, T calving_rate,
-                                            const T* __restrict__ eta_cache) {
-    __shared__ VjpTile<T, ROWS> s;
+// The transpose operator for the outputs (facets and cells) i0 <= i < i0 +
+// rows, j0 <= j < j0 + cols, by nthr threads (tid = 0 .. nthr - 1) of one
+// block, through the block storage S (cell arrays over the one-cell ring,
+// vertex arrays, and with S::kDrag the drag cross-terms; without it the drag
+// must be linear, whose cross-terms vanish).
+template <typename T, typename S>
+__device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restrict__ lu, const T* __restrict__ lv,
+                                         const T* __restrict__ lH, const T* __restrict__ fu, const T* __restrict__ fv,
+                                         const T* __restrict__ fH, T* out_u, T* out_v, T* out_H, T calving_rate,
+                                         const T* __restrict__ eta_cache, int i0, int j0, int rows, int cols, int tid,
+                                         int nthr, S& s) {
     const int ny = a.ny, nx = a.nx;
     const T dxi = a.dxi;
-    const int i0 = blockIdx.y * ROWS, j0 = blockIdx.x * TX;
-    const int tid = threadIdx.y * TX + threadIdx.x;
     const T* __restrict__ u = a.u;
     const T* __restrict__ v = a.v;
     const T* __restrict__ H = a.H;
 
     // phase 1: cells
-    for (int k = tid; k < (ROWS + 2) * (TX + 2); k += (TX * ROWS)) {
-        const int ri = k / (TX + 2), rj = k % (TX + 2);
+    for (int k = tid; k < (rows + 2) * (cols + 2); k += nthr) {
+        const int ri = k / (cols + 2), rj = k % (cols + 2);
         const int ci = i0 - 1 + ri, cj = j0 - 1 + rj;
         T lHm = 0, e = 0, ed = 0, Hh = 0, Lx = 0, Ly = 0, Kx = 0, Ky = 0;
         if (ci >= 0 && ci < ny && cj >= 0 && cj < nx) {
@@ -874,32 +876,32 @@ __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __rest
             Ly = sy.d_ul * cl.ul + sy.d_ur * cl.ur + sy.d_vt * cl.vt + sy.d_vb * cl.vb + sy.d_eH * (Hh * ed);
             Ky = sy.d_eH;
         }
-        s.lHm[ri][rj] = lHm; s.eta[ri][rj] = e; s.etad[ri][rj] = ed; s.Hs[ri][rj] = Hh;
-        s.Lxx[ri][rj] = Lx; s.Lyy[ri][rj] = Ly; s.Kxx[ri][rj] = Kx; s.Kyy[ri][rj] = Ky;
+        s.lHm(ri, rj) = lHm; s.eta(ri, rj) = e; s.etad(ri, rj) = ed; s.Hs(ri, rj) = Hh;
+        s.Lxx(ri, rj) = Lx; s.Lyy(ri, rj) = Ly; s.Kxx(ri, rj) = Kx; s.Kyy(ri, rj) = Ky;
     }
     __syncthreads();
-    // phase 2a: vertices (vi, vj) in [i0, i0+ROWS] x [j0, j0+TX]
-    for (int k = tid; k < (ROWS + 1) * (TX + 1); k += (TX * ROWS)) {
-        const int ri = k / (TX + 1), rj = k % (TX + 1);
+    // phase 2a: vertices (vi, vj) in [i0, i0+rows] x [j0, j0+cols]
+    for (int k = tid; k < (rows + 1) * (cols + 1); k += nthr) {
+        const int ri = k / (cols + 1), rj = k % (cols + 1);
         const int vi = i0 + ri, vj = j0 + rj;
         T L = 0, G = 0;
         if (vi > 0 && vi < ny && vj > 0 && vj < nx) {
             // cells (vi-1,vj-1), (vi-1,vj), (vi,vj-1), (vi,vj) at tile rows ri, ri+1 and cols rj, rj+1
-            T eH = T(0.25) * (s.eta[ri][rj] * s.Hs[ri][rj] + s.eta[ri][rj + 1] * s.Hs[ri][rj + 1] +
-                              s.eta[ri + 1][rj] * s.Hs[ri + 1][rj] + s.eta[ri + 1][rj + 1] * s.Hs[ri + 1][rj + 1]);
-            T eHd = T(0.25) * (s.etad[ri][rj] * s.Hs[ri][rj] + s.etad[ri][rj + 1] * s.Hs[ri][rj + 1] +
-                               s.etad[ri + 1][rj] * s.Hs[ri + 1][rj] + s.etad[ri + 1][rj + 1] * s.Hs[ri + 1][rj + 1]);
+            T eH = T(0.25) * (s.eta(ri, rj) * s.Hs(ri, rj) + s.eta(ri, rj + 1) * s.Hs(ri, rj + 1) +
+                              s.eta(ri + 1, rj) * s.Hs(ri + 1, rj) + s.eta(ri + 1, rj + 1) * s.Hs(ri + 1, rj + 1));
+            T eHd = T(0.25) * (s.etad(ri, rj) * s.Hs(ri, rj) + s.etad(ri, rj + 1) * s.Hs(ri, rj + 1) +
+                               s.etad(ri + 1, rj) * s.Hs(ri + 1, rj) + s.etad(ri + 1, rj + 1) * s.Hs(ri + 1, rj + 1));
             const long ut = (long)(vi - 1) * (nx + 1) + vj, ub = (long)vi * (nx + 1) + vj;
             const long vl = (long)vi * nx + vj - 1, vr = (long)vi * nx + vj;
             ShearStress<T> q = sxy(u[ut], u[ub], v[vl], v[vr], eH, dxi, true);
             L = q.d_ut * lu[ut] + q.d_ub * lu[ub] + q.d_vl * lv[vl] + q.d_vr * lv[vr] + q.d_eH * eHd;
             G = q.d_eH * T(0.25) * (lu[ub] - lu[ut] + lv[vl] - lv[vr]) * dxi;
         }
-        s.Lxy[ri][rj] = L; s.Gxy[ri][rj] = G;
+        s.Lxy(ri, rj) = L; s.Gxy(ri, rj) = G;
     }
     // phase 2b: drag cross-terms of u facets (rows i0-1.., cols j0..) and v facets (rows i0.., cols j0-1..)
-    for (int k = tid; k < (ROWS + 1) * (TX + 1); k += (TX * ROWS)) {
-        const int ri = k / (TX + 1), rj = k % (TX + 1);
+    for (int k = tid; S::kDrag && k < (rows + 1) * (cols + 1); k += nthr) {
+        const int ri = k / (cols + 1), rj = k % (cols + 1);
         {
             const int fi = i0 - 1 + ri, fj = j0 + rj;
             T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
@@ -911,7 +913,7 @@ __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __rest
                 T l = lu[(long)fi * (nx + 1) + fj];
                 q0 = l * t.c0; q1 = l * t.c1; q2 = l * t.c2; q3 = l * t.c3;
             }
-            s.Pu[0][ri][rj] = q0; s.Pu[1][ri][rj] = q1; s.Pu[2][ri][rj] = q2; s.Pu[3][ri][rj] = q3;
+            s.Pu(0, ri, rj) = q0; s.Pu(1, ri, rj) = q1; s.Pu(2, ri, rj) = q2; s.Pu(3, ri, rj) = q3;
         }
         {
             const int fi = i0 + ri, fj = j0 - 1 + rj;
@@ -924,110 +926,241 @@ __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __rest
                 T l = lv[(long)fi * nx + fj];
                 q0 = l * t.c0; q1 = l * t.c1; q2 = l * t.c2; q3 = l * t.c3;
             }
-            s.Pv[0][ri][rj] = q0; s.Pv[1][ri][rj] = q1; s.Pv[2][ri][rj] = q2; s.Pv[3][ri][rj] = q3;
+            s.Pv(0, ri, rj) = q0; s.Pv(1, ri, rj) = q1; s.Pv(2, ri, rj) = q2; s.Pv(3, ri, rj) = q3;
         }
     }
     __syncthreads();
 
     // phase 3: outputs
-    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
-    if (i > ny || j > nx) return;
-    const int ci = threadIdx.y + 1, cj = threadIdx.x + 1;  // cell (i,j) in the ring tile
-    const int vi = threadIdx.y, vj = threadIdx.x;          // vertex (i,j) in the vertex tile
-    if (i < ny) {  // u facet (i,j)
-        const long o = (long)i * (nx + 1) + j;
-        T acc = T(0);
-        if (j > 0 && j < nx) {
-            acc = (s.Lxx[ci][cj] - s.Lxx[ci][cj - 1]) * dxi + (s.Lxy[vi][vj] - s.Lxy[vi + 1][vj]) * dxi;
-            const long cc = (long)i * nx + j;
-            T pl = a.phi[cc - 1], pc = a.phi[cc];
-            Drag<T> t = basal_drag(u[o], v[cc - 1], v[cc], v[cc + nx - 1], v[cc + nx], pl, pc, a.beta[cc - 1],
-                                   a.beta[cc], a.P);
-            acc += lu[o] * t.d_w;
-            // cross terms from v facets (i,j) [ubl], (i,j-1) [ubr], (i+1,j) [utl], (i+1,j-1) [utr]
-            acc += s.Pv[2][vi][cj] + s.Pv[3][vi][cj - 1] + s.Pv[0][vi + 1][cj] + s.Pv[1][vi + 1][cj - 1];
-            Flux<T> f = flux_x(u[o], H[cc - 1], H[cc], true);
-            acc += f.d_w * (s.lHm[ci][cj - 1] - s.lHm[ci][cj]) * dxi;
-            if (fu) acc -= fu[o];
+    for (int k = tid; k < rows * cols; k += nthr) {
+        const int ti = k / cols, tj = k % cols;
+        const int i = i0 + ti, j = j0 + tj;
+        if (i > ny || j > nx) continue;
+        const int ci = ti + 1, cj = tj + 1;  // cell (i,j) in the ring tile
+        const int vi = ti, vj = tj;          // vertex (i,j) in the vertex tile
+        if (i < ny) {  // u facet (i,j)
+            const long o = (long)i * (nx + 1) + j;
+            T acc = T(0);
+            if (j > 0 && j < nx) {
+                acc = (s.Lxx(ci, cj) - s.Lxx(ci, cj - 1)) * dxi + (s.Lxy(vi, vj) - s.Lxy(vi + 1, vj)) * dxi;
+                const long cc = (long)i * nx + j;
+                T pl = a.phi[cc - 1], pc = a.phi[cc];
+                Drag<T> t = basal_drag(u[o], v[cc - 1], v[cc], v[cc + nx - 1], v[cc + nx], pl, pc, a.beta[cc - 1],
+                                       a.beta[cc], a.P);
+                acc += lu[o] * t.d_w;
+                // cross terms from v facets (i,j) [ubl], (i,j-1) [ubr], (i+1,j) [utl], (i+1,j-1) [utr]
+                if (S::kDrag)
+                    acc += s.Pv(2, vi, cj) + s.Pv(3, vi, cj - 1) + s.Pv(0, vi + 1, cj) + s.Pv(1, vi + 1, cj - 1);
+                Flux<T> f = flux_x(u[o], H[cc - 1], H[cc], true);
+                acc += f.d_w * (s.lHm(ci, cj - 1) - s.lHm(ci, cj)) * dxi;
+                if (fu) acc -= fu[o];
+            }
+            out_u[o] = acc;
         }
-        out_u[o] = acc;
+        if (j < nx) {  // v facet (i,j)
+            const long o = (long)i * nx + j;
+            T acc = T(0);
+            if (i > 0 && i < ny) {
+                acc = (s.Lyy(ci - 1, cj) - s.Lyy(ci, cj)) * dxi + (s.Lxy(vi, vj + 1) - s.Lxy(vi, vj)) * dxi;
+                const long ul = (long)i * (nx + 1) + j, utl = ul - (nx + 1);
+                T pt = a.phi[o - nx], pc = a.phi[o];
+                Drag<T> t = basal_drag(v[o], u[utl], u[utl + 1], u[ul], u[ul + 1], pt, pc, a.beta[o - nx], a.beta[o], a.P);
+                acc += lv[o] * t.d_w;
+                // cross terms from u facets (i,j) [vtr], (i,j+1) [vtl], (i-1,j) [vbr], (i-1,j+1) [vbl]
+                if (S::kDrag)
+                    acc += s.Pu(1, vi + 1, vj) + s.Pu(0, vi + 1, vj + 1) + s.Pu(3, vi, vj) + s.Pu(2, vi, vj + 1);
+                Flux<T> f = flux_y(v[o], H[o - nx], H[o], true);
+                acc += f.d_w * (s.lHm(ci, cj) - s.lHm(ci - 1, cj)) * dxi;
+                if (fv) acc -= fv[o];
+            }
+            out_v[o] = acc;
+        }
+        if (i < ny && j < nx) {  // cell (i,j)
+            const long c = (long)i * nx + j;
+            T acc = T(0);
+            if (!(a.mask[c] > T(0))) {
+                const T lc = s.lHm(ci, cj);
+                const T Hc = s.Hs(ci, cj), ec = s.eta(ci, cj);
+                const long ol = (long)i * (nx + 1) + j;
+                acc = lc / a.dt;
+                // thickness fluxes through the four faces
+                {
+                    Flux<T> f = flux_x(u[ol], H[offC(i, j - 1, ny, nx)], Hc, j > 0);
+                    acc += (s.lHm(ci, cj - 1) - lc) * f.d_HB * dxi;
+                }
+                {
+                    Flux<T> f = flux_x(u[ol + 1], Hc, H[offC(i, j + 1, ny, nx)], j + 1 < nx);
+                    acc += (lc - s.lHm(ci, cj + 1)) * f.d_HA * dxi;
+                }
+                {
+                    Flux<T> f = flux_y(v[c], H[offC(i - 1, j, ny, nx)], Hc, i > 0);
+                    acc += (lc - s.lHm(ci - 1, cj)) * f.d_HB * dxi;
+                }
+                {
+                    Flux<T> f = flux_y(v[c + nx], Hc, H[offC(i + 1, j, ny, nx)], i + 1 < ny);
+                    acc += (s.lHm(ci + 1, cj) - lc) * f.d_HA * dxi;
+                }
+                if (calving_rate != T(0)) {
+                    const T pc = a.phi[c];
+                    T k = (T(1) - pc) * ((T(1) - a.phi[offC(i, j - 1, ny, nx)]) + (T(1) - a.phi[offC(i, j + 1, ny, nx)]) +
+                                         (T(1) - a.phi[offC(i - 1, j, ny, nx)]) + (T(1) - a.phi[offC(i + 1, j, ny, nx)]));
+                    acc += lc * k * calving_rate * dxi;
+                }
+                // viscosity-thickness products of the cell's normal stresses
+                acc += s.Kxx(ci, cj) * ec * (lu[ol] - lu[ol + 1]) * dxi;
+                acc += s.Kyy(ci, cj) * ec * (lv[c + nx] - lv[c]) * dxi;
+                // four corner vertices
+                acc += (s.Gxy(vi, vj) + s.Gxy(vi, vj + 1) + s.Gxy(vi + 1, vj) + s.Gxy(vi + 1, vj + 1)) * ec;
+                // driving stress on the four faces
+                {
+                    const T pc = a.phi[c], zc = a.bed[c];
+                    if (j > 0) {
+                        Drive<T> d = drive_x(H[c - 1], Hc, a.bed[c - 1], zc, a.phi[c - 1], pc, dxi, true);
+                        acc -= lu[ol] * d.d_HB;
+                    }
+                    if (j + 1 < nx) {
+                        Drive<T> d = drive_x(Hc, H[c + 1], zc, a.bed[c + 1], pc, a.phi[c + 1], dxi, true);
+                        acc -= lu[ol + 1] * d.d_HA;
+                    }
+                    if (i > 0) {
+                        Drive<T> d = drive_y(H[c - nx], Hc, a.bed[c - nx], zc, a.phi[c - nx], pc, dxi, true);
+                        acc -= lv[c] * d.d_HB;
+                    }
+                    if (i + 1 < ny) {
+                        Drive<T> d = drive_y(Hc, H[c + nx], zc, a.bed[c + nx], pc, a.phi[c + nx], dxi, true);
+                        acc -= lv[c + nx] * d.d_HA;
+                    }
+                }
+                if (fH) acc -= fH[c];
+            }
+            out_H[c] = acc;
+        }
     }
-    if (j < nx) {  // v facet (i,j)
-        const long o = (long)i * nx + j;
-        T acc = T(0);
-        if (i > 0 && i < ny) {
-            acc = (s.Lyy[ci - 1][cj] - s.Lyy[ci][cj]) * dxi + (s.Lxy[vi][vj + 1] - s.Lxy[vi][vj]) * dxi;
-            const long ul = (long)i * (nx + 1) + j, utl = ul - (nx + 1);
-            T pt = a.phi[o - nx], pc = a.phi[o];
-            Drag<T> t = basal_drag(v[o], u[utl], u[utl + 1], u[ul], u[ul + 1], pt, pc, a.beta[o - nx], a.beta[o], a.P);
-            acc += lv[o] * t.d_w;
-            // cross terms from u facets (i,j) [vtr], (i,j+1) [vtl], (i-1,j) [vbr], (i-1,j+1) [vbl]
-            acc += s.Pu[1][vi + 1][vj] + s.Pu[0][vi + 1][vj + 1] + s.Pu[3][vi][vj] + s.Pu[2][vi][vj + 1];
-            Flux<T> f = flux_y(v[o], H[o - nx], H[o], true);
-            acc += f.d_w * (s.lHm[ci][cj] - s.lHm[ci - 1][cj]) * dxi;
-            if (fv) acc -= fv[o];
+}
+
+// Block storage of the transpose operator: a static tile of k_vjp, and a
+// whole level carved from dynamic LDS (linear drag only).
+template <typename T, int ROWS>
+struct VjpTileStore {
+    static constexpr bool kDrag = true;
+    VjpTile<T, ROWS>& t;
+    __device__ T& lHm(int r, int c) { return t.lHm[r][c]; }
+    __device__ T& eta(int r, int c) { return t.eta[r][c]; }
+    __device__ T& etad(int r, int c) { return t.etad[r][c]; }
+    __device__ T& Hs(int r, int c) { return t.Hs[r][c]; }
+    __device__ T& Lxx(int r, int c) { return t.Lxx[r][c]; }
+    __device__ T& Lyy(int r, int c) { return t.Lyy[r][c]; }
+    __device__ T& Kxx(int r, int c) { return t.Kxx[r][c]; }
+    __device__ T& Kyy(int r, int c) { return t.Kyy[r][c]; }
+    __device__ T& Lxy(int r, int c) { return t.Lxy[r][c]; }
+    __device__ T& Gxy(int r, int c) { return t.Gxy[r][c]; }
+    __device__ T& Pu(int q, int r, int c) { return t.Pu[q][r][c]; }
+    __device__ T& Pv(int q, int r, int c) { return t.Pv[q][r][c]; }
+};
+template <typename T>
+struct VjpLevelStore {
+    static constexpr bool kDrag = false;
+    T* cell;  // 8 arrays of (rows + 2) x (cols + 2)
+    T* vert;  // 2 arrays of (rows + 1) x (cols + 1)
+    int cw, nc, vw, nv;
+    __device__ T& lHm(int r, int c) { return cell[r * cw + c]; }
+    __device__ T& eta(int r, int c) { return cell[nc + r * cw + c]; }
+    __device__ T& etad(int r, int c) { return cell[2 * nc + r * cw + c]; }
+    __device__ T& Hs(int r, int c) { return cell[3 * nc + r * cw + c]; }
+    __device__ T& Lxx(int r, int c) { return cell[4 * nc + r * cw + c]; }
+    __device__ T& Lyy(int r, int c) { return cell[5 * nc + r * cw + c]; }
+    __device__ T& Kxx(int r, int c) { return cell[6 * nc + r * cw + c]; }
+    __device__ T& Kyy(int r, int c) { return cell[7 * nc + r * cw + c]; }
+    __device__ T& Lxy(int r, int c) { return vert[r * vw + c]; }
+    __device__ T& Gxy(int r, int c) { return vert[nv + r * vw + c]; }
+    __device__ T& Pu(int, int, int) { return cell[0]; }  // never used without kDrag
+    __device__ T& Pv(int, int, int) { return cell[0]; }
+};
+
+template <typename T, int ROWS>
+__global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __restrict__ lu, const T* __restrict__ lv,
+                                            const T* __restrict__ lH, const T* __restrict__ fu,
+                                            const T* __restrict__ fv, const T* __restrict__ fH, T* __restrict__ out_u,
+                                            T* __restrict__ out_v, T* __restrict__ out_H, T calving_rate,
+                                            const T* __restrict__ eta_cache) {
+    __shared__ VjpTile<T, ROWS> t;
+    VjpTileStore<T, ROWS> s{t};
+    vjp_body(a, lu, lv, lH, fu, fv, fH, out_u, out_v, out_H, calving_rate, eta_cache, (int)blockIdx.y * ROWS,
+             (int)blockIdx.x * TX, ROWS, TX, (int)(threadIdx.y * TX + threadIdx.x), TX * ROWS, s);
+}
+
+// Doubles of dynamic LDS k_adjoint_coarse needs for an ny x nx level.
+inline long adjoint_coarse_lds(int ny, int nx) {
+    const long nc = (long)(ny + 3) * (nx + 3), nv = (long)(ny + 2) * (nx + 2);
+    return 8 * nc + 2 * nv + (long)ny * (nx + 1) + (long)(ny + 1) * nx + (long)ny * nx;
+}
+
+// All n_iter adjoint smoothing passes of a small level (ny * nx <=
+// kAdjointCoarseMaxCells) in one workgroup: per pass the transpose operator
+// minus the forcing into LDS, the cached patch solves (increments into LDS),
+// then the in-place update of k_vanka_adjoint_fused; the multipliers stay in
+// global memory, ordered between passes by the workgroup barrier.  The same
+// arithmetic as k_vjp followed by k_vanka_adjoint_fused, without a launch per
+// pass.
+constexpr int kCoarseThreads = 512;
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void k_adjoint_coarse(SsaArgs<T> a, const T* __restrict__ cache, T* lu,
+                                                                   T* lv, T* lH, const T* __restrict__ fu,
+                                                                   const T* __restrict__ fv, const T* __restrict__ fH,
+                                                                   T calving_rate, T omega, int n_iter) {
+    extern __shared__ double lds[];
+    const int ny = a.ny, nx = a.nx, rows = ny + 1, cols = nx + 1;
+    const int n = ny * nx;
+    VjpLevelStore<T> s;
+    s.cw = cols + 2;
+    s.nc = (rows + 2) * s.cw;
+    s.vw = cols + 1;
+    s.nv = (rows + 1) * s.vw;
+    s.cell = reinterpret_cast<T*>(lds);
+    s.vert = s.cell + 8 * s.nc;
+    T* ru = s.vert + 2 * s.nv;  // transpose operator outputs
+    T* rv = ru + ny * (nx + 1);
+    T* rH = rv + (ny + 1) * nx;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const T* eta_cache = cache + (long)kAdjointCacheEta * n;
+    for (int it = 0; it < n_iter; ++it) {
+        vjp_body(a, lu, lv, lH, fu, fv, fH, ru, rv, rH, calving_rate, eta_cache, 0, 0, rows, cols, tid, nthr, s);
+        __syncthreads();
+        // the cell arrays of the transpose operator are free until the next
+        // pass: the increments w0..w4 of cell c go to W[q * n + c]
+        T* W = s.cell;
+#pragma unroll 1
+        for (int c = tid; c < n; c += nthr) {
+            T w[5];
+            adjoint_cached_increments(cache, ru, rv, rH, c / nx, c % nx, ny, nx, w);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) W[q * n + c] = w[q];
         }
-        out_v[o] = acc;
-    }
-    if (i < ny && j < nx) {  // cell (i,j)
-        const long c = (long)i * nx + j;
-        T acc = T(0);
-        if (!(a.mask[c] > T(0))) {
-            const T lc = s.lHm[ci][cj];
-            const T Hc = s.Hs[ci][cj], ec = s.eta[ci][cj];
-            const long ol = (long)i * (nx + 1) + j;
-            acc = lc / a.dt;
-            // thickness fluxes through the four faces
+        __syncthreads();
+#pragma unroll 1
+        for (int c = tid; c < n; c += nthr) {
+            const int i = c / nx, j = c % nx;
             {
-                Flux<T> f = flux_x(u[ol], H[offC(i, j - 1, ny, nx)], Hc, j > 0);
-                acc += (s.lHm[ci][cj - 1] - lc) * f.d_HB * dxi;
-            }
-            {
-                Flux<T> f = flux_x(u[ol + 1], Hc, H[offC(i, j + 1, ny, nx)], j + 1 < nx);
-                acc += (lc - s.lHm[ci][cj + 1]) * f.d_HA * dxi;
-            }
-            {
-                Flux<T> f = flux_y(v[c], H[offC(i - 1, j, ny, nx)], Hc, i > 0);
-                acc += (lc - s.lHm[ci - 1][cj]) * f.d_HB * dxi;
-            }
-            {
-                Flux<T> f = flux_y(v[c + nx], Hc, H[offC(i + 1, j, ny, nx)], i + 1 < ny);
-                acc += (s.lHm[ci + 1][cj] - lc) * f.d_HA * dxi;
-            }
-            if (calving_rate != T(0)) {
-                const T pc = a.phi[c];
-                T k = (T(1) - pc) * ((T(1) - a.phi[offC(i, j - 1, ny, nx)]) + (T(1) - a.phi[offC(i, j + 1, ny, nx)]) +
-                                     (T(1) - a.phi[offC(i - 1, j, ny, nx)]) + (T(1) - a.phi[offC(i + 1, j, ny, nx)]));
-                acc += lc * k * calving_rate * dxi;
-            }
-            // viscosity-thickness products of the cell's normal stresses
-            acc += s.Kxx[ci][cj] * ec * (lu[ol] - lu[ol + 1]) * dxi;
-            acc += s.Kyy[ci][cj] * ec * (lv[c + nx] - lv[c]) * dxi;
-            // four corner vertices
-            acc += (s.Gxy[vi][vj] + s.Gxy[vi][vj + 1] + s.Gxy[vi + 1][vj] + s.Gxy[vi + 1][vj + 1]) * ec;
-            // driving stress on the four faces
-            {
-                const T pc = a.phi[c], zc = a.bed[c];
-                if (j > 0) {
-                    Drive<T> d = drive_x(H[c - 1], Hc, a.bed[c - 1], zc, a.phi[c - 1], pc, dxi, true);
-                    acc -= lu[ol] * d.d_HB;
-                }
-                if (j + 1 < nx) {
-                    Drive<T> d = drive_x(Hc, H[c + 1], zc, a.bed[c + 1], pc, a.phi[c + 1], dxi, true);
-                    acc -= lu[ol + 1] * d.d_HA;
-                }
-                if (i > 0) {
-                    Drive<T> d = drive_y(H[c - nx], Hc, a.bed[c - nx], zc, a.phi[c - nx], pc, dxi, true);
-                    acc -= lv[c] * d.d_HB;
-                }
-                if (i + 1 < ny) {
-                    Drive<T> d = drive_y(Hc, H[c + nx], zc, a.bed[c + nx], pc, a.phi[c + nx], dxi, true);
-                    acc -= lv[c + nx] * d.d_HA;
+                T d = W[c];
+                if (j > 0) d = d + W[n + c - 1];
+                lu[(long)i * (nx + 1) + j] += omega * d;
+                if (j == nx - 1) {
+                    T e = T(0) + W[n + c];
+                    lu[(long)i * (nx + 1) + nx] += omega * e;
                 }
             }
-            if (fH) acc -= fH[c];
+            {
+                T d = W[2 * n + c];
+                if (i > 0) d = d + W[3 * n + c - nx];
+                lv[c] += omega * d;
+                if (i == ny - 1) {
+                    T e = T(0) + W[3 * n + c];
+                    lv[(long)ny * nx + j] += omega * e;
+                }
+            }
+            lH[c] += omega * W[4 * n + c];
         }
-        out_H[c] = acc;
+        __syncthreads();
     }
 }
 
@@ -1203,6 +1336,28 @@ hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH
 }
 
 template <typename T>
+bool adjoint_coarse_fits(const SsaArgs<T>& a) {
+    if (!a.P.linear_drag || a.ny < 1 || a.nx < 1 || (long)a.ny * a.nx > kAdjointCoarseMaxCells) return false;
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return false;
+    return adjoint_coarse_lds(a.ny, a.nx) * (long)sizeof(T) <= (long)lds;
+}
+template <typename T>
+hipError_t launch_adjoint_coarse(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, const T* fu, const T* fv,
+                                 const T* fH, T calving_rate, T omega, int n_iter, hipStream_t st) {
+    if (!adjoint_coarse_fits(a)) return hipErrorInvalidValue;
+    const size_t bytes = (size_t)adjoint_coarse_lds(a.ny, a.nx) * sizeof(T);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adjoint_coarse<T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx * n_iter, st);
+    k_adjoint_coarse<T><<<1, kCoarseThreads, bytes, st>>>(a, cache, lu, lv, lH, fu, fv, fH, calving_rate, -omega,
+                                                          n_iter);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_adjoint_factor(const SsaArgs<T>& a, T* cache, const VankaArgs<T>& vk, hipStream_t st) {
     TimedLaunch tl(kTimeAdjointFactor, (long)a.ny * a.nx, st);
     k_adjoint_factor<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, cache, vk);
@@ -1247,6 +1402,9 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_vanka_fused<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, hipStream_t);\
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
+    template bool adjoint_coarse_fits<T>(const SsaArgs<T>&);                                                         \
+    template hipError_t launch_adjoint_coarse<T>(const SsaArgs<T>&, const T*, T*, T*, T*, const T*, const T*,          \
+                                                 const T*, T, T, int, hipStream_t);                                  \
     template hipError_t launch_vjp<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, const T*, const T*, \
                                       T*, T*, T*, T, const T*, hipStream_t);                                         \
     template hipError_t launch_adjoint_factor<T>(const SsaArgs<T>&, T*, const VankaArgs<T>&, hipStream_t);          \

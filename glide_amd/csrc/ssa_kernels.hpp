@@ -47,7 +47,10 @@ hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* 
 template <typename T>
 hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst, T* mask, const VankaArgs<T>& vk,
                               T omega, hipStream_t st);
-constexpr long kFusedVankaMaxCells = 48 * 48;
+#ifndef GLIDE_FUSED_VANKA_MAX_SIDE
+#define GLIDE_FUSED_VANKA_MAX_SIDE 48
+#endif
+constexpr long kFusedVankaMaxCells = (long)GLIDE_FUSED_VANKA_MAX_SIDE * GLIDE_FUSED_VANKA_MAX_SIDE;
 template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st);
 template <typename T>
@@ -55,6 +58,18 @@ hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* 
                                      T* work, const VankaArgs<T>& vk, T omega, hipStream_t st);
 // eta_cache: the cell viscosity cached by launch_adjoint_factor for the
 // current state, or nullptr to compute it.
+// All n_iter cached adjoint passes of a small level in one workgroup (linear
+// drag, ny * nx <= kAdjointCoarseMaxCells, its LDS within the device limit:
+// adjoint_coarse_fits); the same results as n_iter launch_vjp +
+// launch_vanka_adjoint_pass_cached.  One compute unit does a pass in about
+// 4 us + 15 ns per cell, a launch pair in about 13 us: the workgroup wins
+// below about 24 x 24 cells.
+constexpr long kAdjointCoarseMaxCells = 512;
+template <typename T>
+bool adjoint_coarse_fits(const SsaArgs<T>& a);
+template <typename T>
+hipError_t launch_adjoint_coarse(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, const T* fu, const T* fv,
+                                 const T* fH, T calving_rate, T omega, int n_iter, hipStream_t st);
 template <typename T>
 hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH, const T* fu, const T* fv,
                       const T* fH, T* ou, T* ov, T* oH, T calving_rate, const T* eta_cache, hipStream_t st);

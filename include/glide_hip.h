@@ -210,6 +210,13 @@ int glide_adjoint_fused_enable(int on);
  * buffers, the level's own and its workspace); 0 solves into the workspace
  * and applies in a second launch.  Both give the same state. */
 int glide_forward_fused_enable(int on);
+/* Cached adjoint smoothing of levels of at most 512 cells with linear drag
+ * (no counterpart in the reference): 1 (default) runs all passes of one call
+ * in a single workgroup (transpose operator, patch solves and update per pass,
+ * separated by workgroup barriers), when the device's LDS per block holds the
+ * level; 0 launches the transpose operator and the adjoint pass per pass.
+ * Both give the same multipliers. */
+int glide_adjoint_coarse_enable(int on);
 
 /* Measurement support (no counterpart in the reference): when enabled, HIP
  * event pairs are recorded around every launch of the hot kernels

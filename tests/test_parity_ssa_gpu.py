@@ -345,6 +345,36 @@ def test_adjoint_cache_matches_uncached_passes(orc, shape):
         assert rel_err(x, y) < 1e-13
 
 
+@pytest.mark.parametrize("shape", [(16, 32), (17, 29), (8, 6)])
+def test_adjoint_coarse_workgroup_matches_launches(shape):
+    """All cached adjoint passes of a level of at most 512 cells in one
+    workgroup agree with the transpose operator and the fused pass launched
+    once per pass (linear drag, mask and calving on, non-square levels)."""
+    from glide_amd import _lib
+    ny, nx = shape
+    print(f"adjoint coarse workgroup shape={shape}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=21, mask_frac=0.15)
+    out = []
+    for coarse in (True, False):
+        _lib.adjoint_coarse_enable(coarse)
+        g = grid_from(F)
+        g.sliding.m.set(1.0)
+        g.calving.calving_rate.set(0.3)
+        for t, x in zip((g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H), _lam(F, 22)):
+            t.set(x)
+        ao = g.adjoint_operators
+        ao.use_cache = True
+        rng = np.random.default_rng(23)
+        for t in (ao.f_u, ao.f_v, ao.f_H):
+            t.copy_(T(rng.standard_normal(tuple(t.shape))))
+        ao.vanka_sweep(10.0, 7)
+        torch.cuda.synchronize()
+        out.append([N(t.data) for t in (g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H)])
+    _lib.adjoint_coarse_enable(True)
+    for x, y in zip(out[0], out[1]):
+        assert rel_err(x, y) < 1e-13
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_gradients(orc, shape):
     ny, nx = shape
