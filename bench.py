@@ -153,15 +153,19 @@ class Stepper:
     The enthalpy step and its adjoint depend only on the forward solution, and
     the momentum adjoint only on the forward state and the rate factor the
     forward used, so with ``concurrent`` the enthalpy work runs on a second
-    stream while the adjoint's latency-bound coarse levels leave the GPU
-    mostly idle; the new rate factor is written to the grid after both."""
+    stream beside the momentum adjoint; the new rate factor is written to the
+    grid after both.  The step runs on a stream of its own rather than the
+    null stream, whose implicit synchronisation with blocking streams could
+    serialise the two."""
 
     def __init__(self, model, thermal, torch, adj_rtol=1e-8):
         self.model, self.thermal, self.torch = model, thermal, torch
         self.adj_rtol = adj_rtol
+        dev = model.mg[0].state.u.data.device
+        self.main = torch.cuda.Stream(device=dev)
         if thermal is not None:
             g = model.mg[0]
-            self.stream = torch.cuda.Stream(device=g.state.u.data.device)
+            self.stream = torch.cuda.Stream(device=dev)
             self.B_next = torch.empty_like(g.rheology.B.data)
             self.dL_dB = torch.ones_like(g.rheology.B.data)
             self.ev_fwd, self.ev_thermal = torch.cuda.Event(), torch.cuda.Event()
@@ -180,6 +184,13 @@ class Stepper:
             self.adjoint_sweeps = res["report"].sweeps
 
     def __call__(self, t, dt, concurrent=True, report=False):
+        caller = self.torch.cuda.current_stream()
+        self.main.wait_stream(caller)
+        with self.torch.cuda.stream(self.main):
+            self._step(t, dt, concurrent, report)
+        caller.wait_stream(self.main)
+
+    def _step(self, t, dt, concurrent, report):
         torch, model, thermal = self.torch, self.model, self.thermal
         if thermal is not None:
             thermal.pre_momentum()

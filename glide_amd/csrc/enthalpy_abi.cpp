@@ -50,7 +50,7 @@ int fail(int code, const std::string& msg) {
 struct EnthScratch {
     double* partials = nullptr;   // kEnthResBlocks
     double* max_out = nullptr;    // 1
-    unsigned long long* acc = nullptr;  // adjoint passes' max accumulator, kept zero between uses
+    unsigned long long* acc = nullptr;  // adjoint passes' maximum and block count, kept zero between uses
     EnthStatus* status = nullptr;
     EnthStatus* host = nullptr;   // pinned mirror
 };
@@ -67,8 +67,8 @@ int scratch(EnthScratch** out) {
     if (!s.partials) {
         HIP_TRY(hipMalloc(&s.partials, sizeof(double) * kEnthResBlocks), "hipMalloc(enthalpy partials)");
         HIP_TRY(hipMalloc(&s.max_out, sizeof(double)), "hipMalloc(enthalpy max)");
-        HIP_TRY(hipMalloc(&s.acc, sizeof(unsigned long long)), "hipMalloc(enthalpy accumulator)");
-        HIP_TRY(hipMemset(s.acc, 0, sizeof(unsigned long long)), "hipMemset(enthalpy accumulator)");
+        HIP_TRY(hipMalloc(&s.acc, 2 * sizeof(unsigned long long)), "hipMalloc(enthalpy accumulator)");
+        HIP_TRY(hipMemset(s.acc, 0, 2 * sizeof(unsigned long long)), "hipMemset(enthalpy accumulator)");
         HIP_TRY(hipMalloc(&s.status, sizeof(EnthStatus)), "hipMalloc(enthalpy status)");
         HIP_TRY(hipHostMalloc(&s.host, sizeof(EnthStatus), hipHostMallocDefault), "hipHostMalloc(enthalpy status)");
     }

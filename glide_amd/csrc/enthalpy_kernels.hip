@@ -333,18 +333,6 @@ __global__ __launch_bounds__(256) void k_enth_reduce(const double* __restrict__ 
     if (status) record_sweep(status, mode, r, rtol, atol);
 }
 
-// The adjoint sweeps' maxima arrive in one accumulator (bit patterns of
-// non-negative doubles, combined by atomic max): read it into the
-// bookkeeping and clear it for the next pass.
-__global__ void k_enth_adj_close(unsigned long long* __restrict__ acc, EnthStatus* status, int mode, double rtol,
-                                 double atol) {
-    if (mode == 1 && status->done) return;  // the skipped passes left it zero
-    const double r = __longlong_as_double((long long)*acc);
-    *acc = 0ull;
-    record_sweep(status, mode, r, rtol, atol);
-}
-
-
 // Column Newton smoother (glide/cuda/enthalpy.cu:932-1112): Newton on the
 // column with horizontal neighbours frozen at E; writes delta = E_col - E
 // and/or E_out = E + omega_relax * delta.
@@ -755,7 +743,9 @@ __device__ __forceinline__ AdjHoriz<T> transposed_couplings(const EnthArgs<T>& A
 // J_{k+1,k} / w_k; planes 3-6 the transposed horizontal couplings (left,
 // right, upper, lower neighbour) per level, or, with a depth-constant
 // velocity, once per column on level 0.  Identity rows on surface nodes and
-// thin columns.
+// thin columns.  (Staging the columns through LDS, as k_enth_adj_layout does,
+// measured slower here: the per-level Jacobian work wants more waves per CU
+// than the staging space allows.)
 template <typename T>
 __global__ __launch_bounds__(256) void k_enth_adj_factor(EnthArgs<T> A, T* __restrict__ rows) {
     const int ny = A.ny, nx = A.nx, nz = A.nz, hw = (nx + 1) / 2;
@@ -832,11 +822,15 @@ constexpr int kAdjChunk = 4;  // levels whose loads a thread issues together
 // (branch-free: a missing neighbour reads the thread's own node against a zero
 // coupling), so that several memory latencies overlap down the column.  Byte
 // offsets within one checkerboard plane fit 32 bits (checked by the host).
+// acc[0] collects the maximum (bit patterns of non-negative doubles, atomic
+// max), acc[1] counts finished blocks; with close_mode >= 0 the last block
+// records the sweep (record_sweep) and clears both.
 template <typename T, int NZB, bool SOLVE>
 __global__ __launch_bounds__(256) void k_enth_adj_pass(const int* __restrict__ skip, int ny, int nx, int nz, int flat,
                                                        const T* __restrict__ rows, const T* __restrict__ g,
                                                        T* __restrict__ lam, T omega_relax, int color,
-                                                       unsigned long long* __restrict__ acc) {
+                                                       unsigned long long* acc, EnthStatus* status,
+                                                       int close_mode, double rtol, double atol) {
     constexpr int KC = kAdjChunk;
     __shared__ double smax[4];
     if (skipped(skip)) return;
@@ -941,25 +935,59 @@ __global__ __launch_bounds__(256) void k_enth_adj_pass(const int* __restrict__ s
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
     __syncthreads();
-    // non-negative doubles order as their bit patterns
-    if (threadIdx.x == 0)
+    if (threadIdx.x != 0) return;
+    const unsigned long long prev =
         atomicMax(acc, (unsigned long long)__double_as_longlong(fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]))));
+    if (close_mode < 0) return;
+    // waiting for the returned value orders this block's maximum before its
+    // count; all of them are device-scope atomics on the same two words
+    asm volatile("" ::"v"(prev));
+    if (atomicAdd(acc + 1, 1ull) != (unsigned long long)gridDim.x - 1) return;
+    const double r = __longlong_as_double((long long)atomicExch(acc, 0ull));
+    atomicExch(acc + 1, 0ull);
+    record_sweep(status, close_mode, r, rtol, atol);
+}
+
+// Threads (columns) per block of k_enth_adj_layout: the block's columns are
+// one contiguous chunk of the (ny, nx, nz) layout, staged through LDS by
+// coalesced loads.
+constexpr int kStageCols = 64;
+
+// Columns [c0, c0 + ncols) of a (ny, nx, nz) array into s[q * nz + k].
+template <typename T>
+__device__ __forceinline__ void stage_columns(T* s, const T* __restrict__ src, long c0, int ncols, int nz) {
+    const long base = c0 * nz;
+    const int n = ncols * nz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) s[q] = src[base + q];
 }
 
 // lam between the (ny, nx, nz) layout and the checkerboard layout, one thread
-// per column.
+// per column, the (ny, nx, nz) side staged through LDS (kStageCols * nz
+// doubles of dynamic LDS).
 template <typename T>
-__global__ __launch_bounds__(256) void k_enth_adj_layout(int ny, int nx, int nz, const T* __restrict__ src,
-                                                         T* __restrict__ dst, int to_cb) {
+__global__ __launch_bounds__(kStageCols) void k_enth_adj_layout(int ny, int nx, int nz, const T* __restrict__ src,
+                                                                T* __restrict__ dst, int to_cb) {
+    extern __shared__ double lds[];
+    T* sc = reinterpret_cast<T*>(lds);
     const int hw = (nx + 1) / 2;
     const long ncol = (long)ny * nx, LS = (long)ny * hw;
-    for (long col = (long)blockIdx.x * 256 + threadIdx.x; col < ncol; col += (long)gridDim.x * 256) {
-        const int i = (int)(col / nx), j = (int)(col % nx);
-        const long o = cb_base((i + j) & 1, i, j >> 1, ny, nz, hw), base = col * nz;
-        for (int k = 0; k < nz; ++k) {
-            if (to_cb) dst[o + k * LS] = src[base + k];
-            else dst[base + k] = src[o + k * LS];
-        }
+    const long c0 = (long)blockIdx.x * kStageCols;
+    const int ncols = (int)(ncol - c0 < kStageCols ? ncol - c0 : kStageCols);
+    const long col = c0 + threadIdx.x;
+    const bool live = (int)threadIdx.x < ncols;
+    const int i = (int)(col / nx), j = (int)(col % nx);
+    const long o = cb_base((i + j) & 1, i, j >> 1, ny, nz, hw);
+    if (to_cb) {
+        stage_columns(sc, src, c0, ncols, nz);
+        __syncthreads();
+        if (live)
+            for (int k = 0; k < nz; ++k) dst[o + k * LS] = sc[threadIdx.x * nz + k];
+    } else {
+        if (live)
+            for (int k = 0; k < nz; ++k) sc[threadIdx.x * nz + k] = src[o + k * LS];
+        __syncthreads();
+        const long base = c0 * nz;
+        for (int q = threadIdx.x; q < ncols * nz; q += blockDim.x) dst[base + q] = sc[q];
     }
 }
 
@@ -1151,50 +1179,55 @@ hipError_t launch_enth_adj_rhs(const EnthArgs<T>& A, const T* gE, const T* gB, T
 }
 template <typename T>
 hipError_t launch_enth_adj_layout(int ny, int nx, int nz, const T* src, T* dst, bool to_cb, hipStream_t st) {
-    k_enth_adj_layout<T><<<flat_blocks((long)ny * nx), 256, 0, st>>>(ny, nx, nz, src, dst, to_cb ? 1 : 0);
+    const unsigned nb = (unsigned)(((long)ny * nx + kStageCols - 1) / kStageCols);
+    k_enth_adj_layout<T><<<nb, kStageCols, kStageCols * nz * sizeof(T), st>>>(ny, nx, nz, src, dst, to_cb ? 1 : 0);
     return hipGetLastError();
 }
 
 template <typename T, bool SOLVE>
 static void adj_pass(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g, T* lam,
-                     T omega_relax, int color, unsigned long long* acc, hipStream_t st) {
+                     T omega_relax, int color, unsigned long long* acc, EnthStatus* status, int close_mode,
+                     double rtol, double atol, hipStream_t st) {
     const unsigned nb = (unsigned)(((long)A.ny * ((A.nx + 1) / 2) + 255) / 256);
     const int ny = A.ny, nx = A.nx, nz = A.nz, f = flat ? 1 : 0;
-    if (nz <= 16)
-        k_enth_adj_pass<T, 16, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc);
-    else if (nz <= 32)
-        k_enth_adj_pass<T, 32, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc);
-    else if (nz <= 48)
-        k_enth_adj_pass<T, 48, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc);
-    else
-        k_enth_adj_pass<T, 64, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc);
+#define GLIDE_ADJ_PASS(NZB)                                                                                        \
+    k_enth_adj_pass<T, NZB, SOLVE><<<nb, 256, 0, st>>>(skip, ny, nx, nz, f, rows, g, lam, omega_relax, color, acc, \
+                                                        status, close_mode, rtol, atol)
+    if (nz <= 16) GLIDE_ADJ_PASS(16);
+    else if (nz <= 32) GLIDE_ADJ_PASS(32);
+    else if (nz <= 48) GLIDE_ADJ_PASS(48);
+    else GLIDE_ADJ_PASS(64);
+#undef GLIDE_ADJ_PASS
+}
+// The passes address a checkerboard plane with 32-bit byte offsets.
+template <typename T>
+static bool adj_shape_ok(const EnthArgs<T>& A) {
+    return A.nz <= 64 && 2L * A.nz * A.ny * ((A.nx + 1) / 2) * (long)sizeof(T) < (1L << 32);
 }
 template <typename T>
 hipError_t launch_enth_adj_sweep_rb(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
                                     T* lam, T omega_relax, unsigned long long* acc, EnthStatus* status, double rtol,
                                     double atol, hipStream_t st) {
-    if (A.nz > 64) return hipErrorInvalidValue;
+    if (!adj_shape_ok(A)) return hipErrorInvalidValue;
     const long half = (long)A.ny * A.nx * A.nz / 2;
     {
         TimedLaunch tl(kTimeEnthColumn, half, st);
-        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 0, acc, st);
+        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 0, acc, status, -1, rtol, atol, st);
     }
     {
         TimedLaunch tl(kTimeEnthColumn, half, st);
-        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 1, acc, st);
+        adj_pass<T, true>(skip, A, flat, rows, g, lam, omega_relax, 1, acc, status, 1, rtol, atol, st);
     }
-    k_enth_adj_close<<<1, 1, 0, st>>>(acc, status, 1, rtol, atol);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_enth_adj_residual(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
                                     const T* lam, unsigned long long* acc, EnthStatus* status, int status_mode,
                                     double rtol, double atol, hipStream_t st) {
-    if (A.nz > 64) return hipErrorInvalidValue;
+    if (!adj_shape_ok(A)) return hipErrorInvalidValue;
     T* l = const_cast<T*>(lam);  // the residual pass does not write lam
-    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 0, acc, st);
-    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 1, acc, st);
-    k_enth_adj_close<<<1, 1, 0, st>>>(acc, status, status_mode, rtol, atol);
+    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 0, acc, status, -1, rtol, atol, st);
+    adj_pass<T, false>(skip, A, flat, rows, g, l, T(1), 1, acc, status, status_mode, rtol, atol, st);
     return hipGetLastError();
 }
 template <typename T>

@@ -104,7 +104,7 @@ hipError_t launch_enth_adj_sweep_rb(const int* skip, const EnthArgs<T>& A, bool 
                                     T* lam, T omega_relax, unsigned long long* acc, EnthStatus* status, double rtol,
                                     double atol, hipStream_t st);
 // max |g - J^T lam| into the sweep bookkeeping (status mode 0 starts it).
-// acc: the passes' max accumulator, zero on entry and left zero.
+// acc: two counters (maximum, finished blocks), zero on entry and left zero.
 template <typename T>
 hipError_t launch_enth_adj_residual(const int* skip, const EnthArgs<T>& A, bool flat, const T* rows, const T* g,
                                     const T* lam, unsigned long long* acc, EnthStatus* status, int status_mode,
