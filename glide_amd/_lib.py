@@ -143,6 +143,7 @@ SIGNATURES = {
     "glide_adjoint_fused_enable": (_I, [_I]),
     "glide_forward_fused_enable": (_I, [_I]),
     "glide_adjoint_coarse_enable": (_I, [_I]),
+    "glide_adjoint_pass_fused_enable": (_I, [_I]),
     "glide_graphs_clear": (_I, []),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -192,6 +193,12 @@ def adjoint_coarse_enable(on: bool) -> None:
     """Cached adjoint passes of levels of at most 512 cells (linear drag) in
     one workgroup (default) or one launch pair per pass."""
     call("glide_adjoint_coarse_enable", int(bool(on)))
+
+
+def adjoint_pass_fused_enable(on: bool) -> None:
+    """Cached adjoint passes of levels up to 512 x 512 cells (linear drag) as one
+    launch each, transpose operator included (default), or two launches."""
+    call("glide_adjoint_pass_fused_enable", int(bool(on)))
 
 
 def graphs_clear() -> None:

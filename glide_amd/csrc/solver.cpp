@@ -158,6 +158,9 @@ bool g_fused_forward = true;
 // Cached adjoint passes of levels of at most 512 cells as one workgroup
 // (glide_adjoint_coarse_enable).
 bool g_coarse_adjoint = true;
+// Cached adjoint passes of levels up to kAdjointPassFusedMaxCells as one launch
+// each, transpose operator included (glide_adjoint_pass_fused_enable).
+bool g_pass_fused_adjoint = true;
 
 int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
                  hipStream_t st) {
@@ -254,6 +257,17 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
             HIP_TRY(launch_adjoint_coarse<R>(a, cache, P(L.lambda_u), P(L.lambda_v), P(L.lambda_H), CP(L.af_u),
                                              CP(L.af_v), CP(L.af_H), vk.calving_rate, vc.omega, n_iter, st),
                     "adjoint passes in one workgroup");
+            return GLIDE_OK;
+        }
+    }
+    if (cache && g_pass_fused_adjoint && (long)L.ny * L.nx <= kAdjointPassFusedMaxCells) {
+        SsaArgs<R> a = make_args(L, dt, true);
+        if (a.P.linear_drag) {
+            for (int it = 0; it < n_iter; ++it)
+                HIP_TRY(launch_adjoint_pass_fused<R>(a, cache, P(L.lambda_u), P(L.lambda_v), P(L.lambda_H),
+                                                     CP(L.af_u), CP(L.af_v), CP(L.af_H), vk.calving_rate, vc.omega,
+                                                     st),
+                        "adjoint pass");
             return GLIDE_OK;
         }
     }
@@ -554,6 +568,7 @@ std::string solve_key(char tag, const FasCtx& X, int start, bool norms) {
     key_append(k, g_fused_forward);
     key_append(k, g_fused_adjoint);
     key_append(k, g_coarse_adjoint);
+    key_append(k, g_pass_fused_adjoint);
     key_append(k, X.n_levels);
     key_append(k, start);
     key_append(k, X.dt);
@@ -894,6 +909,11 @@ int glide_forward_fused_enable(int on) {
 
 int glide_adjoint_coarse_enable(int on) {
     g_coarse_adjoint = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_adjoint_pass_fused_enable(int on) {
+    g_pass_fused_adjoint = on != 0;
     return GLIDE_OK;
 }
 

@@ -704,14 +704,11 @@ __global__ __launch_bounds__(NT) void k_adjoint_factor(SsaArgs<T> a, T* __restri
 // Increments of cell (i,j) from its cached factors and the residual r:
 // w = (d0/2, d1/2, d2/2, d3/2, d4) with J^T d = r (half-weighted facets).
 template <typename T>
-__device__ __forceinline__ void adjoint_cached_increments(const T* __restrict__ cache, const T* __restrict__ ru,
-                                                          const T* __restrict__ rv, const T* __restrict__ rH, int i,
-                                                          int j, int ny, int nx, T w[5]) {
-    const long n = (long)ny * nx, c = (long)i * nx + j;
+__device__ __forceinline__ void adjoint_cached_solve(const T* __restrict__ cache, long n, long c, const T b[5],
+                                                     T w[5]) {
     T A[25];
 #pragma unroll
     for (int q = 0; q < 25; ++q) A[q] = cache[q * n + c];
-    const T b[5] = {ru[(long)i * (nx + 1) + j], ru[(long)i * (nx + 1) + j + 1], rv[c], rv[c + nx], rH[c]};
     T y[5], d[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -732,6 +729,14 @@ __device__ __forceinline__ void adjoint_cached_increments(const T* __restrict__ 
     w[2] = T(0.5) * d[2];
     w[3] = T(0.5) * d[3];
     w[4] = d[4];
+}
+template <typename T>
+__device__ __forceinline__ void adjoint_cached_increments(const T* __restrict__ cache, const T* __restrict__ ru,
+                                                          const T* __restrict__ rv, const T* __restrict__ rH, int i,
+                                                          int j, int ny, int nx, T w[5]) {
+    const long n = (long)ny * nx, c = (long)i * nx + j;
+    const T b[5] = {ru[(long)i * (nx + 1) + j], ru[(long)i * (nx + 1) + j + 1], rv[c], rv[c + nx], rH[c]};
+    adjoint_cached_solve(cache, n, c, b, w);
 }
 
 // Adjoint patch solve from the cached factors: the increments of
@@ -838,14 +843,15 @@ struct VjpTile {
 };
 
 // The transpose operator for the outputs (facets and cells) i0 <= i < i0 +
-// rows, j0 <= j < j0 + cols, by nthr threads (tid = 0 .. nthr - 1) of one
-// block, through the block storage S (cell arrays over the one-cell ring,
-// vertex arrays, and with S::kDrag the drag cross-terms; without it the drag
-// must be linear, whose cross-terms vanish).
-template <typename T, typename S>
+// rows, j0 <= j < j0 + cols inside the grid, by nthr threads (tid = 0 ..
+// nthr - 1) of one block, through the block storage S (cell arrays over the
+// one-cell ring, vertex arrays, and with S::kDrag the drag cross-terms;
+// without it the drag must be linear, whose cross-terms vanish), each output
+// handed to Out with its tile position and global index.
+template <typename T, typename S, typename Out>
 __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restrict__ lu, const T* __restrict__ lv,
                                          const T* __restrict__ lH, const T* __restrict__ fu, const T* __restrict__ fv,
-                                         const T* __restrict__ fH, T* out_u, T* out_v, T* out_H, T calving_rate,
+                                         const T* __restrict__ fH, Out& out, T calving_rate,
                                          const T* __restrict__ eta_cache, int i0, int j0, int rows, int cols, int tid,
                                          int nthr, S& s) {
     const int ny = a.ny, nx = a.nx;
@@ -935,7 +941,7 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
     for (int k = tid; k < rows * cols; k += nthr) {
         const int ti = k / cols, tj = k % cols;
         const int i = i0 + ti, j = j0 + tj;
-        if (i > ny || j > nx) continue;
+        if (i < 0 || j < 0 || i > ny || j > nx) continue;
         const int ci = ti + 1, cj = tj + 1;  // cell (i,j) in the ring tile
         const int vi = ti, vj = tj;          // vertex (i,j) in the vertex tile
         if (i < ny) {  // u facet (i,j)
@@ -955,7 +961,7 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                 acc += f.d_w * (s.lHm(ci, cj - 1) - s.lHm(ci, cj)) * dxi;
                 if (fu) acc -= fu[o];
             }
-            out_u[o] = acc;
+            out.put_u(ti, tj, o, acc);
         }
         if (j < nx) {  // v facet (i,j)
             const long o = (long)i * nx + j;
@@ -973,7 +979,7 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                 acc += f.d_w * (s.lHm(ci, cj) - s.lHm(ci - 1, cj)) * dxi;
                 if (fv) acc -= fv[o];
             }
-            out_v[o] = acc;
+            out.put_v(ti, tj, o, acc);
         }
         if (i < ny && j < nx) {  // cell (i,j)
             const long c = (long)i * nx + j;
@@ -1033,7 +1039,7 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                 }
                 if (fH) acc -= fH[c];
             }
-            out_H[c] = acc;
+            out.put_h(ti, tj, c, acc);
         }
     }
 }
@@ -1077,6 +1083,25 @@ struct VjpLevelStore {
     __device__ T& Pv(int, int, int) { return cell[0]; }
 };
 
+// Output sinks of vjp_body: arrays in the grid's own layouts, or tile-local
+// LDS arrays indexed by the output's position in the tile.
+template <typename T>
+struct VjpGridOut {
+    T *u, *v, *h;
+    __device__ void put_u(int, int, long o, T x) { u[o] = x; }
+    __device__ void put_v(int, int, long o, T x) { v[o] = x; }
+    __device__ void put_h(int, int, long c, T x) { h[c] = x; }
+};
+template <typename T, int RC>
+struct VjpTileOut {
+    T (*u)[RC];
+    T (*v)[RC];
+    T (*h)[RC];
+    __device__ void put_u(int ti, int tj, long, T x) { u[ti][tj] = x; }
+    __device__ void put_v(int ti, int tj, long, T x) { v[ti][tj] = x; }
+    __device__ void put_h(int ti, int tj, long, T x) { h[ti][tj] = x; }
+};
+
 template <typename T, int ROWS>
 __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __restrict__ lu, const T* __restrict__ lv,
                                             const T* __restrict__ lH, const T* __restrict__ fu,
@@ -1085,8 +1110,104 @@ __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __rest
                                             const T* __restrict__ eta_cache) {
     __shared__ VjpTile<T, ROWS> t;
     VjpTileStore<T, ROWS> s{t};
-    vjp_body(a, lu, lv, lH, fu, fv, fH, out_u, out_v, out_H, calving_rate, eta_cache, (int)blockIdx.y * ROWS,
-             (int)blockIdx.x * TX, ROWS, TX, (int)(threadIdx.y * TX + threadIdx.x), TX * ROWS, s);
+    VjpGridOut<T> o{out_u, out_v, out_H};
+    vjp_body(a, lu, lv, lH, fu, fv, fH, o, calving_rate, eta_cache, (int)blockIdx.y * ROWS, (int)blockIdx.x * TX,
+             ROWS, TX, (int)(threadIdx.y * TX + threadIdx.x), TX * ROWS, s);
+}
+
+// A whole cached adjoint pass in one launch (linear drag): the transpose
+// operator minus the forcing for the facets and cells of the tile and of its
+// left column and top row (vjp_body over rows i0 - 1 .. i0 + TY and columns
+// j0 - 1 .. j0 + TX, into LDS), the cached patch solves of the tile's cells
+// and of the cells left of and above it, then the in-place update of
+// k_vanka_adjoint_fused.  The same arithmetic as k_vjp followed by
+// k_vanka_adjoint_fused; it recomputes the ring's residuals (1.4x the
+// transpose operator's work) and saves a launch, the residual planes' round
+// trip through HBM and the second kernel's ramp: 9.7 against 12.6 us per
+// pass at 32², 14.6 against 19.8 us at 256², and still ahead at 512².
+constexpr int kPassFusedRows = 2 * TY;  // thread rows: the ring's patch solves run beside the tile's
+template <typename T>
+__global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaArgs<T> a, const T* __restrict__ cache,
+                                                                           T* lu, T* lv, T* lH,
+                                                                           const T* __restrict__ fu,
+                                                                           const T* __restrict__ fv,
+                                                                           const T* __restrict__ fH, T calving_rate,
+                                                                           T omega) {
+    constexpr int RR = TY + 2, RC = TX + 2;
+    constexpr int CW = RC + 2, NCELL = (RR + 2) * CW, VW = RC + 1, NVERT = (RR + 1) * VW;
+    __shared__ T s_cell[8 * NCELL];
+    __shared__ T s_vert[2 * NVERT];
+    __shared__ T r_u[RR][RC], r_v[RR][RC], r_h[RR][RC];
+    __shared__ T s_w1[TY][TX + 1];  // right-facet increment of cells (i0 + a, j0 - 1 + b)
+    __shared__ T s_w3[TY + 1][TX];  // bottom-facet increment of cells (i0 - 1 + a, j0 + b)
+    VjpLevelStore<T> store;
+    store.cell = s_cell;
+    store.vert = s_vert;
+    store.cw = CW;
+    store.nc = NCELL;
+    store.vw = VW;
+    store.nv = NVERT;
+    VjpTileOut<T, RC> out{r_u, r_v, r_h};
+    const int ny = a.ny, nx = a.nx;
+    const long n = (long)ny * nx;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int ty = threadIdx.y, tx = threadIdx.x, tid = ty * TX + tx;
+    constexpr int NTF = TX * kPassFusedRows;
+    vjp_body(a, lu, lv, lH, fu, fv, fH, out, calving_rate, cache + (long)kAdjointCacheEta * n, i0 - 1, j0 - 1, RR,
+             RC, tid, NTF, store);
+    __syncthreads();
+    auto increments = [&](int ci, int cj, T w[5]) {
+        const int ti = ci - i0 + 1, tj = cj - j0 + 1;
+        const T b[5] = {r_u[ti][tj], r_u[ti][tj + 1], r_v[ti][tj], r_v[ti + 1][tj], r_h[ti][tj]};
+        adjoint_cached_solve(cache, n, (long)ci * nx + cj, b, w);
+    };
+    // thread rows 0 .. TY-1 own the tile's cells, the rest solve the ring
+    const int i = i0 + ty, j = j0 + tx;
+    const bool own = ty < TY && i < ny && j < nx;
+    T w[5] = {T(0), T(0), T(0), T(0), T(0)};
+    if (own) {
+        increments(i, j, w);
+        s_w1[ty][tx + 1] = w[1];
+        s_w3[ty + 1][tx] = w[3];
+    }
+    const int rt = tid - TY * TX;
+    if (rt >= 0 && rt < TY) {
+        const int ci = i0 + rt, cj = j0 - 1;
+        if (cj >= 0 && ci < ny) {
+            T wr[5];
+            increments(ci, cj, wr);
+            s_w1[rt][0] = wr[1];
+        }
+    } else if (rt >= TY && rt < TY + TX) {
+        const int ci = i0 - 1, cj = j0 + (rt - TY);
+        if (ci >= 0 && cj < nx) {
+            T wr[5];
+            increments(ci, cj, wr);
+            s_w3[0][rt - TY] = wr[3];
+        }
+    }
+    __syncthreads();
+    if (!own) return;
+    const long c = (long)i * nx + j;
+    {
+        T d = w[0];
+        if (j > 0) d = d + s_w1[ty][tx];
+        lu[(long)i * (nx + 1) + j] += omega * d;
+        if (j == nx - 1) {
+            T e = T(0) + w[1];
+            lu[(long)i * (nx + 1) + nx] += omega * e;
+        }
+    }
+    {
+        T d = w[2];
+        if (i > 0) d = d + s_w3[ty][tx];
+        lv[c] += omega * d;
+        if (i == ny - 1) {
+            T e = T(0) + w[3];
+            lv[(long)ny * nx + j] += omega * e;
+        }
+    }
+    lH[c] += omega * w[4];
 }
 
 // Doubles of dynamic LDS k_adjoint_coarse needs for an ny x nx level.
@@ -1123,8 +1244,9 @@ __global__ __launch_bounds__(kCoarseThreads) void k_adjoint_coarse(SsaArgs<T> a,
     T* rH = rv + (ny + 1) * nx;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const T* eta_cache = cache + (long)kAdjointCacheEta * n;
+    VjpGridOut<T> out{ru, rv, rH};
     for (int it = 0; it < n_iter; ++it) {
-        vjp_body(a, lu, lv, lH, fu, fv, fH, ru, rv, rH, calving_rate, eta_cache, 0, 0, rows, cols, tid, nthr, s);
+        vjp_body(a, lu, lv, lH, fu, fv, fH, out, calving_rate, eta_cache, 0, 0, rows, cols, tid, nthr, s);
         __syncthreads();
         // the cell arrays of the transpose operator are free until the next
         // pass: the increments w0..w4 of cell c go to W[q * n + c]
@@ -1336,6 +1458,15 @@ hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH
 }
 
 template <typename T>
+hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, const T* fu,
+                                     const T* fv, const T* fH, T calving_rate, T omega, hipStream_t st) {
+    if (!a.P.linear_drag) return hipErrorInvalidValue;
+    TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx, st);
+    k_adjoint_pass_fused<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, kPassFusedRows), 0, st>>>(
+        a, cache, lu, lv, lH, fu, fv, fH, calving_rate, -omega);
+    return hipGetLastError();
+}
+template <typename T>
 bool adjoint_coarse_fits(const SsaArgs<T>& a) {
     if (!a.P.linear_drag || a.ny < 1 || a.nx < 1 || (long)a.ny * a.nx > kAdjointCoarseMaxCells) return false;
     int dev = 0, lds = 0;
@@ -1402,6 +1533,8 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_vanka_fused<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, hipStream_t);\
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
+    template hipError_t launch_adjoint_pass_fused<T>(const SsaArgs<T>&, const T*, T*, T*, T*, const T*, const T*,      \
+                                                     const T*, T, T, hipStream_t);                                   \
     template bool adjoint_coarse_fits<T>(const SsaArgs<T>&);                                                         \
     template hipError_t launch_adjoint_coarse<T>(const SsaArgs<T>&, const T*, T*, T*, T*, const T*, const T*,          \
                                                  const T*, T, T, int, hipStream_t);                                  \

@@ -58,6 +58,16 @@ hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* 
                                      T* work, const VankaArgs<T>& vk, T omega, hipStream_t st);
 // eta_cache: the cell viscosity cached by launch_adjoint_factor for the
 // current state, or nullptr to compute it.
+// One cached adjoint pass (transpose operator, patch solves and update) in one
+// launch; linear drag only.  The same results as launch_vjp +
+// launch_vanka_adjoint_pass_cached; used on levels up to kAdjointPassFusedMaxCells.
+template <typename T>
+hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, const T* fu,
+                                     const T* fv, const T* fH, T calving_rate, T omega, hipStream_t st);
+#ifndef GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE
+#define GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE 512
+#endif
+constexpr long kAdjointPassFusedMaxCells = (long)GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE * GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE;
 // All n_iter cached adjoint passes of a small level in one workgroup (linear
 // drag, ny * nx <= kAdjointCoarseMaxCells, its LDS within the device limit:
 // adjoint_coarse_fits); the same results as n_iter launch_vjp +

@@ -217,6 +217,12 @@ int glide_forward_fused_enable(int on);
  * level; 0 launches the transpose operator and the adjoint pass per pass.
  * Both give the same multipliers. */
 int glide_adjoint_coarse_enable(int on);
+/* Cached adjoint passes of levels of at most 512 x 512 cells with linear drag
+ * (no counterpart in the reference): 1 (default) runs each pass, the
+ * transpose operator included, as one launch (a tile recomputes its left and
+ * top neighbours' residuals); 0 launches the transpose operator and the
+ * adjoint pass separately.  Both give the same multipliers. */
+int glide_adjoint_pass_fused_enable(int on);
 
 /* Measurement support (no counterpart in the reference): when enabled, HIP
  * event pairs are recorded around every launch of the hot kernels
