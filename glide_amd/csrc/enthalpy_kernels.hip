@@ -217,7 +217,10 @@ __device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, int k, T h, T 
 
 // Parallel cyclic reduction over the wavefront: lane k holds row k,
 // a x_{k-1} + b x_k + c x_{k+1} = d, with identity rows beyond the column;
-// returns x_k.  Six doubling steps cover 64 rows.
+// returns x_k.  Six doubling steps cover 64 rows.  Each lane forms the
+// reciprocal of its own pivot once per step (refined hardware estimate, within
+// an ulp or two of the IEEE quotient) and the neighbours read that instead of
+// dividing by the shuffled pivot.
 template <typename T>
 __device__ __forceinline__ T pcr_solve(T a, T b, T c, T d) {
     const int k = lane_id();
@@ -225,10 +228,11 @@ __device__ __forceinline__ T pcr_solve(T a, T b, T c, T d) {
     for (int s = 1; s < 64; s <<= 1) {
         const bool lo = k - s >= 0, hi = k + s < 64;
         const int km = lo ? k - s : k, kp = hi ? k + s : k;
-        T am = lane_get(a, km), bm = lane_get(b, km), cm = lane_get(c, km), dm = lane_get(d, km);
-        T ap = lane_get(a, kp), bp = lane_get(b, kp), cp = lane_get(c, kp), dp = lane_get(d, kp);
-        T al = lo ? -a / bm : T(0);
-        T ga = hi ? -c / bp : T(0);
+        const T rb = fast_rcp(b);
+        T am = lane_get(a, km), rm = lane_get(rb, km), cm = lane_get(c, km), dm = lane_get(d, km);
+        T ap = lane_get(a, kp), rp = lane_get(rb, kp), cp = lane_get(c, kp), dp = lane_get(d, kp);
+        T al = lo ? -a * rm : T(0);
+        T ga = hi ? -c * rp : T(0);
         T na = al * am, nc = ga * cp;
         T nb = b + al * cm + ga * ap;
         T nd = d + al * dm + ga * dp;
@@ -237,7 +241,7 @@ __device__ __forceinline__ T pcr_solve(T a, T b, T c, T d) {
         c = nc;
         d = nd;
     }
-    return d / b;
+    return d * fast_rcp(b);
 }
 
 // Column handled by a wavefront.
