@@ -263,11 +263,29 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
     if (cache && g_pass_fused_adjoint && (long)L.ny * L.nx <= kAdjointPassFusedMaxCells) {
         SsaArgs<R> a = make_args(L, dt, true);
         if (a.P.linear_drag) {
-            for (int it = 0; it < n_iter; ++it)
-                HIP_TRY(launch_adjoint_pass_fused<R>(a, cache, P(L.lambda_u), P(L.lambda_v), P(L.lambda_H),
-                                                     CP(L.af_u), CP(L.af_v), CP(L.af_H), vk.calving_rate, vc.omega,
-                                                     st),
+            // the multipliers alternate between the level's buffers and its
+            // 5-plane workspace (neighbouring tiles read them during a pass);
+            // an odd pass count copies them back
+            R* alt_u = P(L.work);
+            R* alt_v = alt_u + nU(L);
+            R* alt_H = alt_v + nV(L);
+            for (int it = 0; it < n_iter; ++it) {
+                const bool from_alt = (it & 1) != 0;
+                R* su = from_alt ? alt_u : P(L.lambda_u);
+                R* sv = from_alt ? alt_v : P(L.lambda_v);
+                R* sH = from_alt ? alt_H : P(L.lambda_H);
+                R* du = from_alt ? P(L.lambda_u) : alt_u;
+                R* dv = from_alt ? P(L.lambda_v) : alt_v;
+                R* dH = from_alt ? P(L.lambda_H) : alt_H;
+                HIP_TRY(launch_adjoint_pass_fused<R>(a, cache, su, sv, sH, du, dv, dH, CP(L.af_u), CP(L.af_v),
+                                                     CP(L.af_H), vk.calving_rate, vc.omega, st),
                         "adjoint pass");
+            }
+            if (n_iter & 1) {
+                HIP_TRY(launch_copy<R>(P(L.lambda_u), alt_u, (long)nU(L), st), "copy lambda_u");
+                HIP_TRY(launch_copy<R>(P(L.lambda_v), alt_v, (long)nV(L), st), "copy lambda_v");
+                HIP_TRY(launch_copy<R>(P(L.lambda_H), alt_H, (long)nC(L), st), "copy lambda_H");
+            }
             return GLIDE_OK;
         }
     }

@@ -1120,15 +1120,22 @@ __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __rest
 // left column and top row (vjp_body over rows i0 - 1 .. i0 + TY and columns
 // j0 - 1 .. j0 + TX, into LDS), the cached patch solves of the tile's cells
 // and of the cells left of and above it, then the in-place update of
-// k_vanka_adjoint_fused.  The same arithmetic as k_vjp followed by
-// k_vanka_adjoint_fused; it recomputes the ring's residuals (1.4x the
+// k_vanka_adjoint_fused, written to the other of two multiplier buffers:
+// the transpose operator reads the multipliers of neighbouring tiles, which
+// their blocks update during the same pass.  The same arithmetic as k_vjp
+// followed by k_vanka_adjoint_fused; it recomputes the ring's residuals (1.4x the
 // transpose operator's work) and saves a launch, the residual planes' round
 // trip through HBM and the second kernel's ramp: 9.7 against 12.6 us per
 // pass at 32², 14.6 against 19.8 us at 256², and still ahead at 512².
 constexpr int kPassFusedRows = 2 * TY;  // thread rows: the ring's patch solves run beside the tile's
 template <typename T>
 __global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaArgs<T> a, const T* __restrict__ cache,
-                                                                           T* lu, T* lv, T* lH,
+                                                                           const T* __restrict__ lu,
+                                                                           const T* __restrict__ lv,
+                                                                           const T* __restrict__ lH,
+                                                                           T* __restrict__ lu_dst,
+                                                                           T* __restrict__ lv_dst,
+                                                                           T* __restrict__ lH_dst,
                                                                            const T* __restrict__ fu,
                                                                            const T* __restrict__ fv,
                                                                            const T* __restrict__ fH, T calving_rate,
@@ -1192,22 +1199,23 @@ __global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaA
     {
         T d = w[0];
         if (j > 0) d = d + s_w1[ty][tx];
-        lu[(long)i * (nx + 1) + j] += omega * d;
+        const long o = (long)i * (nx + 1) + j;
+        lu_dst[o] = lu[o] + omega * d;
         if (j == nx - 1) {
             T e = T(0) + w[1];
-            lu[(long)i * (nx + 1) + nx] += omega * e;
+            lu_dst[o + 1] = lu[o + 1] + omega * e;
         }
     }
     {
         T d = w[2];
         if (i > 0) d = d + s_w3[ty][tx];
-        lv[c] += omega * d;
+        lv_dst[c] = lv[c] + omega * d;
         if (i == ny - 1) {
             T e = T(0) + w[3];
-            lv[(long)ny * nx + j] += omega * e;
+            lv_dst[c + nx] = lv[c + nx] + omega * e;
         }
     }
-    lH[c] += omega * w[4];
+    lH_dst[c] = lH[c] + omega * w[4];
 }
 
 // Doubles of dynamic LDS k_adjoint_coarse needs for an ny x nx level.
@@ -1458,12 +1466,13 @@ hipError_t launch_vjp(const SsaArgs<T>& a, const T* lu, const T* lv, const T* lH
 }
 
 template <typename T>
-hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, const T* fu,
-                                     const T* fv, const T* fH, T calving_rate, T omega, hipStream_t st) {
+hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, const T* lu, const T* lv, const T* lH,
+                                     T* lu_dst, T* lv_dst, T* lH_dst, const T* fu, const T* fv, const T* fH,
+                                     T calving_rate, T omega, hipStream_t st) {
     if (!a.P.linear_drag) return hipErrorInvalidValue;
     TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx, st);
     k_adjoint_pass_fused<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, kPassFusedRows), 0, st>>>(
-        a, cache, lu, lv, lH, fu, fv, fH, calving_rate, -omega);
+        a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, -omega);
     return hipGetLastError();
 }
 template <typename T>
@@ -1533,8 +1542,8 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_vanka_fused<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, hipStream_t);\
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
-    template hipError_t launch_adjoint_pass_fused<T>(const SsaArgs<T>&, const T*, T*, T*, T*, const T*, const T*,      \
-                                                     const T*, T, T, hipStream_t);                                   \
+    template hipError_t launch_adjoint_pass_fused<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, T*,   \
+                                                     T*, T*, const T*, const T*, const T*, T, T, hipStream_t);       \
     template bool adjoint_coarse_fits<T>(const SsaArgs<T>&);                                                         \
     template hipError_t launch_adjoint_coarse<T>(const SsaArgs<T>&, const T*, T*, T*, T*, const T*, const T*,          \
                                                  const T*, T, T, int, hipStream_t);                                  \

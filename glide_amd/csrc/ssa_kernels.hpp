@@ -59,11 +59,13 @@ hipError_t launch_vanka_adjoint_pass(const SsaArgs<T>& a, const T* ru, const T* 
 // eta_cache: the cell viscosity cached by launch_adjoint_factor for the
 // current state, or nullptr to compute it.
 // One cached adjoint pass (transpose operator, patch solves and update) in one
-// launch; linear drag only.  The same results as launch_vjp +
+// launch, reading the multipliers (lu, lv, lH) and writing the updated ones to
+// separate buffers (dst); linear drag only.  The same results as launch_vjp +
 // launch_vanka_adjoint_pass_cached; used on levels up to kAdjointPassFusedMaxCells.
 template <typename T>
-hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, const T* fu,
-                                     const T* fv, const T* fH, T calving_rate, T omega, hipStream_t st);
+hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, const T* lu, const T* lv, const T* lH,
+                                     T* lu_dst, T* lv_dst, T* lH_dst, const T* fu, const T* fv, const T* fH,
+                                     T calving_rate, T omega, hipStream_t st);
 #ifndef GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE
 #define GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE 512
 #endif

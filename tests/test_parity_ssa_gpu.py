@@ -596,3 +596,42 @@ def test_full_size_kernels_512(orc):
     g2 = grid_from(F)
     g2.forward_operators.vanka_sweep(10.0, 1, freeze_phi=True)
     assert np.array_equal(N(g2.state.u.data), u1)
+
+
+@pytest.mark.parametrize("n_pass", [5, 6])
+def test_adjoint_passes_512_deterministic(n_pass):
+    """At the benchmark size, cached adjoint passes in every launch form give
+    bitwise the same multipliers run to run (tiles read their neighbours'
+    multipliers, so an update in place would race), and the one-launch form
+    agrees with the separate transpose operator and pass."""
+    from glide_amd import _lib
+    print(f"512^2 adjoint passes n={n_pass}", flush=True)
+    F = dome_fields(n=512, seed=27, mask_frac=0.05)
+    lam = _lam(F, 28)
+    rng = np.random.default_rng(29)
+    forcing = None
+
+    def run(fused):
+        nonlocal forcing
+        _lib.adjoint_pass_fused_enable(fused)
+        g = grid_from(F)
+        g.sliding.m.set(1.0)
+        for t, x in zip((g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H), lam):
+            t.set(x)
+        ao = g.adjoint_operators
+        ao.use_cache = True
+        if forcing is None:
+            forcing = [rng.standard_normal(tuple(t.shape)) for t in (ao.f_u, ao.f_v, ao.f_H)]
+        for t, x in zip((ao.f_u, ao.f_v, ao.f_H), forcing):
+            t.copy_(T(x))
+        ao.vanka_sweep(10.0, n_pass)
+        torch.cuda.synchronize()
+        return [N(t.data).copy() for t in (g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H)]
+
+    try:
+        a, b, c = run(True), run(True), run(False)
+    finally:
+        _lib.adjoint_pass_fused_enable(True)
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y)
+        assert rel_err(x, z) < 1e-13
