@@ -48,7 +48,7 @@ template <typename T>
 hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst, T* mask, const VankaArgs<T>& vk,
                               T omega, hipStream_t st);
 #ifndef GLIDE_FUSED_VANKA_MAX_SIDE
-#define GLIDE_FUSED_VANKA_MAX_SIDE 48
+#define GLIDE_FUSED_VANKA_MAX_SIDE 128
 #endif
 constexpr long kFusedVankaMaxCells = (long)GLIDE_FUSED_VANKA_MAX_SIDE * GLIDE_FUSED_VANKA_MAX_SIDE;
 template <typename T>

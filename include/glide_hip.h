@@ -203,7 +203,7 @@ int glide_graphs_clear(void);
  * in-place multiplier update); 0 solves into the workspace and applies in a
  * second launch.  Both give the same multipliers. */
 int glide_adjoint_fused_enable(int on);
-/* Forward smoothing of levels of at most 48 x 48 cells with a frozen
+/* Forward smoothing of levels of at most 128 x 128 cells with a frozen
  * grounded fraction (no counterpart in the reference): 1 (default) runs each
  * pass as one launch (the patch solves of each tile row and of the cells
  * above and left of it, then the owned state written to the other of two

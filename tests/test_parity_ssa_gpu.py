@@ -176,7 +176,7 @@ def test_vanka_single_pass_nonlinear_drag_calving(orc, shape):
 
 
 @pytest.mark.parametrize("shape,n_pass,m", [((32, 32), 6, 1.0), ((24, 90), 5, 1.0), ((17, 129), 3, 1.0 / 3.0),
-                                             ((48, 48), 2, 1.0)])
+                                             ((48, 48), 2, 1.0), ((128, 128), 3, 1.0), ((96, 130), 2, 1.0)])
 def test_vanka_fused_passes_match_two_launch_passes(orc, shape, n_pass, m):
     """Forward passes of a small level as one launch each (state alternating
     with the workspace, odd counts copied back) give the state and active set
