@@ -232,3 +232,39 @@ def test_horizontal_interior_energy_conservation():
     after = np.sum(H_ICE * _np(ops.enthalpy_state.E)[2:-2, 2:-2, :-1])
     print(f"  relative change of interior energy {abs(after - before) / abs(before):.3e}", flush=True)
     assert abs(after - before) / abs(before) < 0.01
+
+
+def test_conservative_balance_uses_the_pre_momentum_thickness():
+    """tests/test_conservative_balance.py: uniform E on a column thinned by
+    1 m in a year leaves a smaller residual with the pre-momentum H_prev than
+    with the thinned one.  The script derives omega with a compute_omega_ssa
+    that the reference's EnthalpyOperators no longer defines; omega here comes
+    from compute_omega with the realised thinning rate."""
+    from glide_amd.enthalpy import EnthalpyOperators
+    from glide_amd.grid import Grid
+    print("enthalpy KAT: conservative balance", flush=True)
+    ny, nx, nz = 4, 4, 11
+    H_old, H_new = 1000.0, 999.0
+    dt = 365.25 * 86400.0
+    smb_si = -1.0 / dt
+    g = Grid(ny, nx, 5000.0, device="cuda")
+    g.state.H.set(np.full((ny, nx), H_new))
+    g.state.H_prev.set(np.full((ny, nx), H_new))
+    g.geometry.bed.set(np.zeros((ny, nx)))
+    g.forcing.smb.set(np.zeros((ny, nx)))
+    g.sliding.beta.set(np.ones((ny, nx)))
+    g.sliding.m.set(1.0)
+    g.sliding.u_reg.set(1.0)
+    ops = EnthalpyOperators(g, nz=nz)
+    ops.term_flags.horizontal_advection = False
+    ops.term_flags.drainage = False
+    ops.initialize_from_temperature(253.15)
+    ops.set_surface_enthalpy_from_temperature(np.full((ny, nx), 253.15))
+    ops.enthalpy_state.E_prev.copy_(ops.enthalpy_state.E)
+    ops.compute_omega(np.full((ny, nx), smb_si))
+    ops.H_prev.fill_(H_old)
+    r_correct = float(ops.compute_residual(dt))
+    ops.H_prev.fill_(H_new)
+    r_wrong = float(ops.compute_residual(dt))
+    print(f"  |r| with pre-momentum H_prev {r_correct:.3e}, with the thinned one {r_wrong:.3e}", flush=True)
+    assert r_correct < r_wrong
