@@ -142,6 +142,7 @@ SIGNATURES = {
     "glide_graphs_enable": (_I, [_I]),
     "glide_adjoint_fused_enable": (_I, [_I]),
     "glide_forward_fused_enable": (_I, [_I]),
+    "glide_forward_persistent_enable": (_I, [_I]),
     "glide_adjoint_coarse_enable": (_I, [_I]),
     "glide_adjoint_pass_fused_enable": (_I, [_I]),
     "glide_graphs_clear": (_I, []),
@@ -187,6 +188,12 @@ def adjoint_fused_enable(on: bool) -> None:
 def forward_fused_enable(on: bool) -> None:
     """Forward passes of small levels as one fused launch (default) or solve + apply."""
     call("glide_forward_fused_enable", int(bool(on)))
+
+
+def forward_persistent_enable(on: bool) -> None:
+    """Forward passes of levels at most 64 x 64 as one persistent launch per
+    pass sequence (default) or one launch per pass."""
+    call("glide_forward_persistent_enable", int(bool(on)))
 
 
 def adjoint_coarse_enable(on: bool) -> None:

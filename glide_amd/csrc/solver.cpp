@@ -161,6 +161,17 @@ bool g_coarse_adjoint = true;
 // Cached adjoint passes of levels up to kAdjointPassFusedMaxCells as one launch
 // each, transpose operator included (glide_adjoint_pass_fused_enable).
 bool g_pass_fused_adjoint = true;
+// Forward passes of levels at most 64 cells wide and 64 rows high as one
+// persistent launch per pass sequence (glide_forward_persistent_enable).
+bool g_persistent_forward = true;
+
+// Three 64-bit words at the end of a level's smoother workspace (5 ny nx
+// values, of which the alternating state of the persistent and fused passes
+// uses 3 ny nx + ny + nx): the arrival counter, timeout flag and XCD mask of a
+// persistent launch.
+unsigned long long* work_sync(const glide_level& L) {
+    return reinterpret_cast<unsigned long long*>(P(L.work) + 5 * nC(L) - 3);
+}
 
 int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
                  hipStream_t st) {
@@ -179,13 +190,19 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
         R* alt_u = P(L.work);
         R* alt_v = alt_u + nU(L);
         R* alt_H = alt_v + nV(L);
-        for (int it = 0; it < n_iter; ++it) {
-            const bool from_alt = (it & 1) != 0;
-            SsaArgs<R> a = make_args(L, dt, false);
-            if (from_alt) { a.u = alt_u; a.v = alt_v; a.H = alt_H; }
-            HIP_TRY(launch_vanka_fused<R>(a, from_alt ? P(L.u) : alt_u, from_alt ? P(L.v) : alt_v,
-                                          from_alt ? P(L.H) : alt_H, P(L.mask), vk, vc.omega, st),
-                    "vanka fused");
+        if (g_persistent_forward && n_iter > 1 && vanka_persistent_fits(L.ny, L.nx)) {
+            HIP_TRY(launch_vanka_persistent<R>(make_args(L, dt, false), alt_u, alt_v, alt_H, P(L.mask), vk, vc.omega,
+                                               n_iter, work_sync(L), st),
+                    "vanka passes (persistent)");
+        } else {
+            for (int it = 0; it < n_iter; ++it) {
+                const bool from_alt = (it & 1) != 0;
+                SsaArgs<R> a = make_args(L, dt, false);
+                if (from_alt) { a.u = alt_u; a.v = alt_v; a.H = alt_H; }
+                HIP_TRY(launch_vanka_fused<R>(a, from_alt ? P(L.u) : alt_u, from_alt ? P(L.v) : alt_v,
+                                              from_alt ? P(L.H) : alt_H, P(L.mask), vk, vc.omega, st),
+                        "vanka fused");
+            }
         }
         if (n_iter & 1) {
             HIP_TRY(launch_copy<R>(P(L.u), alt_u, (long)nU(L), st), "copy u");
@@ -587,6 +604,7 @@ std::string solve_key(char tag, const FasCtx& X, int start, bool norms) {
     key_append(k, g_fused_adjoint);
     key_append(k, g_coarse_adjoint);
     key_append(k, g_pass_fused_adjoint);
+    key_append(k, g_persistent_forward);
     key_append(k, X.n_levels);
     key_append(k, start);
     key_append(k, X.dt);
@@ -927,6 +945,11 @@ int glide_forward_fused_enable(int on) {
 
 int glide_adjoint_coarse_enable(int on) {
     g_coarse_adjoint = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_forward_persistent_enable(int on) {
+    g_persistent_forward = on != 0;
     return GLIDE_OK;
 }
 

@@ -63,6 +63,20 @@ __device__ __forceinline__ T grounded_fraction(T H, T bed, T c) {
     return T(1) / (T(1) + exp(-z));
 }
 
+// Loads of the state: plain, or device-scope relaxed loads (served by the
+// L2, never by a possibly stale L1 line) for kernels whose workgroups read
+// what other workgroups of the same launch wrote.
+struct LdPlain {
+    template <typename T>
+    __device__ static __forceinline__ T get(const T* p) { return *p; }
+};
+struct LdCoherent {
+    template <typename T>
+    __device__ static __forceinline__ T get(const T* p) {
+        return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
 // Velocities around cell (i,j) that its strain-rate invariant reads.
 template <typename T>
 struct CellVel {
@@ -71,22 +85,22 @@ struct CellVel {
     T ubl, vlb, ubr, vrb;      // bottom corners: U(i+1,j), V(i+1,j-1), U(i+1,j+1), V(i+1,j+1)
 };
 
-template <typename T>
+template <typename T, class Ld = LdPlain>
 __device__ __forceinline__ CellVel<T> load_cell_vel(const T* __restrict__ u, const T* __restrict__ v, int i, int j,
                                                     int ny, int nx) {
     CellVel<T> c;
-    c.ul = u[offU(i, j, ny, nx)];
-    c.ur = u[offU(i, j + 1, ny, nx)];
-    c.vt = v[offV(i, j, ny, nx)];
-    c.vb = v[offV(i + 1, j, ny, nx)];
-    c.utl = u[offU(i - 1, j, ny, nx)];
-    c.vlt = v[offV(i, j - 1, ny, nx)];
-    c.utr = u[offU(i - 1, j + 1, ny, nx)];
-    c.vrt = v[offV(i, j + 1, ny, nx)];
-    c.ubl = u[offU(i + 1, j, ny, nx)];
-    c.vlb = v[offV(i + 1, j - 1, ny, nx)];
-    c.ubr = u[offU(i + 1, j + 1, ny, nx)];
-    c.vrb = v[offV(i + 1, j + 1, ny, nx)];
+    c.ul = Ld::get(u + offU(i, j, ny, nx));
+    c.ur = Ld::get(u + offU(i, j + 1, ny, nx));
+    c.vt = Ld::get(v + offV(i, j, ny, nx));
+    c.vb = Ld::get(v + offV(i + 1, j, ny, nx));
+    c.utl = Ld::get(u + offU(i - 1, j, ny, nx));
+    c.vlt = Ld::get(v + offV(i, j - 1, ny, nx));
+    c.utr = Ld::get(u + offU(i - 1, j + 1, ny, nx));
+    c.vrt = Ld::get(v + offV(i, j + 1, ny, nx));
+    c.ubl = Ld::get(u + offU(i + 1, j, ny, nx));
+    c.vlb = Ld::get(v + offV(i + 1, j - 1, ny, nx));
+    c.ubr = Ld::get(u + offU(i + 1, j + 1, ny, nx));
+    c.vrb = Ld::get(v + offV(i + 1, j + 1, ny, nx));
     return c;
 }
 

@@ -37,10 +37,10 @@ constexpr int kVankaRows = GLIDE_VANKA_ROWS;
 #endif
 constexpr int kVjpRows = GLIDE_VJP_ROWS;
 
-template <typename T>
+template <typename T, class Ld = LdPlain>
 __device__ __forceinline__ T cell_viscosity(const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ B,
                                             int ci, int cj, int ny, int nx, T dxi, const Phys<T>& P) {
-    CellVel<T> c = load_cell_vel(u, v, ci, cj, ny, nx);
+    CellVel<T> c = load_cell_vel<T, Ld>(u, v, ci, cj, ny, nx);
     T e2 = strain_invariant(c, ci, cj, ny, nx, dxi, P.eps_reg);
     return T(0.5) * B[(long)ci * nx + cj] * glen_power(e2, P);
 }
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(NT) void k_jvp(SsaArgs<T> a, const T* __restrict__ 
 // ------------------------------------------------------------------ Vanka
 // Gather the patch of cell (i,j); viscosity from the LDS tile (li, lj are the
 // cell's tile coordinates including the one-cell ring).
-template <typename T, int W = TX + 2>
+template <typename T, int W = TX + 2, class Ld = LdPlain>
 __device__ __forceinline__ void load_patch(Patch<T>& p, const SsaArgs<T>& a, const T (*s_eta)[W], int li, int lj,
                                            int i, int j, const T* __restrict__ fu, const T* __restrict__ fv,
                                            const T* __restrict__ fH, const T* __restrict__ phi) {
@@ -314,10 +314,10 @@ __device__ __forceinline__ void load_patch(Patch<T>& p, const SsaArgs<T>& a, con
     p.e_tl = s_eta[li - 1][lj - 1]; p.e_tr = s_eta[li - 1][lj + 1];
     p.e_bl = s_eta[li + 1][lj - 1]; p.e_br = s_eta[li + 1][lj + 1];
     const T* __restrict__ H = a.H;
-    p.H_l = H[offC(i, j - 1, ny, nx)]; p.H_r = H[offC(i, j + 1, ny, nx)];
-    p.H_t = H[offC(i - 1, j, ny, nx)]; p.H_b = H[offC(i + 1, j, ny, nx)];
-    p.H_tl = H[offC(i - 1, j - 1, ny, nx)]; p.H_tr = H[offC(i - 1, j + 1, ny, nx)];
-    p.H_bl = H[offC(i + 1, j - 1, ny, nx)]; p.H_br = H[offC(i + 1, j + 1, ny, nx)];
+    p.H_l = Ld::get(H + offC(i, j - 1, ny, nx)); p.H_r = Ld::get(H + offC(i, j + 1, ny, nx));
+    p.H_t = Ld::get(H + offC(i - 1, j, ny, nx)); p.H_b = Ld::get(H + offC(i + 1, j, ny, nx));
+    p.H_tl = Ld::get(H + offC(i - 1, j - 1, ny, nx)); p.H_tr = Ld::get(H + offC(i - 1, j + 1, ny, nx));
+    p.H_bl = Ld::get(H + offC(i + 1, j - 1, ny, nx)); p.H_br = Ld::get(H + offC(i + 1, j + 1, ny, nx));
     p.p_c = phi[offC(i, j, ny, nx)]; p.p_l = phi[offC(i, j - 1, ny, nx)]; p.p_r = phi[offC(i, j + 1, ny, nx)];
     p.p_t = phi[offC(i - 1, j, ny, nx)]; p.p_b = phi[offC(i + 1, j, ny, nx)];
     p.b_c = a.beta[offC(i, j, ny, nx)]; p.b_l = a.beta[offC(i, j - 1, ny, nx)]; p.b_r = a.beta[offC(i, j + 1, ny, nx)];
@@ -326,12 +326,12 @@ __device__ __forceinline__ void load_patch(Patch<T>& p, const SsaArgs<T>& a, con
     p.z_t = a.bed[offC(i - 1, j, ny, nx)]; p.z_b = a.bed[offC(i + 1, j, ny, nx)];
     const T* __restrict__ u = a.u;
     const T* __restrict__ v = a.v;
-    p.u_ll = u[offU(i, j - 1, ny, nx)]; p.u_rr = u[offU(i, j + 2, ny, nx)];
-    p.u_tl = u[offU(i - 1, j, ny, nx)]; p.u_tr = u[offU(i - 1, j + 1, ny, nx)];
-    p.u_bl = u[offU(i + 1, j, ny, nx)]; p.u_br = u[offU(i + 1, j + 1, ny, nx)];
-    p.v_lt = v[offV(i, j - 1, ny, nx)]; p.v_lb = v[offV(i + 1, j - 1, ny, nx)];
-    p.v_rt = v[offV(i, j + 1, ny, nx)]; p.v_rb = v[offV(i + 1, j + 1, ny, nx)];
-    p.v_tt = v[offV(i - 1, j, ny, nx)]; p.v_bb = v[offV(i + 2, j, ny, nx)];
+    p.u_ll = Ld::get(u + offU(i, j - 1, ny, nx)); p.u_rr = Ld::get(u + offU(i, j + 2, ny, nx));
+    p.u_tl = Ld::get(u + offU(i - 1, j, ny, nx)); p.u_tr = Ld::get(u + offU(i - 1, j + 1, ny, nx));
+    p.u_bl = Ld::get(u + offU(i + 1, j, ny, nx)); p.u_br = Ld::get(u + offU(i + 1, j + 1, ny, nx));
+    p.v_lt = Ld::get(v + offV(i, j - 1, ny, nx)); p.v_lb = Ld::get(v + offV(i + 1, j - 1, ny, nx));
+    p.v_rt = Ld::get(v + offV(i, j + 1, ny, nx)); p.v_rb = Ld::get(v + offV(i + 1, j + 1, ny, nx));
+    p.v_tt = Ld::get(v + offV(i - 1, j, ny, nx)); p.v_bb = Ld::get(v + offV(i + 2, j, ny, nx));
     if (fu) {
         p.f0 = fu[offU(i, j, ny, nx)]; p.f1 = fu[offU(i, j + 1, ny, nx)];
         p.f2 = fv[offV(i, j, ny, nx)]; p.f3 = fv[offV(i + 1, j, ny, nx)];
@@ -367,7 +367,7 @@ __device__ __forceinline__ void fill_eta_tile(T (*s_eta)[TX + 2], const SsaArgs<
 // the active-set flag.  edge: the calling wave may hold cells on the domain's
 // edge (the loop then tests the Dirichlet pins; interior waves carry no pin
 // code).  The wave-uniform edge test keeps one loop copy per wave.
-template <typename T, bool LINEAR, typename TJ, int W>
+template <typename T, bool LINEAR, typename TJ, int W, class Ld = LdPlain>
 __device__ __forceinline__ void cell_newton(const SsaArgs<T>& a, const T (*s_eta)[W], int li, int lj, int i, int j,
                                             const VankaArgs<T>& vk, bool edge, T w[5], T& masked, int& iters) {
     const int ny = a.ny, nx = a.nx;
@@ -375,7 +375,7 @@ __device__ __forceinline__ void cell_newton(const SsaArgs<T>& a, const T (*s_eta
     T thk;
     {
         Patch<T> p;
-        load_patch<T, W>(p, a, s_eta, li, lj, i, j, a.f_u, a.f_v, a.f_H, a.phi);
+        load_patch<T, W, Ld>(p, a, s_eta, li, lj, i, j, a.f_u, a.f_v, a.f_H, a.phi);
         make_cell_sys(cs, p, a.P, vk.calving_rate, a.dxi, a.dt, vk.ssa_damping, vk.mc_damping);
         thk = p.gam;
     }
@@ -385,7 +385,9 @@ __device__ __forceinline__ void cell_newton(const SsaArgs<T>& a, const T (*s_eta
     if constexpr (std::is_same<T, TJ>::value) cq = &cs;
     else cq = &cq_own;
     const long c = (long)i * nx + j;
-    T x[5] = {a.u[(long)i * (nx + 1) + j], a.u[(long)i * (nx + 1) + j + 1], a.v[c], a.v[c + nx], a.H[c]};
+    const T x0[5] = {Ld::get(a.u + (long)i * (nx + 1) + j), Ld::get(a.u + (long)i * (nx + 1) + j + 1), Ld::get(a.v + c),
+                     Ld::get(a.v + c + nx), Ld::get(a.H + c)};
+    T x[5] = {x0[0], x0[1], x0[2], x0[3], x0[4]};
     T cmp[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) cmp[k] = T(0);
@@ -429,11 +431,11 @@ __device__ __forceinline__ void cell_newton(const SsaArgs<T>& a, const T (*s_eta
     };
     if (edge) newton(std::true_type{});
     else newton(std::false_type{});
-    w[0] = T(0.5) * (x[0] - a.u[(long)i * (nx + 1) + j]);
-    w[1] = T(0.5) * (x[1] - a.u[(long)i * (nx + 1) + j + 1]);
-    w[2] = T(0.5) * (x[2] - a.v[c]);
-    w[3] = T(0.5) * (x[3] - a.v[c + nx]);
-    w[4] = x[4] - a.H[c];
+    w[0] = T(0.5) * (x[0] - x0[0]);
+    w[1] = T(0.5) * (x[1] - x0[1]);
+    w[2] = T(0.5) * (x[2] - x0[2]);
+    w[3] = T(0.5) * (x[3] - x0[3]);
+    w[4] = x[4] - x0[4];
     iters = it;
 }
 
@@ -536,6 +538,164 @@ __global__ __launch_bounds__(3 * TX) void k_vanka_fused(SsaArgs<T> a, T* __restr
     }
     H_dst[c] = a.H[c] + omega * w[4];
     mask[c] = masked;
+}
+
+// Barrier of the working workgroups of a persistent launch (a handful, all
+// resident at once).  sync[0] counts the arrivals of the launch (zeroed before
+// it); barrier k of the launch waits for nb * (k + 1) of them.  Every thread's
+// stores have reached the L2 before its workgroup arrives.  When the
+// workgroups run on one XCD they share that L2 and read each other's state
+// with device-scope loads (LdCoherent), so the barrier needs no cache
+// maintenance; otherwise (fenced) the arriving thread's agent-scope fences
+// write its L2 back before the arrival and invalidate it after the wait.  The
+// wait is bounded: after kGridSpinLimit polls a workgroup sets sync[1], after
+// which no barrier of the launch waits (the caller turns sync[1] into an
+// error it can see).
+constexpr int kGridSpinLimit = 1 << 20;
+
+__device__ __forceinline__ void grid_barrier(unsigned long long* sync, unsigned long long target, bool fenced) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        if (fenced) __threadfence();
+        atomicAdd(sync, 1ull);
+        int spins = 0;
+        while (__hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
+               !__hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if (++spins > kGridSpinLimit) {
+                atomicExch(sync + 1, 1ull);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (fenced) __threadfence();
+    }
+    __syncthreads();
+}
+
+// The XCC (XCD) the calling wave runs on.
+__device__ __forceinline__ unsigned xcc_id() {
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return v & 0xf;
+}
+
+// Workgroups of a launch are dispatched to the XCDs in turn, so the
+// persistent kernels launch kXcds times the workgroups they need and work in
+// every kXcds-th one only, which puts the working set on one XCD.
+constexpr int kXcds = 8;
+
+// Start of a persistent launch: the working workgroups record their XCDs in
+// sync[2] and pass the first barrier (fenced); returns whether they all run
+// on one XCD (the same answer in every workgroup).
+__device__ __forceinline__ bool persistent_start(unsigned long long* sync, int nb) {
+    __shared__ int s_one;
+    if (threadIdx.x == 0 && threadIdx.y == 0) atomicOr(sync + 2, 1ull << xcc_id());
+    grid_barrier(sync, (unsigned long long)nb, true);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        s_one = __popcll(__hip_atomic_load(sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 1;
+    __syncthreads();
+    return s_one != 0;
+}
+
+__global__ __launch_bounds__(64) void k_zero_sync(unsigned long long* sync) {
+    if (threadIdx.x < 3) sync[threadIdx.x] = 0ull;
+}
+
+// n_iter forward smoothing passes of a level at most TX cells wide in one
+// launch (bitwise what n_iter k_vanka_fused launches give).  Working
+// workgroup b owns the full rows i0 = b * RW .. i0 + RW - 1 (wave q < RW
+// solves row i0 + q) and also solves row i0 - 1 (wave RW), whose bottom-facet
+// increments its first row's top facets receive.  Pass p reads the state from
+// a.u/v/H (p even) or the alternate buffers (p odd) and writes the other set;
+// a barrier separates the passes.  If a barrier timed out, the last pass
+// writes NaN thicknesses, which the caller's residual norms then show.
+template <typename T, bool LINEAR, int RW>
+__global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a, T* alt_u, T* alt_v, T* alt_H,
+                                                                    T* __restrict__ mask, VankaArgs<T> vk, T omega,
+                                                                    int n_iter, unsigned long long* sync) {
+    using Ld = LdCoherent;
+    constexpr int W = TX + 3;
+    __shared__ T s_eta[RW + 3][W];  // cells (i0 - 2 .. i0 + RW, -2 .. TX)
+    __shared__ T s_w1[RW][TX + 1];  // right-facet increments of the owned rows' cells (cell j at j + 1)
+    __shared__ T s_w3[RW + 1][TX];  // bottom-facet increments of rows i0 + q (q < RW) and i0 - 1 (q = RW)
+    __shared__ int s_fail;
+    if (blockIdx.x % kXcds) return;
+    const int nb = gridDim.x / kXcds;
+    const int ny = a.ny, nx = a.nx;
+    const int i0 = (blockIdx.x / kXcds) * RW;
+    const int wv = threadIdx.y, lane = threadIdx.x;
+    const bool active = (wv < RW ? i0 + wv : i0 - 1) >= 0 && (wv < RW ? i0 + wv : i0 - 1) < ny && lane < nx;
+    const bool own = wv < RW && active;
+    const int above = wv == 0 ? RW : wv - 1;  // the wave that solved row ci - 1
+    T* const base_u = const_cast<T*>(a.u);
+    T* const base_v = const_cast<T*>(a.v);
+    T* const base_H = const_cast<T*>(a.H);
+    const bool fenced = !persistent_start(sync, nb);
+#pragma unroll 1
+    for (int p = 0; p < n_iter; ++p) {
+        // the lane's indices pass through an empty asm statement every pass, so
+        // that the compiler does not hoist the (pass-invariant) addresses of
+        // the parameter loads out of the loop and hold them in registers
+        int tid = wv * TX + lane, cj = lane, ci = wv < RW ? i0 + wv : i0 - 1;
+        asm volatile("" : "+v"(tid), "+v"(cj), "+v"(ci));
+        SsaArgs<T> s = a;
+        T *u_dst = alt_u, *v_dst = alt_v, *H_dst = alt_H;
+        if (p & 1) {
+            s.u = alt_u;
+            s.v = alt_v;
+            s.H = alt_H;
+            u_dst = base_u;
+            v_dst = base_v;
+            H_dst = base_H;
+        }
+        for (int k = tid; k < (RW + 3) * W; k += (RW + 1) * TX) {
+            const int ei = i0 - 2 + k / W, ej = k % W - 2;
+            T e = T(0);
+            if (ei >= 0 && ei < ny && ej >= 0 && ej < nx)
+                e = cell_viscosity<T, Ld>(s.u, s.v, s.B, ei, ej, ny, nx, s.dxi, s.P);
+            s_eta[k / W][k % W] = e;
+        }
+        __syncthreads();
+        T w[5] = {T(0), T(0), T(0), T(0), T(0)};
+        T masked = own ? Ld::get(mask + (long)ci * nx + cj) : T(0);
+        int it = 0;
+        if (active) {
+            cell_newton<T, LINEAR, T, W, Ld>(s, s_eta, ci - i0 + 2, cj + 2, ci, cj, vk, true, w, masked, it);
+            if (wv < RW) s_w1[wv][lane + 1] = w[1];
+            s_w3[wv][lane] = w[3];
+        }
+        __syncthreads();
+        if (own) {
+            const int i = ci, j = cj;
+            const long c = (long)i * nx + j, ou = (long)i * (nx + 1) + j;
+            {
+                T d = w[0];
+                if (j > 0) d = d + s_w1[wv][lane];
+                u_dst[ou] = Ld::get(s.u + ou) + omega * d;
+                if (j == nx - 1) {
+                    const T e = T(0) + w[1];
+                    u_dst[ou + 1] = Ld::get(s.u + ou + 1) + omega * e;
+                }
+            }
+            {
+                T d = w[2];
+                if (i > 0) d = d + s_w3[above][lane];
+                v_dst[c] = Ld::get(s.v + c) + omega * d;
+                if (i == ny - 1) {
+                    const T e = T(0) + w[3];
+                    v_dst[c + nx] = Ld::get(s.v + c + nx) + omega * e;
+                }
+            }
+            H_dst[c] = Ld::get(s.H + c) + omega * w[4];
+            mask[c] = masked;
+        }
+        if (p + 1 < n_iter) grid_barrier(sync, (unsigned long long)nb * (p + 2), fenced);
+    }
+    if (wv == 0 && lane == 0) s_fail = __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    __syncthreads();
+    if (s_fail && own)
+        (((n_iter - 1) & 1) ? base_H : alt_H)[(long)(i0 + wv) * nx + lane] = __longlong_as_double(0x7ff8000000000000ll);
 }
 
 // x += sign * omega * (sum of the cell increments touching each unknown).
@@ -1436,6 +1596,32 @@ hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst,
     return hipGetLastError();
 }
 
+bool vanka_persistent_fits(int ny, int nx) {
+    // the three sync words sit behind the alternating state in the 5 ny nx workspace
+    return nx >= 1 && nx <= TX && ny >= 1 && (ny + kPersistentRows - 1) / kPersistentRows <= kPersistentMaxBlocks &&
+           2L * ny * nx >= (long)ny + nx + 3;
+}
+
+template <typename T>
+hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* alt_H, T* mask,
+                                   const VankaArgs<T>& vk, T omega, int n_iter, unsigned long long* sync,
+                                   hipStream_t st) {
+    if (n_iter <= 0) return hipSuccess;
+    if (!vanka_persistent_fits(a.ny, a.nx)) return hipErrorInvalidValue;
+    // the arrival counter, the timeout flag and the XCD mask start at zero (a
+    // kernel rather than a memset, so that the launch can sit in a captured graph)
+    k_zero_sync<<<1, 64, 0, st>>>(sync);
+    TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx * n_iter, st);
+    const dim3 g(kXcds * ((a.ny + kPersistentRows - 1) / kPersistentRows)), b(TX, kPersistentRows + 1);
+    if (a.P.linear_drag)
+        k_vanka_persistent<T, true, kPersistentRows><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter,
+                                                                      sync);
+    else
+        k_vanka_persistent<T, false, kPersistentRows><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega,
+                                                                       n_iter, sync);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st) {
     k_vanka_dump<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, TY), 0, st>>>(a, J, r, calving_rate);
@@ -1540,6 +1726,8 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
                                              hipStream_t);                                                           \
     template hipError_t launch_vanka_dump<T>(const SsaArgs<T>&, T*, T*, T, hipStream_t);                              \
     template hipError_t launch_vanka_fused<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, hipStream_t);\
+    template hipError_t launch_vanka_persistent<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, int,    \
+                                                  unsigned long long*, hipStream_t);                              \
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
     template hipError_t launch_adjoint_pass_fused<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, T*,   \

@@ -51,6 +51,21 @@ hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst,
 #define GLIDE_FUSED_VANKA_MAX_SIDE 128
 #endif
 constexpr long kFusedVankaMaxCells = (long)GLIDE_FUSED_VANKA_MAX_SIDE * GLIDE_FUSED_VANKA_MAX_SIDE;
+// Forward smoothing passes of levels at most 64 cells wide and
+// kPersistentRows * kPersistentMaxBlocks rows high as one persistent launch
+// (kPersistentRows rows per workgroup, a barrier between passes); sync holds
+// three 64-bit words the launch zeroes first (arrival counter, timeout flag,
+// XCD mask).
+#ifndef GLIDE_PERSISTENT_ROWS
+#define GLIDE_PERSISTENT_ROWS 3
+#endif
+constexpr int kPersistentRows = GLIDE_PERSISTENT_ROWS;
+constexpr int kPersistentMaxBlocks = 16;
+bool vanka_persistent_fits(int ny, int nx);
+template <typename T>
+hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* alt_H, T* mask,
+                                   const VankaArgs<T>& vk, T omega, int n_iter, unsigned long long* sync,
+                                   hipStream_t st);
 template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st);
 template <typename T>

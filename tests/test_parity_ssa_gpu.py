@@ -211,6 +211,31 @@ def test_vanka_fused_passes_match_two_launch_passes(orc, shape, n_pass, m):
         assert np.array_equal(out[0][3], Fo["mask"])
 
 
+@pytest.mark.parametrize("shape,n_pass,m", [((32, 32), 200, 1.0), ((32, 32), 7, 1.0 / 3.0), ((17, 29), 6, 1.0),
+                                             ((5, 64), 4, 1.0), ((48, 64), 3, 1.0), ((2, 3), 5, 1.0)])
+def test_vanka_persistent_passes_match_fused_launches(shape, n_pass, m):
+    """All passes of a level at most 64 x 64 in one persistent launch (rows
+    split over workgroups, device-wide barriers between passes) give bitwise
+    the state and active set of one fused launch per pass."""
+    from glide_amd import _lib
+    ny, nx = shape
+    print(f"vanka persistent shape={shape} passes={n_pass} m={m:.3f}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=43, mask_frac=0.1)
+    out = []
+    for persistent in (True, False):
+        _lib.forward_persistent_enable(persistent)
+        g = grid_from(F)
+        g.sliding.m.set(m)
+        g.calving.calving_rate.set(0.2)
+        g.forward_operators.vanka_sweep(10.0, n_pass, freeze_phi=True)
+        torch.cuda.synchronize()
+        out.append([N(t.data) for t in (g.state.u, g.state.v, g.state.H, g.state.mask)])
+    _lib.forward_persistent_enable(True)
+    assert np.isfinite(out[0][2]).all()
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+
+
 def test_vanka_passes_with_phi_refresh(orc):
     F = dome_fields(n=32, seed=6)
     g = grid_from(F)
