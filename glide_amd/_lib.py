@@ -145,6 +145,7 @@ SIGNATURES = {
     "glide_forward_persistent_enable": (_I, [_I]),
     "glide_adjoint_coarse_enable": (_I, [_I]),
     "glide_adjoint_pass_fused_enable": (_I, [_I]),
+    "glide_adjoint_persistent_enable": (_I, [_I]),
     "glide_graphs_clear": (_I, []),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -206,6 +207,12 @@ def adjoint_pass_fused_enable(on: bool) -> None:
     """Cached adjoint passes of levels up to 512 x 512 cells (linear drag) as one
     launch each, transpose operator included (default), or two launches."""
     call("glide_adjoint_pass_fused_enable", int(bool(on)))
+
+
+def adjoint_persistent_enable(on: bool) -> None:
+    """Cached adjoint passes of levels at most 64 wide, 32 high (linear drag) as one
+    persistent launch per call (default) or one fused launch per pass."""
+    call("glide_adjoint_persistent_enable", int(bool(on)))
 
 
 def graphs_clear() -> None:

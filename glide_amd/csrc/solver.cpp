@@ -164,6 +164,8 @@ bool g_pass_fused_adjoint = true;
 // Forward passes of levels at most 64 cells wide and 64 rows high as one
 // persistent launch per pass sequence (glide_forward_persistent_enable).
 bool g_persistent_forward = true;
+// Adjoint passes of the same levels likewise (glide_adjoint_persistent_enable).
+bool g_persistent_adjoint = true;
 
 // Three 64-bit words at the end of a level's smoother workspace (5 ny nx
 // values, of which the alternating state of the persistent and fused passes
@@ -280,13 +282,21 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
     if (cache && g_pass_fused_adjoint && (long)L.ny * L.nx <= kAdjointPassFusedMaxCells) {
         SsaArgs<R> a = make_args(L, dt, true);
         if (a.P.linear_drag) {
+            int n_iter_done = 0;
             // the multipliers alternate between the level's buffers and its
             // 5-plane workspace (neighbouring tiles read them during a pass);
             // an odd pass count copies them back
             R* alt_u = P(L.work);
             R* alt_v = alt_u + nU(L);
             R* alt_H = alt_v + nV(L);
-            for (int it = 0; it < n_iter; ++it) {
+            if (g_persistent_adjoint && n_iter > 1 && adjoint_persistent_fits(L.ny, L.nx)) {
+                HIP_TRY(launch_adjoint_persistent<R>(a, cache, P(L.lambda_u), P(L.lambda_v), P(L.lambda_H), alt_u,
+                                                     alt_v, alt_H, CP(L.af_u), CP(L.af_v), CP(L.af_H),
+                                                     vk.calving_rate, vc.omega, n_iter, work_sync(L), st),
+                        "adjoint passes (persistent)");
+                n_iter_done = n_iter;
+            }
+            for (int it = n_iter_done; it < n_iter; ++it) {
                 const bool from_alt = (it & 1) != 0;
                 R* su = from_alt ? alt_u : P(L.lambda_u);
                 R* sv = from_alt ? alt_v : P(L.lambda_v);
@@ -605,6 +615,7 @@ std::string solve_key(char tag, const FasCtx& X, int start, bool norms) {
     key_append(k, g_coarse_adjoint);
     key_append(k, g_pass_fused_adjoint);
     key_append(k, g_persistent_forward);
+    key_append(k, g_persistent_adjoint);
     key_append(k, X.n_levels);
     key_append(k, start);
     key_append(k, X.dt);
@@ -950,6 +961,11 @@ int glide_adjoint_coarse_enable(int on) {
 
 int glide_forward_persistent_enable(int on) {
     g_persistent_forward = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_adjoint_persistent_enable(int on) {
+    g_persistent_adjoint = on != 0;
     return GLIDE_OK;
 }
 

@@ -1008,7 +1008,7 @@ struct VjpTile {
 // one-cell ring, vertex arrays, and with S::kDrag the drag cross-terms;
 // without it the drag must be linear, whose cross-terms vanish), each output
 // handed to Out with its tile position and global index.
-template <typename T, typename S, typename Out>
+template <typename T, typename S, typename Out, class Ld = LdPlain>
 __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restrict__ lu, const T* __restrict__ lv,
                                          const T* __restrict__ lH, const T* __restrict__ fu, const T* __restrict__ fv,
                                          const T* __restrict__ fH, Out& out, T calving_rate,
@@ -1028,13 +1028,13 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
         if (ci >= 0 && ci < ny && cj >= 0 && cj < nx) {
             const long c = (long)ci * nx + cj;
             CellVel<T> cv = load_cell_vel(u, v, ci, cj, ny, nx);
-            CellVel<T> cl = load_cell_vel(lu, lv, ci, cj, ny, nx);
+            CellVel<T> cl = load_cell_vel<T, Ld>(lu, lv, ci, cj, ny, nx);
             T e2 = strain_invariant(cv, ci, cj, ny, nx, dxi, a.P.eps_reg);
             T de2 = strain_invariant_dir(cv, cl, ci, cj, ny, nx, dxi);
             e = eta_cache ? eta_cache[c] : T(0.5) * a.B[c] * glen_power(e2, a.P);
             ed = e * a.P.glen_p * de2 / e2;
             Hh = H[c];
-            lHm = lH[c] * (T(1) - a.mask[c]);
+            lHm = Ld::get(lH + c) * (T(1) - a.mask[c]);
             NormalStress<T> sx = sxx(cv.ul, cv.ur, cv.vt, cv.vb, e * Hh, dxi, true);
             Lx = sx.d_ul * cl.ul + sx.d_ur * cl.ur + sx.d_vt * cl.vt + sx.d_vb * cl.vb + sx.d_eH * (Hh * ed);
             Kx = sx.d_eH;
@@ -1060,8 +1060,9 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
             const long ut = (long)(vi - 1) * (nx + 1) + vj, ub = (long)vi * (nx + 1) + vj;
             const long vl = (long)vi * nx + vj - 1, vr = (long)vi * nx + vj;
             ShearStress<T> q = sxy(u[ut], u[ub], v[vl], v[vr], eH, dxi, true);
-            L = q.d_ut * lu[ut] + q.d_ub * lu[ub] + q.d_vl * lv[vl] + q.d_vr * lv[vr] + q.d_eH * eHd;
-            G = q.d_eH * T(0.25) * (lu[ub] - lu[ut] + lv[vl] - lv[vr]) * dxi;
+            const T lut = Ld::get(lu + ut), lub = Ld::get(lu + ub), lvl = Ld::get(lv + vl), lvr = Ld::get(lv + vr);
+            L = q.d_ut * lut + q.d_ub * lub + q.d_vl * lvl + q.d_vr * lvr + q.d_eH * eHd;
+            G = q.d_eH * T(0.25) * (lub - lut + lvl - lvr) * dxi;
         }
         s.Lxy(ri, rj) = L; s.Gxy(ri, rj) = G;
     }
@@ -1076,7 +1077,7 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                                        v[offV(fi + 1, fj - 1, ny, nx)], v[offV(fi + 1, fj, ny, nx)],
                                        a.phi[offC(fi, fj - 1, ny, nx)], a.phi[offC(fi, fj, ny, nx)],
                                        a.beta[offC(fi, fj - 1, ny, nx)], a.beta[offC(fi, fj, ny, nx)], a.P);
-                T l = lu[(long)fi * (nx + 1) + fj];
+                T l = Ld::get(lu + (long)fi * (nx + 1) + fj);
                 q0 = l * t.c0; q1 = l * t.c1; q2 = l * t.c2; q3 = l * t.c3;
             }
             s.Pu(0, ri, rj) = q0; s.Pu(1, ri, rj) = q1; s.Pu(2, ri, rj) = q2; s.Pu(3, ri, rj) = q3;
@@ -1089,7 +1090,7 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                                        u[offU(fi, fj, ny, nx)], u[offU(fi, fj + 1, ny, nx)],
                                        a.phi[offC(fi - 1, fj, ny, nx)], a.phi[offC(fi, fj, ny, nx)],
                                        a.beta[offC(fi - 1, fj, ny, nx)], a.beta[offC(fi, fj, ny, nx)], a.P);
-                T l = lv[(long)fi * nx + fj];
+                T l = Ld::get(lv + (long)fi * nx + fj);
                 q0 = l * t.c0; q1 = l * t.c1; q2 = l * t.c2; q3 = l * t.c3;
             }
             s.Pv(0, ri, rj) = q0; s.Pv(1, ri, rj) = q1; s.Pv(2, ri, rj) = q2; s.Pv(3, ri, rj) = q3;
@@ -1113,7 +1114,7 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                 T pl = a.phi[cc - 1], pc = a.phi[cc];
                 Drag<T> t = basal_drag(u[o], v[cc - 1], v[cc], v[cc + nx - 1], v[cc + nx], pl, pc, a.beta[cc - 1],
                                        a.beta[cc], a.P);
-                acc += lu[o] * t.d_w;
+                acc += Ld::get(lu + o) * t.d_w;
                 // cross terms from v facets (i,j) [ubl], (i,j-1) [ubr], (i+1,j) [utl], (i+1,j-1) [utr]
                 if (S::kDrag)
                     acc += s.Pv(2, vi, cj) + s.Pv(3, vi, cj - 1) + s.Pv(0, vi + 1, cj) + s.Pv(1, vi + 1, cj - 1);
@@ -1131,7 +1132,7 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                 const long ul = (long)i * (nx + 1) + j, utl = ul - (nx + 1);
                 T pt = a.phi[o - nx], pc = a.phi[o];
                 Drag<T> t = basal_drag(v[o], u[utl], u[utl + 1], u[ul], u[ul + 1], pt, pc, a.beta[o - nx], a.beta[o], a.P);
-                acc += lv[o] * t.d_w;
+                acc += Ld::get(lv + o) * t.d_w;
                 // cross terms from u facets (i,j) [vtr], (i,j+1) [vtl], (i-1,j) [vbr], (i-1,j+1) [vbl]
                 if (S::kDrag)
                     acc += s.Pu(1, vi + 1, vj) + s.Pu(0, vi + 1, vj + 1) + s.Pu(3, vi, vj) + s.Pu(2, vi, vj + 1);
@@ -1173,8 +1174,8 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                     acc += lc * k * calving_rate * dxi;
                 }
                 // viscosity-thickness products of the cell's normal stresses
-                acc += s.Kxx(ci, cj) * ec * (lu[ol] - lu[ol + 1]) * dxi;
-                acc += s.Kyy(ci, cj) * ec * (lv[c + nx] - lv[c]) * dxi;
+                acc += s.Kxx(ci, cj) * ec * (Ld::get(lu + ol) - Ld::get(lu + ol + 1)) * dxi;
+                acc += s.Kyy(ci, cj) * ec * (Ld::get(lv + c + nx) - Ld::get(lv + c)) * dxi;
                 // four corner vertices
                 acc += (s.Gxy(vi, vj) + s.Gxy(vi, vj + 1) + s.Gxy(vi + 1, vj) + s.Gxy(vi + 1, vj + 1)) * ec;
                 // driving stress on the four faces
@@ -1182,19 +1183,19 @@ __device__ __forceinline__ void vjp_body(const SsaArgs<T>& a, const T* __restric
                     const T pc = a.phi[c], zc = a.bed[c];
                     if (j > 0) {
                         Drive<T> d = drive_x(H[c - 1], Hc, a.bed[c - 1], zc, a.phi[c - 1], pc, dxi, true);
-                        acc -= lu[ol] * d.d_HB;
+                        acc -= Ld::get(lu + ol) * d.d_HB;
                     }
                     if (j + 1 < nx) {
                         Drive<T> d = drive_x(Hc, H[c + 1], zc, a.bed[c + 1], pc, a.phi[c + 1], dxi, true);
-                        acc -= lu[ol + 1] * d.d_HA;
+                        acc -= Ld::get(lu + ol + 1) * d.d_HA;
                     }
                     if (i > 0) {
                         Drive<T> d = drive_y(H[c - nx], Hc, a.bed[c - nx], zc, a.phi[c - nx], pc, dxi, true);
-                        acc -= lv[c] * d.d_HB;
+                        acc -= Ld::get(lv + c) * d.d_HB;
                     }
                     if (i + 1 < ny) {
                         Drive<T> d = drive_y(Hc, H[c + nx], zc, a.bed[c + nx], pc, a.phi[c + nx], dxi, true);
-                        acc -= lv[c + nx] * d.d_HA;
+                        acc -= Ld::get(lv + c + nx) * d.d_HA;
                     }
                 }
                 if (fH) acc -= fH[c];
@@ -1288,18 +1289,20 @@ __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __rest
 // trip through HBM and the second kernel's ramp: 9.7 against 12.6 us per
 // pass at 32², 14.6 against 19.8 us at 256², and still ahead at 512².
 constexpr int kPassFusedRows = 2 * TY;  // thread rows: the ring's patch solves run beside the tile's
-template <typename T>
-__global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaArgs<T> a, const T* __restrict__ cache,
-                                                                           const T* __restrict__ lu,
-                                                                           const T* __restrict__ lv,
-                                                                           const T* __restrict__ lH,
-                                                                           T* __restrict__ lu_dst,
-                                                                           T* __restrict__ lv_dst,
-                                                                           T* __restrict__ lH_dst,
-                                                                           const T* __restrict__ fu,
-                                                                           const T* __restrict__ fv,
-                                                                           const T* __restrict__ fH, T calving_rate,
-                                                                           T omega) {
+
+// One adjoint pass over the TY x TX tile at (i0, j0) with TX x kPassFusedRows
+// threads: k_vjp's body over the tile plus its left column and top row (into
+// LDS), the cached patch solves of the tile (thread rows < TY) and of the ring
+// (the other threads), then the update of the tile's multipliers into the
+// destination buffers.
+template <typename T, class Ld>
+__device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* __restrict__ cache,
+                                                  const T* __restrict__ lu, const T* __restrict__ lv,
+                                                  const T* __restrict__ lH, T* __restrict__ lu_dst,
+                                                  T* __restrict__ lv_dst, T* __restrict__ lH_dst,
+                                                  const T* __restrict__ fu, const T* __restrict__ fv,
+                                                  const T* __restrict__ fH, T calving_rate, T omega, int i0, int j0,
+                                                  int ty, int tx) {
     constexpr int RR = TY + 2, RC = TX + 2;
     constexpr int CW = RC + 2, NCELL = (RR + 2) * CW, VW = RC + 1, NVERT = (RR + 1) * VW;
     __shared__ T s_cell[8 * NCELL];
@@ -1317,11 +1320,11 @@ __global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaA
     VjpTileOut<T, RC> out{r_u, r_v, r_h};
     const int ny = a.ny, nx = a.nx;
     const long n = (long)ny * nx;
-    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
-    const int ty = threadIdx.y, tx = threadIdx.x, tid = ty * TX + tx;
+    const int tid = ty * TX + tx;
     constexpr int NTF = TX * kPassFusedRows;
-    vjp_body(a, lu, lv, lH, fu, fv, fH, out, calving_rate, cache + (long)kAdjointCacheEta * n, i0 - 1, j0 - 1, RR,
-             RC, tid, NTF, store);
+    vjp_body<T, VjpLevelStore<T>, VjpTileOut<T, RC>, Ld>(a, lu, lv, lH, fu, fv, fH, out, calving_rate,
+                                                          cache + (long)kAdjointCacheEta * n, i0 - 1, j0 - 1, RR, RC,
+                                                          tid, NTF, store);
     __syncthreads();
     auto increments = [&](int ci, int cj, T w[5]) {
         const int ti = ci - i0 + 1, tj = cj - j0 + 1;
@@ -1360,22 +1363,76 @@ __global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaA
         T d = w[0];
         if (j > 0) d = d + s_w1[ty][tx];
         const long o = (long)i * (nx + 1) + j;
-        lu_dst[o] = lu[o] + omega * d;
+        lu_dst[o] = Ld::get(lu + o) + omega * d;
         if (j == nx - 1) {
             T e = T(0) + w[1];
-            lu_dst[o + 1] = lu[o + 1] + omega * e;
+            lu_dst[o + 1] = Ld::get(lu + o + 1) + omega * e;
         }
     }
     {
         T d = w[2];
         if (i > 0) d = d + s_w3[ty][tx];
-        lv_dst[c] = lv[c] + omega * d;
+        lv_dst[c] = Ld::get(lv + c) + omega * d;
         if (i == ny - 1) {
             T e = T(0) + w[3];
-            lv_dst[c + nx] = lv[c + nx] + omega * e;
+            lv_dst[c + nx] = Ld::get(lv + c + nx) + omega * e;
         }
     }
-    lH_dst[c] = lH[c] + omega * w[4];
+    lH_dst[c] = Ld::get(lH + c) + omega * w[4];
+}
+
+template <typename T>
+__global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaArgs<T> a, const T* __restrict__ cache,
+                                                                           const T* __restrict__ lu,
+                                                                           const T* __restrict__ lv,
+                                                                           const T* __restrict__ lH,
+                                                                           T* __restrict__ lu_dst,
+                                                                           T* __restrict__ lv_dst,
+                                                                           T* __restrict__ lH_dst,
+                                                                           const T* __restrict__ fu,
+                                                                           const T* __restrict__ fv,
+                                                                           const T* __restrict__ fH, T calving_rate,
+                                                                           T omega) {
+    adjoint_pass_tile<T, LdPlain>(a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, omega,
+                                  blockIdx.y * TY, blockIdx.x * TX, threadIdx.y, threadIdx.x);
+}
+
+// n_iter adjoint passes of a level at most TX cells wide in one launch
+// (bitwise what n_iter k_adjoint_pass_fused launches give): working workgroup
+// b runs the pass of tile row b; pass p reads the multipliers from lu/lv/lH
+// (p even) or the alternate buffers (p odd) and writes the other set, with the
+// barriers and the one-XCD placement of k_vanka_persistent.  If a barrier
+// timed out, the last pass writes NaN multipliers.
+template <typename T>
+__global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_persistent(SsaArgs<T> a, const T* __restrict__ cache,
+                                                                           T* lu, T* lv, T* lH, T* alt_u, T* alt_v,
+                                                                           T* alt_H, const T* __restrict__ fu,
+                                                                           const T* __restrict__ fv,
+                                                                           const T* __restrict__ fH, T calving_rate,
+                                                                           T omega, int n_iter,
+                                                                           unsigned long long* sync) {
+    __shared__ int s_fail;
+    if (blockIdx.x % kXcds) return;
+    const int nb = gridDim.x / kXcds;
+    const int i0 = (blockIdx.x / kXcds) * TY;
+    const bool fenced = !persistent_start(sync, nb);
+#pragma unroll 1
+    for (int p = 0; p < n_iter; ++p) {
+        // see k_vanka_persistent: keeps the pass-invariant addresses from being hoisted
+        int ty = threadIdx.y, tx = threadIdx.x;
+        asm volatile("" : "+v"(ty), "+v"(tx));
+        const bool odd = (p & 1) != 0;
+        adjoint_pass_tile<T, LdCoherent>(a, cache, odd ? alt_u : lu, odd ? alt_v : lv, odd ? alt_H : lH,
+                                         odd ? lu : alt_u, odd ? lv : alt_v, odd ? lH : alt_H, fu, fv, fH,
+                                         calving_rate, omega, i0, 0, ty, tx);
+        if (p + 1 < n_iter) grid_barrier(sync, (unsigned long long)nb * (p + 2), fenced);
+    }
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        s_fail = __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = threadIdx.x;
+    if (s_fail && threadIdx.y < TY && i < a.ny && j < a.nx)
+        (((n_iter - 1) & 1) ? lH : alt_H)[(long)i * a.nx + j] = __longlong_as_double(0x7ff8000000000000ll);
 }
 
 // Doubles of dynamic LDS k_adjoint_coarse needs for an ny x nx level.
@@ -1661,6 +1718,25 @@ hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, const 
         a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, -omega);
     return hipGetLastError();
 }
+bool adjoint_persistent_fits(int ny, int nx) {
+    // the three sync words sit behind the alternating multipliers in the 5 ny nx workspace
+    return nx >= 1 && nx <= TX && ny >= 1 && (ny + TY - 1) / TY <= kAdjointPersistentMaxBlocks &&
+           2L * ny * nx >= (long)ny + nx + 3;
+}
+
+template <typename T>
+hipError_t launch_adjoint_persistent(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, T* alt_u, T* alt_v,
+                                     T* alt_H, const T* fu, const T* fv, const T* fH, T calving_rate, T omega,
+                                     int n_iter, unsigned long long* sync, hipStream_t st) {
+    if (n_iter <= 0) return hipSuccess;
+    if (!a.P.linear_drag || !adjoint_persistent_fits(a.ny, a.nx)) return hipErrorInvalidValue;
+    k_zero_sync<<<1, 64, 0, st>>>(sync);
+    TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx * n_iter, st);
+    k_adjoint_persistent<T><<<kXcds * ((a.ny + TY - 1) / TY), dim3(TX, kPassFusedRows), 0, st>>>(
+        a, cache, lu, lv, lH, alt_u, alt_v, alt_H, fu, fv, fH, calving_rate, -omega, n_iter, sync);
+    return hipGetLastError();
+}
+
 template <typename T>
 bool adjoint_coarse_fits(const SsaArgs<T>& a) {
     if (!a.P.linear_drag || a.ny < 1 || a.nx < 1 || (long)a.ny * a.nx > kAdjointCoarseMaxCells) return false;
@@ -1733,6 +1809,8 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_adjoint_pass_fused<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, T*,   \
                                                      T*, T*, const T*, const T*, const T*, T, T, hipStream_t);       \
     template bool adjoint_coarse_fits<T>(const SsaArgs<T>&);                                                         \
+    template hipError_t launch_adjoint_persistent<T>(const SsaArgs<T>&, const T*, T*, T*, T*, T*, T*, T*, const T*,  \
+                                                     const T*, const T*, T, T, int, unsigned long long*, hipStream_t); \
     template hipError_t launch_adjoint_coarse<T>(const SsaArgs<T>&, const T*, T*, T*, T*, const T*, const T*,          \
                                                  const T*, T, T, int, hipStream_t);                                  \
     template hipError_t launch_vjp<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, const T*, const T*, \

@@ -85,6 +85,17 @@ hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, const 
 #define GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE 512
 #endif
 constexpr long kAdjointPassFusedMaxCells = (long)GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE * GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE;
+// All adjoint passes of one call on a level at most 64 cells wide and
+// 4 * kAdjointPersistentMaxBlocks rows high as one persistent launch (one tile
+// row per workgroup, the barriers and sync words of launch_vanka_persistent);
+// linear drag only, multipliers alternating with alt_u/v/H as for the fused
+// passes.  At 64 x 64 (16 workgroups) one launch per pass measured faster.
+constexpr int kAdjointPersistentMaxBlocks = 8;
+bool adjoint_persistent_fits(int ny, int nx);
+template <typename T>
+hipError_t launch_adjoint_persistent(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, T* alt_u, T* alt_v,
+                                     T* alt_H, const T* fu, const T* fv, const T* fH, T calving_rate, T omega,
+                                     int n_iter, unsigned long long* sync, hipStream_t st);
 // All n_iter cached adjoint passes of a small level in one workgroup (linear
 // drag, ny * nx <= kAdjointCoarseMaxCells, its LDS within the device limit:
 // adjoint_coarse_fits); the same results as n_iter launch_vjp +

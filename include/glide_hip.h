@@ -231,6 +231,14 @@ int glide_adjoint_coarse_enable(int on);
  * top neighbours' residuals); 0 launches the transpose operator and the
  * adjoint pass separately.  Both give the same multipliers. */
 int glide_adjoint_pass_fused_enable(int on);
+/* Cached adjoint smoothing of levels at most 64 cells wide and 32 rows high
+ * with linear drag, when the one-workgroup form does not apply (no
+ * counterpart in the reference): 1 (default) runs all passes of one call as
+ * one persistent launch of one workgroup per four rows, with the barriers of
+ * glide_forward_persistent_enable (same workspace words; a timed-out barrier
+ * leaves NaN multipliers); 0 launches one fused pass at a time.  Both give
+ * the same multipliers. */
+int glide_adjoint_persistent_enable(int on);
 
 /* Measurement support (no counterpart in the reference): when enabled, HIP
  * event pairs are recorded around every launch of the hot kernels

@@ -435,6 +435,44 @@ def test_adjoint_pass_fused_matches_launches(shape):
         assert rel_err(x, y) < 1e-13
 
 
+@pytest.mark.parametrize("shape,n_pass", [((32, 32), 200), ((17, 29), 6), ((32, 64), 7), ((5, 64), 3), ((2, 3), 4)])
+def test_adjoint_persistent_matches_fused_launches(shape, n_pass):
+    """All cached adjoint passes of a level at most 64 wide and 32 high in one persistent
+    launch give the multipliers of one fused launch per pass (the same
+    arithmetic; the compiler may contract a product into a different FMA in
+    the other kernel, so the bar is relative round-off after up to 200
+    passes)."""
+    from glide_amd import _lib
+    ny, nx = shape
+    print(f"adjoint persistent shape={shape} passes={n_pass}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=27, mask_frac=0.15)
+    out = []
+    _lib.adjoint_coarse_enable(False)
+    try:
+        for persistent in (True, False):
+            _lib.adjoint_persistent_enable(persistent)
+            g = grid_from(F)
+            g.sliding.m.set(1.0)
+            g.calving.calving_rate.set(0.3)
+            for t, x in zip((g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H), _lam(F, 28)):
+                t.set(x)
+            ao = g.adjoint_operators
+            ao.use_cache = True
+            rng = np.random.default_rng(29)
+            for t in (ao.f_u, ao.f_v, ao.f_H):
+                t.copy_(T(rng.standard_normal(tuple(t.shape))))
+            ao.vanka_sweep(10.0, n_pass)
+            torch.cuda.synchronize()
+            out.append([N(t.data) for t in (g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H)])
+    finally:
+        _lib.adjoint_persistent_enable(True)
+        _lib.adjoint_coarse_enable(True)
+    assert all(np.isfinite(x).all() for x in out[0])
+    for x, y in zip(out[0], out[1]):
+        print(f"  persistent vs fused launches rel diff {rel_err(x, y):.3e}", flush=True)
+        assert rel_err(x, y) < 1e-12
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_gradients(orc, shape):
     ny, nx = shape
