@@ -1288,14 +1288,19 @@ __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __rest
 // transpose operator's work) and saves a launch, the residual planes' round
 // trip through HBM and the second kernel's ramp: 9.7 against 12.6 us per
 // pass at 32², 14.6 against 19.8 us at 256², and still ahead at 512².
-constexpr int kPassFusedRows = 2 * TY;  // thread rows: the ring's patch solves run beside the tile's
+// The one-launch adjoint pass uses TY-row tiles (2 TY thread rows, so that the
+// ring's patch solves run beside the tile's).  Each tile recomputes the
+// transpose operator on its top row and left column, (TY + 2) / TY of the
+// work; 8-row tiles (1024 threads, 88 KB of LDS) measured 35.2 against
+// 38.6 us per pass at 512² alone, but the coupled step, whose enthalpy
+// kernels share the GPU with these passes, ran 0.75 ms slower with them.
 
-// One adjoint pass over the TY x TX tile at (i0, j0) with TX x kPassFusedRows
-// threads: k_vjp's body over the tile plus its left column and top row (into
-// LDS), the cached patch solves of the tile (thread rows < TY) and of the ring
-// (the other threads), then the update of the tile's multipliers into the
+// One adjoint pass over the TR x TX tile at (i0, j0) with TX x 2 TR threads:
+// k_vjp's body over the tile plus its left column and top row (into LDS), the
+// cached patch solves of the tile (thread rows < TR) and of the ring (the
+// other threads), then the update of the tile's multipliers into the
 // destination buffers.
-template <typename T, class Ld>
+template <typename T, class Ld, int TR>
 __device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* __restrict__ cache,
                                                   const T* __restrict__ lu, const T* __restrict__ lv,
                                                   const T* __restrict__ lH, T* __restrict__ lu_dst,
@@ -1303,13 +1308,13 @@ __device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* 
                                                   const T* __restrict__ fu, const T* __restrict__ fv,
                                                   const T* __restrict__ fH, T calving_rate, T omega, int i0, int j0,
                                                   int ty, int tx) {
-    constexpr int RR = TY + 2, RC = TX + 2;
+    constexpr int RR = TR + 2, RC = TX + 2;
     constexpr int CW = RC + 2, NCELL = (RR + 2) * CW, VW = RC + 1, NVERT = (RR + 1) * VW;
     __shared__ T s_cell[8 * NCELL];
     __shared__ T s_vert[2 * NVERT];
     __shared__ T r_u[RR][RC], r_v[RR][RC], r_h[RR][RC];
-    __shared__ T s_w1[TY][TX + 1];  // right-facet increment of cells (i0 + a, j0 - 1 + b)
-    __shared__ T s_w3[TY + 1][TX];  // bottom-facet increment of cells (i0 - 1 + a, j0 + b)
+    __shared__ T s_w1[TR][TX + 1];  // right-facet increment of cells (i0 + a, j0 - 1 + b)
+    __shared__ T s_w3[TR + 1][TX];  // bottom-facet increment of cells (i0 - 1 + a, j0 + b)
     VjpLevelStore<T> store;
     store.cell = s_cell;
     store.vert = s_vert;
@@ -1321,7 +1326,7 @@ __device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* 
     const int ny = a.ny, nx = a.nx;
     const long n = (long)ny * nx;
     const int tid = ty * TX + tx;
-    constexpr int NTF = TX * kPassFusedRows;
+    constexpr int NTF = TX * 2 * TR;
     vjp_body<T, VjpLevelStore<T>, VjpTileOut<T, RC>, Ld>(a, lu, lv, lH, fu, fv, fH, out, calving_rate,
                                                           cache + (long)kAdjointCacheEta * n, i0 - 1, j0 - 1, RR, RC,
                                                           tid, NTF, store);
@@ -1331,29 +1336,29 @@ __device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* 
         const T b[5] = {r_u[ti][tj], r_u[ti][tj + 1], r_v[ti][tj], r_v[ti + 1][tj], r_h[ti][tj]};
         adjoint_cached_solve(cache, n, (long)ci * nx + cj, b, w);
     };
-    // thread rows 0 .. TY-1 own the tile's cells, the rest solve the ring
+    // thread rows 0 .. TR-1 own the tile's cells, the rest solve the ring
     const int i = i0 + ty, j = j0 + tx;
-    const bool own = ty < TY && i < ny && j < nx;
+    const bool own = ty < TR && i < ny && j < nx;
     T w[5] = {T(0), T(0), T(0), T(0), T(0)};
     if (own) {
         increments(i, j, w);
         s_w1[ty][tx + 1] = w[1];
         s_w3[ty + 1][tx] = w[3];
     }
-    const int rt = tid - TY * TX;
-    if (rt >= 0 && rt < TY) {
+    const int rt = tid - TR * TX;
+    if (rt >= 0 && rt < TR) {
         const int ci = i0 + rt, cj = j0 - 1;
         if (cj >= 0 && ci < ny) {
             T wr[5];
             increments(ci, cj, wr);
             s_w1[rt][0] = wr[1];
         }
-    } else if (rt >= TY && rt < TY + TX) {
-        const int ci = i0 - 1, cj = j0 + (rt - TY);
+    } else if (rt >= TR && rt < TR + TX) {
+        const int ci = i0 - 1, cj = j0 + (rt - TR);
         if (ci >= 0 && cj < nx) {
             T wr[5];
             increments(ci, cj, wr);
-            s_w3[0][rt - TY] = wr[3];
+            s_w3[0][rt - TR] = wr[3];
         }
     }
     __syncthreads();
@@ -1381,8 +1386,8 @@ __device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* 
     lH_dst[c] = Ld::get(lH + c) + omega * w[4];
 }
 
-template <typename T>
-__global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaArgs<T> a, const T* __restrict__ cache,
+template <typename T, int TR>
+__global__ __launch_bounds__(TX * 2 * TR) void k_adjoint_pass_fused(SsaArgs<T> a, const T* __restrict__ cache,
                                                                            const T* __restrict__ lu,
                                                                            const T* __restrict__ lv,
                                                                            const T* __restrict__ lH,
@@ -1393,8 +1398,8 @@ __global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaA
                                                                            const T* __restrict__ fv,
                                                                            const T* __restrict__ fH, T calving_rate,
                                                                            T omega) {
-    adjoint_pass_tile<T, LdPlain>(a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, omega,
-                                  blockIdx.y * TY, blockIdx.x * TX, threadIdx.y, threadIdx.x);
+    adjoint_pass_tile<T, LdPlain, TR>(a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, omega,
+                                      blockIdx.y * TR, blockIdx.x * TX, threadIdx.y, threadIdx.x);
 }
 
 // n_iter adjoint passes of a level at most TX cells wide in one launch
@@ -1404,7 +1409,7 @@ __global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_pass_fused(SsaA
 // barriers and the one-XCD placement of k_vanka_persistent.  If a barrier
 // timed out, the last pass writes NaN multipliers.
 template <typename T>
-__global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_persistent(SsaArgs<T> a, const T* __restrict__ cache,
+__global__ __launch_bounds__(TX * 2 * TY) void k_adjoint_persistent(SsaArgs<T> a, const T* __restrict__ cache,
                                                                            T* lu, T* lv, T* lH, T* alt_u, T* alt_v,
                                                                            T* alt_H, const T* __restrict__ fu,
                                                                            const T* __restrict__ fv,
@@ -1422,7 +1427,7 @@ __global__ __launch_bounds__(TX * kPassFusedRows) void k_adjoint_persistent(SsaA
         int ty = threadIdx.y, tx = threadIdx.x;
         asm volatile("" : "+v"(ty), "+v"(tx));
         const bool odd = (p & 1) != 0;
-        adjoint_pass_tile<T, LdCoherent>(a, cache, odd ? alt_u : lu, odd ? alt_v : lv, odd ? alt_H : lH,
+        adjoint_pass_tile<T, LdCoherent, TY>(a, cache, odd ? alt_u : lu, odd ? alt_v : lv, odd ? alt_H : lH,
                                          odd ? lu : alt_u, odd ? lv : alt_v, odd ? lH : alt_H, fu, fv, fH,
                                          calving_rate, omega, i0, 0, ty, tx);
         if (p + 1 < n_iter) grid_barrier(sync, (unsigned long long)nb * (p + 2), fenced);
@@ -1714,7 +1719,7 @@ hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, const 
                                      T calving_rate, T omega, hipStream_t st) {
     if (!a.P.linear_drag) return hipErrorInvalidValue;
     TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx, st);
-    k_adjoint_pass_fused<T><<<tile_grid(a.ny, a.nx, false), dim3(TX, kPassFusedRows), 0, st>>>(
+    k_adjoint_pass_fused<T, TY><<<tile_grid(a.ny, a.nx, false), dim3(TX, 2 * TY), 0, st>>>(
         a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, -omega);
     return hipGetLastError();
 }
@@ -1732,7 +1737,7 @@ hipError_t launch_adjoint_persistent(const SsaArgs<T>& a, const T* cache, T* lu,
     if (!a.P.linear_drag || !adjoint_persistent_fits(a.ny, a.nx)) return hipErrorInvalidValue;
     k_zero_sync<<<1, 64, 0, st>>>(sync);
     TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx * n_iter, st);
-    k_adjoint_persistent<T><<<kXcds * ((a.ny + TY - 1) / TY), dim3(TX, kPassFusedRows), 0, st>>>(
+    k_adjoint_persistent<T><<<kXcds * ((a.ny + TY - 1) / TY), dim3(TX, 2 * TY), 0, st>>>(
         a, cache, lu, lv, lH, alt_u, alt_v, alt_H, fu, fv, fH, calving_rate, -omega, n_iter, sync);
     return hipGetLastError();
 }

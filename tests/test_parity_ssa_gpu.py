@@ -400,7 +400,7 @@ def test_adjoint_coarse_workgroup_matches_launches(shape):
         assert rel_err(x, y) < 1e-13
 
 
-@pytest.mark.parametrize("shape", [(36, 52), (70, 130), (17, 29)])
+@pytest.mark.parametrize("shape", [(36, 52), (70, 130), (17, 29), (1030, 200)])
 def test_adjoint_pass_fused_matches_launches(shape):
     """A cached adjoint pass as one launch (transpose operator, patch solves
     and update, each tile recomputing its left/top neighbours' residuals)
