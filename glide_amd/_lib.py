@@ -192,7 +192,7 @@ def forward_fused_enable(on: bool) -> None:
 
 
 def forward_persistent_enable(on: bool) -> None:
-    """Forward passes of levels at most 64 x 64 as one persistent launch per
+    """Forward passes of levels at most 64 wide, 96 high as one persistent launch per
     pass sequence (default) or one launch per pass."""
     call("glide_forward_persistent_enable", int(bool(on)))
 

@@ -161,7 +161,7 @@ bool g_coarse_adjoint = true;
 // Cached adjoint passes of levels up to kAdjointPassFusedMaxCells as one launch
 // each, transpose operator included (glide_adjoint_pass_fused_enable).
 bool g_pass_fused_adjoint = true;
-// Forward passes of levels at most 64 cells wide and 64 rows high as one
+// Forward passes of levels at most 64 cells wide and 96 rows high as one
 // persistent launch per pass sequence (glide_forward_persistent_enable).
 bool g_persistent_forward = true;
 // Adjoint passes of the same levels likewise (glide_adjoint_persistent_enable).

@@ -60,7 +60,10 @@ constexpr long kFusedVankaMaxCells = (long)GLIDE_FUSED_VANKA_MAX_SIDE * GLIDE_FU
 #define GLIDE_PERSISTENT_ROWS 3
 #endif
 constexpr int kPersistentRows = GLIDE_PERSISTENT_ROWS;
-constexpr int kPersistentMaxBlocks = 16;
+#ifndef GLIDE_PERSISTENT_MAX_BLOCKS
+#define GLIDE_PERSISTENT_MAX_BLOCKS 32
+#endif
+constexpr int kPersistentMaxBlocks = GLIDE_PERSISTENT_MAX_BLOCKS;
 bool vanka_persistent_fits(int ny, int nx);
 template <typename T>
 hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* alt_H, T* mask,

@@ -210,9 +210,9 @@ int glide_adjoint_fused_enable(int on);
  * buffers, the level's own and its workspace); 0 solves into the workspace
  * and applies in a second launch.  Both give the same state. */
 int glide_forward_fused_enable(int on);
-/* Forward smoothing of levels at most 64 cells wide and 64 rows high with a
+/* Forward smoothing of levels at most 64 cells wide and 96 rows high with a
  * frozen grounded fraction (no counterpart in the reference): 1 (default) runs
- * all passes of one call as one persistent launch of one workgroup per four
+ * all passes of one call as one persistent launch of one workgroup per three
  * rows, a device-wide barrier between passes (it uses the last two values of
  * the level's workspace as its arrival counter and timeout flag; a timed-out
  * barrier leaves NaN thicknesses); 0 launches each pass as above.  Both give

@@ -212,9 +212,10 @@ def test_vanka_fused_passes_match_two_launch_passes(orc, shape, n_pass, m):
 
 
 @pytest.mark.parametrize("shape,n_pass,m", [((32, 32), 200, 1.0), ((32, 32), 7, 1.0 / 3.0), ((17, 29), 6, 1.0),
-                                             ((5, 64), 4, 1.0), ((48, 64), 3, 1.0), ((2, 3), 5, 1.0)])
+                                             ((5, 64), 4, 1.0), ((48, 64), 3, 1.0), ((2, 3), 5, 1.0),
+                                             ((64, 64), 30, 1.0), ((96, 41), 5, 1.0 / 3.0)])
 def test_vanka_persistent_passes_match_fused_launches(shape, n_pass, m):
-    """All passes of a level at most 64 x 64 in one persistent launch (rows
+    """All passes of a level at most 64 wide, 96 high in one persistent launch (rows
     split over workgroups, device-wide barriers between passes) give bitwise
     the state and active set of one fused launch per pass."""
     from glide_amd import _lib
