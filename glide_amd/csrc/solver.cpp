@@ -188,7 +188,7 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
     // small levels with a frozen grounded fraction: one fused launch per pass,
     // the state alternating between the level's buffers and the workspace
     // (5 planes hold u, v and H); an odd pass count copies it back at the end
-    if (freeze_phi && !vk.local_fp32 && g_fused_forward && (long)L.ny * L.nx <= kFusedVankaMaxCells) {
+    if (freeze_phi && g_fused_forward && (long)L.ny * L.nx <= kFusedVankaMaxCells) {
         R* alt_u = P(L.work);
         R* alt_v = alt_u + nU(L);
         R* alt_H = alt_v + nV(L);

@@ -475,7 +475,7 @@ __global__ __launch_bounds__(TX * ROWS) void k_vanka_cell(SsaArgs<T> a, T* __res
 // pass reads the state from a (unchanged during the pass, so no block sees
 // another block's update) and writes u_dst, v_dst, H_dst; the active-set flag
 // is written by the owner in place (no other cell reads it).
-template <typename T, bool LINEAR>
+template <typename T, bool LINEAR, typename TJ>
 __global__ __launch_bounds__(3 * TX) void k_vanka_fused(SsaArgs<T> a, T* __restrict__ u_dst, T* __restrict__ v_dst,
                                                         T* __restrict__ H_dst, T* __restrict__ mask, VankaArgs<T> vk,
                                                         T omega) {
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(3 * TX) void k_vanka_fused(SsaArgs<T> a, T* __restr
     T masked = (wv == 0 && active) ? mask[(long)i0 * nx + cj] : T(0);
     int it = 0;
     if (active) {
-        cell_newton<T, LINEAR, T, W>(a, s_eta, ci - i0 + 2, cj - j0 + 2, ci, cj, vk, edge, w, masked, it);
+        cell_newton<T, LINEAR, TJ, W>(a, s_eta, ci - i0 + 2, cj - j0 + 2, ci, cj, vk, edge, w, masked, it);
         if (wv == 0) s_w1[lane + 1] = w[1];
         else if (wv == 1) s_w3[lane] = w[3];
         else s_w1[0] = w[1];
@@ -610,7 +610,7 @@ __global__ __launch_bounds__(64) void k_zero_sync(unsigned long long* sync) {
 // a.u/v/H (p even) or the alternate buffers (p odd) and writes the other set;
 // a barrier separates the passes.  If a barrier timed out, the last pass
 // writes NaN thicknesses, which the caller's residual norms then show.
-template <typename T, bool LINEAR, int RW>
+template <typename T, bool LINEAR, typename TJ, int RW>
 __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a, T* alt_u, T* alt_v, T* alt_H,
                                                                     T* __restrict__ mask, VankaArgs<T> vk, T omega,
                                                                     int n_iter, unsigned long long* sync) {
@@ -661,7 +661,7 @@ __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a
         T masked = own ? Ld::get(mask + (long)ci * nx + cj) : T(0);
         int it = 0;
         if (active) {
-            cell_newton<T, LINEAR, T, W, Ld>(s, s_eta, ci - i0 + 2, cj + 2, ci, cj, vk, true, w, masked, it);
+            cell_newton<T, LINEAR, TJ, W, Ld>(s, s_eta, ci - i0 + 2, cj + 2, ci, cj, vk, true, w, masked, it);
             if (wv < RW) s_w1[wv][lane + 1] = w[1];
             s_w3[wv][lane] = w[3];
         }
@@ -1653,8 +1653,14 @@ hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst,
                               T omega, hipStream_t st) {
     TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx, st);
     const dim3 g((a.nx + TX - 1) / TX, a.ny), b(TX, 3);
-    if (a.P.linear_drag) k_vanka_fused<T, true><<<g, b, 0, st>>>(a, u_dst, v_dst, H_dst, mask, vk, omega);
-    else k_vanka_fused<T, false><<<g, b, 0, st>>>(a, u_dst, v_dst, H_dst, mask, vk, omega);
+    // the mixed form evaluates and factorises the patch Jacobian in FP32
+    if (vk.local_fp32) {
+        if (a.P.linear_drag) k_vanka_fused<T, true, float><<<g, b, 0, st>>>(a, u_dst, v_dst, H_dst, mask, vk, omega);
+        else k_vanka_fused<T, false, float><<<g, b, 0, st>>>(a, u_dst, v_dst, H_dst, mask, vk, omega);
+    } else {
+        if (a.P.linear_drag) k_vanka_fused<T, true, T><<<g, b, 0, st>>>(a, u_dst, v_dst, H_dst, mask, vk, omega);
+        else k_vanka_fused<T, false, T><<<g, b, 0, st>>>(a, u_dst, v_dst, H_dst, mask, vk, omega);
+    }
     return hipGetLastError();
 }
 
@@ -1675,12 +1681,19 @@ hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* a
     k_zero_sync<<<1, 64, 0, st>>>(sync);
     TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx * n_iter, st);
     const dim3 g(kXcds * ((a.ny + kPersistentRows - 1) / kPersistentRows)), b(TX, kPersistentRows + 1);
-    if (a.P.linear_drag)
-        k_vanka_persistent<T, true, kPersistentRows><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter,
-                                                                      sync);
-    else
-        k_vanka_persistent<T, false, kPersistentRows><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega,
-                                                                       n_iter, sync);
+    constexpr int R = kPersistentRows;
+    if (vk.local_fp32) {
+        if (a.P.linear_drag)
+            k_vanka_persistent<T, true, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync);
+        else
+            k_vanka_persistent<T, false, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter,
+                                                                    sync);
+    } else {
+        if (a.P.linear_drag)
+            k_vanka_persistent<T, true, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync);
+        else
+            k_vanka_persistent<T, false, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync);
+    }
     return hipGetLastError();
 }
 

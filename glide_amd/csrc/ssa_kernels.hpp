@@ -42,8 +42,9 @@ hipError_t launch_vanka_pass(const SsaArgs<T>& a, T* u, T* v, T* H, T* work, T* 
 // One forward smoothing pass in one launch (patch solves of each tile row and
 // of the cells above / left of it, then the update of the owned state): reads
 // the state of a, writes the updated u, v, H to u_dst, v_dst, H_dst (buffers
-// distinct from a's) and the active-set flags to mask.  Uses the plain FP64
-// form; for levels whose patch solves are latency-bound.
+// distinct from a's) and the active-set flags to mask.  vk.local_fp32 selects
+// the mixed form (patch Jacobian and LU in FP32), as for launch_vanka_pass;
+// for levels whose patch solves are latency-bound.
 template <typename T>
 hipError_t launch_vanka_fused(const SsaArgs<T>& a, T* u_dst, T* v_dst, T* H_dst, T* mask, const VankaArgs<T>& vk,
                               T omega, hipStream_t st);

@@ -237,6 +237,34 @@ def test_vanka_persistent_passes_match_fused_launches(shape, n_pass, m):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("shape,n_pass", [((32, 32), 30), ((64, 64), 5), ((128, 128), 3), ((17, 29), 4)])
+def test_vanka_mixed_one_launch_forms_match_two_launch_passes(shape, n_pass):
+    """The mixed form (FP32 patch Jacobian and LU) through the one-launch
+    (fused, persistent) forward passes gives the state and active set of its
+    solve + apply passes (the same arithmetic; the compiler may contract a
+    product into a different FMA in the other kernel)."""
+    from glide_amd import _lib
+    ny, nx = shape
+    print(f"vanka mixed one-launch shape={shape} passes={n_pass}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=45, mask_frac=0.1)
+    out = []
+    for one_launch in (True, False):
+        _lib.forward_fused_enable(one_launch)
+        g = grid_from(F)
+        g.sliding.m.set(1.0)
+        g.calving.calving_rate.set(0.2)
+        fo = g.forward_operators
+        fo.vanka_config.local_fp32 = True
+        fo.vanka_sweep(10.0, n_pass, freeze_phi=True)
+        torch.cuda.synchronize()
+        out.append([N(t.data) for t in (g.state.u, g.state.v, g.state.H, g.state.mask)])
+    _lib.forward_fused_enable(True)
+    for x, y in zip(out[0][:3], out[1][:3]):
+        print(f"  one-launch vs two-launch max rel diff {rel_err(x, y):.3e}", flush=True)
+        assert rel_err(x, y) < 1e-13
+    assert np.array_equal(out[0][3], out[1][3])
+
+
 def test_vanka_passes_with_phi_refresh(orc):
     F = dome_fields(n=32, seed=6)
     g = grid_from(F)
