@@ -553,6 +553,15 @@ __global__ __launch_bounds__(3 * TX) void k_vanka_fused(SsaArgs<T> a, T* __restr
 // error it can see).
 constexpr int kGridSpinLimit = 1 << 20;
 
+// State loads of the persistent kernels: device-scope loads (default), or
+// plain loads after an agent-scope acquire at every barrier (a tuning switch:
+// the same results; per pass 24.5 against 25.6 us for the 32² forward passes,
+// equal at 64², 8.70 against 8.62 us for the 32² adjoint passes).
+#ifndef GLIDE_PERSISTENT_PLAIN_LOADS
+#define GLIDE_PERSISTENT_PLAIN_LOADS 0
+#endif
+using LdPersistent = std::conditional_t<GLIDE_PERSISTENT_PLAIN_LOADS != 0, LdPlain, LdCoherent>;
+
 __device__ __forceinline__ void grid_barrier(unsigned long long* sync, unsigned long long target, bool fenced) {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -569,6 +578,7 @@ __device__ __forceinline__ void grid_barrier(unsigned long long* sync, unsigned 
             __builtin_amdgcn_s_sleep(1);
         }
         if (fenced) __threadfence();
+        else if (GLIDE_PERSISTENT_PLAIN_LOADS) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
@@ -614,7 +624,7 @@ template <typename T, bool LINEAR, typename TJ, int RW>
 __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a, T* alt_u, T* alt_v, T* alt_H,
                                                                     T* __restrict__ mask, VankaArgs<T> vk, T omega,
                                                                     int n_iter, unsigned long long* sync) {
-    using Ld = LdCoherent;
+    using Ld = LdPersistent;
     constexpr int W = TX + 3;
     __shared__ T s_eta[RW + 3][W];  // cells (i0 - 2 .. i0 + RW, -2 .. TX)
     __shared__ T s_w1[RW][TX + 1];  // right-facet increments of the owned rows' cells (cell j at j + 1)
@@ -1427,7 +1437,7 @@ __global__ __launch_bounds__(TX * 2 * TY) void k_adjoint_persistent(SsaArgs<T> a
         int ty = threadIdx.y, tx = threadIdx.x;
         asm volatile("" : "+v"(ty), "+v"(tx));
         const bool odd = (p & 1) != 0;
-        adjoint_pass_tile<T, LdCoherent, TY>(a, cache, odd ? alt_u : lu, odd ? alt_v : lv, odd ? alt_H : lH,
+        adjoint_pass_tile<T, LdPersistent, TY>(a, cache, odd ? alt_u : lu, odd ? alt_v : lv, odd ? alt_H : lH,
                                          odd ? lu : alt_u, odd ? lv : alt_v, odd ? lH : alt_H, fu, fv, fH,
                                          calving_rate, omega, i0, 0, ty, tx);
         if (p + 1 < n_iter) grid_barrier(sync, (unsigned long long)nb * (p + 2), fenced);
