@@ -146,6 +146,7 @@ SIGNATURES = {
     "glide_adjoint_coarse_enable": (_I, [_I]),
     "glide_adjoint_pass_fused_enable": (_I, [_I]),
     "glide_adjoint_persistent_enable": (_I, [_I]),
+    "glide_persistent_fenced_enable": (_I, [_I]),
     "glide_graphs_clear": (_I, []),
     "glide_timing_enable": (_I, [_I]),
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
@@ -213,6 +214,12 @@ def adjoint_persistent_enable(on: bool) -> None:
     """Cached adjoint passes of levels at most 64 wide, 32 high (linear drag) as one
     persistent launch per call (default) or one fused launch per pass."""
     call("glide_adjoint_persistent_enable", int(bool(on)))
+
+
+def persistent_fenced_enable(on: bool) -> None:
+    """Persistent launches with the cross-XCD cache maintenance at every
+    barrier even when their workgroups share one XCD (testing switch)."""
+    call("glide_persistent_fenced_enable", int(bool(on)))
 
 
 def graphs_clear() -> None:

@@ -166,6 +166,8 @@ bool g_pass_fused_adjoint = true;
 bool g_persistent_forward = true;
 // Adjoint passes of the same levels likewise (glide_adjoint_persistent_enable).
 bool g_persistent_adjoint = true;
+// Persistent launches with fenced barriers even on one XCD (glide_persistent_fenced_enable).
+bool g_persistent_fenced = false;
 
 // Three 64-bit words at the end of a level's smoother workspace (5 ny nx
 // values, of which the alternating state of the persistent and fused passes
@@ -194,7 +196,7 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
         R* alt_H = alt_v + nV(L);
         if (g_persistent_forward && n_iter > 1 && vanka_persistent_fits(L.ny, L.nx)) {
             HIP_TRY(launch_vanka_persistent<R>(make_args(L, dt, false), alt_u, alt_v, alt_H, P(L.mask), vk, vc.omega,
-                                               n_iter, work_sync(L), st),
+                                               n_iter, work_sync(L), g_persistent_fenced ? 1 : 0, st),
                     "vanka passes (persistent)");
         } else {
             for (int it = 0; it < n_iter; ++it) {
@@ -292,7 +294,8 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
             if (g_persistent_adjoint && n_iter > 1 && adjoint_persistent_fits(L.ny, L.nx)) {
                 HIP_TRY(launch_adjoint_persistent<R>(a, cache, P(L.lambda_u), P(L.lambda_v), P(L.lambda_H), alt_u,
                                                      alt_v, alt_H, CP(L.af_u), CP(L.af_v), CP(L.af_H),
-                                                     vk.calving_rate, vc.omega, n_iter, work_sync(L), st),
+                                                     vk.calving_rate, vc.omega, n_iter, work_sync(L),
+                                                     g_persistent_fenced ? 1 : 0, st),
                         "adjoint passes (persistent)");
                 n_iter_done = n_iter;
             }
@@ -616,6 +619,7 @@ std::string solve_key(char tag, const FasCtx& X, int start, bool norms) {
     key_append(k, g_pass_fused_adjoint);
     key_append(k, g_persistent_forward);
     key_append(k, g_persistent_adjoint);
+    key_append(k, g_persistent_fenced);
     key_append(k, X.n_levels);
     key_append(k, start);
     key_append(k, X.dt);
@@ -966,6 +970,11 @@ int glide_forward_persistent_enable(int on) {
 
 int glide_adjoint_persistent_enable(int on) {
     g_persistent_adjoint = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_persistent_fenced_enable(int on) {
+    g_persistent_fenced = on != 0;
     return GLIDE_OK;
 }
 

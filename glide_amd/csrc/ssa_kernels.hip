@@ -623,7 +623,8 @@ __global__ __launch_bounds__(64) void k_zero_sync(unsigned long long* sync) {
 template <typename T, bool LINEAR, typename TJ, int RW>
 __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a, T* alt_u, T* alt_v, T* alt_H,
                                                                     T* __restrict__ mask, VankaArgs<T> vk, T omega,
-                                                                    int n_iter, unsigned long long* sync) {
+                                                                    int n_iter, unsigned long long* sync,
+                                                                    int force_fenced) {
     using Ld = LdPersistent;
     constexpr int W = TX + 3;
     __shared__ T s_eta[RW + 3][W];  // cells (i0 - 2 .. i0 + RW, -2 .. TX)
@@ -641,7 +642,7 @@ __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a
     T* const base_u = const_cast<T*>(a.u);
     T* const base_v = const_cast<T*>(a.v);
     T* const base_H = const_cast<T*>(a.H);
-    const bool fenced = !persistent_start(sync, nb);
+    const bool fenced = !persistent_start(sync, nb) || force_fenced;
 #pragma unroll 1
     for (int p = 0; p < n_iter; ++p) {
         // the lane's indices pass through an empty asm statement every pass, so
@@ -1425,12 +1426,13 @@ __global__ __launch_bounds__(TX * 2 * TY) void k_adjoint_persistent(SsaArgs<T> a
                                                                            const T* __restrict__ fv,
                                                                            const T* __restrict__ fH, T calving_rate,
                                                                            T omega, int n_iter,
-                                                                           unsigned long long* sync) {
+                                                                           unsigned long long* sync,
+                                                                           int force_fenced) {
     __shared__ int s_fail;
     if (blockIdx.x % kXcds) return;
     const int nb = gridDim.x / kXcds;
     const int i0 = (blockIdx.x / kXcds) * TY;
-    const bool fenced = !persistent_start(sync, nb);
+    const bool fenced = !persistent_start(sync, nb) || force_fenced;
 #pragma unroll 1
     for (int p = 0; p < n_iter; ++p) {
         // see k_vanka_persistent: keeps the pass-invariant addresses from being hoisted
@@ -1683,7 +1685,7 @@ bool vanka_persistent_fits(int ny, int nx) {
 template <typename T>
 hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* alt_H, T* mask,
                                    const VankaArgs<T>& vk, T omega, int n_iter, unsigned long long* sync,
-                                   hipStream_t st) {
+                                   int force_fenced, hipStream_t st) {
     if (n_iter <= 0) return hipSuccess;
     if (!vanka_persistent_fits(a.ny, a.nx)) return hipErrorInvalidValue;
     // the arrival counter, the timeout flag and the XCD mask start at zero (a
@@ -1692,17 +1694,20 @@ hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* a
     TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx * n_iter, st);
     const dim3 g(kXcds * ((a.ny + kPersistentRows - 1) / kPersistentRows)), b(TX, kPersistentRows + 1);
     constexpr int R = kPersistentRows;
+    const int ff = force_fenced;
     if (vk.local_fp32) {
         if (a.P.linear_drag)
-            k_vanka_persistent<T, true, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync);
+            k_vanka_persistent<T, true, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync,
+                                                                   ff);
         else
             k_vanka_persistent<T, false, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter,
-                                                                    sync);
+                                                                    sync, ff);
     } else {
         if (a.P.linear_drag)
-            k_vanka_persistent<T, true, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync);
+            k_vanka_persistent<T, true, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync, ff);
         else
-            k_vanka_persistent<T, false, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync);
+            k_vanka_persistent<T, false, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync,
+                                                                ff);
     }
     return hipGetLastError();
 }
@@ -1755,13 +1760,13 @@ bool adjoint_persistent_fits(int ny, int nx) {
 template <typename T>
 hipError_t launch_adjoint_persistent(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, T* alt_u, T* alt_v,
                                      T* alt_H, const T* fu, const T* fv, const T* fH, T calving_rate, T omega,
-                                     int n_iter, unsigned long long* sync, hipStream_t st) {
+                                     int n_iter, unsigned long long* sync, int force_fenced, hipStream_t st) {
     if (n_iter <= 0) return hipSuccess;
     if (!a.P.linear_drag || !adjoint_persistent_fits(a.ny, a.nx)) return hipErrorInvalidValue;
     k_zero_sync<<<1, 64, 0, st>>>(sync);
     TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx * n_iter, st);
     k_adjoint_persistent<T><<<kXcds * ((a.ny + TY - 1) / TY), dim3(TX, 2 * TY), 0, st>>>(
-        a, cache, lu, lv, lH, alt_u, alt_v, alt_H, fu, fv, fH, calving_rate, -omega, n_iter, sync);
+        a, cache, lu, lv, lH, alt_u, alt_v, alt_H, fu, fv, fH, calving_rate, -omega, n_iter, sync, force_fenced);
     return hipGetLastError();
 }
 
@@ -1831,14 +1836,15 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_vanka_dump<T>(const SsaArgs<T>&, T*, T*, T, hipStream_t);                              \
     template hipError_t launch_vanka_fused<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, hipStream_t);\
     template hipError_t launch_vanka_persistent<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, int,    \
-                                                  unsigned long long*, hipStream_t);                              \
+                                                  unsigned long long*, int, hipStream_t);                         \
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
     template hipError_t launch_adjoint_pass_fused<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, T*,   \
                                                      T*, T*, const T*, const T*, const T*, T, T, hipStream_t);       \
     template bool adjoint_coarse_fits<T>(const SsaArgs<T>&);                                                         \
     template hipError_t launch_adjoint_persistent<T>(const SsaArgs<T>&, const T*, T*, T*, T*, T*, T*, T*, const T*,  \
-                                                     const T*, const T*, T, T, int, unsigned long long*, hipStream_t); \
+                                                     const T*, const T*, T, T, int, unsigned long long*, int,         \
+                                                     hipStream_t);                                                    \
     template hipError_t launch_adjoint_coarse<T>(const SsaArgs<T>&, const T*, T*, T*, T*, const T*, const T*,          \
                                                  const T*, T, T, int, hipStream_t);                                  \
     template hipError_t launch_vjp<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, const T*, const T*, \

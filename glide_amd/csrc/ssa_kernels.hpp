@@ -56,7 +56,8 @@ constexpr long kFusedVankaMaxCells = (long)GLIDE_FUSED_VANKA_MAX_SIDE * GLIDE_FU
 // kPersistentRows * kPersistentMaxBlocks rows high as one persistent launch
 // (kPersistentRows rows per workgroup, a barrier between passes); sync holds
 // three 64-bit words the launch zeroes first (arrival counter, timeout flag,
-// XCD mask).
+// XCD mask); force_fenced != 0 runs every barrier with the cache write-back
+// and invalidation the workgroups need when they span XCDs.
 #ifndef GLIDE_PERSISTENT_ROWS
 #define GLIDE_PERSISTENT_ROWS 3
 #endif
@@ -69,7 +70,7 @@ bool vanka_persistent_fits(int ny, int nx);
 template <typename T>
 hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* alt_H, T* mask,
                                    const VankaArgs<T>& vk, T omega, int n_iter, unsigned long long* sync,
-                                   hipStream_t st);
+                                   int force_fenced, hipStream_t st);
 template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st);
 template <typename T>
@@ -99,7 +100,7 @@ bool adjoint_persistent_fits(int ny, int nx);
 template <typename T>
 hipError_t launch_adjoint_persistent(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, T* alt_u, T* alt_v,
                                      T* alt_H, const T* fu, const T* fv, const T* fH, T calving_rate, T omega,
-                                     int n_iter, unsigned long long* sync, hipStream_t st);
+                                     int n_iter, unsigned long long* sync, int force_fenced, hipStream_t st);
 // All n_iter cached adjoint passes of a small level in one workgroup (linear
 // drag, ny * nx <= kAdjointCoarseMaxCells, its LDS within the device limit:
 // adjoint_coarse_fits); the same results as n_iter launch_vjp +

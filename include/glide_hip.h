@@ -239,6 +239,12 @@ int glide_adjoint_pass_fused_enable(int on);
  * leaves NaN multipliers); 0 launches one fused pass at a time.  Both give
  * the same multipliers. */
 int glide_adjoint_persistent_enable(int on);
+/* Persistent launches (the two switches above; no counterpart in the
+ * reference): 1 runs every barrier with the cache write-back and invalidation
+ * that workgroups on different XCDs need, also when the launch finds them all
+ * on one XCD (where the default 0 skips it); for testing that path.  Both give
+ * the same results. */
+int glide_persistent_fenced_enable(int on);
 
 /* Measurement support (no counterpart in the reference): when enabled, HIP
  * event pairs are recorded around every launch of the hot kernels

@@ -502,6 +502,49 @@ def test_adjoint_persistent_matches_fused_launches(shape, n_pass):
         assert rel_err(x, y) < 1e-12
 
 
+@pytest.mark.parametrize("shape,n_pass,m", [((32, 32), 60, 1.0), ((96, 41), 5, 1.0 / 3.0)])
+def test_persistent_fenced_barriers_match_launches(shape, n_pass, m):
+    """The persistent launches' cross-XCD barrier path (cache write-back before
+    each arrival, invalidation after each wait), forced on: the forward passes
+    equal one fused launch per pass bitwise, the adjoint passes to round-off."""
+    from glide_amd import _lib
+    ny, nx = shape
+    print(f"persistent fenced shape={shape} passes={n_pass}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=47, mask_frac=0.1)
+    fwd, adj = [], []
+    _lib.adjoint_coarse_enable(False)
+    try:
+        for persistent in (True, False):
+            _lib.persistent_fenced_enable(persistent)
+            _lib.forward_persistent_enable(persistent)
+            _lib.adjoint_persistent_enable(persistent)
+            g = grid_from(F)
+            g.sliding.m.set(m)
+            g.calving.calving_rate.set(0.2)
+            g.forward_operators.vanka_sweep(10.0, n_pass, freeze_phi=True)
+            for t, x in zip((g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H), _lam(F, 48)):
+                t.set(x)
+            ao = g.adjoint_operators
+            ao.use_cache = True
+            rng = np.random.default_rng(49)
+            for t in (ao.f_u, ao.f_v, ao.f_H):
+                t.copy_(T(rng.standard_normal(tuple(t.shape))))
+            ao.vanka_sweep(10.0, n_pass)
+            torch.cuda.synchronize()
+            fwd.append([N(t.data) for t in (g.state.u, g.state.v, g.state.H, g.state.mask)])
+            adj.append([N(t.data) for t in (g.adjoint.lambda_u, g.adjoint.lambda_v, g.adjoint.lambda_H)])
+    finally:
+        _lib.persistent_fenced_enable(False)
+        _lib.forward_persistent_enable(True)
+        _lib.adjoint_persistent_enable(True)
+        _lib.adjoint_coarse_enable(True)
+    for x, y in zip(fwd[0], fwd[1]):
+        assert np.array_equal(x, y)
+    for x, y in zip(adj[0], adj[1]):
+        assert np.isfinite(x).all()
+        assert rel_err(x, y) < 1e-12
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_gradients(orc, shape):
     ny, nx = shape
