@@ -549,8 +549,8 @@ __global__ __launch_bounds__(3 * TX) void k_vanka_fused(SsaArgs<T> a, T* __restr
 // maintenance; otherwise (fenced) the arriving thread's agent-scope fences
 // write its L2 back before the arrival and invalidate it after the wait.  The
 // wait is bounded: after kGridSpinLimit polls a workgroup sets sync[1], after
-// which no barrier of the launch waits (the caller turns sync[1] into an
-// error it can see).
+// which no barrier of the launch waits and the kernel writes NaN into its last
+// pass's output, so that the failure shows in the results.
 constexpr int kGridSpinLimit = 1 << 20;
 
 // State loads of the persistent kernels: device-scope loads (default), or
