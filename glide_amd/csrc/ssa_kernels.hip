@@ -660,12 +660,15 @@ __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a
             v_dst = base_v;
             H_dst = base_H;
         }
-        for (int k = tid; k < (RW + 3) * W; k += (RW + 1) * TX) {
-            const int ei = i0 - 2 + k / W, ej = k % W - 2;
+        // the patches read columns -2 .. nx of the tile (all of it when nx = TX)
+        const int wn = min(nx + 3, W);
+        for (int k = tid; k < (RW + 3) * wn; k += (RW + 1) * TX) {
+            const int r = k / wn, q = k % wn;
+            const int ei = i0 - 2 + r, ej = q - 2;
             T e = T(0);
             if (ei >= 0 && ei < ny && ej >= 0 && ej < nx)
                 e = cell_viscosity<T, Ld>(s.u, s.v, s.B, ei, ej, ny, nx, s.dxi, s.P);
-            s_eta[k / W][k % W] = e;
+            s_eta[r][q] = e;
         }
         __syncthreads();
         T w[5] = {T(0), T(0), T(0), T(0), T(0)};
