@@ -116,6 +116,9 @@ _CP = ctypes.POINTER(EnthColumn)
 # name -> (restype, argtypes); every symbol the header declares.
 SIGNATURES = {
     "glide_version": (ctypes.c_char_p, []),
+    "glide_init": (_I, []),
+    "glide_check_status": (_I, [_P]),
+    "glide_persistent_spin_limit": (_I, [_I]),
     "glide_last_error": (_I, [ctypes.c_char_p, _S]),
     "glide_compute_grounded": (_I, [_LP, _D, _P]),
     "glide_ssa_residual": (_I, [_LP, _D, _I, _P, _P, _P, _P]),
@@ -152,6 +155,7 @@ SIGNATURES = {
     "glide_timing_read": (_I, [_I, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "glide_timing_read_units": (_I, [_I, ctypes.c_long, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(_D)]),
     # include/glide_enthalpy.h
+    "glide_enth_init": (_I, []),
     "glide_enth_residual": (_I, [_CP, _D, _I, ctypes.POINTER(_D), _P]),
     "glide_enth_column_smooth": (_I, [_CP, _D, _I, _D, _P]),
     "glide_enth_layer_smooth": (_I, [_CP, _D, _P]),
@@ -220,6 +224,33 @@ def persistent_fenced_enable(on: bool) -> None:
     """Persistent launches with the cross-XCD cache maintenance at every
     barrier even when their workgroups share one XCD (testing switch)."""
     call("glide_persistent_fenced_enable", int(bool(on)))
+
+
+_initialised_devices: set = set()
+
+
+def init_device(device: torch.device | None = None) -> None:
+    """Allocate the library's per-device scratch once per device (glide_init),
+    so that later calls allocate nothing, e.g. inside a caller's stream capture."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    if idx in _initialised_devices:
+        return
+    with torch.cuda.device(idx):
+        call("glide_init")
+    _initialised_devices.add(idx)
+
+
+def check_status(device: torch.device | None = None) -> None:
+    """Synchronise torch's current stream and raise if a persistent launch's
+    barrier timed out since the last check (glide_check_status)."""
+    call("glide_check_status", stream_handle(device))
+
+
+def persistent_spin_limit(polls: int) -> None:
+    """Polls of a persistent barrier wait before it gives up (testing switch;
+    negative gives up at once)."""
+    call("glide_persistent_spin_limit", int(polls))
 
 
 def graphs_clear() -> None:

@@ -7,7 +7,8 @@
 // raises a device flag once the stopping test passes; every later launch
 // reads the flag and returns at once.  The host never waits inside the
 // sweep.  Buffers belong to the caller; the library owns a small per-device
-// reduction scratch and status record (allocated on first use).
+// reduction scratch and status record (allocated by glide_enth_init, or by
+// the first call on a device that needs them).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -73,6 +74,14 @@ int scratch(EnthScratch** out) {
         HIP_TRY(hipHostMalloc(&s.host, sizeof(EnthStatus), hipHostMallocDefault), "hipHostMalloc(enthalpy status)");
     }
     *out = &s;
+    return GLIDE_OK;
+}
+
+// A reported sweep whose residual norm went non-finite (NaN propagates through
+// the max reductions and never passes the stopping test) is an error.
+int check_report_finite(const glide_enth_report& rep) {
+    if (!std::isfinite(rep.r0) || !std::isfinite(rep.r))
+        return fail(GLIDE_ERR_NONFINITE, "non-finite enthalpy residual norm (the report holds the norms)");
     return GLIDE_OK;
 }
 
@@ -188,6 +197,7 @@ int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_en
         report->r = s->host->r;
         std::memset(report->norms, 0, sizeof(report->norms));
         for (int q = 0; q < s->host->sweeps && q < GLIDE_ENTH_MAX_SWEEPS; ++q) report->norms[q] = s->host->norms[q];
+        GLIDE_TRY(check_report_finite(*report));
     }
     return GLIDE_OK;
 }
@@ -195,6 +205,11 @@ int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_en
 }  // namespace
 
 extern "C" {
+
+int glide_enth_init(void) {
+    EnthScratch* s = nullptr;
+    return scratch(&s);
+}
 
 int glide_enth_residual(const glide_enth_column* C, double dt, int use_forcing, double* max_abs, void* stream) {
     EnthArgs<R> A;
@@ -400,6 +415,7 @@ int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoin
         report->r = s->host->r;
         std::memset(report->norms, 0, sizeof(report->norms));
         for (int q = 0; q < s->host->sweeps && q < GLIDE_ENTH_MAX_SWEEPS; ++q) report->norms[q] = s->host->norms[q];
+        GLIDE_TRY(check_report_finite(*report));
     }
     return GLIDE_OK;
 }

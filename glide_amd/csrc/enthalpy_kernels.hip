@@ -72,6 +72,10 @@ __device__ __forceinline__ T lane_get(T x, int src) {
 
 __device__ __forceinline__ bool skipped(const int* skip) { return skip && *skip; }
 
+// Maximum that keeps a NaN operand (fmax returns the other operand), so that
+// a non-finite residual reaches the reported norm and fails the stopping test.
+__device__ __forceinline__ double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
 // Horizontal advection pieces of one level that do not depend on the
 // level's own enthalpy: face mass fluxes (zero on the domain edge) and the
 // neighbouring columns' enthalpy (glide/cuda/enthalpy.cu:671-763).
@@ -290,12 +294,12 @@ __global__ __launch_bounds__(256) void k_enth_residual(EnthArgs<T> A, T* __restr
             r = use_forcing ? w.r - A.f_E[base + k] : w.r;
         }
         if (in && r_out) r_out[base + k] = r;
-        m = fmax(m, fabs((double)r));
+        m = nan_max(m, fabs((double)r));
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_down(m, off, 64));
     if (k == 0) smax[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]));
+    if (threadIdx.x == 0) partials[blockIdx.x] = nan_max(nan_max(smax[0], smax[1]), nan_max(smax[2], smax[3]));
 }
 
 // Sweep bookkeeping: mode 0 starts a sweep (r0, zero count, clear the skip
@@ -313,8 +317,11 @@ __device__ __forceinline__ void record_sweep(EnthStatus* status, int mode, doubl
         status->sweeps = it + 1;
         status->r = r;
         const double r0 = status->r0;
-        const double rel = r0 > 0.0 ? r / r0 : 0.0;
-        if (rel < rtol || r < atol) status->done = 1;
+        // a NaN norm (now or at the start) never stops the sweep
+        if (r == r && r0 == r0) {
+            const double rel = r0 > 0.0 ? r / r0 : 0.0;
+            if (rel < rtol || r < atol) status->done = 1;
+        }
     }
 }
 
@@ -327,12 +334,12 @@ __global__ __launch_bounds__(256) void k_enth_reduce(const double* __restrict__ 
     __shared__ double sh[4];
     if (status && mode == 1 && status->done) return;
     double m = 0.0;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, part[q]);
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    for (int q = threadIdx.x; q < n; q += blockDim.x) m = nan_max(m, part[q]);
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const double r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    const double r = nan_max(nan_max(sh[0], sh[1]), nan_max(sh[2], sh[3]));
     if (out) *out = r;
     if (status) record_sweep(status, mode, r, rtol, atol);
 }
@@ -899,7 +906,7 @@ __global__ __launch_bounds__(256) void k_enth_adj_pass(const int* __restrict__ s
                         }
                         const T rhs = gv[q] - hs;
                         const T b = wv[q] + a[q] * cpm, cT = c[q] * wv[q];
-                        m = fmax(m, fabs((double)(rhs - (a[q] * lm + b * l0 + cT * lpk))));
+                        m = nan_max(m, fabs((double)(rhs - (a[q] * lm + b * l0 + cT * lpk))));
                         if (SOLVE) {
                             dprev = (rhs - a[q] * dprev) / wv[q];
                             dp[k] = dprev;
@@ -936,12 +943,14 @@ __global__ __launch_bounds__(256) void k_enth_adj_pass(const int* __restrict__ s
             }
         }
     }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const unsigned long long prev =
-        atomicMax(acc, (unsigned long long)__double_as_longlong(fmax(fmax(smax[0], smax[1]), fmax(smax[2], smax[3]))));
+    // a NaN maximum (positive quiet NaN from fabs) also wins the unsigned
+    // comparison of bit patterns, so it survives the accumulation
+    const unsigned long long prev = atomicMax(
+        acc, (unsigned long long)__double_as_longlong(nan_max(nan_max(smax[0], smax[1]), nan_max(smax[2], smax[3]))));
     if (close_mode < 0) return;
     // waiting for the returned value orders this block's maximum before its
     // count; all of them are device-scope atomics on the same two words

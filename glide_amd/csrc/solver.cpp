@@ -7,7 +7,8 @@
 // tolerances are non-positive: a fixed number of V-cycles, norms read once at
 // the end, and the whole solve replayed from a cached HIP graph).  All field
 // buffers belong to the caller; the library owns one small per-device
-// reduction scratch (allocated on first use) and the graph cache.
+// reduction scratch with a failure latch (allocated by glide_init, or by the
+// first call on a device that needs it) and the graph cache.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "../../include/glide_hip.h"
+#include "../../include/glide_enthalpy.h"
 #include "ssa_kernels.hpp"
 
 using namespace glide;
@@ -52,6 +54,8 @@ struct DeviceScratch {
     double* partials = nullptr;  // 3 * kSumsqBlocks
     double* norms = nullptr;     // 3 * 66 slots
     double* host = nullptr;      // pinned mirror of `norms`
+    unsigned* latch = nullptr;   // failure latch of the persistent launches (bit 0: a barrier timed out)
+    unsigned* latch_host = nullptr;  // pinned: [0] mirror of `latch`, [1] zero (the value that clears it)
 };
 
 std::mutex g_scratch_mutex;
@@ -67,8 +71,29 @@ int scratch_for_current_device(DeviceScratch** out) {
         HIP_TRY(hipMalloc(&s.partials, sizeof(double) * 3 * kSumsqBlocks), "hipMalloc(reduction scratch)");
         HIP_TRY(hipMalloc(&s.norms, sizeof(double) * 3 * 66), "hipMalloc(norm slots)");
         HIP_TRY(hipHostMalloc(&s.host, sizeof(double) * 3 * 66, hipHostMallocDefault), "hipHostMalloc(norms)");
+        HIP_TRY(hipMalloc(&s.latch, sizeof(unsigned)), "hipMalloc(latch)");
+        HIP_TRY(hipMemset(s.latch, 0, sizeof(unsigned)), "hipMemset(latch)");
+        HIP_TRY(hipHostMalloc(&s.latch_host, 2 * sizeof(unsigned), hipHostMallocDefault), "hipHostMalloc(latch)");
+        s.latch_host[0] = s.latch_host[1] = 0u;
     }
     *out = &s;
+    return GLIDE_OK;
+}
+
+// The failure latch: latch_enqueue copies it to the host and clears it on the
+// stream; after the caller's synchronisation latch_check reports
+// GLIDE_ERR_TIMEOUT if a persistent barrier timed out since the last check.
+int latch_enqueue(DeviceScratch* s, hipStream_t st) {
+    HIP_TRY(hipMemcpyAsync(s->latch_host, s->latch, sizeof(unsigned), hipMemcpyDeviceToHost, st), "latch D2H");
+    HIP_TRY(hipMemcpyAsync(s->latch, s->latch_host + 1, sizeof(unsigned), hipMemcpyHostToDevice, st), "latch clear");
+    return GLIDE_OK;
+}
+int latch_check(DeviceScratch* s) {
+    if (s->latch_host[0] & 1u) {
+        s->latch_host[0] = 0u;
+        return fail(GLIDE_ERR_TIMEOUT,
+                    "a persistent launch's barrier wait timed out: the results of that solve are invalid (NaN)");
+    }
     return GLIDE_OK;
 }
 
@@ -168,6 +193,8 @@ bool g_persistent_forward = true;
 bool g_persistent_adjoint = true;
 // Persistent launches with fenced barriers even on one XCD (glide_persistent_fenced_enable).
 bool g_persistent_fenced = false;
+// Polls of a persistent barrier wait before it gives up (glide_persistent_spin_limit).
+int g_spin_limit = kGridSpinLimit;
 
 // Three 64-bit words at the end of a level's smoother workspace (5 ny nx
 // values, of which the alternating state of the persistent and fused passes
@@ -175,6 +202,16 @@ bool g_persistent_fenced = false;
 // persistent launch.
 unsigned long long* work_sync(const glide_level& L) {
     return reinterpret_cast<unsigned long long*>(P(L.work) + 5 * nC(L) - 3);
+}
+
+int persistent_ctl(const glide_level& L, PersistentCtl* ctl) {
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    ctl->sync = work_sync(L);
+    ctl->latch = s->latch;
+    ctl->force_fenced = g_persistent_fenced ? 1 : 0;
+    ctl->spin_limit = g_spin_limit;
+    return GLIDE_OK;
 }
 
 int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
@@ -195,8 +232,10 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
         R* alt_v = alt_u + nU(L);
         R* alt_H = alt_v + nV(L);
         if (g_persistent_forward && n_iter > 1 && vanka_persistent_fits(L.ny, L.nx)) {
+            PersistentCtl ctl;
+            GLIDE_TRY(persistent_ctl(L, &ctl));
             HIP_TRY(launch_vanka_persistent<R>(make_args(L, dt, false), alt_u, alt_v, alt_H, P(L.mask), vk, vc.omega,
-                                               n_iter, work_sync(L), g_persistent_fenced ? 1 : 0, st),
+                                               n_iter, ctl, st),
                     "vanka passes (persistent)");
         } else {
             for (int it = 0; it < n_iter; ++it) {
@@ -292,10 +331,11 @@ int vanka_adjoint_sweeps(const glide_level& L, double dt, int n_iter, const glid
             R* alt_v = alt_u + nU(L);
             R* alt_H = alt_v + nV(L);
             if (g_persistent_adjoint && n_iter > 1 && adjoint_persistent_fits(L.ny, L.nx)) {
+                PersistentCtl ctl;
+                GLIDE_TRY(persistent_ctl(L, &ctl));
                 HIP_TRY(launch_adjoint_persistent<R>(a, cache, P(L.lambda_u), P(L.lambda_v), P(L.lambda_H), alt_u,
                                                      alt_v, alt_H, CP(L.af_u), CP(L.af_v), CP(L.af_H),
-                                                     vk.calving_rate, vc.omega, n_iter, work_sync(L),
-                                                     g_persistent_fenced ? 1 : 0, st),
+                                                     vk.calving_rate, vc.omega, n_iter, ctl, st),
                         "adjoint passes (persistent)");
                 n_iter_done = n_iter;
             }
@@ -563,7 +603,8 @@ int check_hierarchy(const glide_level* levels, int n_levels, int start, const gl
                     const glide_vanka_config* vc) {
     if (!levels || !cfg || !vc) return fail(GLIDE_ERR_INVALID_ARG, "levels, cfg and vc must be non-NULL");
     if (n_levels < 1 || start < 0 || start >= n_levels) return fail(GLIDE_ERR_INVALID_ARG, "bad level range");
-    if (cfg->maximum_vcycles > 64) return fail(GLIDE_ERR_INVALID_ARG, "maximum_vcycles must be <= 64");
+    if (cfg->maximum_vcycles < 0 || cfg->maximum_vcycles > 64)
+        return fail(GLIDE_ERR_INVALID_ARG, "maximum_vcycles must be in [0, 64]");
     for (int l = start; l < n_levels; ++l) {
         GLIDE_TRY(check_level(&levels[l]));
         GLIDE_TRY(check_ssa_inputs(levels[l]));
@@ -688,9 +729,16 @@ int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, 
     const bool fixed = !(X.cfg.relative_tolerance > 0.0) && !(X.cfg.absolute_tolerance > 0.0);
     double r0 = 0.0, absn = 0.0, rel = 1.0;
     double n3[3] = {0, 0, 0};
+    // every synchronisation also reads the failure latch of the persistent
+    // launches; a timed-out barrier or a non-finite norm fails the solve
     auto read_slots = [&](int nslots) -> int {
         HIP_TRY(hipMemcpyAsync(s->host, s->norms, sizeof(double) * 3 * nslots, hipMemcpyDeviceToHost, st), "norms D2H");
+        GLIDE_TRY(latch_enqueue(s, st));
         HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        GLIDE_TRY(latch_check(s));
+        for (int q = 0; q < 3 * nslots; ++q)
+            if (!std::isfinite(s->host[q]))
+                return fail(GLIDE_ERR_NONFINITE, "non-finite residual norm in the multigrid solve");
         return GLIDE_OK;
     };
     int it = 0;
@@ -757,7 +805,27 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 // ====================================================================== C ABI
 extern "C" {
 
-const char* glide_version(void) { return "glide-mi355x 0.1.0 (gfx950, FP64)"; }
+const char* glide_version(void) { return "glide-mi355x 0.2.0 (gfx950, FP64 and mixed FP32/FP64)"; }
+
+int glide_init(void) {
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    return glide_enth_init();
+}
+
+int glide_check_status(void* stream) {
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    hipStream_t st = (hipStream_t)stream;
+    GLIDE_TRY(latch_enqueue(s, st));
+    HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return latch_check(s);
+}
+
+int glide_persistent_spin_limit(int polls) {
+    g_spin_limit = polls;
+    return GLIDE_OK;
+}
 
 int glide_last_error(char* buf, size_t len) {
     const std::string& e = g_last_error;

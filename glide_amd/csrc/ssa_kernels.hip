@@ -548,10 +548,11 @@ __global__ __launch_bounds__(3 * TX) void k_vanka_fused(SsaArgs<T> a, T* __restr
 // with device-scope loads (LdCoherent), so the barrier needs no cache
 // maintenance; otherwise (fenced) the arriving thread's agent-scope fences
 // write its L2 back before the arrival and invalidate it after the wait.  The
-// wait is bounded: after kGridSpinLimit polls a workgroup sets sync[1], after
-// which no barrier of the launch waits and the kernel writes NaN into its last
-// pass's output, so that the failure shows in the results.
-constexpr int kGridSpinLimit = 1 << 20;
+// wait is bounded: after ctl.spin_limit polls (a negative limit gives up at
+// once, which tests the failure path) a workgroup sets sync[1], after which no
+// barrier of the launch waits; the kernel then writes NaN into its last pass's
+// output, so that the failure shows in the results, and latches bit 0 of
+// *ctl.latch, which the host's next synchronising call reports.
 
 // State loads of the persistent kernels: device-scope loads (default), or
 // plain loads after an agent-scope acquire at every barrier (a tuning switch:
@@ -562,16 +563,18 @@ constexpr int kGridSpinLimit = 1 << 20;
 #endif
 using LdPersistent = std::conditional_t<GLIDE_PERSISTENT_PLAIN_LOADS != 0, LdPlain, LdCoherent>;
 
-__device__ __forceinline__ void grid_barrier(unsigned long long* sync, unsigned long long target, bool fenced) {
+__device__ __forceinline__ void grid_barrier(const PersistentCtl& ctl, unsigned long long target, bool fenced) {
+    unsigned long long* sync = ctl.sync;
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         if (fenced) __threadfence();
         atomicAdd(sync, 1ull);
+        if (ctl.spin_limit < 0) atomicExch(sync + 1, 1ull);
         int spins = 0;
         while (__hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target &&
                !__hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            if (++spins > kGridSpinLimit) {
+            if (++spins > ctl.spin_limit) {
                 atomicExch(sync + 1, 1ull);
                 break;
             }
@@ -598,14 +601,26 @@ constexpr int kXcds = 8;
 // Start of a persistent launch: the working workgroups record their XCDs in
 // sync[2] and pass the first barrier (fenced); returns whether they all run
 // on one XCD (the same answer in every workgroup).
-__device__ __forceinline__ bool persistent_start(unsigned long long* sync, int nb) {
+__device__ __forceinline__ bool persistent_start(const PersistentCtl& ctl, int nb) {
     __shared__ int s_one;
-    if (threadIdx.x == 0 && threadIdx.y == 0) atomicOr(sync + 2, 1ull << xcc_id());
-    grid_barrier(sync, (unsigned long long)nb, true);
+    if (threadIdx.x == 0 && threadIdx.y == 0) atomicOr(ctl.sync + 2, 1ull << xcc_id());
+    grid_barrier(ctl, (unsigned long long)nb, true);
     if (threadIdx.x == 0 && threadIdx.y == 0)
-        s_one = __popcll(__hip_atomic_load(sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 1;
+        s_one = __popcll(__hip_atomic_load(ctl.sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 1;
     __syncthreads();
     return s_one != 0;
+}
+
+// End of a persistent launch: whether a barrier timed out (the same answer in
+// every workgroup); the first workgroup latches the failure for the host.
+__device__ __forceinline__ bool persistent_failed(const PersistentCtl& ctl) {
+    __shared__ int s_fail;
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        s_fail = __hip_atomic_load(ctl.sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+        if (s_fail && blockIdx.x == 0 && ctl.latch) atomicOr(ctl.latch, 1u);
+    }
+    __syncthreads();
+    return s_fail != 0;
 }
 
 __global__ __launch_bounds__(64) void k_zero_sync(unsigned long long* sync) {
@@ -623,14 +638,12 @@ __global__ __launch_bounds__(64) void k_zero_sync(unsigned long long* sync) {
 template <typename T, bool LINEAR, typename TJ, int RW>
 __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a, T* alt_u, T* alt_v, T* alt_H,
                                                                     T* __restrict__ mask, VankaArgs<T> vk, T omega,
-                                                                    int n_iter, unsigned long long* sync,
-                                                                    int force_fenced) {
+                                                                    int n_iter, PersistentCtl ctl) {
     using Ld = LdPersistent;
     constexpr int W = TX + 3;
     __shared__ T s_eta[RW + 3][W];  // cells (i0 - 2 .. i0 + RW, -2 .. TX)
     __shared__ T s_w1[RW][TX + 1];  // right-facet increments of the owned rows' cells (cell j at j + 1)
     __shared__ T s_w3[RW + 1][TX];  // bottom-facet increments of rows i0 + q (q < RW) and i0 - 1 (q = RW)
-    __shared__ int s_fail;
     if (blockIdx.x % kXcds) return;
     const int nb = gridDim.x / kXcds;
     const int ny = a.ny, nx = a.nx;
@@ -642,7 +655,7 @@ __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a
     T* const base_u = const_cast<T*>(a.u);
     T* const base_v = const_cast<T*>(a.v);
     T* const base_H = const_cast<T*>(a.H);
-    const bool fenced = !persistent_start(sync, nb) || force_fenced;
+    const bool fenced = !persistent_start(ctl, nb) || ctl.force_fenced;
 #pragma unroll 1
     for (int p = 0; p < n_iter; ++p) {
         // the lane's indices pass through an empty asm statement every pass, so
@@ -704,11 +717,9 @@ __global__ __launch_bounds__(TX * (RW + 1)) void k_vanka_persistent(SsaArgs<T> a
             H_dst[c] = Ld::get(s.H + c) + omega * w[4];
             mask[c] = masked;
         }
-        if (p + 1 < n_iter) grid_barrier(sync, (unsigned long long)nb * (p + 2), fenced);
+        if (p + 1 < n_iter) grid_barrier(ctl, (unsigned long long)nb * (p + 2), fenced);
     }
-    if (wv == 0 && lane == 0) s_fail = __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-    __syncthreads();
-    if (s_fail && own)
+    if (persistent_failed(ctl) && own)
         (((n_iter - 1) & 1) ? base_H : alt_H)[(long)(i0 + wv) * nx + lane] = __longlong_as_double(0x7ff8000000000000ll);
 }
 
@@ -1429,13 +1440,11 @@ __global__ __launch_bounds__(TX * 2 * TY) void k_adjoint_persistent(SsaArgs<T> a
                                                                            const T* __restrict__ fv,
                                                                            const T* __restrict__ fH, T calving_rate,
                                                                            T omega, int n_iter,
-                                                                           unsigned long long* sync,
-                                                                           int force_fenced) {
-    __shared__ int s_fail;
+                                                                           PersistentCtl ctl) {
     if (blockIdx.x % kXcds) return;
     const int nb = gridDim.x / kXcds;
     const int i0 = (blockIdx.x / kXcds) * TY;
-    const bool fenced = !persistent_start(sync, nb) || force_fenced;
+    const bool fenced = !persistent_start(ctl, nb) || ctl.force_fenced;
 #pragma unroll 1
     for (int p = 0; p < n_iter; ++p) {
         // see k_vanka_persistent: keeps the pass-invariant addresses from being hoisted
@@ -1445,13 +1454,11 @@ __global__ __launch_bounds__(TX * 2 * TY) void k_adjoint_persistent(SsaArgs<T> a
         adjoint_pass_tile<T, LdPersistent, TY>(a, cache, odd ? alt_u : lu, odd ? alt_v : lv, odd ? alt_H : lH,
                                          odd ? lu : alt_u, odd ? lv : alt_v, odd ? lH : alt_H, fu, fv, fH,
                                          calving_rate, omega, i0, 0, ty, tx);
-        if (p + 1 < n_iter) grid_barrier(sync, (unsigned long long)nb * (p + 2), fenced);
+        if (p + 1 < n_iter) grid_barrier(ctl, (unsigned long long)nb * (p + 2), fenced);
     }
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-        s_fail = __hip_atomic_load(sync + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-    __syncthreads();
+    const bool failed = persistent_failed(ctl);
     const int i = i0 + threadIdx.y, j = threadIdx.x;
-    if (s_fail && threadIdx.y < TY && i < a.ny && j < a.nx)
+    if (failed && threadIdx.y < TY && i < a.ny && j < a.nx)
         (((n_iter - 1) & 1) ? lH : alt_H)[(long)i * a.nx + j] = __longlong_as_double(0x7ff8000000000000ll);
 }
 
@@ -1687,30 +1694,26 @@ bool vanka_persistent_fits(int ny, int nx) {
 
 template <typename T>
 hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* alt_H, T* mask,
-                                   const VankaArgs<T>& vk, T omega, int n_iter, unsigned long long* sync,
-                                   int force_fenced, hipStream_t st) {
+                                   const VankaArgs<T>& vk, T omega, int n_iter, const PersistentCtl& ctl,
+                                   hipStream_t st) {
     if (n_iter <= 0) return hipSuccess;
     if (!vanka_persistent_fits(a.ny, a.nx)) return hipErrorInvalidValue;
     // the arrival counter, the timeout flag and the XCD mask start at zero (a
     // kernel rather than a memset, so that the launch can sit in a captured graph)
-    k_zero_sync<<<1, 64, 0, st>>>(sync);
+    k_zero_sync<<<1, 64, 0, st>>>(ctl.sync);
     TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx * n_iter, st);
     const dim3 g(kXcds * ((a.ny + kPersistentRows - 1) / kPersistentRows)), b(TX, kPersistentRows + 1);
     constexpr int R = kPersistentRows;
-    const int ff = force_fenced;
     if (vk.local_fp32) {
         if (a.P.linear_drag)
-            k_vanka_persistent<T, true, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync,
-                                                                   ff);
+            k_vanka_persistent<T, true, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, ctl);
         else
-            k_vanka_persistent<T, false, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter,
-                                                                    sync, ff);
+            k_vanka_persistent<T, false, float, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, ctl);
     } else {
         if (a.P.linear_drag)
-            k_vanka_persistent<T, true, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync, ff);
+            k_vanka_persistent<T, true, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, ctl);
         else
-            k_vanka_persistent<T, false, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, sync,
-                                                                ff);
+            k_vanka_persistent<T, false, T, R><<<g, b, 0, st>>>(a, alt_u, alt_v, alt_H, mask, vk, omega, n_iter, ctl);
     }
     return hipGetLastError();
 }
@@ -1763,13 +1766,13 @@ bool adjoint_persistent_fits(int ny, int nx) {
 template <typename T>
 hipError_t launch_adjoint_persistent(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, T* alt_u, T* alt_v,
                                      T* alt_H, const T* fu, const T* fv, const T* fH, T calving_rate, T omega,
-                                     int n_iter, unsigned long long* sync, int force_fenced, hipStream_t st) {
+                                     int n_iter, const PersistentCtl& ctl, hipStream_t st) {
     if (n_iter <= 0) return hipSuccess;
     if (!a.P.linear_drag || !adjoint_persistent_fits(a.ny, a.nx)) return hipErrorInvalidValue;
-    k_zero_sync<<<1, 64, 0, st>>>(sync);
+    k_zero_sync<<<1, 64, 0, st>>>(ctl.sync);
     TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx * n_iter, st);
     k_adjoint_persistent<T><<<kXcds * ((a.ny + TY - 1) / TY), dim3(TX, 2 * TY), 0, st>>>(
-        a, cache, lu, lv, lH, alt_u, alt_v, alt_H, fu, fv, fH, calving_rate, -omega, n_iter, sync, force_fenced);
+        a, cache, lu, lv, lH, alt_u, alt_v, alt_H, fu, fv, fH, calving_rate, -omega, n_iter, ctl);
     return hipGetLastError();
 }
 
@@ -1839,14 +1842,14 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_vanka_dump<T>(const SsaArgs<T>&, T*, T*, T, hipStream_t);                              \
     template hipError_t launch_vanka_fused<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, hipStream_t);\
     template hipError_t launch_vanka_persistent<T>(const SsaArgs<T>&, T*, T*, T*, T*, const VankaArgs<T>&, T, int,    \
-                                                  unsigned long long*, int, hipStream_t);                         \
+                                                  const PersistentCtl&, hipStream_t);                             \
     template hipError_t launch_vanka_adjoint_pass<T>(const SsaArgs<T>&, const T*, const T*, const T*, T*, T*, T*, T*, \
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
     template hipError_t launch_adjoint_pass_fused<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, T*,   \
                                                      T*, T*, const T*, const T*, const T*, T, T, hipStream_t);       \
     template bool adjoint_coarse_fits<T>(const SsaArgs<T>&);                                                         \
     template hipError_t launch_adjoint_persistent<T>(const SsaArgs<T>&, const T*, T*, T*, T*, T*, T*, T*, const T*,  \
-                                                     const T*, const T*, T, T, int, unsigned long long*, int,         \
+                                                     const T*, const T*, T, T, int, const PersistentCtl&,            \
                                                      hipStream_t);                                                    \
     template hipError_t launch_adjoint_coarse<T>(const SsaArgs<T>&, const T*, T*, T*, T*, const T*, const T*,          \
                                                  const T*, T, T, int, hipStream_t);                                  \

@@ -67,10 +67,23 @@ constexpr int kPersistentRows = GLIDE_PERSISTENT_ROWS;
 #endif
 constexpr int kPersistentMaxBlocks = GLIDE_PERSISTENT_MAX_BLOCKS;
 bool vanka_persistent_fits(int ny, int nx);
+// Control words of a persistent launch: sync holds three 64-bit words the
+// launch zeroes first (arrival counter, timeout flag, XCD mask); a timed-out
+// barrier wait latches bit 0 of *latch (if non-null), which the host reads
+// back at its next synchronisation; force_fenced != 0 runs every barrier
+// with the cache maintenance that workgroups on several XCDs need;
+// spin_limit bounds each barrier wait (polls; negative gives up at once).
+struct PersistentCtl {
+    unsigned long long* sync;
+    unsigned int* latch;
+    int force_fenced;
+    int spin_limit;
+};
+constexpr int kGridSpinLimit = 1 << 20;
 template <typename T>
 hipError_t launch_vanka_persistent(const SsaArgs<T>& a, T* alt_u, T* alt_v, T* alt_H, T* mask,
-                                   const VankaArgs<T>& vk, T omega, int n_iter, unsigned long long* sync,
-                                   int force_fenced, hipStream_t st);
+                                   const VankaArgs<T>& vk, T omega, int n_iter, const PersistentCtl& ctl,
+                                   hipStream_t st);
 template <typename T>
 hipError_t launch_vanka_dump(const SsaArgs<T>& a, T* J, T* r, T calving_rate, hipStream_t st);
 template <typename T>
@@ -100,7 +113,7 @@ bool adjoint_persistent_fits(int ny, int nx);
 template <typename T>
 hipError_t launch_adjoint_persistent(const SsaArgs<T>& a, const T* cache, T* lu, T* lv, T* lH, T* alt_u, T* alt_v,
                                      T* alt_H, const T* fu, const T* fv, const T* fH, T calving_rate, T omega,
-                                     int n_iter, unsigned long long* sync, int force_fenced, hipStream_t st);
+                                     int n_iter, const PersistentCtl& ctl, hipStream_t st);
 // All n_iter cached adjoint passes of a small level in one workgroup (linear
 // drag, ny * nx <= kAdjointCoarseMaxCells, its LDS within the device limit:
 // adjoint_coarse_fits); the same results as n_iter launch_vjp +

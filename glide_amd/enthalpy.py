@@ -211,10 +211,11 @@ class EnthalpyOperators:
         if cfg.hook_func is None:
             C = self._column()
             rep = _lib.EnthReport() if cfg.report_norms else None
+            if rep is not None:
+                self.last_report = rep  # kept when the call raises (a non-finite norm)
             call("glide_enth_column_sweep", ctypes.byref(C), float(dt), int(n_iter), ctypes.byref(cfg.to_c(alternating)),
                  ctypes.byref(rep) if rep is not None else None, self._stream())
             if rep is not None:
-                self.last_report = rep
                 print(f"  Enthalpy initial: |r0| = {rep.r0:.2e}")
                 for it in range(rep.sweeps):
                     r = rep.norms[it]

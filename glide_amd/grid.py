@@ -84,6 +84,9 @@ class Grid:
         self.crs = crs
         self.device = torch.device(device) if device is not None else (
             parent.device if parent is not None else default_device())
+        if self.device.type == "cuda" and parent is None:
+            from . import _lib
+            _lib.init_device(self.device)
         self.x_cell = x0 + dx * torch.arange(nx, dtype=DTYPE)
         self.y_cell = y0 - dx * torch.arange(ny, dtype=DTYPE)
         self.x_vfacet = x0 - dx / 2 + dx * torch.arange(nx + 1, dtype=DTYPE)

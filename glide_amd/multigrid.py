@@ -291,6 +291,11 @@ class MGForcingManager:
 
 
 # ====================================================================== configs
+def _check_vcycles(n) -> None:
+    if not 0 <= int(n) <= 64:
+        raise ValueError(f"maximum_vcycles must be in [0, 64] (got {n})")
+
+
 @dataclass
 class FASCDConfig:
     """glide/multigrid.py:671-682; compat_vprolong reproduces the reference's
@@ -313,6 +318,7 @@ class FASCDConfig:
     asynchronous: bool = False
 
     def to_c(self) -> _lib.FasConfigC:
+        _check_vcycles(self.maximum_vcycles)
         return _lib.FasConfigC(
             freeze_coarse_calving=int(bool(self.freeze_coarse_calving)),
             freeze_coarse_phi=int(bool(self.freeze_coarse_phi)), coarsest_steps=int(self.coarsest_steps),
@@ -340,6 +346,7 @@ class FASAdjointConfig:
     asynchronous: bool = False
 
     def to_c(self) -> _lib.FasConfigC:
+        _check_vcycles(self.maximum_vcycles)
         return _lib.FasConfigC(
             freeze_coarse_calving=0, freeze_coarse_phi=1, coarsest_steps=int(self.coarsest_steps),
             pre_steps=int(self.pre_steps), post_steps=int(self.post_steps), finest_steps=int(self.finest_steps),
@@ -528,9 +535,9 @@ class FASCDSolver:
         cfg = self._fas_config.to_c()
         rep = None if _no_report(self._fas_config, report_norms) else _lib.SolveReport()
         st = stream_handle(self.multigrid.device)
+        self.last_report = rep
         call("glide_fascd_solve", arr, self.n_levels, int(start_level), self.dt, ctypes.byref(cfg), vcs,
              ctypes.byref(rep) if rep is not None else None, st)
-        self.last_report = rep
         if rep is not None and self._fas_config.report_norms and report_norms:
             _print_report(rep, list(rep.norms3))
         return rep
@@ -578,9 +585,9 @@ class FASAdjointSolver:
         arr, vcs = self._structs()
         cfg = self._fas_config.to_c()
         rep = None if _no_report(self._fas_config, report_norms) else _lib.SolveReport()
+        self.last_report = rep
         call("glide_fas_adjoint_solve", arr, self.n_levels, int(start_level), self.dt, ctypes.byref(cfg), vcs,
              ctypes.byref(rep) if rep is not None else None, stream_handle(self.multigrid.device))
-        self.last_report = rep
         if rep is not None and self._fas_config.report_norms and report_norms:
             _print_report(rep, list(rep.norms3))
         return rep

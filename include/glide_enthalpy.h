@@ -99,6 +99,10 @@ typedef struct glide_thermal_config {
     int32_t strain_heating; /* phi_strain from the SSA velocities and B before the sweep */
 } glide_thermal_config;
 
+/* Allocates this library's enthalpy scratch (reduction partials and the sweep
+ * status record) on the current device, so that later calls allocate nothing
+ * (no counterpart in the reference; glide_init calls it). */
+int glide_enth_init(void);
 /* r_E = F(E) - f_E (use_forcing != 0) or F(E); when max_abs is non-NULL the
  * stream is synchronised and max |r_E| is stored there. */
 int glide_enth_residual(const glide_enth_column* C, double dt, int use_forcing, double* max_abs, void* stream);
@@ -111,7 +115,9 @@ int glide_enth_apply(const glide_enth_column* C, double omega, void* stream);
 /* Up to n_iter sweeps (column smoother, then layer smoother when alternating),
  * stopping after the first sweep whose max |r| is below the relative or the
  * absolute tolerance.  The stopping test runs on the device: no host
- * synchronisation unless `report` is non-NULL (then one, at the end).
+ * synchronisation unless `report` is non-NULL (then one, at the end).  A NaN
+ * residual is carried through the max-norm, never passes the stopping test
+ * and, with a report, returns GLIDE_ERR_NONFINITE after filling the report.
  * Uses r_E and E_work (alternating) or delta_E (not alternating). */
 int glide_enth_column_sweep(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
                             glide_enth_report* report, void* stream);

@@ -49,7 +49,8 @@ typedef enum glide_status {
     GLIDE_ERR_INVALID_ARG = -1,
     GLIDE_ERR_SHAPE = -2,
     GLIDE_ERR_HIP = -3,
-    GLIDE_ERR_NONFINITE = -4
+    GLIDE_ERR_NONFINITE = -4, /* a residual norm the call computed is NaN or infinite */
+    GLIDE_ERR_TIMEOUT = -5    /* a persistent launch's bounded barrier wait gave up */
 } glide_status;
 
 /* Flags for the residual / VJP entry points. */
@@ -130,6 +131,16 @@ typedef struct glide_solve_report {
 } glide_solve_report;
 
 const char* glide_version(void);
+/* Allocates the library's per-device scratch (reduction partials, norm slots,
+ * the failure latch, the enthalpy status record) on the current device, so
+ * that no later call allocates device memory; calls made inside a caller's
+ * stream capture need it first (no counterpart in the reference). */
+int glide_init(void);
+/* Synchronises `stream` and reports (then clears) the failure latch: GLIDE_OK,
+ * or GLIDE_ERR_TIMEOUT if a persistent launch's barrier wait gave up since
+ * the last check (its solve's results are NaN).  Solves that synchronise for
+ * a report check the latch themselves (no counterpart in the reference). */
+int glide_check_status(void* stream);
 /* Copies the calling thread's last error message into buf (NUL-terminated);
  * returns its full length. */
 int glide_last_error(char* buf, size_t len);
@@ -159,7 +170,9 @@ int glide_prolong_hfacet_bilinear(const void* coarse, void* fine, int ny_fine, i
 int glide_prolong_cell(const void* coarse, void* fine, int ny_fine, int nx_fine, int bilinear, void* stream);
 
 /* Full FAS-CD solve on levels[start_level..n_levels-1]; vc holds one config
- * per level.  Synchronises the stream once per V-cycle for the stopping test
+ * per level; 0 <= maximum_vcycles <= 64.  A non-finite residual norm
+ * returns GLIDE_ERR_NONFINITE, a timed-out persistent barrier
+ * GLIDE_ERR_TIMEOUT (both only where the solve synchronises).  Synchronises the stream once per V-cycle for the stopping test
  * unless both tolerances are <= 0 (then it runs exactly maximum_vcycles
  * V-cycles and synchronises once at the end to fill the report; with report
  * NULL it neither synchronises nor computes the per-cycle residual norms, and
@@ -245,6 +258,12 @@ int glide_adjoint_persistent_enable(int on);
  * on one XCD (where the default 0 skips it); for testing that path.  Both give
  * the same results. */
 int glide_persistent_fenced_enable(int on);
+/* Polls a persistent barrier wait makes before it gives up (default 2^20);
+ * a negative value gives up at the first barrier, which exercises the
+ * failure path (NaN results, GLIDE_ERR_TIMEOUT from the next synchronising
+ * call) without any kernel waiting (testing switch; no counterpart in the
+ * reference). */
+int glide_persistent_spin_limit(int polls);
 
 /* Measurement support (no counterpart in the reference): when enabled, HIP
  * event pairs are recorded around every launch of the hot kernels

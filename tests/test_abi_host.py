@@ -86,6 +86,26 @@ def test_hierarchy_checks_on_host():
     cfg.maximum_vcycles = 65
     assert lib.glide_fascd_solve(levels, 2, 0, 1.0, ctypes.byref(cfg), vc, None, None) == -1
     assert "maximum_vcycles" in _lib.last_error()
+    # a negative count is rejected too (it would size the norm read-back)
+    cfg.maximum_vcycles = -2
+    for fn in (lib.glide_fascd_solve, lib.glide_fas_adjoint_solve):
+        assert fn(levels, 2, 0, 1.0, ctypes.byref(cfg), vc, None, None) == -1
+        assert "maximum_vcycles" in _lib.last_error()
+
+
+def test_python_configs_reject_bad_vcycle_counts():
+    from glide_amd.multigrid import FASAdjointConfig, FASCDConfig
+    for cls in (FASCDConfig, FASAdjointConfig):
+        for bad in (-1, 65):
+            with pytest.raises(ValueError, match="maximum_vcycles"):
+                cls(maximum_vcycles=bad).to_c()
+        assert cls(maximum_vcycles=0).to_c().maximum_vcycles == 0
+
+
+def test_status_codes_in_header():
+    header = (ROOT / "include" / "glide_hip.h").read_text()
+    assert "GLIDE_ERR_NONFINITE = -4" in header
+    assert "GLIDE_ERR_TIMEOUT = -5" in header
 
 
 def test_enthalpy_argument_checks_on_host():
