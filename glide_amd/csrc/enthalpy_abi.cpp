@@ -149,9 +149,13 @@ int make_args(const glide_enth_column* C, double dt, bool needs_state, EnthArgs<
     return GLIDE_OK;
 }
 
-int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
-               glide_enth_report* report, hipStream_t st, const R* u2d = nullptr, const R* v2d = nullptr,
-               double vel_div = 1.0) {
+// The fused sweep with its reduction partials (kEnthResBlocks values) and
+// status record in caller-chosen device memory; `host` is a pinned mirror of
+// the record for the report, or nullptr (then the report is read through a
+// stack copy).
+int sweep_core(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
+               glide_enth_report* report, hipStream_t st, double* partials, EnthStatus* status, EnthStatus* host,
+               const R* u2d, const R* v2d, double vel_div) {
     if (!cfg) return fail(GLIDE_ERR_INVALID_ARG, "sweep config is NULL");
     if (n_iter < 0 || n_iter > GLIDE_ENTH_MAX_SWEEPS)
         return fail(GLIDE_ERR_INVALID_ARG, "n_iter must be in [0, GLIDE_ENTH_MAX_SWEEPS]");
@@ -164,13 +168,11 @@ int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_en
     REQ(C->r_E);
     if (cfg->alternating) REQ(C->E_work);
     else REQ(C->delta_E);
-    EnthScratch* s = nullptr;
-    GLIDE_TRY(scratch(&s));
     const double rtol = cfg->relative_tolerance, atol = cfg->absolute_tolerance;
-    HIP_TRY(launch_enth_residual<R>(A, P(C->r_E), 1, s->partials, nullptr, s->status, 0, rtol, atol, st),
+    HIP_TRY(launch_enth_residual<R>(A, P(C->r_E), 1, partials, nullptr, status, 0, rtol, atol, st),
             "enthalpy residual");
     EnthArgs<R> Ak = A;
-    Ak.skip = &s->status->done;
+    Ak.skip = &status->done;
     for (int it = 0; it < n_iter; ++it) {
         if (cfg->alternating) {
             HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, P(C->E_work), cfg->omega, cfg->n_newton,
@@ -185,24 +187,46 @@ int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_en
                     "enthalpy column smoother");
             HIP_TRY(launch_enth_apply<R>(Ak, P(C->E), CP(C->delta_E), cfg->omega, st), "enthalpy update");
         }
-        HIP_TRY(launch_enth_residual<R>(Ak, P(C->r_E), 1, s->partials, nullptr, s->status, 1, rtol, atol, st),
+        HIP_TRY(launch_enth_residual<R>(Ak, P(C->r_E), 1, partials, nullptr, status, 1, rtol, atol, st),
                 "enthalpy residual");
     }
     if (report) {
-        HIP_TRY(hipMemcpyAsync(s->host, s->status, sizeof(EnthStatus), hipMemcpyDeviceToHost, st), "copy status");
+        EnthStatus local;
+        EnthStatus* h = host ? host : &local;
+        HIP_TRY(hipMemcpyAsync(h, status, sizeof(EnthStatus), hipMemcpyDeviceToHost, st), "copy status");
         HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
-        report->sweeps = s->host->sweeps;
+        report->sweeps = h->sweeps;
         report->pad = 0;
-        report->r0 = s->host->r0;
-        report->r = s->host->r;
+        report->r0 = h->r0;
+        report->r = h->r;
         std::memset(report->norms, 0, sizeof(report->norms));
-        for (int q = 0; q < s->host->sweeps && q < GLIDE_ENTH_MAX_SWEEPS; ++q) report->norms[q] = s->host->norms[q];
+        for (int q = 0; q < h->sweeps && q < GLIDE_ENTH_MAX_SWEEPS; ++q) report->norms[q] = h->norms[q];
         GLIDE_TRY(check_report_finite(*report));
     }
     return GLIDE_OK;
 }
 
+int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
+               glide_enth_report* report, hipStream_t st, const R* u2d = nullptr, const R* v2d = nullptr,
+               double vel_div = 1.0) {
+    // argument errors are reported before any device work (or allocation)
+    if (!cfg) return fail(GLIDE_ERR_INVALID_ARG, "sweep config is NULL");
+    if (n_iter < 0 || n_iter > GLIDE_ENTH_MAX_SWEEPS)
+        return fail(GLIDE_ERR_INVALID_ARG, "n_iter must be in [0, GLIDE_ENTH_MAX_SWEEPS]");
+    EnthScratch* s = nullptr;
+    GLIDE_TRY(scratch(&s));
+    return sweep_core(C, dt, n_iter, cfg, report, st, s->partials, s->status, s->host, u2d, v2d, vel_div);
+}
+
 }  // namespace
+
+namespace glide {
+int enth_column_sweep_with_scratch(const glide_enth_column* C, double dt, int n_iter,
+                                   const glide_enth_sweep_config* cfg, glide_enth_report* report, hipStream_t st,
+                                   double* partials, EnthStatus* status) {
+    return sweep_core(C, dt, n_iter, cfg, report, st, partials, status, nullptr, nullptr, nullptr, 1.0);
+}
+}  // namespace glide
 
 extern "C" {
 

@@ -661,6 +661,7 @@ std::string solve_key(char tag, const FasCtx& X, int start, bool norms) {
     key_append(k, g_persistent_forward);
     key_append(k, g_persistent_adjoint);
     key_append(k, g_persistent_fenced);
+    key_append(k, g_spin_limit);
     key_append(k, X.n_levels);
     key_append(k, start);
     key_append(k, X.dt);
