@@ -340,3 +340,168 @@ def test_adjoint_solve_contracts(orc):
     om.put("af_H", np.zeros((32, 32)))
     rep = om.fas_adjoint_solve(DT, cfg)
     assert rep.norms[rep.vcycles - 1] < 1e-2 * rep.initial_norm
+
+
+# ---------------------------------------------------------------- closed-form value pins
+# Each pins a term's value (not only its consistency with the oracle's own
+# derivatives): a wrong factor or sign in the basal drag, driving stress,
+# stress coefficients, grounded sigmoid or the omitted Jacobian couplings
+# fails one of them.
+
+def _slab(ny, nx, seed=0):
+    F = dome_fields(ny=ny, nx=nx, seed=seed, with_velocity=False)
+    for k in ("f_u", "f_v", "f_H"):
+        F[k] = np.zeros_like(F[k])
+    F["mask"] = np.zeros((ny, nx))
+    return F
+
+
+@pytest.mark.parametrize("m", [1.0, 1.0 / 3.0])
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_plug_flow_slab_closed_form(orc, m, axis):
+    """Uniform thickness on a bed of constant slope s (downhill towards +x, or
+    towards the top row for y), uniform beta and grounded fraction phi, and a
+    uniform plug velocity w along the slope: every strain rate vanishes, so an
+    interior momentum row is the basal drag minus the driving stress,
+        r(w) = H phi s - (beta phi (w^2 + u_reg)^((m-1)/2) + water_drag) w,
+    (head units, glide/cuda/stress.cu:253-548), zero at the sliding speed w*
+    (a scalar root); the cross-slope rows vanish."""
+    from scipy.optimize import brentq
+    ny, nx = 8, 12
+    P = orc.phys(m=m)
+    H, s, beta, phi = 600.0, 2e-3, 7.3, 0.7
+    F = _slab(ny, nx)
+    F["H"] = np.full((ny, nx), H)
+    F["beta"] = np.full((ny, nx), beta)
+    F["phi"] = np.full((ny, nx), phi)
+    if axis == "x":
+        F["bed"] = np.tile(1000.0 - s * (np.arange(nx) + 0.5) * DX, (ny, 1))
+    else:  # row i is the top: the bed rises with i, downhill is towards the top (+v)
+        F["bed"] = np.tile((1000.0 + s * (np.arange(ny) + 0.5) * DX)[:, None], (1, nx))
+
+    def r_closed(w):
+        return H * phi * s - (beta * phi * (w * w + P.u_reg) ** ((m - 1.0) / 2.0) + P.water_drag) * w
+
+    w_star = brentq(r_closed, 0.0, 1e6, xtol=1e-14, rtol=1e-15)
+    for w in (0.0, 0.37 * w_star, w_star, 1.9 * w_star):
+        F["u"] = np.full((ny, nx + 1), w if axis == "x" else 0.0)
+        F["v"] = np.full((ny + 1, nx), w if axis == "y" else 0.0)
+        ru, rv, _ = orc.residual(F, P, DX, DT, use_forcing=False, use_mask=False)
+        along, across = (ru[:, 1:-1], rv[1:-1]) if axis == "x" else (rv[1:-1], ru[:, 1:-1])
+        scale = H * phi * s
+        assert np.abs(along - r_closed(w)).max() <= 1e-12 * scale, w
+        assert np.abs(across).max() <= 1e-14 * scale
+    assert r_closed(w_star) == pytest.approx(0.0, abs=1e-12 * H * phi * s) and w_star > 0
+
+
+def test_manufactured_quadratic_velocity_n1(orc):
+    """n = 1 makes eta = B/2 independent of the strain rate.  For quadratic
+    u = a x^2 + c y^2, v = d x^2 + e y^2 (y up: row i is the top) with uniform
+    H, B, no basal drag (beta = 0) and a flat grounded bed, the centred
+    differences are exact, so interior rows equal the continuum operator
+        r_u = d/dx[2 eta H (2 u_x + v_y)] + d/dy[eta H (u_y + v_x)] - wd u = eta H (8a + 2c) - wd u
+        r_v = d/dy[2 eta H (u_x + 2 v_y)] + d/dx[eta H (u_y + v_x)] - wd v = eta H (8e + 2d) - wd v
+    which fixes the sigma_xx / sigma_yy / sigma_xy coefficients and signs."""
+    ny, nx = 10, 12
+    P = orc.phys(n=1.0)
+    rng = np.random.default_rng(1)
+    a, c, d, e = 1e-7 * rng.standard_normal(4)
+    Hc, Bc = 700.0, 3.0
+    F = _slab(ny, nx)
+    F["H"], F["B"] = np.full((ny, nx), Hc), np.full((ny, nx), Bc)
+    F["bed"], F["beta"], F["phi"] = np.zeros((ny, nx)), np.zeros((ny, nx)), np.ones((ny, nx))
+    xu, yu = np.arange(nx + 1) * DX, -(np.arange(ny) + 0.5) * DX
+    xv, yv = (np.arange(nx) + 0.5) * DX, -np.arange(ny + 1) * DX
+    F["u"] = a * xu[None, :] ** 2 + c * yu[:, None] ** 2
+    F["v"] = d * xv[None, :] ** 2 + e * yv[:, None] ** 2
+    ru, rv, _ = orc.residual(F, P, DX, DT, use_forcing=False, use_mask=False)
+    K = 0.5 * Bc * Hc
+    ref_u = K * (8 * a + 2 * c) - P.water_drag * F["u"]
+    ref_v = K * (8 * e + 2 * d) - P.water_drag * F["v"]
+    # rows whose stencil stays off the boundary vertices (sigma_xy = 0 there)
+    assert rel_err(ru[2:-2, 2:-2], ref_u[2:-2, 2:-2]) < 1e-12
+    assert rel_err(rv[2:-2, 2:-2], ref_v[2:-2, 2:-2]) < 1e-12
+
+
+def test_grounded_fraction_closed_forms(orc):
+    """phi = 1 / (1 + exp(-clamp(c (bed + 0.917 H), -20, 20))): one half at
+    flotation, the clamp at +-20, and a value inside the clamp range
+    (glide/cuda/common.cu:66-82, glide/cuda/viscosity.cu:16-36)."""
+    c = 0.1
+    H = np.array([[100.0, 200.0, 300.0], [10.0, 2500.0, 55.0]])
+    assert np.array_equal(orc.grounded(H, -0.917 * H, c=c), np.full_like(H, 0.5))
+    bed = np.array([[1000.0, -1000.0, 3.0], [-5.0, -2500.0, -60.0]])
+    z = np.clip(c * (bed + 0.917 * H), -20.0, 20.0)
+    got = orc.grounded(H, bed, c=c)
+    assert got[0, 0] == pytest.approx(1.0 / (1.0 + np.exp(-20.0)), rel=1e-15)
+    assert got[0, 1] == pytest.approx(1.0 / (1.0 + np.exp(20.0)), rel=1e-14)
+    assert rel_err(got, 1.0 / (1.0 + np.exp(-z))) < 1e-15
+
+
+def test_driving_stress_flotation_closed_form(orc):
+    """At rest (u = v = 0: no strain, no drag) a momentum row is minus the
+    driving stress -H_avg (S_B - S_A)/dx with the flotation-blended surface
+    S = phi bed - (1 - phi) 0.917 H + H (x: A left, B right; y: A bottom,
+    B top), which is (1 - 0.917) H for floating ice (phi = 0) and bed + H for
+    grounded ice (glide/cuda/stress.cu:418-548)."""
+    ny, nx = 8, 12
+    F = _slab(ny, nx, seed=3)
+    rng = np.random.default_rng(4)
+    F["phi"] = rng.random((ny, nx))
+    F["phi"][:, :3] = 0.0     # a floating strip
+    F["phi"][:, -3:] = 1.0    # a grounded strip
+    F["beta"] = 10.0 * rng.random((ny, nx))
+    F["u"], F["v"] = np.zeros((ny, nx + 1)), np.zeros((ny + 1, nx))
+    ru, rv, _ = orc.residual(F, orc.phys(), DX, DT, use_forcing=False, use_mask=False)
+    H, b, p = F["H"], F["bed"], F["phi"]
+    S = np.where(p == 0.0, (1.0 - 0.917) * H, np.where(p == 1.0, b + H, p * b - (1 - p) * 0.917 * H + H))
+    ref_u = -0.5 * (H[:, 1:] + H[:, :-1]) * (S[:, 1:] - S[:, :-1]) / DX
+    ref_v = -0.5 * (H[1:] + H[:-1]) * (S[:-1] - S[1:]) / DX
+    assert rel_err(ru[:, 1:-1], ref_u) < 1e-13
+    assert rel_err(rv[1:-1], ref_v) < 1e-13
+
+
+@pytest.mark.parametrize("cell", [(1, 1), (3, 4), (5, 8)])
+def test_local_jacobian_omits_exactly_the_shear_cross_couplings(orc, cell):
+    """The reference's 5x5 patch Jacobian leaves out the dependence of the
+    corner shear stresses on the other velocity component (glide/cuda/vanka.cu
+    assembles only the own-component derivative of sigma_xy).  With n = 1
+    (eta = B/2), uniform B and linear drag, the complete derivative of the
+    local residual (central differences, viscosity frozen) minus the
+    reference's entry is, for the four corner vertices v with
+    eta H_v = (B/8) (sum of the four adjacent H),
+        d(u_l, v_t) = d(v_t, u_l) = D^2 eta H_tl,  d(u_l, v_b) = d(v_b, u_l) = -D^2 eta H_bl,
+        d(u_r, v_t) = d(v_t, u_r) = -D^2 eta H_tr, d(u_r, v_b) = d(v_b, u_r) = D^2 eta H_br,
+    and zero for every other entry (D = 1/dx)."""
+    ny, nx = 8, 11
+    F = dome_fields(ny=ny, nx=nx, seed=5)
+    F["B"] = np.full((ny, nx), 2.5)
+    P = orc.phys(n=1.0)
+    i, j = cell
+    x0 = np.array([F["u"][i, j], F["u"][i, j + 1], F["v"][i, j], F["v"][i + 1, j], F["H"][i, j]])
+    J, _ = orc.local_system(x0, F, P, DX, DT, i, j)
+    full = np.zeros((5, 5))
+    for col in range(5):
+        h = 1e-4 * max(1.0, abs(x0[col]))
+        xp, xm = x0.copy(), x0.copy()
+        xp[col] += h
+        xm[col] -= h
+        full[:, col] = (orc.local_system(xp, F, P, DX, DT, i, j)[1] - orc.local_system(xm, F, P, DX, DT, i, j)[1]) / (2 * h)
+    Hf = F["H"]
+
+    def eH(i0, j0):  # vertex at the top-left corner of cell (i0, j0)
+        return 2.5 / 8.0 * (Hf[i0 - 1, j0 - 1] + Hf[i0 - 1, j0] + Hf[i0, j0 - 1] + Hf[i0, j0])
+
+    D2 = 1.0 / DX ** 2
+    tl, tr, bl, br = eH(i, j), eH(i, j + 1), eH(i + 1, j), eH(i + 1, j + 1)
+    dropped = np.zeros((5, 5))
+    dropped[0, 2] = dropped[2, 0] = D2 * tl
+    dropped[0, 3] = dropped[3, 0] = -D2 * bl
+    dropped[1, 2] = dropped[2, 1] = -D2 * tr
+    dropped[1, 3] = dropped[3, 1] = D2 * br
+    scale = np.abs(J).max()
+    assert np.abs((full - J) - dropped).max() <= 1e-7 * scale
+    # the omitted couplings are not negligible against the velocity block
+    assert np.abs(dropped).max() > 1e-3 * np.abs(J[:4, :4]).max()
+    # the reference's velocity block (and the complete one) is symmetric for linear drag
+    assert np.abs(J[:4, :4] - J[:4, :4].T).max() <= 1e-15 * scale
