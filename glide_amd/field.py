@@ -1,52 +1,73 @@
-"""Fields, constants and option wrappers of the drop-in API.
+"""Fields, scalar constants and option handles of the drop-in API.
 
-Mirrors glide/field.py: a :class:`Field` owns one array (here a torch tensor
-on the ROCm device), knows which MAC-grid entity it lives on, sets values in
-place and allocates its gradient lazily (glide/field.py:41-75); a
-:class:`Constant` is one scalar parameter (glide/field.py:212-248);
-:class:`LocalOption` / :class:`BroadcastOption` wrap solver configs
-(glide/field.py:254-286).
+Public names and constructor signatures follow glide/field.py (Field,
+Constant, SubgridField, GridEntity, LocalOption, BroadcastOption; lines
+12-16, 41-75, 212-248, 250-286) because grid.py, multigrid.py and user code
+construct and query them by those names.  The implementation is this
+package's own: a field is a thin view over one float64 torch tensor on the
+ROCm device whose gradient buffer is created on first access, and an option
+handle is a (read, write) pair over solver configuration objects.
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
 from enum import Enum
-from typing import Any
+from typing import Any, Callable, Sequence
 
 import numpy as np
 import torch
 
 
 class GridEntity(str, Enum):
+    """Where a field lives on the MAC grid (the reference's string values)."""
     CELL = "cell"
-    VERTICAL_FACET = "vertical_facet"
-    HORIZONTAL_FACET = "horizontal_facet"
+    VERTICAL_FACET = "vertical_facet"      # u: (ny, nx + 1)
+    HORIZONTAL_FACET = "horizontal_facet"  # v: (ny + 1, nx)
 
 
-@dataclass
+# Cell average of a facet array: the two slices whose mean gives the cell value.
+_CELL_AVERAGE = {
+    GridEntity.VERTICAL_FACET: (lambda a: a[:, :-1], lambda a: a[:, 1:]),
+    GridEntity.HORIZONTAL_FACET: (lambda a: a[:-1], lambda a: a[1:]),
+}
+
+
+def _as_tensor_like(value, ref: torch.Tensor) -> torch.Tensor:
+    t = value if isinstance(value, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(value))
+    return t.to(device=ref.device, dtype=ref.dtype)
+
+
 class Field:
-    data: torch.Tensor
-    grid_entity: GridEntity
-    dx: float
-    grid: Any = field(default=None, repr=False, compare=False)
-    name: str | None = None
-    units: str | None = None
-    attrs: dict = field(default_factory=dict)
-    _grad: torch.Tensor | None = None
+    """One gridded array plus its metadata and a lazily created gradient."""
 
+    __slots__ = ("data", "grid_entity", "dx", "grid", "name", "units", "attrs", "_grad")
+
+    def __init__(self, data: torch.Tensor, grid_entity: GridEntity, dx: float, grid: Any = None,
+                 name: str | None = None, units: str | None = None, attrs: dict | None = None,
+                 _grad: torch.Tensor | None = None):
+        self.data = data
+        self.grid_entity = GridEntity(grid_entity)
+        self.dx = dx
+        self.grid = grid
+        self.name = name
+        self.units = units
+        self.attrs = {} if attrs is None else attrs
+        self._grad = _grad
+
+    # values -------------------------------------------------------------
     def set(self, value) -> None:
-        """Copy `value` into the field in place (scalars fill)."""
-        if hasattr(value, "shape"):
-            t = torch.as_tensor(np.asarray(value) if not isinstance(value, torch.Tensor) else value)
-            if tuple(t.shape) != tuple(self.data.shape):
-                raise ValueError(f"{self.name}: shape {tuple(t.shape)} != {tuple(self.data.shape)}")
-            self.data.copy_(t.to(device=self.data.device, dtype=self.data.dtype))
-        else:
+        """In-place assignment: arrays must match the field's shape, scalars fill."""
+        if not hasattr(value, "shape"):
             self.data.fill_(float(value))
+            return
+        shape = tuple(value.shape)
+        if shape != tuple(self.data.shape):
+            raise ValueError(f"cannot set field {self.name!r} of shape {tuple(self.data.shape)} from shape {shape}")
+        self.data.copy_(_as_tensor_like(value, self.data))
 
     def zero(self) -> None:
         self.data.zero_()
 
+    # gradient -----------------------------------------------------------
     @property
     def grad(self) -> torch.Tensor:
         if self._grad is None:
@@ -60,42 +81,54 @@ class Field:
         if self._grad is not None:
             self._grad.zero_()
 
+    # conversions --------------------------------------------------------
     def to_cell(self) -> "Field":
-        if self.grid_entity is GridEntity.CELL:
+        """This field on cell centres (facet fields averaged over the two faces)."""
+        pick = _CELL_AVERAGE.get(self.grid_entity)
+        if pick is None:
             return self
-        if self.grid_entity is GridEntity.VERTICAL_FACET:
-            d = 0.5 * (self.data[:, 1:] + self.data[:, :-1])
-        else:
-            d = 0.5 * (self.data[1:] + self.data[:-1])
-        return Field(data=d, grid_entity=GridEntity.CELL, dx=self.dx, grid=self.grid, units=self.units,
-                     attrs=self.attrs)
+        lo, hi = pick
+        return Field(0.5 * (lo(self.data) + hi(self.data)), GridEntity.CELL, self.dx, grid=self.grid,
+                     units=self.units, attrs=self.attrs)
 
     @property
     def compact_string(self) -> str:
-        return f"Field: {self.name}, {self.units}, ({self.data.shape[0]}, {self.data.shape[1]})"
+        ny, nx = self.data.shape[:2]
+        return f"{self.name} [{self.units or '-'}] {self.grid_entity.value} {ny}x{nx}"
 
     def __repr__(self) -> str:
-        return f"Field: {self.name}\n{self.data}\n{tuple(self.data.shape)}, {self.data.dtype}, {self.units}"
+        return (f"Field({self.name!r}, entity={self.grid_entity.value}, shape={tuple(self.data.shape)}, "
+                f"dtype={self.data.dtype}, units={self.units!r}, device={self.data.device})")
 
 
-@dataclass
 class SubgridField(Field):
-    quantiles: Any | None = None
+    """A cell field with optional sub-grid quantiles (the bed of glide/grid.py:31)."""
+
+    __slots__ = ("quantiles",)
+
+    def __init__(self, *args, quantiles: Any = None, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.quantiles = quantiles
 
 
-@dataclass
 class Constant:
-    value: float
-    name: str | None = None
-    units: str | None = None
-    attrs: dict = field(default_factory=dict)
-    _grad: Any | None = None
+    """A scalar physical parameter with a scalar gradient slot."""
+
+    __slots__ = ("value", "name", "units", "attrs", "_grad")
+
+    def __init__(self, value: float, name: str | None = None, units: str | None = None, attrs: dict | None = None,
+                 _grad: float | None = None):
+        self.value = float(value)
+        self.name = name
+        self.units = units
+        self.attrs = {} if attrs is None else attrs
+        self._grad = _grad
 
     def set(self, value) -> None:
         self.value = float(value)
 
     @property
-    def grad(self):
+    def grad(self) -> float:
         if self._grad is None:
             self._grad = 0.0
         return self._grad
@@ -108,37 +141,42 @@ class Constant:
             self._grad = 0.0
 
     def __repr__(self) -> str:
-        return f"Constant: {self.name}, {self.value:.3f}, {self.units}"
+        return f"{self.name} = {self.value:g} {self.units or ''}".rstrip()
 
 
-class LocalOption:
-    def __init__(self, getter, setter, name: str):
-        self._getter = getter
-        self._setter = setter
-        self._name = name
+class _OptionHandle:
+    """A named (read, write) pair; prints as ``name=value``."""
 
-    def get(self):
-        return self._getter()
-
-    def set(self, value):
-        self._setter(value)
-
-    def __repr__(self):
-        return f"{self._name}={self.get()!r}"
-
-
-class BroadcastOption:
-    def __init__(self, levels, getter, attr_name: str):
-        self._levels = levels
-        self._getter = getter
-        self._attr_name = attr_name
+    def __init__(self, label: str, read: Callable[[], Any], write: Callable[[Any], None]):
+        self._label, self._read, self._write = label, read, write
 
     def get(self):
-        return getattr(self._getter(self._levels[0]), self._attr_name)
+        return self._read()
 
-    def set(self, value):
-        for lev in self._levels:
-            setattr(self._getter(lev), self._attr_name, value)
+    def set(self, value) -> None:
+        self._write(value)
 
-    def __repr__(self):
-        return f"{self._attr_name}={self.get()!r}"
+    def __repr__(self) -> str:
+        return f"{self._label}={self.get()!r}"
+
+
+class LocalOption(_OptionHandle):
+    """An option of one configuration object (glide/field.py:250-265 signature)."""
+
+    def __init__(self, getter: Callable[[], Any], setter: Callable[[Any], None], name: str):
+        super().__init__(name, getter, setter)
+
+
+class BroadcastOption(_OptionHandle):
+    """An attribute shared by the per-level configurations of a hierarchy:
+    reads the finest level's value, writes every level's (glide/field.py:268-286
+    signature)."""
+
+    def __init__(self, levels: Sequence, getter: Callable[[Any], Any], attr_name: str):
+        configs = lambda: [getter(lev) for lev in levels]  # noqa: E731 (re-resolved on every access)
+
+        def write(value):
+            for cfg in configs():
+                setattr(cfg, attr_name, value)
+
+        super().__init__(attr_name, lambda: getattr(configs()[0], attr_name), write)

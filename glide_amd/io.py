@@ -1,254 +1,259 @@
-"""VTK ImageData (.vti) output with a PVD time manifest (glide/io.py:23-389).
+"""VTK ImageData (.vti) snapshots and a ParaView .pvd time collection.
 
-Files are byte-compatible with the reference's writer: an XML header whose
-point-data arrays are ``Float32`` in ``appended`` raw encoding, each block
-preceded by its byte count as a little-endian uint32, values ordered x fastest
-(then y, then z), rows flipped so that row 0 is the top of the image unless
-``flip_y=False``, and an optional ``TimeValue`` field.  Fields are torch
-tensors (device or host) or NumPy arrays; device tensors are copied to the host
-once per field.  The reference's ``ZarrWriter`` (xarray/zarr) has no
-counterpart here: neither library is available in this environment.
+Output files keep the layout of the reference's writer (glide/io.py:23-389)
+so that existing ParaView state files and readers work unchanged; the layout
+itself is the VTK XML ImageData format:
+
+* one ``<ImageData>`` piece whose point data are ``Float32`` arrays in
+  ``appended`` raw encoding; the appended section starts with ``_`` and holds,
+  per array, a little-endian uint32 byte count followed by the values;
+* points run x fastest, then y, then z, so a (ny, nx, nz) enthalpy array is
+  written as its (nz, ny, nx) transpose;
+* image row 0 is the bottom: the grid's row 0 (the top) is written last
+  unless ``flip_y=False``;
+* an optional ``TimeValue`` in the piece's field data.
+
+Arrays may be torch tensors (on the device or the host) or numpy arrays; a
+device tensor is copied to the host once.  The reference's Zarr/xarray writer
+has no counterpart: neither library is available here.
 """
 from __future__ import annotations
 
 import struct
 import xml.etree.ElementTree as ET
-from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Any, Mapping
-from xml.dom import minidom
 
 import numpy as np
 
 __all__ = ["write_vti", "read_vti", "VTIWriter"]
 
+_XML_DECL = "<?xml version='1.0' encoding='utf-8'?>\n"
+_ESCAPES = (("&", "&amp;"), ("<", "&lt;"), (">", "&gt;"), ('"', "&quot;"), ("\n", "&#10;"))
 
-def _host_f32(value) -> np.ndarray:
+
+def _q(text) -> str:
+    """A quoted XML attribute value."""
+    s = str(text)
+    for a, b in _ESCAPES:
+        s = s.replace(a, b)
+    return f'"{s}"'
+
+
+def _to_host_f32(value) -> np.ndarray:
     if hasattr(value, "detach"):
         value = value.detach().to("cpu").numpy()
     return np.asarray(value, dtype=np.float32)
 
 
-def write_vti(filename, data, dx, dy=None, dz=None, origin=(0.0, 0.0), time_value=None, flip_y=True):
-    """Fields to one .vti file (glide/io.py:23-185).
+def _point_order(a: np.ndarray, flip_y: bool) -> np.ndarray:
+    """A grid array in VTK point order (x fastest), bottom row first if flipped."""
+    if flip_y:
+        a = a[::-1]
+    return np.ascontiguousarray(np.moveaxis(a, 2, 0) if a.ndim == 3 else a)
 
-    ``data`` maps names to 2-D ``(ny, nx)`` or 3-D ``(ny, nx, nz)`` arrays, or
-    (2-D only) to a list of component arrays for a vector field.  All fields of
-    one file share the first field's dimensionality; 3-D output needs ``dz``.
-    """
+
+class _Arrays:
+    """The point-data arrays of one file: name, component count, raw bytes."""
+
+    def __init__(self, data: Mapping[str, Any], flip_y: bool):
+        self.scalars, self.vectors = [], []
+        for name, value in data.items():
+            if isinstance(value, (list, tuple)):
+                comps = [_point_order(_to_host_f32(c), flip_y) for c in value]
+                self.vectors.append((name, len(comps), np.stack(comps, axis=-1)))
+            else:
+                self.scalars.append((name, 1, _point_order(_to_host_f32(value), flip_y)))
+        if not self.scalars and not self.vectors:
+            raise ValueError("write_vti needs at least one field")
+        lead = (self.scalars or self.vectors)[0][2]
+        self.ndim = lead.ndim - (0 if self.scalars else 1)
+        if self.ndim not in (2, 3):
+            raise ValueError(f"fields must be (ny, nx) or (ny, nx, nz) arrays, not {lead.shape}")
+        if self.ndim == 3 and self.vectors:
+            raise NotImplementedError("vector fields are written for 2-D grids only")
+        for name, _, a in self.scalars:
+            if a.ndim != self.ndim:
+                raise ValueError(f"field {name!r} is {a.ndim}-D but the file's first field is {self.ndim}-D")
+        # point counts along x, y, z (the arrays are already in point order)
+        if self.ndim == 2:
+            ny, nx = lead.shape[:2]
+            self.npts = (nx, ny, 1)
+        else:
+            nz, ny, nx = lead.shape
+            self.npts = (nx, ny, nz)
+
+    def ordered(self):
+        return self.scalars + self.vectors
+
+
+def write_vti(filename, data, dx, dy=None, dz=None, origin=(0.0, 0.0), time_value=None, flip_y=True):
+    """Write ``data`` (name -> (ny, nx) or (ny, nx, nz) array, or for 2-D grids
+    name -> list of component arrays) to one .vti file; 3-D output needs dz.
+    Returns the path."""
+    arrays = _Arrays(data, flip_y)
+    if arrays.ndim == 3 and dz is None:
+        raise ValueError("3-D fields need the vertical spacing dz")
     dy = dx if dy is None else dy
+    ox, oy = float(origin[0]), float(origin[1])
+    oz = float(origin[2]) if len(origin) > 2 else 0.0
+    nx, ny, nz = arrays.npts
+    extent = f"0 {nx - 1} 0 {ny - 1} 0 {nz - 1}"
+    spacing = f"{dx} {dy} {dz if arrays.ndim == 3 else 1.0}"
+
+    parts = [_XML_DECL, '<VTKFile type="ImageData" version="1.0" byte_order="LittleEndian">',
+             f"<ImageData WholeExtent={_q(extent)} Origin={_q(f'{ox} {oy} {oz}')} Spacing={_q(spacing)}>",
+             f"<Piece Extent={_q(extent)}>"]
+    if time_value is not None:
+        parts.append('<FieldData><DataArray type="Float32" Name="TimeValue" NumberOfComponents="1" '
+                     f'format="ascii">\n{float(time_value)}\n</DataArray></FieldData>')
+    active = ""
+    if arrays.scalars:
+        active += f" Scalars={_q(arrays.scalars[0][0])}"
+    three = [name for name, nc, _ in arrays.vectors if nc == 3]
+    if three:
+        active += f" Vectors={_q(three[0])}"
+    parts.append(f"<PointData{active}>")
+    blobs, offset = [], 0
+    for name, ncomp, a in arrays.ordered():
+        raw = a.tobytes()
+        parts.append(f'<DataArray type="Float32" Name={_q(name)} NumberOfComponents="{ncomp}" format="appended" '
+                     f'offset="{offset}" />')
+        blobs.append(struct.pack("<I", len(raw)) + raw)
+        offset += 4 + len(raw)
+    parts.append("</PointData></Piece></ImageData>")
+
     path = Path(filename)
     path.parent.mkdir(parents=True, exist_ok=True)
-    scalars, vectors = {}, {}
-    for name, value in data.items():
-        if isinstance(value, (list, tuple)):
-            comps = [_host_f32(c) for c in value]
-            vectors[name] = [np.flip(c, axis=0) if flip_y else c for c in comps]
-        else:
-            a = _host_f32(value)
-            scalars[name] = np.flip(a, axis=0) if flip_y else a
-    first = next(iter(scalars.values())) if scalars else next(iter(vectors.values()))[0]
-    if first.ndim == 2:
-        three_d = False
-        ny, nx = first.shape
-        nz = 1
-    elif first.ndim == 3:
-        three_d = True
-        ny, nx, nz = first.shape
-        if dz is None:
-            raise ValueError(f"dz must be provided for 3D output (got 3D field with shape {first.shape})")
-        if vectors:
-            raise NotImplementedError("Vector fields are only supported in 2D mode.")
-    else:
-        raise ValueError(f"Field arrays must be 2D or 3D; got shape {first.shape}")
-    for name, a in scalars.items():
-        if a.ndim != first.ndim:
-            raise ValueError(f"Field '{name}' has shape {a.shape}; expected {'3D' if three_d else '2D'} like the "
-                             f"first field.")
-    o = (float(origin[0]), float(origin[1]), float(origin[2]) if len(origin) > 2 else 0.0)
-    extent = f"0 {nx - 1} 0 {ny - 1} 0 {nz - 1}"
-    root = ET.Element("VTKFile", type="ImageData", version="1.0", byte_order="LittleEndian")
-    img = ET.SubElement(root, "ImageData", WholeExtent=extent, Origin=f"{o[0]} {o[1]} {o[2]}",
-                        Spacing=f"{dx} {dy} {dz if three_d else 1.0}")
-    piece = ET.SubElement(img, "Piece", Extent=extent)
-    if time_value is not None:
-        fd = ET.SubElement(piece, "FieldData")
-        tv = ET.SubElement(fd, "DataArray", type="Float32", Name="TimeValue", NumberOfComponents="1",
-                           format="ascii")
-        tv.text = f"\n{float(time_value)}\n"
-    attrs = {}
-    if scalars:
-        attrs["Scalars"] = next(iter(scalars))
-    for name, comps in vectors.items():
-        if len(comps) == 3:
-            attrs["Vectors"] = name
-            break
-    pd = ET.SubElement(piece, "PointData", **attrs)
-    blocks = []
-    offset = 0
-    for name, a in scalars.items():
-        # ImageData points run x fastest, then y, then z: (ny, nx, nz) -> (nz, ny, nx)
-        raw = np.ascontiguousarray(a.transpose(2, 0, 1) if a.ndim == 3 else a).tobytes()
-        ET.SubElement(pd, "DataArray", type="Float32", Name=name, NumberOfComponents="1", format="appended",
-                      offset=str(offset))
-        blocks.append(raw)
-        offset += len(raw) + 4
-    for name, comps in vectors.items():
-        raw = np.ascontiguousarray(np.stack(comps, axis=-1), dtype=np.float32).tobytes()
-        ET.SubElement(pd, "DataArray", type="Float32", Name=name, NumberOfComponents=str(len(comps)),
-                      format="appended", offset=str(offset))
-        blocks.append(raw)
-        offset += len(raw) + 4
-    xml = ET.tostring(root, encoding="utf-8", xml_declaration=True).decode("utf-8")
-    head = xml[: xml.rindex("</VTKFile>")]
     with open(path, "wb") as f:
-        f.write(head.encode("utf-8"))
+        f.write("".join(parts).encode("utf-8"))
         f.write(b'  <AppendedData encoding="raw">\n   _')
-        for raw in blocks:
-            f.write(struct.pack("<I", len(raw)))
-            f.write(raw)
+        for b in blobs:
+            f.write(b)
         f.write(b"\n  </AppendedData>\n</VTKFile>\n")
     return path
 
 
 def read_vti(filename) -> dict[str, Any]:
-    """Parse a file written by :func:`write_vti` (or the reference's writer):
-    returns ``{"extent", "origin", "spacing", "time", "fields"}`` with each
-    field as stored (x fastest; scalars reshaped to (nz, ny, nx) or (ny, nx),
-    vectors to (..., ncomp))."""
+    """Read a file in the layout above: ``{"extent", "origin", "spacing",
+    "time", "fields"}``, each field in point order ((ny, nx) or (nz, ny, nx),
+    with a trailing component axis for vectors)."""
     blob = Path(filename).read_bytes()
-    marker = b'<AppendedData encoding="raw">'
-    start = blob.index(marker)
-    us = blob.index(b"_", start + len(marker)) + 1
-    header = blob[:start].decode("utf-8") + "</VTKFile>"
-    root = ET.fromstring(header)
-    img = root.find("ImageData")
-    ext = [int(v) for v in img.get("WholeExtent").split()]
-    nx, ny, nz = ext[1] - ext[0] + 1, ext[3] - ext[2] + 1, ext[5] - ext[4] + 1
-    out = {"extent": ext, "origin": [float(v) for v in img.get("Origin").split()],
-           "spacing": [float(v) for v in img.get("Spacing").split()], "time": None, "fields": {}}
-    piece = img.find("Piece")
-    fd = piece.find("FieldData")
-    if fd is not None:
-        for da in fd.findall("DataArray"):
-            if da.get("Name") == "TimeValue":
-                out["time"] = float(da.text.strip())
-    for da in piece.find("PointData").findall("DataArray"):
-        off = us + int(da.get("offset"))
-        (nbytes,) = struct.unpack("<I", blob[off:off + 4])
-        vals = np.frombuffer(blob[off + 4:off + 4 + nbytes], dtype="<f4")
-        ncomp = int(da.get("NumberOfComponents"))
-        shape = (ny, nx) if nz == 1 else (nz, ny, nx)
-        out["fields"][da.get("Name")] = vals.reshape(shape + ((ncomp,) if ncomp > 1 else ()))
-    return out
+    tag = b'<AppendedData encoding="raw">'
+    at = blob.index(tag)
+    data0 = blob.index(b"_", at + len(tag)) + 1
+    root = ET.fromstring(blob[:at] + b"</VTKFile>")
+    image = root.find("ImageData")
+    e = [int(t) for t in image.get("WholeExtent").split()]
+    nx, ny, nz = e[1] - e[0] + 1, e[3] - e[2] + 1, e[5] - e[4] + 1
+    piece = image.find("Piece")
+    time = None
+    for arr in piece.iterfind("FieldData/DataArray"):
+        if arr.get("Name") == "TimeValue":
+            time = float(arr.text)
+    shape = (ny, nx) if nz == 1 else (nz, ny, nx)
+    fields = {}
+    for arr in piece.find("PointData").iterfind("DataArray"):
+        start = data0 + int(arr.get("offset"))
+        (n,) = struct.unpack_from("<I", blob, start)
+        vals = np.frombuffer(blob, dtype="<f4", count=n // 4, offset=start + 4)
+        nc = int(arr.get("NumberOfComponents"))
+        fields[arr.get("Name")] = vals.reshape(shape + ((nc,) if nc > 1 else ()))
+    return {"extent": e, "origin": [float(t) for t in image.get("Origin").split()],
+            "spacing": [float(t) for t in image.get("Spacing").split()], "time": time, "fields": fields}
 
 
-def _pretty_xml(element) -> str:
-    return minidom.parseString(ET.tostring(element)).toprettyxml(indent="  ")
+def _field_values(value: Any) -> Any:
+    """What a configured output entry writes: callables are evaluated now,
+    grid fields are collocated to cell centres, arrays pass through."""
+    if callable(value) and not hasattr(value, "data"):
+        value = value()
+    if hasattr(value, "to_cell") and not hasattr(value, "detach"):
+        return value.to_cell().data
+    if hasattr(value, "data") and not isinstance(value, np.ndarray) and not hasattr(value, "detach"):
+        return value.data
+    return value
 
 
-@dataclass
 class VTIWriter:
-    """Static and time-dependent .vti output with a .pvd manifest
-    (glide/io.py:188-389): configure ``static_fields`` / ``dynamic_fields``
-    (fields, constants, arrays, callables evaluated at write time, or lists of
-    them for vectors, collocated to cells through ``to_cell()``), call
-    :meth:`initialize` once and :meth:`append` per snapshot."""
+    """Snapshot series of a simulation (interface of glide/io.py:188-389):
+    ``static_fields`` go once to ``{base}_static.vti``; every :meth:`append`
+    writes ``dynamic_fields`` to ``{base}_{step:04d}.vti``; :meth:`write_pvd`
+    writes the ``{base}.pvd`` collection of the steps.  Entries are grid
+    fields, arrays, callables returning either, or lists of them (vectors)."""
 
-    out_dir: str | Path
-    static_fields: Mapping[str, Any] = field(default_factory=dict)
-    dynamic_fields: Mapping[str, Any] = field(default_factory=dict)
-    base: str = "output"
-    dx: float = 1.0
-    dy: float | None = None
-    dz: float | None = None
-    origin: tuple = (0.0, 0.0)
-    flip_y: bool = True
-
-    _step_idx: int = field(default=0, init=False, repr=False)
-    _static_written: bool = field(default=False, init=False, repr=False)
-    records: list = field(default_factory=list, init=False, repr=False)
-
-    def __post_init__(self) -> None:
-        self.out_dir = Path(self.out_dir)
+    def __init__(self, out_dir, static_fields: Mapping[str, Any] | None = None,
+                 dynamic_fields: Mapping[str, Any] | None = None, base: str = "output", dx: float = 1.0,
+                 dy: float | None = None, dz: float | None = None, origin: tuple = (0.0, 0.0), flip_y: bool = True):
+        self.out_dir = Path(out_dir)
         self.out_dir.mkdir(parents=True, exist_ok=True)
-        if self.dy is None:
-            self.dy = self.dx
+        self.static_fields = dict(static_fields or {})
+        self.dynamic_fields = dict(dynamic_fields or {})
+        self.base = base
+        self.dx = dx
+        self.dy = dx if dy is None else dy
+        self.dz = dz
+        self.origin = origin
+        self.flip_y = flip_y
+        self.records: list[tuple[float, str]] = []
+        self._next_step = 0
+        self._static_done = False
+
+    def _write(self, path: Path, entries: Mapping[str, Any], time_value) -> Path:
+        payload = {k: ([_field_values(c) for c in v] if isinstance(v, (list, tuple)) else _field_values(v))
+                   for k, v in entries.items()}
+        return write_vti(path, payload, self.dx, dy=self.dy, dz=self.dz, origin=self.origin, time_value=time_value,
+                         flip_y=self.flip_y)
 
     def initialize(self, grid: Any, *, overwrite: bool = False, write_dynamic_initial: bool = False,
                    time: Any | None = None) -> None:
-        """Prepare the directory; write the static fields and, on request, a
-        first dynamic snapshot."""
+        """Start a series: with ``overwrite`` remove this base's earlier files
+        and records; write the static fields; optionally a first snapshot."""
         if overwrite:
-            for p in self.out_dir.glob(f"{self.base}_*.vti"):
-                p.unlink()
-            pvd = self.out_dir / f"{self.base}.pvd"
-            if pvd.exists():
-                pvd.unlink()
-            self.records.clear()
-            self._step_idx = 0
-            self._static_written = False
-        if self.static_fields:
-            self.write_static(grid, overwrite=overwrite)
+            for old in list(self.out_dir.glob(f"{self.base}_*.vti")) + [self.out_dir / f"{self.base}.pvd"]:
+                old.unlink(missing_ok=True)
+            self.records = []
+            self._next_step = 0
+            self._static_done = False
+        self.write_static(grid, overwrite=overwrite)
         if write_dynamic_initial:
             if time is None:
-                raise ValueError("Argument `time` must be provided when write_dynamic_initial=True.")
+                raise ValueError("an initial snapshot needs its time")
             self.append(grid, time=time)
 
-    def append(self, grid: Any, *, time: Any) -> Path:
-        """One dynamic snapshot at ``time``."""
-        if not self.dynamic_fields:
-            raise ValueError("No dynamic fields were configured for this writer.")
-        return self.append_dynamic(grid, time=time)
-
     def write_static(self, grid: Any, *, overwrite: bool = False) -> Path | None:
-        """The static fields, once, to ``{base}_static.vti``."""
         if not self.static_fields:
             return None
         path = self.out_dir / f"{self.base}_static.vti"
-        if self._static_written and not overwrite:
-            return path
-        write_vti(path, self._payload(self.static_fields), self.dx, dy=self.dy, dz=self.dz, origin=self.origin,
-                  time_value=None, flip_y=self.flip_y)
-        self._static_written = True
+        if overwrite or not self._static_done:
+            self._write(path, self.static_fields, None)
+            self._static_done = True
         return path
+
+    def append(self, grid: Any, *, time: Any) -> Path:
+        if not self.dynamic_fields:
+            raise ValueError("this writer has no dynamic fields to append")
+        return self.append_dynamic(grid, time=time)
 
     def append_dynamic(self, grid: Any, *, time: Any) -> Path:
-        path = self.write_step(self._step_idx, time, self._payload(self.dynamic_fields))
-        self._step_idx += 1
-        return path
+        step = self._next_step
+        self._next_step += 1
+        payload = {k: ([_field_values(c) for c in v] if isinstance(v, (list, tuple)) else _field_values(v))
+                   for k, v in self.dynamic_fields.items()}
+        return self.write_step(step, time, payload)
 
     def write_step(self, step_idx: int, time_value: Any, data: Mapping[str, Any]) -> Path:
-        """``data`` to ``{base}_{step_idx:04d}.vti``, recorded for the manifest."""
         name = f"{self.base}_{step_idx:04d}.vti"
-        write_vti(self.out_dir / name, data, self.dx, dy=self.dy, dz=self.dz, origin=self.origin,
-                  time_value=time_value, flip_y=self.flip_y)
+        path = write_vti(self.out_dir / name, data, self.dx, dy=self.dy, dz=self.dz, origin=self.origin,
+                         time_value=time_value, flip_y=self.flip_y)
         self.records.append((float(time_value), name))
-        return self.out_dir / name
-
-    def write_pvd(self, pvd_name: str | None = None) -> Path:
-        """The .pvd collection of the written steps."""
-        pvd_name = pvd_name or f"{self.base}.pvd"
-        root = ET.Element("VTKFile", type="Collection", version="0.1", byte_order="LittleEndian")
-        coll = ET.SubElement(root, "Collection")
-        for t, name in self.records:
-            ET.SubElement(coll, "DataSet", timestep=str(t), group="", part="0", file=str(name))
-        path = self.out_dir / pvd_name
-        path.write_text(_pretty_xml(root))
         return path
 
-    def _payload(self, fields: Mapping[str, Any]) -> dict[str, Any]:
-        return {name: ([self._array(c) for c in v] if isinstance(v, (list, tuple)) else self._array(v))
-                for name, v in fields.items()}
-
-    @staticmethod
-    def _array(value: Any) -> Any:
-        if callable(value) and not hasattr(value, "data"):
-            value = value()
-        if isinstance(value, np.ndarray) or hasattr(value, "detach"):
-            return value
-        if hasattr(value, "to_cell"):
-            return value.to_cell().data
-        if hasattr(value, "data"):
-            return value.data
-        return value
+    def write_pvd(self, pvd_name: str | None = None) -> Path:
+        lines = ['<?xml version="1.0" ?>', '<VTKFile type="Collection" version="0.1" byte_order="LittleEndian">',
+                 "  <Collection>"]
+        lines += [f'    <DataSet timestep={_q(t)} group="" part="0" file={_q(name)}/>' for t, name in self.records]
+        lines += ["  </Collection>", "</VTKFile>", ""]
+        path = self.out_dir / (pvd_name or f"{self.base}.pvd")
+        path.write_text("\n".join(lines))
+        return path

@@ -8,7 +8,7 @@ passes and transfers are enqueued from C++ on torch's current stream).
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass, fields
+from dataclasses import dataclass
 
 import torch
 
@@ -361,15 +361,18 @@ def _no_report(cfg, report_norms) -> bool:
 
 
 def dataclass_field_names(cls):
-    return [f.name for f in fields(cls)]
+    """Option names of a configuration dataclass, in declaration order."""
+    return list(cls.__dataclass_fields__)
 
 
 def validate_kwargs(config_cls, kwargs):
-    valid = set(dataclass_field_names(config_cls))
-    unknown = set(kwargs) - valid
-    if unknown:
-        raise AttributeError(f"Unknown {config_cls.__name__} option(s): {sorted(unknown)}. "
-                             f"Valid options are: {sorted(valid)}")
+    """Refuse option names the configuration does not define (AttributeError,
+    as the reference's option setters raise)."""
+    known = config_cls.__dataclass_fields__
+    bad = [k for k in kwargs if k not in known]
+    if bad:
+        raise AttributeError(f"{config_cls.__name__} has no option {', '.join(map(repr, bad))} "
+                             f"(options: {', '.join(known)})")
 
 
 class _ConfigOptions:
