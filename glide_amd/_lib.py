@@ -162,6 +162,8 @@ SIGNATURES = {
     "glide_enth_apply": (_I, [_CP, _D, _P]),
     "glide_enth_column_sweep": (_I, [_CP, _D, _I, ctypes.POINTER(EnthSweepConfigC), ctypes.POINTER(EnthReport),
                                      _P]),
+    "glide_enth_column_sweep_noalloc": (_I, [_CP, _D, _I, ctypes.POINTER(EnthSweepConfigC),
+                                             ctypes.POINTER(EnthReport), _P]),
     "glide_enth_omega": (_I, [_CP, _P, _P, _D, _P]),
     "glide_enth_broadcast_velocity": (_I, [_CP, _P, _P, _D, _P]),
     "glide_enth_frictional_heating": (_I, [_CP, _P, _D, _D, _P]),

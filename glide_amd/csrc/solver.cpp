@@ -801,6 +801,7 @@ int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, 
 
 namespace glide {
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+int noalloc_setup();
 }  // namespace glide
 
 // ====================================================================== C ABI
@@ -811,7 +812,8 @@ const char* glide_version(void) { return "glide-mi355x 0.2.0 (gfx950, FP64 and m
 int glide_init(void) {
     DeviceScratch* s = nullptr;
     GLIDE_TRY(scratch_for_current_device(&s));
-    return glide_enth_init();
+    GLIDE_TRY(glide_enth_init());
+    return noalloc_setup();
 }
 
 int glide_check_status(void* stream) {

@@ -121,6 +121,18 @@ int glide_enth_apply(const glide_enth_column* C, double omega, void* stream);
  * Uses r_E and E_work (alternating) or delta_E (not alternating). */
 int glide_enth_column_sweep(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
                             glide_enth_report* report, void* stream);
+/* glide_enth_column_sweep without any device allocation.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc).  Its reduction partials and sweep record live in a file-scope
+ * device array of GLIDE_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends; n_iter beyond the record's GLIDE_ENTH_MAX_SWEEPS norm slots is
+ * rejected.  Every call on one GPU shares that array (calls that overlap on
+ * one GPU race); the results equal glide_enth_column_sweep's.  The report,
+ * when requested, is read through a synchronous copy into `report`. */
+#define GLIDE_NOALLOC_SCRATCH_BYTES 20480
+int glide_enth_column_sweep_noalloc(const glide_enth_column* C, double dt, int n_iter,
+                                    const glide_enth_sweep_config* cfg, glide_enth_report* report, void* stream);
 /* omega = H sigma_dot from dh_dt and the layer velocities (same units), with
  * basal melt bmb (NULL = 0), divided by divide_by. */
 int glide_enth_omega(const glide_enth_column* C, const void* dh_dt, const void* bmb, double divide_by, void* stream);
