@@ -52,7 +52,7 @@ class Level(ctypes.Structure):
 class VankaConfigC(ctypes.Structure):
     _fields_ = [("omega", ctypes.c_double), ("relax_phi", ctypes.c_double), ("newton_steps", ctypes.c_int32),
                 ("relaxation", ctypes.c_double), ("ssa_damping", ctypes.c_double), ("mc_damping", ctypes.c_double),
-                ("local_fp32", ctypes.c_int32)]
+                ("local_fp32", ctypes.c_int32), ("multicolour", ctypes.c_int32), ("newton_tol", ctypes.c_double)]
 
 
 class FasConfigC(ctypes.Structure):
@@ -61,6 +61,12 @@ class FasConfigC(ctypes.Structure):
                 ("finest_steps", ctypes.c_int32), ("maximum_vcycles", ctypes.c_int32),
                 ("relative_tolerance", ctypes.c_double), ("absolute_tolerance", ctypes.c_double),
                 ("compat_vprolong", ctypes.c_int32)]
+
+
+class MixedConfigC(ctypes.Structure):
+    _fields_ = [("max_outer", ctypes.c_int32), ("inner_vcycles", ctypes.c_int32),
+                ("relative_tolerance", ctypes.c_double), ("absolute_tolerance", ctypes.c_double),
+                ("fp64_coarsest", ctypes.c_int32)]
 
 
 class SolveReport(ctypes.Structure):
@@ -135,6 +141,12 @@ SIGNATURES = {
     "glide_fascd_solve": (_I, [_LP, _I, _I, _D, ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC),
                                ctypes.POINTER(SolveReport), _P]),
     "glide_fascd_vcycle": (_I, [_LP, _I, _I, _I, _D, ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC), _P]),
+    "glide_fascd_solve_mixed": (_I, [_LP, _LP, _I, _I, _D, ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC),
+                                     ctypes.POINTER(MixedConfigC), ctypes.POINTER(SolveReport),
+                                     ctypes.POINTER(_I), _P]),
+    "glide_fas_adjoint_solve_mixed": (_I, [_LP, _LP, _I, _I, _D, ctypes.POINTER(FasConfigC),
+                                           ctypes.POINTER(VankaConfigC), ctypes.POINTER(MixedConfigC),
+                                           ctypes.POINTER(SolveReport), ctypes.POINTER(_I), _P]),
     "glide_ssa_vjp": (_I, [_LP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
     "glide_vanka_adjoint_sweeps": (_I, [_LP, _D, _I, ctypes.POINTER(VankaConfigC), _I, _P]),
     "glide_fas_adjoint_solve": (_I, [_LP, _I, _I, _D, ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC),
@@ -317,17 +329,22 @@ def call(name: str, *args) -> None:
     check(fn(*args), name)
 
 
-def ptr(t: torch.Tensor | None) -> int | None:
-    """Device address of a contiguous ROCm tensor (None stays NULL)."""
+def ptr(t: torch.Tensor | None, dtype: torch.dtype = torch.float64) -> int | None:
+    """Device address of a contiguous ROCm tensor of `dtype` (None stays NULL);
+    float64 everywhere except the FP32 buffers of the mixed-precision solves."""
     if t is None:
         return None
     if not t.is_cuda:
         raise GlideError("glide_amd kernels need tensors on a ROCm device (got a CPU tensor)")
     if not t.is_contiguous():
         raise GlideError("glide_amd kernels need contiguous tensors")
-    if t.dtype != torch.float64:
-        raise GlideError(f"glide_amd kernels compute in float64 (got {t.dtype})")
+    if t.dtype != dtype:
+        raise GlideError(f"this glide_amd buffer must be {dtype} (got {t.dtype})")
     return t.data_ptr()
+
+
+def ptr32(t: torch.Tensor | None) -> int | None:
+    return ptr(t, torch.float32)
 
 
 def stream_handle(device: torch.device | None = None) -> int:

@@ -16,7 +16,8 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libglide_hip.so"
 ARCH = os.environ.get("GLIDE_OFFLOAD_ARCH", "gfx950")
 
-DEVICE_SOURCES = ["ssa_kernels.hip", "transfer_kernels.hip", "enthalpy_kernels.hip", "noalloc.hip"]
+DEVICE_SOURCES = ["ssa_kernels.hip", "mc_kernels.hip", "transfer_kernels.hip", "enthalpy_kernels.hip",
+                  "noalloc.hip"]
 HOST_SOURCES = ["solver.cpp", "enthalpy_abi.cpp", "timing.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall"]

@@ -214,6 +214,29 @@ int persistent_ctl(const glide_level& L, PersistentCtl* ctl) {
     return GLIDE_OK;
 }
 
+// Multicolour passes (glide_vanka_config.multicolour): per pass the cell
+// viscosity of the current state into the workspace (behind the alternate
+// state), then the red phase into the alternate buffers and the black phase
+// back into the level's own; the grounded fraction is refreshed before each
+// pass unless frozen.
+int vanka_sweeps_mc(const glide_level& L, double dt, int n_iter, const VankaArgs<R>& vk, double relax_phi,
+                    bool freeze_phi, hipStream_t st) {
+    R* alt_u = P(L.work);
+    R* alt_v = alt_u + nU(L);
+    R* alt_H = alt_v + nV(L);
+    R* eta = alt_H + nC(L);
+    for (int it = 0; it < n_iter; ++it) {
+        if (!freeze_phi) GLIDE_TRY(grounded(L, relax_phi, st));
+        SsaArgs<R> a = make_args(L, dt, false);
+        HIP_TRY(launch_cell_eta<R>(a, eta, st), "cell viscosity");
+        HIP_TRY(launch_vanka_mc<R>(a, eta, alt_u, alt_v, alt_H, P(L.mask), vk, 0, st), "vanka red phase");
+        SsaArgs<R> b = a;
+        b.u = alt_u; b.v = alt_v; b.H = alt_H;
+        HIP_TRY(launch_vanka_mc<R>(b, eta, P(L.u), P(L.v), P(L.H), P(L.mask), vk, 1, st), "vanka black phase");
+    }
+    return GLIDE_OK;
+}
+
 int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
                  hipStream_t st) {
     const bool freeze_phi = (flags & GLIDE_FREEZE_PHI) != 0;
@@ -224,6 +247,9 @@ int vanka_sweeps(const glide_level& L, double dt, int n_iter, const glide_vanka_
     vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
     vk.iters = nullptr;
     vk.local_fp32 = vc.local_fp32;
+    vk.relax = vc.relaxation;
+    vk.tol = vc.newton_tol;
+    if (vc.multicolour) return vanka_sweeps_mc(L, dt, n_iter, vk, vc.relax_phi, freeze_phi, st);
     // small levels with a frozen grounded fraction: one fused launch per pass,
     // the state alternating between the level's buffers and the workspace
     // (5 planes hold u, v and H); an odd pass count copies it back at the end
@@ -294,6 +320,8 @@ VankaArgs<R> adjoint_vanka_args(const glide_level& L, const glide_vanka_config& 
     vk.calving_rate = (flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate;
     vk.iters = nullptr;
     vk.local_fp32 = 0;
+    vk.relax = 1.0;
+    vk.tol = 0.0;
     return vk;
 }
 
@@ -616,6 +644,8 @@ int check_hierarchy(const glide_level* levels, int n_levels, int start, const gl
         if (cfg->compat_vprolong && n_levels - start > 1 && levels[l].ny != levels[l].nx)
             return fail(GLIDE_ERR_SHAPE, "compat_vprolong requires square levels");
         if (vc[l].newton_steps < 0) return fail(GLIDE_ERR_INVALID_ARG, "newton_steps must be >= 0");
+        if (vc[l].multicolour && (long)levels[l].ny * levels[l].nx < (long)levels[l].ny + levels[l].nx)
+            return fail(GLIDE_ERR_SHAPE, "multicolour smoothing needs ny * nx >= ny + nx on every level");
     }
     return GLIDE_OK;
 }
@@ -796,6 +826,574 @@ int solve_loop(FasCtx& X, int start, glide_solve_report* rep, Residual&& resid, 
     }
     return GLIDE_OK;
 }
+
+
+// ====================================================== mixed precision
+// The mixed-precision form (no counterpart in the reference): an FP64
+// defect-correction loop around FP32 FAS-CD V-cycles.  Per outer iteration:
+// the FP64 residual r(x) of the finest level, the FP32 copy x32 = fl(x), the
+// FP32 forcing f32 = F32(x32) - fl(r(x)), FP32 V-cycles with the red-black
+// smoother (the coarsest level in FP64) from x32, and the FP64 update
+// x += y32 - fl(x).  The FP32 operator's own rounding cancels in the
+// correction, so the loop converges to the FP64 solution; the FP64 residual
+// decides when to stop, and the FP64 state owns the active set (masked cells
+// are set to the thickness limit exactly).  An outer iteration whose FP64
+// residual grows is undone and replaced by one FP64 V-cycle of the reference
+// form (additive smoother, default counts).
+
+template <typename T>
+inline T* PT(void* p) { return static_cast<T*>(p); }
+template <typename T>
+inline const T* CPT(const void* p) { return static_cast<const T*>(p); }
+
+template <typename T>
+SsaArgs<T> make_args_t(const glide_level& L, double dt, bool adjoint_forcing) {
+    const Phys<R> q = make_phys(L.phys);
+    SsaArgs<T> a;
+    a.ny = L.ny; a.nx = L.nx; a.dxi = T(1.0 / L.dx); a.dt = T(dt);
+    a.P.n = T(q.n); a.P.eps_reg = T(q.eps_reg); a.P.frd = T(q.frd); a.P.m = T(q.m); a.P.u_reg = T(q.u_reg);
+    a.P.water_drag = T(q.water_drag); a.P.frs = T(q.frs); a.P.calving_rate = T(q.calving_rate); a.P.frc = T(q.frc);
+    a.P.thklim = T(q.thklim); a.P.glen_p = T(q.glen_p); a.P.drag_q = T(q.drag_q);
+    a.P.linear_drag = q.linear_drag; a.P.glen_cube = q.glen_cube;
+    a.u = CPT<T>(L.u); a.v = CPT<T>(L.v); a.H = CPT<T>(L.H); a.phi = CPT<T>(L.phi); a.mask = CPT<T>(L.mask);
+    a.bed = CPT<T>(L.bed); a.B = CPT<T>(L.B); a.beta = CPT<T>(L.beta); a.gamma = CPT<T>(L.gamma);
+    a.f_u = CPT<T>(adjoint_forcing ? L.af_u : L.f_u);
+    a.f_v = CPT<T>(adjoint_forcing ? L.af_v : L.f_v);
+    a.f_H = CPT<T>(adjoint_forcing ? L.af_H : L.f_H);
+    return a;
+}
+
+template <typename T>
+int residual_t(const glide_level& L, double dt, int flags, T* ou, T* ov, T* oH, hipStream_t st) {
+    SsaArgs<T> a = make_args_t<T>(L, dt, false);
+    const T cr = T((flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate);
+    HIP_TRY(launch_residual<T>(a, ou, ov, oH, (flags & GLIDE_USE_FORCING) ? 1 : 0, (flags & GLIDE_USE_MASK) ? 1 : 0,
+                               cr, st),
+            "residual");
+    return GLIDE_OK;
+}
+
+// Multicolour passes on a level of element type T (see vanka_sweeps_mc).
+template <typename T>
+int mc_sweeps_t(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, int flags,
+                hipStream_t st) {
+    const bool freeze_phi = (flags & GLIDE_FREEZE_PHI) != 0;
+    VankaArgs<T> vk;
+    vk.newton_steps = vc.newton_steps;
+    vk.ssa_damping = T(vc.ssa_damping);
+    vk.mc_damping = T(vc.mc_damping);
+    vk.calving_rate = T((flags & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate);
+    vk.iters = nullptr;
+    vk.local_fp32 = 0;
+    vk.relax = T(vc.relaxation);
+    vk.tol = T(vc.newton_tol);
+    T* alt_u = PT<T>(L.work);
+    T* alt_v = alt_u + nU(L);
+    T* alt_H = alt_v + nV(L);
+    T* eta = alt_H + nC(L);
+    for (int it = 0; it < n_iter; ++it) {
+        if (!freeze_phi)
+            HIP_TRY(launch_grounded<T>(PT<T>(L.phi), CPT<T>(L.H), CPT<T>(L.bed), T(L.phys.flotation_reg_driving),
+                                       T(vc.relax_phi), (long)nC(L), st),
+                    "grounded");
+        SsaArgs<T> a = make_args_t<T>(L, dt, false);
+        HIP_TRY(launch_cell_eta<T>(a, eta, st), "cell viscosity");
+        HIP_TRY(launch_vanka_mc<T>(a, eta, alt_u, alt_v, alt_H, PT<T>(L.mask), vk, 0, st), "vanka red phase");
+        SsaArgs<T> b = a;
+        b.u = alt_u; b.v = alt_v; b.H = alt_H;
+        HIP_TRY(launch_vanka_mc<T>(b, eta, PT<T>(L.u), PT<T>(L.v), PT<T>(L.H), PT<T>(L.mask), vk, 1, st),
+                "vanka black phase");
+    }
+    return GLIDE_OK;
+}
+
+struct MixedCtx {
+    const glide_level* L64;  // FP64 hierarchy (the coarsest level's passes, the outer residual)
+    const glide_level* L32;  // FP32 hierarchy of the inner V-cycles
+    int n_levels;
+    double dt;
+    glide_fas_config cfg;
+    const glide_vanka_config* vc;
+    bool fp64_coarsest;
+    hipStream_t st;
+};
+
+int run_mixed(const MixedBatch& b, hipStream_t st) {
+    HIP_TRY(launch_mixed_batch(b, st), "precision conversion");
+    return GLIDE_OK;
+}
+
+// The coarsest level's passes of an FP32 V-cycle, run on the FP64 copy of the
+// level: state, forcing and thickness bounds in, state and active set out.
+int coarsest_fp64(MixedCtx& X, int l, int n_iter, int flags) {
+    const glide_level& A = X.L32[l];
+    const glide_level& D = X.L64[l];
+    MixedBatch in;
+    in.add(kMixTo64, D.u, A.u, nullptr, (long)nU(A));
+    in.add(kMixTo64, D.v, A.v, nullptr, (long)nV(A));
+    in.add(kMixTo64, D.H, A.H, nullptr, (long)nC(A));
+    in.add(kMixTo64, D.mask, A.mask, nullptr, (long)nC(A));
+    in.add(kMixTo64, D.phi, A.phi, nullptr, (long)nC(A));
+    in.add(kMixTo64, D.f_u, A.f_u, nullptr, (long)nU(A));
+    in.add(kMixTo64, D.f_v, A.f_v, nullptr, (long)nV(A));
+    in.add(kMixTo64, D.f_H, A.f_H, nullptr, (long)nC(A));
+    in.add(kMixTo64, D.gamma, A.gamma, nullptr, (long)nC(A));
+    GLIDE_TRY(run_mixed(in, X.st));
+    GLIDE_TRY(mc_sweeps_t<double>(D, X.dt, n_iter, X.vc[l], flags, X.st));
+    MixedBatch out;
+    out.add(kMixTo32, A.u, D.u, nullptr, (long)nU(A));
+    out.add(kMixTo32, A.v, D.v, nullptr, (long)nV(A));
+    out.add(kMixTo32, A.H, D.H, nullptr, (long)nC(A));
+    out.add(kMixTo32, A.mask, D.mask, nullptr, (long)nC(A));
+    return run_mixed(out, X.st);
+}
+
+// FASCDSolver.vcycle (glide/multigrid.py:536-645) on the FP32 hierarchy with
+// the multicolour smoother; the same stages as fas_vcycle.
+int mixed_vcycle(MixedCtx& X, int l, bool finest) {
+    typedef float F;
+    const glide_level& L = X.L32[l];
+    const bool coarse = !finest;
+    int sflags = 0;
+    if (coarse && X.cfg.freeze_coarse_calving) sflags |= GLIDE_FREEZE_CALVING;
+    if (coarse && X.cfg.freeze_coarse_phi) sflags |= GLIDE_FREEZE_PHI;
+    const F thk = F(L.phys.thklim);
+    hipStream_t st = X.st;
+    auto run = [&](const EwBatch<F>& b) -> int {
+        HIP_TRY(launch_ew_batch<F>(b, st), "element-wise batch");
+        return GLIDE_OK;
+    };
+    auto smooth = [&](const glide_level& A, int n, int flags) -> int {
+        if (n <= 0) return GLIDE_OK;
+        return mc_sweeps_t<F>(A, X.dt, n, X.vc[l], flags, st);
+    };
+    if (finest) {
+        EwBatch<F> b;
+        b.copy(PT<F>(L.w_u), CPT<F>(L.u), (long)nU(L));
+        b.copy(PT<F>(L.w_v), CPT<F>(L.v), (long)nV(L));
+        b.copy(PT<F>(L.w_H), CPT<F>(L.H), (long)nC(L));
+        b.axpby(PT<F>(L.chi), F(-1), CPT<F>(L.H), F(0), CPT<F>(L.H), (long)nC(L));
+        b.fill(PT<F>(L.gamma), thk, (long)nC(L));
+        GLIDE_TRY(run(b));
+        EwBatch<F> c;
+        c.axpby(PT<F>(L.chi), F(1), CPT<F>(L.chi), F(1), CPT<F>(L.gamma), (long)nC(L));
+        GLIDE_TRY(run(c));
+    }
+    if (l == X.n_levels - 1) {
+        EwBatch<F> b;
+        b.axpby(PT<F>(L.gamma), F(1), CPT<F>(L.w_H), F(1), CPT<F>(L.chi), (long)nC(L));
+        GLIDE_TRY(run(b));
+        if (X.fp64_coarsest) GLIDE_TRY(coarsest_fp64(X, l, X.cfg.coarsest_steps, sflags));
+        else GLIDE_TRY(smooth(L, X.cfg.coarsest_steps, sflags));
+        EwBatch<F> c;
+        c.fill(PT<F>(L.gamma), thk, (long)nC(L));
+        return run(c);
+    }
+    const glide_level& N = X.L32[l + 1];
+    HIP_TRY(launch_restrict_cell<F>(CPT<F>(L.chi), PT<F>(N.chi), N.ny, N.nx, 1, st), "restrict chi");
+    HIP_TRY(launch_prolong_cell<F>(CPT<F>(N.chi), PT<F>(L.cphi), L.ny, L.nx, 0, st), "prolong chi");
+    {
+        EwBatch<F> b;
+        b.axpby(PT<F>(L.cphi), F(-1), CPT<F>(L.cphi), F(1), CPT<F>(L.chi), (long)nC(L));
+        GLIDE_TRY(run(b));
+        EwBatch<F> c;
+        c.axpby(PT<F>(L.gamma), F(1), CPT<F>(L.w_H), F(1), CPT<F>(L.cphi), (long)nC(L));
+        GLIDE_TRY(run(c));
+    }
+    GLIDE_TRY(smooth(L, X.cfg.pre_steps, sflags));
+    {
+        EwBatch<F> b;
+        b.fill(PT<F>(L.gamma), thk, (long)nC(L));
+        b.axpby(PT<F>(L.y_u), F(1), CPT<F>(L.u), F(-1), CPT<F>(L.w_u), (long)nU(L));
+        b.axpby(PT<F>(L.y_v), F(1), CPT<F>(L.v), F(-1), CPT<F>(L.w_v), (long)nV(L));
+        b.axpby(PT<F>(L.y_H), F(1), CPT<F>(L.H), F(-1), CPT<F>(L.w_H), (long)nC(L));
+        GLIDE_TRY(run(b));
+    }
+    {
+        RestrictBatch<F> rb;
+        rb.add(CPT<F>(L.u), PT<F>(N.u), kRestrictU);
+        rb.add(CPT<F>(L.v), PT<F>(N.v), kRestrictV);
+        rb.add(CPT<F>(L.H), PT<F>(N.H), kRestrictCellMean);
+        rb.add(CPT<F>(L.phi), PT<F>(N.phi), kRestrictCellMean);
+        rb.add(CPT<F>(L.mask), PT<F>(N.mask), kRestrictCellMax);
+        HIP_TRY(launch_restrict_batch<F>(rb, N.ny, N.nx, st), "restrict state");
+    }
+    {
+        EwBatch<F> b;
+        b.copy(PT<F>(N.w_u), CPT<F>(N.u), (long)nU(N));
+        b.copy(PT<F>(N.w_v), CPT<F>(N.v), (long)nV(N));
+        b.copy(PT<F>(N.w_H), CPT<F>(N.H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
+    GLIDE_TRY(residual_t<F>(L, X.dt, GLIDE_USE_FORCING | (sflags & GLIDE_FREEZE_CALVING), PT<F>(L.r_u), PT<F>(L.r_v),
+                            PT<F>(L.r_H), st));
+    {
+        RestrictBatch<F> rb;
+        rb.add(CPT<F>(L.r_u), PT<F>(N.r_u), kRestrictU);
+        rb.add(CPT<F>(L.r_v), PT<F>(N.r_v), kRestrictV);
+        rb.add(CPT<F>(L.r_H), PT<F>(N.r_H), kRestrictCellMean);
+        HIP_TRY(launch_restrict_batch<F>(rb, N.ny, N.nx, st), "restrict residual");
+    }
+    if (!X.cfg.freeze_coarse_phi)
+        HIP_TRY(launch_grounded<F>(PT<F>(N.phi), CPT<F>(N.H), CPT<F>(N.bed), F(N.phys.flotation_reg_driving), F(0),
+                                   (long)nC(N), st),
+                "grounded");
+    GLIDE_TRY(residual_t<F>(N, X.dt, X.cfg.freeze_coarse_calving ? GLIDE_FREEZE_CALVING : 0, PT<F>(N.F_u),
+                            PT<F>(N.F_v), PT<F>(N.F_H), st));
+    {
+        EwBatch<F> b;
+        b.axpby(PT<F>(N.f_u), F(1), CPT<F>(N.F_u), F(-1), CPT<F>(N.r_u), (long)nU(N));
+        b.axpby(PT<F>(N.f_v), F(1), CPT<F>(N.F_v), F(-1), CPT<F>(N.r_v), (long)nV(N));
+        b.axpby(PT<F>(N.f_H), F(1), CPT<F>(N.F_H), F(-1), CPT<F>(N.r_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
+    GLIDE_TRY(mixed_vcycle(X, l + 1, false));
+    {
+        EwBatch<F> b;
+        b.axpby(PT<F>(N.z_u), F(1), CPT<F>(N.u), F(-1), CPT<F>(N.w_u), (long)nU(N));
+        b.axpby(PT<F>(N.z_v), F(1), CPT<F>(N.v), F(-1), CPT<F>(N.w_v), (long)nV(N));
+        b.axpby(PT<F>(N.z_H), F(1), CPT<F>(N.H), F(-1), CPT<F>(N.w_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
+    HIP_TRY(launch_prolong_vfacet_bilinear<F>(CPT<F>(N.z_u), PT<F>(L.z_u), L.ny, L.nx, st), "prolong z_u");
+    if (X.cfg.compat_vprolong)
+        HIP_TRY(launch_prolong_vfacet_bilinear<F>(CPT<F>(N.z_v), PT<F>(L.z_v), L.ny, L.nx, st), "prolong z_v");
+    else
+        HIP_TRY(launch_prolong_hfacet_bilinear<F>(CPT<F>(N.z_v), PT<F>(L.z_v), L.ny, L.nx, st), "prolong z_v");
+    HIP_TRY(launch_prolong_cell<F>(CPT<F>(N.z_H), PT<F>(L.z_H), L.ny, L.nx, 0, st), "prolong z_H");
+    {
+        EwBatch<F> b;
+        b.correct(PT<F>(L.z_u), CPT<F>(L.y_u), PT<F>(L.u), CPT<F>(L.w_u), (long)nU(L));
+        b.correct(PT<F>(L.z_v), CPT<F>(L.y_v), PT<F>(L.v), CPT<F>(L.w_v), (long)nV(L));
+        b.correct(PT<F>(L.z_H), CPT<F>(L.y_H), PT<F>(L.H), CPT<F>(L.w_H), (long)nC(L));
+        b.axpby(PT<F>(L.gamma), F(1), CPT<F>(L.w_H), F(1), CPT<F>(L.chi), (long)nC(L));
+        GLIDE_TRY(run(b));
+    }
+    GLIDE_TRY(smooth(L, X.cfg.post_steps, sflags));
+    {
+        EwBatch<F> b;
+        b.fill(PT<F>(L.gamma), thk, (long)nC(L));
+        GLIDE_TRY(run(b));
+    }
+    if (finest) GLIDE_TRY(smooth(L, X.cfg.finest_steps, 0));
+    return GLIDE_OK;
+}
+
+int check_level32(const glide_level& L, bool adjoint) {
+    REQ(L.u); REQ(L.v); REQ(L.H); REQ(L.phi); REQ(L.mask); REQ(L.bed); REQ(L.B); REQ(L.beta); REQ(L.work);
+    if (!adjoint) {
+        REQ(L.f_u); REQ(L.f_v); REQ(L.f_H); REQ(L.r_u); REQ(L.r_v); REQ(L.r_H); REQ(L.F_u); REQ(L.F_v); REQ(L.F_H);
+        REQ(L.gamma); REQ(L.w_u); REQ(L.w_v); REQ(L.w_H); REQ(L.y_u); REQ(L.y_v); REQ(L.y_H); REQ(L.z_u); REQ(L.z_v);
+        REQ(L.z_H); REQ(L.chi); REQ(L.cphi);
+    } else {
+        REQ(L.lambda_u); REQ(L.lambda_v); REQ(L.lambda_H); REQ(L.af_u); REQ(L.af_v); REQ(L.af_H); REQ(L.ar_u);
+        REQ(L.ar_v); REQ(L.ar_H); REQ(L.vjp_u); REQ(L.vjp_v); REQ(L.vjp_H); REQ(L.wl_u); REQ(L.wl_v); REQ(L.wl_H);
+        REQ(L.zl_u); REQ(L.zl_v); REQ(L.zl_H); REQ(L.adj_cache);
+    }
+    return GLIDE_OK;
+}
+
+int check_mixed(const glide_level* levels64, const glide_level* levels32, int n_levels, int start,
+                const glide_fas_config* cfg, const glide_vanka_config* vc, const glide_mixed_config* mc,
+                bool adjoint) {
+    GLIDE_TRY(check_hierarchy(levels64, n_levels, start, cfg, vc));
+    if (!levels32 || !mc) return fail(GLIDE_ERR_INVALID_ARG, "levels32 and the mixed config must be non-NULL");
+    if (mc->max_outer < 0 || mc->max_outer > 64) return fail(GLIDE_ERR_INVALID_ARG, "max_outer must be in [0, 64]");
+    if (mc->inner_vcycles < 1) return fail(GLIDE_ERR_INVALID_ARG, "inner_vcycles must be >= 1");
+    for (int l = start; l < n_levels; ++l) {
+        const glide_level& A = levels32[l];
+        if (A.ny != levels64[l].ny || A.nx != levels64[l].nx)
+            return fail(GLIDE_ERR_SHAPE, "the FP32 and FP64 hierarchies must have the same level shapes");
+        GLIDE_TRY(check_level32(A, adjoint));
+        if (!vc[l].multicolour) return fail(GLIDE_ERR_INVALID_ARG, "the mixed form runs multicolour smoothing only");
+        if ((long)A.ny * A.nx < (long)A.ny + A.nx)
+            return fail(GLIDE_ERR_SHAPE, "multicolour smoothing needs ny * nx >= ny + nx on every level");
+    }
+    return GLIDE_OK;
+}
+
+// The reference form's smoother and cycle counts (glide/operators.py:314-327,
+// glide/multigrid.py:671-682): the fallback when a mixed outer iteration fails.
+std::vector<glide_vanka_config> reference_vanka_configs(int n) {
+    glide_vanka_config v{};
+    v.omega = 0.5; v.relax_phi = 0.0; v.newton_steps = 30; v.relaxation = 0.5;
+    v.ssa_damping = 0.01; v.mc_damping = 1.0; v.local_fp32 = 0; v.multicolour = 0; v.newton_tol = 1e-6;
+    return std::vector<glide_vanka_config>((size_t)n, v);
+}
+
+// The outer iteration of both mixed solves: `resid` enqueues the FP64
+// residual norm into a slot, `step(stream, phase)` one outer correction
+// (phase 1; with two_phase, phase 2 is an FP64 V-cycle of the same form), `undo`
+// restores the state saved by `step`, `fallback` runs one reference-form
+// cycle.  A phase-1 iteration that reduces the norm by less than a factor
+// kPhaseSwitch, or fails, moves the loop to phase 2 (a failed iteration is
+// undone and redone in phase 2); an iteration that fails in the last phase is
+// undone and replaced by the fallback.  stats (optional): fallbacks, phase-1
+// iterations, phase-2 iterations.
+constexpr double kPhaseSwitch = 0.6;
+
+template <typename Resid, typename Step, typename Undo, typename Fallback, typename Key>
+int mixed_outer_loop(const glide_mixed_config& mc, hipStream_t st, Key&& key, bool two_phase,
+                     glide_solve_report* rep, int* stats, Resid&& resid, Step&& step, Undo&& undo,
+                     Fallback&& fallback) {
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    auto read_norm = [&](double* n3) -> int {
+        HIP_TRY(hipMemcpyAsync(s->host, s->norms, sizeof(double) * 3, hipMemcpyDeviceToHost, st), "norms D2H");
+        GLIDE_TRY(latch_enqueue(s, st));
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        GLIDE_TRY(latch_check(s));
+        n3[0] = s->host[0]; n3[1] = s->host[1]; n3[2] = s->host[2];
+        return GLIDE_OK;
+    };
+    auto attempt = [&](int phase, double* m3) -> int {
+        auto body = [&](hipStream_t cs) -> int {
+            GLIDE_TRY(step(cs, phase));
+            return resid(cs, s->norms);
+        };
+        if (g_graphs_on && !timing_on()) GLIDE_TRY(replay(key(phase), st, body));
+        else GLIDE_TRY(body(st));
+        return read_norm(m3);
+    };
+    double n3[3];
+    GLIDE_TRY(resid(st, s->norms));
+    GLIDE_TRY(read_norm(n3));
+    const double r0 = std::sqrt(n3[0] + n3[1] + n3[2]);
+    if (!std::isfinite(r0)) return fail(GLIDE_ERR_NONFINITE, "non-finite residual norm at the start of the mixed solve");
+    double r = r0;
+    int it = 0, nfall = 0, n1 = 0, n2 = 0;
+    int phase = 1;
+    const int last_phase = two_phase ? 2 : 1;
+    while (it < mc.max_outer && r > mc.absolute_tolerance && (r0 <= 0.0 || r / r0 > mc.relative_tolerance)) {
+        double m3[3];
+        GLIDE_TRY(attempt(phase, m3));
+        (phase == 1 ? n1 : n2)++;
+        double rn = std::sqrt(m3[0] + m3[1] + m3[2]);
+        if (!(rn < r) && phase < last_phase) {  // failed before the last phase: redo in the next one
+            GLIDE_TRY(undo(st));
+            phase = last_phase;
+            GLIDE_TRY(attempt(phase, m3));
+            ++n2;
+            rn = std::sqrt(m3[0] + m3[1] + m3[2]);
+        }
+        if (!(rn < r)) {  // failed in the last phase: undo, one reference-form cycle instead
+            GLIDE_TRY(undo(st));
+            GLIDE_TRY(fallback(st));
+            GLIDE_TRY(resid(st, s->norms));
+            GLIDE_TRY(read_norm(m3));
+            ++nfall;
+        } else if (phase < last_phase && rn > kPhaseSwitch * r) {
+            phase = last_phase;
+        }
+        std::memcpy(n3, m3, sizeof(n3));
+        r = std::sqrt(n3[0] + n3[1] + n3[2]);
+        if (!std::isfinite(r)) return fail(GLIDE_ERR_NONFINITE, "non-finite residual norm in the mixed solve");
+        if (rep && it < 64) rep->norms[it] = r;
+        ++it;
+    }
+    if (rep) {
+        rep->vcycles = it;
+        rep->initial_norm = r0;
+        rep->final_norm = r;
+        rep->norms3[0] = std::sqrt(n3[0]); rep->norms3[1] = std::sqrt(n3[1]); rep->norms3[2] = std::sqrt(n3[2]);
+    }
+    if (stats) {
+        stats[0] = nfall;
+        stats[1] = n1;
+        stats[2] = n2;
+    }
+    return GLIDE_OK;
+}
+
+std::string mixed_key(char tag, const glide_level* l64, const glide_level* l32, int n_levels, int start, double dt,
+                      const glide_fas_config& cfg, const glide_vanka_config* vc, const glide_mixed_config& mc) {
+    std::string k(1, tag);
+    key_append(k, n_levels);
+    key_append(k, start);
+    key_append(k, dt);
+    key_append(k, cfg);
+    key_append(k, mc.inner_vcycles);
+    key_append(k, mc.fp64_coarsest);
+    key_append(k, g_spin_limit);
+    key_append(k, g_persistent_fenced);
+    for (int l = start; l < n_levels; ++l) {
+        key_append(k, l64[l]);
+        key_append(k, l32[l]);
+        key_append(k, vc[l]);
+    }
+    return k;
+}
+
+
+// Adjoint side of the mixed form: the multicolour adjoint passes on a level of
+// element type T (linear drag), with the level's cache prepared for its state.
+template <typename T>
+VankaArgs<T> adjoint_args_t(const glide_level& L, const glide_vanka_config& vc) {
+    VankaArgs<T> vk;
+    vk.newton_steps = 1;
+    vk.ssa_damping = T(vc.ssa_damping);
+    vk.mc_damping = T(vc.mc_damping);
+    vk.calving_rate = T(L.phys.calving_rate);
+    vk.iters = nullptr;
+    vk.local_fp32 = 0;
+    vk.relax = T(1);
+    vk.tol = T(0);
+    return vk;
+}
+
+template <typename T>
+int adjoint_prepare_t(const glide_level& L, double dt, const glide_vanka_config& vc, hipStream_t st) {
+    SsaArgs<T> a = make_args_t<T>(L, dt, true);
+    HIP_TRY(launch_adjoint_factor<T>(a, PT<T>(L.adj_cache), adjoint_args_t<T>(L, vc), st), "adjoint factor");
+    return GLIDE_OK;
+}
+
+template <typename T>
+int mc_adjoint_sweeps_t(const glide_level& L, double dt, int n_iter, const glide_vanka_config& vc, hipStream_t st) {
+    SsaArgs<T> a = make_args_t<T>(L, dt, true);
+    if (!a.P.linear_drag) return fail(GLIDE_ERR_INVALID_ARG, "the multicolour adjoint passes need linear drag");
+    T* alt_u = PT<T>(L.work);
+    T* alt_v = alt_u + nU(L);
+    T* alt_H = alt_v + nV(L);
+    const T cr = T(L.phys.calving_rate);
+    for (int it = 0; it < n_iter; ++it) {
+        HIP_TRY(launch_adjoint_pass_mc<T>(a, CPT<T>(L.adj_cache), CPT<T>(L.lambda_u), CPT<T>(L.lambda_v),
+                                          CPT<T>(L.lambda_H), alt_u, alt_v, alt_H, CPT<T>(L.af_u), CPT<T>(L.af_v),
+                                          CPT<T>(L.af_H), cr, T(vc.omega), 0, st),
+                "adjoint red phase");
+        HIP_TRY(launch_adjoint_pass_mc<T>(a, CPT<T>(L.adj_cache), alt_u, alt_v, alt_H, PT<T>(L.lambda_u),
+                                          PT<T>(L.lambda_v), PT<T>(L.lambda_H), CPT<T>(L.af_u), CPT<T>(L.af_v),
+                                          CPT<T>(L.af_H), cr, T(vc.omega), 1, st),
+                "adjoint black phase");
+    }
+    return GLIDE_OK;
+}
+
+template <typename T>
+int vjp_t(const glide_level& L, double dt, bool forcing, const void* lu, const void* lv, const void* lH, void* ou,
+          void* ov, void* oH, bool cached, hipStream_t st) {
+    SsaArgs<T> a = make_args_t<T>(L, dt, true);
+    HIP_TRY(launch_vjp<T>(a, CPT<T>(lu), CPT<T>(lv), CPT<T>(lH), forcing ? CPT<T>(L.af_u) : nullptr,
+                          forcing ? CPT<T>(L.af_v) : nullptr, forcing ? CPT<T>(L.af_H) : nullptr, PT<T>(ou), PT<T>(ov),
+                          PT<T>(oH), T(L.phys.calving_rate),
+                          cached ? CPT<T>(L.adj_cache) + (size_t)kAdjointCacheEta * nC(L) : nullptr, st),
+            "vjp");
+    return GLIDE_OK;
+}
+
+// The coarsest level's adjoint passes of an FP32 V-cycle on its FP64 copy.
+int coarsest_adjoint_fp64(MixedCtx& X, int l, int n_iter) {
+    const glide_level& A = X.L32[l];
+    const glide_level& D = X.L64[l];
+    MixedBatch in;
+    in.add(kMixTo64, D.u, A.u, nullptr, (long)nU(A));
+    in.add(kMixTo64, D.v, A.v, nullptr, (long)nV(A));
+    in.add(kMixTo64, D.H, A.H, nullptr, (long)nC(A));
+    in.add(kMixTo64, D.phi, A.phi, nullptr, (long)nC(A));
+    in.add(kMixTo64, D.mask, A.mask, nullptr, (long)nC(A));
+    in.add(kMixTo64, D.lambda_u, A.lambda_u, nullptr, (long)nU(A));
+    in.add(kMixTo64, D.lambda_v, A.lambda_v, nullptr, (long)nV(A));
+    in.add(kMixTo64, D.lambda_H, A.lambda_H, nullptr, (long)nC(A));
+    in.add(kMixTo64, D.af_u, A.af_u, nullptr, (long)nU(A));
+    in.add(kMixTo64, D.af_v, A.af_v, nullptr, (long)nV(A));
+    in.add(kMixTo64, D.af_H, A.af_H, nullptr, (long)nC(A));
+    GLIDE_TRY(run_mixed(in, X.st));
+    GLIDE_TRY(adjoint_prepare_t<double>(D, X.dt, X.vc[l], X.st));
+    GLIDE_TRY(mc_adjoint_sweeps_t<double>(D, X.dt, n_iter, X.vc[l], X.st));
+    MixedBatch out;
+    out.add(kMixTo32, A.lambda_u, D.lambda_u, nullptr, (long)nU(A));
+    out.add(kMixTo32, A.lambda_v, D.lambda_v, nullptr, (long)nV(A));
+    out.add(kMixTo32, A.lambda_H, D.lambda_H, nullptr, (long)nC(A));
+    return run_mixed(out, X.st);
+}
+
+// FASAdjointSolver.vcycle (glide/multigrid.py:905-973) on the FP32 hierarchy
+// with the multicolour adjoint passes; the same stages as adj_vcycle.  The
+// start level's cache is prepared by the caller.
+int mixed_adj_vcycle(MixedCtx& X, int l, bool finest) {
+    typedef float F;
+    const glide_level& L = X.L32[l];
+    hipStream_t st = X.st;
+    auto smooth = [&](int n) -> int { return n > 0 ? mc_adjoint_sweeps_t<F>(L, X.dt, n, X.vc[l], st) : GLIDE_OK; };
+    if (l == X.n_levels - 1) {
+        if (X.fp64_coarsest) return coarsest_adjoint_fp64(X, l, X.cfg.coarsest_steps);
+        return smooth(X.cfg.coarsest_steps);
+    }
+    const glide_level& N = X.L32[l + 1];
+    GLIDE_TRY(smooth(X.cfg.pre_steps));
+    {
+        RestrictBatch<F> rb;
+        rb.add(CPT<F>(L.u), PT<F>(N.u), kRestrictU);
+        rb.add(CPT<F>(L.v), PT<F>(N.v), kRestrictV);
+        rb.add(CPT<F>(L.H), PT<F>(N.H), kRestrictCellMean);
+        rb.add(CPT<F>(L.phi), PT<F>(N.phi), kRestrictCellMean);
+        rb.add(CPT<F>(L.mask), PT<F>(N.mask), kRestrictCellMax);
+        rb.add(CPT<F>(L.lambda_u), PT<F>(N.lambda_u), kRestrictU);
+        rb.add(CPT<F>(L.lambda_v), PT<F>(N.lambda_v), kRestrictV);
+        rb.add(CPT<F>(L.lambda_H), PT<F>(N.lambda_H), kRestrictCellMean);
+        HIP_TRY(launch_restrict_batch<F>(rb, N.ny, N.nx, st), "restrict state and multipliers");
+    }
+    HIP_TRY(launch_pin_rows<F>(PT<F>(N.lambda_u), PT<F>(N.lambda_v), PT<F>(N.lambda_H), CPT<F>(N.mask), N.ny, N.nx,
+                               st),
+            "pin rows");
+    if (!(X.fp64_coarsest && l + 1 == X.n_levels - 1)) GLIDE_TRY(adjoint_prepare_t<F>(N, X.dt, X.vc[l + 1], st));
+    auto run = [&](const EwBatch<F>& b) -> int {
+        HIP_TRY(launch_ew_batch<F>(b, st), "element-wise batch");
+        return GLIDE_OK;
+    };
+    {
+        EwBatch<F> b;
+        b.copy(PT<F>(N.wl_u), CPT<F>(N.lambda_u), (long)nU(N));
+        b.copy(PT<F>(N.wl_v), CPT<F>(N.lambda_v), (long)nV(N));
+        b.copy(PT<F>(N.wl_H), CPT<F>(N.lambda_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
+    GLIDE_TRY(vjp_t<F>(L, X.dt, true, L.lambda_u, L.lambda_v, L.lambda_H, L.ar_u, L.ar_v, L.ar_H, true, st));
+    {
+        RestrictBatch<F> rb;
+        rb.add(CPT<F>(L.ar_u), PT<F>(N.ar_u), kRestrictU);
+        rb.add(CPT<F>(L.ar_v), PT<F>(N.ar_v), kRestrictV);
+        rb.add(CPT<F>(L.ar_H), PT<F>(N.ar_H), kRestrictCellMean);
+        HIP_TRY(launch_restrict_batch<F>(rb, N.ny, N.nx, st), "restrict adjoint residual");
+    }
+    const bool n_cached = !(X.fp64_coarsest && l + 1 == X.n_levels - 1);
+    GLIDE_TRY(vjp_t<F>(N, X.dt, false, N.lambda_u, N.lambda_v, N.lambda_H, N.vjp_u, N.vjp_v, N.vjp_H, n_cached, st));
+    {
+        EwBatch<F> b;
+        b.axpby(PT<F>(N.af_u), F(1), CPT<F>(N.vjp_u), F(-1), CPT<F>(N.ar_u), (long)nU(N));
+        b.axpby(PT<F>(N.af_v), F(1), CPT<F>(N.vjp_v), F(-1), CPT<F>(N.ar_v), (long)nV(N));
+        b.axpby(PT<F>(N.af_H), F(1), CPT<F>(N.vjp_H), F(-1), CPT<F>(N.ar_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
+    GLIDE_TRY(mixed_adj_vcycle(X, l + 1, false));
+    {
+        EwBatch<F> b;
+        b.axpby(PT<F>(N.zl_u), F(1), CPT<F>(N.lambda_u), F(-1), CPT<F>(N.wl_u), (long)nU(N));
+        b.axpby(PT<F>(N.zl_v), F(1), CPT<F>(N.lambda_v), F(-1), CPT<F>(N.wl_v), (long)nV(N));
+        b.axpby(PT<F>(N.zl_H), F(1), CPT<F>(N.lambda_H), F(-1), CPT<F>(N.wl_H), (long)nC(N));
+        GLIDE_TRY(run(b));
+    }
+    HIP_TRY(launch_prolong_vfacet_bilinear<F>(CPT<F>(N.zl_u), PT<F>(L.zl_u), L.ny, L.nx, st), "prolong zl_u");
+    if (X.cfg.compat_vprolong)
+        HIP_TRY(launch_prolong_vfacet_bilinear<F>(CPT<F>(N.zl_v), PT<F>(L.zl_v), L.ny, L.nx, st), "prolong zl_v");
+    else
+        HIP_TRY(launch_prolong_hfacet_bilinear<F>(CPT<F>(N.zl_v), PT<F>(L.zl_v), L.ny, L.nx, st), "prolong zl_v");
+    HIP_TRY(launch_prolong_cell<F>(CPT<F>(N.zl_H), PT<F>(L.zl_H), L.ny, L.nx, 1, st), "prolong zl_H");
+    {
+        EwBatch<F> b;
+        b.axpby(PT<F>(L.lambda_u), F(1), CPT<F>(L.lambda_u), F(1), CPT<F>(L.zl_u), (long)nU(L));
+        b.axpby(PT<F>(L.lambda_v), F(1), CPT<F>(L.lambda_v), F(1), CPT<F>(L.zl_v), (long)nV(L));
+        b.axpby(PT<F>(L.lambda_H), F(1), CPT<F>(L.lambda_H), F(1), CPT<F>(L.zl_H), (long)nC(L));
+        GLIDE_TRY(run(b));
+    }
+    HIP_TRY(launch_pin_rows<F>(PT<F>(L.lambda_u), PT<F>(L.lambda_v), PT<F>(L.lambda_H), CPT<F>(L.mask), L.ny, L.nx,
+                               st),
+            "pin rows");
+    GLIDE_TRY(smooth(X.cfg.post_steps));
+    if (finest) GLIDE_TRY(smooth(X.cfg.finest_steps));
+    return GLIDE_OK;
+}
+
 
 }  // namespace
 
@@ -1017,6 +1615,198 @@ int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_l
     };
     auto cycle = [&]() -> int { return adj_vcycle(X, start_level, true); };
     return solve_loop(X, start_level, report, resid, cycle, 'a');
+}
+
+int glide_fascd_solve_mixed(const glide_level* levels64, const glide_level* levels32, int n_levels, int start_level,
+                            double dt, const glide_fas_config* cfg, const glide_vanka_config* vc,
+                            const glide_mixed_config* mc, glide_solve_report* report, int* stats, void* stream) {
+    GLIDE_TRY(check_mixed(levels64, levels32, n_levels, start_level, cfg, vc, mc, false));
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    const glide_level& D = levels64[start_level];
+    const glide_level& A = levels32[start_level];
+    REQ(D.f_H); REQ(D.r_u); REQ(D.r_v); REQ(D.r_H); REQ(D.H_prev); REQ(D.smb); REQ(D.gamma);
+    REQ(D.w_u); REQ(D.w_v); REQ(D.w_H); REQ(D.cphi);
+    hipStream_t st = (hipStream_t)stream;
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));  // allocated before any capture
+    // set_rhs (glide/operators.py:275-276) in FP64, the active-set bound, and
+    // the parameters of every level in FP32
+    HIP_TRY(launch_axpby<R>(P(D.f_H), 1.0 / dt, CP(D.H_prev), 1.0, CP(D.smb), 0.0, nullptr, (long)nC(D), st), "rhs");
+    GLIDE_TRY(fill(D.gamma, D.phys.thklim, nC(D), st));
+    for (int l = start_level; l < n_levels; ++l) {
+        const glide_level& E = levels64[l];
+        const glide_level& G = levels32[l];
+        MixedBatch pb;
+        pb.add(kMixTo32, G.bed, E.bed, nullptr, (long)nC(E));
+        pb.add(kMixTo32, G.B, E.B, nullptr, (long)nC(E));
+        pb.add(kMixTo32, G.beta, E.beta, nullptr, (long)nC(E));
+        GLIDE_TRY(run_mixed(pb, st));
+    }
+    MixedCtx X{levels64, levels32, n_levels, dt, *cfg, vc, mc->fp64_coarsest != 0, st};
+    auto resid = [&](hipStream_t cs, double* slot) -> int {
+        GLIDE_TRY(grounded(D, 0.0, cs));
+        GLIDE_TRY(residual(D, dt, GLIDE_USE_FORCING | GLIDE_USE_MASK, P(D.r_u), P(D.r_v), P(D.r_H), cs));
+        return sumsq3_async(CP(D.r_u), nU(D), CP(D.r_v), nV(D), CP(D.r_H), nC(D), slot, cs);
+    };
+    auto step = [&](hipStream_t cs, int phase) -> int {
+        X.st = cs;
+        EwBatch<R> save;
+        save.copy(P(D.w_u), CP(D.u), (long)nU(D));
+        save.copy(P(D.w_v), CP(D.v), (long)nV(D));
+        save.copy(P(D.w_H), CP(D.H), (long)nC(D));
+        save.copy(P(D.cphi), CP(D.mask), (long)nC(D));
+        HIP_TRY(launch_ew_batch<R>(save, cs), "save state");
+        if (phase == 2) {
+            // one FP64 V-cycle with the same red-black smoother and counts
+            FasCtx F{levels64, n_levels, dt, *cfg, vc, cs};
+            GLIDE_TRY(fas_vcycle(F, start_level, true));
+            X.st = st;
+            return GLIDE_OK;
+        }
+        // the defect without the active set (the inner solve applies the bound)
+        GLIDE_TRY(residual(D, dt, GLIDE_USE_FORCING, P(D.r_u), P(D.r_v), P(D.r_H), cs));
+        MixedBatch to32;
+        to32.add(kMixTo32, A.u, D.u, nullptr, (long)nU(D));
+        to32.add(kMixTo32, A.v, D.v, nullptr, (long)nV(D));
+        to32.add(kMixTo32, A.H, D.H, nullptr, (long)nC(D));
+        to32.add(kMixTo32, A.mask, D.mask, nullptr, (long)nC(D));
+        to32.add(kMixTo32, A.phi, D.phi, nullptr, (long)nC(D));
+        GLIDE_TRY(run_mixed(to32, cs));
+        GLIDE_TRY(residual_t<float>(A, dt, 0, PT<float>(A.F_u), PT<float>(A.F_v), PT<float>(A.F_H), cs));
+        MixedBatch defect;
+        defect.add(kMixDefect, A.f_u, A.F_u, D.r_u, (long)nU(D));
+        defect.add(kMixDefect, A.f_v, A.F_v, D.r_v, (long)nV(D));
+        defect.add(kMixDefect, A.f_H, A.F_H, D.r_H, (long)nC(D));
+        GLIDE_TRY(run_mixed(defect, cs));
+        for (int q = 0; q < mc->inner_vcycles; ++q) GLIDE_TRY(mixed_vcycle(X, start_level, true));
+        MixedBatch upd;
+        upd.add(kMixUpdate, D.u, A.u, nullptr, (long)nU(D));
+        upd.add(kMixUpdate, D.v, A.v, nullptr, (long)nV(D));
+        upd.add(kMixUpdateH, D.H, A.H, A.mask, (long)nC(D), D.phys.thklim);
+        upd.add(kMixTo64, D.mask, A.mask, nullptr, (long)nC(D));
+        GLIDE_TRY(run_mixed(upd, cs));
+        X.st = st;
+        return GLIDE_OK;
+    };
+    auto undo = [&](hipStream_t cs) -> int {
+        EwBatch<R> b;
+        b.copy(P(D.u), CP(D.w_u), (long)nU(D));
+        b.copy(P(D.v), CP(D.w_v), (long)nV(D));
+        b.copy(P(D.H), CP(D.w_H), (long)nC(D));
+        b.copy(P(D.mask), CP(D.cphi), (long)nC(D));
+        HIP_TRY(launch_ew_batch<R>(b, cs), "restore state");
+        return GLIDE_OK;
+    };
+    std::vector<glide_vanka_config> ref = reference_vanka_configs(n_levels);
+    auto fallback = [&](hipStream_t cs) -> int {
+        glide_fas_config fc = *cfg;
+        fc.coarsest_steps = 200; fc.pre_steps = 10; fc.post_steps = 20; fc.finest_steps = 50;
+        FasCtx F{levels64, n_levels, dt, fc, ref.data(), cs};
+        return fas_vcycle(F, start_level, true);
+    };
+    auto key = [&](int phase) {
+        return mixed_key(phase == 1 ? 'm' : 'f', levels64, levels32, n_levels, start_level, dt, *cfg, vc, *mc);
+    };
+    return mixed_outer_loop(*mc, st, key, true, report, stats, resid, step, undo, fallback);
+}
+
+int glide_fas_adjoint_solve_mixed(const glide_level* levels64, const glide_level* levels32, int n_levels,
+                                  int start_level, double dt, const glide_fas_config* cfg,
+                                  const glide_vanka_config* vc, const glide_mixed_config* mc,
+                                  glide_solve_report* report, int* stats, void* stream) {
+    GLIDE_TRY(check_mixed(levels64, levels32, n_levels, start_level, cfg, vc, mc, true));
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    const glide_level& D = levels64[start_level];
+    const glide_level& A = levels32[start_level];
+    REQ(D.lambda_u); REQ(D.lambda_v); REQ(D.lambda_H); REQ(D.af_u); REQ(D.af_v); REQ(D.af_H);
+    REQ(D.ar_u); REQ(D.ar_v); REQ(D.ar_H); REQ(D.wl_u); REQ(D.wl_v); REQ(D.wl_H);
+    for (int l = start_level; l < n_levels; ++l) {
+        if (!levels64[l].adj_cache)
+            return fail(GLIDE_ERR_INVALID_ARG, "the mixed adjoint solve needs an adjoint cache on every FP64 level");
+        if (!(levels64[l].phys.m == 1.0 && levels64[l].phys.u_reg > 0.0))
+            return fail(GLIDE_ERR_INVALID_ARG, "the mixed adjoint solve needs linear drag (m = 1)");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    // the forward state and the parameters in FP32, the start level's cache
+    {
+        MixedBatch b;
+        b.add(kMixTo32, A.u, D.u, nullptr, (long)nU(D));
+        b.add(kMixTo32, A.v, D.v, nullptr, (long)nV(D));
+        b.add(kMixTo32, A.H, D.H, nullptr, (long)nC(D));
+        b.add(kMixTo32, A.phi, D.phi, nullptr, (long)nC(D));
+        b.add(kMixTo32, A.mask, D.mask, nullptr, (long)nC(D));
+        GLIDE_TRY(run_mixed(b, st));
+    }
+    for (int l = start_level; l < n_levels; ++l) {
+        const glide_level& E = levels64[l];
+        const glide_level& G = levels32[l];
+        MixedBatch pb;
+        pb.add(kMixTo32, G.bed, E.bed, nullptr, (long)nC(E));
+        pb.add(kMixTo32, G.B, E.B, nullptr, (long)nC(E));
+        pb.add(kMixTo32, G.beta, E.beta, nullptr, (long)nC(E));
+        GLIDE_TRY(run_mixed(pb, st));
+    }
+    GLIDE_TRY(adjoint_prepare_t<float>(A, dt, vc[start_level], st));
+    MixedCtx X{levels64, levels32, n_levels, dt, *cfg, vc, mc->fp64_coarsest != 0, st};
+    auto resid = [&](hipStream_t cs, double* slot) -> int {
+        GLIDE_TRY(vjp(D, dt, GLIDE_USE_FORCING, CP(D.lambda_u), CP(D.lambda_v), CP(D.lambda_H), P(D.ar_u), P(D.ar_v),
+                      P(D.ar_H), cs));
+        return sumsq3_async(CP(D.ar_u), nU(D), CP(D.ar_v), nV(D), CP(D.ar_H), nC(D), slot, cs);
+    };
+    auto step = [&](hipStream_t cs, int) -> int {
+        X.st = cs;
+        EwBatch<R> save;
+        save.copy(P(D.wl_u), CP(D.lambda_u), (long)nU(D));
+        save.copy(P(D.wl_v), CP(D.lambda_v), (long)nV(D));
+        save.copy(P(D.wl_H), CP(D.lambda_H), (long)nC(D));
+        HIP_TRY(launch_ew_batch<R>(save, cs), "save multipliers");
+        // the correction solve J^T d = -(J^T lambda - g) from d = 0
+        MixedBatch b;
+        b.add(kMixNeg32, A.af_u, D.ar_u, nullptr, (long)nU(D));
+        b.add(kMixNeg32, A.af_v, D.ar_v, nullptr, (long)nV(D));
+        b.add(kMixNeg32, A.af_H, D.ar_H, nullptr, (long)nC(D));
+        GLIDE_TRY(run_mixed(b, cs));
+        EwBatch<float> z;
+        z.fill(PT<float>(A.lambda_u), 0.0f, (long)nU(A));
+        z.fill(PT<float>(A.lambda_v), 0.0f, (long)nV(A));
+        z.fill(PT<float>(A.lambda_H), 0.0f, (long)nC(A));
+        HIP_TRY(launch_ew_batch<float>(z, cs), "zero correction");
+        for (int q = 0; q < mc->inner_vcycles; ++q) GLIDE_TRY(mixed_adj_vcycle(X, start_level, true));
+        // the multipliers of pinned rows (domain-edge facets, active-set
+        // cells) stay zero: the transpose operator does not determine them
+        HIP_TRY(launch_pin_rows<float>(PT<float>(A.lambda_u), PT<float>(A.lambda_v), PT<float>(A.lambda_H),
+                                       CPT<float>(A.mask), A.ny, A.nx, cs),
+                "pin rows");
+        MixedBatch u;
+        u.add(kMixAdd64, D.lambda_u, A.lambda_u, nullptr, (long)nU(D));
+        u.add(kMixAdd64, D.lambda_v, A.lambda_v, nullptr, (long)nV(D));
+        u.add(kMixAdd64, D.lambda_H, A.lambda_H, nullptr, (long)nC(D));
+        GLIDE_TRY(run_mixed(u, cs));
+        X.st = st;
+        return GLIDE_OK;
+    };
+    auto undo = [&](hipStream_t cs) -> int {
+        EwBatch<R> b;
+        b.copy(P(D.lambda_u), CP(D.wl_u), (long)nU(D));
+        b.copy(P(D.lambda_v), CP(D.wl_v), (long)nV(D));
+        b.copy(P(D.lambda_H), CP(D.wl_H), (long)nC(D));
+        HIP_TRY(launch_ew_batch<R>(b, cs), "restore multipliers");
+        return GLIDE_OK;
+    };
+    std::vector<glide_vanka_config> ref = reference_vanka_configs(n_levels);
+    auto fallback = [&](hipStream_t cs) -> int {
+        glide_fas_config fc = *cfg;
+        fc.coarsest_steps = 200; fc.pre_steps = 10; fc.post_steps = 20; fc.finest_steps = 50;
+        FasCtx F{levels64, n_levels, dt, fc, ref.data(), cs};
+        GLIDE_TRY(adjoint_prepare(D, dt, ref[start_level], 0, cs));
+        return adj_vcycle(F, start_level, true);
+    };
+    auto key = [&](int) {
+        return mixed_key('n', levels64, levels32, n_levels, start_level, dt, *cfg, vc, *mc);
+    };
+    return mixed_outer_loop(*mc, st, key, false, report, stats, resid, step, undo, fallback);
 }
 
 int glide_adjoint_fused_enable(int on) {

@@ -259,6 +259,7 @@ __global__ __launch_bounds__(NT) void k_jvp(SsaArgs<T> a, const T* __restrict__ 
             d += g.d_w * dul + g.c0 * dvlt + g.c1 * dvt + g.c2 * dvlb + g.c3 * dvb;
             Drive<T> r = drive_x(H[c - 1], H[c], a.bed[c - 1], a.bed[c], pl, pc, dxi, true);
             d -= r.d_HA * dH[c - 1] + r.d_HB * dH[c];
+
         }
         out_u[o] = d;
     }
@@ -296,6 +297,7 @@ __global__ __launch_bounds__(NT) void k_jvp(SsaArgs<T> a, const T* __restrict__ 
             d += g.d_w * dvt + g.c0 * dutl + g.c1 * dutr + g.c2 * dul + g.c3 * dur;
             Drive<T> r = drive_y(H[o - nx], H[o], a.bed[o - nx], a.bed[o], pt, pc, dxi, true);
             d -= r.d_HA * dH[o - nx] + r.d_HB * dH[o];
+
         }
         out_v[o] = d;
     }
@@ -308,41 +310,7 @@ template <typename T, int W = TX + 2, class Ld = LdPlain>
 __device__ __forceinline__ void load_patch(Patch<T>& p, const SsaArgs<T>& a, const T (*s_eta)[W], int li, int lj,
                                            int i, int j, const T* __restrict__ fu, const T* __restrict__ fv,
                                            const T* __restrict__ fH, const T* __restrict__ phi) {
-    const int ny = a.ny, nx = a.nx;
-    p.e_c = s_eta[li][lj]; p.e_l = s_eta[li][lj - 1]; p.e_r = s_eta[li][lj + 1];
-    p.e_t = s_eta[li - 1][lj]; p.e_b = s_eta[li + 1][lj];
-    p.e_tl = s_eta[li - 1][lj - 1]; p.e_tr = s_eta[li - 1][lj + 1];
-    p.e_bl = s_eta[li + 1][lj - 1]; p.e_br = s_eta[li + 1][lj + 1];
-    const T* __restrict__ H = a.H;
-    p.H_l = Ld::get(H + offC(i, j - 1, ny, nx)); p.H_r = Ld::get(H + offC(i, j + 1, ny, nx));
-    p.H_t = Ld::get(H + offC(i - 1, j, ny, nx)); p.H_b = Ld::get(H + offC(i + 1, j, ny, nx));
-    p.H_tl = Ld::get(H + offC(i - 1, j - 1, ny, nx)); p.H_tr = Ld::get(H + offC(i - 1, j + 1, ny, nx));
-    p.H_bl = Ld::get(H + offC(i + 1, j - 1, ny, nx)); p.H_br = Ld::get(H + offC(i + 1, j + 1, ny, nx));
-    p.p_c = phi[offC(i, j, ny, nx)]; p.p_l = phi[offC(i, j - 1, ny, nx)]; p.p_r = phi[offC(i, j + 1, ny, nx)];
-    p.p_t = phi[offC(i - 1, j, ny, nx)]; p.p_b = phi[offC(i + 1, j, ny, nx)];
-    p.b_c = a.beta[offC(i, j, ny, nx)]; p.b_l = a.beta[offC(i, j - 1, ny, nx)]; p.b_r = a.beta[offC(i, j + 1, ny, nx)];
-    p.b_t = a.beta[offC(i - 1, j, ny, nx)]; p.b_b = a.beta[offC(i + 1, j, ny, nx)];
-    p.z_c = a.bed[offC(i, j, ny, nx)]; p.z_l = a.bed[offC(i, j - 1, ny, nx)]; p.z_r = a.bed[offC(i, j + 1, ny, nx)];
-    p.z_t = a.bed[offC(i - 1, j, ny, nx)]; p.z_b = a.bed[offC(i + 1, j, ny, nx)];
-    const T* __restrict__ u = a.u;
-    const T* __restrict__ v = a.v;
-    p.u_ll = Ld::get(u + offU(i, j - 1, ny, nx)); p.u_rr = Ld::get(u + offU(i, j + 2, ny, nx));
-    p.u_tl = Ld::get(u + offU(i - 1, j, ny, nx)); p.u_tr = Ld::get(u + offU(i - 1, j + 1, ny, nx));
-    p.u_bl = Ld::get(u + offU(i + 1, j, ny, nx)); p.u_br = Ld::get(u + offU(i + 1, j + 1, ny, nx));
-    p.v_lt = Ld::get(v + offV(i, j - 1, ny, nx)); p.v_lb = Ld::get(v + offV(i + 1, j - 1, ny, nx));
-    p.v_rt = Ld::get(v + offV(i, j + 1, ny, nx)); p.v_rb = Ld::get(v + offV(i + 1, j + 1, ny, nx));
-    p.v_tt = Ld::get(v + offV(i - 1, j, ny, nx)); p.v_bb = Ld::get(v + offV(i + 2, j, ny, nx));
-    if (fu) {
-        p.f0 = fu[offU(i, j, ny, nx)]; p.f1 = fu[offU(i, j + 1, ny, nx)];
-        p.f2 = fv[offV(i, j, ny, nx)]; p.f3 = fv[offV(i + 1, j, ny, nx)];
-        p.f4 = fH[offC(i, j, ny, nx)];
-    } else {
-        p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = T(0);
-    }
-    p.gam = a.gamma ? a.gamma[offC(i, j, ny, nx)] : T(0);
-    p.in_l = j > 0; p.in_r = j + 1 < nx; p.in_t = i > 0; p.in_b = i + 1 < ny;
-    p.cl = j > 0; p.cr = j + 1 < nx; p.ct = i > 0; p.cb = i + 1 < ny;
-    p.vtl = i > 0 && j > 0; p.vtr = i > 0 && j + 1 < nx; p.vbl = i + 1 < ny && j > 0; p.vbr = i + 1 < ny && j + 1 < nx;
+    load_patch_with<T, Ld>(p, a, [&](int di, int dj) { return s_eta[li + di][lj + dj]; }, i, j, fu, fv, fH, phi);
 }
 
 // Viscosity tile (+ ring) of cells, optionally refreshing the grounded
@@ -1325,14 +1293,17 @@ __global__ __launch_bounds__(TX * ROWS) void k_vjp(SsaArgs<T> a, const T* __rest
 // cached patch solves of the tile (thread rows < TR) and of the ring (the
 // other threads), then the update of the tile's multipliers into the
 // destination buffers.
-template <typename T, class Ld, int TR>
+// MC: one colour phase of the multicolour adjoint pass instead: only cells
+// with (i + j) % 2 == colour solve, and each unknown receives its (single)
+// solving cell's full increment times omega.
+template <typename T, class Ld, int TR, bool MC = false>
 __device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* __restrict__ cache,
                                                   const T* __restrict__ lu, const T* __restrict__ lv,
                                                   const T* __restrict__ lH, T* __restrict__ lu_dst,
                                                   T* __restrict__ lv_dst, T* __restrict__ lH_dst,
                                                   const T* __restrict__ fu, const T* __restrict__ fv,
                                                   const T* __restrict__ fH, T calving_rate, T omega, int i0, int j0,
-                                                  int ty, int tx) {
+                                                  int ty, int tx, int colour = -1) {
     constexpr int RR = TR + 2, RC = TX + 2;
     constexpr int CW = RC + 2, NCELL = (RR + 2) * CW, VW = RC + 1, NVERT = (RR + 1) * VW;
     __shared__ T s_cell[8 * NCELL];
@@ -1357,9 +1328,20 @@ __device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* 
                                                           tid, NTF, store);
     __syncthreads();
     auto increments = [&](int ci, int cj, T w[5]) {
+        if constexpr (MC) {
+            if (((ci + cj) & 1) != colour) {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) w[q] = T(0);
+                return;
+            }
+        }
         const int ti = ci - i0 + 1, tj = cj - j0 + 1;
         const T b[5] = {r_u[ti][tj], r_u[ti][tj + 1], r_v[ti][tj], r_v[ti + 1][tj], r_h[ti][tj]};
         adjoint_cached_solve(cache, n, (long)ci * nx + cj, b, w);
+        if constexpr (MC) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = T(2) * w[q];  // full facet weight
+        }
     };
     // thread rows 0 .. TR-1 own the tile's cells, the rest solve the ring
     const int i = i0 + ty, j = j0 + tx;
@@ -1411,7 +1393,7 @@ __device__ __forceinline__ void adjoint_pass_tile(const SsaArgs<T>& a, const T* 
     lH_dst[c] = Ld::get(lH + c) + omega * w[4];
 }
 
-template <typename T, int TR>
+template <typename T, int TR, bool MC = false>
 __global__ __launch_bounds__(TX * 2 * TR) void k_adjoint_pass_fused(SsaArgs<T> a, const T* __restrict__ cache,
                                                                            const T* __restrict__ lu,
                                                                            const T* __restrict__ lv,
@@ -1422,9 +1404,9 @@ __global__ __launch_bounds__(TX * 2 * TR) void k_adjoint_pass_fused(SsaArgs<T> a
                                                                            const T* __restrict__ fu,
                                                                            const T* __restrict__ fv,
                                                                            const T* __restrict__ fH, T calving_rate,
-                                                                           T omega) {
-    adjoint_pass_tile<T, LdPlain, TR>(a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, omega,
-                                      blockIdx.y * TR, blockIdx.x * TX, threadIdx.y, threadIdx.x);
+                                                                           T omega, int colour) {
+    adjoint_pass_tile<T, LdPlain, TR, MC>(a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate,
+                                          omega, blockIdx.y * TR, blockIdx.x * TX, threadIdx.y, threadIdx.x, colour);
 }
 
 // n_iter adjoint passes of a level at most TX cells wide in one launch
@@ -1754,7 +1736,18 @@ hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, const 
     if (!a.P.linear_drag) return hipErrorInvalidValue;
     TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx, st);
     k_adjoint_pass_fused<T, TY><<<tile_grid(a.ny, a.nx, false), dim3(TX, 2 * TY), 0, st>>>(
-        a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, -omega);
+        a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, -omega, -1);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_adjoint_pass_mc(const SsaArgs<T>& a, const T* cache, const T* lu, const T* lv, const T* lH,
+                                  T* lu_dst, T* lv_dst, T* lH_dst, const T* fu, const T* fv, const T* fH,
+                                  T calving_rate, T omega, int colour, hipStream_t st) {
+    if (!a.P.linear_drag) return hipErrorInvalidValue;
+    TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx / 2, st);
+    k_adjoint_pass_fused<T, TY, true><<<tile_grid(a.ny, a.nx, false), dim3(TX, 2 * TY), 0, st>>>(
+        a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, -omega, colour & 1);
     return hipGetLastError();
 }
 bool adjoint_persistent_fits(int ny, int nx) {
@@ -1847,6 +1840,8 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
                                                      const VankaArgs<T>&, T, hipStream_t);                           \
     template hipError_t launch_adjoint_pass_fused<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, T*,   \
                                                      T*, T*, const T*, const T*, const T*, T, T, hipStream_t);       \
+    template hipError_t launch_adjoint_pass_mc<T>(const SsaArgs<T>&, const T*, const T*, const T*, const T*, T*, T*, \
+                                                  T*, const T*, const T*, const T*, T, T, int, hipStream_t);         \
     template bool adjoint_coarse_fits<T>(const SsaArgs<T>&);                                                         \
     template hipError_t launch_adjoint_persistent<T>(const SsaArgs<T>&, const T*, T*, T*, T*, T*, T*, T*, const T*,  \
                                                      const T*, const T*, T, T, int, const PersistentCtl&,            \
@@ -1861,5 +1856,6 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_grad_beta_bed<T>(const SsaArgs<T>&, const T*, const T*, T*, T*, hipStream_t);
 
 GLIDE_INSTANTIATE(double)
+GLIDE_INSTANTIATE(float)
 
 }  // namespace glide

@@ -18,12 +18,59 @@ struct SsaArgs {
     const T *f_u, *f_v, *f_H;
 };
 
+// Gather everything the patch of cell (i,j) reads besides its unknowns;
+// eta_at(di, dj) gives the frozen viscosity of cell (i + di, j + dj).
+template <typename T, class Ld = LdPlain, class EtaAt>
+__device__ __forceinline__ void load_patch_with(Patch<T>& p, const SsaArgs<T>& a, EtaAt&& eta_at, int i, int j,
+                                                const T* __restrict__ fu, const T* __restrict__ fv,
+                                                const T* __restrict__ fH, const T* __restrict__ phi) {
+    const int ny = a.ny, nx = a.nx;
+    p.e_c = eta_at(0, 0); p.e_l = eta_at(0, -1); p.e_r = eta_at(0, 1);
+    p.e_t = eta_at(-1, 0); p.e_b = eta_at(1, 0);
+    p.e_tl = eta_at(-1, -1); p.e_tr = eta_at(-1, 1);
+    p.e_bl = eta_at(1, -1); p.e_br = eta_at(1, 1);
+    const T* __restrict__ H = a.H;
+    p.H_l = Ld::get(H + offC(i, j - 1, ny, nx)); p.H_r = Ld::get(H + offC(i, j + 1, ny, nx));
+    p.H_t = Ld::get(H + offC(i - 1, j, ny, nx)); p.H_b = Ld::get(H + offC(i + 1, j, ny, nx));
+    p.H_tl = Ld::get(H + offC(i - 1, j - 1, ny, nx)); p.H_tr = Ld::get(H + offC(i - 1, j + 1, ny, nx));
+    p.H_bl = Ld::get(H + offC(i + 1, j - 1, ny, nx)); p.H_br = Ld::get(H + offC(i + 1, j + 1, ny, nx));
+    p.p_c = phi[offC(i, j, ny, nx)]; p.p_l = phi[offC(i, j - 1, ny, nx)]; p.p_r = phi[offC(i, j + 1, ny, nx)];
+    p.p_t = phi[offC(i - 1, j, ny, nx)]; p.p_b = phi[offC(i + 1, j, ny, nx)];
+    p.b_c = a.beta[offC(i, j, ny, nx)]; p.b_l = a.beta[offC(i, j - 1, ny, nx)]; p.b_r = a.beta[offC(i, j + 1, ny, nx)];
+    p.b_t = a.beta[offC(i - 1, j, ny, nx)]; p.b_b = a.beta[offC(i + 1, j, ny, nx)];
+    p.z_c = a.bed[offC(i, j, ny, nx)]; p.z_l = a.bed[offC(i, j - 1, ny, nx)]; p.z_r = a.bed[offC(i, j + 1, ny, nx)];
+    p.z_t = a.bed[offC(i - 1, j, ny, nx)]; p.z_b = a.bed[offC(i + 1, j, ny, nx)];
+    const T* __restrict__ u = a.u;
+    const T* __restrict__ v = a.v;
+    p.u_ll = Ld::get(u + offU(i, j - 1, ny, nx)); p.u_rr = Ld::get(u + offU(i, j + 2, ny, nx));
+    p.u_tl = Ld::get(u + offU(i - 1, j, ny, nx)); p.u_tr = Ld::get(u + offU(i - 1, j + 1, ny, nx));
+    p.u_bl = Ld::get(u + offU(i + 1, j, ny, nx)); p.u_br = Ld::get(u + offU(i + 1, j + 1, ny, nx));
+    p.v_lt = Ld::get(v + offV(i, j - 1, ny, nx)); p.v_lb = Ld::get(v + offV(i + 1, j - 1, ny, nx));
+    p.v_rt = Ld::get(v + offV(i, j + 1, ny, nx)); p.v_rb = Ld::get(v + offV(i + 1, j + 1, ny, nx));
+    p.v_tt = Ld::get(v + offV(i - 1, j, ny, nx)); p.v_bb = Ld::get(v + offV(i + 2, j, ny, nx));
+    if (fu) {
+        p.f0 = fu[offU(i, j, ny, nx)]; p.f1 = fu[offU(i, j + 1, ny, nx)];
+        p.f2 = fv[offV(i, j, ny, nx)]; p.f3 = fv[offV(i + 1, j, ny, nx)];
+        p.f4 = fH[offC(i, j, ny, nx)];
+    } else {
+        p.f0 = p.f1 = p.f2 = p.f3 = p.f4 = T(0);
+    }
+    p.gam = a.gamma ? a.gamma[offC(i, j, ny, nx)] : T(0);
+    p.in_l = j > 0; p.in_r = j + 1 < nx; p.in_t = i > 0; p.in_b = i + 1 < ny;
+    p.cl = j > 0; p.cr = j + 1 < nx; p.ct = i > 0; p.cb = i + 1 < ny;
+    p.vtl = i > 0 && j > 0; p.vtr = i > 0 && j + 1 < nx; p.vbl = i + 1 < ny && j > 0; p.vbr = i + 1 < ny && j + 1 < nx;
+}
+
+
 template <typename T>
 struct VankaArgs {
     int newton_steps;
     T ssa_damping, mc_damping, calving_rate;
     int* iters;      // optional per-cell Newton iteration count
     int local_fp32;  // mixed form: patch Jacobian and its LU in FP32, residual and update in T
+    // multicolour passes only (the additive pass fixes 0.5 and 1e-6): local
+    // Newton relaxation and the stopping threshold on sum r^2
+    T relax, tol;
 };
 
 template <typename T>
@@ -99,6 +146,14 @@ template <typename T>
 hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, const T* lu, const T* lv, const T* lH,
                                      T* lu_dst, T* lv_dst, T* lH_dst, const T* fu, const T* fv, const T* fH,
                                      T calving_rate, T omega, hipStream_t st);
+// One colour phase of the multicolour adjoint pass (no counterpart in the
+// reference; linear drag): the fused pass's transpose operator and cached
+// patch solves, but only cells with (i + j) % 2 == colour solve, and each
+// multiplier they own moves by omega times the full solution (dst = l - omega d).
+template <typename T>
+hipError_t launch_adjoint_pass_mc(const SsaArgs<T>& a, const T* cache, const T* lu, const T* lv, const T* lH,
+                                  T* lu_dst, T* lv_dst, T* lH_dst, const T* fu, const T* fv, const T* fH,
+                                  T calving_rate, T omega, int colour, hipStream_t st);
 #ifndef GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE
 #define GLIDE_ADJOINT_PASS_FUSED_MAX_SIDE 512
 #endif
@@ -143,6 +198,54 @@ hipError_t launch_vanka_adjoint_pass_cached(int ny, int nx, const T* cache, cons
 template <typename T>
 hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st);
 
+
+// Multicolour (red-black) Vanka smoothing (mc_kernels.hip; no counterpart in
+// the reference).  launch_cell_eta fills eta (ny, nx) with the cell viscosity
+// of the state in a.  launch_vanka_mc runs one colour phase: every cell with
+// (i + j) % 2 == colour solves its 5x5 patch (local Newton with vk.relax,
+// vk.tol, the Vanka dampings, viscosity from eta) against the state in a and
+// writes its five unknowns to u_dst / v_dst / H_dst; the other cells copy their
+// thickness and the domain-edge facets no cell of the colour owns, so the
+// destination holds the complete new state; the active-set flag is written in
+// place.  The destination buffers must differ from a's state.
+template <typename T>
+hipError_t launch_cell_eta(const SsaArgs<T>& a, T* eta, hipStream_t st);
+// Zero the multipliers / corrections of the rows the patch solves pin: the
+// domain-edge facets and the cells of the active set (mask > 0.5).
+template <typename T>
+hipError_t launch_pin_rows(T* du, T* dv, T* dH, const T* mask, int ny, int nx, hipStream_t st);
+template <typename T>
+hipError_t launch_vanka_mc(const SsaArgs<T>& a, const T* eta, T* u_dst, T* v_dst, T* H_dst, T* mask,
+                           const VankaArgs<T>& vk, int colour, hipStream_t st);
+
+// Element-wise jobs between FP64 and FP32 buffers of the mixed-precision
+// solver (mc_kernels.hip), batched into one launch (the jobs must not read
+// each other's outputs).
+enum MixedOp {
+    kMixTo32 = 0,     // out32 = (float) a64
+    kMixTo64 = 1,     // out64 = (double) a32
+    kMixDefect = 2,   // out32 = a32 - (float) b64
+    kMixUpdate = 3,   // out64 += (double) a32 - (double)(float) out64
+    kMixUpdateH = 4,  // kMixUpdate, then out64 = val where b32 (the active-set flag) > 0.5
+    kMixAdd64 = 5,    // out64 += (double) a32
+    kMixNeg32 = 6     // out32 = -(float) a64
+};
+struct MixedJob {
+    void* out;
+    const void *a, *b;
+    double val;
+    long n;
+    int op;
+};
+constexpr int kMixedMaxJobs = 12;
+struct MixedBatch {
+    MixedJob job[kMixedMaxJobs];
+    int count = 0;
+    void add(int op, void* out, const void* a, const void* b, long n, double val = 0.0) {
+        job[count++] = MixedJob{out, a, b, val, n, op};
+    }
+};
+hipError_t launch_mixed_batch(const MixedBatch& b, hipStream_t st);
 
 // transfers and element-wise helpers (transfer_kernels.hip)
 template <typename T>

@@ -306,5 +306,6 @@ hipError_t launch_restrict_batch(const RestrictBatch<T>& b, int nyc, int nxc, hi
     template hipError_t launch_restrict_batch<T>(const RestrictBatch<T>&, int, int, hipStream_t);
 
 GLIDE_TRANSFER_INSTANTIATE(double)
+GLIDE_TRANSFER_INSTANTIATE(float)
 
 }  // namespace glide

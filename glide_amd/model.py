@@ -14,7 +14,7 @@ class IceDynamics:
     ny, nx, dx and a level count (glide/model.py:25-39)."""
 
     def __init__(self, mg=None, n_levels=None, grid=None, ny=None, nx=None, dx=None, x0=0.0, y0=0.0, crs=None,
-                 device=None):
+                 device=None, precision="fp64"):
         if mg is not None:
             self.mg = mg
         elif grid is not None and n_levels is not None:
@@ -28,17 +28,22 @@ class IceDynamics:
         self._adjoint_solver = None
         self.top_level = 0
         self._post_forward_hooks = []
+        # (an addition) "fp64": the reference's solves in FP64; "mixed": the
+        # FP32 solves with FP64 defect correction (glide_amd.mixed)
+        self.precision = precision
 
     @property
     def forward_solver(self) -> FASCDSolver:
         if self._forward_solver is None:
             self._forward_solver = FASCDSolver(self.mg)
+            self._forward_solver.precision = self.precision
         return self._forward_solver
 
     @property
     def adjoint_solver(self) -> FASAdjointSolver:
         if self._adjoint_solver is None:
             self._adjoint_solver = FASAdjointSolver(self.mg)
+            self._adjoint_solver.precision = self.precision
         return self._adjoint_solver
 
     def set_top_level(self, level):

@@ -437,10 +437,9 @@ class VankaOptions:
     def __init__(self, levels, getter):
         self._levels = levels
         self._getter = getter
-        self.options = ["omega", "relax_phi", "local_fp32"]
-        self.omega = BroadcastOption(levels, getter, "omega")
-        self.relax_phi = BroadcastOption(levels, getter, "relax_phi")
-        self.local_fp32 = BroadcastOption(levels, getter, "local_fp32")
+        self.options = ["omega", "relax_phi", "local_fp32", "multicolour", "newton_tol"]
+        for name in self.options:
+            setattr(self, name, BroadcastOption(levels, getter, name))
         self.newton_options = NewtonOptions(levels, getter=lambda lev: getter(lev).newton_config)
 
     def set(self, **kwargs):
@@ -501,10 +500,25 @@ def _print_report(rep, n3_final):
     print(f"  final |r_u| = {n3_final[0]:.2e}, |r_v| = {n3_final[1]:.2e}, |r_H| = {n3_final[2]:.2e}", flush=True)
 
 
+def _mixed_hierarchy(multigrid):
+    """The FP32 mirror of the hierarchy, shared by the forward and adjoint solvers."""
+    from .mixed import MixedHierarchy
+    mh = getattr(multigrid, "_mixed_hierarchy", None)
+    if mh is None:
+        mh = MixedHierarchy(multigrid.levels)
+        multigrid._mixed_hierarchy = mh
+    return mh
+
+
 class FASCDSolver:
-    """FAS constrained-descent solver (glide/multigrid.py:486-645)."""
+    """FAS constrained-descent solver (glide/multigrid.py:486-645).
+
+    ``precision`` (an addition): "fp64" (default) runs the reference's solve in
+    FP64; "mixed" runs the defect-corrected FP32 solve of glide_amd.mixed with
+    ``mixed_options`` (a MixedConfig) and ignores fas_options/vanka_options."""
 
     def __init__(self, multigrid):
+        from .mixed import MixedConfig
         self.multigrid = multigrid
         self.levels = [FASCDLevel(g, FASCDScratch(g)) for g in multigrid.levels]
         self._fas_config = FASCDConfig()
@@ -513,6 +527,10 @@ class FASCDSolver:
         self.n_levels = len(self.levels)
         self.dt = None
         self.last_report = None
+        self.precision = "fp64"
+        self.mixed_options = MixedConfig()
+        self.last_fallbacks = 0
+        self.last_stats = None
 
     def _structs(self):
         arr = (_lib.Level * self.n_levels)()
@@ -530,9 +548,24 @@ class FASCDSolver:
                 if lev.grid.ny != lev.grid.nx:
                     raise ValueError("compat_vprolong needs square levels; set fas_options compat_vprolong=False")
 
+    def _solve_mixed(self, start_level, report_norms):
+        from . import mixed
+        arr, _ = self._structs()
+        rep, self.last_stats = mixed.forward_solve(arr, _mixed_hierarchy(self.multigrid), start_level, self.dt,
+                                                   self.mixed_options, stream_handle(self.multigrid.device))
+        self.last_fallbacks = self.last_stats["fallbacks"]
+        self.last_report = rep
+        if self._fas_config.report_norms and report_norms:
+            _print_report(rep, list(rep.norms3))
+        return rep
+
     def solve(self, dt, start_level=0, report_norms=True):
         """glide/multigrid.py:502-533 (one native call)."""
         self.dt = float(dt)
+        if self.precision == "mixed":
+            return self._solve_mixed(start_level, report_norms)
+        if self.precision != "fp64":
+            raise ValueError(f"precision must be 'fp64' or 'mixed' (got {self.precision!r})")
         self._check_square(start_level)
         arr, vcs = self._structs()
         cfg = self._fas_config.to_c()
@@ -556,9 +589,11 @@ class FASCDSolver:
 
 
 class FASAdjointSolver:
-    """FAS solver of the adjoint system (glide/multigrid.py:857-973)."""
+    """FAS solver of the adjoint system (glide/multigrid.py:857-973); ``precision``
+    and ``mixed_options`` as for FASCDSolver (the mixed form needs linear drag)."""
 
     def __init__(self, multigrid):
+        from .mixed import MixedConfig
         self.multigrid = multigrid
         self.levels = [FASAdjointLevel(g, FASAdjointScratch(g)) for g in multigrid.levels]
         self._fas_config = FASAdjointConfig()
@@ -567,6 +602,10 @@ class FASAdjointSolver:
         self.n_levels = len(self.levels)
         self.dt = None
         self.last_report = None
+        self.precision = "fp64"
+        self.mixed_options = MixedConfig()
+        self.last_fallbacks = 0
+        self.last_stats = None
 
     def _structs(self):
         arr = (_lib.Level * self.n_levels)()
@@ -581,6 +620,19 @@ class FASAdjointSolver:
 
     def solve(self, dt, start_level=0, report_norms=True):
         self.dt = float(dt)
+        if self.precision == "mixed":
+            from . import mixed
+            arr, _ = self._structs()
+            rep, self.last_stats = mixed.adjoint_solve(arr, _mixed_hierarchy(self.multigrid), start_level,
+                                                       self.dt, self.mixed_options,
+                                                       stream_handle(self.multigrid.device))
+            self.last_fallbacks = self.last_stats["fallbacks"]
+            self.last_report = rep
+            if self._fas_config.report_norms and report_norms:
+                _print_report(rep, list(rep.norms3))
+            return rep
+        if self.precision != "fp64":
+            raise ValueError(f"precision must be 'fp64' or 'mixed' (got {self.precision!r})")
         if self._fas_config.compat_vprolong:
             for lev in self.levels[start_level:]:
                 if lev.grid.ny != lev.grid.nx:

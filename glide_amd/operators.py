@@ -42,13 +42,19 @@ class VankaConfig:
     hook_interval: int = 1
     hook_func: Callable[[int], None] | None = None
     local_fp32: bool = False
+    # (an addition) red-black passes instead of the additive one; their local
+    # Newton iteration uses newton_config.relaxation and stops once
+    # sum r^2 <= newton_tol
+    multicolour: bool = False
+    newton_tol: float = 1e-6
 
     def to_c(self) -> _lib.VankaConfigC:
         n = self.newton_config
         return _lib.VankaConfigC(omega=float(self.omega), relax_phi=float(self.relax_phi),
                                  newton_steps=int(n.steps), relaxation=float(n.relaxation),
                                  ssa_damping=float(n.ssa_damping), mc_damping=float(n.mc_damping),
-                                 local_fp32=1 if self.local_fp32 else 0)
+                                 local_fp32=1 if self.local_fp32 else 0, multicolour=1 if self.multicolour else 0,
+                                 newton_tol=float(self.newton_tol))
 
 
 def _norms(a, b, c, stream):

@@ -110,6 +110,14 @@ typedef struct glide_vanka_config {
      * each cell's patch Jacobian in FP32 while the residual, the stopping test
      * and the Newton update stay FP64; 0 is the plain FP64 form. */
     int32_t local_fp32;
+    /* Multicolour smoothing (no counterpart in the reference): 1 replaces the
+     * additive pass by a red-black pass (red cells, then black cells, each
+     * writing its five unknowns at full weight; `omega` unused), whose local
+     * Newton iteration uses `relaxation` and stops once sum r^2 <= newton_tol
+     * (tested before each update, at most newton_steps updates); the adjoint
+     * passes likewise.  The fixed point is the additive smoother's. */
+    int32_t multicolour;
+    double newton_tol;
 } glide_vanka_config;
 
 /* FASCDConfig / FASAdjointConfig (glide/multigrid.py:671-682, 992-1001).
@@ -181,6 +189,60 @@ int glide_prolong_cell(const void* coarse, void* fine, int ny_fine, int nx_fine,
 int glide_fascd_solve(const glide_level* levels, int n_levels, int start_level, double dt,
                       const glide_fas_config* cfg, const glide_vanka_config* vc, glide_solve_report* report,
                       void* stream);
+
+/* The mixed-precision form of glide_fascd_solve (no counterpart in the
+ * reference).  An FP64 defect-correction loop: per outer iteration the FP64
+ * residual r(x) of the start level (GLIDE_USE_FORCING, the active set), the
+ * FP32 copy x32 of the state, the FP32 forcing F32(x32) - r(x), inner_vcycles
+ * FP32 FAS-CD V-cycles with the red-black smoother (vc[l].multicolour must be
+ * 1; the counts, freeze flags and compat_vprolong come from cfg, whose
+ * tolerances and maximum_vcycles are not used) whose coarsest level runs in
+ * FP64 when fp64_coarsest is set, then x += y32 - fl32(x) in FP64 with the
+ * inner active set (masked cells set to thklim exactly).  It stops when the
+ * FP64 norm meets relative_tolerance or absolute_tolerance, or after
+ * max_outer iterations (<= 64).  The FP32 copy of the state rounds every
+ * correction to FP32 steps of the state's size (~6e-8 relative), so once an
+ * outer iteration reduces the norm by less than 0.6 (or fails), the loop
+ * switches to FP64 FAS-CD V-cycles with the same red-black smoother, counts
+ * and flags on levels64 (the finish to the FP64 tolerance).  An outer
+ * iteration whose FP64 norm does not decrease in that phase is undone and
+ * replaced by one FP64 V-cycle of the reference form (additive smoother,
+ * default counts).  stats (optional, 3 ints): fallbacks, FP32 iterations,
+ * FP64 red-black V-cycles.
+ * levels32 mirrors levels64 in FP32 (state, phi, mask, bed, B, beta, f/r/F,
+ * gamma, the FAS scratch and the 5-plane workspace); levels64 needs the FP64
+ * buffers of glide_fascd_solve on every level.  The
+ * outer iteration is
+ * replayed from a cached HIP graph and synchronises once per iteration for the
+ * norm; the report holds the FP64 norm after each outer iteration. */
+typedef struct glide_mixed_config {
+    int32_t max_outer;
+    int32_t inner_vcycles;
+    double relative_tolerance, absolute_tolerance;
+    int32_t fp64_coarsest;
+} glide_mixed_config;
+int glide_fascd_solve_mixed(const glide_level* levels64, const glide_level* levels32, int n_levels, int start_level,
+                            double dt, const glide_fas_config* cfg, const glide_vanka_config* vc,
+                            const glide_mixed_config* mc, glide_solve_report* report, int* stats, void* stream);
+
+/* The mixed-precision form of glide_fas_adjoint_solve (no counterpart in the
+ * reference; linear drag).  Per outer iteration the FP64 adjoint residual
+ * rho = J^T lambda - g of the start level, then inner_vcycles FP32 FAS adjoint
+ * V-cycles for the correction J^T d = -rho from d = 0 with the red-black
+ * adjoint passes (vc[l].multicolour = 1; each colour's patch solves move
+ * their multipliers by omega times the full solution; the cached patch
+ * factors use vc's dampings), the coarsest level in FP64 when fp64_coarsest
+ * is set, and lambda += d in FP64.  Stopping, undo-and-fallback (one FP64
+ * reference adjoint V-cycle), graph replay and the report as for
+ * glide_fascd_solve_mixed (one phase).  The multipliers of the rows the
+ * patch solves pin (domain-edge facets, active-set cells) are kept at zero.
+ * levels32 needs the FP32 state, multiplier,
+ * adjoint-operator and adjoint-scratch buffers, the workspace and adj_cache;
+ * every FP64 level needs adj_cache. */
+int glide_fas_adjoint_solve_mixed(const glide_level* levels64, const glide_level* levels32, int n_levels,
+                                  int start_level, double dt, const glide_fas_config* cfg,
+                                  const glide_vanka_config* vc, const glide_mixed_config* mc,
+                                  glide_solve_report* report, int* stats, void* stream);
 
 /* One V-cycle from `level` down (FASCDSolver.vcycle, glide/multigrid.py:536-645);
  * finest != 0 stores w and chi first and adds the finest-level sweeps. */

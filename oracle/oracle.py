@@ -28,11 +28,13 @@ class Phys(ctypes.Structure):
 class FasConfig(ctypes.Structure):
     _fields_ = [("freeze_coarse_calving", _I), ("freeze_coarse_phi", _I), ("coarsest_steps", _I),
                 ("pre_steps", _I), ("post_steps", _I), ("finest_steps", _I), ("maximum_vcycles", _I),
-                ("relative_tolerance", _D), ("absolute_tolerance", _D), ("compat_hfacet", _I)]
+                ("relative_tolerance", _D), ("absolute_tolerance", _D), ("compat_hfacet", _I),
+                ("project_pinned", _I)]
 
 
 class VankaConfig(ctypes.Structure):
-    _fields_ = [("omega", _D), ("relax_phi", _D), ("newton_steps", _I), ("ssa_damping", _D), ("mc_damping", _D)]
+    _fields_ = [("omega", _D), ("relax_phi", _D), ("newton_steps", _I), ("ssa_damping", _D), ("mc_damping", _D),
+                ("multicolour", _I), ("relaxation", _D), ("newton_tol", _D)]
 
 
 class Report(ctypes.Structure):
@@ -191,6 +193,32 @@ def vanka_sweep(F, P: Phys, dx, dt, newton_steps=30, ssa_damping=0.01, mc_dampin
     return u, v, H, mask, it
 
 
+def mc_sweep(F, P: Phys, dx, dt, newton_steps=4, relaxation=1.0, newton_tol=0.0, ssa_damping=0.0, mc_damping=0.0):
+    """One multicolour (red-black) pass, no counterpart in the reference:
+    returns the new (u, v, H, mask) and the Newton iteration counts."""
+    ny, nx = F["H"].shape
+    u, v, H, mask = F["u"].copy(), F["v"].copy(), F["H"].copy(), F["mask"].copy()
+    it = np.zeros((ny, nx))
+    lib().orc_mc_sweep(_p(u), _p(v), _p(H), _p(mask), *[_p(F[k]) for k in ("phi", "f_u", "f_v", "f_H", "bed", "B",
+                                                                          "beta", "gamma")],
+                       ctypes.byref(P), _D(dx), _D(dt), ny, nx, int(newton_steps), _D(relaxation), _D(newton_tol),
+                       _D(ssa_damping), _D(mc_damping), _p(it))
+    return u, v, H, mask, it
+
+
+def mc_adjoint_sweep(F, lam, af, P: Phys, dx, dt, ssa_damping=0.0, mc_damping=0.0, omega=1.0):
+    """One multicolour adjoint pass (no counterpart in the reference) on the
+    multipliers lam = (l_u, l_v, l_H) with adjoint forcing af; returns the new
+    multipliers."""
+    ny, nx = F["H"].shape
+    lu, lv, lH = (np.ascontiguousarray(x, dtype=np.float64).copy() for x in lam)
+    fu, fv, fH = (np.ascontiguousarray(x, dtype=np.float64) for x in af)
+    lib().orc_mc_adjoint_sweep(_p(lu), _p(lv), _p(lH), _p(F["u"]), _p(F["v"]), _p(F["H"]), _p(F["phi"]),
+                               _p(F["mask"]), _p(fu), _p(fv), _p(fH), _p(F["bed"]), _p(F["B"]), _p(F["beta"]),
+                               ctypes.byref(P), _D(dx), _D(dt), ny, nx, _D(ssa_damping), _D(mc_damping), _D(omega))
+    return lu, lv, lH
+
+
 def vjp(F, lam_u, lam_v, lam_H, P: Phys, dx, dt, use_forcing=False, af=None):
     ny, nx = F["H"].shape
     ou, ov, oH = np.zeros((ny, nx + 1)), np.zeros((ny + 1, nx)), np.zeros((ny, nx))
@@ -309,10 +337,15 @@ class OracleMG:
         return FasConfig(int(freeze_coarse_calving), int(freeze_coarse_phi), coarsest, pre, post, finest, vcycles,
                          rtol, atol, int(compat))
 
-    def vanka_configs(self, omega=0.5, relax_phi=0.0, newton_steps=30, ssa_damping=0.01, mc_damping=1.0):
+    def vanka_configs(self, omega=0.5, relax_phi=0.0, newton_steps=30, ssa_damping=0.01, mc_damping=1.0,
+                      multicolour=False, relaxation=1.0, newton_tol=1e-6):
+        """Per-level smoother settings; multicolour selects the red-black
+        passes (mc_sweep / mc_adjoint_sweep) with the given local Newton
+        relaxation and stopping threshold instead of the additive pass."""
         arr = (VankaConfig * self.n_levels)()
         for k in range(self.n_levels):
-            arr[k] = VankaConfig(omega, relax_phi, newton_steps, ssa_damping, mc_damping)
+            arr[k] = VankaConfig(omega, relax_phi, newton_steps, ssa_damping, mc_damping, int(bool(multicolour)),
+                                 relaxation, newton_tol)
         return arr
 
     def fascd_solve(self, dt, cfg: FasConfig, vcs=None, start_level=0) -> Report:

@@ -734,6 +734,81 @@ void vanka_pass(double* u, double* v, double* H, double* mask, const Fields& F0,
     std::memcpy(mask, newmask.data(), nh * sizeof(double));
 }
 
+// Multicolour Vanka pass (no counterpart in the reference; the smoother of
+// the mixed-precision solver): two phases, red cells ((i + j) even) then
+// black cells.  In a phase every cell of the colour runs a local Newton
+// iteration on the reference's 5x5 patch system (build_local, viscosity frozen
+// at the state the pass starts from) against the state at the start of the
+// phase, with relaxation `relax` and the Vanka dampings, and writes its five
+// unknowns directly: each facet belongs to exactly one cell of a colour, so
+// there is no averaging and no damping of the update.  Cells of one colour
+// read each other's old values (a phase is a block-Jacobi step over
+// non-overlapping patches), which keeps the result independent of the order.
+// Local stopping: sum r^2 < tol, tested before the update as in the additive
+// pass.  Fixed point: the same as the additive pass (r = 0 with the same
+// active set).
+void mc_pass(double* u, double* v, double* H, double* mask, const Fields& F0, const orc_phys& P, double calving_rate,
+             double dx, double dt, int ny, int nx, int newton_steps, double relax, double tol, double ssa_damping,
+             double mc_damping, std::vector<double>* dbg_newton_iters) {
+    Acc A{ny, nx};
+    const size_t nu = (size_t)ny * (nx + 1), nv = (size_t)(ny + 1) * nx, nh = (size_t)ny * nx;
+    std::vector<double> eta;
+    viscosity_field(A, u, v, nullptr, nullptr, F0.B, P.n, P.eps_reg, dx, eta, nullptr);
+    std::vector<double> su(nu), sv(nv), sH(nh);
+    for (int colour = 0; colour < 2; ++colour) {
+        std::memcpy(su.data(), u, nu * sizeof(double));
+        std::memcpy(sv.data(), v, nv * sizeof(double));
+        std::memcpy(sH.data(), H, nh * sizeof(double));
+        Fields S = F0;
+        S.u = su.data(); S.v = sv.data(); S.H = sH.data();
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int i = 0; i < ny; ++i) {
+            for (int j = (i + colour) & 1; j < nx; j += 2) {
+                const double thk = A.C(S.gamma, i, j);
+                double x[5] = {A.U(S.u, i, j), A.U(S.u, i, j + 1), A.V(S.v, i, j), A.V(S.v, i + 1, j), A.C(S.H, i, j)};
+                double masked = A.C(mask, i, j);
+                double rnorm = 1.0;
+                int k = 0;
+                while (k < newton_steps && rnorm > tol) {
+                    double J[25], r[5];
+                    build_local(J, r, x, S, eta, A, P, calving_rate, dx, dt, i, j);
+                    r[0] -= A.U(S.f_u, i, j);
+                    r[1] -= A.U(S.f_u, i, j + 1);
+                    r[2] -= A.V(S.f_v, i, j);
+                    r[3] -= A.V(S.f_v, i + 1, j);
+                    r[4] -= A.C(S.f_H, i, j);
+                    J[0] -= ssa_damping; J[6] -= ssa_damping; J[12] -= ssa_damping; J[18] -= ssa_damping;
+                    J[24] += mc_damping;
+                    if (j == 0) { dirichlet_row(J, 0); r[0] = x[0]; }
+                    if (j == nx - 1) { dirichlet_row(J, 1); r[1] = x[1]; }
+                    if (i == 0) { dirichlet_row(J, 2); r[2] = x[2]; }
+                    if (i == ny - 1) { dirichlet_row(J, 3); r[3] = x[3]; }
+                    if ((x[4] - dt * r[4]) <= thk) {
+                        masked = 1.0;
+                        dirichlet_row(J, 4);
+                        r[4] = x[4] - thk;
+                    } else {
+                        masked = 0.0;
+                    }
+                    double d[5];
+                    lu5(J, r, d);
+                    rnorm = r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3] + r[4] * r[4];
+                    for (int a = 0; a < 5; ++a) x[a] -= relax * d[a];
+                    x[4] = std::fmax(x[4], thk);
+                    ++k;
+                }
+                u[(size_t)i * (nx + 1) + j] = x[0];
+                u[(size_t)i * (nx + 1) + j + 1] = x[1];
+                v[(size_t)i * nx + j] = x[2];
+                v[(size_t)(i + 1) * nx + j] = x[3];
+                H[(size_t)i * nx + j] = x[4];
+                mask[(size_t)i * nx + j] = masked;
+                if (dbg_newton_iters) (*dbg_newton_iters)[(size_t)i * nx + j] = k;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ transpose operator
 // Jᵀλ (minus f when use_forcing), scatter form of glide/cuda/residuals.cu:666-1163.
 void vjp(const Fields& F, const double* lu, const double* lv, const double* lH, double* out_u, double* out_v,
@@ -955,6 +1030,52 @@ void vanka_adjoint_pass(double* lu, double* lv, double* lH, const Fields& F, con
     for (size_t k = 0; k < nh; ++k) lH[k] -= omega * dH[k];
 }
 
+// Multicolour adjoint pass (no counterpart in the reference; the adjoint
+// smoother of the mixed-precision solver): per colour, the residual
+// Jᵀλ - f at the current multipliers, then every cell of the colour solves its
+// transposed, damped and pinned patch system and subtracts omega times the
+// solution from its five multipliers (each unknown belongs to one cell of a
+// colour, so no averaging).
+void mc_adjoint_pass(double* lu, double* lv, double* lH, const Fields& F, const orc_phys& P, double calving_rate,
+                     double dx, double dt, int ny, int nx, double ssa_damping, double mc_damping, double omega) {
+    Acc A{ny, nx};
+    std::vector<double> eta;
+    viscosity_field(A, F.u, F.v, nullptr, nullptr, F.B, P.n, P.eps_reg, dx, eta, nullptr);
+    const size_t nu = (size_t)ny * (nx + 1), nv = (size_t)(ny + 1) * nx, nh = (size_t)ny * nx;
+    std::vector<double> ru(nu), rv(nv), rH(nh);
+    for (int colour = 0; colour < 2; ++colour) {
+        vjp(F, lu, lv, lH, ru.data(), rv.data(), rH.data(), true, P, calving_rate, dx, dt, ny, nx);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < ny; ++i)
+            for (int j = (i + colour) & 1; j < nx; j += 2) {
+                double x[5] = {A.U(F.u, i, j), A.U(F.u, i, j + 1), A.V(F.v, i, j), A.V(F.v, i + 1, j),
+                               A.C(F.H, i, j)};
+                double J[25], rhs[5];
+                build_local(J, rhs, x, F, eta, A, P, calving_rate, dx, dt, i, j);
+                J[0] -= ssa_damping; J[6] -= ssa_damping; J[12] -= ssa_damping; J[18] -= ssa_damping;
+                J[24] += mc_damping;
+                rhs[0] = A.U(ru.data(), i, j); rhs[1] = A.U(ru.data(), i, j + 1);
+                rhs[2] = A.V(rv.data(), i, j); rhs[3] = A.V(rv.data(), i + 1, j);
+                rhs[4] = A.C(rH.data(), i, j);
+                if (j == 0) { dirichlet_row(J, 0); rhs[0] = 0; }
+                if (j == nx - 1) { dirichlet_row(J, 1); rhs[1] = 0; }
+                if (i == 0) { dirichlet_row(J, 2); rhs[2] = 0; }
+                if (i == ny - 1) { dirichlet_row(J, 3); rhs[3] = 0; }
+                if (A.C(F.mask, i, j) > 0.5) { dirichlet_row(J, 4); rhs[4] = 0; }
+                double JT[25];
+                for (int a = 0; a < 5; ++a)
+                    for (int b = 0; b < 5; ++b) JT[a * 5 + b] = J[b * 5 + a];
+                double d[5];
+                lu5(JT, rhs, d);
+                lu[(size_t)i * (nx + 1) + j] -= omega * d[0];
+                lu[(size_t)i * (nx + 1) + j + 1] -= omega * d[1];
+                lv[(size_t)i * nx + j] -= omega * d[2];
+                lv[(size_t)(i + 1) * nx + j] -= omega * d[3];
+                lH[(size_t)i * nx + j] -= omega * d[4];
+            }
+    }
+}
+
 // Parameter gradients (glide/cuda/grad.cu:1-169); the reference's j<nx-1 /
 // i<ny-1 guards are kept.
 void grad_beta(double* g, const Fields& F, const double* lu, const double* lv, const orc_phys& P, int ny, int nx) {
@@ -1131,12 +1252,19 @@ typedef struct orc_fas_config {
     int coarsest_steps, pre_steps, post_steps, finest_steps, maximum_vcycles;
     double relative_tolerance, absolute_tolerance;
     int compat_hfacet;  // 1: v prolongation through the u-facet routine (reference behaviour)
+    // 1: keep the multipliers of pinned rows (domain-edge facets, active-set
+    // cells) at zero in adjoint V-cycles; the reference lets the coarse-grid
+    // correction write into them (0, its behaviour).  Not a reference option.
+    int project_pinned;
 } orc_fas_config;
 
 typedef struct orc_vanka_config {
     double omega, relax_phi;
     int newton_steps;
     double ssa_damping, mc_damping;
+    int multicolour;      // 0: the reference's additive pass; 1: mc_pass / mc_adjoint_pass
+    double relaxation;    // local Newton relaxation of mc_pass (the additive pass fixes 0.5)
+    double newton_tol;    // local stopping threshold on sum r^2 of mc_pass (the additive pass fixes 1e-6)
 } orc_vanka_config;
 
 typedef struct orc_report {
@@ -1213,9 +1341,14 @@ struct Solver {
                 grounded_field(L.f[F_PHI].data(), L.f[F_H].data(), L.f[F_BED].data(), L.P.flotation_reg_driving,
                                V.relax_phi, L.ny * L.nx);
             Fields F = fields_of(L, false);
-            vanka_pass(L.f[F_U].data(), L.f[F_V].data(), L.f[F_H].data(), L.f[F_MASK].data(), F, L.P,
-                       freeze_calving ? 0.0 : L.P.calving_rate, L.dx, dt, L.ny, L.nx, V.newton_steps, V.ssa_damping,
-                       V.mc_damping, V.omega, nullptr);
+            if (V.multicolour)
+                mc_pass(L.f[F_U].data(), L.f[F_V].data(), L.f[F_H].data(), L.f[F_MASK].data(), F, L.P,
+                        freeze_calving ? 0.0 : L.P.calving_rate, L.dx, dt, L.ny, L.nx, V.newton_steps, V.relaxation,
+                        V.newton_tol, V.ssa_damping, V.mc_damping, nullptr);
+            else
+                vanka_pass(L.f[F_U].data(), L.f[F_V].data(), L.f[F_H].data(), L.f[F_MASK].data(), F, L.P,
+                           freeze_calving ? 0.0 : L.P.calving_rate, L.dx, dt, L.ny, L.nx, V.newton_steps,
+                           V.ssa_damping, V.mc_damping, V.omega, nullptr);
         }
     }
 
@@ -1317,12 +1450,33 @@ struct Solver {
         Level& L = mg->L[l];
         const orc_vanka_config& V = vc[l];
         for (int it = 0; it < n_iter; ++it) {
+            if (V.multicolour) {
+                Fields F = fields_of(L, true);
+                mc_adjoint_pass(L.f[F_LU].data(), L.f[F_LV].data(), L.f[F_LH].data(), F, L.P, L.P.calving_rate,
+                                L.dx, dt, L.ny, L.nx, V.ssa_damping, V.mc_damping, V.omega);
+                continue;
+            }
             adj_residual(l, true, false, L.f[F_ARU].data(), L.f[F_ARV].data(), L.f[F_ARH].data(), nullptr);
             Fields F = fields_of(L, true);
             vanka_adjoint_pass(L.f[F_LU].data(), L.f[F_LV].data(), L.f[F_LH].data(), F, L.f[F_ARU].data(),
                                L.f[F_ARV].data(), L.f[F_ARH].data(), L.P, L.P.calving_rate, L.dx, dt, L.ny, L.nx,
                                V.ssa_damping, V.mc_damping, V.omega);
         }
+    }
+
+    // Zero the multipliers the transpose operator leaves undetermined.
+    void pin_multipliers(Level& L) {
+        if (!cfg.project_pinned) return;
+        for (int i = 0; i < L.ny; ++i) {
+            L.f[F_LU][(size_t)i * (L.nx + 1)] = 0.0;
+            L.f[F_LU][(size_t)i * (L.nx + 1) + L.nx] = 0.0;
+        }
+        for (int j = 0; j < L.nx; ++j) {
+            L.f[F_LV][j] = 0.0;
+            L.f[F_LV][(size_t)L.ny * L.nx + j] = 0.0;
+        }
+        for (size_t k = 0; k < L.size(K_C); ++k)
+            if (L.f[F_MASK][k] > 0.5) L.f[F_LH][k] = 0.0;
     }
 
     // FASAdjointSolver.vcycle (glide/multigrid.py:905-973)
@@ -1333,6 +1487,7 @@ struct Solver {
         adj_sweep(l, cfg.pre_steps);
         for (int id : {F_U, F_V, F_H, F_HPREV, F_PHI, F_MASK}) restrict_field(id, L, N);
         for (int id : {F_LU, F_LV, F_LH}) restrict_field(id, L, N);
+        pin_multipliers(N);
         N.wl_u = N.f[F_LU]; N.wl_v = N.f[F_LV]; N.wl_H = N.f[F_LH];
         adj_residual(l, true, false, L.f[F_ARU].data(), L.f[F_ARV].data(), L.f[F_ARH].data(), nullptr);
         for (int id : {F_ARU, F_ARV, F_ARH}) restrict_field(id, L, N);
@@ -1350,6 +1505,7 @@ struct Solver {
         for (size_t k = 0; k < L.size(K_U); ++k) L.f[F_LU][k] += L.zl_u[k];
         for (size_t k = 0; k < L.size(K_V); ++k) L.f[F_LV][k] += L.zl_v[k];
         for (size_t k = 0; k < L.size(K_C); ++k) L.f[F_LH][k] += L.zl_H[k];
+        pin_multipliers(L);
         adj_sweep(l, cfg.post_steps);
         if (finest) adj_sweep(l, cfg.finest_steps);
     }
@@ -1492,6 +1648,27 @@ void orc_vanka_sweep(double* u, double* v, double* H, double* mask, const double
     vanka_pass(u, v, H, mask, F, *P, P->calving_rate, dx, dt, ny, nx, newton_steps, ssa_damping, mc_damping, omega,
                &it);
     if (newton_iters) std::memcpy(newton_iters, it.data(), nh * sizeof(double));
+}
+
+void orc_mc_sweep(double* u, double* v, double* H, double* mask, const double* phi, const double* f_u, const double* f_v,
+                  const double* f_H, const double* bed, const double* B, const double* beta, const double* gamma,
+                  const orc_phys* P, double dx, double dt, int ny, int nx, int newton_steps, double relax, double tol,
+                  double ssa_damping, double mc_damping, double* newton_iters) {
+    const size_t nh = (size_t)ny * nx;
+    Fields F{nullptr, nullptr, nullptr, phi, mask, f_u, f_v, f_H, bed, B, beta, gamma};
+    std::vector<double> it(nh);
+    mc_pass(u, v, H, mask, F, *P, P->calving_rate, dx, dt, ny, nx, newton_steps, relax, tol, ssa_damping, mc_damping,
+            &it);
+    if (newton_iters) std::memcpy(newton_iters, it.data(), nh * sizeof(double));
+}
+
+void orc_mc_adjoint_sweep(double* lu, double* lv, double* lH, const double* u, const double* v, const double* H,
+                          const double* phi, const double* mask, const double* af_u, const double* af_v,
+                          const double* af_H, const double* bed, const double* B, const double* beta,
+                          const orc_phys* P, double dx, double dt, int ny, int nx, double ssa_damping,
+                          double mc_damping, double omega) {
+    Fields F{u, v, H, phi, mask, af_u, af_v, af_H, bed, B, beta, nullptr};
+    mc_adjoint_pass(lu, lv, lH, F, *P, P->calving_rate, dx, dt, ny, nx, ssa_damping, mc_damping, omega);
 }
 
 void orc_vjp(double* out_u, double* out_v, double* out_H, const double* u, const double* v, const double* H,
