@@ -1,29 +1,49 @@
 #!/usr/bin/env python3
-"""Benchmark of the coupled SSA step on one MI355X.
+"""Benchmark of the coupled SSA step on one MI355X: the mixed-precision form
+against the plain FP64 form at equal error.
 
 Workload: the dome on a Gaussian-random-field bed at 512 x 512 cells,
 dx = 1500 m, 5 multigrid levels, dt = 10 a (glide_amd.synthetic), coupled to
 the enthalpy solve on its 512 x 512 columns with 41 sigma levels (surface
 temperature -30 C + 6.5e-3 K/m (2000 m - s), geothermal flux 0.05 W/m^2,
-strain heating from the SSA velocities, frictional heating, rate factor fed
-back to the momentum solver).  One step is a forward FAS-CD solve of exactly
-3 V-cycles (advancing the geometry), the thermal step (column Newton sweeps
-with the device-side stopping test), the adjoint FAS solve of exactly 3
-V-cycles for the loss 0.5 * |u|^2 + 0.5 * |v|^2, and the four parameter
-gradients.  Inputs live in HBM before timing starts.
+strain and frictional heating, rate factor fed back to the momentum solver).
+
+One step: the forward FAS-CD solve (advancing the geometry), the thermal step,
+the enthalpy adjoint, the momentum adjoint for the loss
+0.5 |u|^2 + 0.5 |v|^2 and the parameter gradients.  Inputs live in HBM before
+timing starts.  The two forms:
+
+* fp64  - the reference's algorithm in FP64 (additive Vanka smoother with its
+          default counts) for the forward and the adjoint solves;
+* mixed - glide_amd.mixed: FP32 multigrid with red-black smoothing inside an
+          FP64 defect-correction loop (FP64 residual, coarsest level and
+          accumulation; FP64 red-black V-cycles finish the forward solve).
+
+Both forms share the FP64 enthalpy solve.  Equal error: before timing, each
+form's stopping tolerances (forward, enthalpy, adjoint) are set to the loosest
+values of a ladder for which one step from the warmed-up state lands within
+--target (relative L2) of the truth: the same step with FP64 solves driven to
+the residual floor (forward: FP64 red-black V-cycles to 3e-13; adjoint: FP64
+reference-form V-cycles to 1e-14; enthalpy sweeps to 1e-13).  After
+timing, every timed step is replayed from the saved state (the solves are
+deterministic) and its u, v, H, E, multipliers and basal-drag gradient are
+compared with the truth of that step; the line reports the largest errors.
 
 Prints one JSON line.  Besides the contract fields it carries
-  roofline     - the dominant kernel's algorithmic bytes / measured duration
-                 (HIP events around each launch over a second run of the same
-                 number of steps, launched eagerly) vs HBM peak
-  cpu_baseline - the FP64 oracle on the host cores for a bounded sample of the
-                 same workload
+  fp64         - the FP64 form's rate, tolerances and errors at the same target
+  roofline     - the dominant kernel of the mixed step: algorithmic bytes per
+                 launch / its mean duration (HIP events around each launch in
+                 an eager replay of the timed steps) vs HBM peak
+  cpu_baseline - the FP64 oracle on the host cores: one forward and one
+                 adjoint V-cycle at the full 512 x 512 size, scaled by the
+                 V-cycle counts the FP64 form needed, plus one thermal step
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -33,19 +53,32 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "forward+adjoint steps/sec at 512² MAC grid, mixed vs FP64 at equal 1e-8 error"
 HBM_PEAK_GBS = 8000.0
-FP64_VALU_PEAK_TFS = 78.6  # MI355X vector FP64
-WORD = 8  # bytes per FP64 value
-
-# Algorithmic bytes per cell of one launch (each field counted once at 8 B):
-#   forward smoothing pass (patch solve + increment apply): 12 inputs + 4 outputs
-#   transpose operator: 15 inputs + 3 outputs
-#   adjoint smoothing pass: 12 inputs + 3 outputs
-#   residual: 12 inputs + 3 outputs
-#   enthalpy column / layer smoother, per column of nz nodes: E, E_prev, f_E,
-#   omega in and E out (5 nz) + five 2-D fields (u, v, H, H_prev, E_surface);
-#   enthalpy residual + max-norm: 4 nz + 5 per column (set in main from nz)
-ALG_FIELDS = {"vanka_sweep": 16, "vjp": 18, "adjoint_sweep": 15, "residual": 15}
 SEC_PER_YR = 365.25 * 86400.0
+FORWARD_LADDER = (1e-8, 3e-9, 1e-9, 3e-10, 1e-10, 3e-11, 1e-11, 3e-12, 1e-12, 3e-13)
+ENTHALPY_LADDER = (1e-5, 1e-6, 1e-7, 1e-8, 1e-9, 1e-10, 1e-11, 1e-12)
+ADJOINT_LADDER = FORWARD_LADDER
+ENTHALPY_SWEEPS = 100  # cap of the column sweeps per thermal step (the tolerance stops them)
+TRUTH_FORWARD = 3e-13  # the truth's forward and enthalpy stopping tests (near the FP64 floors)
+TRUTH_ENTHALPY = 1e-13
+TRUTH_ADJOINT = 1e-14
+CALIBRATION_MARGIN = 0.5  # the calibration step must land within this fraction of the target
+
+# Algorithmic bytes of one launch per cell of its level, in values of the
+# launch's precision (each plane counted once):
+#   red-black colour phase: reads u, v, H, the frozen viscosity, phi, beta,
+#     bed, f_u, f_v (every cell), f_H, gamma and the active-set flag (the
+#     solving half); writes u, v, H (every cell) and the flag (the solving
+#     half): 10.5 + 3.5
+#   red-black adjoint colour phase: the transpose operator reads the three
+#     multiplier planes, the three right-hand sides, the cached viscosity and
+#     u, v, H, phi, beta, bed and the flag; the solving half reads its 25 LU
+#     values; writes the three multiplier planes: 16 + 12.5 + 3
+#   FP64 residual: 12 inputs + 3 outputs
+ALG_VALUES = {"vanka_mc32": 14.0, "vanka_mc64": 14.0, "adjoint_mc32": 31.5, "adjoint_mc64": 31.5,
+              "residual": 15.0}
+WORD = {"vanka_mc32": 4, "vanka_mc64": 8, "adjoint_mc32": 4, "adjoint_mc64": 8, "residual": 8}
+TIMED = ("vanka_mc32", "vanka_mc64", "adjoint_mc32", "adjoint_mc64", "residual", "vanka_cell", "vanka_apply",
+         "adjoint_cell", "adjoint_apply", "vjp", "adjoint_factor", "enth_column", "enth_layer", "enth_residual")
 
 
 def parse():
@@ -55,64 +88,118 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--levels", type=int, default=5)
-    ap.add_argument("--vcycles", type=int, default=3)
     ap.add_argument("--dt", type=float, default=10.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--nz", type=int, default=41, help="sigma levels of the enthalpy columns (0: no thermal step)")
-    ap.add_argument("--n-smooth", type=int, default=10, help="maximum enthalpy sweeps per step")
-    ap.add_argument("--enth-adjoint-rtol", type=float, default=1e-8,
-                    help="stopping test of the enthalpy adjoint sweeps (relative to the first residual)")
-    ap.add_argument("--serial-enthalpy", dest="concurrent", action="store_false",
-                    help="run the enthalpy step and its adjoint on the momentum stream (no overlap)")
+    ap.add_argument("--target", type=float, default=1e-8, help="relative L2 error bound against the truth")
+    ap.add_argument("--no-fp64", dest="fp64", action="store_false", help="skip the FP64 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-mixed", dest="mixed", action="store_false",
-                    help="skip the mixed-precision run (FP32 patch Jacobian) and its error against FP64")
-    ap.add_argument("--cpu-sample-n", type=int, default=128,
-                    help="grid side of the oracle sample (same level count and cycle counts)")
+    ap.add_argument("--verbose", action="store_true", help="calibration progress on stderr")
     return ap.parse_args()
 
 
-def build_model(args, torch):
-    from glide_amd import IceDynamics, ThermalModel
-    from glide_amd.synthetic import dome_problem, install, thermal_forcing
-    p = dome_problem(args.n, 1500.0, seed=args.seed)
-    model = IceDynamics(ny=args.n, nx=args.n, dx=1500.0, n_levels=args.levels, device="cuda")
-    install(model.mg, p)
-    for s in (model.forward_solver, model.adjoint_solver):
-        s.fas_options.set(maximum_vcycles=args.vcycles, relative_tolerance=0.0, absolute_tolerance=0.0,
-                          report_norms=False, asynchronous=True)
-    thermal = None
-    if args.nz > 0:
-        thermal = ThermalModel(model.mg[0], nz=args.nz, n_smooth=args.n_smooth, strain_heating=True)
-        Ts, qg = thermal_forcing(p)
-        thermal.initialize(torch.as_tensor(Ts, device="cuda"), Q_geo=torch.as_tensor(qg, device="cuda"))
-    return model, thermal
+def log(args, msg):
+    if args.verbose:
+        print(msg, file=sys.stderr, flush=True)
 
 
-def run_timed(stepper, args, torch):
-    """Warm-up steps, then the timed steps on the production path (fixed-cycle
-    solves replayed from the library's cached HIP graphs, enthalpy work on its
-    own stream); returns (t, seconds)."""
-    t = 0.0
-    for _ in range(args.warmup):
-        stepper(t, args.dt, concurrent=args.concurrent)
-        t += args.dt
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stepper(t, args.dt, concurrent=args.concurrent)
-        t += args.dt
-    torch.cuda.synchronize()
-    return t, time.perf_counter() - t0
+# ------------------------------------------------------------------ the forms
+class Form:
+    """One model of the workload with its solve settings."""
 
+    def __init__(self, args, torch, kind):
+        from glide_amd import IceDynamics, ThermalModel
+        from glide_amd.synthetic import dome_problem, install, thermal_forcing
+        self.args, self.torch, self.kind = args, torch, kind
+        p = dome_problem(args.n, 1500.0, seed=args.seed)
+        prec = "mixed" if kind == "mixed" else "fp64"
+        self.model = IceDynamics(ny=args.n, nx=args.n, dx=1500.0, n_levels=args.levels, device="cuda",
+                                 precision=prec)
+        install(self.model.mg, p)
+        self.thermal = None
+        if args.nz > 0:
+            self.thermal = ThermalModel(self.model.mg[0], nz=args.nz, n_smooth=10, strain_heating=True)
+            Ts, qg = thermal_forcing(p)
+            self.thermal.initialize(torch.as_tensor(Ts, device="cuda"), Q_geo=torch.as_tensor(qg, device="cuda"))
+        fs, ad = self.model.forward_solver, self.model.adjoint_solver
+        for s in (fs, ad):
+            s.fas_options.set(maximum_vcycles=64, absolute_tolerance=0.0, report_norms=False)
+            s.mixed_options.max_outer = 64
+            s.mixed_options.absolute_tolerance = 0.0
+        if kind == "truth":
+            self.set_tolerances(TRUTH_FORWARD, TRUTH_ENTHALPY, TRUTH_ADJOINT)
+        else:
+            self.set_tolerances(1e-8, 1e-6, 1e-8)
+        from glide_amd.coupled import CoupledStep
+        self.stepper = CoupledStep(self.model, self.thermal, enthalpy_adjoint_rtol=1e-10,
+                                   forward=(lambda t, dt: self._truth_forward(dt)) if kind == "truth" else None)
 
-def snapshot(model, thermal):
-    g = model.mg[0]
-    out = {"u": g.state.u.data, "v": g.state.v.data, "H": g.state.H.data, "lambda_u": g.adjoint.lambda_u.data,
-           "lambda_v": g.adjoint.lambda_v.data, "lambda_H": g.adjoint.lambda_H.data, "grad_beta": g.sliding.beta.grad}
-    if thermal is not None:
-        out["E"] = thermal.ops.enthalpy_state.E
-    return {k: v.detach().clone() for k, v in out.items()}
+    def set_tolerances(self, fwd, enth, adj):
+        self.tol = {"forward": fwd, "enthalpy": enth, "adjoint": adj}
+        fs, ad = self.model.forward_solver, self.model.adjoint_solver
+        fs.fas_options.set(relative_tolerance=fwd)
+        fs.mixed_options.relative_tolerance = fwd
+        ad.fas_options.set(relative_tolerance=adj)
+        ad.mixed_options.relative_tolerance = adj
+        if self.thermal is not None:
+            sc = self.thermal.ops.smoother_config
+            sc.relative_tolerance = enth
+            sc.absolute_tolerance = 0.0
+            self.thermal.n_smooth = ENTHALPY_SWEEPS
+
+    def _truth_forward(self, dt):
+        """FP64 to the residual floor: reference-form V-cycles (robust from any
+        state) to 1e-6, then FP64 red-black V-cycles (the fastest FP64 route
+        to the floor) to TRUTH_FORWARD of the first residual."""
+        fs = self.model.forward_solver
+        fs.vanka_options.set(multicolour=False, newton_tol=1e-6)
+        fs.vanka_options.newton_options.set(steps=30, relaxation=0.5, ssa_damping=0.01, mc_damping=1.0)
+        fs.fas_options.set(relative_tolerance=1e-6, absolute_tolerance=0.0, coarsest_steps=200, pre_steps=10,
+                           post_steps=20, finest_steps=50, compat_vprolong=True)
+        rep = fs.solve(dt)
+        fs.vanka_options.set(multicolour=True, newton_tol=0.0)
+        fs.vanka_options.newton_options.set(steps=3, relaxation=0.5, ssa_damping=0.0, mc_damping=0.0)
+        fs.fas_options.set(relative_tolerance=0.0, absolute_tolerance=TRUTH_FORWARD * rep.initial_norm,
+                           coarsest_steps=40,
+                           pre_steps=4, post_steps=4, finest_steps=8, compat_vprolong=False)
+        fs.solve(dt)
+        top = self.model.mg[0]
+        top.state.H_prev.data.copy_(top.state.H.data)
+
+    def step(self, t, dt):
+        """One coupled step (glide_amd.coupled.CoupledStep: the enthalpy step
+        and its adjoint, seeded with dL/dE = E and dL/dB = 1, on a second
+        stream beside the momentum adjoint)."""
+        self.stepper(t, dt)
+
+    def tensors(self):
+        g = self.model.mg[0]
+        d = {"u": g.state.u.data, "v": g.state.v.data, "H": g.state.H.data, "H_prev": g.state.H_prev.data,
+             "mask": g.state.mask.data, "phi": g.state.phi.data, "B": g.rheology.B.data,
+             "lambda_u": g.adjoint.lambda_u.data, "lambda_v": g.adjoint.lambda_v.data,
+             "lambda_H": g.adjoint.lambda_H.data}
+        if self.thermal is not None:
+            ops = self.thermal.ops
+            d.update(E=ops.enthalpy_state.E, E_prev=ops.enthalpy_state.E_prev, H_prev_thermal=ops.H_prev)
+        return d
+
+    def snapshot(self):
+        return {k: v.detach().clone() for k, v in self.tensors().items()}
+
+    def restore(self, snap):
+        for k, v in self.tensors().items():
+            v.copy_(snap[k])
+
+    def results(self):
+        """What the error report compares (multipliers of the rows the
+        transpose operator does not determine are zeroed by the caller)."""
+        g = self.model.mg[0]
+        out = {"u": g.state.u.data, "v": g.state.v.data, "H": g.state.H.data, "lambda_u": g.adjoint.lambda_u.data,
+               "lambda_v": g.adjoint.lambda_v.data, "lambda_H": g.adjoint.lambda_H.data,
+               "grad_beta": g.sliding.beta.grad, "mask": g.state.mask.data}
+        if self.thermal is not None:
+            out["E"] = self.thermal.ops.enthalpy_state.E
+        return {k: v.detach().clone() for k, v in out.items()}
 
 
 def rel_l2(a, b):
@@ -120,12 +207,164 @@ def rel_l2(a, b):
     return float((a - b).norm()) / den if den > 0 else float((a - b).norm())
 
 
-def read_traffic(args, kernel):
-    """HBM bytes per finest-level launch of the dominant pass from the committed
-    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated); None when the
-    summary does not cover this workload."""
-    path = ROOT / "profiles" / "vanka_pass_512_hbm_traffic.md"
-    if kernel != "vanka_sweep" or args.n != 512 or not path.exists():
+def errors(res, truth):
+    """Relative L2 errors against the truth; the multipliers compared on the
+    rows the transpose operator determines (not the domain-edge facets nor the
+    active-set cells of the truth)."""
+    e = {k: rel_l2(res[k], truth[k]) for k in ("u", "v", "H", "grad_beta") + (("E",) if "E" in truth else ())}
+    keep_H = truth["mask"] <= 0.5
+
+    def pinned(d):
+        lu, lv, lH = d["lambda_u"].clone(), d["lambda_v"].clone(), d["lambda_H"].clone()
+        lu[:, 0] = 0.0
+        lu[:, -1] = 0.0
+        lv[0, :] = 0.0
+        lv[-1, :] = 0.0
+        lH = lH * keep_H
+        return lu, lv, lH
+
+    for k, a, b in zip(("lambda_u", "lambda_v", "lambda_H"), pinned(res), pinned(truth)):
+        e[k] = rel_l2(a, b)
+    return e
+
+
+FORWARD_KEYS = ("u", "v", "H")
+ADJOINT_KEYS = ("lambda_u", "lambda_v", "lambda_H", "grad_beta")
+
+
+def calibrate(form, truth_form, snap, t, args):
+    """The loosest tolerances of the ladders for which one step from `snap`
+    is within CALIBRATION_MARGIN x args.target of the truth in every compared
+    quantity (the margin covers the step-to-step spread; the
+    multipliers and gradients inherit the forward solution's error, so the
+    forward tolerance is chosen against all of them): first the forward
+    tolerance with tight enthalpy and adjoint solves, then the adjoint
+    tolerance, then the enthalpy tolerance."""
+    truth_form.restore(snap)
+    truth_form.step(t, args.dt)
+    T = truth_form.results()
+    tight = ENTHALPY_LADDER[-1], ADJOINT_LADDER[-1]
+    bound = CALIBRATION_MARGIN * args.target
+
+    def trial(fwd, enth, adj):
+        form.set_tolerances(fwd, enth, adj)
+        form.restore(snap)
+        form.step(t, args.dt)
+        e = errors(form.results(), T)
+        log(args, f"  {form.kind} forward {fwd:.0e} enthalpy {enth:.0e} adjoint {adj:.0e}: "
+                  + " ".join(f"{k} {v:.1e}" for k, v in e.items()))
+        return max(e.values()) <= bound
+
+    def loosest(ladder, make):
+        for v in ladder:
+            if trial(*make(v)):
+                return v, True
+        return ladder[-1], False
+
+    fwd, ok = loosest(FORWARD_LADDER, lambda v: (v, tight[0], tight[1]))
+    adj, enth = tight[1], tight[0]
+    if ok:
+        adj, _ = loosest(ADJOINT_LADDER, lambda v: (fwd, tight[0], v))
+        if form.thermal is not None:
+            enth, _ = loosest(ENTHALPY_LADDER, lambda v: (fwd, v, adj))
+    form.set_tolerances(fwd, enth, adj)
+    form.restore(snap)
+    return {"forward": fwd, "enthalpy": enth, "adjoint": adj}, ok
+
+
+def run_form(form, truth_form, args, torch):
+    """Warm-up, calibration, timed steps, then the replayed error check."""
+    t = 0.0
+    for _ in range(args.warmup):
+        form.step(t, args.dt)
+        t += args.dt
+    torch.cuda.synchronize()
+    snap = form.snapshot()
+    tols, calibrated = calibrate(form, truth_form, snap, t, args)
+    log(args, f"{form.kind}: tolerances {tols} (met on the calibration step: {calibrated})")
+    # one untimed step with the chosen tolerances (graphs of its solves recorded)
+    form.step(t, args.dt)
+    form.restore(snap)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tt = t
+    for _ in range(args.steps):
+        form.step(tt, args.dt)
+        tt += args.dt
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = _solver_stats(form)
+    # replay: the same steps from the same state, each against its truth
+    form.restore(snap)
+    worst = {}
+    tt = t
+    for _ in range(args.steps):
+        before = form.snapshot()
+        form.step(tt, args.dt)
+        truth_form.restore(before)
+        truth_form.step(tt, args.dt)
+        e = errors(form.results(), truth_form.results())
+        for k, v in e.items():
+            worst[k] = max(worst.get(k, 0.0), v)
+        tt += args.dt
+    torch.cuda.synchronize()
+    return {"elapsed": elapsed, "tolerances": tols, "calibrated": calibrated, "errors": worst, "snap": snap,
+            "t": t, "stats": stats}
+
+
+def _solver_stats(form):
+    fs, ad = form.model.forward_solver, form.model.adjoint_solver
+    out = {}
+    for name, s in (("forward", fs), ("adjoint", ad)):
+        rep = getattr(s, "last_report", None)
+        if rep is not None:
+            out[name + "_iterations"] = int(rep.vcycles)
+        if getattr(s, "last_stats", None):
+            out[name + "_phases"] = dict(s.last_stats)
+    return out
+
+
+# -------------------------------------------------------------- measurement
+def kernel_times(form, res, args, torch, _lib):
+    """Replay the timed steps eagerly with HIP events around every hot launch
+    (events inside graphs cannot be timed; kernel bodies are identical), the
+    enthalpy work in turn rather than beside the momentum adjoint so that no
+    timed kernel shares the GPU."""
+    form.restore(res["snap"])
+    torch.cuda.synchronize()
+    form.stepper.concurrent = False
+    _lib.timing_enable(True)
+    tt = res["t"]
+    for _ in range(args.steps):
+        form.step(tt, args.dt)
+        tt += args.dt
+    torch.cuda.synchronize()
+    form.stepper.concurrent = True
+    fine = args.n * args.n
+    tot = {k: _lib.timing_read(k) for k in TIMED}
+    fine_t = {k: _lib.timing_read_units(k, fine) for k in TIMED}
+    _lib.timing_enable(False)
+    return tot, fine_t
+
+
+def roofline_of(tot, fine_t, args):
+    cands = [k for k in ALG_VALUES if tot[k][0] > 0]
+    dominant = max(cands, key=lambda k: tot[k][1])
+    launches, ms = fine_t[dominant]
+    avg_ms = ms / max(launches, 1)
+    alg = ALG_VALUES[dominant] * WORD[dominant] * args.n * args.n
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(dominant, args),
+            "kernel": dominant, "level": f"{args.n}x{args.n}", "launches": launches,
+            "avg_launch_us": round(1000.0 * avg_ms, 3), "alg_bytes_per_launch": int(alg)}
+
+
+def read_traffic(kernel, args):
+    """HBM bytes per finest-level launch from the committed FETCH_SIZE /
+    WRITE_SIZE summary of this kernel (copy-kernel calibrated), if any."""
+    path = ROOT / "profiles" / f"{kernel}_{args.n}_hbm_traffic.md"
+    if not path.exists():
         return None
     for line in path.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
@@ -133,94 +372,38 @@ def read_traffic(args, kernel):
     return None
 
 
-def read_fp64_flops(args):
-    """FP64 operations per finest-level forward patch-solve launch from the
-    committed VALU instruction-count summary (64 lanes x (add + mul + trans +
-    2 fma) per wave instruction, masked lanes included); None when absent."""
-    path = ROOT / "profiles" / "vanka_cell_512_fp64_valu.md"
-    if args.n != 512 or not path.exists():
-        return None
-    for line in path.read_text().splitlines():
-        if line.startswith("fp64_flops_per_launch:"):
-            return float(line.split(":")[1])
-    return None
+def hbm_fractions(fine_t, args):
+    """Algorithmic HBM rate of every kernel with a byte model, finest level."""
+    out = {}
+    for k in ALG_VALUES:
+        launches, ms = fine_t[k]
+        if launches > 0 and ms > 0:
+            gbs = ALG_VALUES[k] * WORD[k] * args.n * args.n / (ms / launches * 1e-3) / 1e9
+            out[k] = {"launches": launches, "avg_us": round(1000.0 * ms / launches, 3), "alg_GBps": round(gbs, 1),
+                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
 
 
-class Stepper:
-    """One coupled step: momentum + mass forward solve, enthalpy step and its
-    adjoint, momentum adjoint + gradients, then the rate-factor feedback.
-
-    The enthalpy step and its adjoint depend only on the forward solution, and
-    the momentum adjoint only on the forward state and the rate factor the
-    forward used, so with ``concurrent`` the enthalpy work runs on a second
-    stream beside the momentum adjoint; the new rate factor is written to the
-    grid after both.  The step runs on a stream of its own rather than the
-    null stream, whose implicit synchronisation with blocking streams could
-    serialise the two."""
-
-    def __init__(self, model, thermal, torch, adj_rtol=1e-8):
-        self.model, self.thermal, self.torch = model, thermal, torch
-        self.adj_rtol = adj_rtol
-        dev = model.mg[0].state.u.data.device
-        self.main = torch.cuda.Stream(device=dev)
-        if thermal is not None:
-            g = model.mg[0]
-            self.stream = torch.cuda.Stream(device=dev)
-            self.B_next = torch.empty_like(g.rheology.B.data)
-            self.dL_dB = torch.ones_like(g.rheology.B.data)
-            self.ev_fwd, self.ev_thermal = torch.cuda.Event(), torch.cuda.Event()
-        self.adjoint_sweeps = None
-
-    def _thermal(self, dt, report):
-        th = self.thermal
-        th.step(dt * SEC_PER_YR, B_out=self.B_next)
-        # objective 0.5 |E|^2 + sum(B): the seeds only set the right-hand side;
-        # a report synchronises the host, so only the sequential steps ask for one
-        g = self.model.mg[0]
-        res = th.ops.adjoint_solve(dt * SEC_PER_YR, dL_dE=th.ops.enthalpy_state.E, dL_dB=self.dL_dB,
-                                   B_scale=th.B_scale, report=report, relative_tolerance=self.adj_rtol,
-                                   n_iter=64, u2d=g.state.u.data, v2d=g.state.v.data, vel_div=SEC_PER_YR)
-        if report:
-            self.adjoint_sweeps = res["report"].sweeps
-
-    def __call__(self, t, dt, concurrent=True, report=False):
-        caller = self.torch.cuda.current_stream()
-        self.main.wait_stream(caller)
-        with self.torch.cuda.stream(self.main):
-            self._step(t, dt, concurrent, report)
-        caller.wait_stream(self.main)
-
-    def _step(self, t, dt, concurrent, report):
-        torch, model, thermal = self.torch, self.model, self.thermal
-        if thermal is not None:
-            thermal.pre_momentum()
-        model.forward(t, dt, update_geometry=True)
-        g = model.mg[0]
-        main = torch.cuda.current_stream()
-        if thermal is not None:
-            if concurrent:
-                self.ev_fwd.record(main)
-                self.stream.wait_event(self.ev_fwd)
-                with torch.cuda.stream(self.stream):
-                    self._thermal(dt, report)
-                    self.ev_thermal.record(self.stream)
-            else:
-                self._thermal(dt, report)
-        model.backward(t, dt, dJdu=g.state.u.data, dJdv=g.state.v.data, dJdH=None)
-        if thermal is not None:
-            if concurrent:
-                main.wait_event(self.ev_thermal)
-            g.rheology.B.data.copy_(self.B_next)
+def cpu_info():
+    model = None
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"os_cpu_count": os.cpu_count(), "lscpu_model": model, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
-def cpu_baseline(args):
-    """FP64 oracle on the host: one forward and one adjoint V-cycle of the same
-    dome workload at cpu_sample_n^2 with the same level count, scaled to a full
-    step (x vcycles) and to 512^2 by the cell ratio."""
+def cpu_baseline(args, cycles):
+    """The FP64 oracle on the host cores at the full size: one forward and one
+    adjoint V-cycle (the reference's counts) and one thermal step, scaled by
+    the V-cycle counts the FP64 form needed per step on the GPU."""
     import numpy as np
-    from glide_amd.synthetic import dome_problem
+    from glide_amd.synthetic import dome_problem, thermal_forcing
     from oracle import oracle as O
-    n = args.cpu_sample_n
+    n = args.n
     p = dome_problem(n, 1500.0, seed=args.seed)
     om = O.OracleMG(n, n, 1500.0, args.levels)
     for k in ("bed", "B", "beta", "smb"):
@@ -238,49 +421,28 @@ def cpu_baseline(args):
     om.fas_adjoint_solve(args.dt, cfg)
     om.gradients(args.dt)
     t_adj = time.perf_counter() - t0
-    t_th = t_tha = 0.0
+    t_th = 0.0
     if args.nz > 0:
-        from glide_amd.synthetic import thermal_forcing
         Ts, qg = thermal_forcing(p)
         th = O.OracleThermal(n, n, args.nz, 1500.0, p.H)
         th.initialize_from_temperature(Ts)
         th.set_surface_enthalpy_from_temperature(Ts)
         th.Q_geo = qg
-        B = om.get("B")
         u, v = om.get("u"), om.get("v")
-        phi = np.repeat(O.strain_heating(u, v, B, 3.0, 1e-6, 1500.0)[:, :, None], args.nz, axis=2)
+        phi = np.repeat(O.strain_heating(u, v, om.get("B"), 3.0, 1e-6, 1500.0)[:, :, None], args.nz, axis=2)
         t0 = time.perf_counter()
-        O.thermal_step(th, u, v, om.get("H"), p.beta, args.dt * SEC_PER_YR, n_smooth=args.n_smooth,
-                       strain_heating=phi)
+        O.thermal_step(th, u, v, om.get("H"), p.beta, args.dt * SEC_PER_YR, n_smooth=10, strain_heating=phi)
         t_th = time.perf_counter() - t0
-        # the enthalpy adjoint with the GPU step's seeds (dL/dE = E, dL/dB = 1):
-        # the oracle solves J^T lambda = g directly (sparse LU), whose fill-in
-        # grows faster than the cell count, so it is timed on a na^2 sample and
-        # scaled by the cell ratio (which favours the CPU)
-        na = min(n, 24)
-        pa = dome_problem(na, 1500.0, seed=args.seed)
-        Tsa, qga = thermal_forcing(pa)
-        tha = O.OracleThermal(na, na, args.nz, 1500.0, pa.H)
-        tha.initialize_from_temperature(Tsa)
-        tha.set_surface_enthalpy_from_temperature(Tsa)
-        tha.Q_geo = qga
-        O.thermal_step(tha, np.zeros((na, na + 1)), np.zeros((na + 1, na)), pa.H, pa.beta, args.dt * SEC_PER_YR,
-                       n_smooth=args.n_smooth)
-        t0 = time.perf_counter()
-        bs = 917.0 * 9.81 * SEC_PER_YR ** (1.0 / 3.0)
-        gE = tha.E + tha.arrhenius_vjp(np.ones((na, na)), 3.0, bs)
-        lam = tha.adjoint_solve(args.dt * SEC_PER_YR, gE)
-        tha.adjoint_gradients(args.dt * SEC_PER_YR, lam)
-        t_tha = (time.perf_counter() - t0) * (n / na) ** 2
-    scale = (args.n / n) ** 2
-    step_s = (args.vcycles * (t_fwd + t_adj) + t_th + t_tha) * scale
-    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": (f"FP64 oracle, 1 forward + 1 adjoint V-cycle of the dome workload at {n}^2 "
-                       f"({args.levels} levels): {t_fwd:.2f} s + {t_adj:.2f} s, x{args.vcycles} V-cycles per step; "
-                       f"one thermal step on {n}^2 x {args.nz} columns: {t_th:.2f} s, its adjoint "
-                       f"(sparse direct, 24^2 sample scaled to {n}^2): {t_tha:.2f} s; "
-                       f"x{scale:g} cell ratio to {args.n}^2"),
-            "sample_seconds": round(t_fwd + t_adj + t_th + t_tha, 3)}
+    nf, na = cycles
+    step_s = nf * t_fwd + na * t_adj + t_th
+    info = cpu_info()
+    return {"value": round(1.0 / step_s, 6), "unit": "steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": (f"FP64 oracle at {n}x{n} ({args.levels} levels, reference counts): one forward V-cycle "
+                       f"{t_fwd:.2f} s, one adjoint V-cycle + gradients {t_adj:.2f} s, one thermal step on "
+                       f"{n}x{n}x{args.nz} columns {t_th:.2f} s; a step = {nf} forward + {na} adjoint V-cycles "
+                       f"(the FP64 form's counts at equal error on the GPU) + the thermal step; the enthalpy "
+                       f"adjoint is not timed on the host"),
+            "sample_seconds": round(t_fwd + t_adj + t_th, 2), **info}
 
 
 def main():
@@ -295,116 +457,59 @@ def main():
         print("bench.py needs a ROCm device", file=sys.stderr)
         sys.exit(1)
     _lib.load()
-    model, thermal = build_model(args, torch)
-    stepper = Stepper(model, thermal, torch, args.enth_adjoint_rtol)
-    t, elapsed = run_timed(stepper, args, torch)
-    fp64_state = snapshot(model, thermal)
-    mixed = None
-    if args.mixed:
-        # the mixed form (FP32 patch Jacobian/LU in the forward smoother) on a
-        # fresh copy of the workload, same steps; error against the FP64 state
-        m2, th2 = build_model(args, torch)
-        m2.forward_solver.vanka_options.set(local_fp32=True)
-        _, el2 = run_timed(Stepper(m2, th2, torch, args.enth_adjoint_rtol), args, torch)
-        errs = {k: rel_l2(a, b) for (k, a), b in zip(snapshot(m2, th2).items(), fp64_state.values())}
-        mixed = {"value": round(args.steps / el2, 4), "ms_per_step": round(1000.0 * el2 / args.steps, 3),
-                 "speedup_vs_fp64": round(elapsed / el2, 4), "rel_l2_err_vs_fp64": errs,
-                 "form": "forward patch Jacobian and LU in FP32; residuals, stopping tests, updates, transfers, "
-                         "adjoint and enthalpy in FP64"}
-        del m2, th2
-    # per-kernel durations: the same number of steps again with HIP events
-    # around every hot launch (events inside graphs cannot be timed, so these
-    # steps launch eagerly, and on one stream so that kernels do not share the
-    # GPU while timed; kernel bodies are identical)
-    _lib.timing_enable(True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        stepper(t, args.dt, concurrent=False, report=True)
-        t += args.dt
-    torch.cuda.synchronize()
-    elapsed_eager = time.perf_counter() - t1
-    kern = {}
-    for name in ("vanka_cell", "vanka_apply", "adjoint_cell", "adjoint_apply", "vjp", "residual", "enth_column",
-                 "enth_layer", "enth_residual"):
-        kern[name] = _lib.timing_read(name)
-    # finest-level launches only (args.n^2 cells each), read before disabling clears the totals
-    fine = args.n * args.n
-    fine_kern = {name: _lib.timing_read_units(name, fine)
-                 for name in ("vanka_cell", "vanka_apply", "adjoint_cell", "adjoint_apply", "vjp")}
-    _lib.timing_enable(False)
+    truth = Form(args, torch, "truth")
+    mixed = Form(args, torch, "mixed")
+    res_m = run_form(mixed, truth, args, torch)
+    tot, fine_t = kernel_times(mixed, res_m, args, torch, _lib)
+    roofline = roofline_of(tot, fine_t, args)
+    kernels_m = {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in tot.items() if v[0] > 0}
+    fracs = hbm_fractions(fine_t, args)
+    del mixed
 
-    ms_step = 1000.0 * elapsed / args.steps
-    value = args.steps / elapsed
-
-    def bw(fields, ms, cells):
-        return fields * WORD * cells / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-
-    # a smoothing pass is the patch-solve launch plus the increment-apply launch
-    passes = {
-        "vanka_sweep": (kern["vanka_cell"][0], kern["vanka_cell"][1] + kern["vanka_apply"][1], kern["vanka_cell"][2]),
-        "adjoint_sweep": (kern["adjoint_cell"][0], kern["adjoint_cell"][1] + kern["adjoint_apply"][1],
-                          kern["adjoint_cell"][2]),
-        "vjp": kern["vjp"],
-    }
-    dominant = max(passes, key=lambda k: passes[k][1])
-    # roofline of the dominant pass over the finest-level launches (args.n^2 cells each)
-    pair = {"vanka_sweep": ("vanka_cell", "vanka_apply"), "adjoint_sweep": ("adjoint_cell", "adjoint_apply"),
-            "vjp": ("vjp",)}[dominant]
-    f_launches = fine_kern[pair[0]][0]
-    f_ms = sum(fine_kern[k][1] for k in pair)
-    avg_ms = f_ms / max(f_launches, 1)
-    alg_bytes = ALG_FIELDS[dominant] * WORD * fine
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": read_traffic(args, dominant),
-                "kernel": dominant, "level": f"{args.n}x{args.n}", "launches": f_launches,
-                "avg_launch_us": round(1000.0 * avg_ms, 3), "alg_bytes_per_launch": alg_bytes,
-                "all_levels_alg_GBps": round(bw(ALG_FIELDS[dominant], passes[dominant][1], passes[dominant][2]), 2)}
-    # the patch solve's own bound: FP64 VALU issue (per-launch FP64 operations
-    # from the committed counter summary over its measured finest-level duration)
-    flops = read_fp64_flops(args)
-    c_launches, c_ms = fine_kern["vanka_cell"]
-    valu = None
-    if flops is not None and c_launches > 0 and c_ms > 0:
-        tf = flops / (c_ms / c_launches * 1e-3) / 1e12
-        valu = {"bound": "fp64 valu", "kernel": "vanka_cell", "level": f"{args.n}x{args.n}",
-                "achieved": round(tf, 2), "peak": FP64_VALU_PEAK_TFS, "unit": "TFLOP/s",
-                "frac": round(tf / FP64_VALU_PEAK_TFS, 4), "fp64_flops_per_launch": flops,
-                "avg_launch_us": round(1000.0 * c_ms / c_launches, 3)}
-    roofline["valu"] = valu
-    kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "cells": v[2]} for k, v in kern.items()}
-    for k, (c, ms, cells) in passes.items():
-        kernels[k] = {"launches": c, "total_ms": round(ms, 3), "alg_GBps": round(bw(ALG_FIELDS[k], ms, cells), 2)}
-    if args.nz > 0:
-        ALG_FIELDS.update({"enth_column": 5 + 5 / args.nz, "enth_layer": 5 + 5 / args.nz,
-                           "enth_residual": 4 + 5 / args.nz})
-    for k in ("enth_column", "enth_layer", "enth_residual") if args.nz > 0 else ():
-        c, ms, units = kern[k]
-        kernels[k]["alg_GBps_incl_skipped"] = round(bw(ALG_FIELDS[k], ms, units), 2)
+    fp64 = None
+    cycles = (0, 0)
+    if args.fp64:
+        f64 = Form(args, torch, "fp64")
+        res_d = run_form(f64, truth, args, torch)
+        st = res_d["stats"]
+        cycles = (st.get("forward_iterations", 0), st.get("adjoint_iterations", 0))
+        tot_d, fine_d = kernel_times(f64, res_d, args, torch, _lib)
+        fp64 = {"value": round(args.steps / res_d["elapsed"], 4),
+                "ms_per_step": round(1000.0 * res_d["elapsed"] / args.steps, 3),
+                "tolerances": res_d["tolerances"], "target_met_on_calibration": res_d["calibrated"],
+                "rel_l2_err_vs_truth": {k: float(f"{v:.3e}") for k, v in res_d["errors"].items()},
+                "last_step": st,
+                "kernels": {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in tot_d.items() if v[0] > 0},
+                "form": "reference algorithm in FP64: additive Vanka (30 local Newton steps), counts 200/10/20/50"}
+        del f64
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.fp64:
         try:
-            cpu = cpu_baseline(args)
+            cpu = cpu_baseline(args, cycles)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"error": repr(e)}
 
+    value = args.steps / res_m["elapsed"]
+    errs_m = {k: float(f"{v:.3e}") for k, v in res_m["errors"].items()}
     line = {"metric": METRIC, "value": round(value, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "warmup": args.warmup, "ms_per_step": round(1000.0 * res_m["elapsed"] / args.steps, 3),
+            "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 multigrid, f64 residual/accumulation/coarsest", "data": "synthetic",
             "config": {"workload": (f"dome on GRF bed, {args.n}x{args.n} MAC grid, dx=1500 m, {args.levels} levels, "
-                                    f"{args.vcycles}+{args.vcycles} V-cycles forward+adjoint per step, dt={args.dt} a"
+                                    f"dt={args.dt} a, forward + adjoint solved to {args.target:g} relative L2 error"
                                     + (f", coupled enthalpy on {args.n}x{args.n}x{args.nz} columns "
-                                       f"(<= {args.n_smooth} sweeps, strain + frictional heating)"
-                                       if args.nz > 0 else "")),
-                       "sigma_levels": args.nz,
-                       "precision": "plain FP64 form", "grid": f"{args.n}x{args.n}", "levels": args.levels,
-                       "vcycles": args.vcycles, "dt_years": args.dt, "parallelism": "single GPU"},
-            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "kernels": kernels,
-            "ms_per_step_eager_timed": round(1000.0 * elapsed_eager / args.steps, 3),
-            "enthalpy_adjoint_sweeps": stepper.adjoint_sweeps,
-            "omp_threads": os.environ.get("OMP_NUM_THREADS")}
+                                       "(strain + frictional heating, rate-factor feedback)" if args.nz > 0 else "")),
+                       "grid": f"{args.n}x{args.n}", "levels": args.levels, "sigma_levels": args.nz,
+                       "dt_years": args.dt, "target_rel_l2": args.target, "parallelism": "single GPU"},
+            "rel_l2_err_vs_truth": errs_m, "target_met": all(v <= args.target for v in res_m["errors"].values()),
+            "tolerances": res_m["tolerances"], "last_step": res_m["stats"],
+            "speedup_vs_fp64": (round(value / fp64["value"], 3) if fp64 else None),
+            "fp64": fp64, "roofline": roofline, "hbm_fractions": fracs, "cpu_baseline": cpu,
+            "kernels": kernels_m,
+            "truth": (f"FP64 solves to the residual floor from the same state: forward FP64 red-black V-cycles to "
+                      f"{TRUTH_FORWARD:g}, adjoint FP64 reference-form V-cycles to {TRUTH_ADJOINT:g}, enthalpy "
+                      f"sweeps to {TRUTH_ENTHALPY:g} (relative residuals)")}
     print(json.dumps(line), flush=True)
 
 

@@ -192,7 +192,8 @@ SIGNATURES = {
 }
 
 TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3, "residual": 4, "adjoint_apply": 5,
-                 "enth_column": 6, "enth_layer": 7, "enth_residual": 8, "adjoint_factor": 9}
+                 "enth_column": 6, "enth_layer": 7, "enth_residual": 8, "adjoint_factor": 9, "vanka_mc32": 10,
+                 "vanka_mc64": 11, "adjoint_mc32": 12, "adjoint_mc64": 13}
 
 
 def graphs_enable(on: bool) -> None:

@@ -1,0 +1,110 @@
+"""One coupled momentum + mass + enthalpy step and its adjoint on one GPU.
+
+The reference's drivers call the pieces in turn (ThermalModel.pre_momentum,
+IceDynamics.forward, ThermalModel.step, then IceDynamics.backward:
+glide/model.py:65-94, 156-213; examples/greenland/greenland_inverse.py:119-168).
+:class:`CoupledStep` runs the same sequence with the enthalpy step and its
+adjoint on a second stream: both depend only on the forward solution, and the
+momentum adjoint depends only on the forward state and the rate factor the
+forward used, so the two overlap.  The new rate factor goes to a buffer of its
+own and is written to the grid after both, so the momentum adjoint sees the
+rate factor of its forward solve.  The step runs on a stream of its own rather
+than the null stream, whose implicit synchronisation with blocking streams
+would serialise the two.  Every stream keeps the default priority.
+"""
+from __future__ import annotations
+
+from typing import Callable, Optional
+
+import torch
+
+SEC_PER_YR = 365.25 * 86400.0
+
+
+def velocity_loss_seeds(grid):
+    """dJ/du, dJ/dv, dJ/dH of J = 0.5 |u|^2 + 0.5 |v|^2."""
+    return grid.state.u.data, grid.state.v.data, None
+
+
+class CoupledStep:
+    """step(t, dt) advances the coupled state by dt years and leaves the
+    momentum multipliers and parameter gradients of the step in the grid.
+
+    model: IceDynamics; thermal: ThermalModel on the model's finest grid, or
+    None (momentum only).  seeds(grid) gives the momentum adjoint's dJ/du,
+    dJ/dv, dJ/dH (seeds=None: no momentum adjoint); thermal_seeds(thermal,
+    grid) the enthalpy adjoint's dL/dE and dL/dB (default E and ones; a
+    callable returning None skips the enthalpy adjoint).  forward, when
+    given, replaces model.forward(t, dt, update_geometry=True).  concurrent
+    puts the enthalpy work on the second stream."""
+
+    def __init__(self, model, thermal=None, seeds: Callable = velocity_loss_seeds,
+                 thermal_seeds: Optional[Callable] = None, enthalpy_adjoint_rtol: float = 1e-10,
+                 enthalpy_adjoint_iters: int = 64, forward: Optional[Callable] = None, concurrent: bool = True):
+        self.model, self.thermal = model, thermal
+        self.seeds = seeds
+        self.thermal_seeds = thermal_seeds if thermal_seeds is not None else self._default_thermal_seeds
+        self.enthalpy_adjoint_rtol = float(enthalpy_adjoint_rtol)
+        self.enthalpy_adjoint_iters = int(enthalpy_adjoint_iters)
+        self.forward = forward
+        self.concurrent = bool(concurrent)
+        g = model.mg[0]
+        dev = g.state.u.data.device
+        self.main = torch.cuda.Stream(device=dev)
+        self.side = torch.cuda.Stream(device=dev) if thermal is not None else None
+        self.B_next = torch.empty_like(g.rheology.B.data) if thermal is not None else None
+        self._ones = torch.ones_like(g.rheology.B.data) if thermal is not None else None
+        self.ev_fwd = torch.cuda.Event()
+        self.ev_thermal = torch.cuda.Event()
+        self.enthalpy_report = None
+
+    def _default_thermal_seeds(self, thermal, grid):
+        return thermal.ops.enthalpy_state.E, self._ones
+
+    def __call__(self, t: float, dt: float, report: bool = False):
+        caller = torch.cuda.current_stream()
+        self.main.wait_stream(caller)
+        with torch.cuda.stream(self.main):
+            self._step(t, dt, report)
+        caller.wait_stream(self.main)
+
+    def _thermal(self, dt, report):
+        th = self.thermal
+        g = self.model.mg[0]
+        self.enthalpy_report = th.step(dt * SEC_PER_YR, report=report, B_out=self.B_next)
+        seeds = self.thermal_seeds(th, g)
+        if seeds is not None:
+            dL_dE, dL_dB = seeds
+            th.ops.adjoint_solve(dt * SEC_PER_YR, dL_dE=dL_dE, dL_dB=dL_dB, B_scale=th.B_scale,
+                                 relative_tolerance=self.enthalpy_adjoint_rtol, n_iter=self.enthalpy_adjoint_iters,
+                                 u2d=g.state.u.data, v2d=g.state.v.data, vel_div=SEC_PER_YR)
+
+    def _step(self, t, dt, report):
+        model, th = self.model, self.thermal
+        if th is not None:
+            th.pre_momentum()
+        if self.forward is not None:
+            self.forward(t, dt)
+        else:
+            model.forward(t, dt, update_geometry=True)
+        g = model.mg[0]
+        main = torch.cuda.current_stream()
+        if th is not None:
+            if self.concurrent:
+                self.ev_fwd.record(main)
+                self.side.wait_event(self.ev_fwd)
+                with torch.cuda.stream(self.side):
+                    self._thermal(dt, report)
+                    self.ev_thermal.record(self.side)
+            else:
+                self._thermal(dt, report)
+        if self.seeds is not None:
+            dJdu, dJdv, dJdH = self.seeds(g)
+            model.backward(t, dt, dJdu=dJdu, dJdv=dJdv, dJdH=dJdH)
+        if th is not None:
+            if self.concurrent:
+                main.wait_event(self.ev_thermal)
+            g.rheology.B.data.copy_(self.B_next)
+
+
+__all__ = ["CoupledStep", "velocity_loss_seeds", "SEC_PER_YR"]

@@ -159,7 +159,7 @@ hipError_t launch_vanka_mc(const SsaArgs<T>& a, const T* eta, T* u_dst, T* v_dst
                            const VankaArgs<T>& vk, int colour, hipStream_t st) {
     const int pairs = (a.nx + 1) / 2;
     const dim3 g((pairs + kMcLanes - 1) / kMcLanes, (a.ny + kMcRows - 1) / kMcRows), b(kMcLanes, kMcRows);
-    TimedLaunch tl(kTimeVankaCell, (long)a.ny * a.nx / 2, st);
+    TimedLaunch tl(std::is_same<T, float>::value ? kTimeVankaMc32 : kTimeVankaMc64, (long)a.ny * a.nx, st);
     if (a.P.linear_drag)
         k_vanka_mc<T, true><<<g, b, 0, st>>>(a, eta, u_dst, v_dst, H_dst, mask, vk, colour & 1);
     else

@@ -1745,7 +1745,7 @@ hipError_t launch_adjoint_pass_mc(const SsaArgs<T>& a, const T* cache, const T* 
                                   T* lu_dst, T* lv_dst, T* lH_dst, const T* fu, const T* fv, const T* fH,
                                   T calving_rate, T omega, int colour, hipStream_t st) {
     if (!a.P.linear_drag) return hipErrorInvalidValue;
-    TimedLaunch tl(kTimeAdjointCell, (long)a.ny * a.nx / 2, st);
+    TimedLaunch tl(std::is_same<T, float>::value ? kTimeAdjointMc32 : kTimeAdjointMc64, (long)a.ny * a.nx, st);
     k_adjoint_pass_fused<T, TY, true><<<tile_grid(a.ny, a.nx, false), dim3(TX, 2 * TY), 0, st>>>(
         a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, -omega, colour & 1);
     return hipGetLastError();

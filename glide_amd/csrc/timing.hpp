@@ -20,6 +20,10 @@ enum TimedKernel {
     kTimeEnthLayer,
     kTimeEnthResidual,
     kTimeAdjointFactor,
+    kTimeVankaMc32,    // one colour phase of the red-black smoother, FP32 / FP64
+    kTimeVankaMc64,
+    kTimeAdjointMc32,  // one colour phase of the red-black adjoint pass, FP32 / FP64
+    kTimeAdjointMc64,
     kTimedKernels
 };
 

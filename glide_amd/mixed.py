@@ -99,7 +99,7 @@ class MixedConfig:
     relative_tolerance: float = 1e-10
     absolute_tolerance: float = 0.0
     fp64_coarsest: bool = True
-    coarsest_steps: int = 40
+    coarsest_steps: int = 10
     pre_steps: int = 4
     post_steps: int = 4
     finest_steps: int = 8

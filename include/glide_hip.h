@@ -332,7 +332,9 @@ int glide_persistent_spin_limit(int polls);
  * (0 forward patch solve, 1 patch increment apply, 2 adjoint patch solve,
  * 3 transpose operator, 4 residual, 5 adjoint increment apply, 6 enthalpy
  * column smoother, 7 enthalpy layer smoother, 8 enthalpy residual, 9 adjoint
- * patch factorisation into the cache).
+ * patch factorisation into the cache, 10 / 11 one colour phase of the
+ * red-black smoother in FP32 / FP64, 12 / 13 one colour phase of the red-black
+ * adjoint pass in FP32 / FP64; the red-black launches count every cell).
  * glide_timing_read synchronises those events and returns the launch count,
  * summed duration and summed unit count (cells; column nodes for 6-8) of
  * kernel `kernel` since the last enable. */

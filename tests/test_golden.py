@@ -1,6 +1,7 @@
 """Golden fixtures (tests/golden/*.npz, written by tests/golden/make_fixtures.py
 from the FP64 oracle): the oracle must still reproduce them (CPU), and the
-gfx950 path must match them (GPU)."""
+gfx950 path must match them (GPU; the coupled fixture's GPU test is in
+tests/test_coupled_gpu.py)."""
 from pathlib import Path
 
 import numpy as np
@@ -51,6 +52,17 @@ def test_oracle_reproduces_thermal_fixture(orc):
     assert rel_err(B, f["B"]) < 1e-12
     lam = th.adjoint_solve(DT * SEC, f["w"])
     assert rel_err(lam, f["lam"]) < 1e-10
+
+
+def test_oracle_reproduces_coupled_fixture(orc):
+    """Three coupled 128^2 x 41, 5-level steps (about 20 s on 8 cores)."""
+    from tests.golden.make_fixtures import coupled_oracle
+    f = _load("coupled_dome128_l5_nz41_3steps.npz")
+    out = coupled_oracle()
+    assert np.array_equal(out["checksums"], f["checksums"])
+    for k in ("u", "v", "H", "B", "E_sub", "E_colsum"):
+        assert rel_err(out[k], f[k]) < 1e-12, k
+    assert np.array_equal(out["mask"], f["mask"])
 
 
 @pytest.mark.gpu
