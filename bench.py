@@ -19,7 +19,10 @@ timing starts.  The two forms:
           FP64 defect-correction loop (FP64 residual, coarsest level and
           accumulation; FP64 red-black V-cycles finish the forward solve).
 
-Both forms share the FP64 enthalpy solve.  Equal error: before timing, each
+Both forms share the FP64 enthalpy solve with the reference's alternating
+sweep order: near the pressure-melting point the discrete enthalpy system can
+have more than one solution, and another sweep order (the red-black column
+option of glide_amd.enthalpy) can settle on a different one.  Equal error: before timing, each
 form's stopping tolerances (forward, enthalpy, adjoint) are set to the loosest
 values of a ladder for which one step from the warmed-up state lands within
 --target (relative L2) of the truth: the same step with FP64 solves driven to
@@ -121,6 +124,7 @@ class Form:
             self.thermal = ThermalModel(self.model.mg[0], nz=args.nz, n_smooth=10, strain_heating=True)
             Ts, qg = thermal_forcing(p)
             self.thermal.initialize(torch.as_tensor(Ts, device="cuda"), Q_geo=torch.as_tensor(qg, device="cuda"))
+
         fs, ad = self.model.forward_solver, self.model.adjoint_solver
         for s in (fs, ad):
             s.fas_options.set(maximum_vcycles=64, absolute_tolerance=0.0, report_norms=False)

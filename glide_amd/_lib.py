@@ -89,7 +89,7 @@ class EnthColumn(ctypes.Structure):
 class EnthSweepConfigC(ctypes.Structure):
     _fields_ = [("omega", ctypes.c_double), ("relaxation", ctypes.c_double),
                 ("relative_tolerance", ctypes.c_double), ("absolute_tolerance", ctypes.c_double),
-                ("n_newton", ctypes.c_int32), ("alternating", ctypes.c_int32)]
+                ("n_newton", ctypes.c_int32), ("alternating", ctypes.c_int32), ("multicolour", ctypes.c_int32)]
 
 
 class EnthReport(ctypes.Structure):

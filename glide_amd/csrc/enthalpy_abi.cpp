@@ -167,14 +167,29 @@ int sweep_core(const glide_enth_column* C, double dt, int n_iter, const glide_en
     A.vel_div = vel_div;
     REQ(C->r_E);
     if (cfg->alternating) REQ(C->E_work);
-    else REQ(C->delta_E);
+    else if (!cfg->multicolour) REQ(C->delta_E);
     const double rtol = cfg->relative_tolerance, atol = cfg->absolute_tolerance;
     HIP_TRY(launch_enth_residual<R>(A, P(C->r_E), 1, partials, nullptr, status, 0, rtol, atol, st),
             "enthalpy residual");
     EnthArgs<R> Ak = A;
     Ak.skip = &status->done;
     for (int it = 0; it < n_iter; ++it) {
-        if (cfg->alternating) {
+        if (cfg->multicolour && !cfg->alternating) {  // both colour phases in place on E
+            for (int colour = 0; colour < 2; ++colour)
+                HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation,
+                                                     st, colour, P(C->E)),
+                        "enthalpy column smoother (red-black)");
+        } else if (cfg->multicolour) {
+            HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation, st,
+                                                 0, P(C->E_work)),
+                    "enthalpy column smoother (even columns)");
+            EnthArgs<R> Aw = Ak;
+            Aw.E = CP(C->E_work);
+            HIP_TRY(launch_enth_column_smooth<R>(Aw, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation, st,
+                                                 1, P(C->E_work)),
+                    "enthalpy column smoother (odd columns)");
+            HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
+        } else if (cfg->alternating) {
             HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, P(C->E_work), cfg->omega, cfg->n_newton,
                                                  cfg->relaxation, st),
                     "enthalpy column smoother");

@@ -50,17 +50,20 @@ __device__ __forceinline__ T diffusivity(T E, T Ep) {
     T s = T(1) / (T(1) + exp(-z));
     return T(kKCold) * (T(1) - s + T(kKTempFactor) * s);
 }
+// Drained water content (glide/cuda/enthalpy.cu:48-98): a softplus of the
+// enthalpy above the pressure-melting value, log(1 + exp(a)) / s with
+// a = s (E - Ep) E_SCALE clamped to [-20, 20] (the excess itself beyond
+// a = 10), over the latent heat; its slope is the logistic factor
+// exp(a) / (1 + exp(a)) over the latent heat (the reference writes it as
+// 1 / (1 + exp(-a)), equal up to rounding; it only enters the Newton
+// Jacobian).  Both come from one exponential.
 template <typename T>
-__device__ __forceinline__ T water_content(T E, T Ep) {
-    T x = (E - Ep) * T(kEScale);
-    T ax = fmin(fmax(T(kDrainSharpness) * x, T(-20)), T(20));
-    T sp = (ax > T(10)) ? x : log(T(1) + exp(ax)) / T(kDrainSharpness);
-    return sp / T(kLHeat);
-}
-template <typename T>
-__device__ __forceinline__ T dwater(T E, T Ep) {
-    T z = fmin(fmax(T(kDrainSharpness) * (E - Ep) * T(kEScale), T(-20)), T(20));
-    return (T(1) / (T(1) + exp(-z))) / T(kLHeat);
+__device__ __forceinline__ void water_and_slope(T E, T Ep, T& w, T& dw) {
+    const T x = (E - Ep) * T(kEScale);
+    const T ax = fmin(fmax(T(kDrainSharpness) * x, T(-20)), T(20));
+    const T ez = exp(ax);
+    w = ((ax > T(10)) ? x : log(T(1) + ez) / T(kDrainSharpness)) / T(kLHeat);
+    dw = (ez / (T(1) + ez)) / T(kLHeat);
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
@@ -159,9 +162,22 @@ struct Row {
     T r, a, b, c;
 };
 
+// Diffusivity at the interface above level k (between k and k + 1; lane k
+// holds Ek, Ekp1 the level above) and, from lane k - 1, at the interface
+// below: each interface's value is formed once per column.  Every lane of
+// the wavefront must call it (the lower value is a cross-lane read).
+template <typename T>
+__device__ __forceinline__ void interface_K(int k, int nz, T h, T Ek, T Ekp1, T& Ku, T& Kl) {
+    const T dsig = T(1) / T(nz - 1);
+    const T Ep_k = e_pmp(T(k) * dsig, h), Ep_p = e_pmp(T(min(k + 1, nz - 1)) * dsig, h);
+    Ku = diffusivity(T(0.5) * (Ek + Ekp1), T(0.5) * (Ep_k + Ep_p));
+    Kl = lane_get(Ku, max(k - 1, 0));
+}
+
+// Ku / Kl: the interface diffusivities from interface_K.
 template <typename T>
 __device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, int k, T h, T hinv, T hp, T Ek, T Ekm1, T Ekp1,
-                                            T Epk, T om_m, T om_k, T om_p, const Horiz<T>& hz) {
+                                            T Epk, T om_m, T om_k, T om_p, const Horiz<T>& hz, T Ku, T Kl) {
     const int nz = A.nz;
     const T dsig = T(1) / T(nz - 1);
     const T invdt = T(1) / A.dt;
@@ -186,7 +202,7 @@ __device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, int k, T h, T 
             w.b += T(kRhoI) * (wp - om_k) * hdi;
             w.c += T(kRhoI) * wn * hdi;
         }
-        T Kh = diffusivity(T(0.5) * (Ek + Ekp1), T(0.5) * (Ep_k + e_pmp(dsig, h)));
+        T Kh = Ku;
         T dh = T(0.5) * dsig;
         w.r += (-hinv * Kh * (Ekp1 - Ek) / dsig) / dh;
         T co = hinv * Kh / (dsig * dh);
@@ -202,9 +218,6 @@ __device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, int k, T h, T 
             w.c += T(kRhoI) * un * di;
             w.b += T(kRhoI) * (up - ln) * di;
         }
-        T Ep_m = e_pmp(T(k - 1) * dsig, h), Ep_p = e_pmp(T(min(k + 1, nz - 1)) * dsig, h);
-        T Ku = diffusivity(T(0.5) * (Ek + Ekp1), T(0.5) * (Ep_k + Ep_p));
-        T Kl = diffusivity(T(0.5) * (Ek + Ekm1), T(0.5) * (Ep_k + Ep_m));
         T d2 = T(1) / (dsig * dsig);
         w.r += -hinv * d2 * (Ku * (Ekp1 - Ek) - Kl * (Ek - Ekm1));
         T dl = hinv * d2 * Kl, du = hinv * d2 * Ku;
@@ -213,8 +226,10 @@ __device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, int k, T h, T 
         w.b += dl + du;
     }
     if (A.flags & 4) {
-        w.r += h * T(kRhoW) * T(kLHeat) * (A.drain_rate * water_content(Ek, Ep_k)) / T(kEScale);
-        w.b += h * T(kRhoW) * T(kLHeat) * A.drain_rate * dwater(Ek, Ep_k);
+        T wc, dwc;
+        water_and_slope(Ek, Ep_k, wc, dwc);
+        w.r += h * T(kRhoW) * T(kLHeat) * (A.drain_rate * wc) / T(kEScale);
+        w.b += h * T(kRhoW) * T(kLHeat) * A.drain_rate * dwc;
     }
     return w;
 }
@@ -285,12 +300,14 @@ __global__ __launch_bounds__(256) void k_enth_residual(EnthArgs<T> A, T* __restr
         const T omk = (in && A.omega) ? A.omega[base + k] : T(0);
         const T Ekm1 = lane_get(Ek, max(k - 1, 0)), Ekp1 = lane_get(Ek, min(k + 1, 63));
         const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
+        T Ku, Kl;
+        interface_K(k, nz, h, Ek, Ekp1, Ku, Kl);
         T r = T(0);
         if (in && k < nz - 1 && !(h < A.h_thin)) {
             Horiz<T> hz{};
             if (A.flags & 1) hz = load_horiz(A, q.i, q.j, k);
             Row<T> w = level_row(A, k, h, T(1) / h, A.H_prev[col], Ek, Ekm1, Ekp1, A.E_prev[base + k], omm, omk,
-                                 omp, hz);
+                                 omp, hz, Ku, Kl);
             r = use_forcing ? w.r - A.f_E[base + k] : w.r;
         }
         if (in && r_out) r_out[base + k] = r;
@@ -347,13 +364,44 @@ __global__ __launch_bounds__(256) void k_enth_reduce(const double* __restrict__ 
 // Column Newton smoother (glide/cuda/enthalpy.cu:932-1112): Newton on the
 // column with horizontal neighbours frozen at E; writes delta = E_col - E
 // and/or E_out = E + omega_relax * delta.
+// colour >= 0 (no counterpart in the reference): one phase of the red-black
+// column ordering, writing through E_mc instead of E_out, one wavefront per
+// column pair of a row.  Phase 0 smooths the columns with (i + j) even and
+// copies the others (E_mc = E) unless it runs in place (E_mc == A.E); phase 1
+// runs in place on E_mc and smooths the odd columns, whose horizontal
+// neighbours are all even, so that they see the new values (a block
+// Gauss-Seidel step over columns) and no column of a phase reads another
+// column's writes.
 template <typename T>
 __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __restrict__ delta, T* __restrict__ E_out,
-                                                            T omega_relax, int n_newton, T relaxation) {
+                                                            T omega_relax, int n_newton, T relaxation, int colour,
+                                                            T* E_mc) {
     if (skipped(A.skip)) return;
     const long ncol = (long)A.ny * A.nx;
-    const ColIdx q = col_index(wave_column(), A.nx, ncol);
-    if (!q.valid) return;
+    ColIdx q;
+    if (colour < 0) {
+        q = col_index(wave_column(), A.nx, ncol);
+        if (!q.valid) return;
+    } else {
+        // wave w: row i, column pair m; its column of the phase's colour and,
+        // in phase 0, the copy of the other column of the pair
+        const int P = (A.nx + 1) >> 1;
+        const long w = wave_column();
+        if (w >= (long)A.ny * P) return;
+        const int i = (int)(w / P), m = (int)(w % P), p = (i + colour) & 1;
+        const int j = 2 * m + p, jo = 2 * m + 1 - p;
+        if (colour == 0 && E_mc != A.E && jo < A.nx && lane_id() < A.nz) {
+            const long b = ((long)i * A.nx + jo) * A.nz + lane_id();
+            E_mc[b] = A.E[b];
+        }
+        if (j >= A.nx) return;
+        q.col = (long)i * A.nx + j;
+        q.i = i;
+        q.j = j;
+        q.k = lane_id();
+        q.valid = true;
+        E_out = nullptr;
+    }
     const int nz = A.nz, k = q.k;
     const bool in = k < nz;
     const long base = q.col * nz;
@@ -374,9 +422,11 @@ __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __
         if ((A.flags & 1) && interior) hz = load_horiz(A, q.i, q.j, k);
         for (int it = 0; it < n_newton; ++it) {
             const T Ekm1 = lane_get(El, max(k - 1, 0)), Ekp1 = lane_get(El, min(k + 1, 63));
+            T Ku, Kl;
+            interface_K(k, nz, h, El, Ekp1, Ku, Kl);
             T a = T(0), b = T(1), c = T(0), d = T(0);
             if (interior) {
-                Row<T> w = level_row(A, k, h, hinv, hp, El, Ekm1, Ekp1, Epk, omm, omk, omp, hz);
+                Row<T> w = level_row(A, k, h, hinv, hp, El, Ekm1, Ekp1, Epk, omm, omk, omp, hz, Ku, Kl);
                 a = k == 0 ? T(0) : w.a;
                 b = w.b;
                 c = w.c;
@@ -392,6 +442,7 @@ __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __
     const T dE = El - E0;
     if (delta) delta[base + k] = dE;
     if (E_out) E_out[base + k] = E0 + omega_relax * dE;
+    if (colour >= 0) E_mc[base + k] = E0 + omega_relax * dE;
 }
 
 // Pointwise Jacobi layer smoother (glide/cuda/enthalpy.cu:1128-1255):
@@ -412,13 +463,15 @@ __global__ __launch_bounds__(256) void k_enth_layer_smooth(EnthArgs<T> A, T* __r
     const T omk = (in && A.omega) ? A.omega[base + k] : T(0);
     const T Ekm1 = lane_get(Ek, max(k - 1, 0)), Ekp1 = lane_get(Ek, min(k + 1, 63));
     const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
+    T Ku, Kl;
+    interface_K(k, nz, h, Ek, Ekp1, Ku, Kl);
     if (!in) return;
     T out = T(0);
     if (k < nz - 1 && !(h < A.h_thin)) {
         Horiz<T> hz{};
         if (A.flags & 1) hz = load_horiz(A, q.i, q.j, k);
         Row<T> w = level_row(A, k, h, T(1) / h, A.H_prev[q.col], Ek, Ekm1, Ekp1, A.E_prev[base + k], omm, omk, omp,
-                             hz);
+                             hz, Ku, Kl);
         const T r = w.r - A.f_E[base + k];
         out = (fabs(w.b) > T(1e-30)) ? -r / w.b : T(0);
     }
@@ -1112,9 +1165,13 @@ hipError_t launch_enth_residual(const EnthArgs<T>& A, T* r_out, int use_forcing,
 }
 template <typename T>
 hipError_t launch_enth_column_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, int n_newton,
-                                     T relaxation, hipStream_t st) {
+                                     T relaxation, hipStream_t st, int colour, T* E_mc) {
+    if (colour >= 0 && (!E_mc || delta || E_out)) return hipErrorInvalidValue;
+    if (colour == 1 && A.E != E_mc) return hipErrorInvalidValue;
     TimedLaunch tl(kTimeEnthColumn, (long)A.ny * A.nx * A.nz, st);
-    k_enth_column_smooth<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, delta, E_out, omega_relax, n_newton, relaxation);
+    // one wavefront per column, or per column pair of a row in a colour phase
+    const unsigned blocks = colour < 0 ? col_blocks(A.ny, A.nx) : col_blocks(A.ny, (A.nx + 1) / 2);
+    k_enth_column_smooth<T><<<blocks, 256, 0, st>>>(A, delta, E_out, omega_relax, n_newton, relaxation, colour, E_mc);
     return hipGetLastError();
 }
 template <typename T>
@@ -1253,7 +1310,7 @@ hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev
 #define INST(T)                                                                                                      \
     template hipError_t launch_enth_residual<T>(const EnthArgs<T>&, T*, int, double*, double*, EnthStatus*, int,     \
                                                 double, double, hipStream_t);                                        \
-    template hipError_t launch_enth_column_smooth<T>(const EnthArgs<T>&, T*, T*, T, int, T, hipStream_t);            \
+    template hipError_t launch_enth_column_smooth<T>(const EnthArgs<T>&, T*, T*, T, int, T, hipStream_t, int, T*);  \
     template hipError_t launch_enth_layer_smooth<T>(const EnthArgs<T>&, T*, T*, T, hipStream_t);                     \
     template hipError_t launch_enth_apply<T>(const EnthArgs<T>&, T*, const T*, T, hipStream_t);                      \
     template hipError_t launch_enth_omega<T>(const EnthArgs<T>&, const T*, const T*, T, T*, hipStream_t);            \

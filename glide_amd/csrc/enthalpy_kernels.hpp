@@ -40,10 +40,14 @@ template <typename T>
 hipError_t launch_enth_residual(const EnthArgs<T>& A, T* r_out, int use_forcing, double* partials, double* max_out,
                                 EnthStatus* status, int status_mode, double rtol, double atol, hipStream_t st);
 // Column Newton smoother; writes delta = E_col - E (if non-null) and/or
-// E_out = E + omega_relax * delta (if non-null).
+// E_out = E + omega_relax * delta (if non-null).  colour >= 0: one phase of
+// the red-black column ordering through E_mc (delta and E_out null): phase 0
+// smooths the even columns ((i + j) even) from A.E into E_mc and copies the
+// odd ones (nothing to copy when E_mc == A.E), phase 1 smooths the odd
+// columns in place (A.E == E_mc).
 template <typename T>
 hipError_t launch_enth_column_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, int n_newton,
-                                     T relaxation, hipStream_t st);
+                                     T relaxation, hipStream_t st, int colour = -1, T* E_mc = nullptr);
 // Pointwise Jacobi layer smoother; delta = -r / J_kk and/or E_out = E + omega_relax * delta.
 template <typename T>
 hipError_t launch_enth_layer_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, hipStream_t st);

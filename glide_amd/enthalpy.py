@@ -108,12 +108,21 @@ class ColumnSmootherConfig:
     lf_c: float = 1e-12
     report_norms: bool = False
     hook_func: Callable[[int], None] | None = None
+    # (additions) red-black column ordering of the native sweep (block
+    # Gauss-Seidel over columns: fewer sweeps; near the pressure-melting point
+    # the discrete system can have more than one solution, and this order can
+    # settle on a different one than the reference's), and whether the
+    # alternating sweep keeps its pointwise layer pass
+    multicolour: bool = False
+    layer_sweeps: bool = True
 
     def to_c(self, alternating=True) -> _lib.EnthSweepConfigC:
         return _lib.EnthSweepConfigC(omega=float(self.omega), relaxation=float(self.relaxation),
                                      relative_tolerance=float(self.relative_tolerance),
                                      absolute_tolerance=float(self.absolute_tolerance),
-                                     n_newton=int(self.n_newton), alternating=1 if alternating else 0)
+                                     n_newton=int(self.n_newton),
+                                     alternating=1 if (alternating and self.layer_sweeps) else 0,
+                                     multicolour=1 if self.multicolour else 0)
 
 
 def _as_field(value, shape, device) -> torch.Tensor:

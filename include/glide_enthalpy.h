@@ -76,6 +76,14 @@ typedef struct glide_enth_sweep_config {
     double relative_tolerance, absolute_tolerance;
     int32_t n_newton;
     int32_t alternating; /* follow each column sweep by a layer sweep */
+    /* (no counterpart in the reference) red-black column ordering: the
+     * columns with i + j even are smoothed against the previous iterate, then
+     * the odd ones against the even ones' new values (block Gauss-Seidel over
+     * columns; fewer sweeps), followed by the layer sweep when alternating is
+     * set, in place on E (no E_work) when it is not.  Near the pressure-
+     * melting point the discrete system can have more than one solution, and
+     * this order can settle on a different one than the reference's. */
+    int32_t multicolour;
 } glide_enth_sweep_config;
 
 typedef struct glide_enth_report {

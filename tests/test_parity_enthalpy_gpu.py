@@ -272,6 +272,27 @@ def test_thermal_step(n, nz, seed, frictional, strain):
     assert rep.sweeps >= 1
 
 
+def test_red_black_column_sweep_reaches_the_same_fixed_point():
+    """The red-black column ordering (no counterpart in the reference), with
+    and without the layer passes, converges to the alternating sweep's fixed
+    point (with them in no more sweeps)."""
+    print("thermal_step red-black columns", flush=True)
+    out = []
+    for mc, layer in ((False, True), (True, True), (True, False)):
+        F, g, tm, th = _thermal_pair(24, 41, 6, strain=True)
+        tm.n_smooth = 200
+        sc = tm.ops.smoother_config
+        sc.relative_tolerance, sc.absolute_tolerance, sc.multicolour, sc.layer_sweeps = 1e-13, 0.0, mc, layer
+        rep = tm.step(DT, report=True)
+        out.append((N(tm.ops.enthalpy_state.E), N(g.rheology.B.data), rep.sweeps))
+    E0, B0, n0 = out[0]
+    print("  sweeps: alternating, red-black + layer, red-black:", [o[2] for o in out], flush=True)
+    for E1, B1, n1 in out[1:]:
+        assert rel_err(E1, E0) < 1e-10
+        assert rel_err(B1, B0) < 1e-10
+    assert out[1][2] <= n0
+
+
 def test_thermal_step_host_path_matches_fused():
     print("thermal_step host path", flush=True)
     F, g, tm, th = _thermal_pair(16, 11, 4, strain=True)
