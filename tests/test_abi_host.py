@@ -45,7 +45,7 @@ def test_struct_layouts_match_header():
     body = eh[eh.index("typedef struct glide_enth_column"):eh.index("} glide_enth_column;")]
     assert tuple(re.findall(r"\*(\w+)", body)) == _lib.ENTH_POINTERS
     assert ctypes.sizeof(_lib.EnthColumn) == 16 + 32 + 8 * len(_lib.ENTH_POINTERS)
-    assert ctypes.sizeof(_lib.EnthSweepConfigC) == 4 * 8 + 2 * 4
+    assert ctypes.sizeof(_lib.EnthSweepConfigC) == 4 * 8 + 4 * 4  # three int32 fields, padded to 8
     assert ctypes.sizeof(_lib.EnthReport) == 8 + 16 + 8 * _lib.ENTH_MAX_SWEEPS
     assert ctypes.sizeof(_lib.ThermalConfigC) == 6 * 8 + 4 * 4
     assert ctypes.sizeof(_lib.EnthAdjointC) == 11 * 8 + 8
