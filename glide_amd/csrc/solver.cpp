@@ -1878,6 +1878,7 @@ int glide_timing_read_units(int kernel, long units, long* count, double* total_m
 
 int glide_gradients(const glide_level* L, double dt, int which, void* stream) {
     GLIDE_TRY(check_level(L));
+    const int exact = (which & GLIDE_GRAD_EXACT) ? 1 : 0;
     hipStream_t st = (hipStream_t)stream;
     REQ(L->lambda_H);
     if (which & 3) {
@@ -1887,21 +1888,24 @@ int glide_gradients(const glide_level* L, double dt, int which, void* stream) {
         if ((which & 2)) REQ(L->g_bed);
         SsaArgs<R> a = make_args(*L, dt, true);
         HIP_TRY(launch_grad_beta_bed<R>(a, CP(L->lambda_u), CP(L->lambda_v), (which & 1) ? P(L->g_beta) : nullptr,
-                                        (which & 2) ? P(L->g_bed) : nullptr, st),
+                                        (which & 2) ? P(L->g_bed) : nullptr, exact, st),
                 "gradients");
     }
+    if (exact && (which & 12)) REQ(L->mask);
     if (which & 4) {
         REQ(L->g_H_prev);
         if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
         HIP_TRY(launch_axpby<R>(P(L->g_H_prev), -1.0 / dt, CP(L->lambda_H), 0.0, CP(L->lambda_H), 0.0, nullptr,
                                 (long)nC(*L), st),
                 "grad H_prev");
+        if (exact) HIP_TRY(launch_zero_masked<R>(P(L->g_H_prev), CP(L->mask), (long)nC(*L), st), "grad H_prev mask");
     }
     if (which & 8) {
         REQ(L->g_smb);
         HIP_TRY(launch_axpby<R>(P(L->g_smb), -1.0, CP(L->lambda_H), 0.0, CP(L->lambda_H), 0.0, nullptr, (long)nC(*L),
                                 st),
                 "grad smb");
+        if (exact) HIP_TRY(launch_zero_masked<R>(P(L->g_smb), CP(L->mask), (long)nC(*L), st), "grad smb mask");
     }
     return GLIDE_OK;
 }

@@ -1527,7 +1527,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_adjoint_coarse(SsaArgs<T> a,
 template <typename T>
 __global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __restrict__ lu,
                                                        const T* __restrict__ lv, T* __restrict__ gbeta,
-                                                       T* __restrict__ gbed) {
+                                                       T* __restrict__ gbed, int exact) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     const int ny = a.ny, nx = a.nx;
     if (j >= nx) return;
@@ -1554,7 +1554,7 @@ __global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __
             gz -= lu[o] * d.d_zA;
         }
     }
-    if (j < nx - 1) {  // u facet (i, j): this cell is its right cell
+    if (j < nx - 1 || exact) {  // u facet (i, j): this cell is its right cell
         const long o = (long)i * (nx + 1) + j;
         if (gbeta) {
             Drag<T> t = basal_drag(u[o], v[offV(i, j - 1, ny, nx)], v[offV(i, j, ny, nx)], v[offV(i + 1, j - 1, ny, nx)],
@@ -1583,7 +1583,7 @@ __global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __
             gz -= lv[o] * d.d_zA;
         }
     }
-    if (i < ny - 1) {  // v facet (i, j): this cell is its bottom cell
+    if (i < ny - 1 || exact) {  // v facet (i, j): this cell is its bottom cell
         const long o = (long)i * nx + j;
         if (gbeta) {
             Drag<T> t = basal_drag(v[o], u[offU(i - 1, j, ny, nx)], u[offU(i - 1, j + 1, ny, nx)], u[offU(i, j, ny, nx)],
@@ -1819,9 +1819,22 @@ hipError_t launch_vanka_adjoint_pass_cached(int ny, int nx, const T* cache, cons
 }
 
 template <typename T>
-hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st) {
+hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, int exact,
+                                hipStream_t st) {
     dim3 g((a.nx + 255) / 256, a.ny);
-    k_grad_beta_bed<T><<<g, 256, 0, st>>>(a, lu, lv, gbeta, gbed);
+    k_grad_beta_bed<T><<<g, 256, 0, st>>>(a, lu, lv, gbeta, gbed, exact);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_masked(T* __restrict__ g, const T* __restrict__ mask, long n) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && mask[k] > T(0.5)) g[k] = T(0);
+}
+
+template <typename T>
+hipError_t launch_zero_masked(T* g, const T* mask, long n, hipStream_t st) {
+    k_zero_masked<T><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, mask, n);
     return hipGetLastError();
 }
 
@@ -1853,7 +1866,8 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     template hipError_t launch_adjoint_factor<T>(const SsaArgs<T>&, T*, const VankaArgs<T>&, hipStream_t);          \
     template hipError_t launch_vanka_adjoint_pass_cached<T>(int, int, const T*, const T*, const T*, const T*, T*, T*, \
                                                             T*, T*, T, bool, hipStream_t);                           \
-    template hipError_t launch_grad_beta_bed<T>(const SsaArgs<T>&, const T*, const T*, T*, T*, hipStream_t);
+    template hipError_t launch_grad_beta_bed<T>(const SsaArgs<T>&, const T*, const T*, T*, T*, int, hipStream_t); \
+    template hipError_t launch_zero_masked<T>(T*, const T*, long, hipStream_t);
 
 GLIDE_INSTANTIATE(double)
 GLIDE_INSTANTIATE(float)

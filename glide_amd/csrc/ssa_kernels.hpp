@@ -195,8 +195,14 @@ hipError_t launch_adjoint_factor(const SsaArgs<T>& a, T* cache, const VankaArgs<
 template <typename T>
 hipError_t launch_vanka_adjoint_pass_cached(int ny, int nx, const T* cache, const T* ru, const T* rv, const T* rH,
                                             T* lu, T* lv, T* lH, T* work, T omega, bool fused, hipStream_t st);
+// exact: also the right / bottom cell terms of facets nx-1 and ny-1, which
+// the reference's guards drop (grad.cu:62).
 template <typename T>
-hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, hipStream_t st);
+hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T* gbeta, T* gbed, int exact,
+                                hipStream_t st);
+// g = 0 where mask > 0.5.
+template <typename T>
+hipError_t launch_zero_masked(T* g, const T* mask, long n, hipStream_t st);
 
 
 // Multicolour (red-black) Vanka smoothing (mc_kernels.hip; no counterpart in

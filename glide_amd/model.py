@@ -62,8 +62,12 @@ class IceDynamics:
             f(t + dt)
 
     def backward(self, t, dt, dJdu=None, dJdv=None, dJdH=None, compute_beta_grad=True, compute_bed_grad=True,
-                 compute_H_prev_grad=True, compute_smb_grad=True):
-        """Adjoint solve Jᵀλ = -dJ/dx and parameter gradients (glide/model.py:74-94)."""
+                 compute_H_prev_grad=True, compute_smb_grad=True, exact_gradients=False):
+        """Adjoint solve Jᵀλ = -dJ/dx and parameter gradients (glide/model.py:74-94).
+
+        exact_gradients (an addition): the gradient of the discrete problem,
+        with the boundary-facet terms the reference's gradient kernel drops and
+        zero H_prev / smb gradients in active-set cells (GLIDE_GRAD_EXACT)."""
         top = self.mg.levels[self.top_level]
         ao = top.adjoint_operators
         for buf, g in ((ao.f_u, dJdu), (ao.f_v, dJdv), (ao.f_H, dJdH)):
@@ -74,6 +78,8 @@ class IceDynamics:
         self.adjoint_solver.solve(dt, start_level=self.top_level)
         which = ((1 if compute_beta_grad else 0) | (2 if compute_bed_grad else 0) |
                  (4 if compute_H_prev_grad else 0) | (8 if compute_smb_grad else 0))
+        if which and exact_gradients:
+            which |= 16
         if which:
             ao._grad(which, dt)
 

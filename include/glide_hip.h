@@ -259,7 +259,13 @@ int glide_fas_adjoint_solve(const glide_level* levels, int n_levels, int start_l
 /* One adjoint V-cycle (FASAdjointSolver.vcycle, glide/multigrid.py:905-973). */
 int glide_fas_adjoint_vcycle(const glide_level* levels, int n_levels, int level, int finest, double dt,
                              const glide_fas_config* cfg, const glide_vanka_config* vc, void* stream);
-/* which: bit 0 beta, bit 1 bed, bit 2 H_prev, bit 3 smb. */
+/* which: bit 0 beta, bit 1 bed, bit 2 H_prev, bit 3 smb; with
+ * GLIDE_GRAD_EXACT (no counterpart in the reference) the gradient of the
+ * discrete problem: the right / bottom cell terms of facets nx-1 and ny-1
+ * that the reference's guards drop (glide/cuda/grad.cu:62) are kept, and the
+ * H_prev and smb gradients are zero in active-set cells (whose rows do not
+ * depend on them; the reference's multipliers there are not determined). */
+#define GLIDE_GRAD_EXACT 16
 int glide_gradients(const glide_level* L, double dt, int which, void* stream);
 
 /* Graph replay of fixed-cycle solves (no counterpart in the reference; the
