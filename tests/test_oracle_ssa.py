@@ -330,6 +330,50 @@ def test_vanka_fixed_point_at_solution(orc):
     assert rel_err(u, F["u"]) < 1e-5 and rel_err(H, F["H"]) < 1e-6
 
 
+def test_mc_pass_fixed_point_at_solution(orc):
+    """At a converged state a red-black pass with full local Newton solves
+    is the identity: its patch solves see zero residual rows (no counterpart
+    in the reference; pins mc_sweep against the residual's zero)."""
+    _, om = _dome_mg(orc, 16, 2, seed=1)
+    rep = om.fascd_solve(DT, orc.OracleMG.fas_config(vcycles=40, rtol=1e-13, atol=0.0))
+    assert rep.final_norm < 1e-11 * rep.initial_norm
+    F = {k: om.get(k) for k in ("u", "v", "H", "phi", "mask", "bed", "B", "beta")}
+    F["f_u"], F["f_v"] = np.zeros_like(F["u"]), np.zeros_like(F["v"])
+    F["f_H"] = om.get("H_prev") / DT + om.get("smb")
+    F["gamma"] = np.full_like(F["H"], 0.1)
+    u, v, H, mask, _ = orc.mc_sweep(F, orc.phys(), DX, DT, newton_steps=30, relaxation=1.0, newton_tol=0.0)
+    assert rel_err(u, F["u"]) < 1e-9 and rel_err(v, F["v"]) < 1e-9 and rel_err(H, F["H"]) < 1e-12
+    assert np.array_equal(mask, F["mask"])
+    # and away from it the pass moves the state
+    F2 = dict(F)
+    F2["u"] = F["u"] * 1.01
+    u2, _, _, _, _ = orc.mc_sweep(F2, orc.phys(), DX, DT, newton_steps=30, relaxation=1.0, newton_tol=0.0)
+    assert rel_err(u2, F2["u"]) > 1e-4
+
+
+def test_mc_adjoint_pass_fixed_point_at_solution(orc):
+    """At a converged adjoint solution (pinned multipliers zero) a red-black
+    adjoint pass leaves the multipliers unchanged: its patch solves see the
+    zero residual of the transpose operator."""
+    _, om = _dome_mg(orc, 16, 2, seed=5)
+    om.fascd_solve(DT, orc.OracleMG.fas_config(vcycles=40, rtol=1e-13, atol=0.0))
+    rng = np.random.default_rng(9)
+    af = (rng.standard_normal((16, 17)), rng.standard_normal((17, 16)), np.zeros((16, 16)))
+    for k, a in zip(("af_u", "af_v", "af_H"), af):
+        om.put(k, a)
+    cfg = orc.OracleMG.fas_config(vcycles=60, rtol=1e-14, atol=0.0)
+    cfg.project_pinned = 1
+    om.fas_adjoint_solve(DT, cfg)
+    F = {k: om.get(k) for k in ("u", "v", "H", "phi", "mask", "bed", "B", "beta")}
+    lam = tuple(om.get(k) for k in ("lambda_u", "lambda_v", "lambda_H"))
+    out = orc.mc_adjoint_sweep(F, lam, af, orc.phys(), DX, DT, omega=1.0)
+    for a, b in zip(out, lam):
+        assert rel_err(a, b) < 1e-9
+    lam2 = (lam[0] * 1.01, lam[1], lam[2])
+    out2 = orc.mc_adjoint_sweep(F, lam2, af, orc.phys(), DX, DT, omega=1.0)
+    assert rel_err(out2[0], lam2[0]) > 1e-5
+
+
 def test_adjoint_solve_contracts(orc):
     _, om = _dome_mg(orc, 32, 3, seed=2)
     cfg = orc.OracleMG.fas_config(vcycles=3, rtol=0.0, atol=0.0)
