@@ -52,6 +52,7 @@ struct EnthScratch {
     double* partials = nullptr;   // kEnthResBlocks
     double* max_out = nullptr;    // 1
     unsigned long long* acc = nullptr;  // adjoint passes' maximum and block count, kept zero between uses
+    unsigned long long* rmax = nullptr;  // kEnthRmaxSlots: the column pass's max-|r| accumulators
     EnthStatus* status = nullptr;
     EnthStatus* host = nullptr;   // pinned mirror
 };
@@ -71,6 +72,7 @@ int scratch(EnthScratch** out) {
         HIP_TRY(hipMalloc(&s.acc, 2 * sizeof(unsigned long long)), "hipMalloc(enthalpy accumulator)");
         HIP_TRY(hipMemset(s.acc, 0, 2 * sizeof(unsigned long long)), "hipMemset(enthalpy accumulator)");
         HIP_TRY(hipMalloc(&s.status, sizeof(EnthStatus)), "hipMalloc(enthalpy status)");
+        HIP_TRY(hipMalloc(&s.rmax, sizeof(unsigned long long) * kEnthRmaxSlots), "hipMalloc(enthalpy max slots)");
         HIP_TRY(hipHostMalloc(&s.host, sizeof(EnthStatus), hipHostMallocDefault), "hipHostMalloc(enthalpy status)");
     }
     *out = &s;
@@ -155,7 +157,7 @@ int make_args(const glide_enth_column* C, double dt, bool needs_state, EnthArgs<
 // stack copy).
 int sweep_core(const glide_enth_column* C, double dt, int n_iter, const glide_enth_sweep_config* cfg,
                glide_enth_report* report, hipStream_t st, double* partials, EnthStatus* status, EnthStatus* host,
-               const R* u2d, const R* v2d, double vel_div) {
+               const R* u2d, const R* v2d, double vel_div, unsigned long long* rmax = nullptr) {
     if (!cfg) return fail(GLIDE_ERR_INVALID_ARG, "sweep config is NULL");
     if (n_iter < 0 || n_iter > GLIDE_ENTH_MAX_SWEEPS)
         return fail(GLIDE_ERR_INVALID_ARG, "n_iter must be in [0, GLIDE_ENTH_MAX_SWEEPS]");
@@ -173,6 +175,27 @@ int sweep_core(const glide_enth_column* C, double dt, int n_iter, const glide_en
             "enthalpy residual");
     EnthArgs<R> Ak = A;
     Ak.skip = &status->done;
+    if (rmax && cfg->alternating && !cfg->multicolour && cfg->n_newton >= 1) {
+        // The reference's sweep with the residual of each iterate taken from
+        // the next column pass's first Newton rows (the same values the
+        // residual launch computes): sweep it's norm is closed after pass
+        // it + 1's column launch, whose layer launch then skips if the
+        // sweep stopped, leaving E at the iterate the reference stops at.
+        HIP_TRY(launch_enth_reduce_slots(rmax, status, 0, rtol, atol, st), "enthalpy max slots");
+        EnthArgs<R> Aw = Ak;
+        Aw.E = CP(C->E_work);
+        for (int it = 0; it < n_iter; ++it) {
+            HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, P(C->E_work), cfg->omega, cfg->n_newton,
+                                                 cfg->relaxation, st, -1, nullptr, it > 0 ? rmax : nullptr,
+                                                 P(C->r_E)),
+                    "enthalpy column smoother");
+            if (it > 0) HIP_TRY(launch_enth_reduce_slots(rmax, status, 1, rtol, atol, st), "enthalpy sweep norm");
+            HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
+        }
+        if (n_iter > 0)
+            HIP_TRY(launch_enth_residual<R>(Ak, P(C->r_E), 1, partials, nullptr, status, 1, rtol, atol, st),
+                    "enthalpy residual");
+    } else
     for (int it = 0; it < n_iter; ++it) {
         if (cfg->multicolour && !cfg->alternating) {  // both colour phases in place on E
             for (int colour = 0; colour < 2; ++colour)
@@ -230,7 +253,7 @@ int sweep_impl(const glide_enth_column* C, double dt, int n_iter, const glide_en
         return fail(GLIDE_ERR_INVALID_ARG, "n_iter must be in [0, GLIDE_ENTH_MAX_SWEEPS]");
     EnthScratch* s = nullptr;
     GLIDE_TRY(scratch(&s));
-    return sweep_core(C, dt, n_iter, cfg, report, st, s->partials, s->status, s->host, u2d, v2d, vel_div);
+    return sweep_core(C, dt, n_iter, cfg, report, st, s->partials, s->status, s->host, u2d, v2d, vel_div, s->rmax);
 }
 
 }  // namespace

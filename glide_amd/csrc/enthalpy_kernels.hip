@@ -361,6 +361,27 @@ __global__ __launch_bounds__(256) void k_enth_reduce(const double* __restrict__ 
     if (status) record_sweep(status, mode, r, rtol, atol);
 }
 
+// Maximum over the column pass's max-|r| slots (bit patterns of
+// non-negative doubles, or NaN), zeroing them; mode 1 closes one sweep
+// iteration as k_enth_reduce does, mode 0 only zeroes.
+__global__ __launch_bounds__(256) void k_enth_reduce_slots(unsigned long long* __restrict__ slots, EnthStatus* status,
+                                                           int mode, double rtol, double atol) {
+    __shared__ double sh[4];
+    if (status && mode == 1 && status->done) return;
+    double m = 0.0;
+    for (int q = threadIdx.x; q < kEnthRmaxSlots; q += blockDim.x) {
+        m = nan_max(m, __longlong_as_double((long long)slots[q]));
+        slots[q] = 0ull;
+    }
+    if (mode != 1) return;
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double r = nan_max(nan_max(sh[0], sh[1]), nan_max(sh[2], sh[3]));
+    if (status) record_sweep(status, 1, r, rtol, atol);
+}
+
 // Column Newton smoother (glide/cuda/enthalpy.cu:932-1112): Newton on the
 // column with horizontal neighbours frozen at E; writes delta = E_col - E
 // and/or E_out = E + omega_relax * delta.
@@ -375,7 +396,8 @@ __global__ __launch_bounds__(256) void k_enth_reduce(const double* __restrict__ 
 template <typename T>
 __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __restrict__ delta, T* __restrict__ E_out,
                                                             T omega_relax, int n_newton, T relaxation, int colour,
-                                                            T* E_mc) {
+                                                            T* E_mc, unsigned long long* __restrict__ rmax,
+                                                            T* __restrict__ r_out) {
     if (skipped(A.skip)) return;
     const long ncol = (long)A.ny * A.nx;
     ColIdx q;
@@ -411,6 +433,7 @@ __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __
     T El = E0;
     if (h < A.h_thin) {
         El = fmin(Es, e_pmp(T(k) * (T(1) / T(nz - 1)), h));
+        if (r_out && in) r_out[base + k] = T(0);
     } else {
         const T hinv = T(1) / h, hp = A.H_prev[q.col];
         const bool interior = in && k < nz - 1;
@@ -420,6 +443,7 @@ __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __
         const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
         Horiz<T> hz{};
         if ((A.flags & 1) && interior) hz = load_horiz(A, q.i, q.j, k);
+        double r_first = 0.0;  // |r| of the starting state (first Newton iteration's rows)
         for (int it = 0; it < n_newton; ++it) {
             const T Ekm1 = lane_get(El, max(k - 1, 0)), Ekp1 = lane_get(El, min(k + 1, 63));
             T Ku, Kl;
@@ -431,11 +455,23 @@ __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __
                 b = w.b;
                 c = w.c;
                 d = -(w.r - fEk);
+                if (it == 0) {
+                    r_first = fabs((double)d);
+                    if (r_out) r_out[base + k] = -d;
+                }
+            } else if (it == 0 && r_out && in) {
+                r_out[base + k] = T(0);
             } else if (k == nz - 1) {
                 d = -(El - Es);  // Dirichlet surface row
             }
             const T dx = pcr_solve(a, b, c, d);
             if (in) El += relaxation * dx;
+        }
+        if (rmax && n_newton > 0) {  // every lane of the wavefront is here (h is the column's)
+            double m = r_first;
+            for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_down(m, off, 64));
+            if (k == 0)
+                atomicMax(rmax + (wave_column() & (kEnthRmaxSlots - 1)), (unsigned long long)__double_as_longlong(m));
         }
     }
     if (!in) return;
@@ -1165,13 +1201,22 @@ hipError_t launch_enth_residual(const EnthArgs<T>& A, T* r_out, int use_forcing,
 }
 template <typename T>
 hipError_t launch_enth_column_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, int n_newton,
-                                     T relaxation, hipStream_t st, int colour, T* E_mc) {
+                                     T relaxation, hipStream_t st, int colour, T* E_mc, unsigned long long* rmax,
+                                     T* r_out) {
     if (colour >= 0 && (!E_mc || delta || E_out)) return hipErrorInvalidValue;
+    if (colour >= 0 && rmax) return hipErrorInvalidValue;
     if (colour == 1 && A.E != E_mc) return hipErrorInvalidValue;
     TimedLaunch tl(kTimeEnthColumn, (long)A.ny * A.nx * A.nz, st);
     // one wavefront per column, or per column pair of a row in a colour phase
     const unsigned blocks = colour < 0 ? col_blocks(A.ny, A.nx) : col_blocks(A.ny, (A.nx + 1) / 2);
-    k_enth_column_smooth<T><<<blocks, 256, 0, st>>>(A, delta, E_out, omega_relax, n_newton, relaxation, colour, E_mc);
+    k_enth_column_smooth<T><<<blocks, 256, 0, st>>>(A, delta, E_out, omega_relax, n_newton, relaxation, colour, E_mc,
+                                                    rmax, rmax ? r_out : nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_enth_reduce_slots(unsigned long long* slots, EnthStatus* status, int mode, double rtol, double atol,
+                                    hipStream_t st) {
+    k_enth_reduce_slots<<<1, 256, 0, st>>>(slots, status, mode, rtol, atol);
     return hipGetLastError();
 }
 template <typename T>
@@ -1310,7 +1355,8 @@ hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev
 #define INST(T)                                                                                                      \
     template hipError_t launch_enth_residual<T>(const EnthArgs<T>&, T*, int, double*, double*, EnthStatus*, int,     \
                                                 double, double, hipStream_t);                                        \
-    template hipError_t launch_enth_column_smooth<T>(const EnthArgs<T>&, T*, T*, T, int, T, hipStream_t, int, T*);  \
+    template hipError_t launch_enth_column_smooth<T>(const EnthArgs<T>&, T*, T*, T, int, T, hipStream_t, int, T*,   \
+                                                     unsigned long long*, T*);                                      \
     template hipError_t launch_enth_layer_smooth<T>(const EnthArgs<T>&, T*, T*, T, hipStream_t);                     \
     template hipError_t launch_enth_apply<T>(const EnthArgs<T>&, T*, const T*, T, hipStream_t);                      \
     template hipError_t launch_enth_omega<T>(const EnthArgs<T>&, const T*, const T*, T, T*, hipStream_t);            \

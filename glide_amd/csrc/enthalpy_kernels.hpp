@@ -32,6 +32,7 @@ struct EnthStatus {
 };
 
 constexpr int kEnthResBlocks = 2048;  // residual grid (grid-stride over columns)
+constexpr int kEnthRmaxSlots = 256;   // max-|r| accumulators of the column pass (bit patterns of doubles)
 
 // max |r| of the residual over the grid, into `max_out` (device), or, with
 // `status`, the sweep bookkeeping: mode 0 starts a sweep (r0), mode 1 ends an
@@ -45,9 +46,21 @@ hipError_t launch_enth_residual(const EnthArgs<T>& A, T* r_out, int use_forcing,
 // smooths the even columns ((i + j) even) from A.E into E_mc and copies the
 // odd ones (nothing to copy when E_mc == A.E), phase 1 smooths the odd
 // columns in place (A.E == E_mc).
+// rmax (optional, kEnthRmaxSlots values, zero on entry): the column pass
+// also forms max |r| of the state it starts from (its first Newton
+// iteration's rows, the residual launch_enth_residual computes) and folds it
+// into the slots (64-bit atomic max of the double's bit pattern, one slot per
+// wavefront modulo kEnthRmaxSlots; a NaN stays), and writes that residual to
+// r_out when given.
 template <typename T>
 hipError_t launch_enth_column_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, int n_newton,
-                                     T relaxation, hipStream_t st, int colour = -1, T* E_mc = nullptr);
+                                     T relaxation, hipStream_t st, int colour = -1, T* E_mc = nullptr,
+                                     unsigned long long* rmax = nullptr, T* r_out = nullptr);
+// The maximum over the slots of a column pass, zeroing them: mode 0 only
+// zeroes, mode 1 also closes one sweep iteration in the status record (as
+// launch_enth_residual's status mode 1; nothing once the sweep stopped).
+hipError_t launch_enth_reduce_slots(unsigned long long* slots, EnthStatus* status, int mode, double rtol, double atol,
+                                    hipStream_t st);
 // Pointwise Jacobi layer smoother; delta = -r / J_kk and/or E_out = E + omega_relax * delta.
 template <typename T>
 hipError_t launch_enth_layer_smooth(const EnthArgs<T>& A, T* delta, T* E_out, T omega_relax, hipStream_t st);
