@@ -109,6 +109,10 @@ def parse():
     ap.add_argument("--inv-directions", type=int, default=8)
     ap.add_argument("--inv-grounded", action="store_true",
                     help="raise the bed so that all ice is grounded (grounded fraction at its clamp)")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="the batched-ensemble workload instead (own JSON line, see ensemble())")
+    ap.add_argument("--ens-domains", type=int, default=96)
+    ap.add_argument("--ens-streams", type=int, default=8)
     return ap.parse_args()
 
 
@@ -592,6 +596,43 @@ def inversion(args, torch):
     print(json.dumps(line), flush=True)
 
 
+def ensemble(args, torch):
+    """--ensemble: the batched-ensemble workload.  --ens-domains valley
+    glaciers (glide_amd.ensemble.valley_glacier, dx = 100 m) with side lengths
+    drawn log-uniformly from 48 to 384 cells (seeded, multiples of 8), each
+    taking one forward + adjoint step (3 iterations each: FP32 red-black
+    multigrid with FP64 defect correction, no host wait) per ensemble step,
+    spread over --ens-streams streams; --warmup steps, then --steps timed.
+    The same with the FP64 form (3 reference-form V-cycles) for comparison.
+    Prints one JSON line with the summed steps/s."""
+    from glide_amd.ensemble import Ensemble, ensemble_sizes, valley_glacier
+    sizes = ensemble_sizes(args.ens_domains, seed=args.seed)
+    probs = [valley_glacier(n, seed=args.seed + 1000 + k) for k, n in enumerate(sizes)]
+    out = {"workload": (f"{len(probs)} valley glaciers, sides {min(sizes)}-{max(sizes)} cells "
+                        f"(log-uniform, seeded), dx = 100 m, one forward + adjoint step each per ensemble step, "
+                        "3 iterations per solve"),
+           "domains": len(probs), "cells": int(sum(n * n for n in sizes)), "streams": args.ens_streams}
+    for prec in ("mixed", "fp64"):
+        ens = Ensemble(probs, precision=prec, iterations=3, streams=args.ens_streams)
+        t = 0.0
+        for _ in range(args.warmup):
+            ens.step(t, 1.0)
+            t += 1.0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ens.step(t, 1.0)
+            t += 1.0
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        finite = all(bool(torch.isfinite(m.mg[0].state.u.data).all()) for m in ens.models)
+        out[prec] = {"domain_steps_per_s": round(len(probs) * args.steps / el, 2),
+                     "ensemble_step_ms": round(1000.0 * el / args.steps, 2), "finite": finite}
+        print(f"ensemble {prec}: {out[prec]}", flush=True)
+        del ens
+    print(json.dumps(out), flush=True)
+
+
 PROFILE_TOLERANCES = (3e-11, 1e-8, 3e-10)  # forward, enthalpy, adjoint: the calibrated values at 512^2
 
 
@@ -637,6 +678,9 @@ def main():
         return
     if args.inversion:
         inversion(args, torch)
+        return
+    if args.ensemble:
+        ensemble(args, torch)
         return
     truth = Form(args, torch, "truth")
     mixed = Form(args, torch, "mixed")

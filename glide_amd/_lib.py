@@ -155,6 +155,7 @@ SIGNATURES = {
                                       ctypes.POINTER(VankaConfigC), _P]),
     "glide_gradients": (_I, [_LP, _D, _I, _P]),
     "glide_graphs_enable": (_I, [_I]),
+    "glide_graphs_capacity": (_I, [_I]),
     "glide_adjoint_fused_enable": (_I, [_I]),
     "glide_forward_fused_enable": (_I, [_I]),
     "glide_forward_persistent_enable": (_I, [_I]),
@@ -199,6 +200,11 @@ TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3,
 def graphs_enable(on: bool) -> None:
     """Replay fixed-cycle FAS solves from cached HIP graphs (default on)."""
     call("glide_graphs_enable", int(bool(on)))
+
+
+def graphs_capacity(n: int) -> None:
+    """Number of cached graphs kept per device (default 8)."""
+    call("glide_graphs_capacity", int(n))
 
 
 def adjoint_fused_enable(on: bool) -> None:

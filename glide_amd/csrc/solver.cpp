@@ -671,7 +671,7 @@ struct DeviceGraphs {
     std::vector<GraphEntry> entries;
     unsigned long clock = 0;
 };
-constexpr size_t kMaxGraphs = 8;
+size_t g_max_graphs = 8;  // cached graphs per device (least recently used dropped)
 bool g_graphs_on = true;
 std::mutex g_graph_mutex;
 std::vector<DeviceGraphs> g_graphs;
@@ -735,7 +735,7 @@ int replay(const std::string& key, hipStream_t st, Body&& body) {
         hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
-        if (G.entries.size() >= kMaxGraphs) {
+        while (!G.entries.empty() && G.entries.size() >= g_max_graphs) {
             size_t lru = 0;
             for (size_t q = 1; q < G.entries.size(); ++q)
                 if (G.entries[q].last_use < G.entries[lru].last_use) lru = q;
@@ -1132,10 +1132,30 @@ std::vector<glide_vanka_config> reference_vanka_configs(int n) {
 // iterations, phase-2 iterations.
 constexpr double kPhaseSwitch = 0.6;
 
-template <typename Resid, typename Step, typename Undo, typename Fallback, typename Key>
+template <typename Resid, typename Step, typename Undo, typename Fallback, typename Key, typename Prep>
 int mixed_outer_loop(const glide_mixed_config& mc, hipStream_t st, Key&& key, bool two_phase,
                      glide_solve_report* rep, int* stats, Resid&& resid, Step&& step, Undo&& undo,
-                     Fallback&& fallback) {
+                     Fallback&& fallback, Prep&& prep) {
+    if (!rep && !(mc.relative_tolerance > 0.0) && !(mc.absolute_tolerance > 0.0)) {
+        // fixed count without a report: max_outer FP32 iterations enqueued
+        // without a host wait (prep forms what an iteration reads from the
+        // last residual); no norms, no undo or fallback
+        auto body = [&](hipStream_t cs) -> int {
+            for (int it = 0; it < mc.max_outer; ++it) {
+                GLIDE_TRY(prep(cs));
+                GLIDE_TRY(step(cs, 1));
+            }
+            return GLIDE_OK;
+        };
+        if (g_graphs_on && !timing_on()) GLIDE_TRY(replay(key(0), st, body));
+        else GLIDE_TRY(body(st));
+        if (stats) {
+            stats[0] = 0;
+            stats[1] = mc.max_outer;
+            stats[2] = 0;
+        }
+        return GLIDE_OK;
+    }
     DeviceScratch* s = nullptr;
     GLIDE_TRY(scratch_for_current_device(&s));
     auto read_norm = [&](double* n3) -> int {
@@ -1214,6 +1234,7 @@ std::string mixed_key(char tag, const glide_level* l64, const glide_level* l32, 
     key_append(k, cfg);
     key_append(k, mc.inner_vcycles);
     key_append(k, mc.fp64_coarsest);
+    key_append(k, mc.max_outer);
     key_append(k, g_spin_limit);
     key_append(k, g_persistent_fenced);
     for (int l = start; l < n_levels; ++l) {
@@ -1705,9 +1726,11 @@ int glide_fascd_solve_mixed(const glide_level* levels64, const glide_level* leve
         return fas_vcycle(F, start_level, true);
     };
     auto key = [&](int phase) {
-        return mixed_key(phase == 1 ? 'm' : 'f', levels64, levels32, n_levels, start_level, dt, *cfg, vc, *mc);
+        return mixed_key(phase == 0 ? 'a' : phase == 1 ? 'm' : 'f', levels64, levels32, n_levels, start_level, dt,
+                         *cfg, vc, *mc);
     };
-    return mixed_outer_loop(*mc, st, key, true, report, stats, resid, step, undo, fallback);
+    auto prep = [&](hipStream_t cs) -> int { return grounded(D, 0.0, cs); };
+    return mixed_outer_loop(*mc, st, key, true, report, stats, resid, step, undo, fallback, prep);
 }
 
 int glide_fas_adjoint_solve_mixed(const glide_level* levels64, const glide_level* levels32, int n_levels,
@@ -1803,10 +1826,14 @@ int glide_fas_adjoint_solve_mixed(const glide_level* levels64, const glide_level
         GLIDE_TRY(adjoint_prepare(D, dt, ref[start_level], 0, cs));
         return adj_vcycle(F, start_level, true);
     };
-    auto key = [&](int) {
-        return mixed_key('n', levels64, levels32, n_levels, start_level, dt, *cfg, vc, *mc);
+    auto key = [&](int phase) {
+        return mixed_key(phase == 0 ? 'b' : 'n', levels64, levels32, n_levels, start_level, dt, *cfg, vc, *mc);
     };
-    return mixed_outer_loop(*mc, st, key, false, report, stats, resid, step, undo, fallback);
+    auto prep = [&](hipStream_t cs) -> int {
+        return vjp(D, dt, GLIDE_USE_FORCING, CP(D.lambda_u), CP(D.lambda_v), CP(D.lambda_H), P(D.ar_u), P(D.ar_v),
+                   P(D.ar_H), cs);
+    };
+    return mixed_outer_loop(*mc, st, key, false, report, stats, resid, step, undo, fallback, prep);
 }
 
 int glide_adjoint_fused_enable(int on) {
@@ -1846,6 +1873,13 @@ int glide_adjoint_pass_fused_enable(int on) {
 
 int glide_graphs_enable(int on) {
     g_graphs_on = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_graphs_capacity(int n) {
+    if (n < 1 || n > 4096) return fail(GLIDE_ERR_INVALID_ARG, "the graph cache holds 1 to 4096 graphs");
+    std::lock_guard<std::mutex> lk(g_graph_mutex);
+    g_max_graphs = (size_t)n;
     return GLIDE_OK;
 }
 

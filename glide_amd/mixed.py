@@ -93,6 +93,9 @@ class MixedConfig:
     local Newton iteration takes newton_steps steps of `relaxation` (forward)
     and the adjoint passes move the multipliers by adjoint_omega times their
     patch solution; fp64_coarsest runs the coarsest level's passes in FP64.
+    asynchronous with both tolerances <= 0 runs exactly max_outer FP32
+    iterations without a host wait and without a report (no norms, no
+    fallback); the solve then returns None.
     compat_vprolong is off: the mixed form uses the v-facet prolongation."""
     max_outer: int = 30
     inner_vcycles: int = 1
@@ -107,6 +110,7 @@ class MixedConfig:
     relaxation: float = 0.5
     adjoint_omega: float = 0.85
     compat_vprolong: bool = False
+    asynchronous: bool = False
 
     def to_c(self) -> _lib.MixedConfigC:
         if not 0 <= int(self.max_outer) <= 64:
@@ -157,12 +161,13 @@ def _run(name, levels64, mh, start_level, dt, cfg, adjoint, stream):
     vcs = (_lib.VankaConfigC * n)()
     for k in range(n):
         vcs[k] = cfg.vanka_c(adjoint)
-    rep = _lib.SolveReport()
+    fixed = not (cfg.relative_tolerance > 0.0) and not (cfg.absolute_tolerance > 0.0)
+    rep = None if (cfg.asynchronous and fixed) else _lib.SolveReport()
     stats = (ctypes.c_int * 3)()
     fas = cfg.fas_c()
     mc = cfg.to_c()
     _lib.call(name, levels64, l32, n, int(start_level), float(dt), ctypes.byref(fas), vcs, ctypes.byref(mc),
-              ctypes.byref(rep), stats, stream)
+              ctypes.byref(rep) if rep is not None else None, stats, stream)
     return rep, {"fallbacks": stats[0], "fp32": stats[1], "fp64": stats[2]}
 
 

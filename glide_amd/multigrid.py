@@ -555,7 +555,7 @@ class FASCDSolver:
                                                    self.mixed_options, stream_handle(self.multigrid.device))
         self.last_fallbacks = self.last_stats["fallbacks"]
         self.last_report = rep
-        if self._fas_config.report_norms and report_norms:
+        if rep is not None and self._fas_config.report_norms and report_norms:
             _print_report(rep, list(rep.norms3))
         return rep
 
@@ -628,7 +628,7 @@ class FASAdjointSolver:
                                                        stream_handle(self.multigrid.device))
             self.last_fallbacks = self.last_stats["fallbacks"]
             self.last_report = rep
-            if self._fas_config.report_norms and report_norms:
+            if rep is not None and self._fas_config.report_norms and report_norms:
                 _print_report(rep, list(rep.norms3))
             return rep
         if self.precision != "fp64":

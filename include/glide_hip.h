@@ -214,7 +214,10 @@ int glide_fascd_solve(const glide_level* levels, int n_levels, int start_level, 
  * buffers of glide_fascd_solve on every level.  The
  * outer iteration is
  * replayed from a cached HIP graph and synchronises once per iteration for the
- * norm; the report holds the FP64 norm after each outer iteration. */
+ * norm; the report holds the FP64 norm after each outer iteration.  With both
+ * tolerances <= 0 and report NULL the solve runs exactly max_outer FP32
+ * iterations, enqueued (from one cached graph) without any host wait, norm,
+ * undo or fallback; glide_fas_adjoint_solve_mixed alike. */
 typedef struct glide_mixed_config {
     int32_t max_outer;
     int32_t inner_vcycles;
@@ -278,6 +281,9 @@ int glide_gradients(const glide_level* L, double dt, int which, void* stream);
  * unrelated data). */
 int glide_graphs_enable(int on);
 int glide_graphs_clear(void);
+/* Number of graphs kept per device (default 8; the least recently used is
+ * dropped), e.g. two per domain when an ensemble of domains is stepped. */
+int glide_graphs_capacity(int n);
 /* Adjoint passes of levels with an adj_cache (no counterpart in the
  * reference): 1 (default) runs each pass after the transpose operator as one
  * launch (patch solves of the tile and its left/top neighbours, then the
