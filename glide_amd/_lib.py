@@ -75,7 +75,7 @@ class SolveReport(ctypes.Structure):
 
 
 ENTH_POINTERS = ("E", "E_prev", "f_E", "omega", "phi_strain", "r_E", "delta_E", "E_work", "u3d", "v3d",
-                 "H", "H_prev", "E_surface", "Q_geo", "Q_fh", "dh_dt")
+                 "H", "H_prev", "E_surface", "Q_geo", "Q_fh", "dh_dt", "sweep_work")
 ENTH_MAX_SWEEPS = 256
 
 
@@ -177,6 +177,7 @@ SIGNATURES = {
                                      _P]),
     "glide_enth_column_sweep_noalloc": (_I, [_CP, _D, _I, ctypes.POINTER(EnthSweepConfigC),
                                              ctypes.POINTER(EnthReport), _P]),
+    "glide_enth_elementary": (_I, [_P, ctypes.c_int64, _P, _P, _P]),
     "glide_enth_omega": (_I, [_CP, _P, _P, _D, _P]),
     "glide_enth_broadcast_velocity": (_I, [_CP, _P, _P, _D, _P]),
     "glide_enth_frictional_heating": (_I, [_CP, _P, _D, _D, _P]),

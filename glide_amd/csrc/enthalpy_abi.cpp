@@ -171,62 +171,101 @@ int sweep_core(const glide_enth_column* C, double dt, int n_iter, const glide_en
     if (cfg->alternating) REQ(C->E_work);
     else if (!cfg->multicolour) REQ(C->delta_E);
     const double rtol = cfg->relative_tolerance, atol = cfg->absolute_tolerance;
-    HIP_TRY(launch_enth_residual<R>(A, P(C->r_E), 1, partials, nullptr, status, 0, rtol, atol, st),
-            "enthalpy residual");
-    EnthArgs<R> Ak = A;
-    Ak.skip = &status->done;
-    if (rmax && cfg->alternating && !cfg->multicolour && cfg->n_newton >= 1) {
-        // The reference's sweep with the residual of each iterate taken from
-        // the next column pass's first Newton rows (the same values the
-        // residual launch computes): sweep it's norm is closed after pass
-        // it + 1's column launch, whose layer launch then skips if the
-        // sweep stopped, leaving E at the iterate the reference stops at.
+    if (C->sweep_work && u2d && rmax && cfg->alternating && !cfg->multicolour && cfg->n_newton >= 1 && n_iter >= 1 &&
+        enth_lm_supported(C->nz)) {
+        // The same sweep on level-major copies (sweep_work: E, E_prev, f_E,
+        // omega planes; E_work holds the column pass's output in that layout).
+        const long ncol = (long)C->ny * C->nx, plane = ncol * C->nz;
+        R* wk = P(C->sweep_work);
+        R *Elm = wk, *Eplm = wk + plane, *fElm = wk + 2 * plane, *omlm = wk + 3 * plane, *Wlm = P(C->E_work);
+        const int nz = C->nz;
+        HIP_TRY(launch_enth_lm_layout<R>(CP(C->E), Elm, ncol, nz, true, st), "enthalpy level-major copy");
+        HIP_TRY(launch_enth_lm_layout<R>(CP(C->E_prev), Eplm, ncol, nz, true, st), "enthalpy level-major copy");
+        HIP_TRY(launch_enth_lm_layout<R>(CP(C->f_E), fElm, ncol, nz, true, st), "enthalpy level-major copy");
+        EnthArgs<R> L = A;
+        L.E = Elm;
+        L.E_prev = Eplm;
+        L.f_E = fElm;
+        if (A.omega) {
+            HIP_TRY(launch_enth_lm_layout<R>(A.omega, omlm, ncol, nz, true, st), "enthalpy level-major copy");
+            L.omega = omlm;
+        }
+        HIP_TRY(launch_enth_residual_lm<R>(L, nullptr, partials, status, 0, rtol, atol, st), "enthalpy residual");
+        L.skip = &status->done;
+        EnthArgs<R> Lw = L;
+        Lw.E = Wlm;
         HIP_TRY(launch_enth_reduce_slots(rmax, status, 0, rtol, atol, st), "enthalpy max slots");
-        EnthArgs<R> Aw = Ak;
-        Aw.E = CP(C->E_work);
         for (int it = 0; it < n_iter; ++it) {
-            HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, P(C->E_work), cfg->omega, cfg->n_newton,
-                                                 cfg->relaxation, st, -1, nullptr, it > 0 ? rmax : nullptr,
-                                                 P(C->r_E)),
+            HIP_TRY(launch_enth_column_lm<R>(L, Wlm, cfg->omega, cfg->n_newton, cfg->relaxation,
+                                             it > 0 ? rmax : nullptr, st),
                     "enthalpy column smoother");
             if (it > 0) HIP_TRY(launch_enth_reduce_slots(rmax, status, 1, rtol, atol, st), "enthalpy sweep norm");
-            HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
+            HIP_TRY(launch_enth_layer_lm<R>(Lw, Elm, cfg->omega, st), "enthalpy layer smoother");
         }
-        if (n_iter > 0)
+        // the final iterate's residual into r_E even when the sweep stopped
+        // early (the bookkeeping closes nothing then)
+        EnthArgs<R> Lf = L;
+        Lf.skip = nullptr;
+        HIP_TRY(launch_enth_residual_lm<R>(Lf, P(C->r_E), partials, status, 1, rtol, atol, st), "enthalpy residual");
+        HIP_TRY(launch_enth_lm_layout<R>(Elm, P(C->E), ncol, nz, false, st), "enthalpy layout");
+    } else {
+        HIP_TRY(launch_enth_residual<R>(A, P(C->r_E), 1, partials, nullptr, status, 0, rtol, atol, st),
+                "enthalpy residual");
+        EnthArgs<R> Ak = A;
+        Ak.skip = &status->done;
+        if (rmax && cfg->alternating && !cfg->multicolour && cfg->n_newton >= 1) {
+            // The reference's sweep with the residual of each iterate taken from
+            // the next column pass's first Newton rows (the same values the
+            // residual launch computes): sweep it's norm is closed after pass
+            // it + 1's column launch, whose layer launch then skips if the
+            // sweep stopped, leaving E at the iterate the reference stops at.
+            HIP_TRY(launch_enth_reduce_slots(rmax, status, 0, rtol, atol, st), "enthalpy max slots");
+            EnthArgs<R> Aw = Ak;
+            Aw.E = CP(C->E_work);
+            for (int it = 0; it < n_iter; ++it) {
+                HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, P(C->E_work), cfg->omega, cfg->n_newton,
+                                                     cfg->relaxation, st, -1, nullptr, it > 0 ? rmax : nullptr,
+                                                     P(C->r_E)),
+                        "enthalpy column smoother");
+                if (it > 0) HIP_TRY(launch_enth_reduce_slots(rmax, status, 1, rtol, atol, st), "enthalpy sweep norm");
+                HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
+            }
+            if (n_iter > 0)
+                HIP_TRY(launch_enth_residual<R>(Ak, P(C->r_E), 1, partials, nullptr, status, 1, rtol, atol, st),
+                        "enthalpy residual");
+        } else
+        for (int it = 0; it < n_iter; ++it) {
+            if (cfg->multicolour && !cfg->alternating) {  // both colour phases in place on E
+                for (int colour = 0; colour < 2; ++colour)
+                    HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation,
+                                                         st, colour, P(C->E)),
+                            "enthalpy column smoother (red-black)");
+            } else if (cfg->multicolour) {
+                HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation, st,
+                                                     0, P(C->E_work)),
+                        "enthalpy column smoother (even columns)");
+                EnthArgs<R> Aw = Ak;
+                Aw.E = CP(C->E_work);
+                HIP_TRY(launch_enth_column_smooth<R>(Aw, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation, st,
+                                                     1, P(C->E_work)),
+                        "enthalpy column smoother (odd columns)");
+                HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
+            } else if (cfg->alternating) {
+                HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, P(C->E_work), cfg->omega, cfg->n_newton,
+                                                     cfg->relaxation, st),
+                        "enthalpy column smoother");
+                EnthArgs<R> Aw = Ak;
+                Aw.E = CP(C->E_work);
+                HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
+            } else {
+                HIP_TRY(launch_enth_column_smooth<R>(Ak, P(C->delta_E), nullptr, 0.0, cfg->n_newton, cfg->relaxation,
+                                                     st),
+                        "enthalpy column smoother");
+                HIP_TRY(launch_enth_apply<R>(Ak, P(C->E), CP(C->delta_E), cfg->omega, st), "enthalpy update");
+            }
             HIP_TRY(launch_enth_residual<R>(Ak, P(C->r_E), 1, partials, nullptr, status, 1, rtol, atol, st),
                     "enthalpy residual");
-    } else
-    for (int it = 0; it < n_iter; ++it) {
-        if (cfg->multicolour && !cfg->alternating) {  // both colour phases in place on E
-            for (int colour = 0; colour < 2; ++colour)
-                HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation,
-                                                     st, colour, P(C->E)),
-                        "enthalpy column smoother (red-black)");
-        } else if (cfg->multicolour) {
-            HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation, st,
-                                                 0, P(C->E_work)),
-                    "enthalpy column smoother (even columns)");
-            EnthArgs<R> Aw = Ak;
-            Aw.E = CP(C->E_work);
-            HIP_TRY(launch_enth_column_smooth<R>(Aw, nullptr, nullptr, cfg->omega, cfg->n_newton, cfg->relaxation, st,
-                                                 1, P(C->E_work)),
-                    "enthalpy column smoother (odd columns)");
-            HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
-        } else if (cfg->alternating) {
-            HIP_TRY(launch_enth_column_smooth<R>(Ak, nullptr, P(C->E_work), cfg->omega, cfg->n_newton,
-                                                 cfg->relaxation, st),
-                    "enthalpy column smoother");
-            EnthArgs<R> Aw = Ak;
-            Aw.E = CP(C->E_work);
-            HIP_TRY(launch_enth_layer_smooth<R>(Aw, nullptr, P(C->E), cfg->omega, st), "enthalpy layer smoother");
-        } else {
-            HIP_TRY(launch_enth_column_smooth<R>(Ak, P(C->delta_E), nullptr, 0.0, cfg->n_newton, cfg->relaxation,
-                                                 st),
-                    "enthalpy column smoother");
-            HIP_TRY(launch_enth_apply<R>(Ak, P(C->E), CP(C->delta_E), cfg->omega, st), "enthalpy update");
         }
-        HIP_TRY(launch_enth_residual<R>(Ak, P(C->r_E), 1, partials, nullptr, status, 1, rtol, atol, st),
-                "enthalpy residual");
     }
     if (report) {
         EnthStatus local;
@@ -315,6 +354,14 @@ int glide_enth_apply(const glide_enth_column* C, double omega, void* stream) {
     REQ(C->E);
     REQ(C->delta_E);
     HIP_TRY(launch_enth_apply<R>(A, P(C->E), CP(C->delta_E), omega, S(stream)), "enthalpy update");
+    return GLIDE_OK;
+}
+
+int glide_enth_elementary(const void* x, int64_t n, void* exp_out, void* log_out, void* stream) {
+    REQ(x);
+    if (n < 0) return fail(GLIDE_ERR_INVALID_ARG, "n must be non-negative");
+    if (n == 0) return GLIDE_OK;
+    HIP_TRY(launch_enth_elementary(CP(x), (long)n, P(exp_out), P(log_out), S(stream)), "enthalpy elementary functions");
     return GLIDE_OK;
 }
 

@@ -38,16 +38,75 @@ template <typename T>
 __device__ __forceinline__ T t_pmp(T sigma, T H) {
     return T(kTMelt) - T(kClausius) * ((T(1) - sigma) * H);
 }
+// Elementary functions of the rows, written for the arguments the rows pass
+// (finite, far from overflow): without the general routines' special cases
+// each takes well under half their instructions, and each stays within a few
+// ulp of the correctly rounded value (tests compare them with the general
+// routines over the rows' argument ranges).
+//
+// e^x for finite |x| <= 700: Cody-Waite reduction x = n ln2 + r with ln2 in
+// two parts (the reduction by fused multiply-adds), |r| <= ln2 / 2, then e^r
+// by its Taylor polynomial to degree 13 (the first omitted term is below
+// 6e-18 relative) in Horner form, scaled by 2^n.
+__device__ __forceinline__ double exp_rows(double x) {
+    constexpr double kLog2e = 1.4426950408889634, kLn2Hi = 0x1.62e42fefa39efp-1, kLn2Lo = 0x1.abc9e3b39803fp-56;
+    const double n = rint(x * kLog2e);
+    const double r = fma(-n, kLn2Lo, fma(-n, kLn2Hi, x));
+    double p = 1.0 / 6227020800.0;  // 1 / 13!
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+// log(u) for finite normal u > 0: u = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// log m = 2 atanh(s), s = (m - 1) / (m + 1) (m - 1 exact), |s| <= 0.1716, by
+// the atanh series 2 s (1 + s^2/3 + ... + s^18/19) (the first omitted term is
+// below 3e-17 relative), plus e ln2 in two parts.
+__device__ __forceinline__ double log_rows(double u) {
+    constexpr double kLn2Hi = 0x1.62e42fefa39efp-1, kLn2Lo = 0x1.abc9e3b39803fp-56;
+    double m = __builtin_amdgcn_frexp_mant(u);  // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(u);
+    if (m < 0.7071067811865476) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double s = (m - 1.0) * fast_rcp(m + 1.0), s2 = s * s;
+    double p = 1.0 / 19.0;
+    p = fma(p, s2, 1.0 / 17.0);
+    p = fma(p, s2, 1.0 / 15.0);
+    p = fma(p, s2, 1.0 / 13.0);
+    p = fma(p, s2, 1.0 / 11.0);
+    p = fma(p, s2, 1.0 / 9.0);
+    p = fma(p, s2, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    p = fma(p, s2, 1.0);
+    const double de = (double)e;
+    return fma(de, kLn2Hi, fma(de, kLn2Lo, 2.0 * s * p));
+}
+
+// The constant quotients below are formed at compile time, so the hot
+// helpers multiply instead of dividing (within an ulp or two of the quotient).
 template <typename T>
 __device__ __forceinline__ T e_pmp(T sigma, T H) {
-    return T(kCI) * (t_pmp(sigma, H) - T(kTRef)) / T(kEScale);
+    return (t_pmp(sigma, H) - T(kTRef)) * T(kCI / kEScale);
 }
 // glide/cuda/enthalpy.cu:48-98
 template <typename T>
 __device__ __forceinline__ T diffusivity(T E, T Ep) {
     T z = (E - Ep) * T(kEScale) * T(0.01);
     z = fmin(fmax(z, T(-20)), T(20));
-    T s = T(1) / (T(1) + exp(-z));
+    T s = fast_rcp(T(1) + exp_rows(-z));
     return T(kKCold) * (T(1) - s + T(kKTempFactor) * s);
 }
 // Drained water content (glide/cuda/enthalpy.cu:48-98): a softplus of the
@@ -61,9 +120,9 @@ template <typename T>
 __device__ __forceinline__ void water_and_slope(T E, T Ep, T& w, T& dw) {
     const T x = (E - Ep) * T(kEScale);
     const T ax = fmin(fmax(T(kDrainSharpness) * x, T(-20)), T(20));
-    const T ez = exp(ax);
-    w = ((ax > T(10)) ? x : log(T(1) + ez) / T(kDrainSharpness)) / T(kLHeat);
-    dw = (ez / (T(1) + ez)) / T(kLHeat);
+    const T ez = exp_rows(ax);
+    w = ((ax > T(10)) ? x : log_rows(T(1) + ez) * T(1.0 / kDrainSharpness)) * T(1.0 / kLHeat);
+    dw = (ez * fast_rcp(T(1) + ez)) * T(1.0 / kLHeat);
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
@@ -174,16 +233,77 @@ __device__ __forceinline__ void interface_K(int k, int nz, T h, T Ek, T Ekp1, T&
     Kl = lane_get(Ku, max(k - 1, 0));
 }
 
-// Ku / Kl: the interface diffusivities from interface_K.
+// Per-column constants of the rows: functions of nz, dt and the column's
+// thickness only, formed once per column so that the rows divide by nothing.
 template <typename T>
-__device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, int k, T h, T hinv, T hp, T Ek, T Ekm1, T Ekp1,
-                                            T Epk, T om_m, T om_k, T om_p, const Horiz<T>& hz, T Ku, T Kl) {
-    const int nz = A.nz;
-    const T dsig = T(1) / T(nz - 1);
-    const T invdt = T(1) / A.dt;
+struct ColConst {
+    T dsig, di, hdi, d2, idd;  // 1/(nz-1), 1/dsig, 1/(dsig/2), 1/dsig^2, 1/(dsig dsig/2)
+    T hinv, invdt;             // 1/H, 1/dt
+};
+template <typename T>
+__device__ __forceinline__ ColConst<T> col_const(const EnthArgs<T>& A, T h) {
+    ColConst<T> c;
+    c.dsig = T(1) / T(A.nz - 1);
+    c.di = T(1) / c.dsig;
+    c.hdi = T(1) / (T(0.5) * c.dsig);
+    c.d2 = T(1) / (c.dsig * c.dsig);
+    c.idd = T(1) / (c.dsig * (T(0.5) * c.dsig));
+    c.hinv = T(1) / h;
+    c.invdt = T(1) / A.dt;
+    return c;
+}
+
+// The column's own terms of level k's row (sigma advection, diffusion with
+// the interface diffusivities Ku / Kl from interface_K, drainage), added to
+// w, which holds the time and horizontal terms.
+template <typename T>
+__device__ __forceinline__ void column_terms(const EnthArgs<T>& A, const ColConst<T>& c, int k, T h, T Ek, T Ekm1,
+                                             T Ekp1, T om_m, T om_k, T om_p, T Ku, T Kl, Row<T>& w) {
+    const T Ep_k = e_pmp(T(k) * c.dsig, h);
+    if (k == 0) {
+        if (A.flags & 2) {
+            T wh = T(0.5) * (om_k + om_p);
+            T wp = fmax(wh, T(0)), wn = fmin(wh, T(0));
+            w.r += T(kRhoI) * ((wp * Ek + wn * Ekp1) - om_k * Ek) * c.hdi;
+            w.b += T(kRhoI) * (wp - om_k) * c.hdi;
+            w.c += T(kRhoI) * wn * c.hdi;
+        }
+        // half-cell diffusion flux at the bed: -K (E_1 - E_0) / (H dsig) over dsig / 2
+        const T co = c.hinv * Ku * c.idd;
+        w.r += -co * (Ekp1 - Ek);
+        w.b += co;
+        w.c -= co;
+    } else {
+        if (A.flags & 2) {
+            T wu = T(0.5) * (om_k + om_p), wl = T(0.5) * (om_m + om_k);
+            T up = fmax(wu, T(0)), un = fmin(wu, T(0)), lp = fmax(wl, T(0)), ln = fmin(wl, T(0));
+            w.r += T(kRhoI) * ((up * Ek + un * Ekp1) - (lp * Ekm1 + ln * Ek)) * c.di;
+            w.a -= T(kRhoI) * lp * c.di;
+            w.c += T(kRhoI) * un * c.di;
+            w.b += T(kRhoI) * (up - ln) * c.di;
+        }
+        w.r += -c.hinv * c.d2 * (Ku * (Ekp1 - Ek) - Kl * (Ek - Ekm1));
+        T dl = c.hinv * c.d2 * Kl, du = c.hinv * c.d2 * Ku;
+        w.a -= dl;
+        w.c -= du;
+        w.b += dl + du;
+    }
+    if (A.flags & 4) {
+        T wc, dwc;
+        water_and_slope(Ek, Ep_k, wc, dwc);
+        w.r += h * T(kRhoW) * T(kLHeat) * (A.drain_rate * wc) * T(1.0 / kEScale);
+        w.b += h * T(kRhoW) * T(kLHeat) * A.drain_rate * dwc;
+    }
+}
+
+// One level's residual (without forcing) and tridiagonal Jacobian row.
+template <typename T>
+__device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, const ColConst<T>& c, int k, T h, T hp, T Ek,
+                                            T Ekm1, T Ekp1, T Epk, T om_m, T om_k, T om_p, const Horiz<T>& hz, T Ku,
+                                            T Kl) {
     Row<T> w;
-    w.r = T(kRhoI) * (h * Ek - hp * Epk) * invdt;
-    w.b = T(kRhoI) * h * invdt;
+    w.r = T(kRhoI) * (h * Ek - hp * Epk) * c.invdt;
+    w.b = T(kRhoI) * h * c.invdt;
     w.a = T(0);
     w.c = T(0);
     if (A.flags & 1) {
@@ -192,45 +312,7 @@ __device__ __forceinline__ Row<T> level_row(const EnthArgs<T>& A, int k, T h, T 
         w.r += hr;
         w.b += hd;
     }
-    const T Ep_k = e_pmp(T(k) * dsig, h);
-    if (k == 0) {
-        if (A.flags & 2) {
-            T wh = T(0.5) * (om_k + om_p);
-            T wp = fmax(wh, T(0)), wn = fmin(wh, T(0));
-            T hdi = T(1) / (T(0.5) * dsig);
-            w.r += T(kRhoI) * ((wp * Ek + wn * Ekp1) - om_k * Ek) * hdi;
-            w.b += T(kRhoI) * (wp - om_k) * hdi;
-            w.c += T(kRhoI) * wn * hdi;
-        }
-        T Kh = Ku;
-        T dh = T(0.5) * dsig;
-        w.r += (-hinv * Kh * (Ekp1 - Ek) / dsig) / dh;
-        T co = hinv * Kh / (dsig * dh);
-        w.b += co;
-        w.c -= co;
-    } else {
-        if (A.flags & 2) {
-            T wu = T(0.5) * (om_k + om_p), wl = T(0.5) * (om_m + om_k);
-            T up = fmax(wu, T(0)), un = fmin(wu, T(0)), lp = fmax(wl, T(0)), ln = fmin(wl, T(0));
-            T di = T(1) / dsig;
-            w.r += T(kRhoI) * ((up * Ek + un * Ekp1) - (lp * Ekm1 + ln * Ek)) * di;
-            w.a -= T(kRhoI) * lp * di;
-            w.c += T(kRhoI) * un * di;
-            w.b += T(kRhoI) * (up - ln) * di;
-        }
-        T d2 = T(1) / (dsig * dsig);
-        w.r += -hinv * d2 * (Ku * (Ekp1 - Ek) - Kl * (Ek - Ekm1));
-        T dl = hinv * d2 * Kl, du = hinv * d2 * Ku;
-        w.a -= dl;
-        w.c -= du;
-        w.b += dl + du;
-    }
-    if (A.flags & 4) {
-        T wc, dwc;
-        water_and_slope(Ek, Ep_k, wc, dwc);
-        w.r += h * T(kRhoW) * T(kLHeat) * (A.drain_rate * wc) / T(kEScale);
-        w.b += h * T(kRhoW) * T(kLHeat) * A.drain_rate * dwc;
-    }
+    column_terms(A, c, k, h, Ek, Ekm1, Ekp1, om_m, om_k, om_p, Ku, Kl, w);
     return w;
 }
 
@@ -306,7 +388,7 @@ __global__ __launch_bounds__(256) void k_enth_residual(EnthArgs<T> A, T* __restr
         if (in && k < nz - 1 && !(h < A.h_thin)) {
             Horiz<T> hz{};
             if (A.flags & 1) hz = load_horiz(A, q.i, q.j, k);
-            Row<T> w = level_row(A, k, h, T(1) / h, A.H_prev[col], Ek, Ekm1, Ekp1, A.E_prev[base + k], omm, omk,
+            Row<T> w = level_row(A, col_const(A, h), k, h, A.H_prev[col], Ek, Ekm1, Ekp1, A.E_prev[base + k], omm, omk,
                                  omp, hz, Ku, Kl);
             r = use_forcing ? w.r - A.f_E[base + k] : w.r;
         }
@@ -435,7 +517,8 @@ __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __
         El = fmin(Es, e_pmp(T(k) * (T(1) / T(nz - 1)), h));
         if (r_out && in) r_out[base + k] = T(0);
     } else {
-        const T hinv = T(1) / h, hp = A.H_prev[q.col];
+        const ColConst<T> cst = col_const(A, h);
+        const T hp = A.H_prev[q.col];
         const bool interior = in && k < nz - 1;
         const T Epk = in ? A.E_prev[base + k] : T(0);
         const T fEk = in ? A.f_E[base + k] : T(0);
@@ -450,7 +533,7 @@ __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __
             interface_K(k, nz, h, El, Ekp1, Ku, Kl);
             T a = T(0), b = T(1), c = T(0), d = T(0);
             if (interior) {
-                Row<T> w = level_row(A, k, h, hinv, hp, El, Ekm1, Ekp1, Epk, omm, omk, omp, hz, Ku, Kl);
+                Row<T> w = level_row(A, cst, k, h, hp, El, Ekm1, Ekp1, Epk, omm, omk, omp, hz, Ku, Kl);
                 a = k == 0 ? T(0) : w.a;
                 b = w.b;
                 c = w.c;
@@ -459,10 +542,9 @@ __global__ __launch_bounds__(256) void k_enth_column_smooth(EnthArgs<T> A, T* __
                     r_first = fabs((double)d);
                     if (r_out) r_out[base + k] = -d;
                 }
-            } else if (it == 0 && r_out && in) {
-                r_out[base + k] = T(0);
-            } else if (k == nz - 1) {
-                d = -(El - Es);  // Dirichlet surface row
+            } else {
+                if (it == 0 && r_out && in) r_out[base + k] = T(0);
+                if (k == nz - 1) d = -(El - Es);  // Dirichlet surface row
             }
             const T dx = pcr_solve(a, b, c, d);
             if (in) El += relaxation * dx;
@@ -506,7 +588,7 @@ __global__ __launch_bounds__(256) void k_enth_layer_smooth(EnthArgs<T> A, T* __r
     if (k < nz - 1 && !(h < A.h_thin)) {
         Horiz<T> hz{};
         if (A.flags & 1) hz = load_horiz(A, q.i, q.j, k);
-        Row<T> w = level_row(A, k, h, T(1) / h, A.H_prev[q.col], Ek, Ekm1, Ekp1, A.E_prev[base + k], omm, omk, omp,
+        Row<T> w = level_row(A, col_const(A, h), k, h, A.H_prev[q.col], Ek, Ekm1, Ekp1, A.E_prev[base + k], omm, omk, omp,
                              hz, Ku, Kl);
         const T r = w.r - A.f_E[base + k];
         out = (fabs(w.b) > T(1e-30)) ? -r / w.b : T(0);
@@ -1178,6 +1260,381 @@ __global__ __launch_bounds__(256) void k_enth_adj_grads(EnthArgs<T> A, const T* 
     }
 }
 
+// The rows' exponential and logarithm over an array (for tests of their
+// accuracy): e_out = exp_rows(x), l_out = log_rows(x) where x > 0 (else NaN).
+__global__ __launch_bounds__(256) void k_enth_elementary(const double* __restrict__ x, long n,
+                                                         double* __restrict__ e_out, double* __restrict__ l_out) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const double v = x[q];
+        if (e_out) e_out[q] = exp_rows(v);
+        if (l_out) l_out[q] = v > 0.0 ? log_rows(v) : __builtin_nan("");
+    }
+}
+hipError_t launch_enth_elementary(const double* x, long n, double* e_out, double* l_out, hipStream_t st) {
+    const long b = (n + 255) / 256;
+    k_enth_elementary<<<(unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096), 256, 0, st>>>(x, n, e_out, l_out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------ level-major fused sweep
+// (no counterpart in the reference, whose sweep works on the (ny, nx, nz)
+// arrays, glide/enthalpy.py:426-471).  The sweep of a thermal step runs on
+// (nz, ny * nx) copies of E, E_prev, f_E and omega, in which level k of a run
+// of columns is contiguous.  The column pass gives a column P consecutive
+// lanes (4, 8 or 16), lane q holding levels [q MB, (q + 1) MB), MB = 3 or 4
+// (levels from nz on are identity rows), and solves the column's tridiagonal
+// Newton system by a partitioned Thomas algorithm: each lane eliminates its
+// block down to two rows coupling the block's first and last unknowns to the
+// neighbouring blocks', the first rows are eliminated into one row per lane
+// in the blocks' last unknowns, which cyclic reduction across the P lanes
+// solves, and each lane substitutes back.  Neighbouring lanes exchange values
+// through DPP row shifts, not LDS.  The layer pass and the residual take one
+// thread per column and stream up the levels.  The rows are level_row's, with
+// the terms that do not depend on the column's own enthalpy (the previous
+// enthalpy, the forcing and the neighbouring columns' advection) gathered
+// into one value per level.
+
+// Row shifts of a double across the lanes of each 16-lane row (DPP): lane
+// i reads lane i - S (row_shr) or i + S (row_shl) of its row, 0 where that
+// lane is outside the row.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int S>
+__device__ __forceinline__ double from_below(double x) {  // lane i - S
+    static_assert(S >= 1 && S <= 15, "row shift");
+    return dpp_f64<0x110 + S>(x);
+}
+template <int S>
+__device__ __forceinline__ double from_above(double x) {  // lane i + S
+    static_assert(S >= 1 && S <= 15, "row shift");
+    return dpp_f64<0x100 + S>(x);
+}
+
+constexpr int kLmCols = 64;  // columns per block of the layout transposes
+
+// (ny nx, nz) -> (nz, ny nx) (to_lm) or back, kLmCols columns per block
+// through LDS; both sides are read and written in contiguous runs.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_lm_layout(const T* __restrict__ src, T* __restrict__ dst, long ncol,
+                                                        int nz, int to_lm) {
+    __shared__ T s[kLmCols * 65];
+    const long c0 = (long)blockIdx.x * kLmCols;
+    const int nc = (int)min((long)kLmCols, ncol - c0);
+    if (to_lm) {
+        for (int q = threadIdx.x; q < nc * nz; q += blockDim.x) s[(q / nz) * 65 + q % nz] = src[c0 * nz + q];
+        __syncthreads();
+        for (int q = threadIdx.x; q < kLmCols * nz; q += blockDim.x) {
+            const int k = q / kLmCols, c = q % kLmCols;
+            if (c < nc) dst[(long)k * ncol + c0 + c] = s[c * 65 + k];
+        }
+    } else {
+        for (int q = threadIdx.x; q < kLmCols * nz; q += blockDim.x) {
+            const int k = q / kLmCols, c = q % kLmCols;
+            if (c < nc) s[c * 65 + k] = src[(long)k * ncol + c0 + c];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nc * nz; q += blockDim.x) dst[c0 * nz + q] = s[(q / nz) * 65 + q % nz];
+    }
+}
+
+// Horizontal Lax-Friedrichs advection of a column split by its unknowns
+// (horiz_adv with the flux derivatives as coefficients): the coefficient of
+// the column's own enthalpy (horiz_adv's diagonal) and those of the
+// neighbours' E_xp, E_xm, E_ym, E_yp.  Depth-constant velocity (A.u2d).
+template <typename T>
+struct LmHoriz {
+    T self, xp, xm, ym, yp;
+};
+template <typename T>
+__device__ __forceinline__ LmHoriz<T> lm_horiz(const EnthArgs<T>& A, int i, int j) {
+    LmHoriz<T> c{T(0), T(0), T(0), T(0), T(0)};
+    if (!(A.flags & 1)) return c;
+    const Horiz<T> hz = face_fluxes(A, i, j, 0);
+    const T sq = A.lf_sq, f = T(kRhoI) * A.dxi;
+    auto dl = [&](T U) { return T(0.5) * (U + fabs(U) * sq); };
+    auto dr = [&](T U) { return T(0.5) * (U - fabs(U) * sq); };
+    c.self = f * ((dl(hz.Hu_r) - dr(hz.Hu_l)) + (dl(hz.Hv_t) - dr(hz.Hv_b)));
+    c.xp = f * dr(hz.Hu_r);
+    c.xm = -f * dl(hz.Hu_l);
+    c.ym = f * dr(hz.Hv_t);
+    c.yp = -f * dl(hz.Hv_b);
+    return c;
+}
+
+// The E-independent part of level k's row on the (nz, ny nx) planes:
+// -rho H_prev E_prev / dt - f_E plus the neighbours' horizontal advection.
+template <typename T>
+__device__ __forceinline__ T lm_constant(const EnthArgs<T>& A, const LmHoriz<T>& hc, long ncol, long col, int k, int i,
+                                         int j, T hp, T invdt) {
+    const long o = (long)k * ncol;
+    T g = -(T(kRhoI) * hp * A.E_prev[o + col] * invdt) - A.f_E[o + col];
+    if (A.flags & 1) {
+        const int nx = A.nx, im = max(i - 1, 0), ip = min(i + 1, A.ny - 1), jm = max(j - 1, 0),
+                  jp = min(j + 1, nx - 1);
+        const T* E = A.E + o;
+        g += (hc.xp * E[(long)i * nx + jp] + hc.xm * E[(long)i * nx + jm]) +
+             (hc.ym * E[(long)im * nx + j] + hc.yp * E[(long)ip * nx + j]);
+    }
+    return g;
+}
+
+// One stage of parallel cyclic reduction over the P lanes of a column
+// group, one row a z_{q-1} + b z_q + c z_{q+1} = d per lane: a = 0 on the
+// group's first S lanes and c = 0 on its last S lanes (true of the initial
+// rows at S = 1 and kept by every stage), so values read across the group's
+// edge are multiplied by zero.
+template <typename T, int S>
+__device__ __forceinline__ void pcr_stage(T& a, T& b, T& c, T& d) {
+    const T rb = fast_rcp(b);
+    const T an = a * rb, cn = c * rb, dn = d * rb;
+    const T am = from_below<S>(an), cm = from_below<S>(cn), dm = from_below<S>(dn);
+    const T ap = from_above<S>(an), cp = from_above<S>(cn), dp = from_above<S>(dn);
+    const T na = -a * am, nc = -c * cp;
+    b = b - a * cm - c * ap;
+    d = d - a * dm - c * dp;
+    a = na;
+    c = nc;
+}
+
+// Column Newton smoother on the level-major planes (A.E etc. in (nz, ny nx)
+// layout): W = E + omega_relax (E_col - E), and with rmax the max |r| of the
+// starting state (first Newton iteration's rows) folded into the slots as
+// k_enth_column_smooth does.  P lanes per column (4 or 8, consecutive lanes
+// of one 16-lane row), MB >= 3 levels per lane, P MB >= nz.
+template <typename T, int P, int MB>
+__global__ __launch_bounds__(256) void k_enth_column_lm(EnthArgs<T> A, T* __restrict__ W, T omega_relax,
+                                                        int n_newton, T relaxation,
+                                                        unsigned long long* __restrict__ rmax) {
+    static_assert(MB >= 3, "the partitioned solve needs three levels per lane");
+    static_assert(P == 4 || P == 8 || P == 16, "4, 8 or 16 lanes per column (within a 16-lane DPP row)");
+    constexpr int kCols = 256 / P;  // columns per block
+    if (skipped(A.skip)) return;
+    const int nz = A.nz, nx = A.nx;
+    const long ncol = (long)A.ny * nx;
+    const long nvb = (ncol + kCols - 1) / kCols;
+    const long col = xcd_block(blockIdx.x, nvb) * kCols + (threadIdx.x / P);
+    const bool valid = col < ncol;  // the same for the P lanes of a column
+    const long cc = valid ? col : ncol - 1;
+    const int q = threadIdx.x % P, k0 = q * MB;
+    const int i = (int)(cc / nx), j = (int)(cc % nx);
+    const T h = A.H[cc], Es = A.E_surface[cc];
+    const T dsig = T(1) / T(nz - 1);
+    T El[MB];
+#pragma unroll
+    for (int m = 0; m < MB; ++m) El[m] = (k0 + m < nz) ? A.E[(long)(k0 + m) * ncol + cc] : T(0);
+    double r_first = 0.0;
+    if (h < A.h_thin) {
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+            if (k0 + m < nz) El[m] = fmin(Es, e_pmp(T(k0 + m) * dsig, h));
+    } else {
+        const ColConst<T> cst = col_const(A, h);
+        const T hp = A.H_prev[cc], invdt = cst.invdt;
+        const LmHoriz<T> hc = lm_horiz(A, i, j);
+        const T B0 = T(kRhoI) * h * invdt + hc.self;
+        T G[MB], om[MB];
+#pragma unroll
+        for (int m = 0; m < MB; ++m) {
+            const int k = k0 + m;
+            G[m] = (k < nz - 1) ? lm_constant(A, hc, ncol, cc, k, i, j, hp, invdt) : T(0);
+            om[m] = (A.omega && k < nz) ? A.omega[(long)k * ncol + cc] : T(0);
+        }
+        // omega one level below and above the block (clamped as the column kernel's)
+        const T om_lo = (A.omega && k0 < nz) ? A.omega[(long)max(k0 - 1, 0) * ncol + cc] : T(0);
+        const T om_hi = A.omega ? A.omega[(long)min(k0 + MB, nz - 1) * ncol + cc] : T(0);
+        for (int it = 0; it < n_newton; ++it) {
+            // The per-level terms that do not change between iterations
+            // (pressure-melting enthalpies, advection coefficients) are formed
+            // again in each one: held across iterations they would take more
+            // registers than the rows and the elimination together.  The empty
+            // statements only hide these inputs' values from that hoisting.
+            T hh = h;
+            asm volatile("" : "+v"(hh));
+#pragma unroll
+            for (int m = 0; m < MB; ++m) asm volatile("" : "+v"(om[m]));
+            const T Eb = from_below<1>(El[MB - 1]);  // level k0 - 1
+            const T Ea = from_above<1>(El[0]);       // level k0 + MB
+            T Kl;  // diffusivity of the interface below level k0 (the lane below's last)
+            {
+                const int k = k0 + MB - 1;
+                const T Ep_k = e_pmp(T(k) * dsig, hh), Ep_p = e_pmp(T(min(k + 1, nz - 1)) * dsig, hh);
+                Kl = from_below<1>(diffusivity(T(0.5) * (El[MB - 1] + Ea), T(0.5) * (Ep_k + Ep_p)));
+            }
+            // rows and forward elimination: row m becomes
+            // Ac[m] x_0 + x_m + Cc[m] x_{m+1} = Dc[m] (row 0: Ac[0] multiplies
+            // the lane below's last unknown)
+            T Ac[MB], Cc[MB], Dc[MB];
+#pragma unroll
+            for (int m = 0; m < MB; ++m) {
+                const int k = k0 + m;
+                const T Ekm1 = m > 0 ? El[m - 1] : Eb;
+                const T Ekp1 = m < MB - 1 ? El[m + 1] : Ea;
+                T a = T(0), b = T(1), c = T(0), d = T(0), Ku = T(0);
+                if (k < nz - 1) {
+                    const T Ep_k = e_pmp(T(k) * dsig, hh), Ep_p = e_pmp(T(k + 1) * dsig, hh);
+                    Ku = diffusivity(T(0.5) * (El[m] + Ekp1), T(0.5) * (Ep_k + Ep_p));
+                    Row<T> w{B0 * El[m] + G[m], T(0), B0, T(0)};
+                    column_terms(A, cst, k, hh, El[m], Ekm1, Ekp1, m > 0 ? om[m - 1] : om_lo, om[m],
+                                 m < MB - 1 ? om[m + 1] : om_hi, Ku, Kl, w);
+                    a = k == 0 ? T(0) : w.a;
+                    b = w.b;
+                    c = w.c;
+                    d = -w.r;
+                    if (it == 0) r_first = nan_max(r_first, fabs((double)d));
+                } else if (k == nz - 1) {
+                    d = -(El[m] - Es);  // Dirichlet surface row
+                }
+                Kl = Ku;
+                if (m < 2) {
+                    const T rb = fast_rcp(b);
+                    Ac[m] = a * rb;
+                    Cc[m] = c * rb;
+                    Dc[m] = d * rb;
+                } else {
+                    const T rb = fast_rcp(b - a * Cc[m - 1]);
+                    Ac[m] = -a * Ac[m - 1] * rb;
+                    Cc[m] = c * rb;
+                    Dc[m] = (d - a * Dc[m - 1]) * rb;
+                }
+            }
+            // backward elimination: rows 1 .. MB-2 in terms of x_0 and x_{MB-1}
+#pragma unroll
+            for (int m = MB - 3; m >= 1; --m) {
+                Dc[m] = Dc[m] - Cc[m] * Dc[m + 1];
+                Ac[m] = Ac[m] - Cc[m] * Ac[m + 1];
+                Cc[m] = -Cc[m] * Cc[m + 1];
+            }
+            // the block's first row against the lane below's last unknown and
+            // its own last one: fa xl_{q-1} + x_0 + fc xl_q = fd; its last row:
+            // la x_0 + xl_q + lc x0_{q+1} = ld
+            const T s0 = fast_rcp(T(1) - Cc[0] * Ac[1]);
+            const T fa = Ac[0] * s0, fc = -Cc[0] * Cc[1] * s0, fd = (Dc[0] - Cc[0] * Dc[1]) * s0;
+            const T la = Ac[MB - 1], lc = Cc[MB - 1], ld = Dc[MB - 1];
+            // x_0 eliminated through the first rows: one row per lane in the
+            // last unknowns z_q = xl_q, solved by cyclic reduction over the group
+            const T fa_n = from_above<1>(fa), fc_n = from_above<1>(fc), fd_n = from_above<1>(fd);
+            T za = -la * fa, zb = T(1) - la * fc - lc * fa_n, zc = -lc * fc_n, zd = ld - la * fd - lc * fd_n;
+            pcr_stage<T, 1>(za, zb, zc, zd);
+            pcr_stage<T, 2>(za, zb, zc, zd);
+            if (P >= 8) pcr_stage<T, 4>(za, zb, zc, zd);
+            if (P == 16) pcr_stage<T, 8>(za, zb, zc, zd);
+            const T xl = zd * fast_rcp(zb);
+            const T x0 = fd - fa * from_below<1>(xl) - fc * xl;
+            Dc[0] = x0;
+            Dc[MB - 1] = xl;
+#pragma unroll
+            for (int m = 1; m < MB - 1; ++m) Dc[m] = Dc[m] - Ac[m] * x0 - Cc[m] * xl;
+#pragma unroll
+            for (int m = 0; m < MB; ++m)
+                if (k0 + m < nz) El[m] += relaxation * Dc[m];
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < MB; ++m) {
+            const int k = k0 + m;
+            if (k < nz) {
+                const long o = (long)k * ncol + cc;
+                const T E0 = A.E[o];
+                W[o] = E0 + omega_relax * (El[m] - E0);
+            }
+        }
+    }
+    if (rmax && n_newton > 0) {  // every lane of the wavefront is here
+        double mx = valid ? r_first : 0.0;
+        for (int off = 32; off > 0; off >>= 1) mx = nan_max(mx, __shfl_down(mx, off, 64));
+        if ((threadIdx.x & 63) == 0)
+            atomicMax(rmax + (((long)blockIdx.x * 4 + (threadIdx.x >> 6)) & (kEnthRmaxSlots - 1)),
+                      (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+// One column's rows on the level-major planes, streamed up the levels:
+// calls f(k, w) with level k's row (residual with forcing in w.r) for
+// k = 0 .. nz - 2 of a thick column.
+template <typename T, typename F>
+__device__ __forceinline__ void lm_stream_rows(const EnthArgs<T>& A, long ncol, long col, F&& f) {
+    const int nz = A.nz, nx = A.nx, i = (int)(col / nx), j = (int)(col % nx);
+    const T h = A.H[col], hp = A.H_prev[col];
+    const ColConst<T> cst = col_const(A, h);
+    const T invdt = cst.invdt, dsig = cst.dsig;
+    const LmHoriz<T> hc = lm_horiz(A, i, j);
+    const T B0 = T(kRhoI) * h * invdt + hc.self;
+    const T* E = A.E;
+    const T* om = A.omega;
+    T Em = E[col], Ek = E[col], Ep = E[ncol + col];
+    T om_m = om ? om[col] : T(0), om_k = om_m, om_p = om ? om[(long)min(1, nz - 1) * ncol + col] : T(0);
+    T Kl = T(0);
+    for (int k = 0; k < nz - 1; ++k) {
+        const T Ep_k = e_pmp(T(k) * dsig, h), Ep_p = e_pmp(T(k + 1) * dsig, h);
+        const T Ku = diffusivity(T(0.5) * (Ek + Ep), T(0.5) * (Ep_k + Ep_p));
+        Row<T> w{B0 * Ek + lm_constant(A, hc, ncol, col, k, i, j, hp, invdt), T(0), B0, T(0)};
+        column_terms(A, cst, k, h, Ek, Em, Ep, om_m, om_k, om_p, Ku, Kl, w);
+        f(k, Ek, w);
+        Kl = Ku;
+        Em = Ek;
+        Ek = Ep;
+        Ep = (k + 2 < nz) ? E[(long)(k + 2) * ncol + col] : T(0);
+        om_m = om_k;
+        om_k = om_p;
+        om_p = om ? om[(long)min(k + 2, nz - 1) * ncol + col] : T(0);
+    }
+}
+
+// Pointwise Jacobi layer smoother on the level-major planes:
+// E_out = E + omega_relax (-r / J_kk), E_out = E on surface rows and thin columns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_layer_lm(EnthArgs<T> A, T* __restrict__ E_out, T omega_relax) {
+    if (skipped(A.skip)) return;
+    const long ncol = (long)A.ny * A.nx, nvb = (ncol + 255) / 256;
+    const long col = xcd_block(blockIdx.x, nvb) * 256 + threadIdx.x;
+    if (col >= ncol) return;
+    const int nz = A.nz;
+    if (A.H[col] < A.h_thin) {
+        for (int k = 0; k < nz; ++k) E_out[(long)k * ncol + col] = A.E[(long)k * ncol + col];
+        return;
+    }
+    lm_stream_rows(A, ncol, col, [&](int k, T Ek, const Row<T>& w) {
+        const T out = (fabs(w.b) > T(1e-30)) ? -w.r / w.b : T(0);
+        E_out[(long)k * ncol + col] = Ek + omega_relax * out;
+    });
+    E_out[(long)(nz - 1) * ncol + col] = A.E[(long)(nz - 1) * ncol + col];
+}
+
+// Residual (with forcing) on the level-major planes: per-block max |r| into
+// partials[blockIdx.x] (grid-stride over columns) and, when r_out is given,
+// r written in the (ny, nx, nz) layout (zero on surface rows and thin columns).
+template <typename T>
+__global__ __launch_bounds__(256) void k_enth_residual_lm(EnthArgs<T> A, T* __restrict__ r_out,
+                                                          double* __restrict__ partials) {
+    __shared__ double smax[4];
+    if (skipped(A.skip)) return;
+    const int nz = A.nz;
+    const long ncol = (long)A.ny * A.nx;
+    double mx = 0.0;
+    for (long col = (long)blockIdx.x * 256 + threadIdx.x; col < ncol; col += (long)gridDim.x * 256) {
+        const long base = col * nz;
+        if (A.H[col] < A.h_thin) {
+            if (r_out)
+                for (int k = 0; k < nz; ++k) r_out[base + k] = T(0);
+            continue;
+        }
+        lm_stream_rows(A, ncol, col, [&](int k, T, const Row<T>& w) {
+            if (r_out) r_out[base + k] = w.r;
+            mx = nan_max(mx, fabs((double)w.r));
+        });
+        if (r_out) r_out[base + nz - 1] = T(0);
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = nan_max(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = nan_max(nan_max(smax[0], smax[1]), nan_max(smax[2], smax[3]));
+}
+
 // ------------------------------------------------------------------ launchers
 static inline unsigned col_blocks(int ny, int nx) {
     return (unsigned)(((long)ny * nx + kColsPerBlock - 1) / kColsPerBlock);
@@ -1352,6 +1809,62 @@ hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev
     return hipGetLastError();
 }
 
+// Level-major sweep launchers.
+template <typename T>
+hipError_t launch_enth_lm_layout(const T* src, T* dst, long ncol, int nz, bool to_lm, hipStream_t st) {
+    if (nz > 64) return hipErrorInvalidValue;
+    const unsigned nb = (unsigned)((ncol + kLmCols - 1) / kLmCols);
+    k_enth_lm_layout<T><<<nb, 256, 0, st>>>(src, dst, ncol, nz, to_lm ? 1 : 0);
+    return hipGetLastError();
+}
+bool enth_lm_supported(int nz) { return nz >= 9 && nz <= 64; }
+template <typename T>
+hipError_t launch_enth_column_lm(const EnthArgs<T>& A, T* W, T omega_relax, int n_newton, T relaxation,
+                                 unsigned long long* rmax, hipStream_t st) {
+    if (!enth_lm_supported(A.nz) || !A.u2d) return hipErrorInvalidValue;
+    const long ncol = (long)A.ny * A.nx;
+    // three or four levels per lane: 4 lanes per column up to 16 levels, 8 up
+    // to 32, 16 above (measured at 41 levels: 16 lanes 6% faster than 8)
+    const int P = A.nz <= 16 ? 4 : A.nz <= 32 ? 8 : 16, mb = (A.nz + P - 1) / P;
+    const unsigned nb = (unsigned)((ncol + 256 / P - 1) / (256 / P));
+    TimedLaunch tl(kTimeEnthColumn, ncol * A.nz, st);
+#define GLIDE_LM_COLUMN(PP, MB)                                                                                   \
+    if (P == PP && mb == MB) {                                                                                    \
+        k_enth_column_lm<T, PP, MB><<<nb, 256, 0, st>>>(A, W, omega_relax, n_newton, relaxation, rmax);           \
+        return hipGetLastError();                                                                                 \
+    }
+    GLIDE_LM_COLUMN(4, 3)
+    GLIDE_LM_COLUMN(4, 4)
+    GLIDE_LM_COLUMN(8, 3)
+    GLIDE_LM_COLUMN(8, 4)
+    GLIDE_LM_COLUMN(16, 3)
+    GLIDE_LM_COLUMN(16, 4)
+#undef GLIDE_LM_COLUMN
+    return hipErrorInvalidValue;
+}
+template <typename T>
+hipError_t launch_enth_layer_lm(const EnthArgs<T>& A, T* E_out, T omega_relax, hipStream_t st) {
+    if (!enth_lm_supported(A.nz) || !A.u2d) return hipErrorInvalidValue;
+    const long ncol = (long)A.ny * A.nx;
+    TimedLaunch tl(kTimeEnthLayer, ncol * A.nz, st);
+    k_enth_layer_lm<T><<<(unsigned)((ncol + 255) / 256), 256, 0, st>>>(A, E_out, omega_relax);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_enth_residual_lm(const EnthArgs<T>& A, T* r_out, double* partials, EnthStatus* status,
+                                   int status_mode, double rtol, double atol, hipStream_t st) {
+    if (!enth_lm_supported(A.nz) || !A.u2d) return hipErrorInvalidValue;
+    const long ncol = (long)A.ny * A.nx;
+    long g = (ncol + 255) / 256;
+    if (g > kEnthResBlocks) g = kEnthResBlocks;
+    {
+        TimedLaunch tl(kTimeEnthResidual, ncol * A.nz, st);
+        k_enth_residual_lm<T><<<(unsigned)g, 256, 0, st>>>(A, r_out, partials);
+    }
+    k_enth_reduce<<<1, 256, 0, st>>>(partials, (int)g, nullptr, status, status_mode, rtol, atol);
+    return hipGetLastError();
+}
+
 #define INST(T)                                                                                                      \
     template hipError_t launch_enth_residual<T>(const EnthArgs<T>&, T*, int, double*, double*, EnthStatus*, int,     \
                                                 double, double, hipStream_t);                                        \
@@ -1377,7 +1890,13 @@ hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev
     template hipError_t launch_enth_adj_residual<T>(const int*, const EnthArgs<T>&, bool, const T*, const T*,        \
                                                     const T*, unsigned long long*, EnthStatus*, int, double, double, \
                                                     hipStream_t);                                                    \
-    template hipError_t launch_enth_adj_grads<T>(const EnthArgs<T>&, const T*, T*, T*, T*, T*, hipStream_t);
+    template hipError_t launch_enth_adj_grads<T>(const EnthArgs<T>&, const T*, T*, T*, T*, T*, hipStream_t);      \
+    template hipError_t launch_enth_lm_layout<T>(const T*, T*, long, int, bool, hipStream_t);                        \
+    template hipError_t launch_enth_column_lm<T>(const EnthArgs<T>&, T*, T, int, T, unsigned long long*,            \
+                                                 hipStream_t);                                                       \
+    template hipError_t launch_enth_layer_lm<T>(const EnthArgs<T>&, T*, T, hipStream_t);                             \
+    template hipError_t launch_enth_residual_lm<T>(const EnthArgs<T>&, T*, double*, EnthStatus*, int, double, double, \
+                                                   hipStream_t);
 INST(double)
 #undef INST
 

@@ -130,4 +130,29 @@ template <typename T>
 hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev, T* g_phi, T* g_Q, T* g_Es,
                                  hipStream_t st);
 
+// exp and log as the enthalpy rows compute them, over n values (for tests).
+hipError_t launch_enth_elementary(const double* x, long n, double* e_out, double* l_out, hipStream_t st);
+
+// Level-major form of the fused alternating sweep (thermal step with a
+// depth-constant velocity, A.u2d set, 9 <= nz <= 64): A.E, A.E_prev, A.f_E and
+// A.omega point to (nz, ny nx) planes.
+bool enth_lm_supported(int nz);
+// (ny nx, nz) -> (nz, ny nx) when to_lm, else back.
+template <typename T>
+hipError_t launch_enth_lm_layout(const T* src, T* dst, long ncol, int nz, bool to_lm, hipStream_t st);
+// Column Newton smoother: W = E + omega_relax (E_col - E), with rmax as
+// launch_enth_column_smooth's (no residual output).
+template <typename T>
+hipError_t launch_enth_column_lm(const EnthArgs<T>& A, T* W, T omega_relax, int n_newton, T relaxation,
+                                 unsigned long long* rmax, hipStream_t st);
+// Layer smoother: E_out = E + omega_relax delta.
+template <typename T>
+hipError_t launch_enth_layer_lm(const EnthArgs<T>& A, T* E_out, T omega_relax, hipStream_t st);
+// Residual with forcing into the sweep bookkeeping (status mode as
+// launch_enth_residual's) and, when r_out is given, written in the
+// (ny, nx, nz) layout.
+template <typename T>
+hipError_t launch_enth_residual_lm(const EnthArgs<T>& A, T* r_out, double* partials, EnthStatus* status,
+                                   int status_mode, double rtol, double atol, hipStream_t st);
+
 }  // namespace glide

@@ -152,6 +152,8 @@ class EnthalpyOperators:
         self.f_E = z3()
         self.delta_E = z3()
         self._E_work = z3()
+        # level-major copies of E, E_prev, f_E and omega for the fused sweep of the thermal step
+        self._sweep_work = grid.zeros((4, nz, ny, nx))
         self._dh_dt = z2()
         self.smoother_config = ColumnSmootherConfig()
         self.term_flags = EnthalpyTermFlags()
@@ -167,7 +169,7 @@ class EnthalpyOperators:
         bufs = dict(E=st.E, E_prev=st.E_prev, f_E=self.f_E, omega=vel.omega, phi_strain=frc.phi_strain,
                     r_E=self.r_E, delta_E=self.delta_E, E_work=self._E_work, u3d=vel.u3d, v3d=vel.v3d,
                     H=g.state.H.data, H_prev=self.H_prev, E_surface=frc.E_surface, Q_geo=frc.Q_geo, Q_fh=frc.Q_fh,
-                    dh_dt=self._dh_dt)
+                    dh_dt=self._dh_dt, sweep_work=self._sweep_work)
         for name, t in bufs.items():
             setattr(C, name, ptr(t))
         return C

@@ -67,6 +67,13 @@ typedef struct glide_enth_column {
     void *E, *E_prev, *f_E, *omega, *phi_strain, *r_E, *delta_E, *E_work;
     void *u3d, *v3d;
     void *H, *H_prev, *E_surface, *Q_geo, *Q_fh, *dh_dt;
+    /* Optional (no counterpart in the reference), NULL for none: 4 nz ny nx
+     * doubles of workspace.  With it, a fused alternating sweep with a
+     * depth-constant velocity (glide_thermal_step) and 9 <= nz <= 64 runs on
+     * level-major copies of E, E_prev, f_E and omega (E_work then holds the
+     * column pass's output in that layout); results equal the column-major
+     * sweep's to rounding. */
+    void *sweep_work;
 } glide_enth_column;
 
 /* ColumnSmootherConfig (glide/enthalpy.py:183-198) minus the Python hook. */
@@ -201,6 +208,12 @@ typedef struct glide_enth_adjoint {
 int glide_enth_adjoint_solve(const glide_enth_column* C, const glide_enth_adjoint* X, double dt, const void* dL_dE,
                              const void* dL_dB, double n_glen, double B_scale, int n_iter,
                              const glide_enth_sweep_config* cfg, glide_enth_report* report, void* stream);
+
+/* No counterpart in the reference: the exponential and logarithm the
+ * enthalpy rows use (written for the rows' argument ranges), over n doubles:
+ * exp_out = e^x, log_out = log x for x > 0 (NaN otherwise); either output may
+ * be NULL.  For tests of their accuracy against the general routines. */
+int glide_enth_elementary(const void* x, int64_t n, void* exp_out, void* log_out, void* stream);
 
 /* ThermalModel.step after pre_momentum: velocities u2d, v2d in m/a and H the
  * new thickness; fills u3d, v3d (m/s), dh_dt, omega, Q_fh (when

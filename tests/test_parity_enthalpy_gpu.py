@@ -416,3 +416,36 @@ def test_thermal_step_autograd_matches_finite_differences():
         ad = float(leaf.grad[idx])
         print(f"  {name}{idx}: adjoint {ad:.10e}  fd {fd:.10e}", flush=True)
         assert abs(fd - ad) <= 1e-5 * max(abs(fd), abs(ad)), name
+
+
+# ------------------------------------------------------ elementary functions
+def test_row_elementary_functions_within_a_few_ulp():
+    """The exponential and logarithm the enthalpy rows use (written for the
+    rows' argument ranges) against numpy's: exp over the rows' clamped
+    arguments [-20, 20] and over [-700, 700], log over 1 + e^a of the
+    drainage softplus (a in [-20, 10]), over [1e-300, 1e300] and at the
+    reduction's breakpoints, within 8 ulp (the largest is printed)."""
+    from glide_amd import _lib
+    rng = np.random.default_rng(5)
+    s = np.sqrt(0.5)
+    xe = np.concatenate([rng.uniform(-20.0, 20.0, 200000), rng.uniform(-700.0, 700.0, 50000),
+                         [0.0, -20.0, 20.0, 1e-300, -1e-18, np.log(2.0), -np.log(2.0) / 2]])
+    xl = np.concatenate([1.0 + np.exp(rng.uniform(-20.0, 10.0, 200000)), 10.0 ** rng.uniform(-300.0, 300.0, 50000),
+                         [1.0, 2.0, 0.5, s, np.nextafter(s, 0.0), np.nextafter(s, 1.0), 1.0 + 2.0 ** -52]])
+    ulp = 2.0 ** -52
+    for x, which in ((xe, "exp"), (xl, "log")):
+        xt = torch.as_tensor(x, device="cuda")
+        out = torch.empty_like(xt)
+        args = (out, None) if which == "exp" else (None, out)
+        _lib.call("glide_enth_elementary", _lib.ptr(xt), int(x.size), _lib.ptr(args[0]), _lib.ptr(args[1]),
+                  _lib.stream_handle())
+        torch.cuda.synchronize()
+        want = np.exp(x) if which == "exp" else np.log(x)
+        got = N(out)
+        nz = want != 0.0
+        err = np.abs(got - want)
+        rel = np.max(err[nz] / (np.abs(want[nz]) * ulp))
+        print(f"  {which}: largest error {rel:.2f} ulp over {x.size} values", flush=True)
+        assert np.all(np.isfinite(got))
+        assert rel <= 8.0
+        assert np.all(err[~nz] == 0.0)
