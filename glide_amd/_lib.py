@@ -66,7 +66,7 @@ class FasConfigC(ctypes.Structure):
 class MixedConfigC(ctypes.Structure):
     _fields_ = [("max_outer", ctypes.c_int32), ("inner_vcycles", ctypes.c_int32),
                 ("relative_tolerance", ctypes.c_double), ("absolute_tolerance", ctypes.c_double),
-                ("fp64_coarsest", ctypes.c_int32)]
+                ("fp64_coarsest", ctypes.c_int32), ("krylov_dim", ctypes.c_int32), ("krylov_work", ctypes.c_void_p)]
 
 
 class SolveReport(ctypes.Structure):
@@ -154,6 +154,7 @@ SIGNATURES = {
     "glide_fas_adjoint_vcycle": (_I, [_LP, _I, _I, _I, _D, ctypes.POINTER(FasConfigC),
                                       ctypes.POINTER(VankaConfigC), _P]),
     "glide_gradients": (_I, [_LP, _D, _I, _P]),
+    "glide_krylov_work_doubles": (ctypes.c_int64, [_I, _I, _I]),
     "glide_graphs_enable": (_I, [_I]),
     "glide_graphs_capacity": (_I, [_I]),
     "glide_adjoint_fused_enable": (_I, [_I]),

@@ -212,6 +212,83 @@ hipError_t launch_mixed_batch(const MixedBatch& b, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ Krylov helpers
+// The flexible GMRES outer iteration of the mixed adjoint solve works on flat
+// FP64 vectors (the u, v and H multipliers of a level one after another);
+// these are its inner products and updates.  The sums are formed in a fixed
+// order (fixed grid, fixed block reduction), so repeated solves are bitwise
+// identical.
+
+// partial[q * gridDim.x + b]: block b's part of <w, V_q> for q < nq (V_q at
+// V + q * ld), and of <w, w> at q = nq.
+__global__ __launch_bounds__(256) void k_krylov_dots(const double* __restrict__ w, const double* __restrict__ V,
+                                                     long ld, long n, int nq, double* __restrict__ partial) {
+    __shared__ double red[4][kKrylovMaxDots + 1];
+    double acc[kKrylovMaxDots + 1];
+#pragma unroll
+    for (int q = 0; q <= kKrylovMaxDots; ++q) acc[q] = 0.0;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        const double wk = w[k];
+#pragma unroll
+        for (int q = 0; q < kKrylovMaxDots; ++q)
+            if (q < nq) acc[q] = fma(wk, V[q * ld + k], acc[q]);
+        acc[kKrylovMaxDots] = fma(wk, wk, acc[kKrylovMaxDots]);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q <= kKrylovMaxDots; ++q) {
+        if (q < nq || q == kKrylovMaxDots) {
+            double v = acc[q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) red[wv][q] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x <= nq) {
+        const int q = threadIdx.x == nq ? kKrylovMaxDots : threadIdx.x;
+        partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+    }
+}
+
+// out[q] = sum over the nb partials of q, q <= nq (one block per q).
+__global__ __launch_bounds__(256) void k_krylov_reduce(const double* __restrict__ partial, int nb,
+                                                       double* __restrict__ out) {
+    __shared__ double red[4];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += partial[(long)blockIdx.x * nb + b];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// out[k] += sign * sum_{q < nq} c[q] V[q * ld + k]  (c on the device).
+__global__ __launch_bounds__(256) void k_krylov_axpy(double* __restrict__ out, const double* __restrict__ V, long ld,
+                                                     long n, int nq, const double* __restrict__ c, double sign) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < nq; ++q) s = fma(c[q], V[q * ld + k], s);
+        out[k] = fma(sign, s, out[k]);
+    }
+}
+
+hipError_t launch_krylov_dots(const double* w, const double* V, long ld, long n, int nq, double* partial,
+                              double* out, hipStream_t st) {
+    if (nq < 0 || nq > kKrylovMaxDots) return hipErrorInvalidValue;
+    k_krylov_dots<<<kKrylovBlocks, 256, 0, st>>>(w, V, ld, n, nq, partial);
+    k_krylov_reduce<<<nq + 1, 256, 0, st>>>(partial, kKrylovBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_krylov_axpy(double* out, const double* V, long ld, long n, int nq, const double* c, double sign,
+                              hipStream_t st) {
+    if (nq <= 0 || n <= 0) return hipSuccess;
+    long b = (n + 255) / 256;
+    if (b > 2048) b = 2048;
+    k_krylov_axpy<<<(unsigned)b, 256, 0, st>>>(out, V, ld, n, nq, c, sign);
+    return hipGetLastError();
+}
+
 #define GLIDE_MC_INSTANTIATE(T)                                                                                     \
     template hipError_t launch_pin_rows<T>(T*, T*, T*, const T*, int, int, hipStream_t);                           \
     template hipError_t launch_cell_eta<T>(const SsaArgs<T>&, T*, hipStream_t);                                    \

@@ -11,6 +11,7 @@
 // first call on a device that needs it) and the graph cache.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -50,10 +51,15 @@ int hip_fail(hipError_t e, const char* where) {
         if (_r != GLIDE_OK) return _r; \
     } while (0)
 
+// Pinned host doubles per device: the mirror of the 3 * 66 norm slots, then
+// kKrylovHostSlots values of the flexible GMRES iteration.
+constexpr int kKrylovHostSlots = 2 * kKrylovMaxDots + 2;
+constexpr int kHostSlots = 3 * 66 + kKrylovHostSlots;
+
 struct DeviceScratch {
     double* partials = nullptr;  // 3 * kSumsqBlocks
     double* norms = nullptr;     // 3 * 66 slots
-    double* host = nullptr;      // pinned mirror of `norms`
+    double* host = nullptr;      // pinned: mirror of `norms`, then the Krylov slots
     unsigned* latch = nullptr;   // failure latch of the persistent launches (bit 0: a barrier timed out)
     unsigned* latch_host = nullptr;  // pinned: [0] mirror of `latch`, [1] zero (the value that clears it)
 };
@@ -70,7 +76,7 @@ int scratch_for_current_device(DeviceScratch** out) {
     if (!s.partials) {
         HIP_TRY(hipMalloc(&s.partials, sizeof(double) * 3 * kSumsqBlocks), "hipMalloc(reduction scratch)");
         HIP_TRY(hipMalloc(&s.norms, sizeof(double) * 3 * 66), "hipMalloc(norm slots)");
-        HIP_TRY(hipHostMalloc(&s.host, sizeof(double) * 3 * 66, hipHostMallocDefault), "hipHostMalloc(norms)");
+        HIP_TRY(hipHostMalloc(&s.host, sizeof(double) * kHostSlots, hipHostMallocDefault), "hipHostMalloc(norms)");
         HIP_TRY(hipMalloc(&s.latch, sizeof(unsigned)), "hipMalloc(latch)");
         HIP_TRY(hipMemset(s.latch, 0, sizeof(unsigned)), "hipMemset(latch)");
         HIP_TRY(hipHostMalloc(&s.latch_host, 2 * sizeof(unsigned), hipHostMallocDefault), "hipHostMalloc(latch)");
@@ -1100,6 +1106,10 @@ int check_mixed(const glide_level* levels64, const glide_level* levels32, int n_
     if (!levels32 || !mc) return fail(GLIDE_ERR_INVALID_ARG, "levels32 and the mixed config must be non-NULL");
     if (mc->max_outer < 0 || mc->max_outer > 64) return fail(GLIDE_ERR_INVALID_ARG, "max_outer must be in [0, 64]");
     if (mc->inner_vcycles < 1) return fail(GLIDE_ERR_INVALID_ARG, "inner_vcycles must be >= 1");
+    if (adjoint && (mc->krylov_dim < 0 || mc->krylov_dim > 16))
+        return fail(GLIDE_ERR_INVALID_ARG, "krylov_dim must be in [0, 16]");
+    if (adjoint && mc->krylov_dim > 0 && !mc->krylov_work)
+        return fail(GLIDE_ERR_INVALID_ARG, "krylov_dim > 0 needs krylov_work");
     for (int l = start; l < n_levels; ++l) {
         const glide_level& A = levels32[l];
         if (A.ny != levels64[l].ny || A.nx != levels64[l].nx)
@@ -1223,6 +1233,164 @@ int mixed_outer_loop(const glide_mixed_config& mc, hipStream_t st, Key&& key, bo
         stats[2] = n2;
     }
     return GLIDE_OK;
+}
+
+// Flexible GMRES(m) outer iteration of the mixed adjoint solve on flat FP64
+// vectors of the start level's multipliers (u, v, H one after another): the
+// basis V_0..V_m and the preconditioned vectors Z_0..Z_{m-1}, then the
+// reduction partials, two sets of inner products and the update coefficients.
+struct KrylovLayout {
+    long n, nu, nv, nc;
+    double *V, *Z, *partial, *dots, *coef;
+};
+long krylov_vector_len(int ny, int nx) { return (long)ny * (nx + 1) + (long)(ny + 1) * nx + (long)ny * nx; }
+int64_t krylov_doubles(int ny, int nx, int m) {
+    const long n = krylov_vector_len(ny, nx);
+    return (2L * m + 1) * n + (long)(kKrylovMaxDots + 1) * kKrylovBlocks + 2L * (kKrylovMaxDots + 1) + m;
+}
+KrylovLayout krylov_layout(void* work, int ny, int nx, int m) {
+    KrylovLayout K;
+    K.nu = (long)ny * (nx + 1);
+    K.nv = (long)(ny + 1) * nx;
+    K.nc = (long)ny * nx;
+    K.n = K.nu + K.nv + K.nc;
+    K.V = static_cast<double*>(work);
+    K.Z = K.V + (long)(m + 1) * K.n;
+    K.partial = K.Z + (long)m * K.n;
+    K.dots = K.partial + (long)(kKrylovMaxDots + 1) * kKrylovBlocks;
+    K.coef = K.dots + 2 * (kKrylovMaxDots + 1);
+    return K;
+}
+
+// Returned by krylov_outer_loop when a restart cycle does not reduce the
+// true residual: the caller continues with the defect-correction loop.
+constexpr int kKrylovStalled = 1;
+
+// resid(stream, slot) forms ar = J^T lambda - g in ar_{u,v,H} and its squared
+// norms; precond(stream, v, z) z = M v (the FP32 correction V-cycles);
+// apply(stream, z, w) w = J^T z.  The residual r = g - J^T lambda = -ar.
+template <typename Resid, typename Precond, typename Apply>
+int krylov_outer_loop(const glide_mixed_config& mc, hipStream_t st, const KrylovLayout& K, int m,
+                      const glide_level& D, glide_solve_report* rep, int* stats, int* iterations, Resid&& resid,
+                      Precond&& precond, Apply&& apply) {
+    DeviceScratch* s = nullptr;
+    GLIDE_TRY(scratch_for_current_device(&s));
+    double* hk = s->host + 3 * 66;
+    auto read_norm = [&](double* n3) -> int {
+        HIP_TRY(hipMemcpyAsync(s->host, s->norms, sizeof(double) * 3, hipMemcpyDeviceToHost, st), "norms D2H");
+        GLIDE_TRY(latch_enqueue(s, st));
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        GLIDE_TRY(latch_check(s));
+        n3[0] = s->host[0]; n3[1] = s->host[1]; n3[2] = s->host[2];
+        return GLIDE_OK;
+    };
+    R* lam[3] = {P(D.lambda_u), P(D.lambda_v), P(D.lambda_H)};
+    const R* ar[3] = {CP(D.ar_u), CP(D.ar_v), CP(D.ar_H)};
+    const long len[3] = {K.nu, K.nv, K.nc}, off[3] = {0, K.nu, K.nu + K.nv};
+    double n3[3];
+    GLIDE_TRY(resid(st, s->norms));
+    GLIDE_TRY(read_norm(n3));
+    const double r0 = std::sqrt(n3[0] + n3[1] + n3[2]);
+    if (!std::isfinite(r0)) return fail(GLIDE_ERR_NONFINITE, "non-finite residual norm at the start of the mixed solve");
+    auto converged = [&](double r) {
+        return !(r > mc.absolute_tolerance && (r0 <= 0.0 || r / r0 > mc.relative_tolerance));
+    };
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    auto h = [&](int i, int j) -> double& { return H[(size_t)i * m + j]; };
+    double r = r0;
+    int it = 0, rc = GLIDE_OK;
+    while (it < mc.max_outer && !converged(r)) {
+        EwBatch<R> b0;
+        for (int q = 0; q < 3; ++q) b0.axpby(K.V + off[q], -1.0 / r, ar[q], 0.0, ar[q], len[q]);
+        HIP_TRY(launch_ew_batch<R>(b0, st), "Krylov start vector");
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = r;
+        int j = 0;
+        while (j < m && it < mc.max_outer) {
+            double* Vj = K.V + (long)j * K.n;
+            double* Zj = K.Z + (long)j * K.n;
+            double* W = Vj + K.n;
+            GLIDE_TRY(precond(st, Vj, Zj));
+            GLIDE_TRY(apply(st, Zj, W));
+            // classical Gram-Schmidt, twice
+            double* d1 = K.dots;
+            double* d2 = K.dots + (kKrylovMaxDots + 1);
+            HIP_TRY(launch_krylov_dots(W, K.V, K.n, K.n, j + 1, K.partial, d1, st), "Krylov inner products");
+            HIP_TRY(launch_krylov_axpy(W, K.V, K.n, K.n, j + 1, d1, -1.0, st), "Krylov orthogonalisation");
+            HIP_TRY(launch_krylov_dots(W, K.V, K.n, K.n, j + 1, K.partial, d2, st), "Krylov inner products");
+            HIP_TRY(launch_krylov_axpy(W, K.V, K.n, K.n, j + 1, d2, -1.0, st), "Krylov orthogonalisation");
+            HIP_TRY(hipMemcpyAsync(hk, d1, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, st), "Krylov D2H");
+            HIP_TRY(hipMemcpyAsync(hk + (j + 1), d2, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st),
+                    "Krylov D2H");
+            GLIDE_TRY(latch_enqueue(s, st));
+            HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+            GLIDE_TRY(latch_check(s));
+            // |W|^2 after the second pass from the first pass's norm
+            double s2 = 0.0;
+            for (int i = 0; i <= j; ++i) {
+                h(i, j) = hk[i] + hk[j + 1 + i];
+                s2 += hk[j + 1 + i] * hk[j + 1 + i];
+            }
+            const double hn = std::sqrt(std::max(hk[2 * j + 2] - s2, 0.0));
+            if (!std::isfinite(hn)) return fail(GLIDE_ERR_NONFINITE, "non-finite Krylov vector in the mixed solve");
+            if (hn > 0.0) {
+                EwBatch<R> bs;
+                bs.axpby(W, 1.0 / hn, W, 0.0, W, K.n);
+                HIP_TRY(launch_ew_batch<R>(bs, st), "Krylov normalisation");
+            }
+            // Givens rotations of the Hessenberg column
+            for (int i = 0; i < j; ++i) {
+                const double a = h(i, j), c = h(i + 1, j);
+                h(i, j) = cs[i] * a + sn[i] * c;
+                h(i + 1, j) = -sn[i] * a + cs[i] * c;
+            }
+            const double rho = std::hypot(h(j, j), hn);
+            cs[j] = rho > 0.0 ? h(j, j) / rho : 1.0;
+            sn[j] = rho > 0.0 ? hn / rho : 0.0;
+            h(j, j) = rho;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            const double est = std::fabs(g[j + 1]);
+            if (rep && it < 64) rep->norms[it] = est;
+            ++it;
+            ++j;
+            if (!std::isfinite(est)) return fail(GLIDE_ERR_NONFINITE, "non-finite residual estimate in the mixed solve");
+            if (converged(est) || hn == 0.0) break;
+        }
+        // x += Z y with H y = g (upper triangular, j x j)
+        for (int i = j - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int k = i + 1; k < j; ++k) t -= h(i, k) * y[k];
+            y[i] = h(i, i) != 0.0 ? t / h(i, i) : 0.0;
+        }
+        for (int i = 0; i < j; ++i) hk[i] = y[i];
+        HIP_TRY(hipMemcpyAsync(K.coef, hk, sizeof(double) * j, hipMemcpyHostToDevice, st), "Krylov H2D");
+        for (int q = 0; q < 3; ++q)
+            HIP_TRY(launch_krylov_axpy(lam[q], K.Z + off[q], K.n, len[q], j, K.coef, 1.0, st), "Krylov update");
+        GLIDE_TRY(resid(st, s->norms));
+        GLIDE_TRY(read_norm(n3));
+        const double rn = std::sqrt(n3[0] + n3[1] + n3[2]);
+        if (!std::isfinite(rn)) return fail(GLIDE_ERR_NONFINITE, "non-finite residual norm in the mixed solve");
+        const bool stalled = !(rn < r);
+        r = rn;
+        if (stalled) {
+            rc = kKrylovStalled;
+            break;
+        }
+    }
+    if (rep) {
+        rep->vcycles = it;
+        rep->initial_norm = r0;
+        rep->final_norm = r;
+        rep->norms3[0] = std::sqrt(n3[0]); rep->norms3[1] = std::sqrt(n3[1]); rep->norms3[2] = std::sqrt(n3[2]);
+    }
+    if (stats) {
+        stats[0] = 0;
+        stats[1] = it;
+        stats[2] = 0;
+    }
+    *iterations = it;
+    return rc;
 }
 
 std::string mixed_key(char tag, const glide_level* l64, const glide_level* l32, int n_levels, int start, double dt,
@@ -1833,7 +2001,62 @@ int glide_fas_adjoint_solve_mixed(const glide_level* levels64, const glide_level
         return vjp(D, dt, GLIDE_USE_FORCING, CP(D.lambda_u), CP(D.lambda_v), CP(D.lambda_H), P(D.ar_u), P(D.ar_v),
                    P(D.ar_H), cs);
     };
+    const bool fixed = !(mc->relative_tolerance > 0.0) && !(mc->absolute_tolerance > 0.0);
+    if (mc->krylov_dim > 0 && (report || !fixed)) {
+        const KrylovLayout K = krylov_layout(mc->krylov_work, D.ny, D.nx, mc->krylov_dim);
+        // z = M v: the FP32 correction V-cycles for J^T d = v from d = 0 (one
+        // cached graph), the pinned rows zero
+        auto precond = [&](hipStream_t cs, const double* v, double* z) -> int {
+            MixedBatch b;
+            b.add(kMixTo32, A.af_u, v, nullptr, (long)nU(D));
+            b.add(kMixTo32, A.af_v, v + K.nu, nullptr, (long)nV(D));
+            b.add(kMixTo32, A.af_H, v + K.nu + K.nv, nullptr, (long)nC(D));
+            GLIDE_TRY(run_mixed(b, cs));
+            auto body = [&](hipStream_t gs) -> int {
+                X.st = gs;
+                EwBatch<float> zf;
+                zf.fill(PT<float>(A.lambda_u), 0.0f, (long)nU(A));
+                zf.fill(PT<float>(A.lambda_v), 0.0f, (long)nV(A));
+                zf.fill(PT<float>(A.lambda_H), 0.0f, (long)nC(A));
+                HIP_TRY(launch_ew_batch<float>(zf, gs), "zero correction");
+                for (int q = 0; q < mc->inner_vcycles; ++q) GLIDE_TRY(mixed_adj_vcycle(X, start_level, true));
+                HIP_TRY(launch_pin_rows<float>(PT<float>(A.lambda_u), PT<float>(A.lambda_v), PT<float>(A.lambda_H),
+                                               CPT<float>(A.mask), A.ny, A.nx, gs),
+                        "pin rows");
+                X.st = st;
+                return GLIDE_OK;
+            };
+            if (g_graphs_on && !timing_on())
+                GLIDE_TRY(replay(mixed_key('p', levels64, levels32, n_levels, start_level, dt, *cfg, vc, *mc), cs,
+                                 body));
+            else
+                GLIDE_TRY(body(cs));
+            MixedBatch c;
+            c.add(kMixTo64, z, A.lambda_u, nullptr, (long)nU(D));
+            c.add(kMixTo64, z + K.nu, A.lambda_v, nullptr, (long)nV(D));
+            c.add(kMixTo64, z + K.nu + K.nv, A.lambda_H, nullptr, (long)nC(D));
+            return run_mixed(c, cs);
+        };
+        auto apply = [&](hipStream_t cs, const double* zv, double* w) -> int {
+            return vjp(D, dt, 0, zv, zv + K.nu, zv + K.nu + K.nv, w, w + K.nu, w + K.nu + K.nv, cs);
+        };
+        int its = 0;
+        const int rc = krylov_outer_loop(*mc, st, K, mc->krylov_dim, D, report, stats, &its, resid, precond, apply);
+        if (rc != kKrylovStalled) return rc;
+        // a restart cycle that did not reduce the residual: the remaining
+        // iterations as defect correction (with its undo and fallback)
+        glide_mixed_config rest = *mc;
+        rest.max_outer = std::max(0, mc->max_outer - its);
+        const int rc2 = mixed_outer_loop(rest, st, key, false, report, stats, resid, step, undo, fallback, prep);
+        if (stats) stats[1] += its;
+        return rc2;
+    }
     return mixed_outer_loop(*mc, st, key, false, report, stats, resid, step, undo, fallback, prep);
+}
+
+int64_t glide_krylov_work_doubles(int ny, int nx, int m) {
+    if (ny < 1 || nx < 1 || m < 1 || m > 16) return -1;
+    return krylov_doubles(ny, nx, m);
 }
 
 int glide_adjoint_fused_enable(int on) {

@@ -253,6 +253,18 @@ struct MixedBatch {
 };
 hipError_t launch_mixed_batch(const MixedBatch& b, hipStream_t st);
 
+// Inner products and updates of the flexible GMRES outer iteration (flat FP64
+// vectors): out[q] = <w, V_q> for q < nq (V_q at V + q * ld) and out[nq] =
+// <w, w>, through kKrylovBlocks partials per value (partial holds
+// (nq + 1) * kKrylovBlocks doubles); out += sign * sum_q c[q] V_q with c on
+// the device.
+constexpr int kKrylovMaxDots = 18;
+constexpr int kKrylovBlocks = 512;
+hipError_t launch_krylov_dots(const double* w, const double* V, long ld, long n, int nq, double* partial,
+                              double* out, hipStream_t st);
+hipError_t launch_krylov_axpy(double* out, const double* V, long ld, long n, int nq, const double* c, double sign,
+                              hipStream_t st);
+
 // transfers and element-wise helpers (transfer_kernels.hip)
 template <typename T>
 hipError_t launch_restrict_vfacet(const T* f, T* c, int nyc, int nxc, hipStream_t st);

@@ -95,7 +95,10 @@ class MixedConfig:
     patch solution; fp64_coarsest runs the coarsest level's passes in FP64.
     asynchronous with both tolerances <= 0 runs exactly max_outer FP32
     iterations without a host wait and without a report (no norms, no
-    fallback); the solve then returns None.
+    fallback); the solve then returns None.  krylov_dim (adjoint solves only)
+    > 0 runs the outer iteration as flexible GMRES restarted every krylov_dim
+    iterations, the FP32 correction V-cycles preconditioning it; 0 keeps the
+    defect-correction loop.
     compat_vprolong is off: the mixed form uses the v-facet prolongation."""
     max_outer: int = 30
     inner_vcycles: int = 1
@@ -111,14 +114,17 @@ class MixedConfig:
     adjoint_omega: float = 0.85
     compat_vprolong: bool = False
     asynchronous: bool = False
+    krylov_dim: int = 12
 
     def to_c(self) -> _lib.MixedConfigC:
         if not 0 <= int(self.max_outer) <= 64:
             raise ValueError(f"max_outer must be in [0, 64] (got {self.max_outer})")
+        if not 0 <= int(self.krylov_dim) <= 16:
+            raise ValueError(f"krylov_dim must be in [0, 16] (got {self.krylov_dim})")
         return _lib.MixedConfigC(max_outer=int(self.max_outer), inner_vcycles=int(self.inner_vcycles),
                                  relative_tolerance=float(self.relative_tolerance),
                                  absolute_tolerance=float(self.absolute_tolerance),
-                                 fp64_coarsest=int(bool(self.fp64_coarsest)))
+                                 fp64_coarsest=int(bool(self.fp64_coarsest)), krylov_dim=0, krylov_work=None)
 
     def fas_c(self, freeze_coarse_calving=True) -> _lib.FasConfigC:
         return _lib.FasConfigC(freeze_coarse_calving=int(bool(freeze_coarse_calving)), freeze_coarse_phi=1,
@@ -141,6 +147,18 @@ class MixedHierarchy:
     def __init__(self, levels):
         self.levels = levels  # the FP64 Grid objects, finest first
         self.l32 = [None] * len(levels)
+        self._krylov = {}
+
+    def krylov_work(self, start_level: int, m: int) -> torch.Tensor:
+        """The flexible GMRES workspace of an adjoint solve from start_level."""
+        key = (start_level, m)
+        if key not in self._krylov:
+            g = self.levels[start_level]
+            n = int(_lib.load().glide_krylov_work_doubles(g.ny, g.nx, m))
+            if n <= 0:
+                raise ValueError(f"invalid Krylov workspace request ({g.ny} x {g.nx}, m = {m})")
+            self._krylov[key] = torch.zeros(n, dtype=torch.float64, device=g.device)
+        return self._krylov[key]
 
     def structs(self, start_level: int, adjoint: bool):
         arr = (_lib.Level * len(self.levels))()
@@ -166,6 +184,9 @@ def _run(name, levels64, mh, start_level, dt, cfg, adjoint, stream):
     stats = (ctypes.c_int * 3)()
     fas = cfg.fas_c()
     mc = cfg.to_c()
+    if adjoint and int(cfg.krylov_dim) > 0 and (rep is not None or not fixed):
+        mc.krylov_dim = int(cfg.krylov_dim)
+        mc.krylov_work = ptr(mh.krylov_work(start_level, int(cfg.krylov_dim)))
     _lib.call(name, levels64, l32, n, int(start_level), float(dt), ctypes.byref(fas), vcs, ctypes.byref(mc),
               ctypes.byref(rep) if rep is not None else None, stats, stream)
     return rep, {"fallbacks": stats[0], "fp32": stats[1], "fp64": stats[2]}

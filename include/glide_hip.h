@@ -223,13 +223,28 @@ typedef struct glide_mixed_config {
     int32_t inner_vcycles;
     double relative_tolerance, absolute_tolerance;
     int32_t fp64_coarsest;
+    /* Adjoint solve only: krylov_dim m > 0 runs the outer iteration as
+     * flexible GMRES(m) (restarted every m iterations) with the FP32
+     * correction V-cycles as the preconditioner, in krylov_work (device,
+     * glide_krylov_work_doubles(ny, nx, m) doubles for the start level);
+     * 0 keeps the defect-correction loop. */
+    int32_t krylov_dim;
+    void* krylov_work;
 } glide_mixed_config;
+/* Doubles of glide_mixed_config.krylov_work for an ny x nx start level and
+ * restart length m (1 <= m <= 16); -1 for invalid arguments. */
+int64_t glide_krylov_work_doubles(int ny, int nx, int m);
 int glide_fascd_solve_mixed(const glide_level* levels64, const glide_level* levels32, int n_levels, int start_level,
                             double dt, const glide_fas_config* cfg, const glide_vanka_config* vc,
                             const glide_mixed_config* mc, glide_solve_report* report, int* stats, void* stream);
 
 /* The mixed-precision form of glide_fas_adjoint_solve (no counterpart in the
- * reference; linear drag).  Per outer iteration the FP64 adjoint residual
+ * reference; linear drag).  With mc->krylov_dim = m > 0 (and a report or a
+ * tolerance) the outer iteration is flexible GMRES(m) on J^T lambda = g,
+ * right-preconditioned by the FP32 correction V-cycles below, with the FP64
+ * transpose operator, classical Gram-Schmidt with one reorthogonalisation and
+ * the true residual formed at every restart; the report's norms are the
+ * iteration's residual estimates.  Otherwise, per outer iteration the FP64 adjoint residual
  * rho = J^T lambda - g of the start level, then inner_vcycles FP32 FAS adjoint
  * V-cycles for the correction J^T d = -rho from d = 0 with the red-black
  * adjoint passes (vc[l].multicolour = 1; each colour's patch solves move
