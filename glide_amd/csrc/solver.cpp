@@ -52,8 +52,9 @@ int hip_fail(hipError_t e, const char* where) {
     } while (0)
 
 // Pinned host doubles per device: the mirror of the 3 * 66 norm slots, then
-// kKrylovHostSlots values of the flexible GMRES iteration.
-constexpr int kKrylovHostSlots = 2 * kKrylovMaxDots + 2;
+// kKrylovHostSlots values of the flexible GMRES and Anderson iterations (up to
+// seven rows of inner products).
+constexpr int kKrylovHostSlots = 8 * (kKrylovMaxDots + 1);
 constexpr int kHostSlots = 3 * 66 + kKrylovHostSlots;
 
 struct DeviceScratch {
@@ -1106,10 +1107,8 @@ int check_mixed(const glide_level* levels64, const glide_level* levels32, int n_
     if (!levels32 || !mc) return fail(GLIDE_ERR_INVALID_ARG, "levels32 and the mixed config must be non-NULL");
     if (mc->max_outer < 0 || mc->max_outer > 64) return fail(GLIDE_ERR_INVALID_ARG, "max_outer must be in [0, 64]");
     if (mc->inner_vcycles < 1) return fail(GLIDE_ERR_INVALID_ARG, "inner_vcycles must be >= 1");
-    if (adjoint && (mc->krylov_dim < 0 || mc->krylov_dim > 16))
-        return fail(GLIDE_ERR_INVALID_ARG, "krylov_dim must be in [0, 16]");
-    if (adjoint && mc->krylov_dim > 0 && !mc->krylov_work)
-        return fail(GLIDE_ERR_INVALID_ARG, "krylov_dim > 0 needs krylov_work");
+    if (mc->krylov_dim < 0 || mc->krylov_dim > 16) return fail(GLIDE_ERR_INVALID_ARG, "krylov_dim must be in [0, 16]");
+    if (mc->krylov_dim > 0 && !mc->krylov_work) return fail(GLIDE_ERR_INVALID_ARG, "krylov_dim > 0 needs krylov_work");
     for (int l = start; l < n_levels; ++l) {
         const glide_level& A = levels32[l];
         if (A.ny != levels64[l].ny || A.nx != levels64[l].nx)
@@ -1142,10 +1141,14 @@ std::vector<glide_vanka_config> reference_vanka_configs(int n) {
 // iterations, phase-2 iterations.
 constexpr double kPhaseSwitch = 0.6;
 
-template <typename Resid, typename Step, typename Undo, typename Fallback, typename Key, typename Prep>
+// accel(stream, phase, norms3) runs after every accepted iteration of the
+// given phase (0 after a fallback): it may move the state and then replaces
+// norms3 by the squared norms of its new residual.
+template <typename Resid, typename Step, typename Undo, typename Fallback, typename Key, typename Prep,
+          typename Accel>
 int mixed_outer_loop(const glide_mixed_config& mc, hipStream_t st, Key&& key, bool two_phase,
                      glide_solve_report* rep, int* stats, Resid&& resid, Step&& step, Undo&& undo,
-                     Fallback&& fallback, Prep&& prep) {
+                     Fallback&& fallback, Prep&& prep, Accel&& accel) {
     if (!rep && !(mc.relative_tolerance > 0.0) && !(mc.absolute_tolerance > 0.0)) {
         // fixed count without a report: max_outer FP32 iterations enqueued
         // without a host wait (prep forms what an iteration reads from the
@@ -1211,9 +1214,11 @@ int mixed_outer_loop(const glide_mixed_config& mc, hipStream_t st, Key&& key, bo
             GLIDE_TRY(fallback(st));
             GLIDE_TRY(resid(st, s->norms));
             GLIDE_TRY(read_norm(m3));
+            GLIDE_TRY(accel(st, 0, m3));
             ++nfall;
-        } else if (phase < last_phase && rn > kPhaseSwitch * r) {
-            phase = last_phase;
+        } else {
+            GLIDE_TRY(accel(st, phase, m3));
+            if (phase < last_phase && rn > kPhaseSwitch * r) phase = last_phase;
         }
         std::memcpy(n3, m3, sizeof(n3));
         r = std::sqrt(n3[0] + n3[1] + n3[2]);
@@ -1246,7 +1251,8 @@ struct KrylovLayout {
 long krylov_vector_len(int ny, int nx) { return (long)ny * (nx + 1) + (long)(ny + 1) * nx + (long)ny * nx; }
 int64_t krylov_doubles(int ny, int nx, int m) {
     const long n = krylov_vector_len(ny, nx);
-    return (2L * m + 1) * n + (long)(kKrylovMaxDots + 1) * kKrylovBlocks + 2L * (kKrylovMaxDots + 1) + m;
+    const long vectors = std::max(2L * m + 1, 2L * std::min(m, 6) + 4);
+    return vectors * n + (long)(kKrylovMaxDots + 1) * kKrylovBlocks + 8L * (kKrylovMaxDots + 1) + m;
 }
 KrylovLayout krylov_layout(void* work, int ny, int nx, int m) {
     KrylovLayout K;
@@ -1260,6 +1266,31 @@ KrylovLayout krylov_layout(void* work, int ny, int nx, int m) {
     K.dots = K.partial + (long)(kKrylovMaxDots + 1) * kKrylovBlocks;
     K.coef = K.dots + 2 * (kKrylovMaxDots + 1);
     return K;
+}
+
+// Anderson mixing of the mixed forward solve's FP64 cycles in the same
+// workspace: depth min(m, kAndersonMax); the differences dF, dG of the last
+// depth iterations, f and g of this one and the last, the reduction partials,
+// inner products and coefficients.
+constexpr int kAndersonMax = 6;
+int anderson_depth(int m) { return std::min(m, kAndersonMax); }
+struct AndersonLayout {
+    long n;
+    double *dF, *dG, *f, *g, *f_prev, *g_prev, *partial, *dots, *coef;
+};
+AndersonLayout anderson_layout(void* work, int ny, int nx, int depth) {
+    AndersonLayout W;
+    W.n = krylov_vector_len(ny, nx);
+    W.dF = static_cast<double*>(work);
+    W.dG = W.dF + (long)depth * W.n;
+    W.f = W.dG + (long)depth * W.n;
+    W.g = W.f + W.n;
+    W.f_prev = W.g + W.n;
+    W.g_prev = W.f_prev + W.n;
+    W.partial = W.g_prev + W.n;
+    W.dots = W.partial + (long)(kKrylovMaxDots + 1) * kKrylovBlocks;
+    W.coef = W.dots + (long)(kAndersonMax + 1) * (kKrylovMaxDots + 1);
+    return W;
 }
 
 // Returned by krylov_outer_loop when a restart cycle does not reduce the
@@ -1898,7 +1929,108 @@ int glide_fascd_solve_mixed(const glide_level* levels64, const glide_level* leve
                          *cfg, vc, *mc);
     };
     auto prep = [&](hipStream_t cs) -> int { return grounded(D, 0.0, cs); };
-    return mixed_outer_loop(*mc, st, key, true, report, stats, resid, step, undo, fallback, prep);
+    const bool fixed = !(mc->relative_tolerance > 0.0) && !(mc->absolute_tolerance > 0.0);
+    const int aa_m = (mc->krylov_dim > 0 && (report || !fixed)) ? anderson_depth(mc->krylov_dim) : 0;
+    AndersonLayout W{};
+    if (aa_m > 0) W = anderson_layout(mc->krylov_work, D.ny, D.nx, aa_m);
+    int aa_h = 0, aa_next = 0;
+    bool aa_prev = false;
+    R* st3[3] = {P(D.u), P(D.v), P(D.H)};
+    const R* saved3[3] = {CP(D.w_u), CP(D.w_v), CP(D.w_H)};
+    const long len3[3] = {(long)nU(D), (long)nV(D), (long)nC(D)}, off3[3] = {0, (long)nU(D), (long)(nU(D) + nV(D))};
+    double* hk = s->host + 3 * 66;
+    // Anderson mixing of the FP64 finishing cycles x -> G(x): with f = G(x) - x,
+    // x_new = G(x_k) - sum_i gamma_i (G_i - G_{i-1}), gamma minimising
+    // |f_k - sum_i gamma_i (f_i - f_{i-1})| over the last aa_m differences;
+    // kept only if it lowers the FP64 residual norm, else G(x_k) and a
+    // fresh history.
+    auto accel = [&](hipStream_t cs, int phase, double* m3) -> int {
+        if (aa_m == 0) return GLIDE_OK;
+        if (phase != 2) {
+            aa_h = aa_next = 0;
+            aa_prev = false;
+            return GLIDE_OK;
+        }
+        EwBatch<R> b;
+        for (int q = 0; q < 3; ++q) {
+            b.copy(W.g + off3[q], st3[q], len3[q]);                              // g_k = G(x_k)
+            b.axpby(W.f + off3[q], 1.0, st3[q], -1.0, saved3[q], len3[q]);      // f_k = g_k - x_k
+        }
+        HIP_TRY(launch_ew_batch<R>(b, cs), "Anderson residual");
+        if (aa_prev) {
+            const int slot = aa_next % aa_m;
+            EwBatch<R> d;
+            d.axpby(W.dF + (long)slot * W.n, 1.0, W.f, -1.0, W.f_prev, W.n);
+            d.axpby(W.dG + (long)slot * W.n, 1.0, W.g, -1.0, W.g_prev, W.n);
+            HIP_TRY(launch_ew_batch<R>(d, cs), "Anderson differences");
+            aa_next++;
+            aa_h = std::min(aa_h + 1, aa_m);
+        }
+        {
+            EwBatch<R> c;
+            c.copy(W.f_prev, W.f, W.n);
+            c.copy(W.g_prev, W.g, W.n);
+            HIP_TRY(launch_ew_batch<R>(c, cs), "Anderson history");
+        }
+        aa_prev = true;
+        if (aa_h == 0) return GLIDE_OK;
+        const int h = aa_h, row = kKrylovMaxDots + 1;
+        for (int i = 0; i < h; ++i)
+            HIP_TRY(launch_krylov_dots(W.dF + (long)i * W.n, W.dF, W.n, W.n, h, W.partial, W.dots + i * row, cs),
+                    "Anderson inner products");
+        HIP_TRY(launch_krylov_dots(W.f, W.dF, W.n, W.n, h, W.partial, W.dots + h * row, cs), "Anderson inner products");
+        HIP_TRY(hipMemcpyAsync(hk, W.dots, sizeof(double) * (h + 1) * row, hipMemcpyDeviceToHost, cs), "Anderson D2H");
+        HIP_TRY(hipStreamSynchronize(cs), "hipStreamSynchronize");
+        // (dF^T dF + delta I) gamma = dF^T f, Gaussian elimination with partial pivoting
+        double M[kAndersonMax][kAndersonMax + 1];
+        double dmax = 0.0;
+        for (int i = 0; i < h; ++i) dmax = std::max(dmax, hk[i * row + i]);
+        for (int i = 0; i < h; ++i) {
+            for (int j = 0; j < h; ++j) M[i][j] = hk[i * row + j] + (i == j ? 1e-12 * dmax : 0.0);
+            M[i][h] = hk[h * row + i];
+        }
+        for (int c = 0; c < h; ++c) {
+            int piv = c;
+            for (int i = c + 1; i < h; ++i)
+                if (std::fabs(M[i][c]) > std::fabs(M[piv][c])) piv = i;
+            if (piv != c)
+                for (int j = 0; j <= h; ++j) std::swap(M[c][j], M[piv][j]);
+            if (M[c][c] == 0.0) return GLIDE_OK;  // no usable direction: keep G(x_k)
+            for (int i = c + 1; i < h; ++i) {
+                const double f = M[i][c] / M[c][c];
+                for (int j = c; j <= h; ++j) M[i][j] -= f * M[c][j];
+            }
+        }
+        double gam[kAndersonMax];
+        for (int i = h - 1; i >= 0; --i) {
+            double t = M[i][h];
+            for (int j = i + 1; j < h; ++j) t -= M[i][j] * gam[j];
+            gam[i] = t / M[i][i];
+            if (!std::isfinite(gam[i])) return GLIDE_OK;
+        }
+        for (int i = 0; i < h; ++i) hk[i] = gam[i];
+        HIP_TRY(hipMemcpyAsync(W.coef, hk, sizeof(double) * h, hipMemcpyHostToDevice, cs), "Anderson H2D");
+        for (int q = 0; q < 3; ++q)
+            HIP_TRY(launch_krylov_axpy(st3[q], W.dG + off3[q], W.n, len3[q], h, W.coef, -1.0, cs), "Anderson update");
+        double a3[3];
+        GLIDE_TRY(resid(cs, s->norms));
+        HIP_TRY(hipMemcpyAsync(s->host, s->norms, sizeof(double) * 3, hipMemcpyDeviceToHost, cs), "norms D2H");
+        HIP_TRY(hipStreamSynchronize(cs), "hipStreamSynchronize");
+        a3[0] = s->host[0]; a3[1] = s->host[1]; a3[2] = s->host[2];
+        const double ra = std::sqrt(a3[0] + a3[1] + a3[2]), rg = std::sqrt(m3[0] + m3[1] + m3[2]);
+        if (ra < rg) {
+            std::memcpy(m3, a3, sizeof(a3));
+            return GLIDE_OK;
+        }
+        // not better: back to G(x_k), a fresh history
+        EwBatch<R> back;
+        for (int q = 0; q < 3; ++q) back.copy(st3[q], W.g + off3[q], len3[q]);
+        HIP_TRY(launch_ew_batch<R>(back, cs), "Anderson revert");
+        GLIDE_TRY(resid(cs, s->norms));  // the residual buffers and phi of G(x_k) again
+        aa_h = aa_next = 0;
+        return GLIDE_OK;
+    };
+    return mixed_outer_loop(*mc, st, key, true, report, stats, resid, step, undo, fallback, prep, accel);
 }
 
 int glide_fas_adjoint_solve_mixed(const glide_level* levels64, const glide_level* levels32, int n_levels,
@@ -2001,6 +2133,7 @@ int glide_fas_adjoint_solve_mixed(const glide_level* levels64, const glide_level
         return vjp(D, dt, GLIDE_USE_FORCING, CP(D.lambda_u), CP(D.lambda_v), CP(D.lambda_H), P(D.ar_u), P(D.ar_v),
                    P(D.ar_H), cs);
     };
+    auto no_accel = [](hipStream_t, int, double*) -> int { return GLIDE_OK; };
     const bool fixed = !(mc->relative_tolerance > 0.0) && !(mc->absolute_tolerance > 0.0);
     if (mc->krylov_dim > 0 && (report || !fixed)) {
         const KrylovLayout K = krylov_layout(mc->krylov_work, D.ny, D.nx, mc->krylov_dim);
@@ -2047,11 +2180,12 @@ int glide_fas_adjoint_solve_mixed(const glide_level* levels64, const glide_level
         // iterations as defect correction (with its undo and fallback)
         glide_mixed_config rest = *mc;
         rest.max_outer = std::max(0, mc->max_outer - its);
-        const int rc2 = mixed_outer_loop(rest, st, key, false, report, stats, resid, step, undo, fallback, prep);
+        const int rc2 = mixed_outer_loop(rest, st, key, false, report, stats, resid, step, undo, fallback, prep,
+                                         no_accel);
         if (stats) stats[1] += its;
         return rc2;
     }
-    return mixed_outer_loop(*mc, st, key, false, report, stats, resid, step, undo, fallback, prep);
+    return mixed_outer_loop(*mc, st, key, false, report, stats, resid, step, undo, fallback, prep, no_accel);
 }
 
 int64_t glide_krylov_work_doubles(int ny, int nx, int m) {

@@ -95,10 +95,12 @@ class MixedConfig:
     patch solution; fp64_coarsest runs the coarsest level's passes in FP64.
     asynchronous with both tolerances <= 0 runs exactly max_outer FP32
     iterations without a host wait and without a report (no norms, no
-    fallback); the solve then returns None.  krylov_dim (adjoint solves only)
-    > 0 runs the outer iteration as flexible GMRES restarted every krylov_dim
-    iterations, the FP32 correction V-cycles preconditioning it; 0 keeps the
-    defect-correction loop.
+    fallback); the solve then returns None.  krylov_dim > 0 runs the adjoint
+    outer iteration as flexible GMRES restarted every krylov_dim iterations
+    (the FP32 correction V-cycles precondition it) and mixes the forward
+    solve's FP64 finishing cycles by Anderson acceleration of depth
+    min(krylov_dim, 6); 0 keeps the plain loops.  The forward and adjoint
+    solves of a hierarchy share the workspace, so they must not overlap.
     compat_vprolong is off: the mixed form uses the v-facet prolongation."""
     max_outer: int = 30
     inner_vcycles: int = 1
@@ -184,7 +186,7 @@ def _run(name, levels64, mh, start_level, dt, cfg, adjoint, stream):
     stats = (ctypes.c_int * 3)()
     fas = cfg.fas_c()
     mc = cfg.to_c()
-    if adjoint and int(cfg.krylov_dim) > 0 and (rep is not None or not fixed):
+    if int(cfg.krylov_dim) > 0 and (rep is not None or not fixed):
         mc.krylov_dim = int(cfg.krylov_dim)
         mc.krylov_work = ptr(mh.krylov_work(start_level, int(cfg.krylov_dim)))
     _lib.call(name, levels64, l32, n, int(start_level), float(dt), ctypes.byref(fas), vcs, ctypes.byref(mc),

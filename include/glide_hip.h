@@ -223,11 +223,14 @@ typedef struct glide_mixed_config {
     int32_t inner_vcycles;
     double relative_tolerance, absolute_tolerance;
     int32_t fp64_coarsest;
-    /* Adjoint solve only: krylov_dim m > 0 runs the outer iteration as
-     * flexible GMRES(m) (restarted every m iterations) with the FP32
-     * correction V-cycles as the preconditioner, in krylov_work (device,
+    /* krylov_dim m > 0 (with a report or a tolerance): the adjoint solve's
+     * outer iteration is flexible GMRES(m) (restarted every m iterations)
+     * with the FP32 correction V-cycles as the preconditioner, and the
+     * forward solve's FP64 finishing cycles are mixed by Anderson
+     * acceleration of depth min(m, 6) (an extrapolated state is kept only if
+     * it lowers the FP64 residual norm), in krylov_work (device,
      * glide_krylov_work_doubles(ny, nx, m) doubles for the start level);
-     * 0 keeps the defect-correction loop. */
+     * 0 keeps the plain loops. */
     int32_t krylov_dim;
     void* krylov_work;
 } glide_mixed_config;
