@@ -15,9 +15,11 @@ timing starts.  The two forms:
 
 * fp64  - the reference's algorithm in FP64 (additive Vanka smoother with its
           default counts) for the forward and the adjoint solves;
-* mixed - glide_amd.mixed: FP32 multigrid with red-black smoothing inside an
-          FP64 defect-correction loop (FP64 residual, coarsest level and
-          accumulation; FP64 red-black V-cycles finish the forward solve).
+* mixed - glide_amd.mixed: FP32 multigrid with red-black smoothing inside
+          FP64 outer iterations (FP64 residual, coarsest level and
+          accumulation): defect correction then Anderson-mixed FP64
+          red-black V-cycles for the forward solve, flexible GMRES
+          preconditioned by the FP32 V-cycles for the adjoint.
 
 Both forms share the FP64 enthalpy solve with the reference's alternating
 sweep order: near the pressure-melting point the discrete enthalpy system can
@@ -32,6 +34,10 @@ reference-form V-cycles to 1e-14; enthalpy sweeps to 1e-13).  After
 timing, every timed step is replayed from the saved state (the solves are
 deterministic) and its u, v, H, E, multipliers and basal-drag gradient are
 compared with the truth of that step; the line reports the largest errors.
+If a replayed step misses the target, the stages feeding the quantities over
+it are tightened by one notch of their ladders and the timing is run again
+(at most four timed runs; `timed_runs` counts them), so the reported rate is
+the one of the tolerances the replay passed.
 
 Prints one JSON line.  Besides the contract fields it carries
   fp64         - the FP64 form's rate, tolerances and errors at the same target
@@ -66,6 +72,7 @@ TRUTH_FORWARD = 3e-13  # the truth's forward and enthalpy stopping tests (near t
 TRUTH_ENTHALPY = 1e-13
 TRUTH_ADJOINT = 1e-14
 CALIBRATION_MARGIN = 0.5  # the calibration steps must land within this fraction of the target
+VERIFY_ROUNDS = 4  # timed runs at most, each tightening the stages whose quantities missed the target
 
 # Algorithmic bytes of one launch per cell of its level, in values of the
 # launch's precision (each plane counted once):
@@ -325,34 +332,60 @@ def run_form(form, truth_form, args, torch):
     tols, calibrated = calibrate(form, truth_form, cal, args)
     form.restore(snap)
     log(args, f"{form.kind}: tolerances {tols} (met on the calibration steps: {calibrated})")
-    # one untimed step with the chosen tolerances (graphs of its solves recorded)
-    form.step(t, args.dt)
-    form.restore(snap)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tt = t
-    for _ in range(args.steps):
-        form.step(tt, args.dt)
-        tt += args.dt
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stats = _solver_stats(form)
-    # replay: the same steps from the same state, each against its truth
-    form.restore(snap)
-    worst = {}
-    tt = t
-    for _ in range(args.steps):
-        before = form.snapshot()
-        form.step(tt, args.dt)
-        truth_form.restore(before)
-        truth_form.step(tt, args.dt)
-        e = errors(form.results(), truth_form.results())
-        for k, v in e.items():
-            worst[k] = max(worst.get(k, 0.0), v)
-        tt += args.dt
-    torch.cuda.synchronize()
+    for rnd in range(VERIFY_ROUNDS):
+        form.set_tolerances(tols["forward"], tols["enthalpy"], tols["adjoint"])
+        # one untimed step with the chosen tolerances (graphs of its solves recorded)
+        form.restore(snap)
+        form.step(t, args.dt)
+        form.restore(snap)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tt = t
+        for _ in range(args.steps):
+            form.step(tt, args.dt)
+            tt += args.dt
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        stats = _solver_stats(form)
+        # replay: the same steps from the same state, each against its truth
+        # (a truth depends only on the state it starts from)
+        form.restore(snap)
+        worst = {}
+        tt = t
+        for _ in range(args.steps):
+            before = form.snapshot()
+            form.step(tt, args.dt)
+            res = form.results()
+            truth_form.restore(before)
+            truth_form.step(tt, args.dt)
+            e = errors(res, truth_form.results())
+            for q, v in e.items():
+                worst[q] = max(worst.get(q, 0.0), v)
+            tt += args.dt
+        torch.cuda.synchronize()
+        over = [q for q, v in worst.items() if not v <= args.target]
+        log(args, f"{form.kind}: round {rnd}, tolerances {tols}, {1e3 * elapsed / args.steps:.1f} ms/step, largest "
+                  f"errors " + " ".join(f"{q} {v:.1e}" for q, v in worst.items()))
+        if not over or rnd + 1 == VERIFY_ROUNDS:
+            break
+        # a timed step missed the target: tighten the stages that feed the
+        # quantities over it by one notch of their ladders and time again
+        if any(q in FORWARD_KEYS or q in ADJOINT_KEYS for q in over):
+            tols["forward"] = _tighter(FORWARD_LADDER, tols["forward"])
+        if any(q in ADJOINT_KEYS for q in over):
+            tols["adjoint"] = _tighter(ADJOINT_LADDER, tols["adjoint"])
+        if "E" in over:
+            tols["enthalpy"] = _tighter(ENTHALPY_LADDER, tols["enthalpy"])
     return {"elapsed": elapsed, "tolerances": tols, "calibrated": calibrated, "errors": worst, "snap": snap,
-            "t": t, "stats": stats}
+            "t": t, "stats": stats, "verify_rounds": rnd + 1}
+
+
+def _tighter(ladder, v):
+    """The next tighter value of a ladder (the tightest stays)."""
+    for x in ladder:
+        if x < v * (1 - 1e-9):
+            return x
+    return ladder[-1]
 
 
 def _solver_stats(form):
@@ -702,6 +735,8 @@ def main():
         fp64 = {"value": round(args.steps / res_d["elapsed"], 4),
                 "ms_per_step": round(1000.0 * res_d["elapsed"] / args.steps, 3),
                 "tolerances": res_d["tolerances"], "target_met_on_calibration": res_d["calibrated"],
+                "target_met": all(v <= args.target for v in res_d["errors"].values()),
+                "timed_runs": res_d["verify_rounds"],
                 "rel_l2_err_vs_truth": {k: float(f"{v:.3e}") for k, v in res_d["errors"].items()},
                 "last_step": st,
                 "kernels": {k: {"launches": v[0], "total_ms": round(v[1], 3)} for k, v in tot_d.items() if v[0] > 0},
@@ -728,7 +763,7 @@ def main():
                        "grid": f"{args.n}x{args.n}", "levels": args.levels, "sigma_levels": args.nz,
                        "dt_years": args.dt, "target_rel_l2": args.target, "parallelism": "single GPU"},
             "rel_l2_err_vs_truth": errs_m, "target_met": all(v <= args.target for v in res_m["errors"].values()),
-            "tolerances": res_m["tolerances"], "last_step": res_m["stats"],
+            "tolerances": res_m["tolerances"], "timed_runs": res_m["verify_rounds"], "last_step": res_m["stats"],
             "speedup_vs_fp64": (round(value / fp64["value"], 3) if fp64 else None),
             "fp64": fp64, "roofline": roofline, "hbm_fractions": fracs, "cpu_baseline": cpu,
             "kernels": kernels_m,
