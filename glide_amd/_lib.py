@@ -154,6 +154,7 @@ SIGNATURES = {
     "glide_fas_adjoint_vcycle": (_I, [_LP, _I, _I, _I, _D, ctypes.POINTER(FasConfigC),
                                       ctypes.POINTER(VankaConfigC), _P]),
     "glide_gradients": (_I, [_LP, _D, _I, _P]),
+    "glide_gradient_rate_factor": (_I, [_LP, _D, _P, _P]),
     "glide_krylov_work_doubles": (ctypes.c_int64, [_I, _I, _I]),
     "glide_graphs_enable": (_I, [_I]),
     "glide_graphs_capacity": (_I, [_I]),

@@ -2267,6 +2267,15 @@ int glide_timing_read_units(int kernel, long units, long* count, double* total_m
     return GLIDE_OK;
 }
 
+int glide_gradient_rate_factor(const glide_level* L, double dt, void* g_B, void* stream) {
+    GLIDE_TRY(check_level(L));
+    GLIDE_TRY(check_ssa_inputs(*L));
+    REQ(L->lambda_u); REQ(L->lambda_v); REQ(g_B);
+    SsaArgs<R> a = make_args(*L, dt, true);
+    HIP_TRY(launch_grad_B<R>(a, CP(L->lambda_u), CP(L->lambda_v), P(g_B), (hipStream_t)stream), "grad B");
+    return GLIDE_OK;
+}
+
 int glide_gradients(const glide_level* L, double dt, int which, void* stream) {
     GLIDE_TRY(check_level(L));
     const int exact = (which & GLIDE_GRAD_EXACT) ? 1 : 0;

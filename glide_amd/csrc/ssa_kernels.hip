@@ -1826,6 +1826,52 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
     return hipGetLastError();
 }
 
+// Gradient of lambda^T R with respect to the rate factor B (no counterpart in
+// the reference, whose gradients cover beta, bed, H_prev and smb,
+// glide/cuda/grad.cu:1-169).  B enters R only through the cell viscosity
+// eta = B/2 (...)^{(1-n)/(2n)}: eta H scales the cell's normal stresses and,
+// averaged over four cells, the shear stress of each corner vertex, so
+// g_B = S eta H / B with S = d(lambda^T R)/d(eta H) of the cell, the bracket
+// the transpose operator's thickness row multiplies by eta (vjp_body), over
+// the stress rows of interior facets (Dirichlet rows do not depend on B).
+template <typename T>
+__global__ __launch_bounds__(256) void k_grad_B(SsaArgs<T> a, const T* __restrict__ lu, const T* __restrict__ lv,
+                                                T* __restrict__ gB) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    const int ny = a.ny, nx = a.nx;
+    if (j >= nx) return;
+    const T dxi = a.dxi;
+    const long c = (long)i * nx + j;
+    const CellVel<T> cv = load_cell_vel(a.u, a.v, i, j, ny, nx);
+    const T e2 = strain_invariant(cv, i, j, ny, nx, dxi, a.P.eps_reg);
+    const T eta = T(0.5) * a.B[c] * glen_power(e2, a.P);
+    const T Hc = a.H[c];
+    const NormalStress<T> sx = sxx(cv.ul, cv.ur, cv.vt, cv.vb, eta * Hc, dxi, true);
+    const NormalStress<T> sy = syy(cv.ul, cv.ur, cv.vt, cv.vb, eta * Hc, dxi, true);
+    const long ol = (long)i * (nx + 1) + j;
+    const T lul = j > 0 ? lu[ol] : T(0), lur = j + 1 < nx ? lu[ol + 1] : T(0);
+    const T lvt = i > 0 ? lv[c] : T(0), lvb = i + 1 < ny ? lv[c + nx] : T(0);
+    T S = sx.d_eH * (lul - lur) * dxi + sy.d_eH * (lvb - lvt) * dxi;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // corner vertices (i, j), (i, j+1), (i+1, j), (i+1, j+1)
+        const int vi = i + (q >> 1), vj = j + (q & 1);
+        if (vi > 0 && vi < ny && vj > 0 && vj < nx) {
+            const long ut = (long)(vi - 1) * (nx + 1) + vj, ub = (long)vi * (nx + 1) + vj;
+            const long vl = (long)vi * nx + vj - 1, vr = (long)vi * nx + vj;
+            const ShearStress<T> sh = sxy(a.u[ut], a.u[ub], a.v[vl], a.v[vr], T(1), dxi, true);
+            S += sh.d_eH * T(0.25) * (lu[ub] - lu[ut] + lv[vl] - lv[vr]) * dxi;
+        }
+    }
+    gB[c] = a.B[c] != T(0) ? S * eta * Hc / a.B[c] : T(0);
+}
+
+template <typename T>
+hipError_t launch_grad_B(const SsaArgs<T>& a, const T* lu, const T* lv, T* gB, hipStream_t st) {
+    dim3 g((a.nx + 255) / 256, a.ny);
+    k_grad_B<T><<<g, 256, 0, st>>>(a, lu, lv, gB);
+    return hipGetLastError();
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero_masked(T* __restrict__ g, const T* __restrict__ mask, long n) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1867,7 +1913,8 @@ hipError_t launch_zero_masked(T* g, const T* mask, long n, hipStream_t st) {
     template hipError_t launch_vanka_adjoint_pass_cached<T>(int, int, const T*, const T*, const T*, const T*, T*, T*, \
                                                             T*, T*, T, bool, hipStream_t);                           \
     template hipError_t launch_grad_beta_bed<T>(const SsaArgs<T>&, const T*, const T*, T*, T*, int, hipStream_t); \
-    template hipError_t launch_zero_masked<T>(T*, const T*, long, hipStream_t);
+    template hipError_t launch_zero_masked<T>(T*, const T*, long, hipStream_t);                                       \
+    template hipError_t launch_grad_B<T>(const SsaArgs<T>&, const T*, const T*, T*, hipStream_t);
 
 GLIDE_INSTANTIATE(double)
 GLIDE_INSTANTIATE(float)

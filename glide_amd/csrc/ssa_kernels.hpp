@@ -203,6 +203,9 @@ hipError_t launch_grad_beta_bed(const SsaArgs<T>& a, const T* lu, const T* lv, T
 // g = 0 where mask > 0.5.
 template <typename T>
 hipError_t launch_zero_masked(T* g, const T* mask, long n, hipStream_t st);
+// g_B = lambda^T dR/dB through the viscosity of the stresses.
+template <typename T>
+hipError_t launch_grad_B(const SsaArgs<T>& a, const T* lu, const T* lv, T* gB, hipStream_t st);
 
 
 // Multicolour (red-black) Vanka smoothing (mc_kernels.hip; no counterpart in

@@ -29,12 +29,14 @@ class GlideStepWarm(torch.autograd.Function):
     """GlideStep (glide/torch.py:4-59) with the initial iterate as an input.
 
     ``GlideStepWarm.apply(t, dt, model, level, H_prev, bed, beta, smb, u0, v0,
-    mask0) -> (u, v, H, mask)``: the forward solve starts from u0, v0 (the
-    velocities), H = H_prev and the active set mask0; the backward solves the
-    adjoint at the returned state and returns the gradients with respect to
-    H_prev, bed, beta and smb (the converged solution does not depend on the
+    mask0, B=None) -> (u, v, H, mask)``: the forward solve starts from u0, v0
+    (the velocities), H = H_prev and the active set mask0; the backward solves
+    the adjoint at the returned state and returns the gradients with respect
+    to H_prev, bed, beta and smb, and B when it is given (the rate factor, no
+    counterpart in the reference's GlideStep; otherwise the model's own B is
+    used and gets no gradient).  The converged solution does not depend on the
     initial iterate, so u0, v0 and mask0 get none; mask is not
-    differentiable).  The adjoint solve starts from zero multipliers, and the
+    differentiable.  The adjoint solve starts from zero multipliers, and the
     gradients are those of the discrete problem (GLIDE_GRAD_EXACT: the
     reference's gradient kernel drops the boundary-facet terms of the last
     column and row).  The adjoint holds the grounded fraction fixed, as the
@@ -42,30 +44,34 @@ class GlideStepWarm(torch.autograd.Function):
     does not vary with H (fully grounded or fully floating ice)."""
 
     @staticmethod
-    def forward(ctx, t, dt, model, level, H_prev, bed, beta, smb, u0, v0, mask0):
+    def forward(ctx, t, dt, model, level, H_prev, bed, beta, smb, u0, v0, mask0, B=None):
         ctx.t, ctx.dt, ctx.model, ctx.level = t, dt, model, level
         model.set_top_level(level)
         mg = model.mg
-        for field, value in ((mg.state.H_prev, H_prev), (mg.state.H, H_prev), (mg.geometry.bed, bed),
-                             (mg.sliding.beta, beta), (mg.forcing.smb, smb), (mg.state.u, u0), (mg.state.v, v0),
-                             (mg.state.mask, mask0)):
+        fields = [(mg.state.H_prev, H_prev), (mg.state.H, H_prev), (mg.geometry.bed, bed), (mg.sliding.beta, beta),
+                  (mg.forcing.smb, smb), (mg.state.u, u0), (mg.state.v, v0), (mg.state.mask, mask0)]
+        if B is not None:
+            fields.append((mg.rheology.B, B))
+        for field, value in fields:
             field.set(value.detach(), start_level=level)
         model.forward(t, dt, update_geometry=False)
         st = mg[level].state
         u, v, H = st.u.data.clone(), st.v.data.clone(), st.H.data.clone()
         mask, phi = st.mask.data.clone(), st.phi.data.clone()
-        ctx.save_for_backward(u, v, H, mask, phi, H_prev, bed, beta, smb)
+        ctx.has_B = B is not None
+        ctx.save_for_backward(u, v, H, mask, phi, H_prev, bed, beta, smb,
+                              B if B is not None else mg[level].rheology.B.data.clone())
         ctx.mark_non_differentiable(mask)
         return u, v, H, mask
 
     @staticmethod
     def backward(ctx, gu, gv, gH, gmask):
         model, level, dt = ctx.model, ctx.level, ctx.dt
-        u, v, H, mask, phi, H_prev, bed, beta, smb = ctx.saved_tensors
+        u, v, H, mask, phi, H_prev, bed, beta, smb, B = ctx.saved_tensors
         mg = model.mg
         for field, value in ((mg.state.H_prev, H_prev), (mg.geometry.bed, bed), (mg.sliding.beta, beta),
-                             (mg.forcing.smb, smb), (mg.state.u, u), (mg.state.v, v), (mg.state.H, H),
-                             (mg.state.phi, phi), (mg.state.mask, mask)):
+                             (mg.forcing.smb, smb), (mg.rheology.B, B), (mg.state.u, u), (mg.state.v, v),
+                             (mg.state.H, H), (mg.state.phi, phi), (mg.state.mask, mask)):
             field.set(value.detach(), start_level=level)
         # the adjoint solve starts from zero multipliers, so that a backward
         # pass does not depend on what the model solved before it
@@ -74,10 +80,11 @@ class GlideStepWarm(torch.autograd.Function):
             lam.data.zero_()
         z = torch.zeros_like
         model.backward(ctx.t, dt, dJdu=gu if gu is not None else z(u), dJdv=gv if gv is not None else z(v),
-                       dJdH=gH if gH is not None else z(H), exact_gradients=True)
+                       dJdH=gH if gH is not None else z(H), exact_gradients=True, compute_B_grad=ctx.has_B)
         g = mg[level]
         return (None, None, None, None, g.state.H_prev.grad.clone(), g.geometry.bed.grad.clone(),
-                g.sliding.beta.grad.clone(), g.forcing.smb.grad.clone(), None, None, None)
+                g.sliding.beta.grad.clone(), g.forcing.smb.grad.clone(), None, None, None,
+                g.rheology.B.grad.clone() if ctx.has_B else None)
 
 
 def misfit(u, v, u_obs, v_obs):
@@ -94,23 +101,24 @@ def gradient_penalty(log_beta, weight):
 
 
 def rollout(model, H0, bed, beta, smb, n_steps: int, dt: float, step_loss: Callable, t0: float = 0.0,
-            level: int = 0, checkpoint: Optional[int] = 8, u0=None, v0=None, mask0=None):
+            level: int = 0, checkpoint: Optional[int] = 8, u0=None, v0=None, mask0=None, B=None):
     """sum over k of step_loss(k, u_k, v_k, H_k) over n_steps implicit steps
     from H0, each step's H the next one's H_prev, differentiable with respect
-    to H0, bed, beta and smb.  With checkpoint = c, the rollout keeps the state
-    at every c-th step boundary and recomputes each segment in the backward
-    pass; checkpoint None or 0 keeps every step's state.  Returns (loss,
-    (u, v, H, mask) after the last step)."""
+    to H0, bed, beta and smb, and the rate factor B when it is given.  With
+    checkpoint = c, the rollout keeps the state at every c-th step boundary
+    and recomputes each segment in the backward pass; checkpoint None or 0
+    keeps every step's state.  Returns (loss, (u, v, H, mask) after the last
+    step)."""
     g = model.mg[level]
     z = torch.zeros_like
     u = z(g.state.u.data) if u0 is None else u0
     v = z(g.state.v.data) if v0 is None else v0
     mask = z(g.state.mask.data) if mask0 is None else mask0
 
-    def segment(k0, k1, H, u, v, mask, bed, beta, smb):
+    def segment(k0, k1, H, u, v, mask, bed, beta, smb, B):
         loss = H.new_zeros(())
         for k in range(k0, k1):
-            u, v, H, mask = GlideStepWarm.apply(t0 + k * dt, dt, model, level, H, bed, beta, smb, u, v, mask)
+            u, v, H, mask = GlideStepWarm.apply(t0 + k * dt, dt, model, level, H, bed, beta, smb, u, v, mask, B)
             loss = loss + step_loss(k, u, v, H)
         return loss, H, u, v, mask
 
@@ -120,9 +128,9 @@ def rollout(model, H0, bed, beta, smb, n_steps: int, dt: float, step_loss: Calla
     for k0 in range(0, n_steps, c):
         k1 = min(k0 + c, n_steps)
         if checkpoint:
-            loss, H, u, v, mask = _checkpoint(segment, k0, k1, H, u, v, mask, bed, beta, smb, use_reentrant=False)
+            loss, H, u, v, mask = _checkpoint(segment, k0, k1, H, u, v, mask, bed, beta, smb, B, use_reentrant=False)
         else:
-            loss, H, u, v, mask = segment(k0, k1, H, u, v, mask, bed, beta, smb)
+            loss, H, u, v, mask = segment(k0, k1, H, u, v, mask, bed, beta, smb, B)
         total = total + loss
     return total, (u, v, H, mask)
 

@@ -62,12 +62,14 @@ class IceDynamics:
             f(t + dt)
 
     def backward(self, t, dt, dJdu=None, dJdv=None, dJdH=None, compute_beta_grad=True, compute_bed_grad=True,
-                 compute_H_prev_grad=True, compute_smb_grad=True, exact_gradients=False):
+                 compute_H_prev_grad=True, compute_smb_grad=True, exact_gradients=False, compute_B_grad=False):
         """Adjoint solve Jᵀλ = -dJ/dx and parameter gradients (glide/model.py:74-94).
 
         exact_gradients (an addition): the gradient of the discrete problem,
         with the boundary-facet terms the reference's gradient kernel drops and
-        zero H_prev / smb gradients in active-set cells (GLIDE_GRAD_EXACT)."""
+        zero H_prev / smb gradients in active-set cells (GLIDE_GRAD_EXACT).
+        compute_B_grad (an addition): the rate factor's gradient into
+        rheology.B.grad."""
         top = self.mg.levels[self.top_level]
         ao = top.adjoint_operators
         for buf, g in ((ao.f_u, dJdu), (ao.f_v, dJdv), (ao.f_H, dJdH)):
@@ -82,6 +84,8 @@ class IceDynamics:
             which |= 16
         if which:
             ao._grad(which, dt)
+        if compute_B_grad:
+            ao.compute_gradient_B(dt)
 
 
 class ThermalModel:

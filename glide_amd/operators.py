@@ -255,6 +255,13 @@ class AdjointOperators:
         L = self._level()
         call("glide_gradients", ctypes.byref(L), float(dt), int(which), stream_handle(g.device))
 
+    def compute_gradient_B(self, dt=1.0):
+        """rheology.B.grad = lambda^T dR/dB (no counterpart in the reference,
+        whose gradients cover beta, bed, H_prev and smb)."""
+        g = self.grid
+        L = self._level()
+        call("glide_gradient_rate_factor", ctypes.byref(L), float(dt), ptr(g.rheology.B.grad), stream_handle(g.device))
+
     def compute_gradient_beta(self):
         self._grad(1)
 

@@ -288,6 +288,13 @@ int glide_fas_adjoint_vcycle(const glide_level* levels, int n_levels, int level,
  * depend on them; the reference's multipliers there are not determined). */
 #define GLIDE_GRAD_EXACT 16
 int glide_gradients(const glide_level* L, double dt, int which, void* stream);
+/* No counterpart in the reference (its gradients are those of
+ * glide_gradients): g_B = lambda^T dR/dB, the gradient with respect to the
+ * rate factor B through the cell viscosity eta = B/2 (...)^((1-n)/(2n)) of the
+ * normal stresses and the four corners' shear stresses, over the rows of
+ * interior facets (the multipliers of L; every cell, including active-set
+ * cells, whose facets' stress rows still hold their viscosity). */
+int glide_gradient_rate_factor(const glide_level* L, double dt, void* g_B, void* stream);
 
 /* Graph replay of fixed-cycle solves (no counterpart in the reference; the
  * reference launches every kernel from Python): when both tolerances are <= 0,

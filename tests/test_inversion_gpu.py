@@ -5,7 +5,7 @@ log(beta), through the native forward and adjoint solves chained by torch
 
 * Time-step checkpointing recomputes the forward solves bit for bit, so the
   checkpointed gradient equals the stored-state gradient exactly.
-* Taylor test in 8 seeded random directions: the remainder
+* Taylor test in 8 seeded random directions (4 for the rate factor B): the remainder
   |J(m + e d) - J(m) - e <dJ/dm, d>| falls by ~100x per decade of e (second
   order) until it reaches the solves' rounding floor.  The adjoint holds the
   grounded fraction fixed (as the reference's does) while the forward solve
@@ -144,3 +144,55 @@ def test_taylor_remainders_on_the_marine_dome_are_reported():
     print(f"  relative mismatch of the directional derivative (e = 1e-6): {np.round(rel, 3).tolist()}", flush=True)
     assert all(np.isfinite(rel))
     assert min(orders) > 0.2 and np.median(orders) > 0.9
+
+
+def test_taylor_remainders_in_the_rate_factor_are_second_order():
+    """The misfit's gradient with respect to log B (the rate factor, no
+    counterpart in the reference's GlideStep) through the checkpointed
+    rollout, on the grounded dome: second-order Taylor remainders."""
+    from glide_amd.inversion import misfit, rollout
+    print("inversion 64^2 L4, grounded dome: Taylor test in log B, 4 directions, 6 steps", flush=True)
+    model, H0, bed, lb, lb_true, smb, steps = _setup(grounded=True)
+    beta = torch.exp(lb)
+    B0 = model.mg[0].rheology.B.data.clone()
+    gen = torch.Generator(device="cuda").manual_seed(41)
+    logB_true = torch.log(B0) + 0.3 * torch.randn(B0.shape, generator=gen, device="cuda", dtype=B0.dtype)
+    obs = []
+
+    def record(k, u, v, H):
+        obs.append((u.detach().clone(), v.detach().clone()))
+        return u.new_zeros(())
+
+    with torch.no_grad():
+        rollout(model, H0, bed, beta, smb, steps, DT, record, checkpoint=None, B=torch.exp(logB_true))
+
+    def Jf(logB):
+        loss, _ = rollout(model, H0, bed, beta, smb, steps, DT, lambda k, u, v, H: misfit(u, v, *obs[k]),
+                          checkpoint=2, B=torch.exp(logB))
+        return loss
+
+    m = torch.log(B0).clone().requires_grad_()
+    J0 = Jf(m)
+    J0.backward()
+    g = m.grad.clone()
+    J0 = float(J0.detach())
+    eps = [1e-2, 1e-3, 1e-4, 1e-5]
+    floor = 1e-11 * abs(J0)
+    orders, rel = [], []
+    for q in range(4):
+        d = torch.randn(m.shape, generator=gen, device="cuda", dtype=m.dtype)
+        dJ = float((g * d).sum())
+        with torch.no_grad():
+            rem = [abs(float(Jf(torch.log(B0) + e * d)) - J0 - e * dJ) for e in eps]
+        print(f"  direction {q}: <g,d> = {dJ:.4e}, remainders " + " ".join(f"{r:.2e}" for r in rem), flush=True)
+        for k in range(len(eps) - 1):
+            if rem[k + 1] > 10 * floor:
+                orders.append(np.log10(rem[k] / rem[k + 1]))
+        rel.append(rem[-1] / (eps[-1] * abs(dJ)))
+    print(f"  observed orders above the floor: {np.round(np.array(orders), 2).tolist()}", flush=True)
+    # second order down to e = 1e-5: a directional-derivative error d would
+    # add d e to the remainder, so |d| < rem(1e-5) / 1e-5, a bound that the
+    # curvature in log B keeps near 1e-3 |<g,d>|
+    assert max(rel) < 1e-2
+    assert len(orders) >= 4
+    assert min(orders) > 1.7

@@ -568,6 +568,38 @@ def test_gradients(orc, shape):
     assert rel_err(N(g.forcing.smb.grad), -lH) < KTOL
 
 
+@pytest.mark.parametrize("shape,m", [((32, 32), 1.0), ((24, 40), 1.0 / 3.0)])
+def test_rate_factor_gradient_matches_residual_differences(orc, shape, m):
+    """lambda^T dR/dB (no counterpart in the reference) against central
+    differences of lambda^T R(B) with the oracle's residual, in random
+    directions (active-set cells included; Dirichlet rows do not depend on B)."""
+    ny, nx = shape
+    print(f"rate-factor gradient shape={shape} m={m}", flush=True)
+    F = dome_fields(ny=ny, nx=nx, seed=21, mask_frac=0.1)
+    P = orc.phys(m=m)
+    lu, lv, lH = _lam(F, 22)
+    g = grid_from(F)
+    g.sliding.m.set(m)
+    a = g.adjoint
+    a.lambda_u.set(lu)
+    a.lambda_v.set(lv)
+    a.lambda_H.set(lH)
+    g.adjoint_operators.compute_gradient_B(10.0)
+    torch.cuda.synchronize()
+    gB = N(g.rheology.B.grad)
+    rng = np.random.default_rng(23)
+    eps = 1e-6
+    for _ in range(3):
+        d = rng.standard_normal((ny, nx)) * F["B"]
+        Fp, Fm = dict(F), dict(F)
+        Fp["B"], Fm["B"] = F["B"] + eps * d, F["B"] - eps * d
+        rp, rm = orc.residual(Fp, P, 1500.0, 10.0), orc.residual(Fm, P, 1500.0, 10.0)
+        fd = sum(float(np.sum(lam * (x - y))) for lam, x, y in zip((lu, lv, lH), rp, rm)) / (2.0 * eps)
+        an = float(np.sum(gB * d))
+        print(f"  <g_B, d> = {an:.10e}, central difference {fd:.10e}", flush=True)
+        assert abs(an - fd) <= 1e-7 * abs(fd)
+
+
 def _dome_models(orc, n, L, seed=0, dx=1500.0):
     from glide_amd import IceDynamics
     from glide_amd.synthetic import dome_problem, install
