@@ -1458,12 +1458,19 @@ __global__ __launch_bounds__(256) void k_enth_column_lm(EnthArgs<T> A, T* __rest
             for (int m = 0; m < MB; ++m) asm volatile("" : "+v"(om[m]));
             const T Eb = from_below<1>(El[MB - 1]);  // level k0 - 1
             const T Ea = from_above<1>(El[0]);       // level k0 + MB
-            T Kl;  // diffusivity of the interface below level k0 (the lane below's last)
-            {
-                const int k = k0 + MB - 1;
-                const T Ep_k = e_pmp(T(k) * dsig, hh), Ep_p = e_pmp(T(min(k + 1, nz - 1)) * dsig, hh);
-                Kl = from_below<1>(diffusivity(T(0.5) * (El[MB - 1] + Ea), T(0.5) * (Ep_k + Ep_p)));
+            // interface diffusivities above the block's levels; the one below
+            // level k0 is the lane below's last
+            T Ku[MB];
+#pragma unroll
+            for (int m = 0; m < MB; ++m) {
+                const int k = k0 + m;
+                Ku[m] = T(0);
+                if (k < nz - 1) {
+                    const T Ep_k = e_pmp(T(k) * dsig, hh), Ep_p = e_pmp(T(k + 1) * dsig, hh);
+                    Ku[m] = diffusivity(T(0.5) * (El[m] + (m < MB - 1 ? El[m + 1] : Ea)), T(0.5) * (Ep_k + Ep_p));
+                }
             }
+            const T Kl0 = from_below<1>(Ku[MB - 1]);
             // rows and forward elimination: row m becomes
             // Ac[m] x_0 + x_m + Cc[m] x_{m+1} = Dc[m] (row 0: Ac[0] multiplies
             // the lane below's last unknown)
@@ -1473,13 +1480,11 @@ __global__ __launch_bounds__(256) void k_enth_column_lm(EnthArgs<T> A, T* __rest
                 const int k = k0 + m;
                 const T Ekm1 = m > 0 ? El[m - 1] : Eb;
                 const T Ekp1 = m < MB - 1 ? El[m + 1] : Ea;
-                T a = T(0), b = T(1), c = T(0), d = T(0), Ku = T(0);
+                T a = T(0), b = T(1), c = T(0), d = T(0);
                 if (k < nz - 1) {
-                    const T Ep_k = e_pmp(T(k) * dsig, hh), Ep_p = e_pmp(T(k + 1) * dsig, hh);
-                    Ku = diffusivity(T(0.5) * (El[m] + Ekp1), T(0.5) * (Ep_k + Ep_p));
                     Row<T> w{B0 * El[m] + G[m], T(0), B0, T(0)};
                     column_terms(A, cst, k, hh, El[m], Ekm1, Ekp1, m > 0 ? om[m - 1] : om_lo, om[m],
-                                 m < MB - 1 ? om[m + 1] : om_hi, Ku, Kl, w);
+                                 m < MB - 1 ? om[m + 1] : om_hi, Ku[m], m > 0 ? Ku[m - 1] : Kl0, w);
                     a = k == 0 ? T(0) : w.a;
                     b = w.b;
                     c = w.c;
@@ -1488,7 +1493,6 @@ __global__ __launch_bounds__(256) void k_enth_column_lm(EnthArgs<T> A, T* __rest
                 } else if (k == nz - 1) {
                     d = -(El[m] - Es);  // Dirichlet surface row
                 }
-                Kl = Ku;
                 if (m < 2) {
                     const T rb = fast_rcp(b);
                     Ac[m] = a * rb;
