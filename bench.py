@@ -86,8 +86,10 @@ VERIFY_ROUNDS = 4  # timed runs at most, each tightening the stages whose quanti
 #     values; writes the three multiplier planes: 16 + 12.5 + 3
 #   FP64 residual: 12 inputs + 3 outputs
 ALG_VALUES = {"vanka_mc32": 14.0, "vanka_mc64": 14.0, "adjoint_mc32": 31.5, "adjoint_mc64": 31.5,
-              "residual": 15.0}
-WORD = {"vanka_mc32": 4, "vanka_mc64": 8, "adjoint_mc32": 4, "adjoint_mc64": 8, "residual": 8}
+              "residual": 15.0, "vjp": 18.0}
+# bytes per value; at the finest level the transpose operator runs in FP64
+# (the mixed adjoint's Krylov products and residuals)
+WORD = {"vanka_mc32": 4, "vanka_mc64": 8, "adjoint_mc32": 4, "adjoint_mc64": 8, "residual": 8, "vjp": 8}
 TIMED = ("vanka_mc32", "vanka_mc64", "adjoint_mc32", "adjoint_mc64", "residual", "vanka_cell", "vanka_apply",
          "adjoint_cell", "adjoint_apply", "vjp", "adjoint_factor", "enth_column", "enth_layer", "enth_residual")
 
