@@ -112,6 +112,14 @@ class ThermalConfigC(ctypes.Structure):
                 ("update_rheology", ctypes.c_int32), ("strain_heating", ctypes.c_int32)]
 
 
+THERMAL_VJP_POINTERS = ("lambda_", "g_phi_strain", "g_Q", "dL_dB", "work", "g_u", "g_v", "g_H", "g_H_prev", "g_beta",
+                        "g_B")
+
+
+class ThermalVjpC(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in THERMAL_VJP_POINTERS]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -193,6 +201,9 @@ SIGNATURES = {
                                       ctypes.POINTER(EnthSweepConfigC), ctypes.POINTER(EnthReport), _P]),
     "glide_thermal_step": (_I, [_CP, _P, _P, _P, _P, ctypes.POINTER(ThermalConfigC),
                                 ctypes.POINTER(EnthSweepConfigC), ctypes.POINTER(EnthReport), _P]),
+    "glide_thermal_vjp_work_doubles": (ctypes.c_int64, [_I, _I]),
+    "glide_thermal_step_vjp": (_I, [_CP, _P, _P, _P, _P, ctypes.POINTER(ThermalConfigC), ctypes.POINTER(ThermalVjpC),
+                                    _P]),
 }
 
 TIMED_KERNELS = {"vanka_cell": 0, "vanka_apply": 1, "adjoint_cell": 2, "vjp": 3, "residual": 4, "adjoint_apply": 5,

@@ -577,4 +577,64 @@ int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* 
     return GLIDE_OK;
 }
 
+int64_t glide_thermal_vjp_work_doubles(int ny, int nx) {
+    if (ny < 1 || nx < 1) return -1;
+    return (int64_t)kThermalVjpPlanes * ny * nx;
+}
+
+int glide_thermal_step_vjp(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta,
+                           const void* B, const glide_thermal_config* tc, const glide_thermal_vjp* V, void* stream) {
+    if (!tc || !V) return fail(GLIDE_ERR_INVALID_ARG, "thermal config or gradient buffers is NULL");
+    EnthArgs<R> A;
+    GLIDE_TRY(make_args(C, tc->dt, true, &A));
+    REQ(u2d);
+    REQ(v2d);
+    REQ(V->lambda);
+    REQ(V->work);
+    REQ(V->g_u);
+    REQ(V->g_v);
+    REQ(V->g_H);
+    if (tc->frictional_heating) {
+        REQ(beta);
+        REQ(V->g_Q);
+    }
+    if (tc->strain_heating) {
+        REQ(B);
+        REQ(V->g_phi_strain);
+        REQ(C->phi_strain);
+    }
+    if (V->dL_dB && !tc->update_rheology)
+        return fail(GLIDE_ERR_INVALID_ARG, "dL_dB needs update_rheology (the step returns no rate factor)");
+    if ((V->dL_dB || tc->strain_heating) && (!(tc->n_glen > 0.0) || !(tc->rho_i > 0.0)))
+        return fail(GLIDE_ERR_INVALID_ARG, "n_glen and rho_i must be positive");
+    if (tc->strain_heating && !(tc->eps_reg >= 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "eps_reg must be >= 0");
+    A.u2d = CP(u2d);
+    A.v2d = CP(v2d);
+    A.vel_div = kSecPerYear;
+    ThermalVjpArgs<R> W{};
+    W.lam = CP(V->lambda);
+    W.g_phi = tc->strain_heating ? CP(V->g_phi_strain) : nullptr;
+    W.g_Q = tc->frictional_heating ? CP(V->g_Q) : nullptr;
+    W.gB = (tc->update_rheology && V->dL_dB) ? CP(V->dL_dB) : nullptr;
+    W.B_in = tc->strain_heating ? CP(B) : nullptr;
+    W.beta = tc->frictional_heating ? CP(beta) : nullptr;
+    W.n_glen = tc->n_glen;
+    W.B_scale = tc->rho_i * kGravity * std::pow(kSecPerYear, 1.0 / tc->n_glen);
+    W.glen_p = (1.0 - tc->n_glen) / (2.0 * tc->n_glen);
+    W.eps_reg = tc->eps_reg;
+    W.rho_g = tc->rho_i * kGravity;
+    W.m = tc->m;
+    W.u_reg = tc->u_reg;
+    W.dt_yr = tc->dt / kSecPerYear;
+    W.work = P(V->work);
+    W.g_u = P(V->g_u);
+    W.g_v = P(V->g_v);
+    W.g_H = P(V->g_H);
+    W.g_H_prev = P(V->g_H_prev);
+    W.g_beta = tc->frictional_heating ? P(V->g_beta) : nullptr;
+    W.g_B = tc->strain_heating ? P(V->g_B) : nullptr;
+    HIP_TRY(launch_thermal_vjp<R>(A, W, S(stream)), "thermal step input gradients");
+    return GLIDE_OK;
+}
+
 }  // extern "C"

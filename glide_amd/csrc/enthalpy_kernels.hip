@@ -1260,6 +1260,262 @@ __global__ __launch_bounds__(256) void k_enth_adj_grads(EnthArgs<T> A, const T* 
     }
 }
 
+// ------------------------------------------------- thermal-step input gradients
+// dL/dp = -lam^T dG/dp for the inputs the momentum step hands the thermal
+// step (glide_thermal_step): the depth-constant velocities u, v (m/a), the new
+// and previous thicknesses H, H_prev, the sliding coefficient beta (through
+// the frictional heating) and the rate factor the strain heating read.  Each
+// input reaches G directly (time term, diffusion through 1/H and the
+// pressure-melting enthalpy, drainage, h phi forcing, facet mass fluxes of the
+// horizontal advection) and through omega (upward integration of the
+// thickness tendency and the facet-flux divergence), Q_fh and phi_strain; the
+// new rate factor depends on H through the pressure-melting temperature.
+//
+// Two passes, both gathers (no atomics, bitwise reproducible): one wavefront
+// per column reduces its levels' partials to per-column values (direct H
+// terms, the omega tendency adjoint, the adjoint of each of its four facet
+// fluxes, strain-rate and basal-speed weights) in kThermalVjpPlanes (ny, nx)
+// planes; then one thread per facet / cell gathers from the cells around it.
+template <typename T>
+__device__ __forceinline__ T depmp_dh(int k, int nz) {
+    return -T(kClausius) * (T(1) - T(k) * (T(1) / T(nz - 1))) * T(kCI / kEScale);
+}
+
+// Derivative of level k's rate-factor term A_k (1 + 181.25 w_k) with respect
+// to the column thickness (through the pressure-melting temperature).
+template <typename T>
+__device__ __forceinline__ T rate_term_dh(T Ek, T h, int k, int nz) {
+    const T sig = T(k) * (T(1) / T(nz - 1));
+    const T Tp = t_pmp(sig, h), dTp = -T(kClausius) * (T(1) - sig);
+    const T Tu = Ek * T(kEScale) / T(kCI) + T(kTRef);
+    const T Tk = fmin(Tu, Tp);
+    const T Af = Tk < T(263.15) ? T(3.985e-13) : T(1.916e3);
+    const T Qf = Tk < T(263.15) ? T(60e3) : T(139e3);
+    const T Ts = fmax(Tk, T(1));
+    const T Ak = Af * exp(-Qf / (T(8.314) * Ts));
+    const T dTs = (!(Tu < Tp) && Tk > T(1)) ? dTp : T(0);
+    const T wraw = Ek * T(kEScale) - T(kCI) * (Tp - T(kTRef));
+    const T wk = fmax(wraw, T(0)) / T(kLHeat);
+    const T dwk = wraw > T(0) ? -T(kCI) * dTp / T(kLHeat) : T(0);
+    return Ak * Qf / (T(8.314) * Ts * Ts) * dTs * (T(1) + T(181.25) * wk) + Ak * T(181.25) * dwk;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// d/dU of the Lax-Friedrichs flux 0.5 U (El + Er) - 0.5 |U| sq (Er - El).
+template <typename T>
+__device__ __forceinline__ T lf_flux_dU(T U, T El, T Er, T sq) {
+    const T sgn = U > T(0) ? T(1) : (U < T(0) ? T(-1) : T(0));
+    return T(0.5) * (El + Er) - T(0.5) * sgn * sq * (Er - El);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_thermal_vjp_columns(EnthArgs<T> A, ThermalVjpArgs<T> V) {
+    const long ncol = (long)A.ny * A.nx;
+    const ColIdx q = col_index(wave_column(), A.nx, ncol);
+    if (!q.valid) return;
+    const int nz = A.nz, k = q.k, ny = A.ny, nx = A.nx, i = q.i, j = q.j;
+    const bool in = k < nz;
+    const long base = q.col * nz, col = q.col;
+    const T h = A.H[col];
+    const bool thin = h < A.h_thin;
+    const bool live = in && !thin && k < nz - 1;
+    const T dsig = T(1) / T(nz - 1);
+    const T l = in ? V.lam[base + k] : T(0);
+    const T Ek = in ? A.E[base + k] : T(0);
+    const T omk = (in && A.omega) ? A.omega[base + k] : T(0);
+    const T Ekm1 = lane_get(Ek, max(k - 1, 0)), Ekp1 = lane_get(Ek, min(k + 1, 63));
+    const T omm = lane_get(omk, max(k - 1, 0)), omp = lane_get(omk, min(k + 1, nz - 1));
+    T Ku, Kl;
+    interface_K(k, nz, h, Ek, Ekp1, Ku, Kl);
+    // interface diffusivities' thickness derivatives (through E_pmp)
+    const int kp = min(k + 1, nz - 1), kc = min(k, nz - 1);
+    const T Ep_k = e_pmp(T(kc) * dsig, h), Ep_p = e_pmp(T(kp) * dsig, h);
+    const T dEp_k = depmp_dh<T>(kc, nz), dEp_p = depmp_dh<T>(kp, nz);
+    const T dKu = -dK_dE(T(0.5) * (Ek + Ekp1), T(0.5) * (Ep_k + Ep_p)) * T(0.5) * (dEp_k + dEp_p);
+    const T dKl = lane_get(dKu, max(k - 1, 0));
+    // lam_k times the row's partials: H, H_prev, omega_k / omega_k+1 /
+    // omega_k-1, and the four facet fluxes
+    T gh = T(0), ghp = T(0), cs = T(0), cu = T(0), cd = T(0), fl = T(0), fr = T(0), ft = T(0), fb = T(0);
+    if (live) {
+        const T hinv = T(1) / h;
+        T drdh = T(kRhoI) * Ek / A.dt;
+        if (k == 0) {
+            const T idd = T(1) / (dsig * (T(0.5) * dsig));
+            drdh += hinv * hinv * idd * Ku * (Ekp1 - Ek) - hinv * idd * dKu * (Ekp1 - Ek);
+        } else {
+            const T d2 = T(1) / (dsig * dsig);
+            drdh += hinv * hinv * d2 * (Ku * (Ekp1 - Ek) - Kl * (Ek - Ekm1)) -
+                    hinv * d2 * (dKu * (Ekp1 - Ek) - dKl * (Ek - Ekm1));
+        }
+        if (A.flags & 4) {
+            T wc, dwc;
+            water_and_slope(Ek, Ep_k, wc, dwc);
+            drdh += T(kRhoW) * T(kLHeat) * A.drain_rate * wc * T(1.0 / kEScale) -
+                    h * T(kRhoW) * A.drain_rate * dsoftplus_dx(Ek, Ep_k) * dEp_k;
+        }
+        if (A.phi_strain) drdh -= A.phi_strain[base + k] * T(1.0 / kEScale);
+        gh = -l * drdh;
+        ghp = l * T(kRhoI) * A.E_prev[base + k] / A.dt;
+        if (A.flags & 2) {
+            if (k == 0) {
+                const T hdi = T(1) / (T(0.5) * dsig);
+                const T D = (T(0.5) * (omk + omp) > T(0)) ? Ek : Ekp1;
+                cs = l * T(kRhoI) * hdi * (T(0.5) * D - Ek);
+                cu = l * T(kRhoI) * hdi * T(0.5) * D;
+            } else {
+                const T di = T(1) / dsig;
+                const T Du = (T(0.5) * (omk + omp) > T(0)) ? Ek : Ekp1;
+                const T Dl = (T(0.5) * (omm + omk) > T(0)) ? Ekm1 : Ek;
+                cs = l * T(kRhoI) * di * T(0.5) * (Du - Dl);
+                cu = l * T(kRhoI) * di * T(0.5) * Du;
+                cd = -l * T(kRhoI) * di * T(0.5) * Dl;
+            }
+        }
+        if (A.flags & 1) {
+            const Horiz<T> hz = load_horiz(A, i, j, k);
+            const T f = l * T(kRhoI) * A.dxi, sq = A.lf_sq;
+            fl = -f * lf_flux_dU(hz.Hu_l, hz.E_xm, Ek, sq);
+            fr = f * lf_flux_dU(hz.Hu_r, Ek, hz.E_xp, sq);
+            ft = f * lf_flux_dU(hz.Hv_t, Ek, hz.E_ym, sq);
+            fb = -f * lf_flux_dU(hz.Hv_b, hz.E_yp, Ek, sq);
+        }
+    }
+    // clamped thin column: G = E - min(E_surface, E_pmp(h))
+    if (in && thin && !(A.E_surface[col] < Ep_k)) gh = l * dEp_k;
+    // omega_m = -m dsig (dh/dt + div) / SEC_PER_YR (no basal melt): the
+    // tendency and the divergence share one adjoint
+    const T cu_below = lane_get(cu, max(k - 1, 0)), cd_above = lane_get(cd, min(k + 1, 63));
+    const T gom = -(cs + (k > 0 ? cu_below : T(0)) + cd_above);
+    const T gdh = (in && !(h < T(1e-3))) ? gom * (-T(k) * dsig / T(kSecPerYr)) : T(0);
+    T rv = T(0), rdh = T(0);
+    if (V.gB && in) {
+        rv = rate_term(Ek, h, k, nz).val;
+        rdh = rate_term_dh(Ek, h, k, nz);
+    }
+    const T Sgh = wave_sum(gh), Sghp = wave_sum(ghp), Sgdh = wave_sum(gdh);
+    const T Sfl = wave_sum(fl), Sfr = wave_sum(fr), Sft = wave_sum(ft), Sfb = wave_sum(fb);
+    const T Srv = wave_sum(rv), Srdh = wave_sum(rdh);
+    const T Sphi = wave_sum((V.B_in && V.g_phi && in) ? V.g_phi[base + k] : T(0));
+    if (k != 0) return;
+    const long P = ncol;
+    T* W = V.work;
+    T ghd = Sgh;
+    if (V.gB) {
+        const T mean = Srv / T(nz);
+        const T dBdmean = (T(-1) / V.n_glen) * pow(mean, T(-1) / V.n_glen - T(1)) / V.B_scale;
+        ghd += V.gB[col] * dBdmean / T(nz) * Srdh;
+    }
+    W[col] = ghd;
+    W[P + col] = Sgdh;
+    // facet adjoints in the units of the m/a mass flux the omega divergence
+    // uses; the advection's flux is that flux over SEC_PER_YR
+    const T vs = T(1) / T(kSecPerYr), dxi = A.dxi;
+    W[2 * P + col] = -Sfl * vs - Sgdh * dxi;
+    W[3 * P + col] = -Sfr * vs + Sgdh * dxi;
+    W[4 * P + col] = -Sft * vs + Sgdh * dxi;
+    W[5 * P + col] = -Sfb * vs - Sgdh * dxi;
+    if (V.g_H_prev) V.g_H_prev[col] = Sghp - Sgdh / V.dt_yr;
+    if (V.B_in) {  // phi = 4 eta eps_e^2 rho_i g / SEC_PER_YR, eta = B/2 (eps_e^2 + eps_reg)^p
+        const CellVel<T> c = load_cell_vel(A.u2d, A.v2d, i, j, ny, nx);
+        const T X = (c.ur - c.ul) * dxi, Y = (c.vt - c.vb) * dxi;
+        T stl = T(0), str = T(0), sbl = T(0), sbr = T(0);
+        if (i > 0 && j > 0) stl = T(0.5) * ((c.utl - c.ul) * dxi + (c.vt - c.vlt) * dxi);
+        if (i > 0 && j < nx - 1) str = T(0.5) * ((c.utr - c.ur) * dxi + (c.vrt - c.vt) * dxi);
+        if (i < ny - 1 && j > 0) sbl = T(0.5) * ((c.ul - c.ubl) * dxi + (c.vb - c.vlb) * dxi);
+        if (i < ny - 1 && j < nx - 1) sbr = T(0.5) * ((c.ur - c.ubr) * dxi + (c.vrb - c.vb) * dxi);
+        const T e2 = strain_invariant(c, i, j, ny, nx, dxi, T(0));
+        const T pw = pow(e2 + V.eps_reg, V.glen_p);
+        const T coef = T(4) * V.rho_g / T(kSecPerYr);
+        const T G = Sphi * coef * T(0.5) * V.B_in[col] * (pw + V.glen_p * e2 * pw / (e2 + V.eps_reg));
+        W[6 * P + col] = G * dxi * (T(2) * X + Y);
+        W[7 * P + col] = G * dxi * (T(2) * Y + X);
+        W[8 * P + col] = G * T(0.25) * dxi * stl;
+        W[9 * P + col] = G * T(0.25) * dxi * str;
+        W[10 * P + col] = G * T(0.25) * dxi * sbl;
+        W[11 * P + col] = G * T(0.25) * dxi * sbr;
+        if (V.g_B) V.g_B[col] = Sphi * coef * T(0.5) * pw * e2;
+    }
+    if (V.beta) {  // Q_fh = beta |u_b|^(m+1), u_b the cell-centred velocity in m/s
+        const T gq = V.g_Q[col], yr = T(kSecPerYr);
+        const T uc = T(0.5) * (A.u2d[(long)i * (nx + 1) + j + 1] / yr + A.u2d[(long)i * (nx + 1) + j] / yr);
+        const T vc = T(0.5) * (A.v2d[(long)(i + 1) * nx + j] / yr + A.v2d[(long)i * nx + j] / yr);
+        const T speed = sqrt(uc * uc + vc * vc + V.u_reg * V.u_reg);
+        if (V.g_beta) V.g_beta[col] = gq * pow(speed, V.m + T(1));
+        const T FQ = gq * V.beta[col] * (V.m + T(1)) * pow(speed, V.m - T(1));
+        W[12 * P + col] = FQ * uc * T(0.5) / yr;
+        W[13 * P + col] = FQ * vc * T(0.5) / yr;
+    }
+}
+
+// One thread per facet / cell over the (ny + 1, nx + 1) corners: x facet
+// (i, j) for i < ny, y facet (i, j) for j < nx, cell (i, j) for both.
+template <typename T>
+__global__ __launch_bounds__(256) void k_thermal_vjp_gather(EnthArgs<T> A, ThermalVjpArgs<T> V, int strain,
+                                                            int frictional) {
+    const int ny = A.ny, nx = A.nx;
+    const long P = (long)ny * nx, n = (long)(ny + 1) * (nx + 1);
+    const T* W = V.work;
+    const T reg = T(kMassFluxRegYr);
+    auto w = [&](int p, int ii, int jj) -> T {
+        return (ii >= 0 && ii < ny && jj >= 0 && jj < nx) ? W[p * P + (long)ii * nx + jj] : T(0);
+    };
+    auto Hc = [&](int ii, int jj) { return A.H[(long)ii * nx + jj]; };
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t / (nx + 1)), j = (int)(t % (nx + 1));
+        if (i < ny) {  // x facet between cells (i, j - 1) and (i, j)
+            T g = T(0);
+            if (j > 0 && j < nx) {
+                const T U = A.u2d[(long)i * (nx + 1) + j], s = sqrt(U * U + reg);
+                const T HA = Hc(i, j - 1), HB = Hc(i, j);
+                g += (w(3, i, j - 1) + w(2, i, j)) * (T(0.5) * (HA + HB) - T(0.5) * U / s * (HB - HA));
+            }
+            if (strain)
+                g += -w(6, i, j) - w(8, i, j) + w(10, i, j) + w(6, i, j - 1) - w(9, i, j - 1) + w(11, i, j - 1) +
+                     w(8, i + 1, j) + w(9, i + 1, j - 1) - w(10, i - 1, j) - w(11, i - 1, j - 1);
+            if (frictional) g += w(12, i, j - 1) + w(12, i, j);
+            V.g_u[(long)i * (nx + 1) + j] = g;
+        }
+        if (j < nx) {  // y facet between cells (i - 1, j) and (i, j)
+            T g = T(0);
+            if (i > 0 && i < ny) {
+                const T Vv = A.v2d[(long)i * nx + j], s = sqrt(Vv * Vv + reg);
+                const T HA = Hc(i - 1, j), HB = Hc(i, j);
+                g += (w(5, i - 1, j) + w(4, i, j)) * (T(0.5) * (HA + HB) - T(0.5) * Vv / s * (HA - HB));
+            }
+            if (strain)
+                g += w(7, i, j) + w(8, i, j) - w(9, i, j) - w(7, i - 1, j) + w(10, i - 1, j) - w(11, i - 1, j) -
+                     w(8, i, j + 1) + w(9, i, j - 1) - w(10, i - 1, j + 1) + w(11, i - 1, j - 1);
+            if (frictional) g += w(13, i - 1, j) + w(13, i, j);
+            V.g_v[(long)i * nx + j] = g;
+        }
+        if (i < ny && j < nx) {  // the cell's thickness: direct terms, tendency, its four facet fluxes
+            T g = w(0, i, j) + w(1, i, j) / V.dt_yr;
+            if (j > 0) {
+                const T U = A.u2d[(long)i * (nx + 1) + j], s = sqrt(U * U + reg);
+                g += (w(3, i, j - 1) + w(2, i, j)) * (T(0.5) * U - T(0.5) * s);
+            }
+            if (j < nx - 1) {
+                const T U = A.u2d[(long)i * (nx + 1) + j + 1], s = sqrt(U * U + reg);
+                g += (w(3, i, j) + w(2, i, j + 1)) * (T(0.5) * U + T(0.5) * s);
+            }
+            if (i > 0) {
+                const T Vv = A.v2d[(long)i * nx + j], s = sqrt(Vv * Vv + reg);
+                g += (w(5, i - 1, j) + w(4, i, j)) * (T(0.5) * Vv + T(0.5) * s);
+            }
+            if (i < ny - 1) {
+                const T Vv = A.v2d[(long)(i + 1) * nx + j], s = sqrt(Vv * Vv + reg);
+                g += (w(5, i, j) + w(4, i + 1, j)) * (T(0.5) * Vv - T(0.5) * s);
+            }
+            V.g_H[(long)i * nx + j] = g;
+        }
+    }
+}
+
 // The rows' exponential and logarithm over an array (for tests of their
 // accuracy): e_out = exp_rows(x), l_out = log_rows(x) where x > 0 (else NaN).
 __global__ __launch_bounds__(256) void k_enth_elementary(const double* __restrict__ x, long n,
@@ -1813,6 +2069,17 @@ hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_thermal_vjp(const EnthArgs<T>& A, const ThermalVjpArgs<T>& V, hipStream_t st) {
+    if (A.nz < 2 || A.nz > 64 || !A.u2d || !A.v2d) return hipErrorInvalidValue;
+    k_thermal_vjp_columns<T><<<col_blocks(A.ny, A.nx), 256, 0, st>>>(A, V);
+    const long n = (long)(A.ny + 1) * (A.nx + 1);
+    const long nb = (n + 255) / 256;
+    const unsigned g = (unsigned)(nb < 65535 ? nb : 65535);
+    k_thermal_vjp_gather<T><<<g, 256, 0, st>>>(A, V, V.B_in ? 1 : 0, V.beta ? 1 : 0);
+    return hipGetLastError();
+}
+
 // Level-major sweep launchers.
 template <typename T>
 hipError_t launch_enth_lm_layout(const T* src, T* dst, long ncol, int nz, bool to_lm, hipStream_t st) {
@@ -1895,6 +2162,7 @@ hipError_t launch_enth_residual_lm(const EnthArgs<T>& A, T* r_out, double* parti
                                                     const T*, unsigned long long*, EnthStatus*, int, double, double, \
                                                     hipStream_t);                                                    \
     template hipError_t launch_enth_adj_grads<T>(const EnthArgs<T>&, const T*, T*, T*, T*, T*, hipStream_t);      \
+    template hipError_t launch_thermal_vjp<T>(const EnthArgs<T>&, const ThermalVjpArgs<T>&, hipStream_t);           \
     template hipError_t launch_enth_lm_layout<T>(const T*, T*, long, int, bool, hipStream_t);                        \
     template hipError_t launch_enth_column_lm<T>(const EnthArgs<T>&, T*, T, int, T, unsigned long long*,            \
                                                  hipStream_t);                                                       \

@@ -130,6 +130,26 @@ template <typename T>
 hipError_t launch_enth_adj_grads(const EnthArgs<T>& A, const T* lam, T* g_E_prev, T* g_phi, T* g_Q, T* g_Es,
                                  hipStream_t st);
 
+// Gradients of the thermal step with respect to the inputs the momentum step
+// hands it (velocities, thicknesses, sliding coefficient, the rate factor the
+// strain heating read), from the adjoint state of the converged solve.
+constexpr int kThermalVjpPlanes = 14;  // (ny, nx) work planes
+template <typename T>
+struct ThermalVjpArgs {
+    const T* lam;    // (ny, nx, nz) adjoint state
+    const T* g_phi;  // (ny, nx, nz) dL/dphi_strain, read when B_in is set
+    const T* g_Q;    // (ny, nx) dL/dQ_fh, read when beta is set
+    const T* gB;     // (ny, nx) seed of the new rate factor, or null
+    const T* B_in;   // rate factor the strain heating read, or null (no strain heating)
+    const T* beta;   // sliding coefficient, or null (no frictional heating)
+    T n_glen, B_scale, glen_p, eps_reg, rho_g, m, u_reg, dt_yr;
+    T* work;         // kThermalVjpPlanes planes of (ny, nx)
+    T *g_u, *g_v, *g_H, *g_H_prev, *g_beta, *g_B;
+};
+// A.u2d / A.v2d: the step's velocities in m/a.
+template <typename T>
+hipError_t launch_thermal_vjp(const EnthArgs<T>& A, const ThermalVjpArgs<T>& V, hipStream_t st);
+
 // exp and log as the enthalpy rows compute them, over n values (for tests).
 hipError_t launch_enth_elementary(const double* x, long n, double* e_out, double* l_out, hipStream_t st);
 

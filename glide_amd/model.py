@@ -154,12 +154,7 @@ class ThermalModel:
                 raise ValueError("B_out must match grid.rheology.B in shape, dtype and device")
             B_out.copy_(B)  # strain heating reads the current rate factor
             B = B_out
-        n = float(grid.rheology.n.value)
-        tc = _lib.ThermalConfigC(dt=float(dt), m=float(grid.sliding.m.value), u_reg=float(grid.sliding.u_reg.value),
-                                 rho_i=float(self.rho_i), n_glen=n, eps_reg=float(grid.rheology.eps_reg.value),
-                                 n_smooth=int(self.n_smooth), frictional_heating=1 if self.frictional_heating else 0,
-                                 update_rheology=1 if self.update_rheology else 0,
-                                 strain_heating=1 if self.strain_heating else 0)
+        tc = self._config(dt)
         want = report or ops.smoother_config.report_norms
         rep = _lib.EnthReport() if want else None
         C = ops._column()
@@ -170,6 +165,91 @@ class ThermalModel:
              ctypes.byref(rep) if rep is not None else None, ops._stream())
         self.last_report = rep
         return rep
+
+    def _config(self, dt, n_smooth=None, update_rheology=None):
+        from . import _lib
+        grid = self.ops.grid
+        upd = self.update_rheology if update_rheology is None else update_rheology
+        return _lib.ThermalConfigC(dt=float(dt), m=float(grid.sliding.m.value), u_reg=float(grid.sliding.u_reg.value),
+                                   rho_i=float(self.rho_i), n_glen=float(grid.rheology.n.value),
+                                   eps_reg=float(grid.rheology.eps_reg.value),
+                                   n_smooth=int(self.n_smooth if n_smooth is None else n_smooth),
+                                   frictional_heating=1 if self.frictional_heating else 0,
+                                   update_rheology=1 if upd else 0, strain_heating=1 if self.strain_heating else 0)
+
+    def rebuild_step(self, dt, E):
+        """Recompute the fields ``step(dt)`` derives from its inputs (velocity
+        planes, thickness tendency, omega, Q_fh, phi_strain, f_E) from the
+        grid's current u, v, H, beta and B and the snapshot E_prev / H_prev,
+        without sweeping and without touching B, and set the enthalpy to E (the
+        converged state of that step): after other steps have overwritten the
+        operators' buffers, ``adjoint`` then sees the step again exactly (no
+        counterpart in the reference)."""
+        import ctypes
+
+        from . import _lib
+        from ._lib import call, ptr
+        ops = self.ops
+        grid = ops.grid
+        B = grid.rheology.B.data if self.strain_heating else None
+        C = ops._column()
+        call("glide_thermal_step", ctypes.byref(C), ptr(grid.state.u.data), ptr(grid.state.v.data),
+             ptr(grid.sliding.beta.data), ptr(B), ctypes.byref(self._config(dt, n_smooth=0, update_rheology=False)),
+             ctypes.byref(ops.smoother_config.to_c(True)), None, ops._stream())
+        ops.enthalpy_state.E.copy_(E)
+
+    def adjoint(self, dt, dL_dE, dL_dB=None, B_in=None, n_iter=200, relative_tolerance=1e-12,
+                absolute_tolerance=0.0, report=False):
+        """Adjoint of the last ``step(dt)`` (no counterpart in the reference,
+        whose enthalpy solve has no adjoint): solves the transposed system of
+        the converged solve for dL/dE and, when ``update_rheology``, dL/dB of
+        the new rate factor (head units), and returns the gradients
+        ``{"lambda", "E_prev", "phi_strain", "Q_geo", "Q_fh", "E_surface", "u",
+        "v", "H", "H_prev", "beta", "B"}``: u (ny, nx+1) and v (ny+1, nx) are
+        the step's SSA velocities (m/a), H / H_prev the new / pre-momentum
+        thickness, beta the sliding coefficient (through the frictional
+        heating; None without it) and B the rate factor the strain heating
+        read (None without strain heating).  Every input's path counts: the
+        time, diffusion, drainage and forcing terms, the facet mass fluxes of
+        the horizontal advection, omega, Q_fh and phi_strain, and the new rate
+        factor's dependence on H.  The grid and the operators must hold the
+        step's state (``step`` just ran, or ``rebuild_step``); ``B_in`` is the
+        rate factor that step read, needed with strain heating when the step
+        overwrote ``grid.rheology.B``."""
+        import ctypes
+
+        from . import _lib
+        from ._lib import call, ptr
+        ops = self.ops
+        grid = ops.grid
+        ny, nx = grid.ny, grid.nx
+        if self.strain_heating and B_in is None:
+            if self.update_rheology:
+                raise ValueError("B_in (the rate factor the step read) is needed with strain heating when the step "
+                                 "updated grid.rheology.B")
+            B_in = grid.rheology.B.data
+        gB = dL_dB if (dL_dB is not None and self.update_rheology) else None
+        out = ops.adjoint_solve(dt, dL_dE, gB, B_scale=self.B_scale, n_iter=n_iter,
+                                relative_tolerance=relative_tolerance, absolute_tolerance=absolute_tolerance,
+                                report=report)
+        z = grid.zeros
+        n = int(_lib.load().glide_thermal_vjp_work_doubles(ny, nx))
+        if getattr(self, "_vjp_work", None) is None or self._vjp_work.numel() != n:
+            self._vjp_work = z((n,))
+        g = {"u": z((ny, nx + 1)), "v": z((ny + 1, nx)), "H": z((ny, nx)), "H_prev": z((ny, nx)),
+             "beta": z((ny, nx)) if self.frictional_heating else None,
+             "B": z((ny, nx)) if self.strain_heating else None}
+        gBt = None if gB is None else torch.as_tensor(gB, dtype=torch.float64, device=grid.device).expand(
+            ny, nx).contiguous()
+        V = _lib.ThermalVjpC(lambda_=ptr(out["lambda"]), g_phi_strain=ptr(out["phi_strain"]), g_Q=ptr(out["Q_geo"]),
+                             dL_dB=ptr(gBt), work=ptr(self._vjp_work), g_u=ptr(g["u"]), g_v=ptr(g["v"]),
+                             g_H=ptr(g["H"]), g_H_prev=ptr(g["H_prev"]), g_beta=ptr(g["beta"]), g_B=ptr(g["B"]))
+        C = ops._column()
+        call("glide_thermal_step_vjp", ctypes.byref(C), ptr(grid.state.u.data), ptr(grid.state.v.data),
+             ptr(grid.sliding.beta.data), ptr(B_in), ctypes.byref(self._config(dt)), ctypes.byref(V),
+             ops._stream())
+        out.update(g)
+        return out
 
     def _step_host(self, dt):
         ops = self.ops

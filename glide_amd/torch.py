@@ -57,36 +57,66 @@ class ThermalStep(torch.autograd.Function):
     """torch.autograd surface of one thermal step (no counterpart in the
     reference, whose enthalpy solve has no adjoint).
 
-    ``ThermalStep.apply(thermal, dt, E_prev, Q_geo, E_surface) -> (E, B)`` sets
-    the previous enthalpy (also the initial iterate), the geothermal flux and
-    the surface enthalpy of ``thermal`` (a ``ThermalModel`` after its momentum
-    step and ``pre_momentum`` snapshot of H), runs ``thermal.step(dt)`` and
-    returns the new enthalpy and rate factor (head units; zeros when
-    ``update_rheology`` is off).  The backward solves the adjoint of the
-    converged solve at the returned state, so it is exact when the forward
-    sweeps converge (tighten ``thermal.ops.smoother_config`` tolerances and
-    ``thermal.n_smooth`` for gradient work); the thermal model must not step
-    again between the two passes."""
+    ``ThermalStep.apply(thermal, dt, E_prev, Q_geo, E_surface[, u, v, H,
+    H_prev, beta, B]) -> (E, B_new)`` sets the previous enthalpy (also the
+    initial iterate), the geothermal flux and the surface enthalpy of
+    ``thermal`` (a ``ThermalModel``), and, where given, the SSA velocities u
+    (ny, nx+1) and v (ny+1, nx) in m/a, the new thickness H, the pre-momentum
+    thickness H_prev, the sliding coefficient beta and the rate factor B (head
+    units) the step reads; the others keep the model's current values (after
+    ``pre_momentum`` and the momentum step).  It runs ``thermal.step(dt)`` and
+    returns the new enthalpy and rate factor (zeros when ``update_rheology``
+    is off).  The backward restores the step's inputs and converged enthalpy
+    into the model (``rebuild_step``, so other steps may run in between) and
+    solves the adjoint of the converged solve, so it is exact when the
+    forward sweeps converge (tighten ``thermal.ops.smoother_config``
+    tolerances and ``thermal.n_smooth`` for gradient work); it returns the
+    gradients of every tensor input, the momentum step's outputs u, v, H
+    included, which closes the adjoint of the coupled step with ``GlideStep``.
+    It leaves the model's grid holding the step's inputs."""
 
     @staticmethod
-    def forward(ctx, thermal, dt, E_prev, Q_geo, E_surface):
+    def forward(ctx, thermal, dt, E_prev, Q_geo, E_surface, u=None, v=None, H=None, H_prev=None, beta=None, B=None):
         ops = thermal.ops
+        grid = ops.grid
         ops.enthalpy_state.E_prev.copy_(E_prev.detach())
         ops.enthalpy_state.E.copy_(E_prev.detach())
         ops.enthalpy_forcing.Q_geo.copy_(Q_geo.detach())
         ops.enthalpy_forcing.E_surface.copy_(E_surface.detach())
+        for given, dst in ((u, grid.state.u.data), (v, grid.state.v.data), (H, grid.state.H.data),
+                           (H_prev, ops.H_prev), (beta, grid.sliding.beta.data), (B, grid.rheology.B.data)):
+            if given is not None:
+                dst.copy_(given.detach())
+        inputs = tuple(t.clone() for t in (grid.state.u.data, grid.state.v.data, grid.state.H.data, ops.H_prev,
+                                            grid.sliding.beta.data, grid.rheology.B.data))
         thermal.step(dt)
         E = ops.enthalpy_state.E.clone()
-        B = ops.grid.rheology.B.data.clone() if thermal.update_rheology else torch.zeros_like(Q_geo)
+        B_new = grid.rheology.B.data.clone() if thermal.update_rheology else torch.zeros_like(Q_geo)
         ctx.thermal, ctx.dt = thermal, dt
-        return E, B
+        ctx.save_for_backward(E_prev, Q_geo, E_surface, E, *inputs)
+        return E, B_new
 
     @staticmethod
     def backward(ctx, gE, gB):
         thermal = ctx.thermal
         ops = thermal.ops
+        grid = ops.grid
+        E_prev, Q_geo, E_surface, E, u, v, H, H_prev, beta, B_in = ctx.saved_tensors
+        ops.enthalpy_state.E_prev.copy_(E_prev.detach())
+        ops.enthalpy_forcing.Q_geo.copy_(Q_geo.detach())
+        ops.enthalpy_forcing.E_surface.copy_(E_surface.detach())
+        for src, dst in ((u, grid.state.u.data), (v, grid.state.v.data), (H, grid.state.H.data), (H_prev, ops.H_prev),
+                         (beta, grid.sliding.beta.data), (B_in, grid.rheology.B.data)):
+            dst.copy_(src)
+        thermal.rebuild_step(ctx.dt, E)
         if gE is None:
-            gE = torch.zeros_like(ops.enthalpy_state.E)
+            gE = torch.zeros_like(E)
         use_B = gB is not None and thermal.update_rheology
-        g = ops.adjoint_solve(ctx.dt, gE.contiguous(), gB.contiguous() if use_B else None, B_scale=thermal.B_scale)
-        return None, None, g["E_prev"], g["Q_geo"], g["E_surface"]
+        g = thermal.adjoint(ctx.dt, gE.contiguous(), gB.contiguous() if use_B else None, B_in=B_in)
+        need = ctx.needs_input_grad  # one entry per argument given
+
+        def pick(idx, key):
+            return g[key] if idx < len(need) and need[idx] and g.get(key) is not None else None
+        grads = (None, None, g["E_prev"], g["Q_geo"], g["E_surface"], pick(5, "u"), pick(6, "v"), pick(7, "H"),
+                 pick(8, "H_prev"), pick(9, "beta"), pick(10, "B"))
+        return grads[:len(need)]

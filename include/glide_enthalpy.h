@@ -224,6 +224,39 @@ int glide_thermal_step(const glide_enth_column* C, const void* u2d, const void* 
                        const glide_thermal_config* tc, const glide_enth_sweep_config* sc, glide_enth_report* report,
                        void* stream);
 
+/* Gradients of the converged thermal step with respect to the inputs the
+ * momentum step hands it (no counterpart in the reference, whose enthalpy
+ * solve has no adjoint; with the momentum adjoint it closes the adjoint of
+ * the coupled step of glide/model.py:156-213): dL/dp = -lambda^T dG/dp for
+ * the velocities u2d (ny, nx+1) and v2d (ny+1, nx) in m/a, the new and
+ * previous thicknesses H and H_prev, the sliding coefficient beta (through the
+ * frictional heating) and the rate factor B the strain heating read, with
+ * lambda and the phi_strain / Q_fh gradients from glide_enth_adjoint_solve at
+ * the same state.  Every input reaches G directly (time term, 1/H and the
+ * pressure-melting enthalpy in the diffusion and drainage, H phi_strain
+ * forcing, facet mass fluxes of the horizontal advection) and through omega
+ * (thickness tendency and flux divergence), Q_fh and phi_strain; dL_dB (the
+ * seed of the new rate factor, when update_rheology) adds its dependence on
+ * H through the pressure-melting temperature.  Buffers: */
+typedef struct glide_thermal_vjp {
+    const void* lambda;       /* (ny, nx, nz) adjoint state */
+    const void* g_phi_strain; /* (ny, nx, nz) from the same adjoint solve; read when tc->strain_heating */
+    const void* g_Q;          /* (ny, nx) g_Q_geo of the same solve (= dL/dQ_fh); read when tc->frictional_heating */
+    const void* dL_dB;        /* (ny, nx) the solve's rate-factor seed, or NULL */
+    void* work;               /* glide_thermal_vjp_work_doubles(ny, nx) doubles */
+    void *g_u, *g_v, *g_H;    /* outputs (ny, nx+1), (ny+1, nx), (ny, nx) */
+    void *g_H_prev;           /* output (ny, nx), may be NULL */
+    void *g_beta;             /* output (ny, nx) when tc->frictional_heating, may be NULL */
+    void *g_B;                /* output (ny, nx) when tc->strain_heating, may be NULL */
+} glide_thermal_vjp;
+/* Work doubles glide_thermal_step_vjp needs at (ny, nx), or -1 for invalid sizes. */
+int64_t glide_thermal_vjp_work_doubles(int ny, int nx);
+/* C holds the converged state of glide_thermal_step (E, E_prev, H, H_prev,
+ * omega, phi_strain) with the velocities u2d, v2d, beta and the rate factor B
+ * that step read (before its update), and tc its configuration. */
+int glide_thermal_step_vjp(const glide_enth_column* C, const void* u2d, const void* v2d, const void* beta,
+                           const void* B, const glide_thermal_config* tc, const glide_thermal_vjp* V, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

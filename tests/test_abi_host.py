@@ -56,6 +56,11 @@ def test_struct_layouts_match_header():
         f.rstrip("_") for f in _lib.ENTH_ADJOINT_POINTERS)
     assert "double vel_div" in body
     assert f"GLIDE_ENTH_MAX_SWEEPS {_lib.ENTH_MAX_SWEEPS}" in eh
+    body = eh[eh.index("typedef struct glide_thermal_vjp"):eh.index("} glide_thermal_vjp;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    assert tuple(re.findall(r"\*\s*(\w+)\s*[;,]", body)) == tuple(
+        f.rstrip("_") for f in _lib.THERMAL_VJP_POINTERS)
+    assert ctypes.sizeof(_lib.ThermalVjpC) == 8 * len(_lib.THERMAL_VJP_POINTERS)
 
 
 def test_version_and_error_text():
@@ -125,6 +130,15 @@ def test_enthalpy_argument_checks_on_host():
     assert lib.glide_enth_column_sweep(ctypes.byref(C), 1.0, 257, ctypes.byref(cfg), None, None) == -1
     assert "n_iter" in _lib.last_error()
     assert lib.glide_thermal_step(ctypes.byref(C), None, None, None, None, None, None, None, None) == -1
+    assert lib.glide_thermal_vjp_work_doubles(0, 4) == -1
+    assert lib.glide_thermal_vjp_work_doubles(3, 5) == 14 * 15
+    tc = _lib.ThermalConfigC(dt=1.0, m=1.0, u_reg=1.0, rho_i=917.0, n_glen=3.0, eps_reg=1e-6, n_smooth=10,
+                             frictional_heating=1, update_rheology=1, strain_heating=1)
+    V = _lib.ThermalVjpC()
+    assert lib.glide_thermal_step_vjp(ctypes.byref(C), None, None, None, None, ctypes.byref(tc), None, None) == -1
+    assert "NULL" in _lib.last_error()
+    assert lib.glide_thermal_step_vjp(ctypes.byref(C), None, None, None, None, ctypes.byref(tc), ctypes.byref(V),
+                                      None) == -1
 
 
 def test_enthalpy_host_objects_on_cpu():
