@@ -135,4 +135,63 @@ def rollout(model, H0, bed, beta, smb, n_steps: int, dt: float, step_loss: Calla
     return total, (u, v, H, mask)
 
 
-__all__ = ["GlideStepWarm", "rollout", "misfit", "gradient_penalty"]
+def coupled_rollout(model, thermal, H0, E0, bed, beta, smb, Q_geo, E_surface, n_steps: int, dt: float,
+                    step_loss: Callable, t0: float = 0.0, level: int = 0, checkpoint: Optional[int] = 8, u0=None,
+                    v0=None, mask0=None, B0=None):
+    """The coupled step of glide/model.py:156-213 over n_steps, differentiable
+    end to end: each step solves the momentum balance with the current rate
+    factor B (GlideStepWarm, H_prev the previous step's H), then the thermal
+    step from that step's velocities and thickness (ThermalStep, E_prev the
+    previous step's enthalpy), whose new rate factor (when
+    ``thermal.update_rheology``) is the next momentum step's B.  ``thermal``
+    is a ThermalModel on the grid of ``model.mg[level]``; dt is in years (the
+    thermal step gets it in seconds).  Returns
+    (sum over k of step_loss(k, u_k, v_k, H_k, E_k, B_k), (u, v, H, mask, E,
+    B) after the last step); the loss is differentiable with respect to H0,
+    E0, bed, beta, smb, Q_geo, E_surface and B0 (the first step's rate
+    factor; when None, the model's current one without a gradient, which
+    after a rollout is its last thermal step's, so repeated evaluations pass
+    B0).  The
+    gradient carries every coupling: the rate factor through the momentum
+    solve, the velocities, thickness and sliding coefficient through the
+    enthalpy (advection, omega, frictional and strain heating) and back.
+    Checkpointing as in :func:`rollout` (the thermal step's backward rebuilds
+    its state from its saved inputs, so recomputed segments and interleaved
+    steps are exact)."""
+    from .coupled import SEC_PER_YR
+    from .torch import ThermalStep
+    g = model.mg[level]
+    if thermal.ops.grid is not g:
+        raise ValueError("thermal must be a ThermalModel on the grid of model.mg[level]")
+    z = torch.zeros_like
+    u = z(g.state.u.data) if u0 is None else u0
+    v = z(g.state.v.data) if v0 is None else v0
+    mask = z(g.state.mask.data) if mask0 is None else mask0
+    B = g.rheology.B.data.clone() if B0 is None else B0
+
+    def segment(k0, k1, H, E, u, v, mask, B, bed, beta, smb, Q_geo, E_surface):
+        loss = H.new_zeros(())
+        for k in range(k0, k1):
+            H_prev = H
+            u, v, H, mask = GlideStepWarm.apply(t0 + k * dt, dt, model, level, H_prev, bed, beta, smb, u, v, mask, B)
+            E, B_new = ThermalStep.apply(thermal, dt * SEC_PER_YR, E, Q_geo, E_surface, u, v, H, H_prev, beta, B)
+            if thermal.update_rheology:
+                B = B_new
+            loss = loss + step_loss(k, u, v, H, E, B)
+        return loss, H, E, u, v, mask, B
+
+    total = H0.new_zeros(())
+    H, E = H0, E0
+    c = int(checkpoint) if checkpoint else n_steps
+    for k0 in range(0, n_steps, c):
+        k1 = min(k0 + c, n_steps)
+        args = (k0, k1, H, E, u, v, mask, B, bed, beta, smb, Q_geo, E_surface)
+        if checkpoint:
+            loss, H, E, u, v, mask, B = _checkpoint(segment, *args, use_reentrant=False)
+        else:
+            loss, H, E, u, v, mask, B = segment(*args)
+        total = total + loss
+    return total, (u, v, H, mask, E, B)
+
+
+__all__ = ["GlideStepWarm", "rollout", "coupled_rollout", "misfit", "gradient_penalty"]
