@@ -8,10 +8,14 @@ the enthalpy solve on its 512 x 512 columns with 41 sigma levels (surface
 temperature -30 C + 6.5e-3 K/m (2000 m - s), geothermal flux 0.05 W/m^2,
 strain and frictional heating, rate factor fed back to the momentum solver).
 
-One step: the forward FAS-CD solve (advancing the geometry), the thermal step,
-the enthalpy adjoint, the momentum adjoint for the loss
-0.5 |u|^2 + 0.5 |v|^2 and the parameter gradients.  Inputs live in HBM before
-timing starts.  The two forms:
+One step: the forward FAS-CD solve (advancing the geometry), the thermal step
+and the adjoint of the converged coupled step for the loss
+0.5 |u|^2 + 0.5 |v|^2 + 0.5 |E|^2 + sum(B_new): the enthalpy adjoint, the
+thermal step's gradients with respect to u, v, H, H_prev and beta
+(glide_thermal_step_vjp), the momentum adjoint seeded with the velocity terms
+plus those, and the parameter gradients (the basal-drag gradient includes the
+frictional-heating path).  Inputs live in HBM before timing starts.  The two
+forms:
 
 * fp64  - the reference's algorithm in FP64 (additive Vanka smoother with its
           default counts) for the forward and the adjoint solves;
@@ -30,7 +34,8 @@ values of a ladder for which steps from the warmed-up trajectory land within
 --target / 2 (relative L2) of the truth on two steps (the last warm-up step
 and the next one): the same step with FP64 solves driven to
 the residual floor (forward: FP64 red-black V-cycles to 3e-13; adjoint: FP64
-reference-form V-cycles to 1e-14; enthalpy sweeps to 1e-13).  After
+reference-form V-cycles to 1e-14; enthalpy sweeps to 1e-13; enthalpy adjoint
+sweeps to 1e-14, at most 250).  After
 timing, every timed step is replayed from the saved state (the solves are
 deterministic) and its u, v, H, E, multipliers and basal-drag gradient are
 compared with the truth of that step; the line reports the largest errors.
@@ -159,8 +164,10 @@ class Form:
         else:
             self.set_tolerances(1e-8, 1e-6, 1e-8)
         from glide_amd.coupled import CoupledStep
-        self.stepper = CoupledStep(self.model, self.thermal, enthalpy_adjoint_rtol=1e-10,
-                                   forward=(lambda t, dt: self._truth_forward(dt)) if kind == "truth" else None)
+        truth = kind == "truth"
+        self.stepper = CoupledStep(self.model, self.thermal, enthalpy_adjoint_rtol=TRUTH_ADJOINT if truth else 1e-10,
+                                   enthalpy_adjoint_iters=250 if truth else 64,
+                                   forward=(lambda t, dt: self._truth_forward(dt)) if truth else None)
 
     def set_tolerances(self, fwd, enth, adj):
         self.tol = {"forward": fwd, "enthalpy": enth, "adjoint": adj}
@@ -195,9 +202,9 @@ class Form:
         top.state.H_prev.data.copy_(top.state.H.data)
 
     def step(self, t, dt):
-        """One coupled step (glide_amd.coupled.CoupledStep: the enthalpy step
-        and its adjoint, seeded with dL/dE = E and dL/dB = 1, on a second
-        stream beside the momentum adjoint)."""
+        """One coupled step and its adjoint (glide_amd.coupled.CoupledStep:
+        the enthalpy adjoint seeded with dL/dE = E and dL/dB = 1, then the
+        thermal step's input gradients into the momentum adjoint's seeds)."""
         self.stepper(t, dt)
 
     def tensors(self):
@@ -771,7 +778,9 @@ def main():
             "kernels": kernels_m,
             "truth": (f"FP64 solves to the residual floor from the same state: forward FP64 red-black V-cycles to "
                       f"{TRUTH_FORWARD:g}, adjoint FP64 reference-form V-cycles to {TRUTH_ADJOINT:g}, enthalpy "
-                      f"sweeps to {TRUTH_ENTHALPY:g} (relative residuals)")}
+                      f"sweeps to {TRUTH_ENTHALPY:g}, enthalpy adjoint sweeps to {TRUTH_ADJOINT:g} (relative "
+                      f"residuals); the adjoint is the coupled step's (enthalpy adjoint, thermal input gradients "
+                      f"into the momentum adjoint's seeds)")}
     print(json.dumps(line), flush=True)
 
 

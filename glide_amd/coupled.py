@@ -3,14 +3,20 @@
 The reference's drivers call the pieces in turn (ThermalModel.pre_momentum,
 IceDynamics.forward, ThermalModel.step, then IceDynamics.backward:
 glide/model.py:65-94, 156-213; examples/greenland/greenland_inverse.py:119-168).
-:class:`CoupledStep` runs the same sequence with the enthalpy step and its
-adjoint on a second stream: both depend only on the forward solution, and the
-momentum adjoint depends only on the forward state and the rate factor the
-forward used, so the two overlap.  The new rate factor goes to a buffer of its
-own and is written to the grid after both, so the momentum adjoint sees the
-rate factor of its forward solve.  The step runs on a stream of its own rather
-than the null stream, whose implicit synchronisation with blocking streams
-would serialise the two.  Every stream keeps the default priority.
+:class:`CoupledStep` runs the same sequence and, by default, the adjoint of
+the converged coupled step of a loss of the step's velocities, thickness,
+enthalpy and new rate factor: the enthalpy adjoint first, then the thermal
+step's gradients with respect to its inputs (ThermalModel.adjoint), whose
+velocity and thickness parts join the momentum adjoint's seeds and whose
+sliding-coefficient and previous-thickness parts join the momentum
+gradients.  With ``coupled_adjoint=False`` the enthalpy adjoint is solved on
+its own seeds and, with ``concurrent``, on a second stream beside the
+momentum adjoint (both then depend only on the forward solution).  The new
+rate factor goes to a buffer of its own and is written to the grid at the end,
+so the momentum adjoint sees the rate factor of its forward solve.  The step
+runs on a stream of its own rather than the null stream, whose implicit
+synchronisation with blocking streams would serialise the two.  Every stream
+keeps the default priority.
 """
 from __future__ import annotations
 
@@ -35,14 +41,21 @@ class CoupledStep:
     dJ/dv, dJ/dH (seeds=None: no momentum adjoint); thermal_seeds(thermal,
     grid) the enthalpy adjoint's dL/dE and dL/dB (default E and ones; a
     callable returning None skips the enthalpy adjoint).  forward, when
-    given, replaces model.forward(t, dt, update_geometry=True).  concurrent
-    puts the enthalpy work on the second stream."""
+    given, replaces model.forward(t, dt, update_geometry=True).
+    coupled_adjoint (with both adjoints): the momentum adjoint's seeds and
+    gradients take the enthalpy's dependence on u, v, H, H_prev and beta, so
+    the multipliers and gradients are those of the coupled step's loss.
+    concurrent (decoupled adjoints only) puts the enthalpy work on the second
+    stream."""
 
     def __init__(self, model, thermal=None, seeds: Callable = velocity_loss_seeds,
                  thermal_seeds: Optional[Callable] = None, enthalpy_adjoint_rtol: float = 1e-10,
-                 enthalpy_adjoint_iters: int = 64, forward: Optional[Callable] = None, concurrent: bool = True):
+                 enthalpy_adjoint_iters: int = 64, forward: Optional[Callable] = None, concurrent: bool = True,
+                 coupled_adjoint: bool = True):
         self.model, self.thermal = model, thermal
         self.seeds = seeds
+        self.coupled_adjoint = bool(coupled_adjoint)
+        self.thermal_grads = None
         self.thermal_seeds = thermal_seeds if thermal_seeds is not None else self._default_thermal_seeds
         self.enthalpy_adjoint_rtol = float(enthalpy_adjoint_rtol)
         self.enthalpy_adjoint_iters = int(enthalpy_adjoint_iters)
@@ -68,16 +81,23 @@ class CoupledStep:
             self._step(t, dt, report)
         caller.wait_stream(self.main)
 
-    def _thermal(self, dt, report):
+    def _thermal(self, dt, report, coupled):
         th = self.thermal
         g = self.model.mg[0]
         self.enthalpy_report = th.step(dt * SEC_PER_YR, report=report, B_out=self.B_next)
         seeds = self.thermal_seeds(th, g)
+        self.thermal_grads = None
         if seeds is not None:
             dL_dE, dL_dB = seeds
-            th.ops.adjoint_solve(dt * SEC_PER_YR, dL_dE=dL_dE, dL_dB=dL_dB, B_scale=th.B_scale,
-                                 relative_tolerance=self.enthalpy_adjoint_rtol, n_iter=self.enthalpy_adjoint_iters,
-                                 u2d=g.state.u.data, v2d=g.state.v.data, vel_div=SEC_PER_YR)
+            if coupled:  # the grid still holds the rate factor the step read
+                self.thermal_grads = th.adjoint(dt * SEC_PER_YR, dL_dE, dL_dB, B_in=g.rheology.B.data,
+                                                n_iter=self.enthalpy_adjoint_iters,
+                                                relative_tolerance=self.enthalpy_adjoint_rtol)
+            else:
+                th.ops.adjoint_solve(dt * SEC_PER_YR, dL_dE=dL_dE, dL_dB=dL_dB, B_scale=th.B_scale,
+                                     relative_tolerance=self.enthalpy_adjoint_rtol,
+                                     n_iter=self.enthalpy_adjoint_iters, u2d=g.state.u.data, v2d=g.state.v.data,
+                                     vel_div=SEC_PER_YR)
 
     def _step(self, t, dt, report):
         model, th = self.model, self.thermal
@@ -89,20 +109,31 @@ class CoupledStep:
             model.forward(t, dt, update_geometry=True)
         g = model.mg[0]
         main = torch.cuda.current_stream()
+        coupled = self.coupled_adjoint and th is not None and self.seeds is not None
+        side = th is not None and self.concurrent and not coupled
         if th is not None:
-            if self.concurrent:
+            if side:
                 self.ev_fwd.record(main)
                 self.side.wait_event(self.ev_fwd)
                 with torch.cuda.stream(self.side):
-                    self._thermal(dt, report)
+                    self._thermal(dt, report, False)
                     self.ev_thermal.record(self.side)
             else:
-                self._thermal(dt, report)
+                self._thermal(dt, report, coupled)
         if self.seeds is not None:
             dJdu, dJdv, dJdH = self.seeds(g)
+            tg = self.thermal_grads if coupled else None
+            if tg is not None:
+                def add(seed, extra):
+                    return extra if seed is None else seed + extra
+                dJdu, dJdv, dJdH = add(dJdu, tg["u"]), add(dJdv, tg["v"]), add(dJdH, tg["H"])
             model.backward(t, dt, dJdu=dJdu, dJdv=dJdv, dJdH=dJdH)
+            if tg is not None:
+                if tg["beta"] is not None:
+                    g.sliding.beta.grad.add_(tg["beta"])
+                g.state.H_prev.grad.add_(tg["H_prev"])
         if th is not None:
-            if self.concurrent:
+            if side:
                 main.wait_event(self.ev_thermal)
             g.rheology.B.data.copy_(self.B_next)
 

@@ -199,7 +199,7 @@ class ThermalModel:
         ops.enthalpy_state.E.copy_(E)
 
     def adjoint(self, dt, dL_dE, dL_dB=None, B_in=None, n_iter=200, relative_tolerance=1e-12,
-                absolute_tolerance=0.0, report=False):
+                absolute_tolerance=0.0, report=False, flat=True):
         """Adjoint of the last ``step(dt)`` (no counterpart in the reference,
         whose enthalpy solve has no adjoint): solves the transposed system of
         the converged solve for dL/dE and, when ``update_rheology``, dL/dB of
@@ -215,7 +215,9 @@ class ThermalModel:
         factor's dependence on H.  The grid and the operators must hold the
         step's state (``step`` just ran, or ``rebuild_step``); ``B_in`` is the
         rate factor that step read, needed with strain heating when the step
-        overwrote ``grid.rheology.B``."""
+        overwrote ``grid.rheology.B``.  flat: the adjoint sweeps read the
+        depth-constant SSA velocity (the step's, equal results) instead of the
+        per-level velocity planes."""
         import ctypes
 
         from . import _lib
@@ -229,9 +231,10 @@ class ThermalModel:
                                  "updated grid.rheology.B")
             B_in = grid.rheology.B.data
         gB = dL_dB if (dL_dB is not None and self.update_rheology) else None
+        vel = dict(u2d=grid.state.u.data, v2d=grid.state.v.data, vel_div=self.SEC_PER_YR) if flat else {}
         out = ops.adjoint_solve(dt, dL_dE, gB, B_scale=self.B_scale, n_iter=n_iter,
                                 relative_tolerance=relative_tolerance, absolute_tolerance=absolute_tolerance,
-                                report=report)
+                                report=report, **vel)
         z = grid.zeros
         n = int(_lib.load().glide_thermal_vjp_work_doubles(ny, nx))
         if getattr(self, "_vjp_work", None) is None or self._vjp_work.numel() != n:

@@ -133,3 +133,64 @@ def test_coupled_taylor_remainders_are_second_order():
     assert max(rel) < 1e-3
     assert len(orders) >= 4
     assert min(orders) > 1.7
+
+
+def test_coupled_step_adjoint_matches_step_differences():
+    """CoupledStep's adjoint of one coupled step (the enthalpy adjoint, the
+    thermal step's input gradients into the momentum adjoint's seeds and the
+    basal-drag gradient) against central differences of whole coupled steps of
+    L = 0.5 |u|^2 + 0.5 |v|^2 + c (0.5 |E|^2 + sum B_new) in a random
+    direction of beta (interior cells: the momentum gradient in the
+    reference's form drops the boundary-facet terms of the last row and
+    column).  The decoupled adjoint (enthalpy and momentum adjoints on their
+    own seeds) misses the derivative by far more than the tolerance, so the
+    coupling terms are what the agreement checks."""
+    from glide_amd.coupled import CoupledStep
+    print("coupled step adjoint 32^2 L3 x 11 levels against step differences in beta", flush=True)
+    model, th, H0, E0, bed, lb, smb, Q, Es, B0 = _setup()
+    g = model.mg[0]
+    ops = th.ops
+    c = 1e3
+
+    def tensors():
+        return [g.state.u.data, g.state.v.data, g.state.H.data, g.state.H_prev.data, g.state.mask.data,
+                g.state.phi.data, g.rheology.B.data, g.sliding.beta.data, ops.enthalpy_state.E,
+                ops.enthalpy_state.E_prev, ops.H_prev]
+
+    snap = [t.clone() for t in tensors()]
+
+    def run(beta, **kw):
+        for t, s in zip(tensors(), snap):
+            t.copy_(s)
+        model.mg.sliding.beta.set(beta)
+        step = CoupledStep(model, th, thermal_seeds=lambda th_, gr: (c * th_.ops.enthalpy_state.E,
+                                                                     c * torch.ones_like(gr.rheology.B.data)),
+                           enthalpy_adjoint_rtol=1e-14, enthalpy_adjoint_iters=250, **kw)
+        step(0.0, DT)
+        torch.cuda.synchronize()
+        E = ops.enthalpy_state.E
+        L = (0.5 * (g.state.u.data ** 2).sum() + 0.5 * (g.state.v.data ** 2).sum()
+             + c * (0.5 * (E ** 2).sum() + g.rheology.B.data.sum()))
+        return float(L), (g.sliding.beta.grad.clone() if kw.get("seeds", 1) is not None else None)
+
+    beta0 = torch.exp(lb)
+    L0, grad = run(beta0)
+    _, grad_dec = run(beta0, coupled_adjoint=False)
+    gen = torch.Generator(device="cuda").manual_seed(17)
+    d = torch.randn(beta0.shape, generator=gen, device="cuda", dtype=beta0.dtype)
+    d[:2, :] = 0.0
+    d[-2:, :] = 0.0
+    d[:, :2] = 0.0
+    d[:, -2:] = 0.0
+    d = d * beta0
+    h = 1e-5
+    Lp, _ = run(beta0 + h * d, seeds=None)
+    Lm, _ = run(beta0 - h * d, seeds=None)
+    fd = (Lp - Lm) / (2 * h)
+    ad = float((grad * d).sum())
+    dec = float((grad_dec * d).sum())
+    print(f"  L = {L0:.10e}: coupled adjoint {ad:.10e}, decoupled {dec:.10e}, step differences {fd:.10e}",
+          flush=True)
+    tol = 1e-6 * abs(fd)
+    assert abs(ad - fd) <= tol
+    assert abs(dec - fd) > 10 * tol
