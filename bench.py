@@ -643,19 +643,34 @@ def ensemble(args, torch):
     glaciers (glide_amd.ensemble.valley_glacier, dx = 100 m) with side lengths
     drawn log-uniformly from 48 to 384 cells (seeded, multiples of 8), each
     taking one forward + adjoint step (3 iterations each: FP32 red-black
-    multigrid with FP64 defect correction, no host wait) per ensemble step,
-    spread over --ens-streams streams; --warmup steps, then --steps timed.
-    The same with the FP64 form (3 reference-form V-cycles) for comparison.
-    Prints one JSON line with the summed steps/s."""
-    from glide_amd.ensemble import Ensemble, ensemble_sizes, valley_glacier
+    multigrid with FP64 defect correction, no host wait) per ensemble step:
+    batched (one launch per stage over a table of the domains taking part,
+    V-cycles level by level, one graph per step, red-black tiles from
+    autotune_tiles), then per domain over --ens-streams streams (each
+    domain's solves from its own graphs) and the FP64 form (3 reference-form
+    V-cycles); --warmup steps, then --steps timed.  Prints one JSON line with
+    the summed steps/s of each and whether the batched results equal the
+    per-domain ones bitwise."""
+    from glide_amd.ensemble import Ensemble, autotune_tiles, ensemble_sizes, valley_glacier
     sizes = ensemble_sizes(args.ens_domains, seed=args.seed)
     probs = [valley_glacier(n, seed=args.seed + 1000 + k) for k, n in enumerate(sizes)]
     out = {"workload": (f"{len(probs)} valley glaciers, sides {min(sizes)}-{max(sizes)} cells "
                         f"(log-uniform, seeded), dx = 100 m, one forward + adjoint step each per ensemble step, "
                         "3 iterations per solve"),
            "domains": len(probs), "cells": int(sum(n * n for n in sizes)), "streams": args.ens_streams}
-    for prec in ("mixed", "fp64"):
-        ens = Ensemble(probs, precision=prec, iterations=3, streams=args.ens_streams)
+    t0 = time.perf_counter()
+    tiles, timings = autotune_tiles(probs, iterations=3, dt=1.0)
+    out["autotune"] = {"tiles": tiles, "seconds": round(time.perf_counter() - t0, 2),
+                       "candidates_timed": len(timings),
+                       "ms_default_tiles": round(timings[",".join(["4"] * len(tiles))], 2),
+                       "ms_tuned_tiles": round(timings[",".join(str(x) for x in tiles)], 2)}
+    print(f"ensemble autotune: {out['autotune']}", flush=True)
+    final = {}
+    for form in ("batched", "mixed", "fp64"):
+        prec = "fp64" if form == "fp64" else "mixed"
+        ens = Ensemble(probs, precision=prec, iterations=3, streams=args.ens_streams, batched=form == "batched")
+        if form == "batched":
+            ens.set_tiles(tiles)
         t = 0.0
         for _ in range(args.warmup):
             ens.step(t, 1.0)
@@ -668,10 +683,22 @@ def ensemble(args, torch):
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         finite = all(bool(torch.isfinite(m.mg[0].state.u.data).all()) for m in ens.models)
-        out[prec] = {"domain_steps_per_s": round(len(probs) * args.steps / el, 2),
+        out[form] = {"domain_steps_per_s": round(len(probs) * args.steps / el, 2),
                      "ensemble_step_ms": round(1000.0 * el / args.steps, 2), "finite": finite}
-        print(f"ensemble {prec}: {out[prec]}", flush=True)
+        if form == "batched":
+            out[form]["launches_per_step"] = ens.launches_per_step
+        if form != "fp64":
+            final[form] = [torch.cat([m.mg[0].state.u.data.flatten(), m.mg[0].sliding.beta.grad.flatten()]).cpu()
+                           for m in ens.models]
+        print(f"ensemble {form}: {out[form]}", flush=True)
+        if form == "batched":
+            ens.close()
         del ens
+    # the same steps from the same start: the batched step equals the per-domain solves bitwise
+    out["batched"]["bitwise_equal_to_per_domain"] = all(
+        torch.equal(a, b) for a, b in zip(final["batched"], final["mixed"]))
+    out["batched"]["speedup_vs_per_domain"] = round(
+        out["batched"]["domain_steps_per_s"] / out["mixed"]["domain_steps_per_s"], 2)
     print(json.dumps(out), flush=True)
 
 

@@ -155,6 +155,15 @@ SIGNATURES = {
     "glide_fas_adjoint_solve_mixed": (_I, [_LP, _LP, _I, _I, _D, ctypes.POINTER(FasConfigC),
                                            ctypes.POINTER(VankaConfigC), ctypes.POINTER(MixedConfigC),
                                            ctypes.POINTER(SolveReport), ctypes.POINTER(_I), _P]),
+    "glide_ensemble_create": (_I, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_LP),
+                                   ctypes.POINTER(_LP), ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC),
+                                   ctypes.POINTER(VankaConfigC), ctypes.POINTER(MixedConfigC), ctypes.c_int32,
+                                   ctypes.POINTER(_P)]),
+    "glide_ensemble_step": (_I, [_P, _D, _P]),
+    "glide_ensemble_launches": (ctypes.c_int32, [_P]),
+    "glide_ensemble_levels": (ctypes.c_int32, [_P]),
+    "glide_ensemble_set_tiles": (_I, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
+    "glide_ensemble_destroy": (_I, [_P]),
     "glide_ssa_vjp": (_I, [_LP, _D, _I, _P, _P, _P, _P, _P, _P, _P]),
     "glide_vanka_adjoint_sweeps": (_I, [_LP, _D, _I, ctypes.POINTER(VankaConfigC), _I, _P]),
     "glide_fas_adjoint_solve": (_I, [_LP, _I, _I, _D, ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC),

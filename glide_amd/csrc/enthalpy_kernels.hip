@@ -1660,7 +1660,10 @@ __device__ __forceinline__ void pcr_stage(T& a, T& b, T& c, T& d) {
 // layout): W = E + omega_relax (E_col - E), and with rmax the max |r| of the
 // starting state (first Newton iteration's rows) folded into the slots as
 // k_enth_column_smooth does.  P lanes per column (4 or 8, consecutive lanes
-// of one 16-lane row), MB >= 3 levels per lane, P MB >= nz.
+// of one 16-lane row), MB >= 3 levels per lane, P MB >= nz.  Left at the
+// compiler's register allocation (143 VGPRs, 3 waves per SIMD, no spills):
+// held to 4 or 5 waves per SIMD the spills cost more (at 512 x 512 x 41 a
+// sweep measured 425 us unconstrained, 503 us at 4 waves, 807 us at 5).
 template <typename T, int P, int MB>
 __global__ __launch_bounds__(256) void k_enth_column_lm(EnthArgs<T> A, T* __restrict__ W, T omega_relax,
                                                         int n_newton, T relaxation,

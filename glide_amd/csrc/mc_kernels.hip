@@ -25,6 +25,7 @@
 
 #include "ssa_device.hpp"
 #include "ssa_kernels.hpp"
+#include "tab.hpp"
 
 namespace glide {
 
@@ -33,13 +34,20 @@ namespace {
 constexpr int kMcLanes = 64;  // cell pairs per wavefront row segment
 constexpr int kMcRows = 4;    // rows per block
 
+// Kernel bodies take their block index (bx, by) as arguments, so that the
+// domain-table launches (one launch over many domains) run them unchanged.
 template <typename T>
-__global__ __launch_bounds__(256) void k_cell_eta(SsaArgs<T> a, T* __restrict__ eta) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+__device__ __forceinline__ void cell_eta_body(int bx, int by, const SsaArgs<T>& a, T* __restrict__ eta) {
+    const int j = bx * 256 + threadIdx.x, i = by;
     if (j >= a.nx) return;
     CellVel<T> c = load_cell_vel<T>(a.u, a.v, i, j, a.ny, a.nx);
     const T e2 = strain_invariant(c, i, j, a.ny, a.nx, a.dxi, a.P.eps_reg);
     eta[(long)i * a.nx + j] = T(0.5) * a.B[(long)i * a.nx + j] * glen_power(e2, a.P);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cell_eta(SsaArgs<T> a, T* __restrict__ eta) {
+    cell_eta_body<T>(blockIdx.x, blockIdx.y, a, eta);
 }
 
 // Local Newton iteration of cell (i, j) against the state in a; returns the
@@ -91,15 +99,16 @@ __device__ __forceinline__ void mc_newton(const SsaArgs<T>& a, const T* __restri
     }
 }
 
-template <typename T, bool LINEAR>
-__global__ __launch_bounds__(kMcLanes* kMcRows) void k_vanka_mc(SsaArgs<T> a, const T* __restrict__ eta,
-                                                                T* __restrict__ u_dst, T* __restrict__ v_dst,
-                                                                T* __restrict__ H_dst, T* __restrict__ mask,
-                                                                VankaArgs<T> vk, int colour) {
+// ROWS rows of cell pairs per block (kMcLanes pairs each); the arithmetic of
+// a cell does not depend on ROWS.
+template <typename T, bool LINEAR, int ROWS = kMcRows>
+__device__ __forceinline__ void vanka_mc_body(int bx, int by, const SsaArgs<T>& a, const T* __restrict__ eta,
+                                              T* __restrict__ u_dst, T* __restrict__ v_dst, T* __restrict__ H_dst,
+                                              T* __restrict__ mask, const VankaArgs<T>& vk, int colour) {
     const int ny = a.ny, nx = a.nx;
-    const int i = blockIdx.y * kMcRows + threadIdx.y;
+    const int i = by * ROWS + threadIdx.y;
     if (i >= ny) return;
-    const int pair = blockIdx.x * kMcLanes + threadIdx.x;
+    const int pair = bx * kMcLanes + threadIdx.x;
     const int p = (i + colour) & 1;
     const int j = 2 * pair + p;        // the lane's cell of the active colour
     const int jo = 2 * pair + 1 - p;   // and the other colour's cell beside it
@@ -108,7 +117,7 @@ __global__ __launch_bounds__(kMcLanes* kMcRows) void k_vanka_mc(SsaArgs<T> a, co
         T x[5], masked = mask[c];
         // waves holding a row on the domain edge or the first / last column
         // test the Dirichlet pins; the others run a loop copy without them
-        const bool edge = i == 0 || i == ny - 1 || blockIdx.x == 0 || 2 * (blockIdx.x + 1) * kMcLanes >= nx;
+        const bool edge = i == 0 || i == ny - 1 || bx == 0 || 2 * (bx + 1) * kMcLanes >= nx;
         if (edge) mc_newton<T, LINEAR, true>(a, eta, i, j, vk, x, masked);
         else mc_newton<T, LINEAR, false>(a, eta, i, j, vk, x, masked);
         u_dst[(long)i * (nx + 1) + j] = x[0];
@@ -126,6 +135,14 @@ __global__ __launch_bounds__(kMcLanes* kMcRows) void k_vanka_mc(SsaArgs<T> a, co
         if (i == 0) v_dst[jo] = a.v[jo];
         if (i == ny - 1) v_dst[(long)ny * nx + jo] = a.v[(long)ny * nx + jo];
     }
+}
+
+template <typename T, bool LINEAR>
+__global__ __launch_bounds__(kMcLanes* kMcRows) void k_vanka_mc(SsaArgs<T> a, const T* __restrict__ eta,
+                                                                T* __restrict__ u_dst, T* __restrict__ v_dst,
+                                                                T* __restrict__ H_dst, T* __restrict__ mask,
+                                                                VankaArgs<T> vk, int colour) {
+    vanka_mc_body<T, LINEAR>(blockIdx.x, blockIdx.y, a, eta, u_dst, v_dst, H_dst, mask, vk, colour);
 }
 
 template <typename T>
@@ -165,6 +182,65 @@ hipError_t launch_vanka_mc(const SsaArgs<T>& a, const T* eta, T* u_dst, T* v_dst
     else
         k_vanka_mc<T, false><<<g, b, 0, st>>>(a, eta, u_dst, v_dst, H_dst, mask, vk, colour & 1);
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------ domain tables
+namespace {
+template <typename T>
+__global__ __launch_bounds__(256) void k_tab_eta(const TabEta<T>* __restrict__ jobs, const int* __restrict__ start,
+                                                 int n) {
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabEta<T> J = jobs[e];
+    const int lb = blockIdx.x - start[e];
+    cell_eta_body<T>(lb % J.gx, lb / J.gx, J.a, J.eta);
+}
+
+template <typename T, bool LINEAR, int ROWS>
+__global__ __launch_bounds__(kMcLanes* ROWS) void k_tab_vanka_mc(const TabVankaMc<T>* __restrict__ jobs,
+                                                                 const int* __restrict__ start, int n, int colour) {
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabVankaMc<T> J = jobs[e];
+    const int lb = blockIdx.x - start[e];
+    vanka_mc_body<T, LINEAR, ROWS>(lb % J.gx, lb / J.gx, J.a, J.eta, J.u_dst, J.v_dst, J.H_dst, J.mask, J.vk,
+                                   colour);
+}
+}  // namespace
+
+int tab_blocks_eta(int ny, int nx, int* gx) {
+    *gx = (nx + 255) / 256;
+    return *gx * ny;
+}
+bool tab_vanka_mc_rows_ok(int rows) { return rows == 1 || rows == 2 || rows == 4 || rows == 8; }
+int tab_blocks_vanka_mc(int ny, int nx, int rows, int* gx) {
+    const int pairs = (nx + 1) / 2;
+    *gx = (pairs + kMcLanes - 1) / kMcLanes;
+    return *gx * ((ny + rows - 1) / rows);
+}
+
+template <typename T>
+hipError_t launch_tab_eta(const TabEta<T>* jobs, const int* start, int n, int blocks, hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    k_tab_eta<T><<<blocks, 256, 0, st>>>(jobs, start, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_tab_vanka_mc(const TabVankaMc<T>* jobs, const int* start, int n, int blocks, int colour,
+                               bool linear, int rows, hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    const dim3 b(kMcLanes, rows);
+    const int c = colour & 1;
+#define GLIDE_TAB_MC(R)                                                                        \
+    if (rows == R) {                                                                           \
+        if (linear) k_tab_vanka_mc<T, true, R><<<blocks, b, 0, st>>>(jobs, start, n, c);      \
+        else k_tab_vanka_mc<T, false, R><<<blocks, b, 0, st>>>(jobs, start, n, c);            \
+        return hipGetLastError();                                                              \
+    }
+    GLIDE_TAB_MC(1)
+    GLIDE_TAB_MC(2)
+    GLIDE_TAB_MC(4)
+    GLIDE_TAB_MC(8)
+#undef GLIDE_TAB_MC
+    return hipErrorInvalidValue;
 }
 
 // ------------------------------------------------------------ precision bridge
@@ -297,5 +373,13 @@ hipError_t launch_krylov_axpy(double* out, const double* V, long ld, long n, int
 
 GLIDE_MC_INSTANTIATE(double)
 GLIDE_MC_INSTANTIATE(float)
+
+#define GLIDE_MC_TAB_INSTANTIATE(T)                                                                                 \
+    template hipError_t launch_tab_eta<T>(const TabEta<T>*, const int*, int, int, hipStream_t);                    \
+    template hipError_t launch_tab_vanka_mc<T>(const TabVankaMc<T>*, const int*, int, int, int, bool, int,   \
+                                               hipStream_t);
+
+GLIDE_MC_TAB_INSTANTIATE(double)
+GLIDE_MC_TAB_INSTANTIATE(float)
 
 }  // namespace glide

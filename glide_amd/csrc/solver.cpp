@@ -22,6 +22,7 @@
 #include "../../include/glide_hip.h"
 #include "../../include/glide_enthalpy.h"
 #include "ssa_kernels.hpp"
+#include "tab.hpp"
 
 using namespace glide;
 
@@ -1615,6 +1616,967 @@ int mixed_adj_vcycle(MixedCtx& X, int l, bool finest) {
 }
 
 
+// ====================================================== batched ensemble step
+// An ensemble of independent domains of uneven sizes (no counterpart in the
+// reference, which runs one domain per process: glide/model.py:25-39) stepped
+// by domain-table launches (tab.hpp): every stage of the mixed forward solve,
+// of the mixed adjoint solve and of the gradients is one launch over all the
+// domains that take part in it.  The V-cycles run level-synchronously: the
+// descent of level l covers every domain deeper than l + 1, then the
+// coarsest-level passes of all domains (each at its own coarsest level) run
+// together, then the ascents from the deepest level up.  Each domain sees
+// exactly the stages, in the same order and with the same arithmetic, of its
+// own fixed-count asynchronous mixed solves (glide_fascd_solve_mixed and
+// glide_fas_adjoint_solve_mixed with both tolerances zero and no report);
+// only the independent work of other domains is interleaved.  The launch
+// sequence is recorded once per time step length into a device-resident
+// table arena and replayed as one HIP graph.
+
+enum TabKind {
+    kKEw, kKXfer, kKRes32, kKRes64, kKEta32, kKEta64, kKMc32, kKMc64, kKFac32, kKFac64, kKPass32, kKPass64,
+    kKVjp32, kKVjp64, kKGrad64
+};
+
+struct TabOp {
+    int kind;
+    size_t jobs, start;  // byte offsets in the arena
+    int n, blocks, colour, linear, rows;
+};
+
+struct TabProgram {
+    std::vector<unsigned char> arena;
+    std::vector<TabOp> ops;
+    size_t put(const void* p, size_t bytes, size_t align) {
+        size_t off = (arena.size() + align - 1) / align * align;
+        arena.resize(off + bytes);
+        std::memcpy(arena.data() + off, p, bytes);
+        return off;
+    }
+    template <typename Job>
+    void add(int kind, const std::vector<Job>& jobs, const std::vector<int>& blocks, int colour = 0,
+             int linear = 0, int rows = 0) {
+        if (jobs.empty()) return;
+        TabOp op{};
+        op.kind = kind;
+        op.n = (int)jobs.size();
+        op.colour = colour;
+        op.linear = linear;
+        op.rows = rows;
+        op.jobs = put(jobs.data(), sizeof(Job) * jobs.size(), 64);
+        std::vector<int> start(jobs.size());
+        int acc = 0;
+        for (size_t e = 0; e < jobs.size(); ++e) {
+            start[e] = acc;
+            acc += blocks[e];
+        }
+        op.start = put(start.data(), sizeof(int) * start.size(), 16);
+        op.blocks = acc;
+        ops.push_back(op);
+    }
+};
+
+// Element-wise jobs of one stage.
+struct EwList {
+    std::vector<TabEw> j;
+    std::vector<int> b;
+    void add(int op, void* out, const void* x, const void* y, const void* z, double a, double bb, double c, long n,
+             void* out2 = nullptr) {
+        if (n <= 0) return;
+        j.push_back(TabEw{out, out2, x, y, z, a, bb, c, n, op});
+        b.push_back(tab_blocks_ew(n));
+    }
+    void axpby32(void* o, float a, const void* x, float bb, const void* y, size_t n) {
+        add(kTabAxpby32, o, x, y, nullptr, a, bb, 0.0, (long)n);
+    }
+    void copy32(void* o, const void* x, size_t n) { add(kTabCopy32, o, x, nullptr, nullptr, 0, 0, 0, (long)n); }
+    void fill32(void* o, float v, size_t n) { add(kTabFill32, o, nullptr, nullptr, nullptr, v, 0, 0, (long)n); }
+    // z = z + y; x = w + z
+    void correct32(void* z, const void* y, void* x, const void* w, size_t n) {
+        add(kTabCorrect32, z, z, y, w, 1, 1, 1, (long)n, x);
+    }
+    void axpby64(void* o, double a, const void* x, double bb, const void* y, size_t n) {
+        add(kTabAxpby64, o, x, y, nullptr, a, bb, 0.0, (long)n);
+    }
+    void copy64(void* o, const void* x, size_t n) { add(kTabCopy64, o, x, nullptr, nullptr, 0, 0, 0, (long)n); }
+    void fill64(void* o, double v, size_t n) { add(kTabFill64, o, nullptr, nullptr, nullptr, v, 0, 0, (long)n); }
+    void mix(int op, void* o, const void* x, const void* y, size_t n, double val = 0.0) {
+        add(op, o, x, y, nullptr, val, 0, 0, (long)n);
+    }
+};
+
+struct XferList {
+    std::vector<TabXfer> j;
+    std::vector<int> b;
+    // ny, nx: the coarse level (restrictions) or the fine level (prolongations)
+    void add(int op, const void* src, void* dst, int ny, int nx) {
+        TabXfer q{static_cast<const float*>(src), static_cast<float*>(dst), nullptr, nullptr, nullptr, op, ny, nx, 0};
+        b.push_back(tab_blocks_xfer(op, ny, nx, &q.gx));
+        j.push_back(q);
+    }
+    void pin(const glide_level& L) {
+        TabXfer q{nullptr, PT<float>(L.lambda_u), PT<float>(L.lambda_v), PT<float>(L.lambda_H), CPT<float>(L.mask),
+                  kTabPinRows, L.ny, L.nx, 0};
+        b.push_back(tab_blocks_xfer(kTabPinRows, L.ny, L.nx, &q.gx));
+        j.push_back(q);
+    }
+};
+
+struct EnsDomain {
+    std::vector<glide_level> L64, L32;
+    int nl;
+};
+
+struct EnsRef {  // one domain's level in a stage, with the stage flags of that level
+    int d, l, flags;
+};
+
+struct EnsRec {
+    const std::vector<EnsDomain>& D;
+    const glide_fas_config& cfg;
+    const glide_vanka_config& vf;
+    const glide_vanka_config& va;
+    bool fp64_coarsest;
+    int inner;
+    double dt;
+    TabProgram& prog;
+    int lmax;
+    const std::vector<int>& rows;  // red-black phase tile (rows per block) of level l's stages; [lmax]: coarsest
+
+    const glide_level& l64(int d, int l) const { return D[d].L64[l]; }
+    const glide_level& l32(int d, int l) const { return D[d].L32[l]; }
+    int sflags(int l) const {
+        if (l == 0) return 0;
+        return (cfg.freeze_coarse_calving ? GLIDE_FREEZE_CALVING : 0) | (cfg.freeze_coarse_phi ? GLIDE_FREEZE_PHI : 0);
+    }
+    std::vector<int> deeper_than(int l) const {  // domains with a level below l
+        std::vector<int> s;
+        for (int d = 0; d < (int)D.size(); ++d)
+            if (D[d].nl > l + 1) s.push_back(d);
+        return s;
+    }
+    void ew(const EwList& e) { prog.add(kKEw, e.j, e.b); }
+    void xfer(const XferList& x) { prog.add(kKXfer, x.j, x.b); }
+
+    template <typename T>
+    int residual(const std::vector<const glide_level*>& Ls, const std::vector<int>& flags, const std::vector<void*>& out,
+                 int stride) {
+        std::vector<TabResidual<T>> jobs;
+        std::vector<int> blocks;
+        for (size_t q = 0; q < Ls.size(); ++q) {
+            const glide_level& L = *Ls[q];
+            TabResidual<T> J{};
+            J.a = make_args_t<T>(L, dt, false);
+            J.ou = PT<T>(out[q * stride + 0]);
+            J.ov = PT<T>(out[q * stride + 1]);
+            J.oH = PT<T>(out[q * stride + 2]);
+            J.use_forcing = (flags[q] & GLIDE_USE_FORCING) ? 1 : 0;
+            J.use_mask = (flags[q] & GLIDE_USE_MASK) ? 1 : 0;
+            J.cr = T((flags[q] & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate);
+            blocks.push_back(tab_blocks_residual(L.ny, L.nx, &J.gx));
+            jobs.push_back(J);
+        }
+        prog.add(sizeof(T) == 4 ? kKRes32 : kKRes64, jobs, blocks);
+        return GLIDE_OK;
+    }
+
+    // Multicolour passes (mc_sweeps_t) on the given levels of type T.
+    template <typename T>
+    int mc_sweeps(const std::vector<const glide_level*>& Ls, const std::vector<int>& flags, int n_iter, int slot) {
+        if (Ls.empty() || n_iter <= 0) return GLIDE_OK;
+        const int linear = make_phys(Ls[0]->phys).linear_drag;
+        const int tile = rows[slot];
+        std::vector<TabEta<T>> eta;
+        std::vector<TabVankaMc<T>> red, black;
+        std::vector<int> be, bm;
+        EwList g;
+        for (size_t q = 0; q < Ls.size(); ++q) {
+            const glide_level& L = *Ls[q];
+            if (make_phys(L.phys).linear_drag != linear)
+                return fail(GLIDE_ERR_INVALID_ARG, "the ensemble's domains must share the drag law");
+            VankaArgs<T> vk;
+            vk.newton_steps = vf.newton_steps;
+            vk.ssa_damping = T(vf.ssa_damping);
+            vk.mc_damping = T(vf.mc_damping);
+            vk.calving_rate = T((flags[q] & GLIDE_FREEZE_CALVING) ? 0.0 : L.phys.calving_rate);
+            vk.iters = nullptr;
+            vk.local_fp32 = 0;
+            vk.relax = T(vf.relaxation);
+            vk.tol = T(vf.newton_tol);
+            T* alt_u = PT<T>(L.work);
+            T* alt_v = alt_u + nU(L);
+            T* alt_H = alt_v + nV(L);
+            T* e = alt_H + nC(L);
+            if (!(flags[q] & GLIDE_FREEZE_PHI))
+                g.add(sizeof(T) == 4 ? kTabGrounded32 : kTabGrounded64, L.phi, L.H, L.bed, nullptr,
+                      (double)T(L.phys.flotation_reg_driving), (double)T(vf.relax_phi), 0.0, (long)nC(L));
+            TabEta<T> E{};
+            E.a = make_args_t<T>(L, dt, false);
+            E.eta = e;
+            be.push_back(tab_blocks_eta(L.ny, L.nx, &E.gx));
+            eta.push_back(E);
+            TabVankaMc<T> R{};
+            R.a = E.a;
+            R.eta = e;
+            R.u_dst = alt_u; R.v_dst = alt_v; R.H_dst = alt_H;
+            R.mask = PT<T>(L.mask);
+            R.vk = vk;
+            bm.push_back(tab_blocks_vanka_mc(L.ny, L.nx, tile, &R.gx));
+            red.push_back(R);
+            TabVankaMc<T> B = R;
+            B.a.u = alt_u; B.a.v = alt_v; B.a.H = alt_H;
+            B.u_dst = PT<T>(L.u); B.v_dst = PT<T>(L.v); B.H_dst = PT<T>(L.H);
+            black.push_back(B);
+        }
+        const bool f32 = sizeof(T) == 4;
+        for (int it = 0; it < n_iter; ++it) {
+            ew(g);
+            prog.add(f32 ? kKEta32 : kKEta64, eta, be);
+            prog.add(f32 ? kKMc32 : kKMc64, red, bm, 0, linear, tile);
+            prog.add(f32 ? kKMc32 : kKMc64, black, bm, 1, linear, tile);
+        }
+        return GLIDE_OK;
+    }
+
+    int smooth32(const std::vector<int>& S, int l, int n, int flags) {
+        std::vector<const glide_level*> Ls;
+        for (int d : S) Ls.push_back(&l32(d, l));
+        return mc_sweeps<float>(Ls, std::vector<int>(Ls.size(), flags), n, l);
+    }
+
+    // ------------------------------------------------ forward (mixed_vcycle)
+    int fwd_down(int l) {
+        typedef float F;
+        const std::vector<int> S = deeper_than(l);
+        if (S.empty()) return GLIDE_OK;
+        const int sf = sflags(l);
+        {
+            XferList x;
+            for (int d : S) x.add(kTabRestrictMax, l32(d, l).chi, l32(d, l + 1).chi, l32(d, l + 1).ny, l32(d, l + 1).nx);
+            xfer(x);
+            XferList y;
+            for (int d : S) y.add(kTabProlongInject, l32(d, l + 1).chi, l32(d, l).cphi, l32(d, l).ny, l32(d, l).nx);
+            xfer(y);
+        }
+        {
+            EwList a, b;
+            for (int d : S) {
+                const glide_level& L = l32(d, l);
+                a.axpby32(L.cphi, F(-1), L.cphi, F(1), L.chi, nC(L));
+                b.axpby32(L.gamma, F(1), L.w_H, F(1), L.cphi, nC(L));
+            }
+            ew(a);
+            ew(b);
+        }
+        GLIDE_TRY(smooth32(S, l, cfg.pre_steps, sf));
+        {
+            EwList a;
+            for (int d : S) {
+                const glide_level& L = l32(d, l);
+                a.fill32(L.gamma, F(L.phys.thklim), nC(L));
+                a.axpby32(L.y_u, F(1), L.u, F(-1), L.w_u, nU(L));
+                a.axpby32(L.y_v, F(1), L.v, F(-1), L.w_v, nV(L));
+                a.axpby32(L.y_H, F(1), L.H, F(-1), L.w_H, nC(L));
+            }
+            ew(a);
+        }
+        {
+            XferList x;
+            for (int d : S) {
+                const glide_level &L = l32(d, l), &N = l32(d, l + 1);
+                x.add(kTabRestrictU, L.u, N.u, N.ny, N.nx);
+                x.add(kTabRestrictV, L.v, N.v, N.ny, N.nx);
+                x.add(kTabRestrictMean, L.H, N.H, N.ny, N.nx);
+                x.add(kTabRestrictMean, L.phi, N.phi, N.ny, N.nx);
+                x.add(kTabRestrictMax, L.mask, N.mask, N.ny, N.nx);
+            }
+            xfer(x);
+        }
+        {
+            EwList a;
+            for (int d : S) {
+                const glide_level& N = l32(d, l + 1);
+                a.copy32(N.w_u, N.u, nU(N));
+                a.copy32(N.w_v, N.v, nV(N));
+                a.copy32(N.w_H, N.H, nC(N));
+            }
+            ew(a);
+        }
+        {
+            std::vector<const glide_level*> Ls;
+            std::vector<int> fl;
+            std::vector<void*> out;
+            for (int d : S) {
+                const glide_level& L = l32(d, l);
+                Ls.push_back(&L);
+                fl.push_back(GLIDE_USE_FORCING | (sf & GLIDE_FREEZE_CALVING));
+                out.insert(out.end(), {L.r_u, L.r_v, L.r_H});
+            }
+            GLIDE_TRY(residual<F>(Ls, fl, out, 3));
+        }
+        {
+            XferList x;
+            for (int d : S) {
+                const glide_level &L = l32(d, l), &N = l32(d, l + 1);
+                x.add(kTabRestrictU, L.r_u, N.r_u, N.ny, N.nx);
+                x.add(kTabRestrictV, L.r_v, N.r_v, N.ny, N.nx);
+                x.add(kTabRestrictMean, L.r_H, N.r_H, N.ny, N.nx);
+            }
+            xfer(x);
+        }
+        if (!cfg.freeze_coarse_phi) {
+            EwList g;
+            for (int d : S) {
+                const glide_level& N = l32(d, l + 1);
+                g.add(kTabGrounded32, N.phi, N.H, N.bed, nullptr, (double)F(N.phys.flotation_reg_driving), 0.0, 0.0,
+                      (long)nC(N));
+            }
+            ew(g);
+        }
+        {
+            std::vector<const glide_level*> Ls;
+            std::vector<int> fl;
+            std::vector<void*> out;
+            for (int d : S) {
+                const glide_level& N = l32(d, l + 1);
+                Ls.push_back(&N);
+                fl.push_back(cfg.freeze_coarse_calving ? GLIDE_FREEZE_CALVING : 0);
+                out.insert(out.end(), {N.F_u, N.F_v, N.F_H});
+            }
+            GLIDE_TRY(residual<F>(Ls, fl, out, 3));
+        }
+        EwList a;
+        for (int d : S) {
+            const glide_level& N = l32(d, l + 1);
+            a.axpby32(N.f_u, F(1), N.F_u, F(-1), N.r_u, nU(N));
+            a.axpby32(N.f_v, F(1), N.F_v, F(-1), N.r_v, nV(N));
+            a.axpby32(N.f_H, F(1), N.F_H, F(-1), N.r_H, nC(N));
+        }
+        ew(a);
+        return GLIDE_OK;
+    }
+
+    int fwd_coarsest() {
+        typedef float F;
+        EwList a;
+        for (size_t d = 0; d < D.size(); ++d) {
+            const glide_level& L = l32((int)d, D[d].nl - 1);
+            a.axpby32(L.gamma, F(1), L.w_H, F(1), L.chi, nC(L));
+        }
+        ew(a);
+        if (fp64_coarsest) {
+            EwList in, out;
+            std::vector<const glide_level*> Ls;
+            std::vector<int> fl;
+            for (size_t d = 0; d < D.size(); ++d) {
+                const int c = D[d].nl - 1;
+                const glide_level &A = l32((int)d, c), &Dc = l64((int)d, c);
+                in.mix(kTabTo64, Dc.u, A.u, nullptr, nU(A));
+                in.mix(kTabTo64, Dc.v, A.v, nullptr, nV(A));
+                in.mix(kTabTo64, Dc.H, A.H, nullptr, nC(A));
+                in.mix(kTabTo64, Dc.mask, A.mask, nullptr, nC(A));
+                in.mix(kTabTo64, Dc.phi, A.phi, nullptr, nC(A));
+                in.mix(kTabTo64, Dc.f_u, A.f_u, nullptr, nU(A));
+                in.mix(kTabTo64, Dc.f_v, A.f_v, nullptr, nV(A));
+                in.mix(kTabTo64, Dc.f_H, A.f_H, nullptr, nC(A));
+                in.mix(kTabTo64, Dc.gamma, A.gamma, nullptr, nC(A));
+                out.mix(kTabTo32, A.u, Dc.u, nullptr, nU(A));
+                out.mix(kTabTo32, A.v, Dc.v, nullptr, nV(A));
+                out.mix(kTabTo32, A.H, Dc.H, nullptr, nC(A));
+                out.mix(kTabTo32, A.mask, Dc.mask, nullptr, nC(A));
+                Ls.push_back(&Dc);
+                fl.push_back(sflags(c));
+            }
+            ew(in);
+            GLIDE_TRY(mc_sweeps<double>(Ls, fl, cfg.coarsest_steps, lmax));
+            ew(out);
+        } else {
+            std::vector<const glide_level*> Ls;
+            std::vector<int> fl;
+            for (size_t d = 0; d < D.size(); ++d) {
+                const int c = D[d].nl - 1;
+                Ls.push_back(&l32((int)d, c));
+                fl.push_back(sflags(c));
+            }
+            GLIDE_TRY(mc_sweeps<float>(Ls, fl, cfg.coarsest_steps, lmax));
+        }
+        EwList b;
+        for (size_t d = 0; d < D.size(); ++d) {
+            const glide_level& L = l32((int)d, D[d].nl - 1);
+            b.fill32(L.gamma, F(L.phys.thklim), nC(L));
+        }
+        ew(b);
+        return GLIDE_OK;
+    }
+
+    int fwd_up(int l) {
+        typedef float F;
+        const std::vector<int> S = deeper_than(l);
+        if (S.empty()) return GLIDE_OK;
+        {
+            EwList a;
+            for (int d : S) {
+                const glide_level& N = l32(d, l + 1);
+                a.axpby32(N.z_u, F(1), N.u, F(-1), N.w_u, nU(N));
+                a.axpby32(N.z_v, F(1), N.v, F(-1), N.w_v, nV(N));
+                a.axpby32(N.z_H, F(1), N.H, F(-1), N.w_H, nC(N));
+            }
+            ew(a);
+        }
+        {
+            XferList x;
+            for (int d : S) {
+                const glide_level &L = l32(d, l), &N = l32(d, l + 1);
+                x.add(kTabProlongVfacet, N.z_u, L.z_u, L.ny, L.nx);
+                x.add(cfg.compat_vprolong ? kTabProlongVfacet : kTabProlongHfacet, N.z_v, L.z_v, L.ny, L.nx);
+                x.add(kTabProlongInject, N.z_H, L.z_H, L.ny, L.nx);
+            }
+            xfer(x);
+        }
+        {
+            EwList a;
+            for (int d : S) {
+                const glide_level& L = l32(d, l);
+                a.correct32(L.z_u, L.y_u, L.u, L.w_u, nU(L));
+                a.correct32(L.z_v, L.y_v, L.v, L.w_v, nV(L));
+                a.correct32(L.z_H, L.y_H, L.H, L.w_H, nC(L));
+                a.axpby32(L.gamma, F(1), L.w_H, F(1), L.chi, nC(L));
+            }
+            ew(a);
+        }
+        GLIDE_TRY(smooth32(S, l, cfg.post_steps, sflags(l)));
+        {
+            EwList a;
+            for (int d : S) a.fill32(l32(d, l).gamma, F(l32(d, l).phys.thklim), nC(l32(d, l)));
+            ew(a);
+        }
+        if (l == 0) GLIDE_TRY(smooth32(S, 0, cfg.finest_steps, 0));
+        return GLIDE_OK;
+    }
+
+    int fwd_vcycle() {
+        typedef float F;
+        EwList a, b;
+        for (size_t d = 0; d < D.size(); ++d) {
+            const glide_level& L = l32((int)d, 0);
+            a.copy32(L.w_u, L.u, nU(L));
+            a.copy32(L.w_v, L.v, nV(L));
+            a.copy32(L.w_H, L.H, nC(L));
+            a.axpby32(L.chi, F(-1), L.H, F(0), L.H, nC(L));
+            a.fill32(L.gamma, F(L.phys.thklim), nC(L));
+            b.axpby32(L.chi, F(1), L.chi, F(1), L.gamma, nC(L));
+        }
+        ew(a);
+        ew(b);
+        for (int l = 0; l + 1 < lmax; ++l) GLIDE_TRY(fwd_down(l));
+        GLIDE_TRY(fwd_coarsest());
+        for (int l = lmax - 2; l >= 0; --l) GLIDE_TRY(fwd_up(l));
+        return GLIDE_OK;
+    }
+
+    // glide_fascd_solve_mixed, fixed count without a report
+    int forward(int iterations) {
+        {
+            EwList a;
+            for (size_t d = 0; d < D.size(); ++d) {
+                const glide_level& L = l64((int)d, 0);
+                a.axpby64(L.f_H, 1.0 / dt, L.H_prev, 1.0, L.smb, nC(L));
+                a.fill64(L.gamma, L.phys.thklim, nC(L));
+                for (int l = 0; l < D[d].nl; ++l) {
+                    const glide_level &E = l64((int)d, l), &G = l32((int)d, l);
+                    a.mix(kTabTo32, G.bed, E.bed, nullptr, nC(E));
+                    a.mix(kTabTo32, G.B, E.B, nullptr, nC(E));
+                    a.mix(kTabTo32, G.beta, E.beta, nullptr, nC(E));
+                }
+            }
+            ew(a);
+        }
+        for (int it = 0; it < iterations; ++it) {
+            EwList g, to32, defect, upd;
+            std::vector<const glide_level*> L64s, L32s;
+            std::vector<void*> out64, out32;
+            for (size_t d = 0; d < D.size(); ++d) {
+                const glide_level &Dd = l64((int)d, 0), &A = l32((int)d, 0);
+                g.add(kTabGrounded64, Dd.phi, Dd.H, Dd.bed, nullptr, Dd.phys.flotation_reg_driving, 0.0, 0.0,
+                      (long)nC(Dd));
+                L64s.push_back(&Dd);
+                out64.insert(out64.end(), {Dd.r_u, Dd.r_v, Dd.r_H});
+                to32.mix(kTabTo32, A.u, Dd.u, nullptr, nU(Dd));
+                to32.mix(kTabTo32, A.v, Dd.v, nullptr, nV(Dd));
+                to32.mix(kTabTo32, A.H, Dd.H, nullptr, nC(Dd));
+                to32.mix(kTabTo32, A.mask, Dd.mask, nullptr, nC(Dd));
+                to32.mix(kTabTo32, A.phi, Dd.phi, nullptr, nC(Dd));
+                L32s.push_back(&A);
+                out32.insert(out32.end(), {A.F_u, A.F_v, A.F_H});
+                defect.mix(kTabDefect, A.f_u, A.F_u, Dd.r_u, nU(Dd));
+                defect.mix(kTabDefect, A.f_v, A.F_v, Dd.r_v, nV(Dd));
+                defect.mix(kTabDefect, A.f_H, A.F_H, Dd.r_H, nC(Dd));
+                upd.mix(kTabUpdate, Dd.u, A.u, nullptr, nU(Dd));
+                upd.mix(kTabUpdate, Dd.v, A.v, nullptr, nV(Dd));
+                upd.mix(kTabUpdateH, Dd.H, A.H, A.mask, nC(Dd), Dd.phys.thklim);
+                upd.mix(kTabTo64, Dd.mask, A.mask, nullptr, nC(Dd));
+            }
+            ew(g);
+            GLIDE_TRY(residual<double>(L64s, std::vector<int>(L64s.size(), GLIDE_USE_FORCING), out64, 3));
+            ew(to32);
+            GLIDE_TRY(residual<float>(L32s, std::vector<int>(L32s.size(), 0), out32, 3));
+            ew(defect);
+            for (int q = 0; q < inner; ++q) GLIDE_TRY(fwd_vcycle());
+            ew(upd);
+        }
+        return GLIDE_OK;
+    }
+
+    // ------------------------------------------------ adjoint (mixed_adj_vcycle)
+    template <typename T>
+    int factor(const std::vector<const glide_level*>& Ls) {
+        std::vector<TabAdjFactor<T>> jobs;
+        std::vector<int> blocks;
+        for (const glide_level* p : Ls) {
+            TabAdjFactor<T> J{};
+            J.a = make_args_t<T>(*p, dt, true);
+            J.cache = PT<T>(p->adj_cache);
+            J.vk = adjoint_args_t<T>(*p, va);
+            blocks.push_back(tab_blocks_adj_factor(p->ny, p->nx, &J.gx));
+            jobs.push_back(J);
+        }
+        prog.add(sizeof(T) == 4 ? kKFac32 : kKFac64, jobs, blocks);
+        return GLIDE_OK;
+    }
+
+    template <typename T>
+    int adj_sweeps(const std::vector<const glide_level*>& Ls, int n_iter) {
+        if (Ls.empty() || n_iter <= 0) return GLIDE_OK;
+        std::vector<TabAdjPass<T>> red, black;
+        std::vector<int> blocks;
+        for (const glide_level* p : Ls) {
+            const glide_level& L = *p;
+            if (!make_phys(L.phys).linear_drag)
+                return fail(GLIDE_ERR_INVALID_ARG, "the multicolour adjoint passes need linear drag");
+            T* alt_u = PT<T>(L.work);
+            T* alt_v = alt_u + nU(L);
+            T* alt_H = alt_v + nV(L);
+            TabAdjPass<T> R{};
+            R.a = make_args_t<T>(L, dt, true);
+            R.cache = CPT<T>(L.adj_cache);
+            R.lu = CPT<T>(L.lambda_u); R.lv = CPT<T>(L.lambda_v); R.lH = CPT<T>(L.lambda_H);
+            R.lu_dst = alt_u; R.lv_dst = alt_v; R.lH_dst = alt_H;
+            R.fu = CPT<T>(L.af_u); R.fv = CPT<T>(L.af_v); R.fH = CPT<T>(L.af_H);
+            R.cr = T(L.phys.calving_rate);
+            R.omega = -T(va.omega);
+            blocks.push_back(tab_blocks_adj_pass(L.ny, L.nx, &R.gx));
+            red.push_back(R);
+            TabAdjPass<T> B = R;
+            B.lu = alt_u; B.lv = alt_v; B.lH = alt_H;
+            B.lu_dst = PT<T>(L.lambda_u); B.lv_dst = PT<T>(L.lambda_v); B.lH_dst = PT<T>(L.lambda_H);
+            black.push_back(B);
+        }
+        const int k = sizeof(T) == 4 ? kKPass32 : kKPass64;
+        for (int it = 0; it < n_iter; ++it) {
+            prog.add(k, red, blocks, 0);
+            prog.add(k, black, blocks, 1);
+        }
+        return GLIDE_OK;
+    }
+
+    template <typename T>
+    int vjp_stage(const std::vector<const glide_level*>& Ls, const std::vector<int>& forcing,
+                  const std::vector<int>& cached, const std::vector<void*>& in, const std::vector<void*>& out) {
+        std::vector<TabVjp<T>> jobs;
+        std::vector<int> blocks;
+        for (size_t q = 0; q < Ls.size(); ++q) {
+            const glide_level& L = *Ls[q];
+            TabVjp<T> J{};
+            J.a = make_args_t<T>(L, dt, true);
+            J.lu = CPT<T>(in[3 * q]); J.lv = CPT<T>(in[3 * q + 1]); J.lH = CPT<T>(in[3 * q + 2]);
+            J.fu = forcing[q] ? CPT<T>(L.af_u) : nullptr;
+            J.fv = forcing[q] ? CPT<T>(L.af_v) : nullptr;
+            J.fH = forcing[q] ? CPT<T>(L.af_H) : nullptr;
+            J.ou = PT<T>(out[3 * q]); J.ov = PT<T>(out[3 * q + 1]); J.oH = PT<T>(out[3 * q + 2]);
+            J.cr = T(L.phys.calving_rate);
+            J.eta_cache = cached[q] ? CPT<T>(L.adj_cache) + (size_t)kAdjointCacheEta * nC(L) : nullptr;
+            blocks.push_back(tab_blocks_vjp(L.ny, L.nx, &J.gx));
+            jobs.push_back(J);
+        }
+        prog.add(sizeof(T) == 4 ? kKVjp32 : kKVjp64, jobs, blocks);
+        return GLIDE_OK;
+    }
+
+    int adj_smooth32(const std::vector<int>& S, int l, int n) {
+        std::vector<const glide_level*> Ls;
+        for (int d : S) Ls.push_back(&l32(d, l));
+        return adj_sweeps<float>(Ls, n);
+    }
+
+    int adj_down(int l) {
+        typedef float F;
+        const std::vector<int> S = deeper_than(l);
+        if (S.empty()) return GLIDE_OK;
+        GLIDE_TRY(adj_smooth32(S, l, cfg.pre_steps));
+        {
+            XferList x;
+            for (int d : S) {
+                const glide_level &L = l32(d, l), &N = l32(d, l + 1);
+                x.add(kTabRestrictU, L.u, N.u, N.ny, N.nx);
+                x.add(kTabRestrictV, L.v, N.v, N.ny, N.nx);
+                x.add(kTabRestrictMean, L.H, N.H, N.ny, N.nx);
+                x.add(kTabRestrictMean, L.phi, N.phi, N.ny, N.nx);
+                x.add(kTabRestrictMax, L.mask, N.mask, N.ny, N.nx);
+                x.add(kTabRestrictU, L.lambda_u, N.lambda_u, N.ny, N.nx);
+                x.add(kTabRestrictV, L.lambda_v, N.lambda_v, N.ny, N.nx);
+                x.add(kTabRestrictMean, L.lambda_H, N.lambda_H, N.ny, N.nx);
+            }
+            xfer(x);
+            XferList p;
+            for (int d : S) p.pin(l32(d, l + 1));
+            xfer(p);
+        }
+        {
+            std::vector<const glide_level*> Ls;
+            for (int d : S)
+                if (!(fp64_coarsest && l + 1 == D[d].nl - 1)) Ls.push_back(&l32(d, l + 1));
+            GLIDE_TRY(factor<F>(Ls));
+        }
+        {
+            EwList a;
+            for (int d : S) {
+                const glide_level& N = l32(d, l + 1);
+                a.copy32(N.wl_u, N.lambda_u, nU(N));
+                a.copy32(N.wl_v, N.lambda_v, nV(N));
+                a.copy32(N.wl_H, N.lambda_H, nC(N));
+            }
+            ew(a);
+        }
+        {
+            std::vector<const glide_level*> Ls;
+            std::vector<void*> in, out;
+            for (int d : S) {
+                const glide_level& L = l32(d, l);
+                Ls.push_back(&L);
+                in.insert(in.end(), {L.lambda_u, L.lambda_v, L.lambda_H});
+                out.insert(out.end(), {L.ar_u, L.ar_v, L.ar_H});
+            }
+            GLIDE_TRY(vjp_stage<F>(Ls, std::vector<int>(Ls.size(), 1), std::vector<int>(Ls.size(), 1), in, out));
+        }
+        {
+            XferList x;
+            for (int d : S) {
+                const glide_level &L = l32(d, l), &N = l32(d, l + 1);
+                x.add(kTabRestrictU, L.ar_u, N.ar_u, N.ny, N.nx);
+                x.add(kTabRestrictV, L.ar_v, N.ar_v, N.ny, N.nx);
+                x.add(kTabRestrictMean, L.ar_H, N.ar_H, N.ny, N.nx);
+            }
+            xfer(x);
+        }
+        {
+            std::vector<const glide_level*> Ls;
+            std::vector<int> cached;
+            std::vector<void*> in, out;
+            for (int d : S) {
+                const glide_level& N = l32(d, l + 1);
+                Ls.push_back(&N);
+                cached.push_back(!(fp64_coarsest && l + 1 == D[d].nl - 1) ? 1 : 0);
+                in.insert(in.end(), {N.lambda_u, N.lambda_v, N.lambda_H});
+                out.insert(out.end(), {N.vjp_u, N.vjp_v, N.vjp_H});
+            }
+            GLIDE_TRY(vjp_stage<F>(Ls, std::vector<int>(Ls.size(), 0), cached, in, out));
+        }
+        EwList a;
+        for (int d : S) {
+            const glide_level& N = l32(d, l + 1);
+            a.axpby32(N.af_u, F(1), N.vjp_u, F(-1), N.ar_u, nU(N));
+            a.axpby32(N.af_v, F(1), N.vjp_v, F(-1), N.ar_v, nV(N));
+            a.axpby32(N.af_H, F(1), N.vjp_H, F(-1), N.ar_H, nC(N));
+        }
+        ew(a);
+        return GLIDE_OK;
+    }
+
+    int adj_coarsest() {
+        if (!fp64_coarsest) {
+            std::vector<const glide_level*> Ls;
+            for (size_t d = 0; d < D.size(); ++d) Ls.push_back(&l32((int)d, D[d].nl - 1));
+            return adj_sweeps<float>(Ls, cfg.coarsest_steps);
+        }
+        EwList in, out;
+        std::vector<const glide_level*> Ls;
+        for (size_t d = 0; d < D.size(); ++d) {
+            const int c = D[d].nl - 1;
+            const glide_level &A = l32((int)d, c), &Dc = l64((int)d, c);
+            in.mix(kTabTo64, Dc.u, A.u, nullptr, nU(A));
+            in.mix(kTabTo64, Dc.v, A.v, nullptr, nV(A));
+            in.mix(kTabTo64, Dc.H, A.H, nullptr, nC(A));
+            in.mix(kTabTo64, Dc.phi, A.phi, nullptr, nC(A));
+            in.mix(kTabTo64, Dc.mask, A.mask, nullptr, nC(A));
+            in.mix(kTabTo64, Dc.lambda_u, A.lambda_u, nullptr, nU(A));
+            in.mix(kTabTo64, Dc.lambda_v, A.lambda_v, nullptr, nV(A));
+            in.mix(kTabTo64, Dc.lambda_H, A.lambda_H, nullptr, nC(A));
+            in.mix(kTabTo64, Dc.af_u, A.af_u, nullptr, nU(A));
+            in.mix(kTabTo64, Dc.af_v, A.af_v, nullptr, nV(A));
+            in.mix(kTabTo64, Dc.af_H, A.af_H, nullptr, nC(A));
+            out.mix(kTabTo32, A.lambda_u, Dc.lambda_u, nullptr, nU(A));
+            out.mix(kTabTo32, A.lambda_v, Dc.lambda_v, nullptr, nV(A));
+            out.mix(kTabTo32, A.lambda_H, Dc.lambda_H, nullptr, nC(A));
+            Ls.push_back(&Dc);
+        }
+        ew(in);
+        GLIDE_TRY(factor<double>(Ls));
+        GLIDE_TRY(adj_sweeps<double>(Ls, cfg.coarsest_steps));
+        ew(out);
+        return GLIDE_OK;
+    }
+
+    int adj_up(int l) {
+        typedef float F;
+        const std::vector<int> S = deeper_than(l);
+        if (S.empty()) return GLIDE_OK;
+        {
+            EwList a;
+            for (int d : S) {
+                const glide_level& N = l32(d, l + 1);
+                a.axpby32(N.zl_u, F(1), N.lambda_u, F(-1), N.wl_u, nU(N));
+                a.axpby32(N.zl_v, F(1), N.lambda_v, F(-1), N.wl_v, nV(N));
+                a.axpby32(N.zl_H, F(1), N.lambda_H, F(-1), N.wl_H, nC(N));
+            }
+            ew(a);
+        }
+        {
+            XferList x;
+            for (int d : S) {
+                const glide_level &L = l32(d, l), &N = l32(d, l + 1);
+                x.add(kTabProlongVfacet, N.zl_u, L.zl_u, L.ny, L.nx);
+                x.add(cfg.compat_vprolong ? kTabProlongVfacet : kTabProlongHfacet, N.zl_v, L.zl_v, L.ny, L.nx);
+                x.add(kTabProlongBilinear, N.zl_H, L.zl_H, L.ny, L.nx);
+            }
+            xfer(x);
+        }
+        {
+            EwList a;
+            for (int d : S) {
+                const glide_level& L = l32(d, l);
+                a.axpby32(L.lambda_u, F(1), L.lambda_u, F(1), L.zl_u, nU(L));
+                a.axpby32(L.lambda_v, F(1), L.lambda_v, F(1), L.zl_v, nV(L));
+                a.axpby32(L.lambda_H, F(1), L.lambda_H, F(1), L.zl_H, nC(L));
+            }
+            ew(a);
+            XferList p;
+            for (int d : S) p.pin(l32(d, l));
+            xfer(p);
+        }
+        GLIDE_TRY(adj_smooth32(S, l, cfg.post_steps));
+        if (l == 0) GLIDE_TRY(adj_smooth32(S, 0, cfg.finest_steps));
+        return GLIDE_OK;
+    }
+
+    int adj_vcycle() {
+        for (int l = 0; l + 1 < lmax; ++l) GLIDE_TRY(adj_down(l));
+        GLIDE_TRY(adj_coarsest());
+        for (int l = lmax - 2; l >= 0; --l) GLIDE_TRY(adj_up(l));
+        return GLIDE_OK;
+    }
+
+    // glide_fas_adjoint_solve_mixed, fixed count without a report, then the
+    // gradients `which` (glide_gradients)
+    int adjoint(int iterations, int which) {
+        {
+            EwList a;
+            std::vector<const glide_level*> A0;
+            for (size_t d = 0; d < D.size(); ++d) {
+                const glide_level &Dd = l64((int)d, 0), &A = l32((int)d, 0);
+                a.mix(kTabTo32, A.u, Dd.u, nullptr, nU(Dd));
+                a.mix(kTabTo32, A.v, Dd.v, nullptr, nV(Dd));
+                a.mix(kTabTo32, A.H, Dd.H, nullptr, nC(Dd));
+                a.mix(kTabTo32, A.phi, Dd.phi, nullptr, nC(Dd));
+                a.mix(kTabTo32, A.mask, Dd.mask, nullptr, nC(Dd));
+                for (int l = 0; l < D[d].nl; ++l) {
+                    const glide_level &E = l64((int)d, l), &G = l32((int)d, l);
+                    a.mix(kTabTo32, G.bed, E.bed, nullptr, nC(E));
+                    a.mix(kTabTo32, G.B, E.B, nullptr, nC(E));
+                    a.mix(kTabTo32, G.beta, E.beta, nullptr, nC(E));
+                }
+                A0.push_back(&A);
+            }
+            ew(a);
+            GLIDE_TRY(factor<float>(A0));
+        }
+        for (int it = 0; it < iterations; ++it) {
+            std::vector<const glide_level*> L64s;
+            std::vector<void*> in, out;
+            EwList neg, add;
+            XferList pin;
+            for (size_t d = 0; d < D.size(); ++d) {
+                const glide_level &Dd = l64((int)d, 0), &A = l32((int)d, 0);
+                L64s.push_back(&Dd);
+                in.insert(in.end(), {Dd.lambda_u, Dd.lambda_v, Dd.lambda_H});
+                out.insert(out.end(), {Dd.ar_u, Dd.ar_v, Dd.ar_H});
+                neg.mix(kTabNeg32, A.af_u, Dd.ar_u, nullptr, nU(Dd));
+                neg.mix(kTabNeg32, A.af_v, Dd.ar_v, nullptr, nV(Dd));
+                neg.mix(kTabNeg32, A.af_H, Dd.ar_H, nullptr, nC(Dd));
+                neg.fill32(A.lambda_u, 0.0f, nU(A));
+                neg.fill32(A.lambda_v, 0.0f, nV(A));
+                neg.fill32(A.lambda_H, 0.0f, nC(A));
+                pin.pin(A);
+                add.mix(kTabAdd64, Dd.lambda_u, A.lambda_u, nullptr, nU(Dd));
+                add.mix(kTabAdd64, Dd.lambda_v, A.lambda_v, nullptr, nV(Dd));
+                add.mix(kTabAdd64, Dd.lambda_H, A.lambda_H, nullptr, nC(Dd));
+            }
+            GLIDE_TRY(vjp_stage<double>(L64s, std::vector<int>(L64s.size(), 1), std::vector<int>(L64s.size(), 0), in,
+                                        out));
+            ew(neg);
+            for (int q = 0; q < inner; ++q) GLIDE_TRY(adj_vcycle());
+            xfer(pin);
+            ew(add);
+        }
+        if (which & 3) {
+            std::vector<TabGrad<double>> jobs;
+            std::vector<int> blocks;
+            for (size_t d = 0; d < D.size(); ++d) {
+                const glide_level& L = l64((int)d, 0);
+                TabGrad<double> J{};
+                J.a = make_args(L, dt, true);
+                J.lu = CP(L.lambda_u);
+                J.lv = CP(L.lambda_v);
+                J.gbeta = (which & 1) ? P(L.g_beta) : nullptr;
+                J.gbed = (which & 2) ? P(L.g_bed) : nullptr;
+                J.exact = (which & GLIDE_GRAD_EXACT) ? 1 : 0;
+                blocks.push_back(tab_blocks_grad(L.ny, L.nx, &J.gx));
+                jobs.push_back(J);
+            }
+            prog.add(kKGrad64, jobs, blocks);
+        }
+        EwList g, z;
+        for (size_t d = 0; d < D.size(); ++d) {
+            const glide_level& L = l64((int)d, 0);
+            if (which & 4) {
+                g.axpby64(L.g_H_prev, -1.0 / dt, L.lambda_H, 0.0, L.lambda_H, nC(L));
+                if (which & GLIDE_GRAD_EXACT) z.add(kTabZeroMasked64, L.g_H_prev, L.mask, nullptr, nullptr, 0, 0, 0, (long)nC(L));
+            }
+            if (which & 8) {
+                g.axpby64(L.g_smb, -1.0, L.lambda_H, 0.0, L.lambda_H, nC(L));
+                if (which & GLIDE_GRAD_EXACT) z.add(kTabZeroMasked64, L.g_smb, L.mask, nullptr, nullptr, 0, 0, 0, (long)nC(L));
+            }
+        }
+        ew(g);
+        ew(z);
+        return GLIDE_OK;
+    }
+
+    // The ensemble step: forward solve, H_prev <- H, adjoint seeds
+    // dJ/du = u, dJ/dv = v (forcing -u, -v, 0), adjoint solve and gradients.
+    int step(int iterations, int which) {
+        GLIDE_TRY(forward(iterations));
+        EwList a;
+        for (size_t d = 0; d < D.size(); ++d) {
+            const glide_level& L = l64((int)d, 0);
+            a.copy64(L.H_prev, L.H, nC(L));
+            a.axpby64(L.af_u, -1.0, L.u, 0.0, L.u, nU(L));
+            a.axpby64(L.af_v, -1.0, L.v, 0.0, L.v, nV(L));
+            a.fill64(L.af_H, 0.0, nC(L));
+        }
+        ew(a);
+        return adjoint(iterations, which);
+    }
+};
+
+int run_tab_op(const TabOp& op, const unsigned char* dev, hipStream_t st) {
+    const int* start = reinterpret_cast<const int*>(dev + op.start);
+    const void* jobs = dev + op.jobs;
+    hipError_t e = hipSuccess;
+    switch (op.kind) {
+        case kKEw: e = launch_tab_ew(static_cast<const TabEw*>(jobs), start, op.n, op.blocks, st); break;
+        case kKXfer: e = launch_tab_xfer(static_cast<const TabXfer*>(jobs), start, op.n, op.blocks, st); break;
+        case kKRes32:
+            e = launch_tab_residual<float>(static_cast<const TabResidual<float>*>(jobs), start, op.n, op.blocks, st);
+            break;
+        case kKRes64:
+            e = launch_tab_residual<double>(static_cast<const TabResidual<double>*>(jobs), start, op.n, op.blocks, st);
+            break;
+        case kKEta32: e = launch_tab_eta<float>(static_cast<const TabEta<float>*>(jobs), start, op.n, op.blocks, st); break;
+        case kKEta64: e = launch_tab_eta<double>(static_cast<const TabEta<double>*>(jobs), start, op.n, op.blocks, st); break;
+        case kKMc32:
+            e = launch_tab_vanka_mc<float>(static_cast<const TabVankaMc<float>*>(jobs), start, op.n, op.blocks,
+                                           op.colour, op.linear != 0, op.rows, st);
+            break;
+        case kKMc64:
+            e = launch_tab_vanka_mc<double>(static_cast<const TabVankaMc<double>*>(jobs), start, op.n, op.blocks,
+                                            op.colour, op.linear != 0, op.rows, st);
+            break;
+        case kKFac32:
+            e = launch_tab_adj_factor<float>(static_cast<const TabAdjFactor<float>*>(jobs), start, op.n, op.blocks, st);
+            break;
+        case kKFac64:
+            e = launch_tab_adj_factor<double>(static_cast<const TabAdjFactor<double>*>(jobs), start, op.n, op.blocks,
+                                              st);
+            break;
+        case kKPass32:
+            e = launch_tab_adj_pass<float>(static_cast<const TabAdjPass<float>*>(jobs), start, op.n, op.blocks,
+                                           op.colour, st);
+            break;
+        case kKPass64:
+            e = launch_tab_adj_pass<double>(static_cast<const TabAdjPass<double>*>(jobs), start, op.n, op.blocks,
+                                            op.colour, st);
+            break;
+        case kKVjp32: e = launch_tab_vjp<float>(static_cast<const TabVjp<float>*>(jobs), start, op.n, op.blocks, st); break;
+        case kKVjp64: e = launch_tab_vjp<double>(static_cast<const TabVjp<double>*>(jobs), start, op.n, op.blocks, st); break;
+        case kKGrad64:
+            e = launch_tab_grad<double>(static_cast<const TabGrad<double>*>(jobs), start, op.n, op.blocks, st);
+            break;
+        default: return fail(GLIDE_ERR_INVALID_ARG, "unknown domain-table launch");
+    }
+    HIP_TRY(e, "domain-table launch");
+    return GLIDE_OK;
+}
+
+}  // namespace
+
+struct glide_ensemble {
+    std::vector<EnsDomain> dom;
+    glide_fas_config cfg;
+    glide_vanka_config vf, va;
+    glide_mixed_config mc;
+    int which = 0;
+    int lmax = 0;
+    std::vector<int> rows;  // red-black tiles: levels 0 .. lmax - 1, then the coarsest stage
+    double dt = 0.0;
+    TabProgram prog;
+    unsigned char* dev = nullptr;
+    size_t dev_bytes = 0;
+    int device = 0;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;
+};
+
+namespace {
+
+void ensemble_drop(glide_ensemble* e) {
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    e->exec = nullptr;
+    if (e->dev) hipFree(e->dev);
+    e->dev = nullptr;
+    e->dev_bytes = 0;
+    e->prog = TabProgram{};
+    e->dt = 0.0;
+}
+
+int ensemble_record(glide_ensemble* e, double dt) {
+    ensemble_drop(e);
+    EnsRec R{e->dom, e->cfg, e->vf, e->va, e->mc.fp64_coarsest != 0, e->mc.inner_vcycles, dt, e->prog, e->lmax,
+             e->rows};
+    GLIDE_TRY(R.step(e->mc.max_outer, e->which));
+    if (!e->prog.arena.empty()) {
+        HIP_TRY(hipMalloc(&e->dev, e->prog.arena.size()), "hipMalloc(domain tables)");
+        e->dev_bytes = e->prog.arena.size();
+        HIP_TRY(hipMemcpy(e->dev, e->prog.arena.data(), e->prog.arena.size(), hipMemcpyHostToDevice),
+                "hipMemcpy(domain tables)");
+    }
+    e->dt = dt;
+    return GLIDE_OK;
+}
+
+int ensemble_enqueue(glide_ensemble* e, hipStream_t st) {
+    for (const TabOp& op : e->prog.ops) GLIDE_TRY(run_tab_op(op, e->dev, st));
+    return GLIDE_OK;
+}
+
 }  // namespace
 
 namespace glide {
@@ -2307,6 +3269,142 @@ int glide_gradients(const glide_level* L, double dt, int which, void* stream) {
                 "grad smb");
         if (exact) HIP_TRY(launch_zero_masked<R>(P(L->g_smb), CP(L->mask), (long)nC(*L), st), "grad smb mask");
     }
+    return GLIDE_OK;
+}
+
+
+// ---------------------------------------------------------------- ensemble
+static int ensemble_check_level(const glide_level& L, bool fp64, int l, int nl) {
+    GLIDE_TRY(check_level(&L));
+    REQ(L.u); REQ(L.v); REQ(L.H); REQ(L.phi); REQ(L.mask); REQ(L.bed); REQ(L.B); REQ(L.beta); REQ(L.work);
+    REQ(L.f_u); REQ(L.f_v); REQ(L.f_H); REQ(L.r_u); REQ(L.r_v); REQ(L.r_H); REQ(L.gamma);
+    REQ(L.lambda_u); REQ(L.lambda_v); REQ(L.lambda_H); REQ(L.af_u); REQ(L.af_v); REQ(L.af_H);
+    if (!fp64) {
+        REQ(L.F_u); REQ(L.F_v); REQ(L.F_H); REQ(L.w_u); REQ(L.w_v); REQ(L.w_H); REQ(L.y_u); REQ(L.y_v); REQ(L.y_H);
+        REQ(L.z_u); REQ(L.z_v); REQ(L.z_H); REQ(L.chi); REQ(L.cphi); REQ(L.ar_u); REQ(L.ar_v); REQ(L.ar_H);
+        REQ(L.vjp_u); REQ(L.vjp_v); REQ(L.vjp_H); REQ(L.wl_u); REQ(L.wl_v); REQ(L.wl_H); REQ(L.zl_u); REQ(L.zl_v);
+        REQ(L.zl_H); REQ(L.adj_cache);
+    } else {
+        if (l == 0) { REQ(L.H_prev); REQ(L.smb); REQ(L.ar_u); REQ(L.ar_v); REQ(L.ar_H); }
+        if (l == nl - 1) REQ(L.adj_cache);
+    }
+    if (!(L.phys.m == 1.0 && L.phys.u_reg > 0.0))
+        return fail(GLIDE_ERR_INVALID_ARG, "the ensemble's mixed adjoint solves need linear drag (m = 1)");
+    return GLIDE_OK;
+}
+
+int glide_ensemble_create(int32_t n_domains, const int32_t* n_levels, const glide_level* const* levels64,
+                          const glide_level* const* levels32, const glide_fas_config* cfg,
+                          const glide_vanka_config* forward_vc, const glide_vanka_config* adjoint_vc,
+                          const glide_mixed_config* mc, int32_t grad_which, glide_ensemble** out) {
+    if (!out) return fail(GLIDE_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (n_domains < 1 || !n_levels || !levels64 || !levels32 || !cfg || !forward_vc || !adjoint_vc || !mc)
+        return fail(GLIDE_ERR_INVALID_ARG, "the ensemble needs at least one domain and every argument");
+    if (mc->max_outer < 0 || mc->max_outer > 64 || mc->inner_vcycles < 1)
+        return fail(GLIDE_ERR_INVALID_ARG, "max_outer must be in [0, 64] and inner_vcycles >= 1");
+    if (cfg->coarsest_steps < 0 || cfg->pre_steps < 0 || cfg->post_steps < 0 || cfg->finest_steps < 0)
+        return fail(GLIDE_ERR_INVALID_ARG, "negative smoothing counts");
+    if (grad_which & ~(15 | GLIDE_GRAD_EXACT)) return fail(GLIDE_ERR_INVALID_ARG, "unknown gradient flags");
+    glide_ensemble* e = new glide_ensemble();
+    for (int d = 0; d < n_domains; ++d) {
+        const int nl = n_levels[d];
+        if (nl < 1 || nl > 16 || !levels64[d] || !levels32[d]) {
+            delete e;
+            return fail(GLIDE_ERR_INVALID_ARG, "each domain needs 1 to 16 levels");
+        }
+        EnsDomain D;
+        D.nl = nl;
+        D.L64.assign(levels64[d], levels64[d] + nl);
+        D.L32.assign(levels32[d], levels32[d] + nl);
+        for (int l = 0; l < nl; ++l) {
+            int rc = ensemble_check_level(D.L64[l], true, l, nl);
+            if (rc == GLIDE_OK) rc = ensemble_check_level(D.L32[l], false, l, nl);
+            if (rc == GLIDE_OK && (D.L32[l].ny != D.L64[l].ny || D.L32[l].nx != D.L64[l].nx))
+                rc = fail(GLIDE_ERR_SHAPE, "FP32 and FP64 levels differ in shape");
+            if (rc == GLIDE_OK && l > 0 &&
+                (D.L64[l - 1].ny != 2 * D.L64[l].ny || D.L64[l - 1].nx != 2 * D.L64[l].nx))
+                rc = fail(GLIDE_ERR_SHAPE, "each level must halve the one above it");
+            if (rc == GLIDE_OK && (grad_which & 1) && l == 0 && !D.L64[0].g_beta) rc = fail(GLIDE_ERR_INVALID_ARG, "g_beta is NULL");
+            if (rc == GLIDE_OK && (grad_which & 2) && l == 0 && !D.L64[0].g_bed) rc = fail(GLIDE_ERR_INVALID_ARG, "g_bed is NULL");
+            if (rc == GLIDE_OK && (grad_which & 4) && l == 0 && !D.L64[0].g_H_prev) rc = fail(GLIDE_ERR_INVALID_ARG, "g_H_prev is NULL");
+            if (rc == GLIDE_OK && (grad_which & 8) && l == 0 && !D.L64[0].g_smb) rc = fail(GLIDE_ERR_INVALID_ARG, "g_smb is NULL");
+            if (rc != GLIDE_OK) {
+                delete e;
+                return rc;
+            }
+        }
+        e->lmax = std::max(e->lmax, nl);
+        e->dom.push_back(std::move(D));
+    }
+    e->cfg = *cfg;
+    e->vf = *forward_vc;
+    e->va = *adjoint_vc;
+    e->mc = *mc;
+    e->which = grad_which;
+    e->rows.assign(e->lmax + 1, 4);
+    if (hipGetDevice(&e->device) != hipSuccess) {
+        delete e;
+        return fail(GLIDE_ERR_HIP, "hipGetDevice");
+    }
+    *out = e;
+    return GLIDE_OK;
+}
+
+int glide_ensemble_step(glide_ensemble* e, double dt, void* stream) {
+    if (!e) return fail(GLIDE_ERR_INVALID_ARG, "ensemble is NULL");
+    if (!(dt > 0.0)) return fail(GLIDE_ERR_INVALID_ARG, "dt must be positive");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != e->device) return fail(GLIDE_ERR_INVALID_ARG, "the ensemble belongs to another device");
+    hipStream_t st = (hipStream_t)stream;
+    if (e->dt != dt) {
+        // a replay of the previous tables may still be running
+        if (e->exec || e->dev) HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        GLIDE_TRY(ensemble_record(e, dt));
+    }
+    if (!g_graphs_on || timing_on()) return ensemble_enqueue(e, st);
+    if (!e->exec) {
+        if (!e->cap) HIP_TRY(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking), "hipStreamCreate(capture)");
+        HIP_TRY(hipStreamBeginCapture(e->cap, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        const int rc = ensemble_enqueue(e, e->cap);
+        hipGraph_t graph = nullptr;
+        const hipError_t ee = hipStreamEndCapture(e->cap, &graph);
+        if (rc != GLIDE_OK || ee != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            if (rc != GLIDE_OK) return rc;
+            return hip_fail(ee, "hipStreamEndCapture");
+        }
+        const hipError_t ei = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            e->exec = nullptr;
+            return hip_fail(ei, "hipGraphInstantiate");
+        }
+    }
+    HIP_TRY(hipGraphLaunch(e->exec, st), "hipGraphLaunch");
+    return GLIDE_OK;
+}
+
+int32_t glide_ensemble_launches(const glide_ensemble* e) { return e ? (int32_t)e->prog.ops.size() : -1; }
+
+int32_t glide_ensemble_levels(const glide_ensemble* e) { return e ? (int32_t)e->lmax : -1; }
+
+int glide_ensemble_set_tiles(glide_ensemble* e, const int32_t* rows, int32_t n) {
+    if (!e || !rows) return fail(GLIDE_ERR_INVALID_ARG, "ensemble or rows is NULL");
+    if (n != e->lmax + 1) return fail(GLIDE_ERR_INVALID_ARG, "rows needs one entry per level and one for the coarsest stage");
+    for (int q = 0; q < n; ++q)
+        if (!tab_vanka_mc_rows_ok(rows[q])) return fail(GLIDE_ERR_INVALID_ARG, "rows per block must be 1, 2, 4 or 8");
+    e->rows.assign(rows, rows + n);
+    e->dt = 0.0;  // recorded again (after the caller's stream drains) at the next step
+    return GLIDE_OK;
+}
+
+int glide_ensemble_destroy(glide_ensemble* e) {
+    if (!e) return GLIDE_OK;
+    ensemble_drop(e);
+    if (e->cap) hipStreamDestroy(e->cap);
+    delete e;
     return GLIDE_OK;
 }
 

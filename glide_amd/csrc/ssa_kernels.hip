@@ -20,6 +20,7 @@
 
 #include "ssa_device.hpp"
 #include "ssa_kernels.hpp"
+#include "tab.hpp"
 
 namespace glide {
 
@@ -56,14 +57,16 @@ __global__ __launch_bounds__(256) void k_grounded(T* __restrict__ phi, const T* 
 }
 
 // ------------------------------------------------------------------ residual
+// Kernel bodies take their block index (bx, by) as arguments, so that the
+// domain-table launches (one launch over many domains) run them unchanged.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_residual(SsaArgs<T> a, T* __restrict__ out_u, T* __restrict__ out_v,
-                                                 T* __restrict__ out_H, int use_forcing, int use_mask,
-                                                 T calving_rate) {
+__device__ __forceinline__ void residual_body(int bx, int by, const SsaArgs<T>& a, T* __restrict__ out_u,
+                                              T* __restrict__ out_v, T* __restrict__ out_H, int use_forcing,
+                                              int use_mask, T calving_rate) {
     __shared__ T s_eta[TY + 2][TX + 2];
     const int ny = a.ny, nx = a.nx;
     const T dxi = a.dxi;
-    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int i0 = by * TY, j0 = bx * TX;
     const int tid = threadIdx.y * TX + threadIdx.x;
     for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
         int ci = i0 - 1 + k / (TX + 2), cj = j0 - 1 + k % (TX + 2);
@@ -159,6 +162,13 @@ __global__ __launch_bounds__(NT) void k_residual(SsaArgs<T> a, T* __restrict__ o
         }
         out_v[o] = rv;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_residual(SsaArgs<T> a, T* __restrict__ out_u, T* __restrict__ out_v,
+                                                 T* __restrict__ out_H, int use_forcing, int use_mask,
+                                                 T calving_rate) {
+    residual_body<T>(blockIdx.x, blockIdx.y, a, out_u, out_v, out_H, use_forcing, use_mask, calving_rate);
 }
 
 // ------------------------------------------------------------------ JVP
@@ -808,9 +818,10 @@ __global__ __launch_bounds__(NT) void k_vanka_adjoint_cell(SsaArgs<T> a, const T
 // that its increment is exactly zero as when its right-hand side is cleared;
 // cache[25 * n + c] the cell viscosity, reused by the transpose operator.
 template <typename T>
-__global__ __launch_bounds__(NT) void k_adjoint_factor(SsaArgs<T> a, T* __restrict__ cache, VankaArgs<T> vk) {
+__device__ __forceinline__ void adjoint_factor_body(int bx, int by, const SsaArgs<T>& a, T* __restrict__ cache,
+                                                    const VankaArgs<T>& vk) {
     __shared__ T s_eta[TY + 2][TX + 2];
-    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int i0 = by * TY, j0 = bx * TX;
     const int tid = threadIdx.y * TX + threadIdx.x;
     fill_eta_tile(s_eta, a, i0, j0, tid);
     __syncthreads();
@@ -852,6 +863,11 @@ __global__ __launch_bounds__(NT) void k_adjoint_factor(SsaArgs<T> a, T* __restri
 #pragma unroll
     for (int q = 0; q < 25; ++q) cache[q * n + c] = A[q];
     cache[25 * n + c] = s_eta[threadIdx.y + 1][threadIdx.x + 1];
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adjoint_factor(SsaArgs<T> a, T* __restrict__ cache, VankaArgs<T> vk) {
+    adjoint_factor_body<T>(blockIdx.x, blockIdx.y, a, cache, vk);
 }
 
 // Increments of cell (i,j) from its cached factors and the residual r:
@@ -1525,10 +1541,10 @@ __global__ __launch_bounds__(kCoarseThreads) void k_adjoint_coarse(SsaArgs<T> a,
 // keeping the reference's facet guards (j < nx-1, i < ny-1) on the
 // right/bottom-cell contributions.
 template <typename T>
-__global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __restrict__ lu,
-                                                       const T* __restrict__ lv, T* __restrict__ gbeta,
-                                                       T* __restrict__ gbed, int exact) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+__device__ __forceinline__ void grad_beta_bed_body(int bx, int by, const SsaArgs<T>& a, const T* __restrict__ lu,
+                                                   const T* __restrict__ lv, T* __restrict__ gbeta,
+                                                   T* __restrict__ gbed, int exact) {
+    const int j = bx * 256 + threadIdx.x, i = by;
     const int ny = a.ny, nx = a.nx;
     if (j >= nx) return;
     const T* __restrict__ u = a.u;
@@ -1599,6 +1615,13 @@ __global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __
     }
     if (gbeta) gbeta[c] = gb;
     if (gbed) gbed[c] = gz;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_grad_beta_bed(SsaArgs<T> a, const T* __restrict__ lu,
+                                                       const T* __restrict__ lv, T* __restrict__ gbeta,
+                                                       T* __restrict__ gbed, int exact) {
+    grad_beta_bed_body<T>(blockIdx.x, blockIdx.y, a, lu, lv, gbeta, gbed, exact);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1883,6 +1906,124 @@ hipError_t launch_zero_masked(T* g, const T* mask, long n, hipStream_t st) {
     k_zero_masked<T><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, mask, n);
     return hipGetLastError();
 }
+
+// ------------------------------------------------------------ domain tables
+// One launch over many domains (tab.hpp): each block runs the single-domain
+// body of its job with its block index inside the job's own grid.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_tab_residual(const TabResidual<T>* __restrict__ jobs,
+                                                     const int* __restrict__ start, int n) {
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabResidual<T> J = jobs[e];
+    const int lb = blockIdx.x - start[e];
+    residual_body<T>(lb % J.gx, lb / J.gx, J.a, J.ou, J.ov, J.oH, J.use_forcing, J.use_mask, J.cr);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_tab_adj_factor(const TabAdjFactor<T>* __restrict__ jobs,
+                                                       const int* __restrict__ start, int n) {
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabAdjFactor<T> J = jobs[e];
+    const int lb = blockIdx.x - start[e];
+    adjoint_factor_body<T>(lb % J.gx, lb / J.gx, J.a, J.cache, J.vk);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TX * 2 * TY) void k_tab_adj_pass(const TabAdjPass<T>* __restrict__ jobs,
+                                                              const int* __restrict__ start, int n, int colour) {
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabAdjPass<T> J = jobs[e];
+    const int lb = blockIdx.x - start[e];
+    const int bx = lb % J.gx, by = lb / J.gx;
+    adjoint_pass_tile<T, LdPlain, TY, true>(J.a, J.cache, J.lu, J.lv, J.lH, J.lu_dst, J.lv_dst, J.lH_dst, J.fu, J.fv,
+                                            J.fH, J.cr, J.omega, by * TY, bx * TX, threadIdx.y, threadIdx.x, colour);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TX * kVjpRows) void k_tab_vjp(const TabVjp<T>* __restrict__ jobs,
+                                                           const int* __restrict__ start, int n) {
+    constexpr int ROWS = kVjpRows;
+    __shared__ VjpTile<T, ROWS> t;
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabVjp<T> J = jobs[e];
+    const int lb = blockIdx.x - start[e];
+    const int bx = lb % J.gx, by = lb / J.gx;
+    VjpTileStore<T, ROWS> s{t};
+    VjpGridOut<T> o{J.ou, J.ov, J.oH};
+    vjp_body(J.a, J.lu, J.lv, J.lH, J.fu, J.fv, J.fH, o, J.cr, J.eta_cache, by * ROWS, bx * TX, ROWS, TX,
+             (int)(threadIdx.y * TX + threadIdx.x), TX * ROWS, s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tab_grad(const TabGrad<T>* __restrict__ jobs, const int* __restrict__ start,
+                                                  int n) {
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabGrad<T> J = jobs[e];
+    const int lb = blockIdx.x - start[e];
+    grad_beta_bed_body<T>(lb % J.gx, lb / J.gx, J.a, J.lu, J.lv, J.gbeta, J.gbed, J.exact);
+}
+
+int tab_blocks_residual(int ny, int nx, int* gx) {
+    const dim3 g = tile_grid(ny, nx, true);
+    *gx = (int)g.x;
+    return (int)(g.x * g.y);
+}
+int tab_blocks_adj_factor(int ny, int nx, int* gx) {
+    const dim3 g = tile_grid(ny, nx, false);
+    *gx = (int)g.x;
+    return (int)(g.x * g.y);
+}
+int tab_blocks_adj_pass(int ny, int nx, int* gx) { return tab_blocks_adj_factor(ny, nx, gx); }
+int tab_blocks_vjp(int ny, int nx, int* gx) {
+    *gx = (nx + 1 + TX - 1) / TX;
+    return *gx * ((ny + 1 + kVjpRows - 1) / kVjpRows);
+}
+int tab_blocks_grad(int ny, int nx, int* gx) {
+    *gx = (nx + 255) / 256;
+    return *gx * ny;
+}
+
+template <typename T>
+hipError_t launch_tab_residual(const TabResidual<T>* jobs, const int* start, int n, int blocks, hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    k_tab_residual<T><<<blocks, dim3(TX, TY), 0, st>>>(jobs, start, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_tab_adj_factor(const TabAdjFactor<T>* jobs, const int* start, int n, int blocks, hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    k_tab_adj_factor<T><<<blocks, dim3(TX, TY), 0, st>>>(jobs, start, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_tab_adj_pass(const TabAdjPass<T>* jobs, const int* start, int n, int blocks, int colour,
+                               hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    k_tab_adj_pass<T><<<blocks, dim3(TX, 2 * TY), 0, st>>>(jobs, start, n, colour & 1);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_tab_vjp(const TabVjp<T>* jobs, const int* start, int n, int blocks, hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    k_tab_vjp<T><<<blocks, dim3(TX, kVjpRows), 0, st>>>(jobs, start, n);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_tab_grad(const TabGrad<T>* jobs, const int* start, int n, int blocks, hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    k_tab_grad<T><<<blocks, 256, 0, st>>>(jobs, start, n);
+    return hipGetLastError();
+}
+
+#define GLIDE_TAB_INSTANTIATE(T)                                                                                     \
+    template hipError_t launch_tab_residual<T>(const TabResidual<T>*, const int*, int, int, hipStream_t);            \
+    template hipError_t launch_tab_adj_factor<T>(const TabAdjFactor<T>*, const int*, int, int, hipStream_t);         \
+    template hipError_t launch_tab_adj_pass<T>(const TabAdjPass<T>*, const int*, int, int, int, hipStream_t);        \
+    template hipError_t launch_tab_vjp<T>(const TabVjp<T>*, const int*, int, int, hipStream_t);                      \
+    template hipError_t launch_tab_grad<T>(const TabGrad<T>*, const int*, int, int, hipStream_t);
+
+GLIDE_TAB_INSTANTIATE(double)
+GLIDE_TAB_INSTANTIATE(float)
 
 #define GLIDE_INSTANTIATE(T)                                                                                         \
     template hipError_t launch_grounded<T>(T*, const T*, const T*, T, T, long, hipStream_t);                          \

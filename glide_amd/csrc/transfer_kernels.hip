@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ssa_kernels.hpp"
+#include "tab.hpp"
 
 namespace glide {
 
@@ -287,6 +288,211 @@ template <typename T>
 hipError_t launch_restrict_batch(const RestrictBatch<T>& b, int nyc, int nxc, hipStream_t st) {
     if (b.count <= 0) return hipSuccess;
     k_restrict_batch<T><<<dim3((nxc + 1 + 255) / 256, nyc + 1, b.count), 256, 0, st>>>(b, nyc, nxc);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ domain tables
+// Element-wise jobs of many domains in one launch (tab.hpp): each keeps the
+// arithmetic of its single-domain kernel (k_ew_batch, k_axpby, k_mixed_batch,
+// k_grounded, k_zero_masked).
+template <typename T>
+__device__ __forceinline__ void tab_ew_one(const TabEw& q, int op, long k) {
+    T* out = static_cast<T*>(q.out);
+    const T* x = static_cast<const T*>(q.x);
+    switch (op) {
+        case 0: {  // axpby
+            T r = T(q.a) * x[k] + T(q.b) * static_cast<const T*>(q.y)[k];
+            if (q.z) r += T(q.c) * static_cast<const T*>(q.z)[k];
+            out[k] = r;
+            break;
+        }
+        case 1: out[k] = x[k]; break;
+        case 2: out[k] = T(q.a); break;
+        default: {  // correct: z = z + y, then x = w + z
+            const T t = x[k] + static_cast<const T*>(q.y)[k];
+            out[k] = t;
+            static_cast<T*>(q.out2)[k] = static_cast<const T*>(q.z)[k] + t;
+            break;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tab_ew(const TabEw* __restrict__ jobs, const int* __restrict__ start, int n) {
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabEw q = jobs[e];
+    const long base = (long)(blockIdx.x - start[e]) * kTabEwPerBlock;
+    const int op = q.op;
+#pragma unroll
+    for (int r = 0; r < kTabEwPerBlock / 256; ++r) {
+        const long k = base + r * 256 + threadIdx.x;
+        if (k >= q.n) return;
+        if (op <= kTabCorrect32) {
+            tab_ew_one<float>(q, op - kTabAxpby32, k);
+        } else if (op <= kTabCorrect64) {
+            tab_ew_one<double>(q, op - kTabAxpby64, k);
+        } else {
+            switch (op) {
+                case kTabTo32:
+                    static_cast<float*>(q.out)[k] = (float)static_cast<const double*>(q.x)[k];
+                    break;
+                case kTabTo64:
+                    static_cast<double*>(q.out)[k] = (double)static_cast<const float*>(q.x)[k];
+                    break;
+                case kTabDefect:  // out32 = x32 - (float) y64
+                    static_cast<float*>(q.out)[k] =
+                        static_cast<const float*>(q.x)[k] - (float)static_cast<const double*>(q.y)[k];
+                    break;
+                case kTabUpdate:
+                case kTabUpdateH: {  // out64 += x32 - fl32(out64); H: out64 = a where y32 is set
+                    double* xo = static_cast<double*>(q.out);
+                    const double x0 = xo[k];
+                    double xn = x0 + ((double)static_cast<const float*>(q.x)[k] - (double)(float)x0);
+                    if (op == kTabUpdateH && static_cast<const float*>(q.y)[k] > 0.5f) xn = q.a;
+                    xo[k] = xn;
+                    break;
+                }
+                case kTabAdd64:
+                    static_cast<double*>(q.out)[k] += (double)static_cast<const float*>(q.x)[k];
+                    break;
+                case kTabNeg32:
+                    static_cast<float*>(q.out)[k] = -(float)static_cast<const double*>(q.x)[k];
+                    break;
+                case kTabGrounded32: {
+                    float* phi = static_cast<float*>(q.out);
+                    const float c = (float)q.a, relax = (float)q.b;
+                    const float g = grounded_fraction(static_cast<const float*>(q.x)[k],
+                                                      static_cast<const float*>(q.y)[k], c);
+                    phi[k] = relax == 0.0f ? g : (1.0f - relax) * g + relax * phi[k];
+                    break;
+                }
+                case kTabGrounded64: {
+                    double* phi = static_cast<double*>(q.out);
+                    const double g = grounded_fraction(static_cast<const double*>(q.x)[k],
+                                                       static_cast<const double*>(q.y)[k], q.a);
+                    phi[k] = q.b == 0.0 ? g : (1.0 - q.b) * g + q.b * phi[k];
+                    break;
+                }
+                default:  // kTabZeroMasked64
+                    if (static_cast<const double*>(q.x)[k] > 0.5) static_cast<double*>(q.out)[k] = 0.0;
+                    break;
+            }
+        }
+    }
+}
+
+// FP32 transfers and pinning of many domains in one launch: the arithmetic of
+// k_restrict_batch, k_prolong_* and k_pin_rows (row J = block row, column I).
+__global__ __launch_bounds__(256) void k_tab_xfer(const TabXfer* __restrict__ jobs, const int* __restrict__ start,
+                                                  int n) {
+    typedef float T;
+    const int e = tab_find(start, n, blockIdx.x);
+    const TabXfer q = jobs[e];
+    const int lb = blockIdx.x - start[e];
+    const int I = (lb % q.gx) * 256 + threadIdx.x, J = lb / q.gx;
+    const int ny = q.ny, nx = q.nx;
+    const T* __restrict__ src = q.src;
+    T* __restrict__ dst = q.dst;
+    switch (q.op) {
+        case kTabRestrictU: {
+            if (I > nx) return;
+            const long sf = 2 * nx + 1;
+            dst[(long)J * (nx + 1) + I] = T(0.5) * (src[2L * J * sf + 2 * I] + src[(2L * J + 1) * sf + 2 * I]);
+            break;
+        }
+        case kTabRestrictV: {
+            if (I >= nx) return;
+            const long sf = 2 * nx;
+            dst[(long)J * nx + I] = T(0.5) * (src[2L * J * sf + 2 * I] + src[2L * J * sf + 2 * I + 1]);
+            break;
+        }
+        case kTabRestrictMean:
+        case kTabRestrictMax: {
+            if (I >= nx) return;
+            const long sf = 2 * nx;
+            T a = src[2L * J * sf + 2 * I], bb = src[2L * J * sf + 2 * I + 1];
+            T d = src[(2L * J + 1) * sf + 2 * I], ee = src[(2L * J + 1) * sf + 2 * I + 1];
+            dst[(long)J * nx + I] =
+                q.op == kTabRestrictMax ? fmax(fmax(a, bb), fmax(d, ee)) : T(0.25) * (a + bb + d + ee);
+            break;
+        }
+        case kTabProlongVfacet: {
+            if (I > nx) return;
+            const int nyc = ny / 2, nxc = nx / 2;
+            int jl, jh, il, ih;
+            T ty, tx;
+            bilin_axis((T(J) - T(0.5)) * T(0.5), nyc - 1, jl, jh, ty);
+            bilin_axis(T(I) * T(0.5), nxc, il, ih, tx);
+            const int s = nxc + 1;
+            T v00 = src[jl * s + il], v01 = src[jl * s + ih], v10 = src[jh * s + il], v11 = src[jh * s + ih];
+            dst[(long)J * (nx + 1) + I] =
+                (T(1) - ty) * ((T(1) - tx) * v00 + tx * v01) + ty * ((T(1) - tx) * v10 + tx * v11);
+            break;
+        }
+        case kTabProlongHfacet: {
+            if (I >= nx) return;
+            const int nyc = ny / 2, nxc = nx / 2;
+            int jl, jh, il, ih;
+            T ty, tx;
+            bilin_axis(T(J) * T(0.5), nyc, jl, jh, ty);
+            bilin_axis((T(I) - T(0.5)) * T(0.5), nxc - 1, il, ih, tx);
+            T v00 = src[jl * nxc + il], v01 = src[jl * nxc + ih], v10 = src[jh * nxc + il], v11 = src[jh * nxc + ih];
+            dst[(long)J * nx + I] = (T(1) - ty) * ((T(1) - tx) * v00 + tx * v01) + ty * ((T(1) - tx) * v10 + tx * v11);
+            break;
+        }
+        case kTabProlongInject: {
+            if (I >= nx) return;
+            dst[(long)J * nx + I] = src[(long)(J / 2) * (nx / 2) + I / 2];
+            break;
+        }
+        case kTabProlongBilinear: {
+            if (I >= nx) return;
+            const int nyc = ny / 2, nxc = nx / 2;
+            int jl, jh, il, ih;
+            T ty, tx;
+            bilin_axis((T(J) - T(0.5)) * T(0.5), nyc - 1, jl, jh, ty);
+            bilin_axis((T(I) - T(0.5)) * T(0.5), nxc - 1, il, ih, tx);
+            T v00 = src[jl * nxc + il], v01 = src[jl * nxc + ih], v10 = src[jh * nxc + il], v11 = src[jh * nxc + ih];
+            dst[(long)J * nx + I] = (T(1) - ty) * ((T(1) - tx) * v00 + tx * v01) + ty * ((T(1) - tx) * v10 + tx * v11);
+            break;
+        }
+        default: {  // kTabPinRows: (i, j) = (J, I) over (ny + 1) x (nx + 1)
+            const int i = J, j = I;
+            if (j > nx) return;
+            if (i < ny && (j == 0 || j == nx)) dst[(long)i * (nx + 1) + j] = T(0);
+            if (j < nx) {
+                if (i == 0 || i == ny) q.dv[(long)i * nx + j] = T(0);
+                if (i < ny && q.mask[(long)i * nx + j] > T(0.5)) q.dH[(long)i * nx + j] = T(0);
+            }
+            break;
+        }
+    }
+}
+
+int tab_blocks_xfer(int op, int ny, int nx, int* gx) {
+    int rows = 0, cols = 0;
+    switch (op) {
+        case kTabRestrictU: rows = ny; cols = nx + 1; break;
+        case kTabRestrictV: rows = ny + 1; cols = nx; break;
+        case kTabRestrictMean:
+        case kTabRestrictMax: rows = ny; cols = nx; break;
+        case kTabProlongVfacet: rows = ny; cols = nx + 1; break;
+        case kTabProlongHfacet: rows = ny + 1; cols = nx; break;
+        case kTabProlongInject:
+        case kTabProlongBilinear: rows = ny; cols = nx; break;
+        default: rows = ny + 1; cols = nx + 1; break;
+    }
+    *gx = (cols + 255) / 256;
+    return *gx * rows;
+}
+
+hipError_t launch_tab_ew(const TabEw* jobs, const int* start, int n, int blocks, hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    k_tab_ew<<<blocks, 256, 0, st>>>(jobs, start, n);
+    return hipGetLastError();
+}
+hipError_t launch_tab_xfer(const TabXfer* jobs, const int* start, int n, int blocks, hipStream_t st) {
+    if (n <= 0 || blocks <= 0) return hipSuccess;
+    k_tab_xfer<<<blocks, 256, 0, st>>>(jobs, start, n);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@ altitude.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 from typing import List, Sequence
 
@@ -100,12 +101,25 @@ class Ensemble:
     forward solve (advancing the geometry) and one adjoint solve with the
     gradients of 0.5 |u|^2 + 0.5 |v|^2 on every domain, each with
     `iterations` fixed iterations (mixed: FP32 defect-correction iterations;
-    fp64: reference-form V-cycles), spread over `streams` streams."""
+    fp64: reference-form V-cycles).
+
+    batched (mixed only): every stage of the step is one launch over all the
+    domains taking part in it (glide_ensemble_step: domain tables, the
+    V-cycles level by level, the whole step replayed as one HIP graph);
+    per domain bitwise the results of its own solves.  Otherwise each
+    domain's solves are replayed from their own cached graphs, round-robin
+    over `streams` streams."""
 
     def __init__(self, problems: Sequence[GlacierProblem], precision: str = "mixed", iterations: int = 3,
-                 streams: int = 8, device: str = "cuda"):
+                 streams: int = 8, device: str = "cuda", batched: bool = False):
         self.problems = list(problems)
         self.precision = precision
+        self.iterations = int(iterations)
+        self.batched = bool(batched)
+        if self.batched and precision != "mixed":
+            raise ValueError("the batched ensemble runs the mixed form (precision='mixed')")
+        self._handle = None
+        self.tiles = None
         self.models = []
         for p in self.problems:
             m = IceDynamics(ny=p.n, nx=p.n, dx=p.dx, n_levels=levels_for(p.n), device=device, precision=precision)
@@ -120,11 +134,88 @@ class Ensemble:
                 mo.asynchronous = True
             self.models.append(m)
         dev = self.models[0].mg[0].state.u.data.device
-        self.streams = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(streams)))]
-        # two solves per domain, plus the graphs of other callers
-        _lib.graphs_capacity(max(8, 2 * len(self.models) + 8))
+        self.device = dev
+        if self.batched:
+            self.streams = []
+            self._create_batched()
+        else:
+            self.streams = [torch.cuda.Stream(device=dev) for _ in range(max(1, int(streams)))]
+            # two solves per domain, plus the graphs of other callers
+            _lib.graphs_capacity(max(8, 2 * len(self.models) + 8))
+
+    def _create_batched(self) -> None:
+        """Level descriptors of every domain (FP64 with the forward and adjoint
+        buffers and the gradients, and the FP32 mirror) for glide_ensemble_create."""
+        from .multigrid import _mixed_hierarchy
+        n = len(self.models)
+        nl = (ctypes.c_int32 * n)()
+        p64 = (ctypes.POINTER(_lib.Level) * n)()
+        p32 = (ctypes.POINTER(_lib.Level) * n)()
+        self._descriptors = []
+        for d, m in enumerate(self.models):
+            fs, ads = m.forward_solver, m.adjoint_solver
+            top = m.mg[0]
+            for t in (top.sliding.beta, top.geometry.bed, top.state.H_prev, top.forcing.smb):
+                t.grad  # allocate the gradient buffers
+            L = len(fs.levels)
+            a64 = (_lib.Level * L)()
+            for k in range(L):
+                g = fs.levels[k].grid
+                g.forward_operators
+                g.adjoint_operators
+                a64[k] = g.level_struct(fas_scratch=fs.levels[k].scratch, adj_scratch=ads.levels[k].scratch)
+            a32 = _mixed_hierarchy(m.mg).structs(0, adjoint=True)
+            self._descriptors.append((a64, a32))
+            nl[d] = L
+            p64[d] = ctypes.cast(a64, ctypes.POINTER(_lib.Level))
+            p32[d] = ctypes.cast(a32, ctypes.POINTER(_lib.Level))
+        fo, ao = self.models[0].forward_solver.mixed_options, self.models[0].adjoint_solver.mixed_options
+        fas, vcf, vca, mc = fo.fas_c(), fo.vanka_c(False), ao.vanka_c(True), fo.to_c()
+        mc.max_outer = self.iterations
+        mc.relative_tolerance = 0.0
+        mc.absolute_tolerance = 0.0
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.call("glide_ensemble_create", n, nl, p64, p32, ctypes.byref(fas), ctypes.byref(vcf),
+                      ctypes.byref(vca), ctypes.byref(mc), 15, ctypes.byref(h))
+        self._handle = h
+
+    def set_tiles(self, rows: Sequence[int]) -> None:
+        """Rows of cell pairs per block of the batched red-black phases: one
+        entry per level (the most levels of any domain) and one for the
+        coarsest-level passes (glide_ensemble_set_tiles).  The results do not
+        depend on them."""
+        if not self._handle:
+            raise ValueError("tiles apply to the batched ensemble")
+        r = (ctypes.c_int32 * len(rows))(*[int(x) for x in rows])
+        _lib.call("glide_ensemble_set_tiles", self._handle, r, len(rows))
+        self.tiles = [int(x) for x in rows]
+
+    @property
+    def levels(self) -> int:
+        return int(_lib.load().glide_ensemble_levels(self._handle)) if self._handle else 0
+
+    @property
+    def launches_per_step(self) -> int:
+        """Launches of one batched step (recorded at the first step)."""
+        return int(_lib.load().glide_ensemble_launches(self._handle)) if self._handle else 0
+
+    def close(self) -> None:
+        if self._handle:
+            _lib.load().glide_ensemble_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def step(self, t: float, dt: float) -> None:
+        if self.batched:
+            with torch.cuda.device(self.device):
+                _lib.call("glide_ensemble_step", self._handle, float(dt), _lib.stream_handle(self.device))
+            return
         caller = torch.cuda.current_stream()
         for s in self.streams:
             s.wait_stream(caller)
@@ -147,4 +238,65 @@ class Ensemble:
         return sum(p.n * p.n for p in self.problems)
 
 
-__all__ = ["GlacierProblem", "valley_glacier", "ensemble_sizes", "levels_for", "install_glacier", "Ensemble"]
+TILE_CANDIDATES = (1, 2, 4, 8)
+
+
+def autotune_tiles(problems: Sequence[GlacierProblem], iterations: int = 3, dt: float = 1.0, reps: int = 3,
+                   candidates: Sequence[int] = TILE_CANDIDATES, cache: str | None = None, device: str = "cuda"):
+    """Tile shapes of the batched ensemble's red-black phases (rows of cell
+    pairs per block, per level and for the coarsest-level passes), chosen by
+    coordinate search over timed steps of a scratch ensemble of the same
+    domains: each entry in turn takes the candidate with the shortest step
+    (`reps` graph replays after one warm-up step), the others held at their
+    current choice.  The results do not depend on the tiles, so the search
+    only measures.  With `cache` (a JSON file) the choice is stored under the
+    device name, the domain sizes and the iteration count, and reused.
+    Returns (tiles, {candidate tiles as text: ms per step})."""
+    import json
+    import os
+    sizes = [p.n for p in problems]
+    key = f"{torch.cuda.get_device_name(torch.device(device))}|{sizes}|{int(iterations)}"
+    table = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            table = json.load(f)
+        if key in table:
+            return list(table[key]["tiles"]), dict(table[key]["timings"])
+    ens = Ensemble(problems, precision="mixed", iterations=iterations, device=device, batched=True)
+    tiles = [4] * (ens.levels + 1)
+    timings = {}
+
+    def timed(rows):
+        k = ",".join(str(x) for x in rows)
+        if k not in timings:
+            ens.set_tiles(rows)
+            ens.step(0.0, dt)  # records the tables and captures the graph
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                ens.step(0.0, dt)
+            e1.record()
+            torch.cuda.synchronize()
+            timings[k] = e0.elapsed_time(e1) / reps
+        return timings[k]
+
+    for slot in range(len(tiles)):
+        best = None
+        for c in candidates:
+            trial = list(tiles)
+            trial[slot] = int(c)
+            ms = timed(trial)
+            if best is None or ms < best[0]:
+                best = (ms, int(c))
+        tiles[slot] = best[1]
+    ens.close()
+    if cache:
+        table[key] = {"tiles": tiles, "timings": timings}
+        with open(cache, "w") as f:
+            json.dump(table, f, indent=1)
+    return tiles, timings
+
+
+__all__ = ["GlacierProblem", "valley_glacier", "ensemble_sizes", "levels_for", "install_glacier", "Ensemble",
+           "autotune_tiles", "TILE_CANDIDATES"]

@@ -380,6 +380,45 @@ int glide_timing_read(int kernel, long* count, double* total_ms, double* total_c
 /* The same restricted to launches over exactly `units` cells (one grid level). */
 int glide_timing_read_units(int kernel, long units, long* count, double* total_ms);
 
+/* Batched ensemble step (no counterpart in the reference, which builds one
+ * hierarchy per domain and process: glide/model.py:25-39,
+ * glide/multigrid.py:8-44).  An ensemble of independent domains of uneven
+ * sizes and level counts takes one forward + adjoint step per
+ * glide_ensemble_step: per domain exactly the work of
+ * glide_fascd_solve_mixed (max_outer fixed FP32 iterations, no report, start
+ * level 0), H_prev = H, the adjoint seeds dJ/du = u, dJ/dv = v (forcing -u, -v,
+ * 0), glide_fas_adjoint_solve_mixed (the same counts) and glide_gradients
+ * (grad_which), but every stage is one launch over all the domains taking
+ * part in it, from a table of per-domain arguments: the V-cycles run level by
+ * level (the descent of a level covers every domain deeper than it, all the
+ * domains' coarsest-level passes run together, then the ascents).  Per domain
+ * the results are bitwise those of its own asynchronous mixed solves.
+ * levels64[d] / levels32[d] point at domain d's n_levels[d] FP64 / FP32
+ * level descriptors (the buffers of glide_fascd_solve_mixed and
+ * glide_fas_adjoint_solve_mixed together, each level half the one above;
+ * linear drag); the descriptors are copied, the buffers must outlive the
+ * ensemble.  The launch sequence and its tables (device memory the ensemble
+ * owns) are recorded at the first step of a time step length and replayed as
+ * one HIP graph (no allocation or host wait in later steps of that length).
+ * glide_ensemble_launches gives the launches of one recorded step. */
+typedef struct glide_ensemble glide_ensemble;
+int glide_ensemble_create(int32_t n_domains, const int32_t* n_levels, const glide_level* const* levels64,
+                          const glide_level* const* levels32, const glide_fas_config* cfg,
+                          const glide_vanka_config* forward_vc, const glide_vanka_config* adjoint_vc,
+                          const glide_mixed_config* mc, int32_t grad_which, glide_ensemble** out);
+int glide_ensemble_step(glide_ensemble* e, double dt, void* stream);
+int32_t glide_ensemble_launches(const glide_ensemble* e);
+/* Tile shapes of the red-black smoothing phases (no counterpart in the
+ * reference): rows[l] cell-pair rows per block (1, 2, 4 or 8; default 4) for
+ * the stages of level l, l < glide_ensemble_levels(e) (the most levels of any
+ * domain), and rows[levels] for the coarsest-level passes; n = levels + 1.
+ * The results do not depend on the tiles (each cell's arithmetic is the
+ * same); the tables are recorded again at the next step.  Chosen by
+ * glide_amd.ensemble.autotune_tiles from timed steps. */
+int32_t glide_ensemble_levels(const glide_ensemble* e);
+int glide_ensemble_set_tiles(glide_ensemble* e, const int32_t* rows, int32_t n);
+int glide_ensemble_destroy(glide_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -255,3 +255,34 @@ def test_synthetic_generators_are_seeded():
     assert a.H.max() <= 2000.0 and a.H.min() == 0.0
     # dome radius 0.45 W: the corner cells carry no ice
     assert a.H[0, 0] == 0.0 and a.H[32, 32] > 1900.0
+
+
+def test_ensemble_rejects_invalid_arguments():
+    """glide_ensemble_create checks its arguments on the host (no device work):
+    no domains, a missing level descriptor, a NULL level buffer."""
+    from glide_amd import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    fas, vc, mc = _lib.FasConfigC(), _lib.VankaConfigC(), _lib.MixedConfigC(max_outer=3, inner_vcycles=1)
+    assert lib.glide_ensemble_create(0, None, None, None, ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
+                                     ctypes.byref(mc), 15, ctypes.byref(h)) == -1
+    assert not h.value
+    nl = (ctypes.c_int32 * 1)(2)
+    p = (ctypes.POINTER(_lib.Level) * 1)()
+    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
+                                     ctypes.byref(mc), 15, ctypes.byref(h)) == -1
+    assert "levels" in _lib.last_error()
+    levels = (_lib.Level * 2)()
+    for k, n in enumerate((16, 8)):
+        levels[k].ny = levels[k].nx = n
+        levels[k].dx = 100.0
+    p[0] = ctypes.cast(levels, ctypes.POINTER(_lib.Level))
+    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
+                                     ctypes.byref(mc), 15, ctypes.byref(h)) == -1
+    assert "NULL" in _lib.last_error()
+    mc.max_outer = -1
+    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
+                                     ctypes.byref(mc), 15, ctypes.byref(h)) == -1
+    assert lib.glide_ensemble_step(None, 1.0, None) == -1
+    assert lib.glide_ensemble_launches(None) == -1
+    assert lib.glide_ensemble_destroy(None) == 0
