@@ -64,14 +64,20 @@ class DomeProblem:
 
 def dome_problem(n: int = 64, dx: float = 1500.0, seed: int = 0, corr_len: float = 20e3,
                  grounding_line: bool = False) -> DomeProblem:
-    """The dome workload on an n x n grid (fields in head-year units)."""
+    """The dome workload on an n x n grid (fields in head-year units).
+
+    grounding_line: beta = 0 where the bed lies below -300 m and the ice (or
+    an ice-free cell) is afloat there, 0.917 H + b < 0: floating ice over the
+    deep parts of the seeded bed carries no basal drag.  (Zeroing beta under
+    grounded ice as well makes frictionless grounded patches whose forward
+    solve does not converge at 2048 x 2048.)"""
     rng = np.random.default_rng(seed)
     bed = -100.0 + 300.0 * gaussian_random_field(rng, n, n, dx, corr_len)
     H = dome_thickness(n, n, dx)
     B = np.full((n, n), 1.7e8 / (RHO_I * GRAVITY * SEC_PER_YR ** (1.0 / 3.0)))
     beta = 1e3 * np.exp(gaussian_random_field(rng, n, n, dx, corr_len)) / (RHO_I * GRAVITY)
     if grounding_line:
-        beta = np.where(bed < -300.0, 0.0, beta)
+        beta = np.where((bed < -300.0) & (0.917 * H + bed < 0.0), 0.0, beta)
     smb = 0.5 - 1e-3 * flotation_surface(bed, H)
     return DomeProblem(n, n, dx, bed, H, B, beta, smb, seed)
 
