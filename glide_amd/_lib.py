@@ -8,12 +8,14 @@ device, a :class:`GlideError` is raised.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
 import torch
 
-LIB_PATH = Path(__file__).resolve().parent / "libglide_hip.so"
+# GLIDE_LIB names another build of the library (A/B measurements)
+LIB_PATH = Path(os.environ.get("GLIDE_LIB") or Path(__file__).resolve().parent / "libglide_hip.so")
 
 USE_FORCING = 1
 USE_MASK = 2

@@ -2163,7 +2163,7 @@ struct EnsRec {
             R.fu = CPT<T>(L.af_u); R.fv = CPT<T>(L.af_v); R.fH = CPT<T>(L.af_H);
             R.cr = T(L.phys.calving_rate);
             R.omega = -T(va.omega);
-            blocks.push_back(tab_blocks_adj_pass(L.ny, L.nx, &R.gx));
+            blocks.push_back(tab_blocks_adj_pass(L.ny, L.nx, tab_adj_pass_rows(sizeof(T) == 4), &R.gx));
             red.push_back(R);
             TabAdjPass<T> B = R;
             B.lu = alt_u; B.lv = alt_v; B.lH = alt_H;

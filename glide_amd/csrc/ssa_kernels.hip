@@ -870,14 +870,11 @@ __global__ __launch_bounds__(NT) void k_adjoint_factor(SsaArgs<T> a, T* __restri
     adjoint_factor_body<T>(blockIdx.x, blockIdx.y, a, cache, vk);
 }
 
-// Increments of cell (i,j) from its cached factors and the residual r:
-// w = (d0/2, d1/2, d2/2, d3/2, d4) with J^T d = r (half-weighted facets).
+// Increments of a cell from its 25 cached factors A (in registers) and the
+// residual b: w = (d0/2, d1/2, d2/2, d3/2, d4) with J^T d = b (half-weighted
+// facets); adjoint_cached_solve loads the factors of cell c first.
 template <typename T>
-__device__ __forceinline__ void adjoint_cached_solve(const T* __restrict__ cache, long n, long c, const T b[5],
-                                                     T w[5]) {
-    T A[25];
-#pragma unroll
-    for (int q = 0; q < 25; ++q) A[q] = cache[q * n + c];
+__device__ __forceinline__ void adjoint_solve_factors(const T A[25], const T b[5], T w[5]) {
     T y[5], d[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
@@ -898,6 +895,14 @@ __device__ __forceinline__ void adjoint_cached_solve(const T* __restrict__ cache
     w[2] = T(0.5) * d[2];
     w[3] = T(0.5) * d[3];
     w[4] = d[4];
+}
+template <typename T>
+__device__ __forceinline__ void adjoint_cached_solve(const T* __restrict__ cache, long n, long c, const T b[5],
+                                                     T w[5]) {
+    T A[25];
+#pragma unroll
+    for (int q = 0; q < 25; ++q) A[q] = cache[q * n + c];
+    adjoint_solve_factors(A, b, w);
 }
 template <typename T>
 __device__ __forceinline__ void adjoint_cached_increments(const T* __restrict__ cache, const T* __restrict__ ru,
@@ -1763,14 +1768,28 @@ hipError_t launch_adjoint_pass_fused(const SsaArgs<T>& a, const T* cache, const 
     return hipGetLastError();
 }
 
+// Tile rows of the multicolour adjoint phase: TY, or GLIDE_MC_ADJ_ROWS32 rows
+// in FP32 (the FP64 tile keeps TY rows, whose LDS would double).  8 rows
+// halve the ring's share of the transpose operator but measured no faster:
+// 25.4 against 26.8 us per phase at 512², 75.3 against 73.5 ms per batched
+// 96-domain ensemble step.
+#ifndef GLIDE_MC_ADJ_ROWS32
+#define GLIDE_MC_ADJ_ROWS32 4
+#endif
+template <typename T>
+constexpr int mc_adj_rows() { return std::is_same<T, float>::value ? GLIDE_MC_ADJ_ROWS32 : TY; }
+int tab_adj_pass_rows(bool fp32) { return fp32 ? mc_adj_rows<float>() : mc_adj_rows<double>(); }
+
 template <typename T>
 hipError_t launch_adjoint_pass_mc(const SsaArgs<T>& a, const T* cache, const T* lu, const T* lv, const T* lH,
                                   T* lu_dst, T* lv_dst, T* lH_dst, const T* fu, const T* fv, const T* fH,
                                   T calving_rate, T omega, int colour, hipStream_t st) {
     if (!a.P.linear_drag) return hipErrorInvalidValue;
     TimedLaunch tl(std::is_same<T, float>::value ? kTimeAdjointMc32 : kTimeAdjointMc64, (long)a.ny * a.nx, st);
-    k_adjoint_pass_fused<T, TY, true><<<tile_grid(a.ny, a.nx, false), dim3(TX, 2 * TY), 0, st>>>(
-        a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu, fv, fH, calving_rate, -omega, colour & 1);
+    constexpr int R = mc_adj_rows<T>();
+    const dim3 g((a.nx + TX - 1) / TX, (a.ny + R - 1) / R);
+    k_adjoint_pass_fused<T, R, true><<<g, dim3(TX, 2 * R), 0, st>>>(a, cache, lu, lv, lH, lu_dst, lv_dst, lH_dst, fu,
+                                                                   fv, fH, calving_rate, -omega, colour & 1);
     return hipGetLastError();
 }
 bool adjoint_persistent_fits(int ny, int nx) {
@@ -1928,15 +1947,15 @@ __global__ __launch_bounds__(NT) void k_tab_adj_factor(const TabAdjFactor<T>* __
     adjoint_factor_body<T>(lb % J.gx, lb / J.gx, J.a, J.cache, J.vk);
 }
 
-template <typename T>
-__global__ __launch_bounds__(TX * 2 * TY) void k_tab_adj_pass(const TabAdjPass<T>* __restrict__ jobs,
+template <typename T, int TR>
+__global__ __launch_bounds__(TX * 2 * TR) void k_tab_adj_pass(const TabAdjPass<T>* __restrict__ jobs,
                                                               const int* __restrict__ start, int n, int colour) {
     const int e = tab_find(start, n, blockIdx.x);
     const TabAdjPass<T> J = jobs[e];
     const int lb = blockIdx.x - start[e];
     const int bx = lb % J.gx, by = lb / J.gx;
-    adjoint_pass_tile<T, LdPlain, TY, true>(J.a, J.cache, J.lu, J.lv, J.lH, J.lu_dst, J.lv_dst, J.lH_dst, J.fu, J.fv,
-                                            J.fH, J.cr, J.omega, by * TY, bx * TX, threadIdx.y, threadIdx.x, colour);
+    adjoint_pass_tile<T, LdPlain, TR, true>(J.a, J.cache, J.lu, J.lv, J.lH, J.lu_dst, J.lv_dst, J.lH_dst, J.fu, J.fv,
+                                            J.fH, J.cr, J.omega, by * TR, bx * TX, threadIdx.y, threadIdx.x, colour);
 }
 
 template <typename T>
@@ -1973,7 +1992,10 @@ int tab_blocks_adj_factor(int ny, int nx, int* gx) {
     *gx = (int)g.x;
     return (int)(g.x * g.y);
 }
-int tab_blocks_adj_pass(int ny, int nx, int* gx) { return tab_blocks_adj_factor(ny, nx, gx); }
+int tab_blocks_adj_pass(int ny, int nx, int rows, int* gx) {
+    *gx = (nx + TX - 1) / TX;
+    return *gx * ((ny + rows - 1) / rows);
+}
 int tab_blocks_vjp(int ny, int nx, int* gx) {
     *gx = (nx + 1 + TX - 1) / TX;
     return *gx * ((ny + 1 + kVjpRows - 1) / kVjpRows);
@@ -1999,7 +2021,8 @@ template <typename T>
 hipError_t launch_tab_adj_pass(const TabAdjPass<T>* jobs, const int* start, int n, int blocks, int colour,
                                hipStream_t st) {
     if (n <= 0 || blocks <= 0) return hipSuccess;
-    k_tab_adj_pass<T><<<blocks, dim3(TX, 2 * TY), 0, st>>>(jobs, start, n, colour & 1);
+    constexpr int R = mc_adj_rows<T>();
+    k_tab_adj_pass<T, R><<<blocks, dim3(TX, 2 * R), 0, st>>>(jobs, start, n, colour & 1);
     return hipGetLastError();
 }
 template <typename T>

@@ -133,7 +133,9 @@ int tab_blocks_eta(int ny, int nx, int* gx);
 bool tab_vanka_mc_rows_ok(int rows);
 int tab_blocks_vanka_mc(int ny, int nx, int rows, int* gx);
 int tab_blocks_adj_factor(int ny, int nx, int* gx);
-int tab_blocks_adj_pass(int ny, int nx, int* gx);
+// the multicolour adjoint phase tiles `rows` rows (tab_adj_pass_rows of the precision)
+int tab_adj_pass_rows(bool fp32);
+int tab_blocks_adj_pass(int ny, int nx, int rows, int* gx);
 int tab_blocks_vjp(int ny, int nx, int* gx);
 int tab_blocks_grad(int ny, int nx, int* gx);
 int tab_blocks_xfer(int op, int ny, int nx, int* gx);
