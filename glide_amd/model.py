@@ -14,7 +14,9 @@ class IceDynamics:
     ny, nx, dx and a level count (glide/model.py:25-39)."""
 
     def __init__(self, mg=None, n_levels=None, grid=None, ny=None, nx=None, dx=None, x0=0.0, y0=0.0, crs=None,
-                 device=None, precision="fp64"):
+                 device=None, precision="fp64", dtype=None):
+        if dtype is not None:
+            precision = self.precision_from_dtype(dtype)
         if mg is not None:
             self.mg = mg
         elif grid is not None and n_levels is not None:
@@ -31,6 +33,23 @@ class IceDynamics:
         # (an addition) "fp64": the reference's solves in FP64; "mixed": the
         # FP32 solves with FP64 defect correction (glide_amd.mixed)
         self.precision = precision
+
+    @staticmethod
+    def precision_from_dtype(dtype) -> str:
+        """The solve form of a `dtype` argument (an addition: the reference
+        solves in FP32 only): torch.float64 / "float64" -> the FP64 form,
+        "mixed" -> FP32 multigrid with FP64 defect correction.  A plain FP32
+        form is not built (its accuracy is that of the mixed form's inner
+        iteration, ~1e-7), so float32 is refused."""
+        name = str(dtype).replace("torch.", "")
+        if name in ("float64", "fp64", "double"):
+            return "fp64"
+        if name == "mixed":
+            return "mixed"
+        if name in ("float32", "fp32", "float"):
+            raise ValueError("dtype float32: a plain FP32 form is not built; use dtype='mixed' "
+                             "(FP32 multigrid with FP64 defect correction) or torch.float64")
+        raise ValueError(f"dtype must be torch.float64, 'float64' or 'mixed' (got {dtype!r})")
 
     @property
     def forward_solver(self) -> FASCDSolver:

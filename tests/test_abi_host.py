@@ -286,3 +286,22 @@ def test_ensemble_rejects_invalid_arguments():
     assert lib.glide_ensemble_step(None, 1.0, None) == -1
     assert lib.glide_ensemble_launches(None) == -1
     assert lib.glide_ensemble_destroy(None) == 0
+
+
+def test_dtype_argument_selects_the_solve_form():
+    """IceDynamics(dtype=...): float64 -> the FP64 form, 'mixed' -> the mixed
+    form; float32 is refused (not built)."""
+    import torch
+    from glide_amd import IceDynamics
+    assert IceDynamics.precision_from_dtype(torch.float64) == "fp64"
+    assert IceDynamics.precision_from_dtype("float64") == "fp64"
+    assert IceDynamics.precision_from_dtype("mixed") == "mixed"
+    m = IceDynamics(ny=16, nx=16, dx=10.0, n_levels=2, device="cpu", dtype="mixed")
+    assert m.precision == "mixed"
+    for bad in (torch.float32, "float32", "int8"):
+        try:
+            IceDynamics.precision_from_dtype(bad)
+        except ValueError:
+            pass
+        else:
+            raise AssertionError(bad)
