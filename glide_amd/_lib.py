@@ -68,7 +68,8 @@ class FasConfigC(ctypes.Structure):
 class MixedConfigC(ctypes.Structure):
     _fields_ = [("max_outer", ctypes.c_int32), ("inner_vcycles", ctypes.c_int32),
                 ("relative_tolerance", ctypes.c_double), ("absolute_tolerance", ctypes.c_double),
-                ("fp64_coarsest", ctypes.c_int32), ("krylov_dim", ctypes.c_int32), ("krylov_work", ctypes.c_void_p)]
+                ("fp64_coarsest", ctypes.c_int32), ("krylov_dim", ctypes.c_int32), ("krylov_work", ctypes.c_void_p),
+                ("fp64_newton_steps", ctypes.c_int32), ("fp64_relaxation", ctypes.c_double)]
 
 
 class SolveReport(ctypes.Structure):

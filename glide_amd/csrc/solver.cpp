@@ -1109,6 +1109,8 @@ int check_mixed(const glide_level* levels64, const glide_level* levels32, int n_
     if (mc->max_outer < 0 || mc->max_outer > 64) return fail(GLIDE_ERR_INVALID_ARG, "max_outer must be in [0, 64]");
     if (mc->inner_vcycles < 1) return fail(GLIDE_ERR_INVALID_ARG, "inner_vcycles must be >= 1");
     if (mc->krylov_dim < 0 || mc->krylov_dim > 16) return fail(GLIDE_ERR_INVALID_ARG, "krylov_dim must be in [0, 16]");
+    if (mc->fp64_newton_steps < 0 || mc->fp64_newton_steps > 64 || !(mc->fp64_relaxation >= 0.0))
+        return fail(GLIDE_ERR_INVALID_ARG, "fp64_newton_steps must be in [0, 64] and fp64_relaxation >= 0");
     if (mc->krylov_dim > 0 && !mc->krylov_work) return fail(GLIDE_ERR_INVALID_ARG, "krylov_dim > 0 needs krylov_work");
     for (int l = start; l < n_levels; ++l) {
         const glide_level& A = levels32[l];
@@ -1435,6 +1437,8 @@ std::string mixed_key(char tag, const glide_level* l64, const glide_level* l32, 
     key_append(k, mc.inner_vcycles);
     key_append(k, mc.fp64_coarsest);
     key_append(k, mc.max_outer);
+    key_append(k, mc.fp64_newton_steps);
+    key_append(k, mc.fp64_relaxation);
     key_append(k, g_spin_limit);
     key_append(k, g_persistent_fenced);
     for (int l = start; l < n_levels; ++l) {
@@ -2825,6 +2829,11 @@ int glide_fascd_solve_mixed(const glide_level* levels64, const glide_level* leve
         GLIDE_TRY(run_mixed(pb, st));
     }
     MixedCtx X{levels64, levels32, n_levels, dt, *cfg, vc, mc->fp64_coarsest != 0, st};
+    std::vector<glide_vanka_config> vc64(vc, vc + n_levels);
+    for (auto& q : vc64) {
+        if (mc->fp64_newton_steps > 0) q.newton_steps = mc->fp64_newton_steps;
+        if (mc->fp64_relaxation > 0.0) q.relaxation = mc->fp64_relaxation;
+    }
     auto resid = [&](hipStream_t cs, double* slot) -> int {
         GLIDE_TRY(grounded(D, 0.0, cs));
         GLIDE_TRY(residual(D, dt, GLIDE_USE_FORCING | GLIDE_USE_MASK, P(D.r_u), P(D.r_v), P(D.r_H), cs));
@@ -2840,7 +2849,8 @@ int glide_fascd_solve_mixed(const glide_level* levels64, const glide_level* leve
         HIP_TRY(launch_ew_batch<R>(save, cs), "save state");
         if (phase == 2) {
             // one FP64 V-cycle with the same red-black smoother and counts
-            FasCtx F{levels64, n_levels, dt, *cfg, vc, cs};
+            // (its local Newton steps and relaxation from mc if set)
+            FasCtx F{levels64, n_levels, dt, *cfg, vc64.data(), cs};
             GLIDE_TRY(fas_vcycle(F, start_level, true));
             X.st = st;
             return GLIDE_OK;

@@ -101,6 +101,10 @@ class MixedConfig:
     solve's FP64 finishing cycles by Anderson acceleration of depth
     min(krylov_dim, 6); 0 keeps the plain loops.  The forward and adjoint
     solves of a hierarchy share the workspace, so they must not overlap.
+    fp64_newton_steps / fp64_relaxation set the local Newton iteration of
+    the forward solve's FP64 finishing cycles (default one full step: near
+    the solution the local problems are close to linear; 0 keeps
+    newton_steps / relaxation).
     compat_vprolong is off: the mixed form uses the v-facet prolongation."""
     max_outer: int = 30
     inner_vcycles: int = 1
@@ -117,6 +121,10 @@ class MixedConfig:
     compat_vprolong: bool = False
     asynchronous: bool = False
     krylov_dim: int = 12
+    # the forward solve's FP64 finishing cycles: one full local Newton step
+    # per cell (0 keeps newton_steps / relaxation)
+    fp64_newton_steps: int = 1
+    fp64_relaxation: float = 1.0
 
     def to_c(self) -> _lib.MixedConfigC:
         if not 0 <= int(self.max_outer) <= 64:
@@ -126,7 +134,9 @@ class MixedConfig:
         return _lib.MixedConfigC(max_outer=int(self.max_outer), inner_vcycles=int(self.inner_vcycles),
                                  relative_tolerance=float(self.relative_tolerance),
                                  absolute_tolerance=float(self.absolute_tolerance),
-                                 fp64_coarsest=int(bool(self.fp64_coarsest)), krylov_dim=0, krylov_work=None)
+                                 fp64_coarsest=int(bool(self.fp64_coarsest)), krylov_dim=0, krylov_work=None,
+                                 fp64_newton_steps=int(self.fp64_newton_steps),
+                                 fp64_relaxation=float(self.fp64_relaxation))
 
     def fas_c(self, freeze_coarse_calving=True) -> _lib.FasConfigC:
         return _lib.FasConfigC(freeze_coarse_calving=int(bool(freeze_coarse_calving)), freeze_coarse_phi=1,

@@ -233,6 +233,12 @@ typedef struct glide_mixed_config {
      * 0 keeps the plain loops. */
     int32_t krylov_dim;
     void* krylov_work;
+    /* The forward solve's FP64 finishing V-cycles: local Newton steps and
+     * relaxation of their red-black passes (0 keeps the vc values).  Near
+     * the solution one full step (1, 1.0) suffices: at 512^2 the finishing
+     * needs fewer cycles and each pass a third of the work. */
+    int32_t fp64_newton_steps;
+    double fp64_relaxation;
 } glide_mixed_config;
 /* Doubles of glide_mixed_config.krylov_work for an ny x nx start level and
  * restart length m (1 <= m <= 16); -1 for invalid arguments. */
