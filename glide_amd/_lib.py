@@ -159,7 +159,8 @@ SIGNATURES = {
                                            ctypes.POINTER(VankaConfigC), ctypes.POINTER(MixedConfigC),
                                            ctypes.POINTER(SolveReport), ctypes.POINTER(_I), _P]),
     "glide_ensemble_create": (_I, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_LP),
-                                   ctypes.POINTER(_LP), ctypes.POINTER(FasConfigC), ctypes.POINTER(VankaConfigC),
+                                   ctypes.POINTER(_LP), ctypes.POINTER(FasConfigC), ctypes.POINTER(FasConfigC),
+                                   ctypes.POINTER(VankaConfigC),
                                    ctypes.POINTER(VankaConfigC), ctypes.POINTER(MixedConfigC), ctypes.c_int32,
                                    ctypes.POINTER(_P)]),
     "glide_ensemble_step": (_I, [_P, _D, _P]),

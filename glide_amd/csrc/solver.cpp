@@ -1736,7 +1736,8 @@ struct EnsRef {  // one domain's level in a stage, with the stage flags of that 
 
 struct EnsRec {
     const std::vector<EnsDomain>& D;
-    const glide_fas_config& cfg;
+    const glide_fas_config& cfg;   // forward V-cycles
+    const glide_fas_config& acfg;  // adjoint V-cycles
     const glide_vanka_config& vf;
     const glide_vanka_config& va;
     bool fp64_coarsest;
@@ -2215,7 +2216,7 @@ struct EnsRec {
         typedef float F;
         const std::vector<int> S = deeper_than(l);
         if (S.empty()) return GLIDE_OK;
-        GLIDE_TRY(adj_smooth32(S, l, cfg.pre_steps));
+        GLIDE_TRY(adj_smooth32(S, l, acfg.pre_steps));
         {
             XferList x;
             for (int d : S) {
@@ -2299,7 +2300,7 @@ struct EnsRec {
         if (!fp64_coarsest) {
             std::vector<const glide_level*> Ls;
             for (size_t d = 0; d < D.size(); ++d) Ls.push_back(&l32((int)d, D[d].nl - 1));
-            return adj_sweeps<float>(Ls, cfg.coarsest_steps);
+            return adj_sweeps<float>(Ls, acfg.coarsest_steps);
         }
         EwList in, out;
         std::vector<const glide_level*> Ls;
@@ -2324,7 +2325,7 @@ struct EnsRec {
         }
         ew(in);
         GLIDE_TRY(factor<double>(Ls));
-        GLIDE_TRY(adj_sweeps<double>(Ls, cfg.coarsest_steps));
+        GLIDE_TRY(adj_sweeps<double>(Ls, acfg.coarsest_steps));
         ew(out);
         return GLIDE_OK;
     }
@@ -2348,7 +2349,7 @@ struct EnsRec {
             for (int d : S) {
                 const glide_level &L = l32(d, l), &N = l32(d, l + 1);
                 x.add(kTabProlongVfacet, N.zl_u, L.zl_u, L.ny, L.nx);
-                x.add(cfg.compat_vprolong ? kTabProlongVfacet : kTabProlongHfacet, N.zl_v, L.zl_v, L.ny, L.nx);
+                x.add(acfg.compat_vprolong ? kTabProlongVfacet : kTabProlongHfacet, N.zl_v, L.zl_v, L.ny, L.nx);
                 x.add(kTabProlongBilinear, N.zl_H, L.zl_H, L.ny, L.nx);
             }
             xfer(x);
@@ -2366,8 +2367,8 @@ struct EnsRec {
             for (int d : S) p.pin(l32(d, l));
             xfer(p);
         }
-        GLIDE_TRY(adj_smooth32(S, l, cfg.post_steps));
-        if (l == 0) GLIDE_TRY(adj_smooth32(S, 0, cfg.finest_steps));
+        GLIDE_TRY(adj_smooth32(S, l, acfg.post_steps));
+        if (l == 0) GLIDE_TRY(adj_smooth32(S, 0, acfg.finest_steps));
         return GLIDE_OK;
     }
 
@@ -2534,7 +2535,7 @@ int run_tab_op(const TabOp& op, const unsigned char* dev, hipStream_t st) {
 
 struct glide_ensemble {
     std::vector<EnsDomain> dom;
-    glide_fas_config cfg;
+    glide_fas_config cfg, acfg;
     glide_vanka_config vf, va;
     glide_mixed_config mc;
     int which = 0;
@@ -2563,7 +2564,7 @@ void ensemble_drop(glide_ensemble* e) {
 
 int ensemble_record(glide_ensemble* e, double dt) {
     ensemble_drop(e);
-    EnsRec R{e->dom, e->cfg, e->vf, e->va, e->mc.fp64_coarsest != 0, e->mc.inner_vcycles, dt, e->prog, e->lmax,
+    EnsRec R{e->dom, e->cfg, e->acfg, e->vf, e->va, e->mc.fp64_coarsest != 0, e->mc.inner_vcycles, dt, e->prog, e->lmax,
              e->rows};
     GLIDE_TRY(R.step(e->mc.max_outer, e->which));
     if (!e->prog.arena.empty()) {
@@ -3305,16 +3306,19 @@ static int ensemble_check_level(const glide_level& L, bool fp64, int l, int nl) 
 
 int glide_ensemble_create(int32_t n_domains, const int32_t* n_levels, const glide_level* const* levels64,
                           const glide_level* const* levels32, const glide_fas_config* cfg,
-                          const glide_vanka_config* forward_vc, const glide_vanka_config* adjoint_vc,
-                          const glide_mixed_config* mc, int32_t grad_which, glide_ensemble** out) {
+                          const glide_fas_config* adjoint_cfg, const glide_vanka_config* forward_vc,
+                          const glide_vanka_config* adjoint_vc, const glide_mixed_config* mc, int32_t grad_which,
+                          glide_ensemble** out) {
     if (!out) return fail(GLIDE_ERR_INVALID_ARG, "out is NULL");
     *out = nullptr;
-    if (n_domains < 1 || !n_levels || !levels64 || !levels32 || !cfg || !forward_vc || !adjoint_vc || !mc)
+    if (n_domains < 1 || !n_levels || !levels64 || !levels32 || !cfg || !adjoint_cfg || !forward_vc || !adjoint_vc ||
+        !mc)
         return fail(GLIDE_ERR_INVALID_ARG, "the ensemble needs at least one domain and every argument");
     if (mc->max_outer < 0 || mc->max_outer > 64 || mc->inner_vcycles < 1)
         return fail(GLIDE_ERR_INVALID_ARG, "max_outer must be in [0, 64] and inner_vcycles >= 1");
-    if (cfg->coarsest_steps < 0 || cfg->pre_steps < 0 || cfg->post_steps < 0 || cfg->finest_steps < 0)
-        return fail(GLIDE_ERR_INVALID_ARG, "negative smoothing counts");
+    for (const glide_fas_config* c : {cfg, adjoint_cfg})
+        if (c->coarsest_steps < 0 || c->pre_steps < 0 || c->post_steps < 0 || c->finest_steps < 0)
+            return fail(GLIDE_ERR_INVALID_ARG, "negative smoothing counts");
     if (grad_which & ~(15 | GLIDE_GRAD_EXACT)) return fail(GLIDE_ERR_INVALID_ARG, "unknown gradient flags");
     glide_ensemble* e = new glide_ensemble();
     for (int d = 0; d < n_domains; ++d) {
@@ -3348,6 +3352,7 @@ int glide_ensemble_create(int32_t n_domains, const int32_t* n_levels, const glid
         e->dom.push_back(std::move(D));
     }
     e->cfg = *cfg;
+    e->acfg = *adjoint_cfg;
     e->vf = *forward_vc;
     e->va = *adjoint_vc;
     e->mc = *mc;

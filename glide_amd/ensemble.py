@@ -170,13 +170,16 @@ class Ensemble:
             p64[d] = ctypes.cast(a64, ctypes.POINTER(_lib.Level))
             p32[d] = ctypes.cast(a32, ctypes.POINTER(_lib.Level))
         fo, ao = self.models[0].forward_solver.mixed_options, self.models[0].adjoint_solver.mixed_options
-        fas, vcf, vca, mc = fo.fas_c(), fo.vanka_c(False), ao.vanka_c(True), fo.to_c()
+        fas, afas, vcf, vca, mc = fo.fas_c(), ao.fas_c(), fo.vanka_c(False), ao.vanka_c(True), fo.to_c()
+        ac = ao.to_c()
+        if (ac.inner_vcycles, ac.fp64_coarsest) != (mc.inner_vcycles, mc.fp64_coarsest):
+            raise ValueError("the batched ensemble needs the same inner_vcycles and fp64_coarsest in both solves")
         mc.max_outer = self.iterations
         mc.relative_tolerance = 0.0
         mc.absolute_tolerance = 0.0
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            _lib.call("glide_ensemble_create", n, nl, p64, p32, ctypes.byref(fas), ctypes.byref(vcf),
+            _lib.call("glide_ensemble_create", n, nl, p64, p32, ctypes.byref(fas), ctypes.byref(afas), ctypes.byref(vcf),
                       ctypes.byref(vca), ctypes.byref(mc), 15, ctypes.byref(h))
         self._handle = h
 

@@ -402,6 +402,7 @@ int glide_timing_read_units(int kernel, long units, long* count, double* total_m
  * levels64[d] / levels32[d] point at domain d's n_levels[d] FP64 / FP32
  * level descriptors (the buffers of glide_fascd_solve_mixed and
  * glide_fas_adjoint_solve_mixed together, each level half the one above;
+ * cfg / adjoint_cfg the V-cycle counts of the forward / adjoint solves;
  * linear drag); the descriptors are copied, the buffers must outlive the
  * ensemble.  The launch sequence and its tables (device memory the ensemble
  * owns) are recorded at the first step of a time step length and replayed as
@@ -410,8 +411,9 @@ int glide_timing_read_units(int kernel, long units, long* count, double* total_m
 typedef struct glide_ensemble glide_ensemble;
 int glide_ensemble_create(int32_t n_domains, const int32_t* n_levels, const glide_level* const* levels64,
                           const glide_level* const* levels32, const glide_fas_config* cfg,
-                          const glide_vanka_config* forward_vc, const glide_vanka_config* adjoint_vc,
-                          const glide_mixed_config* mc, int32_t grad_which, glide_ensemble** out);
+                          const glide_fas_config* adjoint_cfg, const glide_vanka_config* forward_vc,
+                          const glide_vanka_config* adjoint_vc, const glide_mixed_config* mc, int32_t grad_which,
+                          glide_ensemble** out);
 int glide_ensemble_step(glide_ensemble* e, double dt, void* stream);
 int32_t glide_ensemble_launches(const glide_ensemble* e);
 /* Tile shapes of the red-black smoothing phases (no counterpart in the

@@ -264,12 +264,12 @@ def test_ensemble_rejects_invalid_arguments():
     lib = _lib.load()
     h = ctypes.c_void_p()
     fas, vc, mc = _lib.FasConfigC(), _lib.VankaConfigC(), _lib.MixedConfigC(max_outer=3, inner_vcycles=1)
-    assert lib.glide_ensemble_create(0, None, None, None, ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
+    assert lib.glide_ensemble_create(0, None, None, None, ctypes.byref(fas), ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
                                      ctypes.byref(mc), 15, ctypes.byref(h)) == -1
     assert not h.value
     nl = (ctypes.c_int32 * 1)(2)
     p = (ctypes.POINTER(_lib.Level) * 1)()
-    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
+    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
                                      ctypes.byref(mc), 15, ctypes.byref(h)) == -1
     assert "levels" in _lib.last_error()
     levels = (_lib.Level * 2)()
@@ -277,11 +277,11 @@ def test_ensemble_rejects_invalid_arguments():
         levels[k].ny = levels[k].nx = n
         levels[k].dx = 100.0
     p[0] = ctypes.cast(levels, ctypes.POINTER(_lib.Level))
-    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
+    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
                                      ctypes.byref(mc), 15, ctypes.byref(h)) == -1
     assert "NULL" in _lib.last_error()
     mc.max_outer = -1
-    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
+    assert lib.glide_ensemble_create(1, nl, p, p, ctypes.byref(fas), ctypes.byref(fas), ctypes.byref(vc), ctypes.byref(vc),
                                      ctypes.byref(mc), 15, ctypes.byref(h)) == -1
     assert lib.glide_ensemble_step(None, 1.0, None) == -1
     assert lib.glide_ensemble_launches(None) == -1
