@@ -715,7 +715,7 @@ def ensemble(args, torch):
     print(json.dumps(out), flush=True)
 
 
-PROFILE_TOLERANCES = (1e-10, 1e-8, 1e-11)  # forward, enthalpy, adjoint: the verified values at 512^2
+PROFILE_TOLERANCES = (3e-10, 1e-8, 3e-10)  # forward, enthalpy, adjoint: the verified values at 512^2
 
 
 def profile_kernels(args, torch):
