@@ -124,3 +124,40 @@ def test_mixed_solves_are_deterministic():
         out.append([N(t) for t in (g.state.u.data, g.state.H.data, g.adjoint.lambda_u.data, g.sliding.beta.grad)])
     for x, y in zip(*out):
         assert np.array_equal(x, y)
+
+
+def test_mixed_forward_with_floating_ice_reaches_fp64_solution(orc):
+    """The 2048² workload's grounding line at 64² (dome_problem(grounding_line=
+    True), seed 1: beta = 0 on the floating ice over bed below -300 m, 44 ice
+    cells and 684 ice-free ones): the mixed forward solve reaches the FP64
+    solution as on grounded ice."""
+    from glide_amd import IceDynamics
+    from glide_amd.synthetic import dome_problem, install
+    n, L, dx = 64, 4, 1500.0
+    print(f"mixed forward, floating ice, {n}x{n} L{L}", flush=True)
+    p = dome_problem(n, dx, seed=1, grounding_line=True)
+    q = dome_problem(n, dx, seed=1)
+    assert int(((p.beta == 0) & (q.beta > 0) & (p.H > 0)).sum()) == 44
+    model = IceDynamics(ny=n, nx=n, dx=dx, n_levels=L, device="cuda")
+    install(model.mg, p)
+    om = orc.OracleMG(n, n, dx, L)
+    for k in ("bed", "B", "beta", "smb"):
+        om.set(k, getattr(p, k))
+    om.set("H", p.H)
+    om.set("H_prev", p.H)
+    fs = model.forward_solver
+    fs.fas_options.set(maximum_vcycles=2, relative_tolerance=0.0, absolute_tolerance=0.0, report_norms=False)
+    fs.solve(10.0)
+    om.fascd_solve(10.0, orc.OracleMG.fas_config(vcycles=2, rtol=0.0, atol=0.0))
+    model.mg.state.H_prev.set(model.mg[0].state.H.data.clone())
+    om.set("H_prev", om.get("H"))
+    _mixed(model)
+    rep = fs.solve(10.0)
+    torch.cuda.synchronize()
+    _oracle_converged(orc, om)
+    g = model.mg[0]
+    errs = {k: l2_rel(N(getattr(g.state, k).data), om.get(k)) for k in ("u", "v", "H")}
+    print(f"  {rep.vcycles} outer iterations {fs.last_stats}, errors {errs}", flush=True)
+    for k, e in errs.items():
+        assert e < BAR, (k, e)
+    assert np.array_equal(N(g.state.mask.data), om.get("mask"))
