@@ -3374,8 +3374,9 @@ int glide_ensemble_step(glide_ensemble* e, double dt, void* stream) {
     if (dev != e->device) return fail(GLIDE_ERR_INVALID_ARG, "the ensemble belongs to another device");
     hipStream_t st = (hipStream_t)stream;
     if (e->dt != dt) {
-        // a replay of the previous tables may still be running
-        if (e->exec || e->dev) HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        // a replay of the previous tables may still be running (on any stream
+        // an earlier step was given)
+        if (e->exec || e->dev) HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
         GLIDE_TRY(ensemble_record(e, dt));
     }
     if (!g_graphs_on || timing_on()) return ensemble_enqueue(e, st);
@@ -3411,12 +3412,13 @@ int glide_ensemble_set_tiles(glide_ensemble* e, const int32_t* rows, int32_t n) 
     for (int q = 0; q < n; ++q)
         if (!tab_vanka_mc_rows_ok(rows[q])) return fail(GLIDE_ERR_INVALID_ARG, "rows per block must be 1, 2, 4 or 8");
     e->rows.assign(rows, rows + n);
-    e->dt = 0.0;  // recorded again (after the caller's stream drains) at the next step
+    e->dt = 0.0;  // recorded again (after the device drains) at the next step
     return GLIDE_OK;
 }
 
 int glide_ensemble_destroy(glide_ensemble* e) {
     if (!e) return GLIDE_OK;
+    if (e->exec || e->dev) HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");  // no replay in flight
     ensemble_drop(e);
     if (e->cap) hipStreamDestroy(e->cap);
     delete e;

@@ -406,7 +406,8 @@ int glide_timing_read_units(int kernel, long units, long* count, double* total_m
  * linear drag); the descriptors are copied, the buffers must outlive the
  * ensemble.  The launch sequence and its tables (device memory the ensemble
  * owns) are recorded at the first step of a time step length and replayed as
- * one HIP graph (no allocation or host wait in later steps of that length).
+ * one HIP graph (no allocation or host wait in later steps of that length;
+ * recording again and glide_ensemble_destroy first synchronise the device).
  * glide_ensemble_launches gives the launches of one recorded step. */
 typedef struct glide_ensemble glide_ensemble;
 int glide_ensemble_create(int32_t n_domains, const int32_t* n_levels, const glide_level* const* levels64,
