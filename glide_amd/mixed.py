@@ -111,7 +111,7 @@ class MixedConfig:
     relative_tolerance: float = 1e-10
     absolute_tolerance: float = 0.0
     fp64_coarsest: bool = True
-    coarsest_steps: int = 10
+    coarsest_steps: int = 5
     pre_steps: int = 4
     post_steps: int = 4
     finest_steps: int = 8
