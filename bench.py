@@ -746,7 +746,7 @@ def profile_kernels(args, torch):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path.",
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); there is no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -3,13 +3,20 @@ together on one GPU (no counterpart in the reference, which runs one domain
 per process: glide/model.py:25-39, glide/multigrid.py:212-235).
 
 Each domain is an ``IceDynamics`` of its own size and level count.  A step
-enqueues every domain's forward and adjoint solve with a fixed number of
-iterations and no host wait (``asynchronous``), replayed from the library's
-cached HIP graphs (one per domain and solve), round-robin over a few streams,
-so that the small domains' latency-bound launches overlap.  The persistent
-coarse-level kernels of the FP64 form assume their workgroups are resident
-together, which concurrent streams do not guarantee, so the ensemble turns
-them off for its steps (the red-black passes of the mixed form have none).
+runs every domain's forward and adjoint solve with a fixed number of
+iterations and no host wait (``asynchronous``).  Batched
+(``Ensemble(batched=True)``, the mixed form): every stage of the step is one
+launch over a device table of the domains taking part in it
+(glide_ensemble_step: the V-cycles level by level, all coarsest-level passes
+together, the whole step one HIP graph), bitwise the per-domain results;
+:func:`autotune_tiles` picks the red-black phases' workgroup shapes from timed
+steps.  Per domain (the default): each domain's solves are replayed from the
+library's cached HIP graphs (one per domain and solve), round-robin over a
+few streams, so that the small domains' latency-bound launches overlap; the
+persistent coarse-level kernels of the FP64 form assume their workgroups are
+resident together, which concurrent streams do not guarantee, so that path
+turns them off for its steps (the red-black passes of the mixed form have
+none).
 
 :func:`valley_glacier` is the seeded mountain-glacier workload: a valley bed
 b = z0 - slope x + 150 m (2 y / W - 1)^2 + 30 m x GRF, initial thickness 0.3 of
